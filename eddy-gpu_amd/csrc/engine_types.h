@@ -1,0 +1,94 @@
+// engine_types.h - layouts shared by the host engine and the gfx950 kernels.
+#pragma once
+#include <stdint.h>
+
+// Engine limits (the reference's are MAX_GENESET_SIZE 200 and a 1024-thread
+// block per network, definitions.cuh:19, main.cu:931).
+#define EDDY_MAX_G 1024     // genes per set (10-bit family keys)
+#define EDDY_MAX_N 2046     // samples (S = N + 2 <= 2048 networks)
+#define EDDY_MAX_MP 16      // parent cap held per row
+#define EDDY_NWAVES 4       // waves per workgroup in the per-item kernels
+
+// Per-pair info word written by the test kernels and read by the network
+// kernel.  Class g in {0,1} = class 1 / class 2 of the reference.
+//   bit g            full-data test of class g is an edge
+//   bits 2+9g..10+9g leave-one-out test per cell (r*3+k) of class g; a cell
+//                    with no sample holds the full bit (it never occurs)
+//   bit 20+g         the leave-one-out tests of class g were evaluated
+#define INFO_FULL(g) (1u << (g))
+#define INFO_LOO_SHIFT(g) (2 + 9 * (g))
+#define INFO_EVAL(g) (1u << (20 + (g)))
+
+struct SetMeta {
+  int G;            // genes in the set
+  int c;            // G(G-1)/2
+  long long xoff;   // byte offset of the set's rows [G x N] in the gathered data
+  long long poff;   // u32-word offset of the set's prior bits (one bit per pair)
+};
+
+struct ItemMeta {   // one (gene set, permutation)
+  int set;
+  int perm;
+  long long infooff;  // pair offset into info / tables
+  long long laboff;   // u32-word offset of the label bitmask (bit = class 2)
+  long long evoff;    // event offset (capacity ev_cap per item)
+  long long soff;     // network-indexed scratch offset (S per item)
+  long long goff;     // gene-indexed scratch offset (G per item)
+  long long foff;     // family scratch offset (fam_cap per item)
+};
+
+struct Event {        // a row of one leave-one-out network that differs from its class's full network
+  uint16_t s, t, n, pad;
+  uint16_t par[EDDY_MAX_MP];
+};
+
+struct KArgs {
+  // dataset-level
+  int N, n1, n2, S;
+  int mp, p_adjust, plain;
+  int ev_cap, fam_cap;
+  double aE, aEP;
+  double lg15, lg2, lg3;   // lgamma(1.5), lgamma(2), lgamma(3) (AS 147's lgamma(p+1))
+  const double *lgtab;     // [82][lgstride]: lgamma(n + 4/d)
+  int lgstride;
+  uint64_t hseed;
+  // sets
+  const SetMeta *sets;
+  const uint8_t *xs;       // gathered set rows
+  const uint32_t *prior;   // prior bits
+  // items
+  const ItemMeta *items;
+  const uint32_t *labels;
+  // work lists
+  const uint32_t *tiles;   // K1 work list: item<<16 | bi<<8 | bj
+  // per-pair outputs
+  uint32_t *info;
+  uint16_t *tab;           // 18 counts per pair (cutoff mode)
+  // per-item scratch
+  uint16_t *lfull;         // [goff*2 + g*G + t][mp]
+  uint8_t *lcnt;           // [goff*2 + g*G + t]
+  uint8_t *sflags;         // [goff*2 + g*G + t] observed-state flags (bit v)
+  Event *ev;
+  int *evcount;            // [item]
+  uint64_t *hnet;          // [soff + s]
+  int *rep;                // [soff + s] first occurrence of each network
+  uint16_t *clsg;          // [soff + k]: class-1 samples, then class-2 samples
+  int *evorder;            // [evoff + e]
+  int *evfam;              // [evoff + e]
+  uint64_t *famkey;        // [foff + f]
+  double *famscore;        // [(foff + f)*2 + g]
+  double *lval;            // [soff*2 ...] 2 per network slot
+  int *uniq_u;             // [item]
+  int *famcount;           // [item] distinct families scored
+  double *js;              // [set*perms + perm]
+  int perms;
+  int *status;             // [item] error bits
+  long long *tests;        // [item] evaluated tests
+};
+
+enum {
+  ST_EV_OVERFLOW = 1,
+  ST_FAM_OVERFLOW = 2,
+  ST_COLLISION = 4,
+  ST_NOT_EVALUATED = 8,
+};

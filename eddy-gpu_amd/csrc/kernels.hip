@@ -1,0 +1,943 @@
+// kernels.hip - gfx950 kernels of the EDDY permutation loop.
+//
+// Work unit: an "item" = (gene set, permutation).  For every item the engine
+// produces, exactly as the reference's per-permutation launches do
+// (main.cu:656-1428):
+//   k_count_test  full-class 3x3 tables of every gene pair as an int8 one-hot
+//                 GEMM on MFMA (v_mfma_i32_16x16x64_i8, int32 accumulate),
+//                 then the chi-square/AS 147 edge tests (kernel1.cu:10-477)
+//   k_loo         leave-one-out tests up to each row's cutoff (cutoff mode)
+//   k_rows        capped parent lists of the two full networks, the rows of
+//                 each leave-one-out network that differ from them, and a
+//                 64-bit hash per network (kernel2.cu:114, kernel3.cu:41-165)
+//   k_score       first-occurrence dedup with exact confirmation
+//                 (kernel4.cu, main.cu:1060-1121), BDeu family histograms
+//                 (kernel5.cu), scaled softmax likelihoods and JS divergence
+//                 (main.cu:1294-1428)
+//   k_pvalue      beta fit + ibetac per set (main.cu:1462-1502)
+//
+// Leave-one-out identity used throughout: the tally of network s (class g,
+// sample k left out) is the class-g full tally minus one count in the cell
+// (x_row[k], x_col[k]) (kernel1.cu:86-119 skips exactly that sample), so each
+// (pair, class) has at most 10 distinct tables and 10 distinct decisions.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "engine_types.h"
+#include "fpmath.h"
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// --------------------------------------------------------------------------
+// chi-square test of one 3x3 tally: kernel1.cu:121-203 and 420-431, the same
+// operation order as the reference loops (row-major chi-square sum)
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double d_gammds(double x, double p, double lgp1) {
+  if (x != x) return 0.0;  // unreachable for integer tallies; keeps the series finite
+  if (x <= 0.0) return 0.0;
+  if (p <= 0.0) return 0.0;
+  double arg = p * eddy_log(x) - lgp1 - x;
+  if (arg < eddy_log(1.0e-37)) return 0.0;
+  double f = eddy_exp(arg);
+  if (f == 0.0) return 0.0;
+  double c = 1.0, value = 1.0, a = p;
+  for (;;) {
+    a = a + 1.0;
+    c = c * x / a;
+    value = value + c;
+    if (c <= 1.0e-9 * value) break;
+  }
+  return value * f;
+}
+
+__device__ __forceinline__ int d_edge_test(const int *t, int prior, const KArgs &a, double cpairs) {
+  int ex[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      ex[k] += t[r * 3 + k];      // column sums
+      ex[3 + r] += t[r * 3 + k];  // row sums
+    }
+  double divisor = (double)(ex[0] + ex[1] + ex[2]);
+  double expected[9];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) expected[r * 3 + k] = (double)ex[k] * (double)ex[r + 3] / divisor;
+  double yates = 0.0;
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      if (expected[r * 3 + k] < 5 && ex[k] && ex[r + 3]) yates = 0.5;
+  double chi = 0.0;
+  int dofm = 0, dofn = 0;
+#pragma unroll
+  for (int ii = 0; ii < 3; ii++) {
+    dofm += ex[ii] == 0;
+    dofn += ex[ii + 3] == 0;
+#pragma unroll
+    for (int jj = 0; jj < 3; jj++) {
+      if (ex[jj] * ex[ii + 3] != 0) {
+        double d = eddy_fabs((double)t[ii * 3 + jj] - expected[ii * 3 + jj]) - yates;
+        chi = chi + d * d / expected[ii * 3 + jj];
+      }
+    }
+  }
+  int dof = ((3 - dofm) - 1) * ((3 - dofn) - 1);
+  double lgp1 = dof == 1 ? a.lg15 : dof == 2 ? a.lg2 : a.lg3;
+  double pv = 1.0 - d_gammds(chi / 2.0, ((double)dof) / 2.0, lgp1);
+  if (a.p_adjust) {
+    double adj = pv * cpairs;
+    pv = adj < 1.0 ? adj : 1.0;
+  }
+  return (pv < a.aE || (prior == 1 && pv < a.aEP)) ? 1 : 0;
+}
+
+// Leave-one-out decisions of one (pair, class): bit cell set when the tally
+// minus one count in `cell` is an edge; empty cells copy the full decision.
+__device__ __forceinline__ uint32_t d_loo_mask(const int *t, int full, int prior, const KArgs &a,
+                                               double cpairs, int *ntests) {
+  uint32_t m = 0;
+  int tt[9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) tt[i] = t[i];
+  for (int cell = 0; cell < 9; cell++) {
+    int bit = full;
+    if (t[cell] > 0) {
+      tt[cell] = t[cell] - 1;
+      bit = d_edge_test(tt, prior, a, cpairs);
+      tt[cell] = t[cell];
+      (*ntests)++;
+    }
+    m |= (uint32_t)bit << cell;
+  }
+  return m;
+}
+
+__device__ __forceinline__ long long pair_index(int t, int col) { return (long long)t * (t - 1) / 2 + col; }
+
+__device__ __forceinline__ int prior_bit(const KArgs &a, const SetMeta &sm, long long p) {
+  return (a.prior[sm.poff + (p >> 5)] >> (p & 31)) & 1;
+}
+
+// bytes equal to v -> 0x01, others 0x00 (byte values 0..3)
+__device__ __forceinline__ uint32_t eq_bytes(uint32_t x, uint32_t v) {
+  uint32_t t = x ^ (v * 0x01010101u);
+  uint32_t z = ~(((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t) & 0x80808080u;
+  return z >> 7;
+}
+
+// LDS ordering among the lanes of one wave (a wave executes its LDS accesses
+// in order; this keeps the compiler from moving them across the point)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// --------------------------------------------------------------------------
+// K1: tables on MFMA + tests.  One workgroup = one item x one 32x32 block of
+// gene pairs (row block bi >= column block bj).  One-hot rows are gene*3+state
+// (96 per block); class masking is folded into the A operand, so one pass
+// over all N samples accumulates both classes' tables.
+// --------------------------------------------------------------------------
+#define K1_TB 32
+#define K1_KCH 512
+#define K1_ROWS (2 * K1_TB)
+
+__global__ __launch_bounds__(256) void k_count_test(KArgs a) {
+  const uint2 w = ((const uint2 *)a.tiles)[blockIdx.x];
+  const int item = (int)w.x;
+  const int bi = (int)(w.y >> 16), bj = (int)(w.y & 0xffff);
+  const ItemMeta im = a.items[item];
+  const SetMeta sm = a.sets[im.set];
+  const int G = sm.G, N = a.N;
+  const int NP = (N + 63) & ~63;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+  __shared__ __align__(16) uint8_t rows[K1_ROWS * K1_KCH > 2 * 96 * 96 * 2 ? K1_ROWS * K1_KCH : 2 * 96 * 96 * 2];
+  __shared__ __align__(16) uint8_t clsb[EDDY_MAX_N + 66];
+  __shared__ int tcount;
+
+  // class-1 membership byte per sample (label bit 0 = class 1)
+  const uint32_t *lab = a.labels + im.laboff;
+  for (int s = tid; s < NP; s += 256) clsb[s] = (s < N) ? (uint8_t)(((lab[s >> 5] >> (s & 31)) & 1) ^ 1) : 0;
+  if (tid == 0) tcount = 0;
+
+  v4i acc1[3][3], acc2[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      acc1[i][j] = (v4i){0, 0, 0, 0};
+      acc2[i][j] = (v4i){0, 0, 0, 0};
+    }
+
+  const uint8_t *xs = a.xs + sm.xoff;
+  const int rt0 = (wave >> 1) * 3;  // this wave's row tiles (of 6)
+  const int ct0 = (wave & 1) * 3;   // this wave's column tiles (of 6)
+
+  for (int k0 = 0; k0 < NP; k0 += K1_KCH) {
+    const int kch = (NP - k0) < K1_KCH ? (NP - k0) : K1_KCH;
+    __syncthreads();
+    // stage 64 gene rows x kch samples (rows 0..31 = block bi, 32..63 = block bj)
+    const int vecs = kch >> 4;
+    for (int v = tid; v < K1_ROWS * vecs; v += 256) {
+      int r = v / vecs, c = v - r * vecs;
+      int gene = (r < K1_TB) ? bi * K1_TB + r : bj * K1_TB + (r - K1_TB);
+      uint4 val = make_uint4(0x03030303u, 0x03030303u, 0x03030303u, 0x03030303u);
+      if (gene < G) val = *(const uint4 *)(xs + (size_t)gene * NP + k0 + c * 16);
+      *(uint4 *)(rows + r * K1_KCH + c * 16) = val;
+    }
+    __syncthreads();
+    for (int kk = 0; kk < kch; kk += 64) {
+      const int kofs = kk + (lane >> 4) * 16;
+      const uint4 m = *(const uint4 *)(clsb + k0 + kofs);
+      v4i A1[3], A2[3], B[3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        int R = (rt0 + i) * 16 + (lane & 15);
+        int gl = R / 3, st = R - gl * 3;
+        uint4 x = *(const uint4 *)(rows + gl * K1_KCH + kofs);
+        uint32_t o0 = eq_bytes(x.x, st), o1 = eq_bytes(x.y, st), o2 = eq_bytes(x.z, st), o3 = eq_bytes(x.w, st);
+        A1[i] = (v4i){(int)(o0 & m.x), (int)(o1 & m.y), (int)(o2 & m.z), (int)(o3 & m.w)};
+        A2[i] = (v4i){(int)(o0 & (m.x ^ 0x01010101u)), (int)(o1 & (m.y ^ 0x01010101u)),
+                      (int)(o2 & (m.z ^ 0x01010101u)), (int)(o3 & (m.w ^ 0x01010101u))};
+        int C = (ct0 + i) * 16 + (lane & 15);
+        int gc = C / 3, sc = C - gc * 3;
+        uint4 y = *(const uint4 *)(rows + (K1_TB + gc) * K1_KCH + kofs);
+        B[i] = (v4i){(int)eq_bytes(y.x, sc), (int)eq_bytes(y.y, sc), (int)eq_bytes(y.z, sc), (int)eq_bytes(y.w, sc)};
+      }
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          acc1[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A1[i], B[j], acc1[i][j], 0, 0, 0);
+          acc2[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A2[i], B[j], acc2[i][j], 0, 0, 0);
+        }
+    }
+  }
+  __syncthreads();
+  // accumulators -> LDS tables T[g][R][C] (u16)
+  uint16_t *T = (uint16_t *)rows;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        int R = (rt0 + i) * 16 + 4 * (lane >> 4) + e;
+        int C = (ct0 + j) * 16 + (lane & 15);
+        T[R * 96 + C] = (uint16_t)acc1[i][j][e];
+        T[96 * 96 + R * 96 + C] = (uint16_t)acc2[i][j][e];
+      }
+  __syncthreads();
+
+  const double cpairs = (double)sm.c;
+  int ntests = 0;
+  for (int idx = tid; idx < K1_TB * K1_TB; idx += 256) {
+    int li = idx >> 5, lj = idx & 31;
+    int gi = bi * K1_TB + li, gj = bj * K1_TB + lj;
+    if (gi >= G || gj >= gi) continue;
+    int t1[9], t2[9];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        t1[r * 3 + k] = T[(li * 3 + r) * 96 + lj * 3 + k];
+        t2[r * 3 + k] = T[96 * 96 + (li * 3 + r) * 96 + lj * 3 + k];
+      }
+    long long p = pair_index(gi, gj);
+    int pr = prior_bit(a, sm, p);
+    int f1 = d_edge_test(t1, pr, a, cpairs);
+    int f2 = d_edge_test(t2, pr, a, cpairs);
+    ntests += 2;
+    uint32_t info = (uint32_t)f1 | ((uint32_t)f2 << 1);
+    if (a.plain) {
+      info |= d_loo_mask(t1, f1, pr, a, cpairs, &ntests) << INFO_LOO_SHIFT(0);
+      info |= d_loo_mask(t2, f2, pr, a, cpairs, &ntests) << INFO_LOO_SHIFT(1);
+      info |= INFO_EVAL(0) | INFO_EVAL(1);
+    } else {
+      uint16_t *tb = a.tab + (im.infooff + p) * 18;
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        tb[i] = (uint16_t)t1[i];
+        tb[9 + i] = (uint16_t)t2[i];
+      }
+    }
+    a.info[im.infooff + p] = info;
+  }
+  atomicAdd(&tcount, ntests);
+  __syncthreads();
+  if (tid == 0) atomicAdd((unsigned long long *)&a.tests[item], (unsigned long long)tcount);
+}
+
+// --------------------------------------------------------------------------
+// K2 (cutoff mode): leave-one-out tests of row t, class g, in ascending
+// column order until the row holds mp "solid" edges (edges in the full test
+// and in every leave-one-out test).  Every network of class g then takes its
+// first mp parents (kernel3.cu:70-109) from columns at or before that point,
+// so later columns cannot change any capped network.  One workgroup per item.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_loo(KArgs a) {
+  const int item = blockIdx.x;
+  const ItemMeta im = a.items[item];
+  const SetMeta sm = a.sets[im.set];
+  const int G = sm.G;
+  const double cpairs = (double)sm.c;
+  __shared__ int tcount;
+  if (threadIdx.x == 0) tcount = 0;
+  __syncthreads();
+  int ntests = 0;
+  for (int idx = threadIdx.x; idx < 2 * G; idx += blockDim.x) {
+    int t = G - 1 - (idx >> 1);  // long rows first
+    int g = idx & 1;
+    int solid = 0;
+    for (int col = 0; col < t && solid < a.mp; col++) {
+      long long p = pair_index(t, col);
+      uint32_t info = a.info[im.infooff + p];
+      int full = (info >> g) & 1;
+      const uint16_t *tb = a.tab + (im.infooff + p) * 18 + 9 * g;
+      int tt[9];
+#pragma unroll
+      for (int i = 0; i < 9; i++) tt[i] = tb[i];
+      int pr = prior_bit(a, sm, p);
+      uint32_t m = d_loo_mask(tt, full, pr, a, cpairs, &ntests);
+      atomicOr(&a.info[im.infooff + p], (m << INFO_LOO_SHIFT(g)) | INFO_EVAL(g));
+      if (full && m == 0x1FFu) solid++;
+    }
+  }
+  atomicAdd(&tcount, ntests);
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd((unsigned long long *)&a.tests[item], (unsigned long long)tcount);
+}
+
+// --------------------------------------------------------------------------
+// shared per-item helpers
+// --------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint64_t row_hash(uint64_t seed, int t, int n, const uint16_t *par) {
+  uint64_t h = mix64(seed ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(t + 1)));
+  for (int i = 0; i < n; i++) h = mix64(h ^ (((uint64_t)par[i] + 1) << 16) ^ (uint64_t)i);
+  return mix64(h ^ (uint64_t)n * 0xD6E8FEB86659FD93ULL);
+}
+
+// class lists: cls[g][k] = k-th sample of class g in ascending sample order
+__device__ void build_class_lists(const uint32_t *lab, int N, uint16_t *cls0, uint16_t *cls1, int *counts) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ int wsum[2][EDDY_NWAVES];
+  __shared__ int base[2];
+  if (tid == 0) base[0] = base[1] = 0;
+  __syncthreads();
+  for (int s0 = 0; s0 < N; s0 += 64 * EDDY_NWAVES) {
+    int s = s0 + tid;
+    int v = s < N;
+    int c2 = v ? (lab[s >> 5] >> (s & 31)) & 1 : 0;
+    uint64_t b1 = __ballot(v && !c2), b2 = __ballot(v && c2);
+    uint64_t lt = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+    int r1 = __popcll(b1 & lt), r2 = __popcll(b2 & lt);
+    if (lane == 0) {
+      wsum[0][wave] = __popcll(b1);
+      wsum[1][wave] = __popcll(b2);
+    }
+    __syncthreads();
+    int o1 = base[0], o2 = base[1];
+    for (int w = 0; w < wave; w++) {
+      o1 += wsum[0][w];
+      o2 += wsum[1][w];
+    }
+    if (v && !c2) cls0[o1 + r1] = (uint16_t)s;
+    if (v && c2) cls1[o2 + r2] = (uint16_t)s;
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 0; w < EDDY_NWAVES; w++) {
+        base[0] += wsum[0][w];
+        base[1] += wsum[1][w];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    counts[0] = base[0];
+    counts[1] = base[1];
+  }
+  __syncthreads();
+}
+
+// --------------------------------------------------------------------------
+// K3: rows.  Per item: for each row t and class g, the row's cutoff C (the
+// column of its mp-th solid edge, or t-1), its capped parent list in the full
+// network (kernel3.cu:70-109, 111-132), and the leave-one-out networks whose
+// row differs (events).  Network hash = XOR over rows of row_hash.
+// One workgroup (4 waves) per item; a wave owns one row at a time.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_rows(KArgs a) {
+  const int item = blockIdx.x;
+  const ItemMeta im = a.items[item];
+  const SetMeta sm = a.sets[im.set];
+  const int G = sm.G, N = a.N, S = a.S, mp = a.mp;
+  const int NP = (N + 63) & ~63;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint8_t *xs = a.xs + sm.xoff;
+
+  __shared__ uint16_t cls[2][EDDY_MAX_N];
+  __shared__ int ncls[2];
+  __shared__ unsigned long long delta[EDDY_MAX_N + 2];
+  __shared__ unsigned long long hfull[2];
+  __shared__ uint32_t flips[EDDY_NWAVES][EDDY_MAX_G];  // col | mask << 16
+  __shared__ uint16_t lrow[EDDY_NWAVES][EDDY_MAX_MP];
+  __shared__ uint8_t sfl[2][EDDY_MAX_G];
+
+  build_class_lists(a.labels + im.laboff, N, cls[0], cls[1], ncls);
+  for (int k = tid; k < ncls[0]; k += 256) a.clsg[im.soff + k] = cls[0][k];
+  for (int k = tid; k < ncls[1]; k += 256) a.clsg[im.soff + a.n1 + k] = cls[1][k];
+  for (int s = tid; s < S; s += 256) delta[s] = 0ULL;
+  for (int i = tid; i < 2 * G; i += 256) sfl[i / G][i % G] = 0;
+  if (tid == 0) hfull[0] = hfull[1] = 0ULL;
+  __syncthreads();
+
+  // observed states per gene per class on the class's full data (kernel1.cu:338-387)
+  for (int i = tid; i < G * 2; i += 256) {
+    int gene = i >> 1, g = i & 1;
+    uint32_t f = 0;
+    const uint8_t *row = xs + (size_t)gene * NP;
+    for (int k = 0; k < ncls[g]; k++) {
+      uint8_t v = row[cls[g][k]];
+      if (v < 3) f |= 1u << v;
+    }
+    sfl[g][gene] = (uint8_t)f;
+    a.sflags[im.goff * 2 + g * G + gene] = (uint8_t)f;
+  }
+  __syncthreads();
+
+  uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
+  for (int rg = wave; rg < 2 * G; rg += EDDY_NWAVES) {
+    const int t = rg >> 1, g = rg & 1;
+    int nL = 0, nSolid = 0, nflip = 0, C = t - 1;
+    bool done = (t == 0);
+    for (int base = 0; base < t && !done; base += 64) {
+      int col = base + lane;
+      bool valid = col < t;
+      uint32_t w = valid ? a.info[im.infooff + pair_index(t, col)] : 0u;
+      int full = (w >> g) & 1;
+      uint32_t loo = (w >> INFO_LOO_SHIFT(g)) & 0x1FFu;
+      bool ev = (w & INFO_EVAL(g)) != 0;
+      bool solid = valid && full && ev && loo == 0x1FFu;
+      uint64_t sm_ = __ballot(solid);
+      int need = mp - nSolid;
+      int limit = 63;
+      if (__popcll(sm_) >= need) {
+        uint64_t x = sm_;
+        for (int i = 1; i < need; i++) x &= x - 1;
+        limit = __ffsll((long long)x) - 1;
+        C = base + limit;
+        done = true;
+      }
+      bool inRange = valid && lane <= limit;
+      if (__any(inRange && !ev)) {
+        if (lane == 0) atomicOr(&a.status[item], ST_NOT_EVALUATED);
+      }
+      uint64_t hm = __ballot(inRange && full);
+      int rank = __popcll(hm & lt_mask);
+      if (inRange && full && nL + rank < mp) lrow[wave][nL + rank] = (uint16_t)col;
+      nL += __popcll(hm);
+      if (nL > mp) nL = mp;
+      uint32_t flip = loo ^ (full ? 0x1FFu : 0u);
+      bool isf = inRange && flip != 0;
+      uint64_t fm = __ballot(isf);
+      if (isf) flips[wave][nflip + __popcll(fm & lt_mask)] = (uint32_t)col | (flip << 16);
+      nflip += __popcll(fm);
+      nSolid += __popcll(sm_ & (limit == 63 ? ~0ULL : ((2ULL << limit) - 1)));
+      wave_sync();
+    }
+    wave_sync();
+    // full row list
+    const size_t lbase = (size_t)(im.goff * 2 + g * G + t);
+    if (lane < nL) a.lfull[lbase * EDDY_MAX_MP + lane] = lrow[wave][lane];
+    if (lane == 0) a.lcnt[lbase] = (uint8_t)nL;
+    uint64_t hrow = row_hash(a.hseed, t, nL, lrow[wave]);
+    if (lane == 0) atomicXor(&hfull[g], (unsigned long long)hrow);
+    if (nflip == 0) {
+      wave_sync();
+      continue;
+    }
+    // leave-one-out networks of class g whose row t differs
+    const int ng = ncls[g];
+    const uint8_t *xt = xs + (size_t)t * NP;
+    for (int k = lane; k < ng; k += 64) {
+      int sid = cls[g][k];
+      int vt = xt[sid];
+      if (vt >= 3) continue;
+      bool changed = false;
+      for (int f = 0; f < nflip && !changed; f++) {
+        uint32_t e = flips[wave][f];
+        int vc = xs[(size_t)(e & 0xffff) * NP + sid];
+        if (vc < 3 && ((e >> 16) >> (vt * 3 + vc)) & 1) changed = true;
+      }
+      if (!changed) continue;
+      uint16_t nl[EDDY_MAX_MP];
+      int cnt = 0, fi = 0;
+      for (int col = 0; col <= C && cnt < mp; col++) {
+        uint32_t w = a.info[im.infooff + pair_index(t, col)];
+        int bit = (w >> g) & 1;
+        if (fi < nflip && (int)(flips[wave][fi] & 0xffff) == col) {
+          uint32_t e = flips[wave][fi];
+          int vc = xs[(size_t)col * NP + sid];
+          if (vc < 3) bit ^= ((e >> 16) >> (vt * 3 + vc)) & 1;
+          fi++;
+        }
+        if (bit) nl[cnt++] = (uint16_t)col;
+      }
+      bool same = cnt == nL;
+      for (int i = 0; i < cnt && same; i++) same = nl[i] == lrow[wave][i];
+      if (same) continue;
+      int s = g == 0 ? 1 + k : a.n1 + 1 + k;
+      int e = atomicAdd(&a.evcount[item], 1);
+      if (e < a.ev_cap) {
+        Event ev;
+        ev.s = (uint16_t)s;
+        ev.t = (uint16_t)t;
+        ev.n = (uint16_t)cnt;
+        ev.pad = 0;
+        for (int i = 0; i < EDDY_MAX_MP; i++) ev.par[i] = i < cnt ? nl[i] : 0;
+        a.ev[im.evoff + e] = ev;
+      } else {
+        atomicOr(&a.status[item], ST_EV_OVERFLOW);
+      }
+      atomicXor(&delta[s], (unsigned long long)(hrow ^ row_hash(a.hseed, t, cnt, nl)));
+    }
+    wave_sync();
+  }
+  __syncthreads();
+  for (int s = tid; s < S; s += 256) {
+    int g = (s <= a.n1) ? 0 : 1;
+    a.hnet[im.soff + s] = hfull[g] ^ delta[s];
+  }
+}
+
+// --------------------------------------------------------------------------
+// K4: dedup, BDeu, likelihood, JS.  One workgroup per item.
+// --------------------------------------------------------------------------
+#define HT_SIZE 4096
+
+__device__ __forceinline__ int net_class(int s, int n1) { return s <= n1 ? 0 : 1; }
+
+// parent list of row t in network s (events of s sorted by t: es[0..ne))
+__device__ __forceinline__ int row_list(const KArgs &a, const ItemMeta &im, int G, int s, int t, const int *es,
+                                        int ne, int *ep, const uint16_t **out) {
+  while (*ep < ne && a.ev[im.evoff + es[*ep]].t < t) (*ep)++;
+  if (*ep < ne && a.ev[im.evoff + es[*ep]].t == t) {
+    const Event *e = &a.ev[im.evoff + es[*ep]];
+    *out = e->par;
+    return e->n;
+  }
+  int g = net_class(s, a.n1);
+  size_t lb = (size_t)(im.goff * 2 + g * G + t);
+  *out = a.lfull + lb * EDDY_MAX_MP;
+  return a.lcnt[lb];
+}
+
+__device__ __forceinline__ uint64_t fam_key(int t, int n, const uint16_t *par) {
+  if (n > 3) n = 3;  // BDeu uses at most 3 parents (kernel5.cu:119-130)
+  uint64_t k = (uint64_t)t | ((uint64_t)n << 10);
+  for (int i = 0; i < n; i++) k |= (uint64_t)par[i] << (12 + 10 * i);
+  return k + 1;  // 0 marks an empty slot
+}
+
+__device__ int ht_insert_min(unsigned long long *keys, int *vals, uint64_t key, int val) {
+  uint32_t h = (uint32_t)mix64(key) & (HT_SIZE - 1);
+  for (int probe = 0; probe < HT_SIZE; probe++) {
+    unsigned long long old = atomicCAS(&keys[h], 0ULL, (unsigned long long)key);
+    if (old == 0ULL || old == key) {
+      atomicMin(&vals[h], val);
+      return (int)h;
+    }
+    h = (h + 1) & (HT_SIZE - 1);
+  }
+  return -1;
+}
+
+__device__ int ht_find(const unsigned long long *keys, uint64_t key) {
+  uint32_t h = (uint32_t)mix64(key) & (HT_SIZE - 1);
+  for (int probe = 0; probe < HT_SIZE; probe++) {
+    unsigned long long k = keys[h];
+    if (k == key) return (int)h;
+    if (k == 0ULL) return -1;
+    h = (h + 1) & (HT_SIZE - 1);
+  }
+  return -1;
+}
+
+__global__ __launch_bounds__(256) void k_score(KArgs a) {
+  const int item = blockIdx.x;
+  const ItemMeta im = a.items[item];
+  const SetMeta sm = a.sets[im.set];
+  const int G = sm.G, N = a.N, S = a.S, n1 = a.n1;
+  const int NP = (N + 63) & ~63;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint8_t *xs = a.xs + sm.xoff;
+
+  __shared__ unsigned long long hkeys[HT_SIZE];
+  __shared__ int hvals[HT_SIZE];
+  __shared__ int rep[EDDY_MAX_N + 2];
+  __shared__ int uid[EDDY_MAX_N + 2];
+  __shared__ int evcnt[EDDY_MAX_N + 2];
+  __shared__ int evst[EDDY_MAX_N + 3];
+  __shared__ int U, F, flag;
+  __shared__ int hist[EDDY_NWAVES][84];
+
+  const int E = min(a.evcount[item], a.ev_cap);
+
+  // ---- 1. first occurrence of each network hash -------------------------
+  for (int i = tid; i < HT_SIZE; i += 256) {
+    hkeys[i] = 0ULL;
+    hvals[i] = 0x7fffffff;
+  }
+  for (int s = tid; s < S; s += 256) evcnt[s] = 0;
+  if (tid == 0) flag = 0;
+  __syncthreads();
+  for (int s = tid; s < S; s += 256) {
+    uint64_t h = a.hnet[im.soff + s];
+    if (h == 0ULL) h = 1ULL;
+    if (ht_insert_min(hkeys, hvals, h, s) < 0) atomicOr(&flag, 1);
+  }
+  for (int e = tid; e < E; e += 256) atomicAdd(&evcnt[a.ev[im.evoff + e].s], 1);
+  __syncthreads();
+  for (int s = tid; s < S; s += 256) {
+    uint64_t h = a.hnet[im.soff + s];
+    if (h == 0ULL) h = 1ULL;
+    int slot = ht_find(hkeys, h);
+    rep[s] = slot >= 0 ? hvals[slot] : s;
+    if (slot < 0) atomicOr(&flag, 1);
+  }
+  // events grouped by network (exclusive scan of counts, serial: S <= 2048)
+  if (tid == 0) {
+    int acc = 0;
+    for (int s = 0; s < S; s++) {
+      evst[s] = acc;
+      acc += evcnt[s];
+    }
+    evst[S] = acc;
+  }
+  __syncthreads();
+  for (int s = tid; s < S; s += 256) evcnt[s] = 0;
+  __syncthreads();
+  int *es = a.evorder + im.evoff;
+  for (int e = tid; e < E; e += 256) {
+    int s = a.ev[im.evoff + e].s;
+    es[evst[s] + atomicAdd(&evcnt[s], 1)] = e;
+  }
+  __syncthreads();
+  // sort each network's events by row
+  for (int s = tid; s < S; s += 256) {
+    int b = evst[s], n = evst[s + 1] - b;
+    for (int i = 1; i < n; i++) {
+      int v = es[b + i];
+      int tv = a.ev[im.evoff + v].t;
+      int j = i - 1;
+      while (j >= 0 && a.ev[im.evoff + es[b + j]].t > tv) {
+        es[b + j + 1] = es[b + j];
+        j--;
+      }
+      es[b + j + 1] = v;
+    }
+  }
+  __syncthreads();
+  // ---- 2. exact confirmation of every merge -----------------------------
+  for (int s = tid; s < S; s += 256) {
+    int r = rep[s];
+    if (r == s) continue;
+    int gs = net_class(s, n1), gr = net_class(r, n1);
+    int ns = evst[s + 1] - evst[s], nr = evst[r + 1] - evst[r];
+    if (gs == gr && ns == 0 && nr == 0) continue;
+    bool eq = true;
+    int ps = 0, pr = 0;
+    for (int t = 0; t < G && eq; t++) {
+      if (gs == gr) {
+        // rows without events are the same full row; compare event rows only
+        while (ps < ns && a.ev[im.evoff + es[evst[s] + ps]].t < t) ps++;
+        while (pr < nr && a.ev[im.evoff + es[evst[r] + pr]].t < t) pr++;
+        bool hs = ps < ns && a.ev[im.evoff + es[evst[s] + ps]].t == t;
+        bool hr = pr < nr && a.ev[im.evoff + es[evst[r] + pr]].t == t;
+        if (!hs && !hr) continue;
+      }
+      const uint16_t *ls, *lr;
+      int cs = row_list(a, im, G, s, t, es + evst[s], ns, &ps, &ls);
+      int cr = row_list(a, im, G, r, t, es + evst[r], nr, &pr, &lr);
+      if (cs != cr) eq = false;
+      for (int i = 0; i < cs && eq; i++) eq = ls[i] == lr[i];
+    }
+    if (!eq) atomicOr(&flag, 2);
+  }
+  __syncthreads();
+  if (flag) {
+    if (tid == 0) {
+      atomicOr(&a.status[item], ST_COLLISION);
+      a.js[im.set * a.perms + im.perm] = 0.0;
+    }
+    return;
+  }
+  // unique ids in network order
+  if (tid == 0) {
+    int u = 0;
+    for (int s = 0; s < S; s++) {
+      uid[s] = (rep[s] == s) ? u++ : -1;
+    }
+    U = u;
+    a.uniq_u[item] = u;
+  }
+  for (int s = tid; s < S; s += 256) a.rep[im.soff + s] = rep[s];
+  // ---- 3. families of the unique networks --------------------------------
+  for (int i = tid; i < HT_SIZE; i += 256) {
+    hkeys[i] = 0ULL;
+    hvals[i] = 0x7fffffff;
+  }
+  if (tid == 0) F = 0;
+  __syncthreads();
+  // full rows of both classes, then event rows of unique networks
+  for (int i = tid; i < 2 * G + E; i += 256) {
+    uint64_t key;
+    if (i < 2 * G) {
+      int g = i / G, t = i - g * G;
+      size_t lb = (size_t)(im.goff * 2 + g * G + t);
+      key = fam_key(t, a.lcnt[lb], a.lfull + lb * EDDY_MAX_MP);
+    } else {
+      const Event *e = &a.ev[im.evoff + (i - 2 * G)];
+      if (rep[e->s] != e->s) continue;
+      key = fam_key(e->t, e->n, e->par);
+    }
+    if (ht_insert_min(hkeys, hvals, key, 0) < 0) atomicOr(&flag, 4);
+  }
+  __syncthreads();
+  // number the distinct families
+  for (int i = tid; i < HT_SIZE; i += 256) {
+    if (hkeys[i] != 0ULL) {
+      int f = atomicAdd(&F, 1);
+      hvals[i] = f;
+      if (f < a.fam_cap) a.famkey[im.foff + f] = hkeys[i];
+    }
+  }
+  __syncthreads();
+  if (tid == 0) a.famcount[item] = min(F, a.fam_cap);
+  if (flag || F > a.fam_cap) {
+    if (tid == 0) atomicOr(&a.status[item], ST_FAM_OVERFLOW);
+    return;
+  }
+  for (int e = tid; e < E; e += 256) {
+    const Event *ev = &a.ev[im.evoff + e];
+    int slot = rep[ev->s] == ev->s ? ht_find(hkeys, fam_key(ev->t, ev->n, ev->par)) : -1;
+    a.evfam[im.evoff + e] = slot >= 0 ? hvals[slot] : -1;
+  }
+  // full-row family ids reuse the hash keys region afterwards: store them in evcnt? keep lookups
+  __syncthreads();
+  // ---- 4. BDeu per (family, scoring class): kernel5.cu:4-236 -------------
+  // waves take (family, class) pairs in rounds; every wave joins every
+  // round's barriers so the per-wave histograms can be reused safely
+  for (int fg0 = 0; fg0 < 2 * F; fg0 += EDDY_NWAVES) {
+    const int fg = fg0 + wave;
+    const bool act = fg < 2 * F;
+    const int f = fg >> 1, g = fg & 1;
+    int t = 0, np = 0, par[3] = {0, 0, 0}, r = 0, q = 1, radix[3] = {1, 1, 1};
+    const uint8_t *sf = a.sflags + im.goff * 2 + g * G;
+    if (act) {
+      const uint64_t key = a.famkey[im.foff + f] - 1;
+      t = (int)(key & 1023);
+      np = (int)((key >> 10) & 3);
+      for (int i = 0; i < np; i++) par[i] = (int)((key >> (12 + 10 * i)) & 1023);
+      // ranks of observed states in class g's full data
+      r = __popc(sf[t]);
+      for (int i = 0; i < np; i++) {
+        radix[i] = q;
+        q *= __popc(sf[par[i]]);
+      }
+    }
+    for (int i = lane; i < 84; i += 64) hist[wave][i] = 0;
+    __syncthreads();
+    if (act && q > 0 && r > 0) {
+      const uint8_t *xt = xs + (size_t)t * NP;
+      const uint16_t *cl = a.clsg + im.soff + (g ? a.n1 : 0);
+      const int ng = g ? a.n2 : a.n1;
+      for (int k = lane; k < ng; k += 64) {
+        int sid = cl[k];
+        int v = xt[sid];
+        if (v >= 3) continue;
+        int kk = __popc(sf[t] & ((1u << v) - 1));
+        int j = 0;
+        bool ok = true;
+        for (int i = 0; i < np; i++) {
+          int pv = xs[(size_t)par[i] * NP + sid];
+          if (pv >= 3) {
+            ok = false;
+            break;
+          }
+          j += __popc(sf[par[i]] & ((1u << pv) - 1)) * radix[i];
+        }
+        if (ok) atomicAdd(&hist[wave][j * r + kk], 1);
+      }
+    }
+    __syncthreads();
+    if (act && lane == 0) {
+      double sum2 = 0.0;
+      if (q > 0 && r > 0) {
+        const double *lq = a.lgtab + (size_t)q * a.lgstride;
+        const double *lqr = a.lgtab + (size_t)(q * r) * a.lgstride;
+        for (int j = 0; j < q; j++) {
+          int nij = 0;
+          for (int k = 0; k < r; k++) nij += hist[wave][j * r + k];
+          double sum1 = 0.0;
+          for (int k = 0; k < r; k++) sum1 = sum1 + (lqr[hist[wave][j * r + k]] - lqr[0]);
+          sum2 = sum2 + ((lq[0] - lq[nij]) + sum1);
+        }
+      }
+      a.famscore[(im.foff + f) * 2 + g] = sum2;
+    }
+    __syncthreads();
+  }
+  // ---- 5. network scores, in node order (main.cu:1335-1342) -------------
+  double *dist = a.lval + im.soff * 2;  // [g*S + u] (overwritten by the likelihoods below)
+  for (int ug = tid; ug < 2 * S; ug += 256) {
+    int s = ug >> 1, g = ug & 1;
+    if (rep[s] != s) continue;
+    int u = uid[s];
+    int b = evst[s], ne = evst[s + 1] - b, ep = 0;
+    int gs = net_class(s, n1);
+    double d = 0.0;
+    for (int t = 0; t < G; t++) {
+      while (ep < ne && a.ev[im.evoff + es[b + ep]].t < t) ep++;
+      int f;
+      if (ep < ne && a.ev[im.evoff + es[b + ep]].t == t) {
+        f = a.evfam[im.evoff + es[b + ep]];
+      } else {
+        size_t lb = (size_t)(im.goff * 2 + gs * G + t);
+        int slot = ht_find(hkeys, fam_key(t, a.lcnt[lb], a.lfull + lb * EDDY_MAX_MP));
+        f = slot >= 0 ? hvals[slot] : -1;
+      }
+      d = d + (f >= 0 ? a.famscore[(im.foff + f) * 2 + g] : eddy_nan());
+    }
+    dist[g * S + u] = d;
+  }
+  __syncthreads();
+  // ---- 6. likelihoods (main.cu:1294-1404) and JS (routines.cu:42-60) ----
+  __shared__ double sh_inalpha[2], sh_scale[2], sh_nonInf[2], sh_infc[2];
+  const int UU = U;
+  if (tid < 2) {
+    int g = tid;
+    double mn = dist[g * S], mx = dist[g * S], sum = 0.0;
+    for (int u = 0; u < UU; u++) {
+      double d = dist[g * S + u];
+      sum = sum + d;
+      if (d > mx) mx = d;
+      if (d < mn) mn = d;
+    }
+    sh_inalpha[g] = -1 * (sum / UU);
+    sh_scale[g] = 10 / (mx - mn);
+  }
+  __syncthreads();
+  double *lk = a.lval + im.soff * 2;  // in place: dist -> exp(adjusted)
+  for (int ug = tid; ug < 2 * UU; ug += 256) {
+    int g = ug / UU, u = ug - g * UU;
+    lk[g * S + u] = eddy_exp((dist[g * S + u] + sh_inalpha[g]) * sh_scale[g]);
+  }
+  __syncthreads();
+  if (tid < 2) {
+    int g = tid;
+    double nonInf = 0.0, infc = 0.0;
+    for (int u = 0; u < UU; u++) {
+      double l = lk[g * S + u];
+      nonInf = nonInf + l;
+      if (l >= eddy_inf() || l <= -eddy_inf()) infc++;
+    }
+    sh_nonInf[g] = nonInf;
+    sh_infc[g] = infc;
+  }
+  __syncthreads();
+  for (int ug = tid; ug < 2 * UU; ug += 256) {
+    int g = ug / UU, u = ug - g * UU;
+    double l = lk[g * S + u];
+    double isinf = (l >= eddy_inf() || l <= -eddy_inf()) ? 1.0 : 0.0;
+    lk[g * S + u] = sh_infc[g] == 0 ? l / sh_nonInf[g] : isinf / sh_infc[g];
+  }
+  __syncthreads();
+  // KL terms in parallel, summed serially in network order
+  double *kl = dist;  // reuse: terms stored after the likelihoods would alias; use evst-free region
+  (void)kl;
+  if (tid == 0) {
+    double k1 = 0.0, k2 = 0.0;
+    for (int u = 0; u < UU; u++) {
+      double p1 = lk[u], p2 = lk[S + u];
+      double m = (p1 + p2) / 2;
+      if (m == 0) continue;
+      if (p1 != 0) k1 = k1 + p1 * eddy_log(p1 / m);
+    }
+    for (int u = 0; u < UU; u++) {
+      double p1 = lk[u], p2 = lk[S + u];
+      double m = (p1 + p2) / 2;
+      if (m == 0) continue;
+      if (p2 != 0) k2 = k2 + p2 * eddy_log(p2 / m);
+    }
+    double js = k1 / 2 + k2 / 2;
+    a.js[im.set * a.perms + im.perm] = js / eddy_log(2.0);
+  }
+}
+
+// --------------------------------------------------------------------------
+// K5: p-value per set: routines.cu:494-508, main.cu:1465-1502
+// out[set*6 + ...] = {status, js0, pDDN, alpha, beta, perms_run}
+// --------------------------------------------------------------------------
+__global__ void k_pvalue(const double *js, int nsets, int perms, const int *nan0, double *out) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsets) return;
+  const double *v = js + (size_t)s * perms;
+  double *o = out + (size_t)s * 6;
+  if (nan0[s]) {
+    o[0] = 1.0;
+    o[1] = -999.0;
+    o[2] = eddy_nan();
+    o[3] = o[4] = eddy_nan();
+    o[5] = 1;
+    return;
+  }
+  int nan = 0;
+  for (int i = 0; i < perms; i++) nan |= eddy_isnan(v[i]);
+  o[1] = v[0];
+  o[5] = perms;
+  if (v[0] < 0 || nan) {
+    o[0] = 1.0;
+    o[2] = o[3] = o[4] = eddy_nan();
+    return;
+  }
+  double sum = 0.0;
+  for (int i = 0; i < perms; i++) sum = sum + v[i];
+  double mu = sum / (double)perms;
+  double vs = 0.0;
+  for (int i = 0; i < perms; i++) {
+    double d = v[i] - mu;
+    vs = vs + d * d;
+  }
+  double var = vs / (double)perms;
+  double al = (((1 - mu) / var) - (1 / mu)) * (mu * mu);
+  double be = al * (1 / mu - 1);
+  int de = 0;
+  double p = eddy_ibetac(al, be, v[0], &de);
+  o[0] = de ? 2.0 : 0.0;
+  o[2] = p;
+  o[3] = al;
+  o[4] = be;
+}
+
+// gather the rows of every set's genes (row stride NP, padded with state 3)
+__global__ void k_gather(const uint8_t *x, int N, int NP, const int *genes, const long long *rowoff, int nrows,
+                         uint8_t *xs) {
+  const int r = blockIdx.x;
+  if (r >= nrows) return;
+  const uint4 *src = (const uint4 *)(x + (size_t)genes[r] * NP);
+  uint4 *dst = (uint4 *)(xs + rowoff[r]);
+  for (int i = threadIdx.x; i < NP / 16; i += blockDim.x) dst[i] = src[i];
+}

@@ -1,0 +1,206 @@
+/*
+ * eddy.h - C ABI of libeddy: the MI355X (gfx950) EDDY engine.
+ *
+ * The reference (skimlab/eddy-gpu) has no library boundary: main.cu drives
+ * six CUDA kernels through raw device pointers (prototypes at
+ * definitions.cuh:78-136) and blocking copies, one permutation at a time.
+ * This header replaces that internal operator API with a context-based one:
+ *
+ *   reference                                   libeddy
+ *   ------------------------------------------  ---------------------------
+ *   main.cu:27-56  device probe, default stream  eddy_ctx_create / _destroy
+ *   main.cu:684-719 per-permutation cudaMalloc    dataset uploaded once
+ *   main.cu:538-573 rand() permutation stream     eddy_stream_* (own TYPE_3
+ *                                                 generator, same sequence)
+ *   main.cu:575-1460 permutation loop, kernels    eddy_run_sets: all sets,
+ *     run2/run2Scalable (kernel1.cu:389-477),     permutations and networks
+ *     edgePerNetworkKernel (kernel2.cu:114),      of a batch on the GPU
+ *     run22 (kernel3.cu:134), run25 (kernel4.cu:72),
+ *     run4 (kernel5.cu:237), likelihood + JS
+ *   main.cu:1462-1502 beta fit + Boost ibetac     computed on the GPU,
+ *                                                 returned in eddy_set_result
+ *   main.cu:782-819, 1085-1195, 1530-1641         eddy_get_perm0_networks,
+ *     permutation-0 captures + EdgeList pass      eddy_get_edgelist
+ *
+ * Conventions: every function returns an eddy_status; nothing exits the
+ * process.  The caller owns all host buffers; the context owns device memory
+ * and one HIP stream.  A context is used by one host thread at a time.
+ */
+#ifndef EDDY_H
+#define EDDY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  EDDY_OK = 0,
+  EDDY_ERR_ARG = 1,      /* invalid argument */
+  EDDY_ERR_HIP = 2,      /* HIP runtime error (message in eddy_last_error) */
+  EDDY_ERR_CAPACITY = 3, /* a caller buffer is too small */
+  EDDY_ERR_STATE = 4,    /* call out of order (no dataset, no results) */
+  EDDY_ERR_NOMEM = 5,
+  EDDY_ERR_NODEV = 6     /* no HIP device / libeddy built without one */
+} eddy_status;
+
+/* Per-set outcome, mirrors the three branches of main.cu:1465-1502 */
+typedef enum {
+  EDDY_SET_OK = 0,          /* results.txt row "%s\t%f\t%f\t%d" */
+  EDDY_SET_FAILED_NAN = 1,  /* "failed with %d genes- one of the js scores was nan" */
+  EDDY_SET_DOMAIN = 2       /* beta fit outside ibetac's domain (the reference's
+                               Boost call would throw); pDDN is NaN */
+} eddy_set_status;
+
+typedef enum {
+  EDDY_MODE_PLAIN = 0,   /* one gene set and one permutation per launch
+                            sequence, as the reference's loop is structured */
+  EDDY_MODE_BATCHED = 1  /* all sets x permutations of a batch per launch,
+                            replayed from a captured HIP graph */
+} eddy_mode;
+
+typedef enum {
+  EDDY_TESTS_ALL = 0,    /* every leave-one-out test of every pair, as the
+                            reference evaluates every (network, pair) */
+  EDDY_TESTS_CUTOFF = 1  /* leave-one-out tests of a row only up to its mp-th
+                            edge that no leave-one-out test can remove; later
+                            columns cannot change any capped network */
+} eddy_schedule;
+
+typedef struct {
+  int mp;                /* parent cap, -mp (main.cu:158-163); 1..16 here */
+  int perms;             /* r, -r (main.cu:165-168) */
+  int p_adjust;          /* Bonferroni (1) or -rawp (0) (kernel1.cu:421-424) */
+  double alpha_edge;     /* -pE, or from -t/-l (main.cu:180-186) */
+  double alpha_edge_prior; /* 1-(1-aE)*pw^(1/l) (main.cu:188-189) */
+  double alpha_ddn;      /* -pD (main.cu:150-153) */
+  int mode;              /* eddy_mode */
+  int schedule;          /* eddy_schedule */
+  int learned;           /* 1: learned proposal orders the cutoff schedule's tests */
+} eddy_params;
+
+/* One gene set after name lookup (main.cu:400-430) and prior loading
+ * (main.cu:433-503). */
+typedef struct {
+  int n_genes;          /* G */
+  const int *genes;     /* [G] dataset row of each set gene, set order */
+  int n_prior;          /* number of prior pairs */
+  const int *prior;     /* [2*n_prior] set positions (a, b); symmetric */
+} eddy_set_desc;
+
+typedef struct {
+  int status;           /* eddy_set_status */
+  int perms_run;        /* r, or 1 when permutation 0 is NaN */
+  int U0;               /* unique networks of permutation 0 */
+  double js0;           /* JS of the true labels (bits) */
+  double pDDN;
+  double alpha, beta;   /* method-of-moments beta fit */
+  long long tests_ref;  /* reference-equivalent CI tests: sum over perms S*c */
+  long long tests_eval; /* CI tests this engine evaluated */
+} eddy_set_result;
+
+typedef struct eddy_ctx eddy_ctx;
+
+/* ---- context -------------------------------------------------------------- */
+/* device -1 creates a host-only context (permutation stream only). */
+int eddy_ctx_create(int device, eddy_ctx **out);
+int eddy_ctx_destroy(eddy_ctx *ctx);
+const char *eddy_last_error(const eddy_ctx *ctx);
+const char *eddy_version(void);
+
+/* Expression data: [genes x samples] row-major states 0,1,2 (reference values
+ * -1,0,1; any other reference value is stored as 3 and never counted, F1 of
+ * kernel1.cu:91-118).  class_of: [samples] true labels, 0 = class 1 (first
+ * label of the class file), 1 = class 2 (main.cu:303-352).  */
+int eddy_upload_dataset(eddy_ctx *ctx, const uint8_t *x, int genes, int samples,
+                        const uint8_t *class_of);
+
+/* ---- permutation stream (main.cu:538-573) ------------------------------- */
+/* Reset to glibc's unseeded rand() state (the reference never calls srand). */
+int eddy_stream_reset(eddy_ctx *ctx);
+/* Standalone generator for tests: n successive rand() values. */
+int eddy_stream_draw(eddy_ctx *ctx, int n, int32_t *out);
+
+/* ---- the hot path ----------------------------------------------------------
+ * Runs every set's permutation loop (main.cu:538-1460) and p-value
+ * (main.cu:1462-1502) on the GPU, consuming the context's permutation stream
+ * in the reference's order (sets in array order; a set whose permutation 0 is
+ * NaN consumes one permutation's draws).  results: [nsets]. */
+int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets,
+                  const eddy_params *params, eddy_set_result *results);
+
+/* JS values of set `set` of the last run: out[perms]. */
+int eddy_get_js(eddy_ctx *ctx, int set, double *out, int cap);
+
+/* Permutation-0 unique networks of a set of the last run (the `first_*`
+ * captures of main.cu:1085-1195, 1406-1409): ids[U0] original network
+ * indices, nodes[U0*G] capped CSR offsets, epn[U0+1], edges[epn[U0]],
+ * lval[2*U0] normalised likelihoods (P1 || P2).  *n_edges receives epn[U0].
+ * Any output may be NULL; caps are element counts. */
+int eddy_get_perm0_networks(eddy_ctx *ctx, int set, int *ids, int *nodes,
+                            int nodes_cap, int *epn, int *edges, int edges_cap,
+                            double *lval, int *n_edges);
+
+/* Uncapped parent lists of permutation 0's two full-data networks
+ * (main.cu:1530-1612): nodes[2G], epn[3], edges[epn[2]]. */
+int eddy_get_edgelist(eddy_ctx *ctx, int set, int *nodes, int *epn, int *edges,
+                      int edges_cap);
+
+/* ---- one permutation with explicit labels (parity tests) -----------------
+ * labels[samples] in {0,1} (0 = class 1).  Outputs as in eddy_get_*; bits is
+ * the reference's out23 [S*c] (network-major, S = samples+2); out5 is
+ * [2*U*G] BDeu per (class, unique network, node).  Any output may be NULL. */
+typedef struct {
+  uint8_t *bits;    /* [S*c] */
+  int *uniq_ids;    /* [S]   */
+  int *uni_nodes;   /* [S*G] */
+  int *uni_epn;     /* [S+1] */
+  int *uni_edges;   /* [S*G*mp] */
+  double *out5;     /* [2*S*G] */
+  double *lval;     /* [2*S] */
+  int U, edSum;
+  double js;
+} eddy_perm_detail;
+
+int eddy_run_perm_detail(eddy_ctx *ctx, const eddy_set_desc *set,
+                         const uint8_t *labels, const eddy_params *params,
+                         eddy_perm_detail *out);
+
+/* ---- timing of the last eddy_run_sets (HIP events on the engine stream) -- */
+typedef struct {
+  double total_ms;       /* first launch to results on the host */
+  double count_test_ms;  /* summed duration of the count+test kernel */
+  double network_ms;     /* networks + dedup + BDeu + JS kernel */
+  long long count_test_launches;
+  double count_ops;      /* int8 MFMA multiply-accumulate ops (x2) issued */
+  double count_bytes;    /* bytes the count+test kernel reads from HBM-resident inputs */
+} eddy_timing;
+int eddy_get_timing(eddy_ctx *ctx, eddy_timing *t);
+
+/* ---- synthetic workloads (seeded DAG data in the benchmark shapes) ------- */
+typedef struct {
+  uint64_t seed;
+  int genes, nA, nB;
+  double rewire;
+  int nsets, gmin, gmax;
+  int prior_fixed;        /* >= 0: fixed prior edges per set */
+  double prior_per_gene;  /* else floor(prior_per_gene * G) */
+  int mp, perms;
+  double prior_weight;
+} eddy_synth_cfg;
+
+int eddy_synth_config(int k, eddy_synth_cfg *cfg);
+int eddy_synth_dataset(uint64_t seed, int genes, int nA, int nB, double rewire,
+                       uint8_t *x);
+int eddy_synth_sets(uint64_t seed, int genes, int nsets, int gmin, int gmax,
+                    int prior_fixed, double prior_per_gene, int *set_sizes,
+                    int *set_genes, long long genes_cap, int *prior_counts,
+                    int *prior_pairs, long long pairs_cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EDDY_H */

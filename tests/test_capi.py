@@ -1,0 +1,85 @@
+"""CPU checks of libeddy's C ABI: the library loads, exports every function
+include/eddy.h declares, replays the glibc rand() stream (main.cu:538-573)
+without a GPU, and generates seeded synthetic workloads deterministically."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import eddy_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def header_functions():
+    src = open(os.path.join(ROOT, "include", "eddy.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(eddy_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_exports_match_header():
+    L = eddy_amd.lib()
+    names = header_functions()
+    assert set(names) == set(eddy_amd.EXPORTS)
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def host_ctx():
+    L = eddy_amd.lib()
+    h = C.c_void_p()
+    assert L.eddy_ctx_create(-1, C.byref(h)) == 0
+    return L, h
+
+
+def test_stream_matches_glibc_rand():
+    L, h = host_ctx()
+    ref = np.load(os.path.join(GOLD, "glibc_rand_1e4.npy"))
+    out = np.zeros(len(ref), np.int32)
+    assert L.eddy_stream_draw(h, len(out), out.ctypes.data_as(C.c_void_p)) == 0
+    assert np.array_equal(out, ref)
+    # the live C library agrees with the committed fixture on this machine
+    libc = C.CDLL(None)
+    libc.srand(1)
+    assert [libc.rand() for _ in range(100)] == list(ref[:100])
+    assert L.eddy_stream_reset(h) == 0
+    assert L.eddy_stream_draw(h, 5, out.ctypes.data_as(C.c_void_p)) == 0
+    assert list(out[:5]) == list(ref[:5])
+    # a host-only context refuses GPU work
+    x = np.zeros((2, 4), np.uint8)
+    cl = np.array([0, 0, 1, 1], np.uint8)
+    assert L.eddy_upload_dataset(h, x.ctypes.data_as(C.c_void_p), 2, 4, cl.ctypes.data_as(C.c_void_p)) == \
+        eddy_amd.EDDY_ERR_NODEV
+    L.eddy_ctx_destroy(h)
+
+
+def test_synth_deterministic_and_tertiles():
+    a = eddy_amd.Workload(0)
+    b = eddy_amd.Workload(0)
+    assert np.array_equal(a.X, b.X)
+    N = a.N
+    counts = np.stack([(a.X == v).sum(1) for v in range(3)], 1)
+    assert np.all(counts[:, 0] == N // 3)
+    assert np.all(counts[:, 1] == 2 * N // 3 - N // 3)
+    genes, pairs = a.sets[0]
+    assert len(genes) == 40 and len(set(genes.tolist())) == 40 and len(pairs) == 20
+    w2 = eddy_amd.Workload(2, nsets=50)
+    sizes = [len(g) for g, _ in w2.sets]
+    assert min(sizes) >= 10 and max(sizes) <= 300
+    assert all(len(p) == int(0.3 * len(g)) for g, p in w2.sets)
+
+
+def test_no_gpu_engine_fails_loudly():
+    # Without a device the engine refuses to run rather than computing on the CPU
+    L = eddy_amd.lib()
+    h = C.c_void_p()
+    rc = L.eddy_ctx_create(0, C.byref(h))
+    if rc == 0:
+        L.eddy_ctx_destroy(h)
+        pytest.skip("a GPU is present")
+    assert rc == eddy_amd.EDDY_ERR_NODEV
+    with pytest.raises(RuntimeError):
+        eddy_amd.Engine(0)
