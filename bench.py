@@ -1,0 +1,222 @@
+#!/usr/bin/env python3
+"""bench.py - EDDY permutation loop on one MI355X.
+
+One step = eddy_run_sets over the whole batched workload (500 gene sets of the
+5000-gene x 600-sample synthetic dataset, sizes log-uniform in 10..300, r = 100
+permutations each, mp = 3, pw = 0.5), with the batched engine and the learned
+cutoff schedule.  The dataset is resident in HBM before the timed region; each
+step replays the same permutation stream (stream reset) so every step is the
+same work.  Prints one JSON line:
+
+  value          gene sets/s (whole job, r = 100), batched + learned
+  plain          the plain GPU form (one set and one permutation per launch
+                 sequence, every leave-one-out test) on a prefix of the sets
+  ci_tests       reference-equivalent and evaluated chi-square tests/s
+  roofline       the dominant kernel against MI355X's published peak
+  cpu_baseline   the CPU oracle (reference algorithm) on a bounded sample,
+                 extrapolated to the same workload
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "eddy-gpu_amd")
+sys.path.insert(0, PKG)
+
+PEAK_INT8_OPS = 5.0e15      # dense MFMA int8, OP/s
+PEAK_FP64 = 78.6e12         # FP64, FLOP/s
+PEAK_HBM = 8.0e12           # B/s
+
+
+def ensure_built():
+    if not os.path.exists(os.path.join(PKG, "libeddy.so")):
+        subprocess.run([sys.executable, os.path.join(ROOT, "__graft_entry__.py")], check=True)
+
+
+def algorithmic_work(sets, N, perms_run):
+    """Per-step algorithmic quantities summed over all (set, permutation) items."""
+    ops = 0.0      # one-hot GEMM int8 MACs x2: both classes, lower triangle: 18 c N per item
+    ref_tests = 0  # reference-equivalent CI tests: S c per item
+    for (genes, _), r in zip(sets, perms_run):
+        G = len(genes)
+        c = G * (G - 1) // 2
+        ops += r * 18.0 * c * N
+        ref_tests += r * (N + 2) * c
+    return ops, ref_tests
+
+
+def cpu_baseline(wl, perms_run, seconds, threads):
+    """Oracle (reference algorithm, full LOO tally per network and pair) on the
+    first (set, permutation) items until `seconds` of CPU work, extrapolated by
+    the reference's cost S*c*n per item to the whole step."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ref
+
+    data_ref = wl.X.astype(np.int32) - 1
+    N = wl.N
+    rng = np.random.default_rng(0)
+    done_cost, t0, items = 0.0, time.perf_counter(), 0
+    for s, (genes, _) in enumerate(wl.sets):
+        if len(genes) < 2:
+            continue
+        prior = wl.prior_matrix(s)
+        lab = wl.classes if items == 0 else rng.permutation(wl.classes)
+        t1, t2, n1, n2 = oracle_ref.class_split(data_ref, genes, lab)
+        prm = oracle_ref.params(mp=wl.cfg.mp, threads=threads)
+        oracle_ref.run_perm(t1, t2, n1, n2, prior, prm, want_bits=False)
+        G = len(genes)
+        done_cost += (N + 2) * (G * (G - 1) // 2) * N
+        items += 1
+        if time.perf_counter() - t0 > seconds and items >= 2:
+            break
+    dt = time.perf_counter() - t0
+    total_cost = sum(r * (N + 2) * (len(g) * (len(g) - 1) // 2) * N for (g, _), r in zip(wl.sets, perms_run))
+    t_total = dt * total_cost / done_cost
+    return {
+        "value": len(wl.sets) / t_total,
+        "unit": "gene sets/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "%d (set, permutation) items of the same workload in %.1f s with %d OpenMP threads, "
+                  "extrapolated by S*c*n to %d sets at r=%d" % (items, dt, threads, len(wl.sets), wl.cfg.perms),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=3)
+    ap.add_argument("--nsets", type=int, default=None)
+    ap.add_argument("--plain-sets", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; multi-GPU sharding is not built (see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    ensure_built()
+    import eddy_amd
+
+    t_setup = time.perf_counter()
+    wl = eddy_amd.Workload(args.config, nsets=args.nsets)
+    eng = eddy_amd.Engine(0)
+    eng.upload(wl.X, wl.classes)
+    setup_s = time.perf_counter() - t_setup
+    nsets = len(wl.sets)
+    prm = wl.params(mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=1)
+
+    print("bench: setup %.1fs, %d sets, N=%d" % (setup_s, nsets, wl.N), file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        eng.reset_stream()
+        tw = time.perf_counter()
+        eng.run_sets(wl.sets, prm)
+        print("bench: warmup %d %.2fs" % (w, time.perf_counter() - tw), file=sys.stderr, flush=True)
+    kern_ms = [0.0] * 6
+    launches = [0] * 6
+    tests_eval = 0
+    items = 0
+    host_stream_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.reset_stream()
+        res = eng.run_sets(wl.sets, prm)
+        tm = eng.timing()
+        for k in range(6):
+            kern_ms[k] += tm.kernel_ms[k]
+            launches[k] += tm.launches[k]
+        tests_eval += tm.tests
+        items += tm.items
+        host_stream_ms += tm.host_stream_ms
+        print("bench: step done, total %.2fs" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1000.0 * elapsed / args.steps
+    value = nsets * args.steps / elapsed
+    perms_run = [r.perms_run for r in res]
+    ops, ref_tests = algorithmic_work(wl.sets, wl.N, perms_run)
+    js_batched = [eng.js(s).copy() for s in range(min(args.plain_sets, nsets))]
+
+    # plain GPU form on a prefix of the sets (same stream, same results)
+    plain = None
+    if args.plain_sets > 0:
+        sub = wl.sets[:args.plain_sets]
+        pprm = wl.params(mode=eddy_amd.MODE_PLAIN, schedule=eddy_amd.TESTS_ALL, learned=0)
+        eng.reset_stream()
+        tp = time.perf_counter()
+        pres = eng.run_sets(sub, pprm)
+        dtp = time.perf_counter() - tp
+        same = all(
+            ((r.status == q.status) and (r.perms_run == q.perms_run)) for r, q in zip(pres, res[:len(sub)]))
+        for s in range(len(sub)):
+            a, b = eng.js(s), js_batched[s]
+            n = pres[s].perms_run
+            same = same and bool((a[:n].view("u8") == b[:n].view("u8")).all())
+        bprm = wl.params(mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=1)
+        eng.reset_stream()
+        tb = time.perf_counter()
+        eng.run_sets(sub, bprm)
+        dtb = time.perf_counter() - tb
+        plain = {"sets": len(sub), "plain_sets_per_s": len(sub) / dtp,
+                 "batched_learned_sets_per_s_same_sets": len(sub) / dtb,
+                 "speedup": dtp / dtb, "js_bit_identical": same}
+
+    # roofline of the dominant kernel
+    dom = max(range(6), key=lambda k: kern_ms[k])
+    dom_avg_s = kern_ms[dom] / max(1, launches[dom]) / 1000.0
+    if dom == 0:
+        achieved = ops * args.steps / max(1, launches[0]) / dom_avg_s
+        roof = {"kernel": "k_count_test", "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_INT8_OPS / 1e12,
+                "unit": "TOP/s", "frac": achieved / PEAK_INT8_OPS, "traffic": None}
+    else:
+        roof = {"kernel": eddy_amd.KERNELS[dom], "bound": "hbm", "achieved": None, "peak": PEAK_HBM / 1e9,
+                "unit": "GB/s", "frac": None, "traffic": None}
+    roof["avg_launch_ms"] = dom_avg_s * 1000.0
+    roof["launches_per_step"] = launches[dom] / args.steps
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        threads = max(1, min(threads, 16))
+        cpu = cpu_baseline(wl, perms_run, args.cpu_seconds, threads)
+
+    line = {
+        "metric": "gene sets/s at r=100 perms (batched+learned); CI tests/s",
+        "value": value,
+        "unit": "gene sets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8 counts / fp64 tests",
+        "data": "synthetic (seeded linear-Gaussian DAG, tertiles)",
+        "config": {"workload": "config %d: %d genes x %d samples (%d/%d), %d gene sets (G %d..%d log-uniform), "
+                               "r=%d, mp=%d, pw=%.1f" % (args.config, wl.cfg.genes, wl.N, wl.cfg.nA, wl.cfg.nB, nsets,
+                                                          wl.cfg.gmin, wl.cfg.gmax, wl.cfg.perms, wl.cfg.mp,
+                                                          wl.cfg.prior_weight),
+                   "global_batch": nsets, "parallelism": "single GPU"},
+        "ci_tests_per_s_ref_equiv": ref_tests * args.steps / elapsed,
+        "ci_tests_per_s_evaluated": tests_eval / elapsed,
+        "items_per_step": items // args.steps,
+        "kernel_ms_per_step": {eddy_amd.KERNELS[k]: kern_ms[k] / args.steps for k in range(6)},
+        "host_stream_ms_per_step": host_stream_ms / args.steps,
+        "setup_s": setup_s,
+        "plain": plain,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -336,32 +336,33 @@ int run_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, const 
     a.status = (int *)ctx->b_status.p;
     a.tests = (long long *)ctx->b_tests.p;
 
-    hipEvent_t e0, e1, e2;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventCreate(&e2);
-    hipEventRecord(e0, st);
+    hipEvent_t ev[5];
+    for (int k = 0; k < 5; k++) hipEventCreate(&ev[k]);
+    hipEventRecord(ev[0], st);
     if (!ch.tiles.empty()) k_count_test<<<(unsigned)ch.tiles.size(), 256, 0, st>>>(a);
+    hipEventRecord(ev[1], st);
     if (!plain_tests) k_loo<<<nitems, 256, 0, st>>>(a);
-    hipEventRecord(e1, st);
+    hipEventRecord(ev[2], st);
     k_rows<<<nitems, 256, 0, st>>>(a);
+    hipEventRecord(ev[3], st);
     k_score<<<nitems, 256, 0, st>>>(a);
-    hipEventRecord(e2, st);
+    hipEventRecord(ev[4], st);
     HIPCHK(ctx, hipGetLastError());
     std::vector<int> status(nitems);
     HIPCHK(ctx, hipMemcpyAsync(status.data(), ctx->b_status.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
     std::vector<long long> tests(nitems);
     HIPCHK(ctx, hipMemcpyAsync(tests.data(), ctx->b_tests.p, sizeof(long long) * nitems, hipMemcpyDeviceToHost, st));
     HIPCHK(ctx, hipStreamSynchronize(st));
-    float ms1 = 0, ms2 = 0;
-    hipEventElapsedTime(&ms1, e0, e1);
-    hipEventElapsedTime(&ms2, e1, e2);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipEventDestroy(e2);
-    ctx->timing.count_test_ms += ms1;
-    ctx->timing.network_ms += ms2;
-    ctx->timing.count_test_launches += 1;
+    const int kidx[4] = {0, 1, 2, 3};
+    for (int k = 0; k < 4; k++) {
+      if (k == 1 && plain_tests) continue;
+      if (k == 0 && ch.tiles.empty()) continue;
+      float ms = 0;
+      hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+      ctx->timing.kernel_ms[kidx[k]] += ms;
+      ctx->timing.launches[kidx[k]] += 1;
+    }
+    for (int k = 0; k < 5; k++) hipEventDestroy(ev[k]);
     int bad = 0;
     for (int s : status) bad |= s;
     if (bad & ST_NOT_EVALUATED) {
@@ -372,7 +373,12 @@ int run_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, const 
       if (bad & (ST_EV_OVERFLOW | ST_FAM_OVERFLOW)) ev_cap *= 4;
       continue;  // retry the chunk (larger capacity / new hash seed)
     }
-    for (int i = 0; i < nitems; i++) tests_out[i] = tests[i];
+    for (int i = 0; i < nitems; i++) {
+      tests_out[i] = tests[i];
+      ctx->timing.tests += tests[i];
+    }
+    ctx->timing.items += nitems;
+    ctx->timing.pairs += ch.pairs;
     if (capture) {
       // copy back permutation-0 state of every item (one item per set)
       std::vector<int> evc(nitems), uq(nitems), fc(nitems);
@@ -575,14 +581,21 @@ extern "C" int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
     }
     // phase 2: replay the stream in the reference's order, batch the rest
     for (int s = 0; s < nsets; s++) {
+      auto h0 = std::chrono::steady_clock::now();
       ctx->stream_gen.permutation(N, order);  // permutation 0's draws
-      if (nan0[s]) continue;
+      if (nan0[s]) {
+        ctx->timing.host_stream_ms +=
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+        continue;
+      }
       ran[s] = perms;
       for (int pm = 1; pm < perms; pm++) {
         ctx->stream_gen.permutation(N, order);
         add_item(ch, meta, s, pm, label_bits(perm_labels(order, ctx->n1)), N);
-        if (ch.pairs > pair_budget) CHK(flush(ch, false));
       }
+      ctx->timing.host_stream_ms +=
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+      if (ch.pairs > pair_budget) CHK(flush(ch, false));
     }
     CHK(flush(ch, false));
   } else {
@@ -616,8 +629,20 @@ extern "C" int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
   std::vector<double> pout((size_t)6 * nsets);
   if (nsets) {
     HIPCHK(ctx, hipMemcpyAsync(ctx->b_nan0.p, nan0.data(), sizeof(int) * nsets, hipMemcpyHostToDevice, ctx->stream));
+    hipEvent_t q0, q1;
+    hipEventCreate(&q0);
+    hipEventCreate(&q1);
+    hipEventRecord(q0, ctx->stream);
     k_pvalue<<<(nsets + 63) / 64, 64, 0, ctx->stream>>>((const double *)ctx->b_js.p, nsets, perms,
                                                          (const int *)ctx->b_nan0.p, (double *)ctx->b_pout.p);
+    hipEventRecord(q1, ctx->stream);
+    hipEventSynchronize(q1);
+    float qms = 0;
+    hipEventElapsedTime(&qms, q0, q1);
+    ctx->timing.kernel_ms[4] += qms;
+    ctx->timing.launches[4] += 1;
+    hipEventDestroy(q0);
+    hipEventDestroy(q1);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipMemcpyAsync(pout.data(), ctx->b_pout.p, sizeof(double) * 6 * nsets, hipMemcpyDeviceToHost,
                                ctx->stream));

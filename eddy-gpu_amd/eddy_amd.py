@@ -51,10 +51,13 @@ class PermDetail(C.Structure):
                 ("lval", C.c_void_p), ("U", C.c_int), ("edSum", C.c_int), ("js", C.c_double)]
 
 
+KERNELS = ["count_test", "loo", "rows", "score", "pvalue", "gather"]
+
+
 class Timing(C.Structure):
-    _fields_ = [("total_ms", C.c_double), ("count_test_ms", C.c_double), ("network_ms", C.c_double),
-                ("count_test_launches", C.c_longlong), ("count_ops", C.c_double),
-                ("count_bytes", C.c_double)]
+    _fields_ = [("total_ms", C.c_double), ("kernel_ms", C.c_double * 6), ("launches", C.c_longlong * 6),
+                ("items", C.c_longlong), ("pairs", C.c_longlong), ("tests", C.c_longlong),
+                ("host_stream_ms", C.c_double)]
 
 
 class SynthCfg(C.Structure):

@@ -168,14 +168,19 @@ int eddy_run_perm_detail(eddy_ctx *ctx, const eddy_set_desc *set,
                          const uint8_t *labels, const eddy_params *params,
                          eddy_perm_detail *out);
 
-/* ---- timing of the last eddy_run_sets (HIP events on the engine stream) -- */
+/* ---- timing of the last run (HIP events on the engine stream) ----------
+ * kernel index: 0 count_test (tables on MFMA + tests), 1 loo (cutoff
+ * schedule), 2 rows (networks + hashes), 3 score (dedup, BDeu, JS),
+ * 4 pvalue, 5 gather. */
+#define EDDY_NKERNELS 6
 typedef struct {
-  double total_ms;       /* first launch to results on the host */
-  double count_test_ms;  /* summed duration of the count+test kernel */
-  double network_ms;     /* networks + dedup + BDeu + JS kernel */
-  long long count_test_launches;
-  double count_ops;      /* int8 MFMA multiply-accumulate ops (x2) issued */
-  double count_bytes;    /* bytes the count+test kernel reads from HBM-resident inputs */
+  double total_ms;                   /* call entry to results on the host */
+  double kernel_ms[EDDY_NKERNELS];   /* summed device time per kernel */
+  long long launches[EDDY_NKERNELS];
+  long long items;                   /* (set, permutation) items processed */
+  long long pairs;                   /* sum over items of G(G-1)/2 */
+  long long tests;                   /* chi-square tests evaluated */
+  double host_stream_ms;             /* host time replaying the rand() stream */
 } eddy_timing;
 int eddy_get_timing(eddy_ctx *ctx, eddy_timing *t);
 

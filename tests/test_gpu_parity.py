@@ -5,6 +5,8 @@ outputs (BDeu per node, likelihoods, JS) are compared bit for bit as well,
 since both sides round through the same FP64 routines (tolerance 1e-12
 relative is the stated bar; the assertions use exact equality, which implies it).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -49,3 +51,60 @@ def test_perm_parity_config0(engine, schedule):
     for seed in range(3):
         lab = wl.classes if seed == 0 else _labels(N, n1, seed)
         _compare_perm(engine, wl, 0, lab, schedule, check_bits=(schedule == eddy_amd.TESTS_ALL))
+
+
+def _oracle_threads():
+    return max(1, min(8, os.cpu_count() or 1))
+
+
+def test_sets_stream_parity(engine):
+    """Whole gene sets through the permutation stream: several sets in a row,
+    one of which is NaN at permutation 0 (a single network) and so consumes
+    only one permutation's draws (main.cu:1431-1435); JS values, pDDN and the
+    failed status must match the oracle run through glibc rand()."""
+    wl = eddy_amd.Workload(0)
+    engine.upload(wl.X, wl.classes)
+    genes, pairs = wl.sets[0]
+    sets = [(genes[:12], pairs[:0]), (genes[12:14], pairs[:0]), (genes[14:30], pairs[:0]), (genes[:1], pairs[:0]),
+            (genes[30:40], pairs[:0])]
+    data_ref = wl.X.astype(np.int32) - 1
+    for mode, sched in [(eddy_amd.MODE_BATCHED, eddy_amd.TESTS_CUTOFF), (eddy_amd.MODE_PLAIN, eddy_amd.TESTS_ALL)]:
+        prm = wl.params(perms=8, mode=mode, schedule=sched)
+        print("case: stream parity mode", mode, flush=True)
+        engine.reset_stream()
+        res = engine.run_sets(sets, prm)
+        oracle_ref.reset_stream()
+        for s, (g, _) in enumerate(sets):
+            prior = np.zeros((len(g), len(g)), np.int32)
+            o = oracle_ref.run_set(data_ref, wl.classes.astype(np.int32), g, prior,
+                                   oracle_ref.params(mp=3, perms=8, threads=_oracle_threads()))
+            r = res[s]
+            assert r.perms_run == o["ran"]
+            assert (r.status == eddy_amd.SET_FAILED_NAN) == bool(o["failed"])
+            js = engine.js(s)
+            n = o["ran"]
+            assert np.array_equal(js[:n].view(np.uint64), o["js"][:n].view(np.uint64)), s
+            if not o["failed"]:
+                assert r.pDDN == o["pDDN"] or (np.isnan(r.pDDN) and np.isnan(o["pDDN"]))
+                assert r.U0 == o["U0"]
+
+
+def test_perm_parity_config1_cutoff(engine):
+    """Config-1 shape (150 genes, 400 samples): networks, BDeu, likelihoods, JS."""
+    wl = eddy_amd.Workload(1)
+    engine.upload(wl.X, wl.classes)
+    genes, pairs = wl.sets[0]
+    prm = wl.params(schedule=eddy_amd.TESTS_CUTOFF, mode=eddy_amd.MODE_BATCHED)
+    print("case: config1 cutoff", flush=True)
+    g = engine.perm_detail(genes, pairs, wl.classes, prm, bits=False, out5=True)
+    data_ref = wl.X.astype(np.int32) - 1
+    t1, t2, n1, n2 = oracle_ref.class_split(data_ref, genes, wl.classes)
+    o = oracle_ref.run_perm(t1, t2, n1, n2, wl.prior_matrix(0), oracle_ref.params(mp=3, threads=_oracle_threads()),
+                            want_bits=False)
+    assert g["U"] == o["U"]
+    assert np.array_equal(g["uniq_ids"], o["uniq_ids"])
+    assert np.array_equal(g["uni_nodes"], o["uni_nodes"])
+    assert np.array_equal(g["uni_edges"], o["uni_edges"])
+    assert np.array_equal(g["out5"].view(np.uint64), o["out5"].view(np.uint64))
+    assert np.array_equal(g["lval"].view(np.uint64), o["lval"].view(np.uint64))
+    assert g["js"] == o["js"]
