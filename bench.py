@@ -125,6 +125,7 @@ def main():
     tests_eval = 0
     items = 0
     host_stream_ms = 0.0
+    sched = [0, 0, 0, 0]
     t0 = time.perf_counter()
     for _ in range(args.steps):
         eng.reset_stream()
@@ -136,6 +137,8 @@ def main():
         tests_eval += tm.tests
         items += tm.items
         host_stream_ms += tm.host_stream_ms
+        for k in range(4):
+            sched[k] += tm.sched[k]
         print("bench: step done, total %.2fs" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     ms_per_step = 1000.0 * elapsed / args.steps
@@ -210,6 +213,8 @@ def main():
         "items_per_step": items // args.steps,
         "kernel_ms_per_step": {eddy_amd.KERNELS[k]: kern_ms[k] / args.steps for k in range(6)},
         "host_stream_ms_per_step": host_stream_ms / args.steps,
+        "schedule": {"rows_past_plan": sched[0] // args.steps, "cols_tested_past_plan": sched[1] // args.steps,
+                     "planned_entries": sched[2] // args.steps, "rows": sched[3] // args.steps},
         "setup_s": setup_s,
         "plain": plain,
         "roofline": roof,

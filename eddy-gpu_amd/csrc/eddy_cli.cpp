@@ -10,6 +10,7 @@
 //     alias of `-pD`;
 //   * `--mode plain|batched`, `--tests all|cutoff` and `--no-learned` choose
 //     the engine schedule (results are identical in every mode);
+//     `--parse-only` prints the parsed inputs and stops (no GPU);
 //   * a degenerate beta fit (e.g. -r 1), where the reference's Boost call
 //     throws and aborts, prints pDDN as nan and writes no files;
 //   * inputs the reference would read out of bounds (missing class labels,
@@ -35,6 +36,7 @@ int main(int argc, char **argv) {
   double priorWeight = 0.5, alphaDDN = 0.05, theta = -1, lambda = 2.0, alphaEdge = 0.05;
   bool pAdjust = true;
   int mode = EDDY_MODE_BATCHED, schedule = EDDY_TESTS_CUTOFF, learned = 1;
+  bool parse_only = false;
   for (int i = 1; i < argc; i++) {
     std::string a = argv[i];
     const char *nx = i + 1 < argc ? argv[i + 1] : "";
@@ -55,6 +57,7 @@ int main(int argc, char **argv) {
     else if (a == "--mode") mode = std::string(nx) == "plain" ? EDDY_MODE_PLAIN : EDDY_MODE_BATCHED;
     else if (a == "--tests") schedule = std::string(nx) == "all" ? EDDY_TESTS_ALL : EDDY_TESTS_CUTOFF;
     else if (a == "--no-learned") learned = 0;
+    else if (a == "--parse-only") parse_only = true;
   }
   if (input.empty()) fail("Invalid input file entered. Exiting...");
   if (classfile.empty()) fail("Invalid class file entered. Exiting...");
@@ -90,6 +93,15 @@ int main(int argc, char **argv) {
   printf("%d samples\n%d genes\n", expr.samples, (int)expr.genes.size());
   printf("Classes: %d %s, %d %s\n", cls.n1, cls.class1.c_str(), cls.n2, cls.class2.c_str());
   for (auto &s : sets) load_prior("PRIORS/", s);
+  if (parse_only) {  // inputs as the engine would receive them; no GPU needed
+    for (auto &s : sets) {
+      printf("set %s genes %d prior %d :", s.name.c_str(), (int)s.rows.size(), (int)s.prior.size() / 2);
+      for (int r : s.rows) printf(" %d", r);
+      printf("\n");
+    }
+    fclose(res);
+    return 0;
+  }
 
   eddy_ctx *ctx = nullptr;
   int rc = eddy_ctx_create(0, &ctx);

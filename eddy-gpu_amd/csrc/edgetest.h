@@ -1,0 +1,114 @@
+// edgetest.h - the chi-square edge decision of one 3x3 tally (kernel1.cu:10-203,
+// 420-431), shared by the gfx950 kernels and a host build used by the tests.
+//
+// eddy_edge_decision() returns exactly what the reference's decision computes
+// (chi-square with its Yates rule, AS 147 series, Bonferroni, prior
+// threshold), but skips the AS 147 series when the outcome is already certain
+// from x = chi2/2 alone.  For dof 1, 2, 4 (p = 1/2, 1, 2) the upper tail
+// Q(p,x) = 1 - P(p,x) has closed forms (erfc(sqrt x), e^-x, e^-x (1+x)), so the
+// host computes per gene set (its Bonferroni factor c) the x intervals where:
+//   x <= xlo       Q(x) >= thr (1 + 1e-5): the series (which only
+//                  underestimates P, by truncation at 1e-9 of the sum, and
+//                  overestimates it by at most ~6e-14 relative through
+//                  rounding) cannot bring 1 - P below thr: no edge;
+//   xhi <= x < xclo  Q(x) <= (thr - 5e-9)(1 - 1e-5): the truncation error
+//                  (measured <= 1.4e-9, bounded by 2e-9) cannot lift 1 - P
+//                  to thr, and x is below AS 147's underflow cut: an edge;
+//   x > xchi       AS 147's underflow cut p ln x - lgamma(p+1) - x < ln 1e-37
+//                  holds with margin: P = 0, p-value 1, no edge;
+// and runs the exact computation everywhere else (within the margins of a
+// threshold, and for every other dof).  thr = max(alphaEdge, alphaEdgePrior
+// if the pair has a prior edge) / c with Bonferroni, without it thr = alpha.
+#pragma once
+#include "fpmath.h"
+
+typedef struct {
+  double aE, aEP;           // alphaEdge, alphaEdgePrior
+  double lg15, lg2, lg3;    // lgamma(1.5), lgamma(2), lgamma(3) (AS 147's lgamma(p+1))
+  int p_adjust;             // Bonferroni on/off
+  int shortcut;             // 0: always the exact series
+  double xclo[3], xchi[3];  // underflow-cut interval per p = 1/2, 1, 2
+} eddy_edge_params;
+
+typedef struct {
+  double xlo[2][3];         // [prior][p index]; < 0 disables
+  double xhi[2][3];         // > xclo disables
+} eddy_edge_bounds;
+
+// AS 147: kernel1.cu:10-74
+EDDY_HD double eddy_gammds(double x, double p, double lgp1) {
+  if (x != x) return 0.0;  // unreachable for integer tallies; keeps the series finite
+  if (x <= 0.0) return 0.0;
+  if (p <= 0.0) return 0.0;
+  double arg = p * eddy_log(x) - lgp1 - x;
+  if (arg < eddy_log(1.0e-37)) return 0.0;
+  double f = eddy_exp(arg);
+  if (f == 0.0) return 0.0;
+  double c = 1.0, value = 1.0, a = p;
+  for (;;) {
+    a = a + 1.0;
+    c = c * x / a;
+    value = value + c;
+    if (c <= 1.0e-9 * value) break;
+  }
+  return value * f;
+}
+
+// chi-square and dof: kernel1.cu:121-203 (row-major sum, Yates on all cells
+// when any expected count of a non-empty row/column is below 5)
+EDDY_HD double eddy_chi2(const int *t, int *dof) {
+  int ex[6] = {0, 0, 0, 0, 0, 0};
+  for (int r = 0; r < 3; r++)
+    for (int k = 0; k < 3; k++) {
+      ex[k] += t[r * 3 + k];      // column sums
+      ex[3 + r] += t[r * 3 + k];  // row sums
+    }
+  double divisor = (double)(ex[0] + ex[1] + ex[2]);
+  double expected[9];
+  for (int r = 0; r < 3; r++)
+    for (int k = 0; k < 3; k++) expected[r * 3 + k] = (double)ex[k] * (double)ex[r + 3] / divisor;
+  double yates = 0.0;
+  for (int r = 0; r < 3; r++)
+    for (int k = 0; k < 3; k++)
+      if (expected[r * 3 + k] < 5 && ex[k] && ex[r + 3]) yates = 0.5;
+  double chi = 0.0;
+  int dofm = 0, dofn = 0;
+  for (int ii = 0; ii < 3; ii++) {
+    dofm += ex[ii] == 0;
+    dofn += ex[ii + 3] == 0;
+    for (int jj = 0; jj < 3; jj++) {
+      if (ex[jj] * ex[ii + 3] != 0) {
+        double d = eddy_fabs((double)t[ii * 3 + jj] - expected[ii * 3 + jj]) - yates;
+        chi = chi + d * d / expected[ii * 3 + jj];
+      }
+    }
+  }
+  *dof = ((3 - dofm) - 1) * ((3 - dofn) - 1);
+  return chi;
+}
+
+// exact decision: kernel1.cu:420-431
+EDDY_HD int eddy_edge_exact(double chi, int dof, int prior, const eddy_edge_params *ep, double cpairs) {
+  double lgp1 = dof == 1 ? ep->lg15 : dof == 2 ? ep->lg2 : ep->lg3;
+  double pv = 1.0 - eddy_gammds(chi / 2.0, ((double)dof) / 2.0, lgp1);
+  if (ep->p_adjust) {
+    double adj = pv * cpairs;
+    pv = adj < 1.0 ? adj : 1.0;
+  }
+  return (pv < ep->aE || (prior == 1 && pv < ep->aEP)) ? 1 : 0;
+}
+
+EDDY_HD int eddy_edge_decision(const int *t, int prior, const eddy_edge_params *ep, const eddy_edge_bounds *eb,
+                               double cpairs) {
+  int dof;
+  double chi = eddy_chi2(t, &dof);
+  if (ep->shortcut && (dof == 1 || dof == 2 || dof == 4)) {
+    const int pi = dof == 1 ? 0 : dof == 2 ? 1 : 2;
+    const double x = chi / 2.0;
+    const int pr = prior == 1 ? 1 : 0;
+    if (x <= eb->xlo[pr][pi]) return 0;
+    if (x > ep->xchi[pi]) return 0;
+    if (x >= eb->xhi[pr][pi] && x < ep->xclo[pi]) return 1;
+  }
+  return eddy_edge_exact(chi, dof, prior, ep, cpairs);
+}

@@ -4,9 +4,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -62,7 +65,7 @@ struct eddy_ctx {
   // device buffers, grown on demand
   DevBuf b_sets, b_xs, b_prior, b_items, b_labels, b_tiles, b_info, b_tab, b_lfull, b_lcnt, b_sflags, b_ev,
       b_evcount, b_hnet, b_rep, b_clsg, b_evorder, b_evfam, b_famkey, b_famscore, b_lval, b_uniq, b_js,
-      b_status, b_tests, b_genes, b_rowoff, b_nan0, b_pout, b_famcount;
+      b_status, b_tests, b_genes, b_rowoff, b_nan0, b_pout, b_famcount, b_counters;
   // last run
   int last_perms = 0;
   std::vector<int> last_G;
@@ -141,7 +144,7 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
                    &ctx->b_info, &ctx->b_tab, &ctx->b_lfull, &ctx->b_lcnt, &ctx->b_sflags, &ctx->b_ev,
                    &ctx->b_evcount, &ctx->b_hnet, &ctx->b_rep, &ctx->b_clsg, &ctx->b_evorder, &ctx->b_evfam,
                    &ctx->b_famkey, &ctx->b_famscore, &ctx->b_lval, &ctx->b_uniq, &ctx->b_js, &ctx->b_status,
-                   &ctx->b_tests, &ctx->b_genes, &ctx->b_rowoff, &ctx->b_nan0, &ctx->b_pout, &ctx->b_famcount};
+                   &ctx->b_tests, &ctx->b_genes, &ctx->b_rowoff, &ctx->b_nan0, &ctx->b_pout, &ctx->b_famcount, &ctx->b_counters};
   for (DevBuf *b : all)
     if (b->p) hipFree(b->p);
   if (ctx->dX) hipFree(ctx->dX);
@@ -218,6 +221,72 @@ extern "C" int eddy_upload_dataset(eddy_ctx *ctx, const uint8_t *x, int genes, i
 
 namespace {
 
+// ---- certainty intervals of the edge decision (see edgetest.h) ------------
+// upper tail 1 - P(p, x) for p = 1/2, 1, 2 (closed forms)
+long double q_tail(int pi, long double x) {
+  if (pi == 0) return erfcl(sqrtl(x));
+  if (pi == 1) return expl(-x);
+  return expl(-x) * (1.0L + x);
+}
+
+// boundary x where the decreasing tail crosses target: returns (lo, hi)
+// bracketing it to long-double precision
+void tail_root(int pi, long double target, long double &lo, long double &hi) {
+  lo = 0.0L;
+  hi = 4000.0L;
+  for (int i = 0; i < 400; i++) {
+    long double mid = 0.5L * (lo + hi);
+    if (q_tail(pi, mid) > target) lo = mid; else hi = mid;
+  }
+}
+
+const double kPs[3] = {0.5, 1.0, 2.0};
+
+void edge_params(eddy_ctx *ctx, const eddy_params *p, eddy_edge_params &ep) {
+  ep.aE = p->alpha_edge;
+  ep.aEP = p->alpha_edge_prior;
+  ep.lg15 = ctx->lg15;
+  ep.lg2 = ctx->lg2;
+  ep.lg3 = ctx->lg3;
+  ep.p_adjust = p->p_adjust;
+  // shortcuts only for thresholds strictly inside (0, 1)
+  ep.shortcut = (p->alpha_edge > 1e-300 && p->alpha_edge < 1.0 && p->alpha_edge_prior < 1.0) ? 1 : 0;
+  const long double L = (long double)eddy_log(1.0e-37);
+  const double lg[3] = {ctx->lg15, ctx->lg2, ctx->lg3};
+  for (int pi = 0; pi < 3; pi++) {
+    // AS 147 underflow cut: p ln x - lgamma(p+1) - x = ln 1e-37, decreasing for x > p
+    long double lo = kPs[pi] + 1.0L, hi = 4000.0L;
+    for (int i = 0; i < 400; i++) {
+      long double mid = 0.5L * (lo + hi);
+      long double g = (long double)kPs[pi] * logl(mid) - (long double)lg[pi] - mid - L;
+      if (g > 0) lo = mid; else hi = mid;
+    }
+    ep.xclo[pi] = (double)(lo - 1e-6L);
+    ep.xchi[pi] = (double)(hi + 1e-6L);
+  }
+}
+
+void edge_bounds(const eddy_params *p, int c, eddy_edge_bounds &eb) {
+  const double m = 1e-5;
+  for (int pr = 0; pr < 2; pr++) {
+    double alpha = pr ? std::max(p->alpha_edge, p->alpha_edge_prior) : p->alpha_edge;
+    double thr = p->p_adjust ? alpha / (double)std::max(c, 1) : alpha;
+    for (int pi = 0; pi < 3; pi++) {
+      long double lo, hi;
+      eb.xlo[pr][pi] = -1.0;
+      eb.xhi[pr][pi] = 1e300;
+      if (thr > 1e-12 && thr * (1 + m) < 1.0) {
+        tail_root(pi, (long double)thr * (1.0L + m), lo, hi);
+        eb.xlo[pr][pi] = (double)(lo * (1.0L - 1e-9L));
+      }
+      if (thr - 5e-9 > 1e-12) {
+        tail_root(pi, ((long double)thr - 5e-9L) * (1.0L - m), lo, hi);
+        eb.xhi[pr][pi] = (double)(hi * (1.0L + 1e-9L));
+      }
+    }
+  }
+}
+
 struct SetPrep {
   std::vector<SetMeta> meta;
   std::vector<uint32_t> prior;
@@ -247,7 +316,9 @@ int run_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, const 
   int ev_cap = std::max(1024, 2 * S);
   int maxG = 0;
   for (const ItemMeta &im : ch.items) maxG = std::max(maxG, sets[im.set].G);
-  for (int attempt = 0; attempt < 4; attempt++) {
+  int ht = 1024;
+  while (ht < 2 * std::max(S, 2 * maxG + 128)) ht *= 2;
+  for (int attempt = 0; attempt < 6; attempt++) {
     int fam_cap = 2 * maxG + ev_cap;
     // offsets
     long long evoff = 0, foff = 0;
@@ -300,11 +371,11 @@ int run_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, const 
     a.plain = plain_tests ? 1 : 0;
     a.ev_cap = ev_cap;
     a.fam_cap = fam_cap;
-    a.aE = p->alpha_edge;
-    a.aEP = p->alpha_edge_prior;
-    a.lg15 = ctx->lg15;
-    a.lg2 = ctx->lg2;
-    a.lg3 = ctx->lg3;
+    a.ht_size = ht;
+    a.plan_extra = nullptr;
+    a.counters = (unsigned long long *)ctx->b_counters.p;
+    edge_params(ctx, p, a.ep);
+    if (getenv("EDDY_NO_SHORTCUT")) a.ep.shortcut = 0;
     a.lgtab = ctx->dLG;
     a.lgstride = ctx->lgstride;
     a.hseed = 0x243F6A8885A308D3ULL + (uint64_t)attempt * 0x9E3779B97F4A7C15ULL;
@@ -345,7 +416,8 @@ int run_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, const 
     hipEventRecord(ev[2], st);
     k_rows<<<nitems, 256, 0, st>>>(a);
     hipEventRecord(ev[3], st);
-    k_score<<<nitems, 256, 0, st>>>(a);
+    const size_t score_lds = (size_t)ht * 12 + (size_t)(4 * S + 1 + 2 * maxG) * 4;
+    k_score<<<nitems, 256, score_lds, st>>>(a);
     hipEventRecord(ev[4], st);
     HIPCHK(ctx, hipGetLastError());
     std::vector<int> status(nitems);
@@ -370,7 +442,8 @@ int run_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, const 
       return EDDY_ERR_STATE;
     }
     if (bad & (ST_EV_OVERFLOW | ST_FAM_OVERFLOW | ST_COLLISION)) {
-      if (bad & (ST_EV_OVERFLOW | ST_FAM_OVERFLOW)) ev_cap *= 4;
+      if (bad & ST_EV_OVERFLOW) ev_cap *= 4;
+      if (bad & ST_FAM_OVERFLOW) ht *= 2;
       continue;  // retry the chunk (larger capacity / new hash seed)
     }
     for (int i = 0; i < nitems; i++) {
@@ -459,7 +532,8 @@ std::vector<uint8_t> perm_labels(const std::vector<int> &order, int n1) {
   return cls;
 }
 
-int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, std::vector<SetMeta> &meta) {
+int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *p,
+                 std::vector<SetMeta> &meta) {
   const int N = ctx->N, NP = (N + 63) & ~63;
   meta.resize(nsets);
   std::vector<int> rows;
@@ -478,6 +552,7 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, std::vecto
     m.c = d.n_genes * (d.n_genes - 1) / 2;
     m.xoff = xoff;
     m.poff = (long long)prior.size();
+    edge_bounds(p, m.c, m.eb);
     for (int i = 0; i < d.n_genes; i++) {
       if (d.genes[i] < 0 || d.genes[i] >= ctx->genes) {
         ctx->err = "gene index out of range";
@@ -535,9 +610,11 @@ extern "C" int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
   auto t0 = std::chrono::steady_clock::now();
   const int N = ctx->N, perms = p->perms, S = N + 2;
   std::vector<SetMeta> meta;
-  CHK(prepare_sets(ctx, sets, nsets, meta));
+  CHK(prepare_sets(ctx, sets, nsets, p, meta));
   GROW(b_js, sizeof(double) * (size_t)std::max(1, nsets) * perms);
   HIPCHK(ctx, hipMemsetAsync(ctx->b_js.p, 0, sizeof(double) * (size_t)std::max(1, nsets) * perms, ctx->stream));
+  GROW(b_counters, sizeof(unsigned long long) * 8);
+  HIPCHK(ctx, hipMemsetAsync(ctx->b_counters.p, 0, sizeof(unsigned long long) * 8, ctx->stream));
   ctx->last_perms = perms;
   ctx->last_G.assign(nsets, 0);
   ctx->last_js.assign(nsets, std::vector<double>());
@@ -579,24 +656,49 @@ extern "C" int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
       HIPCHK(ctx, hipMemcpy(&v, (double *)ctx->b_js.p + (size_t)s * perms, sizeof(double), hipMemcpyDeviceToHost));
       if (std::isnan(v)) nan0[s] = 1;
     }
-    // phase 2: replay the stream in the reference's order, batch the rest
-    for (int s = 0; s < nsets; s++) {
+    // phase 2: a producer thread replays the stream in the reference's order
+    // (sets in order; a NaN-at-permutation-0 set consumes one permutation's
+    // draws) while this thread batches the items and runs the GPU chunks
+    std::vector<std::vector<uint32_t>> setlab(nsets);
+    std::atomic<int> ready{0};
+    double stream_ms = 0.0;
+    std::thread producer([&]() {
       auto h0 = std::chrono::steady_clock::now();
-      ctx->stream_gen.permutation(N, order);  // permutation 0's draws
-      if (nan0[s]) {
-        ctx->timing.host_stream_ms +=
-            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
-        continue;
+      std::vector<int> ord;
+      std::vector<uint8_t> cls(N);
+      const int words = (N + 31) / 32;
+      for (int s = 0; s < nsets; s++) {
+        ctx->stream_gen.permutation(N, ord);  // permutation 0's draws
+        if (!nan0[s]) {
+          std::vector<uint32_t> &L = setlab[s];
+          L.assign((size_t)(perms - 1) * words, 0u);
+          for (int pm = 1; pm < perms; pm++) {
+            ctx->stream_gen.permutation(N, ord);
+            uint32_t *w = &L[(size_t)(pm - 1) * words];
+            for (int i = ctx->n1; i < N; i++) w[ord[i] >> 5] |= 1u << (ord[i] & 31);  // class 2 (main.cu:561-570)
+          }
+        }
+        ready.store(s + 1, std::memory_order_release);
       }
+      stream_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    });
+    int rc2 = EDDY_OK;
+    const int words = (N + 31) / 32;
+    std::vector<uint32_t> one(words);
+    for (int s = 0; s < nsets && rc2 == EDDY_OK; s++) {
+      while (ready.load(std::memory_order_acquire) <= s) std::this_thread::yield();
+      if (nan0[s]) continue;
       ran[s] = perms;
       for (int pm = 1; pm < perms; pm++) {
-        ctx->stream_gen.permutation(N, order);
-        add_item(ch, meta, s, pm, label_bits(perm_labels(order, ctx->n1)), N);
+        std::copy(setlab[s].begin() + (size_t)(pm - 1) * words, setlab[s].begin() + (size_t)pm * words, one.begin());
+        add_item(ch, meta, s, pm, one, N);
       }
-      ctx->timing.host_stream_ms +=
-          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
-      if (ch.pairs > pair_budget) CHK(flush(ch, false));
+      std::vector<uint32_t>().swap(setlab[s]);
+      if (ch.pairs > pair_budget) rc2 = flush(ch, false);
     }
+    producer.join();
+    ctx->timing.host_stream_ms += stream_ms;
+    CHK(rc2);
     CHK(flush(ch, false));
   } else {
     // plain: one set and one permutation at a time, like main.cu:538-1460
@@ -668,6 +770,9 @@ extern "C" int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
     ctx->last_js[s].assign(alljs.begin() + (size_t)s * perms, alljs.begin() + (size_t)(s + 1) * perms);
     if (nan0[s]) ctx->last_js[s][0] = -999.0;
   }
+  unsigned long long cnt[8];
+  HIPCHK(ctx, hipMemcpy(cnt, ctx->b_counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
+  for (int k = 0; k < 4; k++) ctx->timing.sched[k] = (long long)cnt[k];
   ctx->timing.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return EDDY_OK;
 }
@@ -787,9 +892,11 @@ extern "C" int eddy_run_perm_detail(eddy_ctx *ctx, const eddy_set_desc *set, con
     return EDDY_ERR_ARG;
   }
   std::vector<SetMeta> meta;
-  CHK(prepare_sets(ctx, set, 1, meta));
+  CHK(prepare_sets(ctx, set, 1, p, meta));
   GROW(b_js, sizeof(double) * p->perms);
   HIPCHK(ctx, hipMemsetAsync(ctx->b_js.p, 0, sizeof(double) * p->perms, ctx->stream));
+  GROW(b_counters, sizeof(unsigned long long) * 8);
+  HIPCHK(ctx, hipMemsetAsync(ctx->b_counters.p, 0, sizeof(unsigned long long) * 8, ctx->stream));
   ctx->last_p0.assign(1, Perm0());
   ctx->last_G.assign(1, meta[0].G);
   ctx->last_perms = p->perms;
@@ -878,4 +985,19 @@ extern "C" int eddy_run_perm_detail(eddy_ctx *ctx, const eddy_set_desc *set, con
       }
   }
   return EDDY_OK;
+}
+
+// Host build of the engine's edge decision (edgetest.h), for the tests that
+// compare it with the oracle on many tables.  c: Bonferroni factor.
+extern "C" int eddy_edge_decision_host(const int *tally9, int c, int prior, const eddy_params *p) {
+  if (!tally9 || !p) return -1;
+  eddy_edge_params ep;
+  eddy_ctx tmp;
+  tmp.lg15 = std::lgamma(1.5);
+  tmp.lg2 = std::lgamma(2.0);
+  tmp.lg3 = std::lgamma(3.0);
+  edge_params(&tmp, p, ep);
+  eddy_edge_bounds eb;
+  edge_bounds(p, c, eb);
+  return eddy_edge_decision(tally9, prior, &ep, &eb, (double)c);
 }

@@ -2,6 +2,8 @@
 #pragma once
 #include <stdint.h>
 
+#include "edgetest.h"
+
 // Engine limits (the reference's are MAX_GENESET_SIZE 200 and a 1024-thread
 // block per network, definitions.cuh:19, main.cu:931).
 #define EDDY_MAX_G 1024     // genes per set (10-bit family keys)
@@ -24,6 +26,7 @@ struct SetMeta {
   int c;            // G(G-1)/2
   long long xoff;   // byte offset of the set's rows [G x N] in the gathered data
   long long poff;   // u32-word offset of the set's prior bits (one bit per pair)
+  eddy_edge_bounds eb;  // x intervals where the edge decision is certain (c-dependent)
 };
 
 struct ItemMeta {   // one (gene set, permutation)
@@ -46,9 +49,8 @@ struct KArgs {
   // dataset-level
   int N, n1, n2, S;
   int mp, p_adjust, plain;
-  int ev_cap, fam_cap;
-  double aE, aEP;
-  double lg15, lg2, lg3;   // lgamma(1.5), lgamma(2), lgamma(3) (AS 147's lgamma(p+1))
+  int ev_cap, fam_cap, ht_size;
+  eddy_edge_params ep;     // thresholds, lgamma(p+1) constants, underflow-cut interval
   const double *lgtab;     // [82][lgstride]: lgamma(n + 4/d)
   int lgstride;
   uint64_t hseed;
@@ -83,6 +85,8 @@ struct KArgs {
   double *js;              // [set*perms + perm]
   int perms;
   int *status;             // [item] error bits
+  const uint8_t *plan_extra;  // [goff*2 + g*G + t] extra full edges in the plan (learned), or null
+  unsigned long long *counters;  // schedule statistics (optional)
   long long *tests;        // [item] evaluated tests
 };
 

@@ -24,81 +24,22 @@
 #include <stdint.h>
 
 #include "engine_types.h"
+#include "edgetest.h"
 #include "fpmath.h"
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-// --------------------------------------------------------------------------
-// chi-square test of one 3x3 tally: kernel1.cu:121-203 and 420-431, the same
-// operation order as the reference loops (row-major chi-square sum)
-// --------------------------------------------------------------------------
-__device__ __forceinline__ double d_gammds(double x, double p, double lgp1) {
-  if (x != x) return 0.0;  // unreachable for integer tallies; keeps the series finite
-  if (x <= 0.0) return 0.0;
-  if (p <= 0.0) return 0.0;
-  double arg = p * eddy_log(x) - lgp1 - x;
-  if (arg < eddy_log(1.0e-37)) return 0.0;
-  double f = eddy_exp(arg);
-  if (f == 0.0) return 0.0;
-  double c = 1.0, value = 1.0, a = p;
-  for (;;) {
-    a = a + 1.0;
-    c = c * x / a;
-    value = value + c;
-    if (c <= 1.0e-9 * value) break;
-  }
-  return value * f;
-}
-
-__device__ __forceinline__ int d_edge_test(const int *t, int prior, const KArgs &a, double cpairs) {
-  int ex[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-  for (int r = 0; r < 3; r++)
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      ex[k] += t[r * 3 + k];      // column sums
-      ex[3 + r] += t[r * 3 + k];  // row sums
-    }
-  double divisor = (double)(ex[0] + ex[1] + ex[2]);
-  double expected[9];
-#pragma unroll
-  for (int r = 0; r < 3; r++)
-#pragma unroll
-    for (int k = 0; k < 3; k++) expected[r * 3 + k] = (double)ex[k] * (double)ex[r + 3] / divisor;
-  double yates = 0.0;
-#pragma unroll
-  for (int r = 0; r < 3; r++)
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-      if (expected[r * 3 + k] < 5 && ex[k] && ex[r + 3]) yates = 0.5;
-  double chi = 0.0;
-  int dofm = 0, dofn = 0;
-#pragma unroll
-  for (int ii = 0; ii < 3; ii++) {
-    dofm += ex[ii] == 0;
-    dofn += ex[ii + 3] == 0;
-#pragma unroll
-    for (int jj = 0; jj < 3; jj++) {
-      if (ex[jj] * ex[ii + 3] != 0) {
-        double d = eddy_fabs((double)t[ii * 3 + jj] - expected[ii * 3 + jj]) - yates;
-        chi = chi + d * d / expected[ii * 3 + jj];
-      }
-    }
-  }
-  int dof = ((3 - dofm) - 1) * ((3 - dofn) - 1);
-  double lgp1 = dof == 1 ? a.lg15 : dof == 2 ? a.lg2 : a.lg3;
-  double pv = 1.0 - d_gammds(chi / 2.0, ((double)dof) / 2.0, lgp1);
-  if (a.p_adjust) {
-    double adj = pv * cpairs;
-    pv = adj < 1.0 ? adj : 1.0;
-  }
-  return (pv < a.aE || (prior == 1 && pv < a.aEP)) ? 1 : 0;
+// chi-square edge decision of one 3x3 tally: edgetest.h (kernel1.cu:10-203,
+// 420-431 with an exact shortcut of the AS 147 series)
+__device__ __forceinline__ int d_edge_test(const int *t, int prior, const KArgs &a, const SetMeta &sm,
+                                           double cpairs) {
+  return eddy_edge_decision(t, prior, &a.ep, &sm.eb, cpairs);
 }
 
 // Leave-one-out decisions of one (pair, class): bit cell set when the tally
 // minus one count in `cell` is an edge; empty cells copy the full decision.
 __device__ __forceinline__ uint32_t d_loo_mask(const int *t, int full, int prior, const KArgs &a,
-                                               double cpairs, int *ntests) {
+                                               const SetMeta &sm, double cpairs, int *ntests) {
   uint32_t m = 0;
   int tt[9];
 #pragma unroll
@@ -107,7 +48,7 @@ __device__ __forceinline__ uint32_t d_loo_mask(const int *t, int full, int prior
     int bit = full;
     if (t[cell] > 0) {
       tt[cell] = t[cell] - 1;
-      bit = d_edge_test(tt, prior, a, cpairs);
+      bit = d_edge_test(tt, prior, a, sm, cpairs);
       tt[cell] = t[cell];
       (*ntests)++;
     }
@@ -251,13 +192,13 @@ __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
       }
     long long p = pair_index(gi, gj);
     int pr = prior_bit(a, sm, p);
-    int f1 = d_edge_test(t1, pr, a, cpairs);
-    int f2 = d_edge_test(t2, pr, a, cpairs);
+    int f1 = d_edge_test(t1, pr, a, sm, cpairs);
+    int f2 = d_edge_test(t2, pr, a, sm, cpairs);
     ntests += 2;
     uint32_t info = (uint32_t)f1 | ((uint32_t)f2 << 1);
     if (a.plain) {
-      info |= d_loo_mask(t1, f1, pr, a, cpairs, &ntests) << INFO_LOO_SHIFT(0);
-      info |= d_loo_mask(t2, f2, pr, a, cpairs, &ntests) << INFO_LOO_SHIFT(1);
+      info |= d_loo_mask(t1, f1, pr, a, sm, cpairs, &ntests) << INFO_LOO_SHIFT(0);
+      info |= d_loo_mask(t2, f2, pr, a, sm, cpairs, &ntests) << INFO_LOO_SHIFT(1);
       info |= INFO_EVAL(0) | INFO_EVAL(1);
     } else {
       uint16_t *tb = a.tab + (im.infooff + p) * 18;
@@ -275,43 +216,136 @@ __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
 }
 
 // --------------------------------------------------------------------------
-// K2 (cutoff mode): leave-one-out tests of row t, class g, in ascending
-// column order until the row holds mp "solid" edges (edges in the full test
-// and in every leave-one-out test).  Every network of class g then takes its
-// first mp parents (kernel3.cu:70-109) from columns at or before that point,
-// so later columns cannot change any capped network.  One workgroup per item.
+// K2 (cutoff schedule): leave-one-out tests of each row, class g, up to the
+// row's cutoff C = the column of its mp-th "solid" edge (an edge in the full
+// test and in every leave-one-out test), or the whole row.  Every network of
+// class g takes its first mp parents (kernel3.cu:70-109) from columns <= C,
+// so later columns cannot change any capped network.
+//
+// C is not known before the leave-one-out tests, so the kernel works in
+// three phases per item (one workgroup):
+//   A  plan: P = column of the (mp + extra)-th full-test edge of the row
+//      (every column when the row has fewer); extra = 0 in the fixed plan,
+//      or the learned plan's count of full edges predicted not solid
+//   B  all (pair, class) entries with column <= P, flattened over the item's
+//      rows and tested in parallel (packed: no row-length divergence)
+//   C  exact check per row: count solid edges up to P; a row short of mp
+//      continues column by column (the fallback) until mp solid edges or the
+//      row's end.  Correctness never depends on the plan.
 // --------------------------------------------------------------------------
+#define K2_MAXROWS (2 * EDDY_MAX_G)
+
+__device__ __forceinline__ uint32_t loo_of_pair(const KArgs &a, const ItemMeta &im, const SetMeta &sm, long long p,
+                                                int g, int full, double cpairs, int *ntests) {
+  const uint16_t *tb = a.tab + (im.infooff + p) * 18 + 9 * g;
+  int tt[9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) tt[i] = tb[i];
+  return d_loo_mask(tt, full, prior_bit(a, sm, p), a, sm, cpairs, ntests);
+}
+
 __global__ __launch_bounds__(256) void k_loo(KArgs a) {
   const int item = blockIdx.x;
   const ItemMeta im = a.items[item];
   const SetMeta sm = a.sets[im.set];
-  const int G = sm.G;
+  const int G = sm.G, mp = a.mp;
   const double cpairs = (double)sm.c;
-  __shared__ int tcount;
-  if (threadIdx.x == 0) tcount = 0;
+  const int tid = threadIdx.x;
+  __shared__ int plan[K2_MAXROWS];       // planned last column per (row, class) entry
+  __shared__ int off[K2_MAXROWS + 1];    // exclusive prefix of entry counts
+  __shared__ int wsum[256];
+  __shared__ int tcount, nfb, nfbcols;
+  if (tid == 0) tcount = nfb = nfbcols = 0;
+  const int R = 2 * G;  // entries r = 2t + g
+  // ---- A: plan ----------------------------------------------------------
+  for (int r = tid; r < R; r += 256) {
+    const int t = r >> 1, g = r & 1;
+    const int want = mp + (a.plan_extra ? a.plan_extra[(im.goff * 2 + g * G + t)] : 0);
+    int hits = 0, P = t - 1;
+    for (int col = 0; col < t; col++) {
+      uint32_t w = a.info[im.infooff + pair_index(t, col)];
+      if ((w >> g) & 1) {
+        if (++hits >= want) {
+          P = col;
+          break;
+        }
+      }
+    }
+    plan[r] = P;
+  }
   __syncthreads();
+  // exclusive scan of the entry counts (P + 1 columns per entry)
+  {
+    const int per = (R + 255) / 256;
+    int b = tid * per, e = min(R, b + per), acc = 0;
+    for (int r = b; r < e; r++) acc += plan[r] + 1;
+    wsum[tid] = acc;
+    __syncthreads();
+    if (tid == 0) {
+      int run = 0;
+      for (int i = 0; i < 256; i++) {
+        int v = wsum[i];
+        wsum[i] = run;
+        run += v;
+      }
+      off[R] = run;
+    }
+    __syncthreads();
+    acc = wsum[tid];
+    for (int r = b; r < e; r++) {
+      off[r] = acc;
+      acc += plan[r] + 1;
+    }
+  }
+  __syncthreads();
+  // ---- B: packed leave-one-out tests of every planned (pair, class) ------
+  const int total = off[R];
   int ntests = 0;
-  for (int idx = threadIdx.x; idx < 2 * G; idx += blockDim.x) {
-    int t = G - 1 - (idx >> 1);  // long rows first
-    int g = idx & 1;
-    int solid = 0;
-    for (int col = 0; col < t && solid < a.mp; col++) {
-      long long p = pair_index(t, col);
-      uint32_t info = a.info[im.infooff + p];
-      int full = (info >> g) & 1;
-      const uint16_t *tb = a.tab + (im.infooff + p) * 18 + 9 * g;
-      int tt[9];
-#pragma unroll
-      for (int i = 0; i < 9; i++) tt[i] = tb[i];
-      int pr = prior_bit(a, sm, p);
-      uint32_t m = d_loo_mask(tt, full, pr, a, cpairs, &ntests);
+  for (int e = tid; e < total; e += 256) {
+    int lo = 0, hi = R - 1;  // last r with off[r] <= e
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= e) lo = mid; else hi = mid - 1;
+    }
+    const int t = lo >> 1, g = lo & 1, col = e - off[lo];
+    const long long p = pair_index(t, col);
+    uint32_t w = a.info[im.infooff + p];
+    uint32_t m = loo_of_pair(a, im, sm, p, g, (w >> g) & 1, cpairs, &ntests);
+    atomicOr(&a.info[im.infooff + p], (m << INFO_LOO_SHIFT(g)) | INFO_EVAL(g));
+  }
+  __syncthreads();
+  // ---- C: exact cutoff check, fallback past the plan ---------------------
+  int fb = 0, fbc = 0;
+  for (int r = tid; r < R; r += 256) {
+    const int t = r >> 1, g = r & 1;
+    int solid = 0, col = 0;
+    for (; col <= plan[r] && col < t && solid < mp; col++) {
+      uint32_t w = a.info[im.infooff + pair_index(t, col)];
+      if (((w >> g) & 1) && ((w >> INFO_LOO_SHIFT(g)) & 0x1FFu) == 0x1FFu) solid++;
+    }
+    if (solid < mp && col < t) fb++;
+    for (; col < t && solid < mp; col++) {
+      const long long p = pair_index(t, col);
+      uint32_t w = a.info[im.infooff + p];
+      uint32_t m = loo_of_pair(a, im, sm, p, g, (w >> g) & 1, cpairs, &ntests);
       atomicOr(&a.info[im.infooff + p], (m << INFO_LOO_SHIFT(g)) | INFO_EVAL(g));
-      if (full && m == 0x1FFu) solid++;
+      if (((w >> g) & 1) && m == 0x1FFu) solid++;
+      fbc++;
     }
   }
   atomicAdd(&tcount, ntests);
+  atomicAdd(&nfb, fb);
+  atomicAdd(&nfbcols, fbc);
   __syncthreads();
-  if (threadIdx.x == 0) atomicAdd((unsigned long long *)&a.tests[item], (unsigned long long)tcount);
+  if (tid == 0) {
+    atomicAdd((unsigned long long *)&a.tests[item], (unsigned long long)tcount);
+    if (a.counters) {
+      atomicAdd((unsigned long long *)&a.counters[0], (unsigned long long)nfb);      // rows needing fallback
+      atomicAdd((unsigned long long *)&a.counters[1], (unsigned long long)nfbcols);  // columns tested in fallback
+      atomicAdd((unsigned long long *)&a.counters[2], (unsigned long long)total);    // planned entries
+      atomicAdd((unsigned long long *)&a.counters[3], (unsigned long long)R);        // rows
+    }
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -525,7 +559,6 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
 // --------------------------------------------------------------------------
 // K4: dedup, BDeu, likelihood, JS.  One workgroup per item.
 // --------------------------------------------------------------------------
-#define HT_SIZE 4096
 
 __device__ __forceinline__ int net_class(int s, int n1) { return s <= n1 ? 0 : 1; }
 
@@ -551,28 +584,57 @@ __device__ __forceinline__ uint64_t fam_key(int t, int n, const uint16_t *par) {
   return k + 1;  // 0 marks an empty slot
 }
 
-__device__ int ht_insert_min(unsigned long long *keys, int *vals, uint64_t key, int val) {
-  uint32_t h = (uint32_t)mix64(key) & (HT_SIZE - 1);
-  for (int probe = 0; probe < HT_SIZE; probe++) {
+__device__ int ht_insert_min(unsigned long long *keys, int *vals, uint64_t key, int val, int hs) {
+  uint32_t h = (uint32_t)mix64(key) & (hs - 1);
+  for (int probe = 0; probe < hs; probe++) {
     unsigned long long old = atomicCAS(&keys[h], 0ULL, (unsigned long long)key);
     if (old == 0ULL || old == key) {
       atomicMin(&vals[h], val);
       return (int)h;
     }
-    h = (h + 1) & (HT_SIZE - 1);
+    h = (h + 1) & (hs - 1);
   }
   return -1;
 }
 
-__device__ int ht_find(const unsigned long long *keys, uint64_t key) {
-  uint32_t h = (uint32_t)mix64(key) & (HT_SIZE - 1);
-  for (int probe = 0; probe < HT_SIZE; probe++) {
+__device__ int ht_find(const unsigned long long *keys, uint64_t key, int hs) {
+  uint32_t h = (uint32_t)mix64(key) & (hs - 1);
+  for (int probe = 0; probe < hs; probe++) {
     unsigned long long k = keys[h];
     if (k == key) return (int)h;
     if (k == 0ULL) return -1;
-    h = (h + 1) & (HT_SIZE - 1);
+    h = (h + 1) & (hs - 1);
   }
   return -1;
+}
+
+// block-wide exclusive scan of n ints in LDS (in place); returns the total
+__device__ int block_exscan(int *v, int n, int *wsum) {
+  const int tid = threadIdx.x;
+  const int per = (n + 255) / 256;
+  int b = tid * per, e = min(n, b + per), acc = 0;
+  for (int i = b; i < e; i++) acc += v[i];
+  wsum[tid] = acc;
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int i = 0; i < 256; i++) {
+      int x = wsum[i];
+      wsum[i] = run;
+      run += x;
+    }
+    wsum[256] = run;
+  }
+  __syncthreads();
+  acc = wsum[tid];
+  for (int i = b; i < e; i++) {
+    int x = v[i];
+    v[i] = acc;
+    acc += x;
+  }
+  int total = wsum[256];
+  __syncthreads();
+  return total;
 }
 
 __global__ __launch_bounds__(256) void k_score(KArgs a) {
@@ -583,66 +645,65 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   const int NP = (N + 63) & ~63;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint8_t *xs = a.xs + sm.xoff;
+  const int HT = a.ht_size;
 
-  __shared__ unsigned long long hkeys[HT_SIZE];
-  __shared__ int hvals[HT_SIZE];
-  __shared__ int rep[EDDY_MAX_N + 2];
-  __shared__ int uid[EDDY_MAX_N + 2];
-  __shared__ int evcnt[EDDY_MAX_N + 2];
-  __shared__ int evst[EDDY_MAX_N + 3];
+  extern __shared__ __align__(16) unsigned char dyn[];
+  unsigned long long *hkeys = (unsigned long long *)dyn;  // [HT]
+  int *hvals = (int *)(hkeys + HT);                       // [HT]
+  int *rep = hvals + HT;                                  // [S]
+  int *evst = rep + S;                                    // [S + 1]
+  int *work = evst + S + 1;                               // [S]: scratch counts
+  int *ulist = work + S;                                  // [S]: unique networks in order
+  int *famfull = ulist + S;                               // [2G]
+  __shared__ int wsum[257];
   __shared__ int U, F, flag;
   __shared__ int hist[EDDY_NWAVES][84];
 
   const int E = min(a.evcount[item], a.ev_cap);
+  int *es = a.evorder + im.evoff;
+  const Event *ev = a.ev + im.evoff;
 
   // ---- 1. first occurrence of each network hash -------------------------
-  for (int i = tid; i < HT_SIZE; i += 256) {
+  for (int i = tid; i < HT; i += 256) {
     hkeys[i] = 0ULL;
     hvals[i] = 0x7fffffff;
   }
-  for (int s = tid; s < S; s += 256) evcnt[s] = 0;
+  for (int s = tid; s < S; s += 256) work[s] = 0;
   if (tid == 0) flag = 0;
   __syncthreads();
   for (int s = tid; s < S; s += 256) {
     uint64_t h = a.hnet[im.soff + s];
     if (h == 0ULL) h = 1ULL;
-    if (ht_insert_min(hkeys, hvals, h, s) < 0) atomicOr(&flag, 1);
+    if (ht_insert_min(hkeys, hvals, h, s, HT) < 0) atomicOr(&flag, 1);
   }
-  for (int e = tid; e < E; e += 256) atomicAdd(&evcnt[a.ev[im.evoff + e].s], 1);
+  for (int e = tid; e < E; e += 256) atomicAdd(&work[ev[e].s], 1);
   __syncthreads();
   for (int s = tid; s < S; s += 256) {
     uint64_t h = a.hnet[im.soff + s];
     if (h == 0ULL) h = 1ULL;
-    int slot = ht_find(hkeys, h);
+    int slot = ht_find(hkeys, h, HT);
     rep[s] = slot >= 0 ? hvals[slot] : s;
     if (slot < 0) atomicOr(&flag, 1);
-  }
-  // events grouped by network (exclusive scan of counts, serial: S <= 2048)
-  if (tid == 0) {
-    int acc = 0;
-    for (int s = 0; s < S; s++) {
-      evst[s] = acc;
-      acc += evcnt[s];
-    }
-    evst[S] = acc;
+    evst[s] = work[s];
   }
   __syncthreads();
-  for (int s = tid; s < S; s += 256) evcnt[s] = 0;
+  // events grouped by network
+  block_exscan(evst, S, wsum);
+  if (tid == 0) evst[S] = E;
+  for (int s = tid; s < S; s += 256) work[s] = 0;
   __syncthreads();
-  int *es = a.evorder + im.evoff;
   for (int e = tid; e < E; e += 256) {
-    int s = a.ev[im.evoff + e].s;
-    es[evst[s] + atomicAdd(&evcnt[s], 1)] = e;
+    int s = ev[e].s;
+    es[evst[s] + atomicAdd(&work[s], 1)] = e;
   }
   __syncthreads();
-  // sort each network's events by row
-  for (int s = tid; s < S; s += 256) {
+  for (int s = tid; s < S; s += 256) {  // sort each network's events by row
     int b = evst[s], n = evst[s + 1] - b;
     for (int i = 1; i < n; i++) {
       int v = es[b + i];
-      int tv = a.ev[im.evoff + v].t;
+      int tv = ev[v].t;
       int j = i - 1;
-      while (j >= 0 && a.ev[im.evoff + es[b + j]].t > tv) {
+      while (j >= 0 && ev[es[b + j]].t > tv) {
         es[b + j + 1] = es[b + j];
         j--;
       }
@@ -662,10 +723,10 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
     for (int t = 0; t < G && eq; t++) {
       if (gs == gr) {
         // rows without events are the same full row; compare event rows only
-        while (ps < ns && a.ev[im.evoff + es[evst[s] + ps]].t < t) ps++;
-        while (pr < nr && a.ev[im.evoff + es[evst[r] + pr]].t < t) pr++;
-        bool hs = ps < ns && a.ev[im.evoff + es[evst[s] + ps]].t == t;
-        bool hr = pr < nr && a.ev[im.evoff + es[evst[r] + pr]].t == t;
+        while (ps < ns && ev[es[evst[s] + ps]].t < t) ps++;
+        while (pr < nr && ev[es[evst[r] + pr]].t < t) pr++;
+        bool hs = ps < ns && ev[es[evst[s] + ps]].t == t;
+        bool hr = pr < nr && ev[es[evst[r] + pr]].t == t;
         if (!hs && !hr) continue;
       }
       const uint16_t *ls, *lr;
@@ -684,24 +745,28 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
     }
     return;
   }
-  // unique ids in network order
-  if (tid == 0) {
-    int u = 0;
-    for (int s = 0; s < S; s++) {
-      uid[s] = (rep[s] == s) ? u++ : -1;
+  // unique networks in network order: ulist[u] = s
+  for (int s = tid; s < S; s += 256) work[s] = rep[s] == s ? 1 : 0;
+  __syncthreads();
+  {
+    int nu = block_exscan(work, S, wsum);
+    for (int s = tid; s < S; s += 256)
+      if (rep[s] == s) ulist[work[s]] = s;
+    if (tid == 0) {
+      U = nu;
+      a.uniq_u[item] = nu;
     }
-    U = u;
-    a.uniq_u[item] = u;
   }
+  __syncthreads();
   for (int s = tid; s < S; s += 256) a.rep[im.soff + s] = rep[s];
+  const int UU = U;
   // ---- 3. families of the unique networks --------------------------------
-  for (int i = tid; i < HT_SIZE; i += 256) {
+  for (int i = tid; i < HT; i += 256) {
     hkeys[i] = 0ULL;
     hvals[i] = 0x7fffffff;
   }
   if (tid == 0) F = 0;
   __syncthreads();
-  // full rows of both classes, then event rows of unique networks
   for (int i = tid; i < 2 * G + E; i += 256) {
     uint64_t key;
     if (i < 2 * G) {
@@ -709,15 +774,14 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
       size_t lb = (size_t)(im.goff * 2 + g * G + t);
       key = fam_key(t, a.lcnt[lb], a.lfull + lb * EDDY_MAX_MP);
     } else {
-      const Event *e = &a.ev[im.evoff + (i - 2 * G)];
+      const Event *e = &ev[i - 2 * G];
       if (rep[e->s] != e->s) continue;
       key = fam_key(e->t, e->n, e->par);
     }
-    if (ht_insert_min(hkeys, hvals, key, 0) < 0) atomicOr(&flag, 4);
+    if (ht_insert_min(hkeys, hvals, key, 0, HT) < 0) atomicOr(&flag, 4);
   }
   __syncthreads();
-  // number the distinct families
-  for (int i = tid; i < HT_SIZE; i += 256) {
+  for (int i = tid; i < HT; i += 256) {
     if (hkeys[i] != 0ULL) {
       int f = atomicAdd(&F, 1);
       hvals[i] = f;
@@ -726,41 +790,42 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   }
   __syncthreads();
   if (tid == 0) a.famcount[item] = min(F, a.fam_cap);
-  if (flag || F > a.fam_cap) {
+  if (flag || F > a.fam_cap || 2 * F > HT) {
     if (tid == 0) atomicOr(&a.status[item], ST_FAM_OVERFLOW);
     return;
   }
+  for (int i = tid; i < 2 * G; i += 256) {
+    int g = i / G, t = i - g * G;
+    size_t lb = (size_t)(im.goff * 2 + g * G + t);
+    famfull[i] = hvals[ht_find(hkeys, fam_key(t, a.lcnt[lb], a.lfull + lb * EDDY_MAX_MP), HT)];
+  }
   for (int e = tid; e < E; e += 256) {
-    const Event *ev = &a.ev[im.evoff + e];
-    int slot = rep[ev->s] == ev->s ? ht_find(hkeys, fam_key(ev->t, ev->n, ev->par)) : -1;
+    const Event *x = &ev[e];
+    int slot = rep[x->s] == x->s ? ht_find(hkeys, fam_key(x->t, x->n, x->par), HT) : -1;
     a.evfam[im.evoff + e] = slot >= 0 ? hvals[slot] : -1;
   }
-  // full-row family ids reuse the hash keys region afterwards: store them in evcnt? keep lookups
   __syncthreads();
+  // the hash table is done; its space now holds the family scores [f*2 + g]
+  double *fscore = (double *)dyn;
+  const int NF = F;
   // ---- 4. BDeu per (family, scoring class): kernel5.cu:4-236 -------------
-  // waves take (family, class) pairs in rounds; every wave joins every
-  // round's barriers so the per-wave histograms can be reused safely
-  for (int fg0 = 0; fg0 < 2 * F; fg0 += EDDY_NWAVES) {
-    const int fg = fg0 + wave;
-    const bool act = fg < 2 * F;
+  // each wave scores its own (family, class) pairs with a private histogram
+  for (int fg = wave; fg < 2 * NF; fg += EDDY_NWAVES) {
     const int f = fg >> 1, g = fg & 1;
-    int t = 0, np = 0, par[3] = {0, 0, 0}, r = 0, q = 1, radix[3] = {1, 1, 1};
+    const uint64_t key = a.famkey[im.foff + f] - 1;
+    const int t = (int)(key & 1023), np = (int)((key >> 10) & 3);
+    int par[3] = {0, 0, 0};
+    for (int i = 0; i < np; i++) par[i] = (int)((key >> (12 + 10 * i)) & 1023);
     const uint8_t *sf = a.sflags + im.goff * 2 + g * G;
-    if (act) {
-      const uint64_t key = a.famkey[im.foff + f] - 1;
-      t = (int)(key & 1023);
-      np = (int)((key >> 10) & 3);
-      for (int i = 0; i < np; i++) par[i] = (int)((key >> (12 + 10 * i)) & 1023);
-      // ranks of observed states in class g's full data
-      r = __popc(sf[t]);
-      for (int i = 0; i < np; i++) {
-        radix[i] = q;
-        q *= __popc(sf[par[i]]);
-      }
+    const int r = __popc(sf[t]);
+    int q = 1, radix[3] = {1, 1, 1};
+    for (int i = 0; i < np; i++) {
+      radix[i] = q;
+      q *= __popc(sf[par[i]]);
     }
     for (int i = lane; i < 84; i += 64) hist[wave][i] = 0;
-    __syncthreads();
-    if (act && q > 0 && r > 0) {
+    wave_sync();
+    if (q > 0 && r > 0) {
       const uint8_t *xt = xs + (size_t)t * NP;
       const uint16_t *cl = a.clsg + im.soff + (g ? a.n1 : 0);
       const int ng = g ? a.n2 : a.n1;
@@ -782,8 +847,8 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
         if (ok) atomicAdd(&hist[wave][j * r + kk], 1);
       }
     }
-    __syncthreads();
-    if (act && lane == 0) {
+    wave_sync();
+    if (lane == 0) {
       double sum2 = 0.0;
       if (q > 0 && r > 0) {
         const double *lq = a.lgtab + (size_t)q * a.lgstride;
@@ -796,37 +861,31 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
           sum2 = sum2 + ((lq[0] - lq[nij]) + sum1);
         }
       }
+      fscore[f * 2 + g] = sum2;
       a.famscore[(im.foff + f) * 2 + g] = sum2;
     }
-    __syncthreads();
+    wave_sync();
   }
+  __syncthreads();
   // ---- 5. network scores, in node order (main.cu:1335-1342) -------------
   double *dist = a.lval + im.soff * 2;  // [g*S + u] (overwritten by the likelihoods below)
-  for (int ug = tid; ug < 2 * S; ug += 256) {
-    int s = ug >> 1, g = ug & 1;
-    if (rep[s] != s) continue;
-    int u = uid[s];
-    int b = evst[s], ne = evst[s + 1] - b, ep = 0;
-    int gs = net_class(s, n1);
+  for (int ug = tid; ug < 2 * UU; ug += 256) {
+    const int u = ug >> 1, g = ug & 1;
+    const int s = ulist[u];
+    const int b = evst[s], ne = evst[s + 1] - b;
+    const int gs = net_class(s, n1);
+    int ep = 0;
     double d = 0.0;
     for (int t = 0; t < G; t++) {
-      while (ep < ne && a.ev[im.evoff + es[b + ep]].t < t) ep++;
-      int f;
-      if (ep < ne && a.ev[im.evoff + es[b + ep]].t == t) {
-        f = a.evfam[im.evoff + es[b + ep]];
-      } else {
-        size_t lb = (size_t)(im.goff * 2 + gs * G + t);
-        int slot = ht_find(hkeys, fam_key(t, a.lcnt[lb], a.lfull + lb * EDDY_MAX_MP));
-        f = slot >= 0 ? hvals[slot] : -1;
-      }
-      d = d + (f >= 0 ? a.famscore[(im.foff + f) * 2 + g] : eddy_nan());
+      int f = famfull[gs * G + t];
+      if (ep < ne && ev[es[b + ep]].t == t) f = a.evfam[im.evoff + es[b + ep++]];
+      d = d + (f >= 0 ? fscore[f * 2 + g] : eddy_nan());
     }
     dist[g * S + u] = d;
   }
   __syncthreads();
   // ---- 6. likelihoods (main.cu:1294-1404) and JS (routines.cu:42-60) ----
   __shared__ double sh_inalpha[2], sh_scale[2], sh_nonInf[2], sh_infc[2];
-  const int UU = U;
   if (tid < 2) {
     int g = tid;
     double mn = dist[g * S], mx = dist[g * S], sum = 0.0;
@@ -865,23 +924,23 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
     lk[g * S + u] = sh_infc[g] == 0 ? l / sh_nonInf[g] : isinf / sh_infc[g];
   }
   __syncthreads();
-  // KL terms in parallel, summed serially in network order
-  double *kl = dist;  // reuse: terms stored after the likelihoods would alias; use evst-free region
-  (void)kl;
+  // KL terms in parallel (the family-score space is free now); summed
+  // serially in network order.  A skipped term (M = 0 or P = 0,
+  // routines.cu:47-53) adds +0.0, which leaves every partial sum unchanged:
+  // the sums start at +0.0 and no term is -0.0.
+  double *kt = (double *)dyn;
+  for (int ug = tid; ug < 2 * UU; ug += 256) {
+    int g = ug / UU, u = ug - g * UU;
+    double p1 = lk[u], p2 = lk[S + u];
+    double m = (p1 + p2) / 2;
+    double pp = g ? p2 : p1;
+    kt[ug] = (m == 0 || pp == 0) ? 0.0 : pp * eddy_log(pp / m);
+  }
+  __syncthreads();
   if (tid == 0) {
     double k1 = 0.0, k2 = 0.0;
-    for (int u = 0; u < UU; u++) {
-      double p1 = lk[u], p2 = lk[S + u];
-      double m = (p1 + p2) / 2;
-      if (m == 0) continue;
-      if (p1 != 0) k1 = k1 + p1 * eddy_log(p1 / m);
-    }
-    for (int u = 0; u < UU; u++) {
-      double p1 = lk[u], p2 = lk[S + u];
-      double m = (p1 + p2) / 2;
-      if (m == 0) continue;
-      if (p2 != 0) k2 = k2 + p2 * eddy_log(p2 / m);
-    }
+    for (int u = 0; u < UU; u++) k1 = k1 + kt[u];
+    for (int u = 0; u < UU; u++) k2 = k2 + kt[UU + u];
     double js = k1 / 2 + k2 / 2;
     a.js[im.set * a.perms + im.perm] = js / eddy_log(2.0);
   }

@@ -127,6 +127,9 @@ std::string load_classes(const std::string &path, int samples, ClassData &out) {
     }
   }
   if (out.n1 + out.n2 != samples) return "Every sample must carry one of the two class labels";
+  // e.g. "A\tA\tA\n": the last token "A\n" is picked as class 2 before its
+  // newline is cut, leaving class 2 empty (main.cu:312-336)
+  if (out.n1 == 0 || out.n2 == 0) return "Class file needs two class labels with samples each";
   return "";
 }
 

@@ -24,6 +24,7 @@ EXPORTS = [
     "eddy_upload_dataset", "eddy_stream_reset", "eddy_stream_draw", "eddy_run_sets",
     "eddy_get_js", "eddy_get_perm0_networks", "eddy_get_edgelist", "eddy_run_perm_detail",
     "eddy_get_timing", "eddy_synth_config", "eddy_synth_dataset", "eddy_synth_sets",
+    "eddy_edge_decision_host",
 ]
 
 
@@ -57,7 +58,7 @@ KERNELS = ["count_test", "loo", "rows", "score", "pvalue", "gather"]
 class Timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("kernel_ms", C.c_double * 6), ("launches", C.c_longlong * 6),
                 ("items", C.c_longlong), ("pairs", C.c_longlong), ("tests", C.c_longlong),
-                ("host_stream_ms", C.c_double)]
+                ("host_stream_ms", C.c_double), ("sched", C.c_longlong * 4)]
 
 
 class SynthCfg(C.Structure):
@@ -99,6 +100,7 @@ def lib():
         L.eddy_synth_sets.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                       C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p,
                                       C.c_longlong]
+        L.eddy_edge_decision_host.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(Params)]
         _lib = L
     return _lib
 

@@ -181,8 +181,16 @@ typedef struct {
   long long pairs;                   /* sum over items of G(G-1)/2 */
   long long tests;                   /* chi-square tests evaluated */
   double host_stream_ms;             /* host time replaying the rand() stream */
+  long long sched[4];                /* cutoff schedule: rows past their plan,
+                                        columns tested past plans, planned
+                                        (pair, class) entries, rows */
 } eddy_timing;
 int eddy_get_timing(eddy_ctx *ctx, eddy_timing *t);
+
+/* Host build of the engine's own edge decision for one 3x3 tally
+ * (tally9[r*3+k], r = row-gene state): 1 edge, 0 none.  Tests compare it
+ * with the CPU oracle; it runs on the CPU and needs no context. */
+int eddy_edge_decision_host(const int *tally9, int c, int prior, const eddy_params *p);
 
 /* ---- synthetic workloads (seeded DAG data in the benchmark shapes) ------- */
 typedef struct {
