@@ -170,6 +170,15 @@ extern "C" int eddy_stream_draw(eddy_ctx *ctx, int n, int32_t *out) {
   return EDDY_OK;
 }
 
+extern "C" int eddy_stream_permutation(eddy_ctx *ctx, int samples, int n1, int perm, uint8_t *labels) {
+  if (!ctx || samples < 1 || n1 < 0 || n1 > samples || !labels) return EDDY_ERR_ARG;
+  std::vector<int> ord;
+  ctx->stream_gen.permutation(samples, ord);
+  if (perm > 0)
+    for (int i = 0; i < samples; i++) labels[ord[i]] = i < n1 ? 0 : 1;  // main.cu:561-570
+  return EDDY_OK;
+}
+
 extern "C" int eddy_upload_dataset(eddy_ctx *ctx, const uint8_t *x, int genes, int samples, const uint8_t *class_of) {
   if (!ctx || !x || !class_of || genes <= 0 || samples <= 1) return EDDY_ERR_ARG;
   if (ctx->device < 0) return EDDY_ERR_NODEV;

@@ -13,6 +13,7 @@
 #pragma once
 #include <stdint.h>
 
+#include <algorithm>
 #include <vector>
 
 class PermStream {
@@ -20,16 +21,21 @@ class PermStream {
   PermStream() { reset(1); }
   void reset(uint32_t seed) {
     int32_t s = (int32_t)(seed == 0 ? 1 : seed);
-    r_[0] = (uint32_t)s;
+    uint32_t r[34];
+    r[0] = (uint32_t)s;
     int32_t w = s;
     for (int i = 1; i < 31; i++) {
       int32_t hi = w / 127773, lo = w % 127773;
       w = 16807 * lo - 2836 * hi;
       if (w < 0) w += 2147483647;
-      r_[i] = (uint32_t)w;
+      r[i] = (uint32_t)w;
     }
-    for (int i = 31; i < 34; i++) r_[i] = r_[i - 31];
-    i_ = 34;
+    for (int i = 31; i < 34; i++) r[i] = r[i - 31];
+    // buf_ holds r[base_ - 34 .. base_ + kBlock): the first 34 entries are
+    // the recurrence's history, the rest the next block of values
+    for (int i = 0; i < 34; i++) buf_[i] = r[i];
+    pos_ = kHist;
+    end_ = kHist;  // empty block: the next draw refills
     for (int k = 34; k < 344; k++) next_raw();
   }
   // next rand() value in [0, 2^31)
@@ -37,7 +43,9 @@ class PermStream {
 
   // One permutation's consumption (main.cu:541-558): `n` discarded draws,
   // then rejection sampling of a permutation of [0, n).  order receives the
-  // accepted values in acceptance order.
+  // accepted values in acceptance order.  `rand() % n` is computed with an
+  // exact multiply-shift reduction (Lemire, "Faster remainder by direct
+  // computation", 2019), which equals % for every 32-bit input.
   void permutation(int n, std::vector<int> &order) {
     order.resize(n);
     if ((int)stamp_.size() < n) stamp_.assign(n, 0);
@@ -45,26 +53,53 @@ class PermStream {
       std::fill(stamp_.begin(), stamp_.end(), 0);
       gen_ = 1;
     }
-    for (int c = 0; c < n; c++) next_raw();
+    const uint64_t M = UINT64_C(0xFFFFFFFFFFFFFFFF) / (uint32_t)n + 1;
+    skip(n);
+    uint32_t *stamp = stamp_.data();
+    const uint32_t gen = gen_;
     int k = 0;
     while (k < n) {
-      uint32_t v = next() % (uint32_t)n;
-      if (stamp_[v] != gen_) {
-        stamp_[v] = gen_;
-        order[k++] = (int)v;
+      if (pos_ == end_) refill();
+      const int avail = end_ - pos_;
+      const uint32_t *src = buf_ + pos_;
+      int i = 0;
+      for (; i < avail && k < n; i++) {
+        uint32_t x = src[i] >> 1;
+        uint32_t v = (uint32_t)(((__uint128_t)(M * x) * (uint32_t)n) >> 64);
+        int fresh = stamp[v] != gen;
+        stamp[v] = gen;
+        order[k] = (int)v;
+        k += fresh;
       }
+      pos_ += i;
     }
   }
 
  private:
-  inline uint32_t next_raw() {
-    uint32_t v = r_[(i_ - 31) & 63] + r_[(i_ - 3) & 63];
-    r_[i_ & 63] = v;
-    i_++;
-    return v;
+  static const int kHist = 34;
+  static const int kBlock = 8192;
+  // next block: r[i] = r[i-31] + r[i-3] over a linear array (3-way independent)
+  void refill() {
+    for (int i = 0; i < kHist; i++) buf_[i] = buf_[end_ - kHist + i];
+    uint32_t *b = buf_;
+    for (int i = kHist; i < kHist + kBlock; i++) b[i] = b[i - 31] + b[i - 3];
+    pos_ = kHist;
+    end_ = kHist + kBlock;
   }
-  uint32_t r_[64];
-  uint64_t i_ = 0;
+  inline uint32_t next_raw() {
+    if (pos_ == end_) refill();
+    return buf_[pos_++];
+  }
+  void skip(int n) {
+    while (n > 0) {
+      if (pos_ == end_) refill();
+      int m = std::min(n, end_ - pos_);
+      pos_ += m;
+      n -= m;
+    }
+  }
+  uint32_t buf_[kHist + kBlock];
+  int pos_ = 0, end_ = 0;
   std::vector<uint32_t> stamp_;
   uint32_t gen_ = 0;
 };

@@ -24,7 +24,7 @@ EXPORTS = [
     "eddy_upload_dataset", "eddy_stream_reset", "eddy_stream_draw", "eddy_run_sets",
     "eddy_get_js", "eddy_get_perm0_networks", "eddy_get_edgelist", "eddy_run_perm_detail",
     "eddy_get_timing", "eddy_synth_config", "eddy_synth_dataset", "eddy_synth_sets",
-    "eddy_edge_decision_host",
+    "eddy_edge_decision_host", "eddy_stream_permutation",
 ]
 
 
@@ -100,6 +100,7 @@ def lib():
         L.eddy_synth_sets.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                       C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p,
                                       C.c_longlong]
+        L.eddy_stream_permutation.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
         L.eddy_edge_decision_host.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(Params)]
         _lib = L
     return _lib

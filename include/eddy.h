@@ -122,6 +122,9 @@ int eddy_upload_dataset(eddy_ctx *ctx, const uint8_t *x, int genes, int samples,
 int eddy_stream_reset(eddy_ctx *ctx);
 /* Standalone generator for tests: n successive rand() values. */
 int eddy_stream_draw(eddy_ctx *ctx, int n, int32_t *out);
+/* One permutation's draws (main.cu:541-570): for perm > 0 labels[samples]
+ * receives the relabelling (0 = class 1); for perm 0 labels is untouched. */
+int eddy_stream_permutation(eddy_ctx *ctx, int samples, int n1, int perm, uint8_t *labels);
 
 /* ---- the hot path ----------------------------------------------------------
  * Runs every set's permutation loop (main.cu:538-1460) and p-value

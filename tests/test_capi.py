@@ -83,3 +83,22 @@ def test_no_gpu_engine_fails_loudly():
     assert rc == eddy_amd.EDDY_ERR_NODEV
     with pytest.raises(RuntimeError):
         eddy_amd.Engine(0)
+
+
+def test_stream_permutations_match_oracle():
+    """The engine's permutation labels (main.cu:538-573) equal the oracle's,
+    which calls the C library's rand(), over consecutive permutations."""
+    import oracle_ref
+
+    L, h = host_ctx()
+    for N, n1 in ((200, 100), (601, 300), (1200, 300)):
+        L.eddy_stream_reset(h)
+        oracle_ref.reset_stream()
+        lab_e = np.array([0] * n1 + [1] * (N - n1), np.uint8)
+        lab_o = lab_e.astype(np.int32)
+        for perm in range(12):
+            assert L.eddy_stream_permutation(h, N, n1, perm, lab_e.ctypes.data_as(C.c_void_p)) == 0
+            oracle_ref.lib().eo_perm_labels(N, n1, perm, lab_o.ctypes.data_as(C.c_void_p))
+            assert np.array_equal(lab_e, lab_o.astype(np.uint8)), (N, perm)
+            assert (lab_e == 0).sum() == n1
+    L.eddy_ctx_destroy(h)
