@@ -88,6 +88,12 @@ def cpu_baseline(wl, perms_run, seconds, threads):
     }
 
 
+def run_step(eng, wl, prm, cfg):
+    eng.reset_stream()
+    res, rep = eng.run_learned(wl.sets, prm, cfg)
+    return res, rep, eng.timing()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,9 +101,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--nsets", type=int, default=None)
-    ap.add_argument("--plain-sets", type=int, default=20)
+    ap.add_argument("--train-sets", type=int, default=100)
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--margin", type=float, default=1.0)
+    ap.add_argument("--plain-sets", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU sharding is not built (see DESIGN.md)",
@@ -113,12 +123,13 @@ def main():
     setup_s = time.perf_counter() - t_setup
     nsets = len(wl.sets)
     prm = wl.params(mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=1)
+    cfg = eddy_amd.learn_cfg(train_sets=args.train_sets, epochs=args.epochs, batch=4096, lr=3e-3,
+                             margin=args.margin, seed=1)
 
     print("bench: setup %.1fs, %d sets, N=%d" % (setup_s, nsets, wl.N), file=sys.stderr, flush=True)
     for w in range(args.warmup):
-        eng.reset_stream()
         tw = time.perf_counter()
-        eng.run_sets(wl.sets, prm)
+        run_step(eng, wl, prm, cfg)
         print("bench: warmup %d %.2fs" % (w, time.perf_counter() - tw), file=sys.stderr, flush=True)
     kern_ms = [0.0] * 6
     launches = [0] * 6
@@ -126,11 +137,13 @@ def main():
     items = 0
     host_stream_ms = 0.0
     sched = [0, 0, 0, 0]
+    stats = [0, 0, 0, 0]
+    phase = [0.0] * 10
+    reps = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        eng.reset_stream()
-        res = eng.run_sets(wl.sets, prm)
-        tm = eng.timing()
+        res, rep, tm = run_step(eng, wl, prm, cfg)
+        reps.append(rep)
         for k in range(6):
             kern_ms[k] += tm.kernel_ms[k]
             launches[k] += tm.launches[k]
@@ -139,15 +152,47 @@ def main():
         host_stream_ms += tm.host_stream_ms
         for k in range(4):
             sched[k] += tm.sched[k]
-        print("bench: step done, total %.2fs" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
+            stats[k] += tm.stats[k]
+        for k in range(10):
+            phase[k] += tm.phase_ms[k]
+        print("bench: step done, total %.2fs (collect %.0f ms, train %.0f ms, apply %.0f ms)"
+              % (time.perf_counter() - t0, rep.collect_ms, rep.train_ms, rep.apply_ms), file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     ms_per_step = 1000.0 * elapsed / args.steps
     value = nsets * args.steps / elapsed
     perms_run = [r.perms_run for r in res]
     ops, ref_tests = algorithmic_work(wl.sets, wl.N, perms_run)
-    js_batched = [eng.js(s).copy() for s in range(min(args.plain_sets, nsets))]
+    js_head = [eng.js(s).copy() for s in range(nsets)]
+    rep = reps[-1]
+    learned = {
+        "train_sets": min(args.train_sets, nsets), "train_rows": rep.train_rows, "epochs": args.epochs,
+        "loss_initial": rep.loss_initial, "loss_final": rep.loss_final, "margin": args.margin,
+        "collect_ms": rep.collect_ms, "train_ms": rep.train_ms, "apply_ms": rep.apply_ms,
+        "tests_applied_sets": rep.tests_apply,
+    }
 
-    # plain GPU form on a prefix of the sets (same stream, same results)
+    compare = None
+    if not args.no_compare:
+        # the same job without the network: identical results, tests and time beside the learned job
+        uprm = wl.params(mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=0)
+        eng.reset_stream()
+        tu = time.perf_counter()
+        ures = eng.run_sets(wl.sets, uprm)
+        dtu = time.perf_counter() - tu
+        same = all((a.status, a.perms_run, a.U0) == (b.status, b.perms_run, b.U0) for a, b in zip(ures, res))
+        same = same and all(bool((eng.js(s).view("u8") == js_head[s].view("u8")).all()) for s in range(nsets))
+        k = min(args.train_sets, nsets)
+        tests_u = sum(r.tests_eval for r in ures[k:])
+        compare = {"unlearned_sets_per_s": nsets / dtu, "unlearned_ms": dtu * 1000.0,
+                   "learned_ms_per_step": ms_per_step,
+                   "tests_applied_sets_unlearned": tests_u, "tests_applied_sets_learned": rep.tests_apply,
+                   "test_reduction_applied_sets": 1.0 - rep.tests_apply / max(1, tests_u),
+                   "results_bit_identical": same}
+        learned["vs_unlearned"] = compare
+
+    # plain GPU form (one set and one permutation per launch sequence, every
+    # test) on a prefix of the sets; its time is extrapolated to the whole
+    # workload by the reference's cost S*c*r per set
     plain = None
     if args.plain_sets > 0:
         sub = wl.sets[:args.plain_sets]
@@ -156,28 +201,34 @@ def main():
         tp = time.perf_counter()
         pres = eng.run_sets(sub, pprm)
         dtp = time.perf_counter() - tp
-        same = all(
-            ((r.status == q.status) and (r.perms_run == q.perms_run)) for r, q in zip(pres, res[:len(sub)]))
+        same = all((r.status, r.perms_run, r.U0) == (q.status, q.perms_run, q.U0) for r, q in zip(pres, res))
         for s in range(len(sub)):
-            a, b = eng.js(s), js_batched[s]
-            n = pres[s].perms_run
-            same = same and bool((a[:n].view("u8") == b[:n].view("u8")).all())
-        bprm = wl.params(mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=1)
-        eng.reset_stream()
-        tb = time.perf_counter()
-        eng.run_sets(sub, bprm)
-        dtb = time.perf_counter() - tb
-        plain = {"sets": len(sub), "plain_sets_per_s": len(sub) / dtp,
-                 "batched_learned_sets_per_s_same_sets": len(sub) / dtb,
-                 "speedup": dtp / dtb, "js_bit_identical": same}
+            same = same and bool((eng.js(s).view("u8") == js_head[s].view("u8")).all())
+        cost = [(wl.N + 2) * len(g) * (len(g) - 1) // 2 * r for (g, _), r in zip(wl.sets, perms_run)]
+        frac = sum(cost[:len(sub)]) / max(1, sum(cost))
+        plain_rate = nsets / (dtp / max(frac, 1e-12))
+        plain = {"sets": len(sub), "seconds": dtp, "cost_fraction_of_workload": frac,
+                 "plain_sets_per_s_extrapolated": plain_rate, "speedup_batched_learned_vs_plain": value / plain_rate,
+                 "results_bit_identical": same}
 
     # roofline of the dominant kernel
     dom = max(range(6), key=lambda k: kern_ms[k])
     dom_avg_s = kern_ms[dom] / max(1, launches[dom]) / 1000.0
+    per_launch = lambda total: total / max(1, launches[dom])
     if dom == 0:
-        achieved = ops * args.steps / max(1, launches[0]) / dom_avg_s
+        achieved = per_launch(ops * args.steps) / dom_avg_s
         roof = {"kernel": "k_count_test", "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_INT8_OPS / 1e12,
                 "unit": "TOP/s", "frac": achieved / PEAK_INT8_OPS, "traffic": None}
+    elif dom == 3:
+        # BDeu: every (family, sample) reads the child's and parents' states
+        achieved = per_launch(stats[0]) / dom_avg_s
+        roof = {"kernel": "k_score", "bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
+                "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None}
+    elif dom == 1:
+        # leave-one-out tests: each planned (pair, class) entry reads its 9 u16 cells and 4 B of info
+        achieved = per_launch(sched[2] * 22) / dom_avg_s
+        roof = {"kernel": "k_loo", "bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
+                "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None}
     else:
         roof = {"kernel": eddy_amd.KERNELS[dom], "bound": "hbm", "achieved": None, "peak": PEAK_HBM / 1e9,
                 "unit": "GB/s", "frac": None, "traffic": None}
@@ -203,10 +254,10 @@ def main():
         "vs_baseline": None,
         "dtype": "int8 counts / fp64 tests",
         "data": "synthetic (seeded linear-Gaussian DAG, tertiles)",
-        "config": {"workload": "config %d: %d genes x %d samples (%d/%d), %d gene sets (G %d..%d log-uniform), "
-                               "r=%d, mp=%d, pw=%.1f" % (args.config, wl.cfg.genes, wl.N, wl.cfg.nA, wl.cfg.nB, nsets,
-                                                          wl.cfg.gmin, wl.cfg.gmax, wl.cfg.perms, wl.cfg.mp,
-                                                          wl.cfg.prior_weight),
+        "config": {"workload": "%d genes x %d samples (%d/%d), %d gene sets (G %d..%d log-uniform), r=%d, mp=%d, "
+                               "pw=%.1f; network trained on the first %d sets, applied to the rest"
+                               % (wl.cfg.genes, wl.N, wl.cfg.nA, wl.cfg.nB, nsets, wl.cfg.gmin, wl.cfg.gmax,
+                                  wl.cfg.perms, wl.cfg.mp, wl.cfg.prior_weight, min(args.train_sets, nsets)),
                    "global_batch": nsets, "parallelism": "single GPU"},
         "ci_tests_per_s_ref_equiv": ref_tests * args.steps / elapsed,
         "ci_tests_per_s_evaluated": tests_eval / elapsed,
@@ -215,7 +266,11 @@ def main():
         "host_stream_ms_per_step": host_stream_ms / args.steps,
         "schedule": {"rows_past_plan": sched[0] // args.steps, "cols_tested_past_plan": sched[1] // args.steps,
                      "planned_entries": sched[2] // args.steps, "rows": sched[3] // args.steps},
+        "score_stats": {"hist_bytes": stats[0] // args.steps, "families": stats[1] // args.steps,
+                        "unique_networks": stats[2] // args.steps, "events": stats[3] // args.steps},
+        "phase_wg_ms": [round(x / args.steps, 1) for x in phase],
         "setup_s": setup_s,
+        "learned": learned,
         "plain": plain,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -9,7 +9,9 @@
 //   * `-p` (documented in the reference README but never parsed there) is an
 //     alias of `-pD`;
 //   * `--mode plain|batched`, `--tests all|cutoff` and `--no-learned` choose
-//     the engine schedule (results are identical in every mode);
+//     the engine schedule (results are identical in every mode); by default
+//     the first `--learn-sets K` (100) sets train the learned proposal's
+//     network and the rest use it;
 //     `--parse-only` prints the parsed inputs and stops (no GPU);
 //   * a degenerate beta fit (e.g. -r 1), where the reference's Boost call
 //     throws and aborts, prints pDDN as nan and writes no files;
@@ -35,7 +37,7 @@ int main(int argc, char **argv) {
   int parentCap = 0, perms = 100;
   double priorWeight = 0.5, alphaDDN = 0.05, theta = -1, lambda = 2.0, alphaEdge = 0.05;
   bool pAdjust = true;
-  int mode = EDDY_MODE_BATCHED, schedule = EDDY_TESTS_CUTOFF, learned = 1;
+  int mode = EDDY_MODE_BATCHED, schedule = EDDY_TESTS_CUTOFF, learned = 1, learn_sets = 100;
   bool parse_only = false;
   for (int i = 1; i < argc; i++) {
     std::string a = argv[i];
@@ -57,6 +59,7 @@ int main(int argc, char **argv) {
     else if (a == "--mode") mode = std::string(nx) == "plain" ? EDDY_MODE_PLAIN : EDDY_MODE_BATCHED;
     else if (a == "--tests") schedule = std::string(nx) == "all" ? EDDY_TESTS_ALL : EDDY_TESTS_CUTOFF;
     else if (a == "--no-learned") learned = 0;
+    else if (a == "--learn-sets") learn_sets = atoi(nx);
     else if (a == "--parse-only") parse_only = true;
   }
   if (input.empty()) fail("Invalid input file entered. Exiting...");
@@ -117,7 +120,7 @@ int main(int argc, char **argv) {
   p.alpha_ddn = alphaDDN;
   p.mode = mode;
   p.schedule = schedule;
-  p.learned = learned;
+  p.learned = 0;
   std::vector<eddy_set_desc> desc(sets.size());
   for (size_t i = 0; i < sets.size(); i++) {
     desc[i].n_genes = (int)sets[i].rows.size();
@@ -126,7 +129,18 @@ int main(int argc, char **argv) {
     desc[i].prior = sets[i].prior.data();
   }
   std::vector<eddy_set_result> out(sets.size());
-  rc = eddy_run_sets(ctx, desc.data(), (int)desc.size(), &p, out.data());
+  if (learned && mode == EDDY_MODE_BATCHED && schedule == EDDY_TESTS_CUTOFF && learn_sets >= 0) {
+    eddy_learn_cfg lc{};
+    lc.train_sets = learn_sets;
+    lc.epochs = 4;
+    lc.batch = 4096;
+    lc.lr = 3e-3f;
+    lc.margin = 1.0f;
+    lc.seed = 1;
+    rc = eddy_run_learned(ctx, desc.data(), (int)desc.size(), &p, &lc, out.data(), nullptr);
+  } else {
+    rc = eddy_run_sets(ctx, desc.data(), (int)desc.size(), &p, out.data());
+  }
   if (rc != EDDY_OK) fail(std::string("engine failed: ") + eddy_last_error(ctx));
 
   fprintf(res, "Pathway\tJS\tPval\tnGenes\n");

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <random>
 #include <thread>
 #include <chrono>
 #include <cmath>
@@ -16,6 +17,7 @@
 
 #include "eddy.h"
 #include "engine_types.h"
+#include "mlp.h"
 #include "perm_stream.h"
 
 // kernels (kernels.hip)
@@ -26,6 +28,7 @@ __global__ void k_score(KArgs a);
 __global__ void k_pvalue(const double *js, int nsets, int perms, const int *nan0, double *out);
 __global__ void k_gather(const uint8_t *x, int N, int NP, const int *genes, const long long *rowoff, int nrows,
                          uint8_t *xs);
+__global__ void k_window(const SetMeta *sets, int nsets, const float *score, int mp, float kappa, int *win);
 
 namespace {
 
@@ -50,9 +53,14 @@ struct Perm0 {
 
 }  // namespace
 
+struct HostProf {  // host-side time split of one eddy_run_sets call (ms)
+  double launch = 0, sync = 0, capture = 0, wait = 0, build = 0, js0 = 0, tail = 0;
+};
+
 struct eddy_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  HostProf hp;
   std::string err;
   // dataset
   int genes = 0, N = 0, n1 = 0, n2 = 0;
@@ -65,7 +73,7 @@ struct eddy_ctx {
   // device buffers, grown on demand
   DevBuf b_sets, b_xs, b_prior, b_items, b_labels, b_tiles, b_info, b_tab, b_lfull, b_lcnt, b_sflags, b_ev,
       b_evcount, b_hnet, b_rep, b_clsg, b_evorder, b_evfam, b_famkey, b_famscore, b_lval, b_uniq, b_js,
-      b_status, b_tests, b_genes, b_rowoff, b_nan0, b_pout, b_famcount, b_counters;
+      b_status, b_tests, b_genes, b_rowoff, b_nan0, b_pout, b_famcount, b_counters, b_famht, b_famhv;
   // last run
   int last_perms = 0;
   std::vector<int> last_G;
@@ -73,6 +81,16 @@ struct eddy_ctx {
   std::vector<Perm0> last_p0;
   eddy_timing timing{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // learned proposal: network parameters + Adam state, per-run feature,
+  // label and window arrays (device pointers handed to the kernels)
+  DevBuf b_mlp, b_mlpm, b_mlpv, b_mlpg, b_mlploss, b_feat, b_labhits, b_score, b_win, b_mx, b_my, b_mxb, b_myb,
+      b_midx, b_mscr, b_mlbase, b_mlinv;
+  bool mlp_ready = false;
+  int mlp_step = 0;
+  float mlp_kappa = 1.0f;
+  float *k_feat = nullptr;
+  unsigned *k_labhits = nullptr;
+  const int *k_win = nullptr;
 };
 
 #define HIPCHK(ctx, x)                                                                  \
@@ -144,7 +162,9 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
                    &ctx->b_info, &ctx->b_tab, &ctx->b_lfull, &ctx->b_lcnt, &ctx->b_sflags, &ctx->b_ev,
                    &ctx->b_evcount, &ctx->b_hnet, &ctx->b_rep, &ctx->b_clsg, &ctx->b_evorder, &ctx->b_evfam,
                    &ctx->b_famkey, &ctx->b_famscore, &ctx->b_lval, &ctx->b_uniq, &ctx->b_js, &ctx->b_status,
-                   &ctx->b_tests, &ctx->b_genes, &ctx->b_rowoff, &ctx->b_nan0, &ctx->b_pout, &ctx->b_famcount, &ctx->b_counters};
+                   &ctx->b_tests, &ctx->b_genes, &ctx->b_rowoff, &ctx->b_nan0, &ctx->b_pout, &ctx->b_famcount, &ctx->b_counters, &ctx->b_famht, &ctx->b_famhv, &ctx->b_mlp, &ctx->b_mlpm,
+                   &ctx->b_mlpv, &ctx->b_mlpg, &ctx->b_mlploss, &ctx->b_feat, &ctx->b_labhits, &ctx->b_score,
+                   &ctx->b_win, &ctx->b_mx, &ctx->b_my, &ctx->b_mxb, &ctx->b_myb, &ctx->b_midx, &ctx->b_mscr, &ctx->b_mlbase, &ctx->b_mlinv};
   for (DevBuf *b : all)
     if (b->p) hipFree(b->p);
   if (ctx->dX) hipFree(ctx->dX);
@@ -317,193 +337,278 @@ struct Chunk {
   long long pairs = 0, gslots = 0, sslots = 0;
 };
 
+void add_item(Chunk &ch, const std::vector<SetMeta> &sets, int set, int perm, const std::vector<uint32_t> &labbits,
+              int N);
+
+// items `idx` of `src` as a chunk of their own (fresh offsets)
+void sub_chunk(const Chunk &src, const std::vector<int> &idx, const std::vector<SetMeta> &sets, int N, Chunk &dst) {
+  dst = Chunk();
+  const int words = (N + 31) / 32;
+  std::vector<uint32_t> lab(words);
+  for (int i : idx) {
+    const ItemMeta &im = src.items[i];
+    std::copy(src.labels.begin() + im.laboff, src.labels.begin() + im.laboff + words, lab.begin());
+    add_item(dst, sets, im.set, im.perm, lab, N);
+  }
+}
+
+// one launch sequence over a chunk; per-item event capacity `evcap`
+int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, const eddy_params *p, int perms,
+                 const std::vector<int> &evcap, int attempt, std::vector<int> &status, std::vector<int> &evc,
+                 std::vector<long long> &tests) {
+  auto hl0 = std::chrono::steady_clock::now();
+  const int N = ctx->N, S = N + 2;
+  const int nitems = (int)ch.items.size();
+  int maxG = 0;
+  for (const ItemMeta &im : ch.items) maxG = std::max(maxG, sets[im.set].G);
+  int ht = 1024;
+  while (ht < 2 * S) ht *= 2;
+  long long evoff = 0, foff = 0, fhoff = 0;
+  for (int i = 0; i < nitems; i++) {
+    ItemMeta &im = ch.items[i];
+    const int G = sets[im.set].G;
+    im.evcap = evcap[i];
+    im.fcap = 2 * G + evcap[i];
+    im.fhsize = 1024;
+    while (im.fhsize < 2 * im.fcap) im.fhsize *= 2;
+    im.evoff = evoff;
+    im.foff = foff;
+    im.fhoff = fhoff;
+    evoff += im.evcap;
+    foff += im.fcap;
+    fhoff += im.fhsize;
+  }
+  GROW(b_items, sizeof(ItemMeta) * nitems);
+  GROW(b_labels, sizeof(uint32_t) * ch.labels.size());
+  GROW(b_tiles, sizeof(uint2) * ch.tiles.size());
+  GROW(b_info, sizeof(uint32_t) * ch.pairs);
+  const bool plain_tests = p->schedule == EDDY_TESTS_ALL;
+  if (!plain_tests) GROW(b_tab, sizeof(uint16_t) * 18 * ch.pairs);
+  GROW(b_lfull, sizeof(uint16_t) * EDDY_MAX_MP * 2 * ch.gslots);
+  GROW(b_lcnt, 2 * ch.gslots);
+  GROW(b_sflags, 2 * ch.gslots);
+  GROW(b_ev, sizeof(Event) * evoff);
+  GROW(b_evcount, sizeof(int) * nitems);
+  GROW(b_hnet, sizeof(uint64_t) * ch.sslots);
+  GROW(b_rep, sizeof(int) * ch.sslots);
+  GROW(b_clsg, sizeof(uint16_t) * ch.sslots);
+  GROW(b_evorder, sizeof(int) * evoff);
+  GROW(b_evfam, sizeof(int) * evoff);
+  GROW(b_famkey, sizeof(uint64_t) * foff);
+  GROW(b_famscore, sizeof(double) * 2 * foff);
+  GROW(b_famht, sizeof(unsigned long long) * fhoff);
+  GROW(b_famhv, sizeof(int) * fhoff);
+  GROW(b_lval, sizeof(double) * 2 * ch.sslots);
+  GROW(b_uniq, sizeof(int) * nitems);
+  GROW(b_famcount, sizeof(int) * nitems);
+  GROW(b_status, sizeof(int) * nitems);
+  GROW(b_tests, sizeof(long long) * nitems);
+  hipStream_t st = ctx->stream;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->b_items.p, ch.items.data(), sizeof(ItemMeta) * nitems, hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->b_labels.p, ch.labels.data(), sizeof(uint32_t) * ch.labels.size(),
+                             hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->b_tiles.p, ch.tiles.data(), sizeof(uint2) * ch.tiles.size(), hipMemcpyHostToDevice,
+                             st));
+  HIPCHK(ctx, hipMemsetAsync(ctx->b_evcount.p, 0, sizeof(int) * nitems, st));
+  HIPCHK(ctx, hipMemsetAsync(ctx->b_status.p, 0, sizeof(int) * nitems, st));
+  HIPCHK(ctx, hipMemsetAsync(ctx->b_tests.p, 0, sizeof(long long) * nitems, st));
+
+  KArgs a{};
+  a.N = N;
+  a.n1 = ctx->n1;
+  a.n2 = ctx->n2;
+  a.S = S;
+  a.mp = p->mp;
+  a.p_adjust = p->p_adjust;
+  a.plain = plain_tests ? 1 : 0;
+  a.ht_size = ht;
+  a.plan_extra = nullptr;
+  a.feat = ctx->k_feat;
+  a.labhits = ctx->k_labhits;
+  a.win = ctx->k_win;
+  a.counters = (unsigned long long *)ctx->b_counters.p;
+  edge_params(ctx, p, a.ep);
+  if (getenv("EDDY_NO_SHORTCUT")) a.ep.shortcut = 0;
+  a.lgtab = ctx->dLG;
+  a.lgstride = ctx->lgstride;
+  a.hseed = 0x243F6A8885A308D3ULL + (uint64_t)attempt * 0x9E3779B97F4A7C15ULL;
+  a.sets = (const SetMeta *)ctx->b_sets.p;
+  a.xs = (const uint8_t *)ctx->b_xs.p;
+  a.prior = (const uint32_t *)ctx->b_prior.p;
+  a.items = (const ItemMeta *)ctx->b_items.p;
+  a.labels = (const uint32_t *)ctx->b_labels.p;
+  a.tiles = (const uint32_t *)ctx->b_tiles.p;
+  a.info = (uint32_t *)ctx->b_info.p;
+  a.tab = (uint16_t *)ctx->b_tab.p;
+  a.lfull = (uint16_t *)ctx->b_lfull.p;
+  a.lcnt = (uint8_t *)ctx->b_lcnt.p;
+  a.sflags = (uint8_t *)ctx->b_sflags.p;
+  a.ev = (Event *)ctx->b_ev.p;
+  a.evcount = (int *)ctx->b_evcount.p;
+  a.hnet = (uint64_t *)ctx->b_hnet.p;
+  a.rep = (int *)ctx->b_rep.p;
+  a.clsg = (uint16_t *)ctx->b_clsg.p;
+  a.evorder = (int *)ctx->b_evorder.p;
+  a.evfam = (int *)ctx->b_evfam.p;
+  a.famkey = (uint64_t *)ctx->b_famkey.p;
+  a.famht = (unsigned long long *)ctx->b_famht.p;
+  a.famhv = (int *)ctx->b_famhv.p;
+  a.famscore = (double *)ctx->b_famscore.p;
+  a.lval = (double *)ctx->b_lval.p;
+  a.uniq_u = (int *)ctx->b_uniq.p;
+  a.famcount = (int *)ctx->b_famcount.p;
+  a.js = (double *)ctx->b_js.p;
+  a.perms = perms;
+  a.status = (int *)ctx->b_status.p;
+  a.tests = (long long *)ctx->b_tests.p;
+
+  hipEvent_t ev[5];
+  for (int k = 0; k < 5; k++) hipEventCreate(&ev[k]);
+  hipEventRecord(ev[0], st);
+  if (!ch.tiles.empty()) k_count_test<<<(unsigned)ch.tiles.size(), 256, 0, st>>>(a);
+  hipEventRecord(ev[1], st);
+  if (!plain_tests) k_loo<<<nitems, 256, 0, st>>>(a);
+  hipEventRecord(ev[2], st);
+  k_rows<<<nitems, 256, 0, st>>>(a);
+  hipEventRecord(ev[3], st);
+  const size_t score_lds = (size_t)ht * 12 + (size_t)(4 * S + 1 + 2 * maxG) * 4;
+  k_score<<<nitems, 256, score_lds, st>>>(a);
+  hipEventRecord(ev[4], st);
+  HIPCHK(ctx, hipGetLastError());
+  status.assign(nitems, 0);
+  evc.assign(nitems, 0);
+  tests.assign(nitems, 0);
+  HIPCHK(ctx, hipMemcpyAsync(status.data(), ctx->b_status.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(evc.data(), ctx->b_evcount.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(tests.data(), ctx->b_tests.p, sizeof(long long) * nitems, hipMemcpyDeviceToHost, st));
+  auto hl1 = std::chrono::steady_clock::now();
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  auto hl2 = std::chrono::steady_clock::now();
+  ctx->hp.launch += std::chrono::duration<double, std::milli>(hl1 - hl0).count();
+  ctx->hp.sync += std::chrono::duration<double, std::milli>(hl2 - hl1).count();
+  for (int k = 0; k < 4; k++) {
+    if (k == 1 && plain_tests) continue;
+    if (k == 0 && ch.tiles.empty()) continue;
+    float ms = 0;
+    hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+    ctx->timing.kernel_ms[k] += ms;
+    ctx->timing.launches[k] += 1;
+  }
+  for (int k = 0; k < 5; k++) hipEventDestroy(ev[k]);
+  if (getenv("EDDY_DEBUG")) {
+    int mx = 0, bad = 0, nb = 0;
+    for (int i = 0; i < nitems; i++) {
+      mx = std::max(mx, evc[i]);
+      bad |= status[i];
+      nb += status[i] != 0;
+    }
+    fprintf(stderr, "[eddy] launch items=%d pairs=%lld attempt=%d max_events=%d failing=%d status=%d\n", nitems,
+            ch.pairs, attempt, mx, nb, bad);
+  }
+  return EDDY_OK;
+}
+
+// Permutation-0 state of the finished items `idx` of chunk `ch` (one item per
+// set): the device buffers are copied back in bulk, then sliced per set.
+int capture_items(eddy_ctx *ctx, const std::vector<SetMeta> &sets, const Chunk &ch, const std::vector<int> &idx,
+                  const std::vector<int> &evc, std::vector<Perm0> *p0) {
+  auto hc0 = std::chrono::steady_clock::now();
+  const int S = ctx->N + 2;
+  const int nitems = (int)ch.items.size();
+  hipStream_t st = ctx->stream;
+  std::vector<int> uq(nitems), fc(nitems);
+  std::vector<uint8_t> lcnt(2 * ch.gslots);
+  std::vector<uint16_t> lfull((size_t)2 * ch.gslots * EDDY_MAX_MP);
+  std::vector<int> rep(ch.sslots);
+  std::vector<double> lval(2 * ch.sslots);
+  std::vector<uint32_t> info(ch.pairs);
+  HIPCHK(ctx, hipMemcpyAsync(uq.data(), ctx->b_uniq.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(fc.data(), ctx->b_famcount.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(lcnt.data(), ctx->b_lcnt.p, lcnt.size(), hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(lfull.data(), ctx->b_lfull.p, sizeof(uint16_t) * lfull.size(), hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(rep.data(), ctx->b_rep.p, sizeof(int) * rep.size(), hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(lval.data(), ctx->b_lval.p, sizeof(double) * lval.size(), hipMemcpyDeviceToHost, st));
+  if (!info.empty())
+    HIPCHK(ctx, hipMemcpyAsync(info.data(), ctx->b_info.p, sizeof(uint32_t) * info.size(), hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  for (int k : idx) {
+    const ItemMeta &im = ch.items[k];
+    const SetMeta &sm = sets[im.set];
+    Perm0 &q = (*p0)[im.set];
+    q.U = uq[k];
+    q.lcnt.assign(lcnt.begin() + im.goff * 2, lcnt.begin() + im.goff * 2 + 2 * sm.G);
+    q.lfull.assign(lfull.begin() + im.goff * 2 * EDDY_MAX_MP, lfull.begin() + (im.goff * 2 + 2 * sm.G) * EDDY_MAX_MP);
+    q.rep.assign(rep.begin() + im.soff, rep.begin() + im.soff + S);
+    q.lval.assign(lval.begin() + im.soff * 2, lval.begin() + im.soff * 2 + 2 * S);
+    q.info.assign(info.begin() + im.infooff, info.begin() + im.infooff + sm.c);
+    q.ev.resize(std::min(evc[k], im.evcap));
+    q.famkey.resize(fc[k]);
+    q.famscore.resize(2 * (size_t)fc[k]);
+    if (!q.ev.empty())
+      HIPCHK(ctx, hipMemcpyAsync(q.ev.data(), (Event *)ctx->b_ev.p + im.evoff, sizeof(Event) * q.ev.size(),
+                                 hipMemcpyDeviceToHost, st));
+    if (fc[k]) {
+      HIPCHK(ctx, hipMemcpyAsync(q.famkey.data(), (uint64_t *)ctx->b_famkey.p + im.foff, sizeof(uint64_t) * fc[k],
+                                 hipMemcpyDeviceToHost, st));
+      HIPCHK(ctx, hipMemcpyAsync(q.famscore.data(), (double *)ctx->b_famscore.p + im.foff * 2,
+                                 sizeof(double) * 2 * fc[k], hipMemcpyDeviceToHost, st));
+    }
+  }
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  ctx->hp.capture += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hc0).count();
+  return EDDY_OK;
+}
+
 int run_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, const eddy_params *p, int perms,
               bool capture, std::vector<Perm0> *p0, std::vector<long long> &tests_out) {
   const int N = ctx->N, S = N + 2;
   const int nitems = (int)ch.items.size();
   if (nitems == 0) return EDDY_OK;
-  int ev_cap = std::max(1024, 2 * S);
-  int maxG = 0;
-  for (const ItemMeta &im : ch.items) maxG = std::max(maxG, sets[im.set].G);
-  int ht = 1024;
-  while (ht < 2 * std::max(S, 2 * maxG + 128)) ht *= 2;
-  for (int attempt = 0; attempt < 6; attempt++) {
-    int fam_cap = 2 * maxG + ev_cap;
-    // offsets
-    long long evoff = 0, foff = 0;
-    for (int i = 0; i < nitems; i++) {
-      ch.items[i].evoff = evoff;
-      ch.items[i].foff = foff;
-      evoff += ev_cap;
-      foff += fam_cap;
+  // first guess of the events per item (rows of leave-one-out networks that
+  // differ from the full network); an item that overflows is re-run alone
+  // with its exact count
+  std::vector<int> cap(nitems);
+  for (int i = 0; i < nitems; i++) cap[i] = std::max(512, 2 * S + 24 * sets[ch.items[i].set].G);
+  std::vector<int> todo(nitems);
+  for (int i = 0; i < nitems; i++) todo[i] = i;
+  Chunk sub;
+  for (int attempt = 0; attempt < 6 && !todo.empty(); attempt++) {
+    Chunk *cur = &ch;
+    if (attempt > 0) {
+      sub_chunk(ch, todo, sets, N, sub);
+      cur = &sub;
     }
-    GROW(b_items, sizeof(ItemMeta) * nitems);
-    GROW(b_labels, sizeof(uint32_t) * ch.labels.size());
-    GROW(b_tiles, sizeof(uint2) * ch.tiles.size());
-    GROW(b_info, sizeof(uint32_t) * ch.pairs);
-    const bool plain_tests = p->schedule == EDDY_TESTS_ALL;
-    if (!plain_tests) GROW(b_tab, sizeof(uint16_t) * 18 * ch.pairs);
-    GROW(b_lfull, sizeof(uint16_t) * EDDY_MAX_MP * 2 * ch.gslots);
-    GROW(b_lcnt, 2 * ch.gslots);
-    GROW(b_sflags, 2 * ch.gslots);
-    GROW(b_ev, sizeof(Event) * evoff);
-    GROW(b_evcount, sizeof(int) * nitems);
-    GROW(b_hnet, sizeof(uint64_t) * ch.sslots);
-    GROW(b_rep, sizeof(int) * ch.sslots);
-    GROW(b_clsg, sizeof(uint16_t) * ch.sslots);
-    GROW(b_evorder, sizeof(int) * evoff);
-    GROW(b_evfam, sizeof(int) * evoff);
-    GROW(b_famkey, sizeof(uint64_t) * foff);
-    GROW(b_famscore, sizeof(double) * 2 * foff);
-    GROW(b_lval, sizeof(double) * 2 * ch.sslots);
-    GROW(b_uniq, sizeof(int) * nitems);
-    GROW(b_famcount, sizeof(int) * nitems);
-    GROW(b_status, sizeof(int) * nitems);
-    GROW(b_tests, sizeof(long long) * nitems);
-    hipStream_t st = ctx->stream;
-    HIPCHK(ctx, hipMemcpyAsync(ctx->b_items.p, ch.items.data(), sizeof(ItemMeta) * nitems, hipMemcpyHostToDevice, st));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->b_labels.p, ch.labels.data(), sizeof(uint32_t) * ch.labels.size(),
-                               hipMemcpyHostToDevice, st));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->b_tiles.p, ch.tiles.data(), sizeof(uint2) * ch.tiles.size(), hipMemcpyHostToDevice,
-                               st));
-    HIPCHK(ctx, hipMemsetAsync(ctx->b_evcount.p, 0, sizeof(int) * nitems, st));
-    HIPCHK(ctx, hipMemsetAsync(ctx->b_status.p, 0, sizeof(int) * nitems, st));
-    HIPCHK(ctx, hipMemsetAsync(ctx->b_tests.p, 0, sizeof(long long) * nitems, st));
-
-    KArgs a{};
-    a.N = N;
-    a.n1 = ctx->n1;
-    a.n2 = ctx->n2;
-    a.S = S;
-    a.mp = p->mp;
-    a.p_adjust = p->p_adjust;
-    a.plain = plain_tests ? 1 : 0;
-    a.ev_cap = ev_cap;
-    a.fam_cap = fam_cap;
-    a.ht_size = ht;
-    a.plan_extra = nullptr;
-    a.counters = (unsigned long long *)ctx->b_counters.p;
-    edge_params(ctx, p, a.ep);
-    if (getenv("EDDY_NO_SHORTCUT")) a.ep.shortcut = 0;
-    a.lgtab = ctx->dLG;
-    a.lgstride = ctx->lgstride;
-    a.hseed = 0x243F6A8885A308D3ULL + (uint64_t)attempt * 0x9E3779B97F4A7C15ULL;
-    a.sets = (const SetMeta *)ctx->b_sets.p;
-    a.xs = (const uint8_t *)ctx->b_xs.p;
-    a.prior = (const uint32_t *)ctx->b_prior.p;
-    a.items = (const ItemMeta *)ctx->b_items.p;
-    a.labels = (const uint32_t *)ctx->b_labels.p;
-    a.tiles = (const uint32_t *)ctx->b_tiles.p;
-    a.info = (uint32_t *)ctx->b_info.p;
-    a.tab = (uint16_t *)ctx->b_tab.p;
-    a.lfull = (uint16_t *)ctx->b_lfull.p;
-    a.lcnt = (uint8_t *)ctx->b_lcnt.p;
-    a.sflags = (uint8_t *)ctx->b_sflags.p;
-    a.ev = (Event *)ctx->b_ev.p;
-    a.evcount = (int *)ctx->b_evcount.p;
-    a.hnet = (uint64_t *)ctx->b_hnet.p;
-    a.rep = (int *)ctx->b_rep.p;
-    a.clsg = (uint16_t *)ctx->b_clsg.p;
-    a.evorder = (int *)ctx->b_evorder.p;
-    a.evfam = (int *)ctx->b_evfam.p;
-    a.famkey = (uint64_t *)ctx->b_famkey.p;
-    a.famscore = (double *)ctx->b_famscore.p;
-    a.lval = (double *)ctx->b_lval.p;
-    a.uniq_u = (int *)ctx->b_uniq.p;
-    a.famcount = (int *)ctx->b_famcount.p;
-    a.js = (double *)ctx->b_js.p;
-    a.perms = perms;
-    a.status = (int *)ctx->b_status.p;
-    a.tests = (long long *)ctx->b_tests.p;
-
-    hipEvent_t ev[5];
-    for (int k = 0; k < 5; k++) hipEventCreate(&ev[k]);
-    hipEventRecord(ev[0], st);
-    if (!ch.tiles.empty()) k_count_test<<<(unsigned)ch.tiles.size(), 256, 0, st>>>(a);
-    hipEventRecord(ev[1], st);
-    if (!plain_tests) k_loo<<<nitems, 256, 0, st>>>(a);
-    hipEventRecord(ev[2], st);
-    k_rows<<<nitems, 256, 0, st>>>(a);
-    hipEventRecord(ev[3], st);
-    const size_t score_lds = (size_t)ht * 12 + (size_t)(4 * S + 1 + 2 * maxG) * 4;
-    k_score<<<nitems, 256, score_lds, st>>>(a);
-    hipEventRecord(ev[4], st);
-    HIPCHK(ctx, hipGetLastError());
-    std::vector<int> status(nitems);
-    HIPCHK(ctx, hipMemcpyAsync(status.data(), ctx->b_status.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
-    std::vector<long long> tests(nitems);
-    HIPCHK(ctx, hipMemcpyAsync(tests.data(), ctx->b_tests.p, sizeof(long long) * nitems, hipMemcpyDeviceToHost, st));
-    HIPCHK(ctx, hipStreamSynchronize(st));
-    const int kidx[4] = {0, 1, 2, 3};
-    for (int k = 0; k < 4; k++) {
-      if (k == 1 && plain_tests) continue;
-      if (k == 0 && ch.tiles.empty()) continue;
-      float ms = 0;
-      hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
-      ctx->timing.kernel_ms[kidx[k]] += ms;
-      ctx->timing.launches[kidx[k]] += 1;
-    }
-    for (int k = 0; k < 5; k++) hipEventDestroy(ev[k]);
-    int bad = 0;
-    for (int s : status) bad |= s;
-    if (bad & ST_NOT_EVALUATED) {
-      ctx->err = "internal: a row read a leave-one-out test that was not evaluated";
-      return EDDY_ERR_STATE;
-    }
-    if (bad & (ST_EV_OVERFLOW | ST_FAM_OVERFLOW | ST_COLLISION)) {
-      if (bad & ST_EV_OVERFLOW) ev_cap *= 4;
-      if (bad & ST_FAM_OVERFLOW) ht *= 2;
-      continue;  // retry the chunk (larger capacity / new hash seed)
-    }
-    for (int i = 0; i < nitems; i++) {
-      tests_out[i] = tests[i];
-      ctx->timing.tests += tests[i];
-    }
-    ctx->timing.items += nitems;
-    ctx->timing.pairs += ch.pairs;
-    if (capture) {
-      // copy back permutation-0 state of every item (one item per set)
-      std::vector<int> evc(nitems), uq(nitems), fc(nitems);
-      HIPCHK(ctx, hipMemcpy(evc.data(), ctx->b_evcount.p, sizeof(int) * nitems, hipMemcpyDeviceToHost));
-      HIPCHK(ctx, hipMemcpy(fc.data(), ctx->b_famcount.p, sizeof(int) * nitems, hipMemcpyDeviceToHost));
-      HIPCHK(ctx, hipMemcpy(uq.data(), ctx->b_uniq.p, sizeof(int) * nitems, hipMemcpyDeviceToHost));
-      for (int i = 0; i < nitems; i++) {
-        const ItemMeta &im = ch.items[i];
-        const SetMeta &sm = sets[im.set];
-        Perm0 &q = (*p0)[im.set];
-        q.U = uq[i];
-        q.lcnt.resize(2 * sm.G);
-        q.lfull.resize((size_t)2 * sm.G * EDDY_MAX_MP);
-        q.rep.resize(S);
-        q.ev.resize(evc[i]);
-        q.lval.resize(2 * S);
-        q.info.resize(sm.c);
-        HIPCHK(ctx, hipMemcpy(q.lcnt.data(), (uint8_t *)ctx->b_lcnt.p + im.goff * 2, 2 * sm.G, hipMemcpyDeviceToHost));
-        HIPCHK(ctx, hipMemcpy(q.lfull.data(), (uint16_t *)ctx->b_lfull.p + im.goff * 2 * EDDY_MAX_MP,
-                              sizeof(uint16_t) * 2 * sm.G * EDDY_MAX_MP, hipMemcpyDeviceToHost));
-        HIPCHK(ctx, hipMemcpy(q.rep.data(), (int *)ctx->b_rep.p + im.soff, sizeof(int) * S, hipMemcpyDeviceToHost));
-        if (evc[i])
-          HIPCHK(ctx, hipMemcpy(q.ev.data(), (Event *)ctx->b_ev.p + im.evoff, sizeof(Event) * evc[i],
-                                hipMemcpyDeviceToHost));
-        HIPCHK(ctx, hipMemcpy(q.lval.data(), (double *)ctx->b_lval.p + im.soff * 2, sizeof(double) * 2 * S,
-                              hipMemcpyDeviceToHost));
-        if (sm.c)
-          HIPCHK(ctx, hipMemcpy(q.info.data(), (uint32_t *)ctx->b_info.p + im.infooff, sizeof(uint32_t) * sm.c,
-                                hipMemcpyDeviceToHost));
-        q.famkey.resize(fc[i]);
-        q.famscore.resize(2 * (size_t)fc[i]);
-        if (fc[i]) {
-          HIPCHK(ctx, hipMemcpy(q.famkey.data(), (uint64_t *)ctx->b_famkey.p + im.foff, sizeof(uint64_t) * fc[i],
-                                hipMemcpyDeviceToHost));
-          HIPCHK(ctx, hipMemcpy(q.famscore.data(), (double *)ctx->b_famscore.p + im.foff * 2,
-                                sizeof(double) * 2 * fc[i], hipMemcpyDeviceToHost));
-        }
+    std::vector<int> subcap(todo.size());
+    for (size_t k = 0; k < todo.size(); k++) subcap[k] = cap[todo[k]];
+    std::vector<int> status, evc;
+    std::vector<long long> tests;
+    CHK(launch_chunk(ctx, sets, *cur, p, perms, subcap, attempt, status, evc, tests));
+    std::vector<int> next, done_idx;
+    for (size_t k = 0; k < todo.size(); k++) {
+      const int i = todo[k];
+      if (status[k] & ST_NOT_EVALUATED) {
+        ctx->err = "internal: a row read a leave-one-out test that was not evaluated";
+        return EDDY_ERR_STATE;
       }
+      if (status[k] & (ST_EV_OVERFLOW | ST_FAM_OVERFLOW | ST_COLLISION)) {
+        if (status[k] & ST_EV_OVERFLOW) cap[i] = evc[k] + evc[k] / 8 + 64;
+        next.push_back(i);
+        continue;
+      }
+      tests_out[i] = tests[k];
+      ctx->timing.tests += tests[k];
+      ctx->timing.items += 1;
+      ctx->timing.pairs += sets[ch.items[i].set].c;
+      if (capture) done_idx.push_back((int)k);
     }
-    return EDDY_OK;
+    if (capture && !done_idx.empty()) CHK(capture_items(ctx, sets, *cur, done_idx, evc, p0));
+    todo.swap(next);
   }
-  ctx->err = "event/family capacity or hash collisions persisted after retries";
-  return EDDY_ERR_CAPACITY;
+  if (!todo.empty()) {
+    ctx->err = "event capacity or hash collisions persisted after retries";
+    return EDDY_ERR_CAPACITY;
+  }
+  return EDDY_OK;
 }
 
 void add_item(Chunk &ch, const std::vector<SetMeta> &sets, int set, int perm, const std::vector<uint32_t> &labbits,
@@ -548,7 +653,7 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
   std::vector<int> rows;
   std::vector<long long> rowoff;
   std::vector<uint32_t> prior;
-  long long xoff = 0;
+  long long xoff = 0, pairbase = 0, genebase = 0;
   for (int s = 0; s < nsets; s++) {
     const eddy_set_desc &d = sets[s];
     if (d.n_genes < 0 || d.n_genes > EDDY_MAX_G || (d.n_genes && !d.genes) || d.n_prior < 0 ||
@@ -561,6 +666,10 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
     m.c = d.n_genes * (d.n_genes - 1) / 2;
     m.xoff = xoff;
     m.poff = (long long)prior.size();
+    m.pairbase = pairbase;
+    m.genebase = genebase;
+    pairbase += m.c;
+    genebase += m.G;
     edge_bounds(p, m.c, m.eb);
     for (int i = 0; i < d.n_genes; i++) {
       if (d.genes[i] < 0 || d.genes[i] >= ctx->genes) {
@@ -616,14 +725,15 @@ extern "C" int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
   CHK(check_params(ctx, p));
   hipSetDevice(ctx->device);
   ctx->timing = eddy_timing{};
+  ctx->hp = HostProf();
   auto t0 = std::chrono::steady_clock::now();
   const int N = ctx->N, perms = p->perms, S = N + 2;
   std::vector<SetMeta> meta;
   CHK(prepare_sets(ctx, sets, nsets, p, meta));
   GROW(b_js, sizeof(double) * (size_t)std::max(1, nsets) * perms);
   HIPCHK(ctx, hipMemsetAsync(ctx->b_js.p, 0, sizeof(double) * (size_t)std::max(1, nsets) * perms, ctx->stream));
-  GROW(b_counters, sizeof(unsigned long long) * 8);
-  HIPCHK(ctx, hipMemsetAsync(ctx->b_counters.p, 0, sizeof(unsigned long long) * 8, ctx->stream));
+  GROW(b_counters, sizeof(unsigned long long) * 32);
+  HIPCHK(ctx, hipMemsetAsync(ctx->b_counters.p, 0, sizeof(unsigned long long) * 32, ctx->stream));
   ctx->last_perms = perms;
   ctx->last_G.assign(nsets, 0);
   ctx->last_js.assign(nsets, std::vector<double>());
@@ -645,6 +755,30 @@ extern "C" int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
   };
 
   std::vector<int> order;
+  long long sumc = 0, sumg = 0;
+  for (const SetMeta &m : meta) {
+    sumc += m.c;
+    sumg += m.G;
+  }
+  const bool learned = batched && p->learned == 1 && p->schedule == EDDY_TESTS_CUTOFF;
+  const bool collect = batched && p->learned == 2;  // training-data collection (eddy_learn_collect)
+  ctx->k_feat = nullptr;
+  ctx->k_labhits = nullptr;
+  ctx->k_win = nullptr;
+  if (learned && !ctx->mlp_ready) {
+    ctx->err = "learned schedule needs a trained network (eddy_mlp_train or eddy_mlp_set_params)";
+    return EDDY_ERR_STATE;
+  }
+  if (learned || collect) {
+    GROW(b_feat, sizeof(float) * 12 * std::max(1LL, sumc));
+    ctx->k_feat = (float *)ctx->b_feat.p;
+  }
+  if (collect) {
+    GROW(b_labhits, sizeof(unsigned) * std::max(1LL, sumc));
+    HIPCHK(ctx, hipMemsetAsync(ctx->b_labhits.p, 0, sizeof(unsigned) * std::max(1LL, sumc), ctx->stream));
+    ctx->k_labhits = (unsigned *)ctx->b_labhits.p;
+  }
+  auto tp1 = std::chrono::steady_clock::now();
   if (batched) {
     // phase 1: permutation 0 (true labels) of every set
     Chunk ch;
@@ -658,12 +792,30 @@ extern "C" int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
       if (ch.pairs > pair_budget) CHK(flush(ch, true));
     }
     CHK(flush(ch, true));
-    std::vector<double> js0(nsets);
-    for (int s = 0; s < nsets; s++) {
-      if (nan0[s]) continue;
-      double v;
-      HIPCHK(ctx, hipMemcpy(&v, (double *)ctx->b_js.p + (size_t)s * perms, sizeof(double), hipMemcpyDeviceToHost));
-      if (std::isnan(v)) nan0[s] = 1;
+    ctx->timing.phase1_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp1).count();
+    if (learned && sumc > 0) {
+      // score every pair once per set from its permutation-0 features, then
+      // the per-row windows the count kernel uses for permutations > 0
+      GROW(b_score, sizeof(float) * sumc);
+      GROW(b_win, sizeof(int) * std::max(1LL, sumg));
+      HIPCHK(ctx, mlp_forward((const float *)ctx->b_feat.p, (int)sumc, (const float *)ctx->b_mlp.p,
+                              (float *)ctx->b_score.p, ctx->stream));
+      k_window<<<nsets, 256, 0, ctx->stream>>>((const SetMeta *)ctx->b_sets.p, nsets, (const float *)ctx->b_score.p,
+                                              p->mp, ctx->mlp_kappa, (int *)ctx->b_win.p);
+      HIPCHK(ctx, hipGetLastError());
+      ctx->k_win = (const int *)ctx->b_win.p;
+    }
+    ctx->k_feat = nullptr;
+    {
+      auto hj = std::chrono::steady_clock::now();
+      // JS of permutation 0 of every set: one strided copy
+      std::vector<double> js0(std::max(1, nsets));
+      if (nsets)
+        HIPCHK(ctx, hipMemcpy2D(js0.data(), sizeof(double), ctx->b_js.p, sizeof(double) * perms, sizeof(double), nsets,
+                                hipMemcpyDeviceToHost));
+      for (int s = 0; s < nsets; s++)
+        if (!nan0[s] && std::isnan(js0[s])) nan0[s] = 1;
+      ctx->hp.js0 += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hj).count();
     }
     // phase 2: a producer thread replays the stream in the reference's order
     // (sets in order; a NaN-at-permutation-0 set consumes one permutation's
@@ -695,7 +847,10 @@ extern "C" int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
     const int words = (N + 31) / 32;
     std::vector<uint32_t> one(words);
     for (int s = 0; s < nsets && rc2 == EDDY_OK; s++) {
+      auto hw = std::chrono::steady_clock::now();
       while (ready.load(std::memory_order_acquire) <= s) std::this_thread::yield();
+      auto hb = std::chrono::steady_clock::now();
+      ctx->hp.wait += std::chrono::duration<double, std::milli>(hb - hw).count();
       if (nan0[s]) continue;
       ran[s] = perms;
       for (int pm = 1; pm < perms; pm++) {
@@ -703,6 +858,7 @@ extern "C" int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
         add_item(ch, meta, s, pm, one, N);
       }
       std::vector<uint32_t>().swap(setlab[s]);
+      ctx->hp.build += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hb).count();
       if (ch.pairs > pair_budget) rc2 = flush(ch, false);
     }
     producer.join();
@@ -734,6 +890,7 @@ extern "C" int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
       }
     }
   }
+  auto htail = std::chrono::steady_clock::now();
   // p-values on the GPU
   GROW(b_nan0, sizeof(int) * std::max(1, nsets));
   GROW(b_pout, sizeof(double) * 6 * std::max(1, nsets));
@@ -779,10 +936,25 @@ extern "C" int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
     ctx->last_js[s].assign(alljs.begin() + (size_t)s * perms, alljs.begin() + (size_t)(s + 1) * perms);
     if (nan0[s]) ctx->last_js[s][0] = -999.0;
   }
-  unsigned long long cnt[8];
+  ctx->k_win = nullptr;
+  ctx->k_labhits = nullptr;
+  ctx->k_feat = nullptr;
+  unsigned long long cnt[32];
   HIPCHK(ctx, hipMemcpy(cnt, ctx->b_counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
   for (int k = 0; k < 4; k++) ctx->timing.sched[k] = (long long)cnt[k];
+  for (int k = 0; k < 4; k++) ctx->timing.stats[k] = (long long)cnt[4 + k];
+  int wcr = 0;
+  if (hipDeviceGetAttribute(&wcr, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess || wcr <= 0) wcr = 100000;
+  for (int k = 0; k < 10; k++) ctx->timing.phase_ms[k] = (double)cnt[16 + k] / (double)wcr;  // kHz -> ms
   ctx->timing.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  ctx->hp.tail = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - htail).count();
+  if (getenv("EDDY_PROFILE"))
+    fprintf(stderr,
+            "[eddy] run_sets %d sets: total %.1f ms, phase1 %.1f | launch %.1f sync %.1f capture %.1f wait %.1f "
+            "build %.1f js0 %.1f tail %.1f | kernels %.1f %.1f %.1f %.1f\n",
+            nsets, ctx->timing.total_ms, ctx->timing.phase1_ms, ctx->hp.launch, ctx->hp.sync, ctx->hp.capture,
+            ctx->hp.wait, ctx->hp.build, ctx->hp.js0, ctx->hp.tail, ctx->timing.kernel_ms[0],
+            ctx->timing.kernel_ms[1], ctx->timing.kernel_ms[2], ctx->timing.kernel_ms[3]);
   return EDDY_OK;
 }
 
@@ -904,8 +1076,8 @@ extern "C" int eddy_run_perm_detail(eddy_ctx *ctx, const eddy_set_desc *set, con
   CHK(prepare_sets(ctx, set, 1, p, meta));
   GROW(b_js, sizeof(double) * p->perms);
   HIPCHK(ctx, hipMemsetAsync(ctx->b_js.p, 0, sizeof(double) * p->perms, ctx->stream));
-  GROW(b_counters, sizeof(unsigned long long) * 8);
-  HIPCHK(ctx, hipMemsetAsync(ctx->b_counters.p, 0, sizeof(unsigned long long) * 8, ctx->stream));
+  GROW(b_counters, sizeof(unsigned long long) * 32);
+  HIPCHK(ctx, hipMemsetAsync(ctx->b_counters.p, 0, sizeof(unsigned long long) * 32, ctx->stream));
   ctx->last_p0.assign(1, Perm0());
   ctx->last_G.assign(1, meta[0].G);
   ctx->last_perms = p->perms;
@@ -1009,4 +1181,290 @@ extern "C" int eddy_edge_decision_host(const int *tally9, int c, int prior, cons
   eddy_edge_bounds eb;
   edge_bounds(p, c, eb);
   return eddy_edge_decision(tally9, prior, &ep, &eb, (double)c);
+}
+
+// ---------------------------------------------------------------------------
+// learned proposal: the network and its training (mlp.hip)
+// ---------------------------------------------------------------------------
+static int mlp_alloc(eddy_ctx *ctx) {
+  GROW(b_mlp, sizeof(float) * MLP_NPARAMS);
+  GROW(b_mlpm, sizeof(float) * MLP_NPARAMS);
+  GROW(b_mlpv, sizeof(float) * MLP_NPARAMS);
+  GROW(b_mlpg, sizeof(float) * MLP_NPARAMS);
+  GROW(b_mlploss, sizeof(float) * 4);
+  return EDDY_OK;
+}
+
+extern "C" int eddy_mlp_init(eddy_ctx *ctx, uint64_t seed) {
+  if (!ctx || ctx->device < 0) return ctx ? EDDY_ERR_NODEV : EDDY_ERR_ARG;
+  hipSetDevice(ctx->device);
+  CHK(mlp_alloc(ctx));
+  // He initialisation (normal, variance 2/fan_in), zero biases; Box-Muller on
+  // a seeded mt19937_64 so the initial network is reproducible
+  std::mt19937_64 rng(seed);
+  auto u01 = [&]() { return (double)(rng() >> 11) * 0x1p-53; };
+  auto normal = [&]() { return std::sqrt(-2.0 * std::log(1.0 - u01())) * std::cos(6.283185307179586 * u01()); };
+  std::vector<float> P(MLP_NPARAMS, 0.f);
+  for (int i = 0; i < MLP_H * MLP_IN; i++) P[MLP_OFF_W1 + i] = (float)(normal() * std::sqrt(2.0 / MLP_IN));
+  for (int i = 0; i < MLP_H * MLP_H; i++) P[MLP_OFF_W2 + i] = (float)(normal() * std::sqrt(2.0 / MLP_H));
+  for (int i = 0; i < MLP_H; i++) P[MLP_OFF_W3 + i] = (float)(normal() * std::sqrt(1.0 / MLP_H));
+  HIPCHK(ctx, hipMemcpy(ctx->b_mlp.p, P.data(), sizeof(float) * MLP_NPARAMS, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemset(ctx->b_mlpm.p, 0, sizeof(float) * MLP_NPARAMS));
+  HIPCHK(ctx, hipMemset(ctx->b_mlpv.p, 0, sizeof(float) * MLP_NPARAMS));
+  ctx->mlp_step = 0;
+  ctx->mlp_ready = true;
+  return EDDY_OK;
+}
+
+extern "C" int eddy_mlp_get_params(eddy_ctx *ctx, float *out, int n) {
+  if (!ctx || !out || n < MLP_NPARAMS) return EDDY_ERR_ARG;
+  if (!ctx->mlp_ready) return EDDY_ERR_STATE;
+  HIPCHK(ctx, hipMemcpy(out, ctx->b_mlp.p, sizeof(float) * MLP_NPARAMS, hipMemcpyDeviceToHost));
+  return EDDY_OK;
+}
+
+extern "C" int eddy_mlp_set_params(eddy_ctx *ctx, const float *in, int n) {
+  if (!ctx || !in || n < MLP_NPARAMS) return EDDY_ERR_ARG;
+  if (ctx->device < 0) return EDDY_ERR_NODEV;
+  hipSetDevice(ctx->device);
+  CHK(mlp_alloc(ctx));
+  HIPCHK(ctx, hipMemcpy(ctx->b_mlp.p, in, sizeof(float) * MLP_NPARAMS, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemset(ctx->b_mlpm.p, 0, sizeof(float) * MLP_NPARAMS));
+  HIPCHK(ctx, hipMemset(ctx->b_mlpv.p, 0, sizeof(float) * MLP_NPARAMS));
+  ctx->mlp_step = 0;
+  ctx->mlp_ready = true;
+  return EDDY_OK;
+}
+
+static int mlp_upload(eddy_ctx *ctx, const float *X, const float *Y, long long n) {
+  GROW(b_mx, sizeof(float) * MLP_IN * std::max(1LL, n));
+  HIPCHK(ctx, hipMemcpy(ctx->b_mx.p, X, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice));
+  if (Y) {
+    GROW(b_my, sizeof(float) * std::max(1LL, n));
+    HIPCHK(ctx, hipMemcpy(ctx->b_my.p, Y, sizeof(float) * n, hipMemcpyHostToDevice));
+  }
+  return EDDY_OK;
+}
+
+extern "C" int eddy_mlp_forward(eddy_ctx *ctx, const float *X, int n, float *prob) {
+  if (!ctx || n < 0 || (n > 0 && (!X || !prob))) return EDDY_ERR_ARG;
+  if (!ctx->mlp_ready) return EDDY_ERR_STATE;
+  if (n == 0) return EDDY_OK;
+  hipSetDevice(ctx->device);
+  CHK(mlp_upload(ctx, X, nullptr, n));
+  GROW(b_score, sizeof(float) * n);
+  HIPCHK(ctx, mlp_forward((const float *)ctx->b_mx.p, n, (const float *)ctx->b_mlp.p, (float *)ctx->b_score.p,
+                          ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpy(prob, ctx->b_score.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+  return EDDY_OK;
+}
+
+// gradient and loss over device rows (idx[b0 .. b0 + n) or rows b0 ..)
+static int mlp_grad_dev(eddy_ctx *ctx, const int *dIdx, int b0, int n) {
+  GROW(b_mscr, sizeof(float) * mlp_part_floats(n));
+  HIPCHK(ctx, mlp_gradient((const float *)ctx->b_mx.p, (const float *)ctx->b_my.p, dIdx, b0, n,
+                           (const float *)ctx->b_mlp.p, (float *)ctx->b_mscr.p, (float *)ctx->b_mlpg.p,
+                           (float *)ctx->b_mlploss.p, ctx->stream));
+  return EDDY_OK;
+}
+
+extern "C" int eddy_mlp_gradient(eddy_ctx *ctx, const float *X, const float *Y, int n, float *grad, float *loss) {
+  if (!ctx || !X || !Y || !grad || n <= 0) return EDDY_ERR_ARG;
+  if (!ctx->mlp_ready) return EDDY_ERR_STATE;
+  hipSetDevice(ctx->device);
+  CHK(mlp_upload(ctx, X, Y, n));
+  CHK(mlp_grad_dev(ctx, nullptr, 0, n));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpy(grad, ctx->b_mlpg.p, sizeof(float) * MLP_NPARAMS, hipMemcpyDeviceToHost));
+  if (loss) HIPCHK(ctx, hipMemcpy(loss, ctx->b_mlploss.p, sizeof(float), hipMemcpyDeviceToHost));
+  return EDDY_OK;
+}
+
+// mean loss over (up to) the first 65536 training rows with the current network
+static int mlp_eval_loss(eddy_ctx *ctx, long long n, float *out) {
+  CHK(mlp_grad_dev(ctx, nullptr, 0, (int)std::min<long long>(n, 65536)));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpy(out, ctx->b_mlploss.p, sizeof(float), hipMemcpyDeviceToHost));
+  return EDDY_OK;
+}
+
+// Adam over shuffled mini-batches of the n rows already in b_mx / b_my; the
+// last partial batch of each epoch is dropped (n >= batch)
+static int mlp_train_dev(eddy_ctx *ctx, long long n, int epochs, int batch, float lr, uint64_t seed) {
+  batch = (int)std::min<long long>(batch, n);
+  GROW(b_mscr, sizeof(float) * mlp_part_floats(batch));
+  GROW(b_midx, sizeof(int) * n);
+  std::mt19937_64 rng(seed ^ 0xADA3ULL);
+  // two host copies of the order: epoch e shuffles into h[e & 1] while the
+  // GPU still runs epoch e - 1 (whose upload read h[(e - 1) & 1])
+  std::vector<int> h[2];
+  h[1].resize(n);
+  for (long long i = 0; i < n; i++) h[1][i] = (int)i;
+  hipStream_t st = ctx->stream;
+  for (int ep = 0; ep < epochs; ep++) {
+    std::vector<int> &idx = h[ep & 1];
+    idx = h[(ep + 1) & 1];
+    for (long long i = n - 1; i > 0; i--) std::swap(idx[i], idx[rng() % (uint64_t)(i + 1)]);
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->b_midx.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    for (long long b0 = 0; b0 + batch <= n; b0 += batch) {
+      ctx->mlp_step++;
+      HIPCHK(ctx, mlp_train_step((const float *)ctx->b_mx.p, (const float *)ctx->b_my.p, (const int *)ctx->b_midx.p,
+                                 (int)b0, batch, (float *)ctx->b_mlp.p, (float *)ctx->b_mscr.p,
+                                 (float *)ctx->b_mlpg.p, (float *)ctx->b_mlploss.p, (float *)ctx->b_mlpm.p,
+                                 (float *)ctx->b_mlpv.p, lr, ctx->mlp_step, st));
+    }
+  }
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  return EDDY_OK;
+}
+
+extern "C" int eddy_mlp_train(eddy_ctx *ctx, const float *X, const float *Y, long long n, int epochs, int batch,
+                              float lr, uint64_t seed, float *loss_out) {
+  if (!ctx || !X || !Y || n <= 0 || n > INT32_MAX || epochs < 0 || batch <= 0 || !(lr > 0)) return EDDY_ERR_ARG;
+  if (ctx->device < 0) return EDDY_ERR_NODEV;
+  hipSetDevice(ctx->device);
+  if (!ctx->mlp_ready) CHK(eddy_mlp_init(ctx, seed));
+  CHK(mlp_upload(ctx, X, Y, n));
+  CHK(mlp_train_dev(ctx, n, epochs, batch, lr, seed));
+  if (loss_out) CHK(mlp_eval_loss(ctx, n, loss_out));
+  return EDDY_OK;
+}
+
+// Labels of the sets of the last logged run: fraction of (permutation > 0,
+// class) full-test edges per pair, written to b_my; the features move to b_mx.
+static int mlp_take_logged(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_set_result *res,
+                           long long *n_out) {
+  std::vector<long long> base(std::max(1, nsets));
+  std::vector<float> inv(std::max(1, nsets));
+  long long n = 0;
+  for (int s = 0; s < nsets; s++) {
+    base[s] = n;
+    n += (long long)sets[s].n_genes * (sets[s].n_genes - 1) / 2;
+    inv[s] = res[s].perms_run > 1 ? (float)(1.0 / (2.0 * (res[s].perms_run - 1))) : 0.f;
+  }
+  *n_out = n;
+  if (n == 0) return EDDY_OK;
+  std::swap(ctx->b_mx, ctx->b_feat);
+  GROW(b_my, sizeof(float) * n);
+  GROW(b_mlbase, sizeof(long long) * nsets);
+  GROW(b_mlinv, sizeof(float) * nsets);
+  HIPCHK(ctx, hipMemcpyAsync(ctx->b_mlbase.p, base.data(), sizeof(long long) * nsets, hipMemcpyHostToDevice,
+                             ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->b_mlinv.p, inv.data(), sizeof(float) * nsets, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, mlp_labels((const unsigned *)ctx->b_labhits.p, (const float *)ctx->b_mlinv.p,
+                         (const long long *)ctx->b_mlbase.p, nsets, n, (float *)ctx->b_my.p, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return EDDY_OK;
+}
+
+// Training data from the method without the network (every full test, the
+// leave-one-out tests up to each row's exact cutoff): each pair gets its
+// permutation-0 features and, as label, the fraction of (permutation > 0,
+// class) full-test edges.
+extern "C" int eddy_learn_collect(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *params,
+                                  float *X, float *Y, long long cap, long long *n_out) {
+  if (!ctx || !params || !n_out || nsets < 0 || (cap > 0 && (!X || !Y))) return EDDY_ERR_ARG;
+  eddy_params q = *params;
+  q.mode = EDDY_MODE_BATCHED;
+  q.schedule = EDDY_TESTS_CUTOFF;
+  q.learned = 2;
+  std::vector<eddy_set_result> res(std::max(1, nsets));
+  CHK(eddy_run_sets(ctx, sets, nsets, &q, res.data()));
+  long long n = 0;
+  CHK(mlp_take_logged(ctx, sets, nsets, res.data(), &n));
+  *n_out = n;
+  if (n > cap) return EDDY_ERR_CAPACITY;
+  if (n == 0) return EDDY_OK;
+  HIPCHK(ctx, hipMemcpy(X, ctx->b_mx.p, sizeof(float) * MLP_IN * n, hipMemcpyDeviceToHost));
+  HIPCHK(ctx, hipMemcpy(Y, ctx->b_my.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+  return EDDY_OK;
+}
+
+static void timing_add(eddy_timing &a, const eddy_timing &b) {
+  for (int k = 0; k < EDDY_NKERNELS; k++) {
+    a.kernel_ms[k] += b.kernel_ms[k];
+    a.launches[k] += b.launches[k];
+  }
+  a.items += b.items;
+  a.pairs += b.pairs;
+  a.tests += b.tests;
+  a.host_stream_ms += b.host_stream_ms;
+  for (int k = 0; k < 4; k++) {
+    a.sched[k] += b.sched[k];
+    a.stats[k] += b.stats[k];
+  }
+  a.phase1_ms += b.phase1_ms;
+  for (int k = 0; k < 10; k++) a.phase_ms[k] += b.phase_ms[k];
+  a.total_ms += b.total_ms;
+}
+
+// The learned job: the first cfg->train_sets sets run without the network
+// (every full test) with logging, the network is trained on the GPU from those logs, and
+// the remaining sets run the learned schedule.  Results, networks and the
+// permutation stream are exactly those of eddy_run_sets over all sets.
+extern "C" int eddy_run_learned(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *p,
+                                const eddy_learn_cfg *cfg, eddy_set_result *results, eddy_learn_report *rep) {
+  if (!ctx || !p || !cfg || nsets < 0 || (nsets && (!sets || !results))) return EDDY_ERR_ARG;
+  if (cfg->train_sets < 0 || cfg->epochs < 0 || cfg->batch <= 0 || !(cfg->lr > 0) || !(cfg->margin >= 0))
+    return EDDY_ERR_ARG;
+  if (ctx->device < 0) return EDDY_ERR_NODEV;
+  eddy_learn_report r{};
+  const int k = std::min(cfg->train_sets, nsets);
+  auto t0 = std::chrono::steady_clock::now();
+  eddy_params q = *p;
+  q.mode = EDDY_MODE_BATCHED;
+  q.schedule = EDDY_TESTS_CUTOFF;
+  q.learned = 2;
+  CHK(eddy_run_sets(ctx, sets, k, &q, results));
+  eddy_timing tcol = ctx->timing;
+  std::vector<int> G0 = std::move(ctx->last_G);
+  std::vector<std::vector<double>> js0 = std::move(ctx->last_js);
+  std::vector<Perm0> p00 = std::move(ctx->last_p0);
+  long long n = 0;
+  CHK(mlp_take_logged(ctx, sets, k, results, &n));
+  for (int s = 0; s < k; s++) r.tests_collect += results[s].tests_eval;
+  auto t1 = std::chrono::steady_clock::now();
+  r.train_rows = n;
+  const bool trained = n >= 64;
+  if (trained) {
+    CHK(eddy_mlp_init(ctx, cfg->seed));
+    CHK(mlp_eval_loss(ctx, n, &r.loss_initial));
+    CHK(mlp_train_dev(ctx, n, cfg->epochs, cfg->batch, cfg->lr, cfg->seed));
+    CHK(mlp_eval_loss(ctx, n, &r.loss_final));
+  }
+  auto t2 = std::chrono::steady_clock::now();
+  eddy_params a = *p;
+  a.mode = EDDY_MODE_BATCHED;
+  a.schedule = EDDY_TESTS_CUTOFF;
+  a.learned = trained ? 1 : 0;
+  const float kappa = ctx->mlp_kappa;
+  ctx->mlp_kappa = cfg->margin;
+  int rc = eddy_run_sets(ctx, sets + k, nsets - k, &a, results + k);
+  ctx->mlp_kappa = kappa;
+  CHK(rc);
+  for (int s = k; s < nsets; s++) r.tests_apply += results[s].tests_eval;
+  auto t3 = std::chrono::steady_clock::now();
+  // one record over all sets, in set order
+  G0.insert(G0.end(), ctx->last_G.begin(), ctx->last_G.end());
+  for (auto &v : ctx->last_js) js0.push_back(std::move(v));
+  for (auto &v : ctx->last_p0) p00.push_back(std::move(v));
+  ctx->last_G = std::move(G0);
+  ctx->last_js = std::move(js0);
+  ctx->last_p0 = std::move(p00);
+  timing_add(ctx->timing, tcol);
+  ctx->timing.total_ms = std::chrono::duration<double, std::milli>(t3 - t0).count();
+  r.collect_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  r.train_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  r.apply_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
+  r.trained = trained ? 1 : 0;
+  if (rep) *rep = r;
+  return EDDY_OK;
+}
+
+extern "C" int eddy_mlp_set_margin(eddy_ctx *ctx, float kappa) {
+  if (!ctx || !(kappa >= 0)) return EDDY_ERR_ARG;
+  ctx->mlp_kappa = kappa;
+  return EDDY_OK;
 }

@@ -20,6 +20,7 @@
 #define INFO_FULL(g) (1u << (g))
 #define INFO_LOO_SHIFT(g) (2 + 9 * (g))
 #define INFO_EVAL(g) (1u << (20 + (g)))
+#define INFO_FEVAL(g) (1u << (22 + (g)))  // the full test of class g was evaluated
 
 struct SetMeta {
   int G;            // genes in the set
@@ -27,6 +28,8 @@ struct SetMeta {
   long long xoff;   // byte offset of the set's rows [G x N] in the gathered data
   long long poff;   // u32-word offset of the set's prior bits (one bit per pair)
   eddy_edge_bounds eb;  // x intervals where the edge decision is certain (c-dependent)
+  long long pairbase;   // sum of c over the run's earlier sets (per-set pair arrays)
+  long long genebase;   // sum of G over the run's earlier sets (per-set row arrays)
 };
 
 struct ItemMeta {   // one (gene set, permutation)
@@ -37,7 +40,11 @@ struct ItemMeta {   // one (gene set, permutation)
   long long evoff;    // event offset (capacity ev_cap per item)
   long long soff;     // network-indexed scratch offset (S per item)
   long long goff;     // gene-indexed scratch offset (G per item)
-  long long foff;     // family scratch offset (fam_cap per item)
+  long long foff;     // family scratch offset (fcap per item)
+  int evcap;          // event capacity of this item
+  int fcap;           // family capacity of this item
+  long long fhoff;    // family hash table offset (fhsize entries)
+  int fhsize;         // power of two >= 2 * fcap
 };
 
 struct Event {        // a row of one leave-one-out network that differs from its class's full network
@@ -49,7 +56,7 @@ struct KArgs {
   // dataset-level
   int N, n1, n2, S;
   int mp, p_adjust, plain;
-  int ev_cap, fam_cap, ht_size;
+  int ht_size;             // network hash table entries (power of two >= 2S)
   eddy_edge_params ep;     // thresholds, lgamma(p+1) constants, underflow-cut interval
   const double *lgtab;     // [82][lgstride]: lgamma(n + 4/d)
   int lgstride;
@@ -78,6 +85,8 @@ struct KArgs {
   int *evorder;            // [evoff + e]
   int *evfam;              // [evoff + e]
   uint64_t *famkey;        // [foff + f]
+  unsigned long long *famht;  // [fhoff + i] family hash table keys
+  int *famhv;              // [fhoff + i] family hash table values
   double *famscore;        // [(foff + f)*2 + g]
   double *lval;            // [soff*2 ...] 2 per network slot
   int *uniq_u;             // [item]
@@ -86,6 +95,9 @@ struct KArgs {
   int perms;
   int *status;             // [item] error bits
   const uint8_t *plan_extra;  // [goff*2 + g*G + t] extra full edges in the plan (learned), or null
+  float *feat;             // [(pairbase + p) * 12] learned-proposal features of permutation 0, or null
+  unsigned *labhits;       // [pairbase + p] full-test edges over permutations > 0 (training labels), or null
+  const int *win;          // [genebase + t] last column whose full test the count kernel runs, or null
   unsigned long long *counters;  // schedule statistics (optional)
   long long *tests;        // [item] evaluated tests
 };

@@ -70,12 +70,71 @@ __device__ __forceinline__ uint32_t eq_bytes(uint32_t x, uint32_t v) {
   return z >> 7;
 }
 
+// per-phase workgroup time (constant-rate wall clock), summed over items into
+// counters[16 + phase] when the engine collects statistics
+__device__ __forceinline__ void phase_mark(const KArgs &a, int phase, unsigned long long &t0) {
+  if (!a.counters || threadIdx.x != 0) return;
+  unsigned long long t = wall_clock64();
+  if (phase >= 0) atomicAdd(&a.counters[16 + phase], t - t0);
+  t0 = t;
+}
+
 // LDS ordering among the lanes of one wave (a wave executes its LDS accesses
 // in order; this keeps the compiler from moving them across the point)
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Learned-proposal features of a pair from its pooled table (both classes of
+// permutation 0 = all samples; it does not change under permutation):
+// Pearson chi2/n, log1p chi2, mutual information, the two marginal entropies,
+// fraction of empty cells, smallest expected count/(n/9), n/1000, dof/4,
+// prior flag, log(c)/12, Cramer's V
+__device__ void pair_features(const int *t1, const int *t2, int prior, int c, float *out) {
+  float t[9], n = 0.f, r[3] = {0, 0, 0}, k[3] = {0, 0, 0};
+  for (int i = 0; i < 9; i++) {
+    t[i] = (float)(t1[i] + t2[i]);
+    n += t[i];
+    r[i / 3] += t[i];
+    k[i % 3] += t[i];
+  }
+  float chi = 0.f, mi = 0.f, hr = 0.f, hk = 0.f, zeros = 0.f, emin = 1e30f;
+  int nr = 0, nk = 0;
+  for (int i = 0; i < 3; i++) {
+    if (r[i] > 0) {
+      nr++;
+      hr -= (r[i] / n) * __logf(r[i] / n);
+    }
+    if (k[i] > 0) {
+      nk++;
+      hk -= (k[i] / n) * __logf(k[i] / n);
+    }
+  }
+  for (int i = 0; i < 9; i++) {
+    float e = r[i / 3] * k[i % 3] / n;
+    zeros += t[i] == 0.f;
+    if (e > 0.f) {
+      chi += (t[i] - e) * (t[i] - e) / e;
+      emin = fminf(emin, e);
+      if (t[i] > 0.f) mi += (t[i] / n) * __logf(t[i] / e);
+    }
+  }
+  if (n <= 0.f) n = 1.f;
+  int mn = min(nr, nk);
+  out[0] = chi / n;
+  out[1] = log1pf(chi);
+  out[2] = mi;
+  out[3] = hr;
+  out[4] = hk;
+  out[5] = zeros / 9.f;
+  out[6] = emin < 1e29f ? emin / (n / 9.f) : 0.f;
+  out[7] = n / 1000.f;
+  out[8] = (float)((nr - 1) * (nk - 1)) / 4.f;
+  out[9] = (float)prior;
+  out[10] = __logf((float)max(c, 1)) / 12.f;
+  out[11] = mn > 1 ? sqrtf(chi / (n * (float)(mn - 1))) : 0.f;
 }
 
 // --------------------------------------------------------------------------
@@ -192,10 +251,16 @@ __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
       }
     long long p = pair_index(gi, gj);
     int pr = prior_bit(a, sm, p);
-    int f1 = d_edge_test(t1, pr, a, sm, cpairs);
-    int f2 = d_edge_test(t2, pr, a, sm, cpairs);
-    ntests += 2;
-    uint32_t info = (uint32_t)f1 | ((uint32_t)f2 << 1);
+    if (a.feat && im.perm == 0) pair_features(t1, t2, pr, sm.c, a.feat + (sm.pairbase + p) * 12);
+    uint32_t info = 0;
+    int f1 = 0, f2 = 0;
+    if (!a.win || im.perm == 0 || gj <= a.win[sm.genebase + gi]) {
+      f1 = d_edge_test(t1, pr, a, sm, cpairs);
+      f2 = d_edge_test(t2, pr, a, sm, cpairs);
+      ntests += 2;
+      info = (uint32_t)f1 | ((uint32_t)f2 << 1) | INFO_FEVAL(0) | INFO_FEVAL(1);
+      if (a.labhits && im.perm > 0) atomicAdd(&a.labhits[sm.pairbase + p], (unsigned)(f1 + f2));
+    }
     if (a.plain) {
       info |= d_loo_mask(t1, f1, pr, a, sm, cpairs, &ntests) << INFO_LOO_SHIFT(0);
       info |= d_loo_mask(t2, f2, pr, a, sm, cpairs, &ntests) << INFO_LOO_SHIFT(1);
@@ -244,6 +309,21 @@ __device__ __forceinline__ uint32_t loo_of_pair(const KArgs &a, const ItemMeta &
   return d_loo_mask(tt, full, prior_bit(a, sm, p), a, sm, cpairs, ntests);
 }
 
+// full-test decision of (pair, class), evaluated here when the count kernel
+// skipped it (outside the learned window)
+__device__ __forceinline__ int full_of(const KArgs &a, const ItemMeta &im, const SetMeta &sm, long long p, int g,
+                                       uint32_t w, double cpairs, int *ntests) {
+  if (w & INFO_FEVAL(g)) return (w >> g) & 1;
+  const uint16_t *tb = a.tab + (im.infooff + p) * 18 + 9 * g;
+  int tt[9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) tt[i] = tb[i];
+  int f = d_edge_test(tt, prior_bit(a, sm, p), a, sm, cpairs);
+  (*ntests)++;
+  atomicOr(&a.info[im.infooff + p], ((uint32_t)f << g) | INFO_FEVAL(g));
+  return f;
+}
+
 __global__ __launch_bounds__(256) void k_loo(KArgs a) {
   const int item = blockIdx.x;
   const ItemMeta im = a.items[item];
@@ -257,14 +337,16 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
   __shared__ int tcount, nfb, nfbcols;
   if (tid == 0) tcount = nfb = nfbcols = 0;
   const int R = 2 * G;  // entries r = 2t + g
+  int ntests0 = 0;
   // ---- A: plan ----------------------------------------------------------
   for (int r = tid; r < R; r += 256) {
     const int t = r >> 1, g = r & 1;
     const int want = mp + (a.plan_extra ? a.plan_extra[(im.goff * 2 + g * G + t)] : 0);
     int hits = 0, P = t - 1;
     for (int col = 0; col < t; col++) {
-      uint32_t w = a.info[im.infooff + pair_index(t, col)];
-      if ((w >> g) & 1) {
+      const long long pc = pair_index(t, col);
+      uint32_t w = a.info[im.infooff + pc];
+      if (full_of(a, im, sm, pc, g, w, cpairs, &ntests0)) {
         if (++hits >= want) {
           P = col;
           break;
@@ -300,7 +382,7 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
   __syncthreads();
   // ---- B: packed leave-one-out tests of every planned (pair, class) ------
   const int total = off[R];
-  int ntests = 0;
+  int ntests = ntests0;
   for (int e = tid; e < total; e += 256) {
     int lo = 0, hi = R - 1;  // last r with off[r] <= e
     while (lo < hi) {
@@ -327,9 +409,10 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
     for (; col < t && solid < mp; col++) {
       const long long p = pair_index(t, col);
       uint32_t w = a.info[im.infooff + p];
-      uint32_t m = loo_of_pair(a, im, sm, p, g, (w >> g) & 1, cpairs, &ntests);
+      const int full = full_of(a, im, sm, p, g, w, cpairs, &ntests);
+      uint32_t m = loo_of_pair(a, im, sm, p, g, full, cpairs, &ntests);
       atomicOr(&a.info[im.infooff + p], (m << INFO_LOO_SHIFT(g)) | INFO_EVAL(g));
-      if (((w >> g) & 1) && m == 0x1FFu) solid++;
+      if (full && m == 0x1FFu) solid++;
       fbc++;
     }
   }
@@ -427,8 +510,11 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
   __shared__ unsigned long long hfull[2];
   __shared__ uint32_t flips[EDDY_NWAVES][EDDY_MAX_G];  // col | mask << 16
   __shared__ uint16_t lrow[EDDY_NWAVES][EDDY_MAX_MP];
+  __shared__ uint32_t rinfo[EDDY_NWAVES][EDDY_MAX_G];  // this row's info words up to its cutoff
   __shared__ uint8_t sfl[2][EDDY_MAX_G];
 
+  unsigned long long tph = 0;
+  phase_mark(a, -1, tph);
   build_class_lists(a.labels + im.laboff, N, cls[0], cls[1], ncls);
   for (int k = tid; k < ncls[0]; k += 256) a.clsg[im.soff + k] = cls[0][k];
   for (int k = tid; k < ncls[1]; k += 256) a.clsg[im.soff + a.n1 + k] = cls[1][k];
@@ -451,6 +537,7 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
   }
   __syncthreads();
 
+  phase_mark(a, 7, tph);
   uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
   for (int rg = wave; rg < 2 * G; rg += EDDY_NWAVES) {
     const int t = rg >> 1, g = rg & 1;
@@ -462,7 +549,7 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
       uint32_t w = valid ? a.info[im.infooff + pair_index(t, col)] : 0u;
       int full = (w >> g) & 1;
       uint32_t loo = (w >> INFO_LOO_SHIFT(g)) & 0x1FFu;
-      bool ev = (w & INFO_EVAL(g)) != 0;
+      bool ev = (w & INFO_EVAL(g)) != 0 && (w & INFO_FEVAL(g)) != 0;  // both tests of this cell ran
       bool solid = valid && full && ev && loo == 0x1FFu;
       uint64_t sm_ = __ballot(solid);
       int need = mp - nSolid;
@@ -475,6 +562,7 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
         done = true;
       }
       bool inRange = valid && lane <= limit;
+      if (inRange) rinfo[wave][col] = w;
       if (__any(inRange && !ev)) {
         if (lane == 0) atomicOr(&a.status[item], ST_NOT_EVALUATED);
       }
@@ -519,7 +607,7 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
       uint16_t nl[EDDY_MAX_MP];
       int cnt = 0, fi = 0;
       for (int col = 0; col <= C && cnt < mp; col++) {
-        uint32_t w = a.info[im.infooff + pair_index(t, col)];
+        uint32_t w = rinfo[wave][col];
         int bit = (w >> g) & 1;
         if (fi < nflip && (int)(flips[wave][fi] & 0xffff) == col) {
           uint32_t e = flips[wave][fi];
@@ -534,7 +622,7 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
       if (same) continue;
       int s = g == 0 ? 1 + k : a.n1 + 1 + k;
       int e = atomicAdd(&a.evcount[item], 1);
-      if (e < a.ev_cap) {
+      if (e < im.evcap) {
         Event ev;
         ev.s = (uint16_t)s;
         ev.t = (uint16_t)t;
@@ -554,6 +642,7 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
     int g = (s <= a.n1) ? 0 : 1;
     a.hnet[im.soff + s] = hfull[g] ^ delta[s];
   }
+  phase_mark(a, 8, tph);
 }
 
 // --------------------------------------------------------------------------
@@ -658,10 +747,14 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   __shared__ int wsum[257];
   __shared__ int U, F, flag;
   __shared__ int hist[EDDY_NWAVES][84];
+  __shared__ double lgv[EDDY_NWAVES][84];
+  __shared__ double lgj[EDDY_NWAVES][28];
 
-  const int E = min(a.evcount[item], a.ev_cap);
+  const int E = min(a.evcount[item], im.evcap);
   int *es = a.evorder + im.evoff;
   const Event *ev = a.ev + im.evoff;
+  unsigned long long tph = 0;
+  phase_mark(a, -1, tph);
 
   // ---- 1. first occurrence of each network hash -------------------------
   for (int i = tid; i < HT; i += 256) {
@@ -711,6 +804,7 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
     }
   }
   __syncthreads();
+  phase_mark(a, 0, tph);
   // ---- 2. exact confirmation of every merge -----------------------------
   for (int s = tid; s < S; s += 256) {
     int r = rep[s];
@@ -745,6 +839,7 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
     }
     return;
   }
+  phase_mark(a, 1, tph);
   // unique networks in network order: ulist[u] = s
   for (int s = tid; s < S; s += 256) work[s] = rep[s] == s ? 1 : 0;
   __syncthreads();
@@ -760,10 +855,15 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   __syncthreads();
   for (int s = tid; s < S; s += 256) a.rep[im.soff + s] = rep[s];
   const int UU = U;
-  // ---- 3. families of the unique networks --------------------------------
-  for (int i = tid; i < HT; i += 256) {
-    hkeys[i] = 0ULL;
-    hvals[i] = 0x7fffffff;
+  phase_mark(a, 2, tph);
+  // ---- 3. families of the unique networks (hash table in global memory) --
+  unsigned long long *fk = a.famht + im.fhoff;
+  int *fv = a.famhv + im.fhoff;
+  int FH = 1024;
+  while (FH < 2 * (2 * G + E) && FH < im.fhsize) FH *= 2;
+  for (int i = tid; i < FH; i += 256) {
+    fk[i] = 0ULL;
+    fv[i] = 0x7fffffff;
   }
   if (tid == 0) F = 0;
   __syncthreads();
@@ -778,36 +878,49 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
       if (rep[e->s] != e->s) continue;
       key = fam_key(e->t, e->n, e->par);
     }
-    if (ht_insert_min(hkeys, hvals, key, 0, HT) < 0) atomicOr(&flag, 4);
+    if (ht_insert_min(fk, fv, key, 0, FH) < 0) atomicOr(&flag, 4);
   }
   __syncthreads();
-  for (int i = tid; i < HT; i += 256) {
-    if (hkeys[i] != 0ULL) {
+  for (int i = tid; i < FH; i += 256) {
+    if (fk[i] != 0ULL) {
       int f = atomicAdd(&F, 1);
-      hvals[i] = f;
-      if (f < a.fam_cap) a.famkey[im.foff + f] = hkeys[i];
+      fv[i] = f;
+      if (f < im.fcap) a.famkey[im.foff + f] = fk[i];
     }
   }
   __syncthreads();
-  if (tid == 0) a.famcount[item] = min(F, a.fam_cap);
-  if (flag || F > a.fam_cap || 2 * F > HT) {
+  if (tid == 0) a.famcount[item] = min(F, im.fcap);
+  if (flag || F > im.fcap || 2 * F > FH) {
     if (tid == 0) atomicOr(&a.status[item], ST_FAM_OVERFLOW);
     return;
   }
   for (int i = tid; i < 2 * G; i += 256) {
     int g = i / G, t = i - g * G;
     size_t lb = (size_t)(im.goff * 2 + g * G + t);
-    famfull[i] = hvals[ht_find(hkeys, fam_key(t, a.lcnt[lb], a.lfull + lb * EDDY_MAX_MP), HT)];
+    int slot = ht_find(fk, fam_key(t, a.lcnt[lb], a.lfull + lb * EDDY_MAX_MP), FH);
+    famfull[i] = slot >= 0 ? fv[slot] : -1;
   }
   for (int e = tid; e < E; e += 256) {
     const Event *x = &ev[e];
-    int slot = rep[x->s] == x->s ? ht_find(hkeys, fam_key(x->t, x->n, x->par), HT) : -1;
-    a.evfam[im.evoff + e] = slot >= 0 ? hvals[slot] : -1;
+    int slot = rep[x->s] == x->s ? ht_find(fk, fam_key(x->t, x->n, x->par), FH) : -1;
+    a.evfam[im.evoff + e] = slot >= 0 ? fv[slot] : -1;
   }
   __syncthreads();
-  // the hash table is done; its space now holds the family scores [f*2 + g]
-  double *fscore = (double *)dyn;
+  phase_mark(a, 3, tph);
+  // family scores live in global memory (their count is unbounded by LDS)
+  double *fscore = a.famscore + im.foff * 2;
   const int NF = F;
+  if (a.counters && tid == 0) {
+    unsigned long long hb = 0;
+    for (int f = 0; f < NF; f++) {
+      uint64_t key = a.famkey[im.foff + f] - 1;
+      hb += (1 + ((key >> 10) & 3)) * (unsigned long long)(a.n1 + a.n2);
+    }
+    atomicAdd(&a.counters[4], hb);                          // histogram bytes read (both classes)
+    atomicAdd(&a.counters[5], (unsigned long long)NF);      // families
+    atomicAdd(&a.counters[6], (unsigned long long)UU);      // unique networks
+    atomicAdd(&a.counters[7], (unsigned long long)E);       // events
+  }
   // ---- 4. BDeu per (family, scoring class): kernel5.cu:4-236 -------------
   // each wave scores its own (family, class) pairs with a private histogram
   for (int fg = wave; fg < 2 * NF; fg += EDDY_NWAVES) {
@@ -848,25 +961,39 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
       }
     }
     wave_sync();
+    // lanes gather the lgamma terms; lane 0 adds them in the reference's
+    // order (j outer, k inner: kernel5.cu:200-218)
+    if (q > 0 && r > 0) {
+      const double *lq = a.lgtab + (size_t)q * a.lgstride;
+      const double *lqr = a.lgtab + (size_t)(q * r) * a.lgstride;
+      for (int i = lane; i < q * r; i += 64) lgv[wave][i] = lqr[hist[wave][i]];
+      for (int j = lane; j < q; j += 64) {
+        int nij = 0;
+        for (int k = 0; k < r; k++) nij += hist[wave][j * r + k];
+        lgj[wave][j] = lq[nij];
+      }
+      if (lane == 0) {
+        lgj[wave][27] = lq[0];
+        lgv[wave][81] = lqr[0];
+      }
+    }
+    wave_sync();
     if (lane == 0) {
       double sum2 = 0.0;
       if (q > 0 && r > 0) {
-        const double *lq = a.lgtab + (size_t)q * a.lgstride;
-        const double *lqr = a.lgtab + (size_t)(q * r) * a.lgstride;
+        const double lq0 = lgj[wave][27], lqr0 = lgv[wave][81];
         for (int j = 0; j < q; j++) {
-          int nij = 0;
-          for (int k = 0; k < r; k++) nij += hist[wave][j * r + k];
           double sum1 = 0.0;
-          for (int k = 0; k < r; k++) sum1 = sum1 + (lqr[hist[wave][j * r + k]] - lqr[0]);
-          sum2 = sum2 + ((lq[0] - lq[nij]) + sum1);
+          for (int k = 0; k < r; k++) sum1 = sum1 + (lgv[wave][j * r + k] - lqr0);
+          sum2 = sum2 + ((lq0 - lgj[wave][j]) + sum1);
         }
       }
       fscore[f * 2 + g] = sum2;
-      a.famscore[(im.foff + f) * 2 + g] = sum2;
     }
     wave_sync();
   }
   __syncthreads();
+  phase_mark(a, 4, tph);
   // ---- 5. network scores, in node order (main.cu:1335-1342) -------------
   double *dist = a.lval + im.soff * 2;  // [g*S + u] (overwritten by the likelihoods below)
   for (int ug = tid; ug < 2 * UU; ug += 256) {
@@ -884,6 +1011,7 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
     dist[g * S + u] = d;
   }
   __syncthreads();
+  phase_mark(a, 5, tph);
   // ---- 6. likelihoods (main.cu:1294-1404) and JS (routines.cu:42-60) ----
   __shared__ double sh_inalpha[2], sh_scale[2], sh_nonInf[2], sh_infc[2];
   if (tid < 2) {
@@ -944,6 +1072,7 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
     double js = k1 / 2 + k2 / 2;
     a.js[im.set * a.perms + im.perm] = js / eddy_log(2.0);
   }
+  phase_mark(a, 6, tph);
 }
 
 // --------------------------------------------------------------------------
@@ -999,4 +1128,26 @@ __global__ void k_gather(const uint8_t *x, int N, int NP, const int *genes, cons
   const uint4 *src = (const uint4 *)(x + (size_t)genes[r] * NP);
   uint4 *dst = (uint4 *)(xs + rowoff[r]);
   for (int i = threadIdx.x; i < NP / 16; i += blockDim.x) dst[i] = src[i];
+}
+
+// Learned window per (set, row): the first column at which the expected number
+// of full-test edges (the network's scores, summed in column order) reaches
+// mp + kappa; the whole row when it never does.  One block per set.
+__global__ void k_window(const SetMeta *sets, int nsets, const float *score, int mp, float kappa, int *win) {
+  const int s = blockIdx.x;
+  if (s >= nsets) return;
+  const SetMeta sm = sets[s];
+  for (int t = threadIdx.x; t < sm.G; t += blockDim.x) {
+    float acc = 0.f;
+    int W = t - 1;
+    const float *sc = score + sm.pairbase + (long long)t * (t - 1) / 2;
+    for (int col = 0; col < t; col++) {
+      acc += sc[col];
+      if (acc >= (float)mp + kappa) {
+        W = col;
+        break;
+      }
+    }
+    win[sm.genebase + t] = W;
+  }
 }

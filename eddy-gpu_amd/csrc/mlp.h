@@ -1,0 +1,28 @@
+// mlp.h - layout and host entry points of the learned proposal's network.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#define MLP_IN 12
+#define MLP_H 64
+#define MLP_OFF_W1 0
+#define MLP_OFF_B1 (MLP_OFF_W1 + MLP_H * MLP_IN)
+#define MLP_OFF_W2 (MLP_OFF_B1 + MLP_H)
+#define MLP_OFF_B2 (MLP_OFF_W2 + MLP_H * MLP_H)
+#define MLP_OFF_W3 (MLP_OFF_B2 + MLP_H)
+#define MLP_OFF_B3 (MLP_OFF_W3 + MLP_H)
+#define MLP_NPARAMS (MLP_OFF_B3 + 1)
+
+// floats of block-partial scratch a batch of n samples needs
+size_t mlp_part_floats(int n);
+// out[i] = sigmoid(net(X[i]))
+hipError_t mlp_forward(const float *dX, int n, const float *dP, float *dOut, hipStream_t st);
+// mean BCE gradient and loss over samples idx[b0 .. b0+n) (rows b0.. when idx is null)
+hipError_t mlp_gradient(const float *dX, const float *dY, const int *dIdx, int b0, int n, const float *dP,
+                        float *dPart, float *dGrad, float *dLoss, hipStream_t st);
+// the same gradient followed by one Adam step (beta 0.9 / 0.999, eps 1e-8)
+hipError_t mlp_train_step(const float *dX, const float *dY, const int *dIdx, int b0, int n, float *dP, float *dPart,
+                          float *dGrad, float *dLoss, float *dM, float *dV, float lr, int step, hipStream_t st);
+// Y[i] = hits[i] * inv_den[set of i], sets starting at setbase[] (ascending)
+hipError_t mlp_labels(const unsigned *dHits, const float *dInvDen, const long long *dSetBase, int nsets, long long n,
+                      float *dY, hipStream_t st);

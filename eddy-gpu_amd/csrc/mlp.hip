@@ -1,0 +1,227 @@
+// mlp.hip - the learned proposal's network: 12 FP32 features -> 64 -> 64 -> 1
+// (ReLU, ReLU, sigmoid), trained with binary cross-entropy on soft labels by
+// Adam.  Hand-written HIP kernels; every reduction runs in a fixed order, so a
+// training run is reproducible bit for bit.
+//
+// Parameter layout (MLP_NPARAMS floats):
+//   W1 [64][12] at 0, b1 [64] at 768, W2 [64][64] at 832, b2 [64] at 4928,
+//   w3 [64] at 4992, b3 at 5056.
+//
+// Training step = k_mlp_part (one workgroup per 64 samples: forward and
+// backward with the activations in LDS, then that block's gradient sums) +
+// k_mlp_reduce (sums the block partials in block order, divides by the batch
+// size and, in training, applies the Adam update in the same pass).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mlp.h"
+
+#define MB 64          // samples per workgroup
+#define HS (MLP_H + 1) // padded LDS row: lanes of one wave hit distinct banks
+#define XS (MLP_IN + 1)
+
+__device__ __forceinline__ float relu(float x) { return x > 0.f ? x : 0.f; }
+__device__ __forceinline__ float sigmoidf_(float z) { return 1.f / (1.f + __expf(-z)); }
+
+// inference: one thread per sample, parameters in LDS, hidden layer 1 in registers
+__global__ __launch_bounds__(256) void k_mlp_forward(const float *X, int n, const float *params, float *out) {
+  __shared__ float P[MLP_NPARAMS];
+  for (int i = threadIdx.x; i < MLP_NPARAMS; i += blockDim.x) P[i] = params[i];
+  __syncthreads();
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  float x[MLP_IN];
+#pragma unroll
+  for (int i = 0; i < MLP_IN; i++) x[i] = X[(size_t)b * MLP_IN + i];
+  float a1[MLP_H];
+#pragma unroll
+  for (int j = 0; j < MLP_H; j++) {
+    float s = P[MLP_OFF_B1 + j];
+#pragma unroll
+    for (int i = 0; i < MLP_IN; i++) s += P[MLP_OFF_W1 + j * MLP_IN + i] * x[i];
+    a1[j] = relu(s);
+  }
+  float z = P[MLP_OFF_B3];
+  for (int k = 0; k < MLP_H; k++) {
+    float s = P[MLP_OFF_B2 + k];
+#pragma unroll
+    for (int j = 0; j < MLP_H; j++) s += P[MLP_OFF_W2 + k * MLP_H + j] * a1[j];
+    z += P[MLP_OFF_W3 + k] * relu(s);
+  }
+  out[b] = sigmoidf_(z);
+}
+
+// One workgroup = MB samples (rows idx[b0 + b] of X/Y when idx is given, else
+// rows b0 + b), 256 threads = 4 waves; wave q owns hidden units 16q..16q+15 in
+// the forward and backward passes (lane = sample), then every thread sums a
+// strided subset of the NPARAMS + 1 outputs (gradients, then the loss) over the
+// block's samples in sample order.  part[blk][q] receives the sums.
+__global__ __launch_bounds__(256) void k_mlp_part(const float *X, const float *Y, const int *idx, int b0, int n,
+                                                  const float *__restrict__ P, float *part) {
+  __shared__ float H1[MB * HS], H2[MB * HS], D1[MB * HS];
+  __shared__ float Xs[MB * XS];
+  __shared__ float ZQ[4 * MB];
+  __shared__ float DZ[MB], LS[MB];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, q = tid >> 6;
+  const int blk = blockIdx.x;
+  const int b = lane;                        // sample within the block
+  const int gb = blk * MB + b;               // sample within the batch
+  const bool live = gb < n;
+  const int row = live ? (idx ? idx[b0 + gb] : b0 + gb) : 0;
+  if (q == 0) {
+#pragma unroll
+    for (int i = 0; i < MLP_IN; i++) Xs[b * XS + i] = live ? X[(size_t)row * MLP_IN + i] : 0.f;
+  }
+  __syncthreads();
+  // layer 1: this wave's 16 units of h1
+  for (int jj = 0; jj < 16; jj++) {
+    const int j = q * 16 + jj;
+    float s = P[MLP_OFF_B1 + j];
+#pragma unroll
+    for (int i = 0; i < MLP_IN; i++) s += P[MLP_OFF_W1 + j * MLP_IN + i] * Xs[b * XS + i];
+    H1[b * HS + j] = live ? relu(s) : 0.f;
+  }
+  __syncthreads();
+  // layer 2: 16 units of h2 and their share of the logit
+  float zq = 0.f;
+  for (int kk = 0; kk < 16; kk++) {
+    const int k = q * 16 + kk;
+    float s = P[MLP_OFF_B2 + k];
+    for (int j = 0; j < MLP_H; j++) s += P[MLP_OFF_W2 + k * MLP_H + j] * H1[b * HS + j];
+    const float h = live ? relu(s) : 0.f;
+    H2[b * HS + k] = h;
+    zq += P[MLP_OFF_W3 + k] * h;
+  }
+  ZQ[q * MB + b] = zq;
+  __syncthreads();
+  if (q == 0) {
+    const float z = P[MLP_OFF_B3] + ((ZQ[b] + ZQ[MB + b]) + (ZQ[2 * MB + b] + ZQ[3 * MB + b]));
+    const float y = live ? Y[row] : 0.f;
+    // BCE(sigmoid(z), y) = softplus(z) - y z ; d/dz = sigmoid(z) - y
+    const float sp = z > 0.f ? z + log1pf(__expf(-z)) : log1pf(__expf(z));
+    LS[b] = live ? sp - y * z : 0.f;
+    DZ[b] = live ? sigmoidf_(z) - y : 0.f;
+  }
+  __syncthreads();
+  // backward to layer 1: d1[j] = [h1_j > 0] sum_k W2[k][j] d2[k],
+  // d2[k] = [h2_k > 0] dz w3[k]
+  {
+    const float dz = DZ[b];
+    for (int jj = 0; jj < 16; jj++) {
+      const int j = q * 16 + jj;
+      float s = 0.f;
+      for (int k = 0; k < MLP_H; k++) {
+        const float d2 = H2[b * HS + k] > 0.f ? dz * P[MLP_OFF_W3 + k] : 0.f;
+        s += P[MLP_OFF_W2 + k * MLP_H + j] * d2;
+      }
+      D1[b * HS + j] = H1[b * HS + j] > 0.f ? s : 0.f;
+    }
+  }
+  __syncthreads();
+  // block sums of the gradient, one output per thread at a time
+  float *out = part + (size_t)blk * (MLP_NPARAMS + 1);
+  for (int o = tid; o <= MLP_NPARAMS; o += 256) {
+    float s = 0.f;
+    if (o < MLP_OFF_B1) {  // W1[j][i]
+      const int j = o / MLP_IN, i = o % MLP_IN;
+      for (int t = 0; t < MB; t++) s += D1[t * HS + j] * Xs[t * XS + i];
+    } else if (o < MLP_OFF_W2) {  // b1[j]
+      const int j = o - MLP_OFF_B1;
+      for (int t = 0; t < MB; t++) s += D1[t * HS + j];
+    } else if (o < MLP_OFF_B2) {  // W2[k][j]
+      const int r = o - MLP_OFF_W2, k = r / MLP_H, j = r % MLP_H;
+      const float w3 = P[MLP_OFF_W3 + k];
+      for (int t = 0; t < MB; t++) {
+        const float d2 = H2[t * HS + k] > 0.f ? DZ[t] * w3 : 0.f;
+        s += d2 * H1[t * HS + j];
+      }
+    } else if (o < MLP_OFF_W3) {  // b2[k]
+      const int k = o - MLP_OFF_B2;
+      const float w3 = P[MLP_OFF_W3 + k];
+      for (int t = 0; t < MB; t++) s += H2[t * HS + k] > 0.f ? DZ[t] * w3 : 0.f;
+    } else if (o < MLP_OFF_B3) {  // w3[k]
+      const int k = o - MLP_OFF_W3;
+      for (int t = 0; t < MB; t++) s += DZ[t] * H2[t * HS + k];
+    } else if (o == MLP_OFF_B3) {
+      for (int t = 0; t < MB; t++) s += DZ[t];
+    } else {  // o == MLP_NPARAMS: loss
+      for (int t = 0; t < MB; t++) s += LS[t];
+    }
+    out[o] = s;
+  }
+}
+
+// grad[q] = (sum over blocks of part[blk][q]) / n in block order; loss likewise.
+// With adam != 0 the same thread then applies Adam to parameter q.
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float *part, int nblk, int n, float *grad, float *loss,
+                                                    int adam, float *params, float *m, float *v, float lr, float b1,
+                                                    float b2, float eps, int step) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > MLP_NPARAMS) return;
+  float s = 0.f;
+  for (int k = 0; k < nblk; k++) s += part[(size_t)k * (MLP_NPARAMS + 1) + q];
+  const float g = s / (float)n;
+  if (q == MLP_NPARAMS) {
+    *loss = g;
+    return;
+  }
+  grad[q] = g;
+  if (!adam) return;
+  const float mq = b1 * m[q] + (1.f - b1) * g;
+  const float vq = b2 * v[q] + (1.f - b2) * g * g;
+  m[q] = mq;
+  v[q] = vq;
+  const float mh = mq / (1.f - powf(b1, (float)step));
+  const float vh = vq / (1.f - powf(b2, (float)step));
+  params[q] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// Y[i] = hits[i] / den[set of i] (soft labels from the logged edge counts)
+__global__ void k_mlp_labels(const unsigned *hits, const float *inv_den, const long long *setbase, int nsets,
+                             long long n, float *Y) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int lo = 0, hi = nsets - 1;  // last set with setbase <= i
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (setbase[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  Y[i] = (float)hits[i] * inv_den[lo];
+}
+
+// ---- host helpers (device pointers; stream-ordered) -------------------------
+size_t mlp_part_floats(int n) { return (size_t)((n + MB - 1) / MB) * (MLP_NPARAMS + 1); }
+
+hipError_t mlp_forward(const float *dX, int n, const float *dP, float *dOut, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  k_mlp_forward<<<(n + 255) / 256, 256, 0, st>>>(dX, n, dP, dOut);
+  return hipGetLastError();
+}
+
+hipError_t mlp_gradient(const float *dX, const float *dY, const int *dIdx, int b0, int n, const float *dP,
+                        float *dPart, float *dGrad, float *dLoss, hipStream_t st) {
+  if (n <= 0) return hipErrorInvalidValue;
+  const int nblk = (n + MB - 1) / MB;
+  k_mlp_part<<<nblk, 256, 0, st>>>(dX, dY, dIdx, b0, n, dP, dPart);
+  k_mlp_reduce<<<(MLP_NPARAMS + 1 + 255) / 256, 256, 0, st>>>(dPart, nblk, n, dGrad, dLoss, 0, nullptr, nullptr,
+                                                               nullptr, 0.f, 0.f, 0.f, 0.f, 0);
+  return hipGetLastError();
+}
+
+hipError_t mlp_train_step(const float *dX, const float *dY, const int *dIdx, int b0, int n, float *dP, float *dPart,
+                          float *dGrad, float *dLoss, float *dM, float *dV, float lr, int step, hipStream_t st) {
+  if (n <= 0) return hipErrorInvalidValue;
+  const int nblk = (n + MB - 1) / MB;
+  k_mlp_part<<<nblk, 256, 0, st>>>(dX, dY, dIdx, b0, n, dP, dPart);
+  k_mlp_reduce<<<(MLP_NPARAMS + 1 + 255) / 256, 256, 0, st>>>(dPart, nblk, n, dGrad, dLoss, 1, dP, dM, dV, lr, 0.9f,
+                                                               0.999f, 1e-8f, step);
+  return hipGetLastError();
+}
+
+hipError_t mlp_labels(const unsigned *dHits, const float *dInvDen, const long long *dSetBase, int nsets, long long n,
+                      float *dY, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  k_mlp_labels<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(dHits, dInvDen, dSetBase, nsets, n, dY);
+  return hipGetLastError();
+}

@@ -25,7 +25,11 @@ EXPORTS = [
     "eddy_get_js", "eddy_get_perm0_networks", "eddy_get_edgelist", "eddy_run_perm_detail",
     "eddy_get_timing", "eddy_synth_config", "eddy_synth_dataset", "eddy_synth_sets",
     "eddy_edge_decision_host", "eddy_stream_permutation",
+    "eddy_mlp_init", "eddy_mlp_get_params", "eddy_mlp_set_params", "eddy_mlp_set_margin", "eddy_mlp_forward",
+    "eddy_mlp_gradient", "eddy_mlp_train", "eddy_learn_collect", "eddy_run_learned",
 ]
+MLP_FEATURES = 12
+MLP_NPARAMS = 5057
 
 
 class Params(C.Structure):
@@ -58,7 +62,23 @@ KERNELS = ["count_test", "loo", "rows", "score", "pvalue", "gather"]
 class Timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("kernel_ms", C.c_double * 6), ("launches", C.c_longlong * 6),
                 ("items", C.c_longlong), ("pairs", C.c_longlong), ("tests", C.c_longlong),
-                ("host_stream_ms", C.c_double), ("sched", C.c_longlong * 4)]
+                ("host_stream_ms", C.c_double), ("sched", C.c_longlong * 4), ("stats", C.c_longlong * 4),
+                ("phase1_ms", C.c_double), ("phase_ms", C.c_double * 10)]
+
+
+class LearnCfg(C.Structure):
+    _fields_ = [("train_sets", C.c_int), ("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_float),
+                ("margin", C.c_float), ("seed", C.c_uint64)]
+
+
+class LearnReport(C.Structure):
+    _fields_ = [("collect_ms", C.c_double), ("train_ms", C.c_double), ("apply_ms", C.c_double),
+                ("train_rows", C.c_longlong), ("loss_initial", C.c_float), ("loss_final", C.c_float),
+                ("tests_collect", C.c_longlong), ("tests_apply", C.c_longlong), ("trained", C.c_int)]
+
+
+def learn_cfg(train_sets=100, epochs=4, batch=4096, lr=3e-3, margin=1.0, seed=1):
+    return LearnCfg(train_sets, epochs, batch, lr, margin, seed)
 
 
 class SynthCfg(C.Structure):
@@ -102,6 +122,18 @@ def lib():
                                       C.c_longlong]
         L.eddy_stream_permutation.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
         L.eddy_edge_decision_host.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(Params)]
+        L.eddy_mlp_init.argtypes = [C.c_void_p, C.c_uint64]
+        L.eddy_mlp_get_params.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        L.eddy_mlp_set_params.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        L.eddy_mlp_set_margin.argtypes = [C.c_void_p, C.c_float]
+        L.eddy_mlp_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        L.eddy_mlp_gradient.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        L.eddy_mlp_train.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, C.c_int,
+                                     C.c_float, C.c_uint64, C.c_void_p]
+        L.eddy_learn_collect.argtypes = [C.c_void_p, C.POINTER(SetDesc), C.c_int, C.POINTER(Params), C.c_void_p,
+                                         C.c_void_p, C.c_longlong, C.POINTER(C.c_longlong)]
+        L.eddy_run_learned.argtypes = [C.c_void_p, C.POINTER(SetDesc), C.c_int, C.POINTER(Params),
+                                       C.POINTER(LearnCfg), C.POINTER(SetResult), C.POINTER(LearnReport)]
         _lib = L
     return _lib
 
@@ -227,6 +259,68 @@ class Engine:
         self.last_perms = params.perms
         self.last_G = [len(g) for g, _ in sets]
         return [res[i] for i in range(len(sets))]
+
+    def run_learned(self, sets, params, cfg):
+        """The learned job (eddy_run_learned): returns (results, report)."""
+        arr, keep = self._descs(sets)
+        res = (SetResult * max(1, len(sets)))()
+        rep = LearnReport()
+        self._chk(self.L.eddy_run_learned(self.h, arr, len(sets), C.byref(params), C.byref(cfg), res,
+                                          C.byref(rep)), "run_learned")
+        self.last_perms = params.perms
+        self.last_G = [len(g) for g, _ in sets]
+        return [res[i] for i in range(len(sets))], rep
+
+    # ---- learned proposal network
+    def mlp_init(self, seed):
+        self._chk(self.L.eddy_mlp_init(self.h, seed), "mlp_init")
+
+    def mlp_params(self):
+        out = np.zeros(MLP_NPARAMS, np.float32)
+        self._chk(self.L.eddy_mlp_get_params(self.h, _ptr(out), MLP_NPARAMS), "mlp_get_params")
+        return out
+
+    def mlp_set_params(self, P):
+        P = np.ascontiguousarray(P, np.float32)
+        self._chk(self.L.eddy_mlp_set_params(self.h, _ptr(P), len(P)), "mlp_set_params")
+
+    def mlp_set_margin(self, kappa):
+        self._chk(self.L.eddy_mlp_set_margin(self.h, kappa), "mlp_set_margin")
+
+    def mlp_forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros(len(X), np.float32)
+        self._chk(self.L.eddy_mlp_forward(self.h, _ptr(X), len(X), _ptr(out)), "mlp_forward")
+        return out
+
+    def mlp_gradient(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        g = np.zeros(MLP_NPARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._chk(self.L.eddy_mlp_gradient(self.h, _ptr(X), _ptr(Y), len(X), _ptr(g), _ptr(loss)), "mlp_gradient")
+        return g, float(loss[0])
+
+    def mlp_train(self, X, Y, epochs, batch, lr, seed=1):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._chk(self.L.eddy_mlp_train(self.h, _ptr(X), _ptr(Y), len(X), epochs, batch, lr, seed, _ptr(loss)),
+                  "mlp_train")
+        return float(loss[0])
+
+    def learn_collect(self, sets, params):
+        """Features and soft labels of every pair of `sets`, from the plain method."""
+        arr, keep = self._descs(sets)
+        n = sum(len(g) * (len(g) - 1) // 2 for g, _ in sets)
+        X = np.zeros((max(1, n), MLP_FEATURES), np.float32)
+        Y = np.zeros(max(1, n), np.float32)
+        got = C.c_longlong()
+        self._chk(self.L.eddy_learn_collect(self.h, arr, len(sets), C.byref(params), _ptr(X), _ptr(Y), n,
+                                            C.byref(got)), "learn_collect")
+        self.last_perms = params.perms
+        self.last_G = [len(g) for g, _ in sets]
+        return X[:got.value], Y[:got.value]
 
     def js(self, s):
         out = np.zeros(self.last_perms, np.float64)

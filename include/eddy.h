@@ -187,6 +187,13 @@ typedef struct {
   long long sched[4];                /* cutoff schedule: rows past their plan,
                                         columns tested past plans, planned
                                         (pair, class) entries, rows */
+  long long stats[4];                /* BDeu histogram bytes read, families
+                                        scored, unique networks, events */
+  double phase1_ms;                  /* host wall time of permutation 0 of every set */
+  double phase_ms[10];               /* workgroup time per phase summed over
+                                        items: score 0 dedup, 1 confirm, 2
+                                        unique list, 3 families, 4 BDeu, 5 sums,
+                                        6 likelihood/JS; rows 7 setup, 8 rows */
 } eddy_timing;
 int eddy_get_timing(eddy_ctx *ctx, eddy_timing *t);
 
@@ -194,6 +201,62 @@ int eddy_get_timing(eddy_ctx *ctx, eddy_timing *t);
  * (tally9[r*3+k], r = row-gene state): 1 edge, 0 none.  Tests compare it
  * with the CPU oracle; it runs on the CPU and needs no context. */
 int eddy_edge_decision_host(const int *tally9, int c, int prior, const eddy_params *p);
+
+/* ---- learned proposal ------------------------------------------------------
+ * A 12 -> 64 -> 64 -> 1 network (ReLU, ReLU, sigmoid; FP32) scores each gene
+ * pair from 12 features of its pooled table (Pearson chi2/n, log1p chi2,
+ * mutual information, both marginal entropies, empty-cell fraction, smallest
+ * expected count, n, dof, prior flag, Bonferroni factor, Cramer's V): the
+ * probability that the pair is a full-test edge in a permuted class.  With
+ * eddy_params.learned = 1 (batched, cutoff schedule) the engine sums the scores
+ * along each row and runs the count kernel's full tests only up to the column
+ * where they reach mp + margin; every later test the cutoff still needs is run
+ * on demand, so results never depend on the network.  No counterpart in the
+ * reference (whose tests run in a fixed order, kernel3.cu:70-109). */
+#define EDDY_MLP_FEATURES 12
+#define EDDY_MLP_NPARAMS 5057
+int eddy_mlp_init(eddy_ctx *ctx, uint64_t seed);
+int eddy_mlp_get_params(eddy_ctx *ctx, float *out, int n);
+int eddy_mlp_set_params(eddy_ctx *ctx, const float *in, int n);
+int eddy_mlp_set_margin(eddy_ctx *ctx, float kappa);
+/* prob[n] = network output for X[n][12] */
+int eddy_mlp_forward(eddy_ctx *ctx, const float *X, int n, float *prob);
+/* mean binary cross-entropy gradient over (X[n][12], Y[n]) and its loss */
+int eddy_mlp_gradient(eddy_ctx *ctx, const float *X, const float *Y, int n, float *grad, float *loss);
+/* Adam on the GPU, `epochs` passes over shuffled mini-batches */
+int eddy_mlp_train(eddy_ctx *ctx, const float *X, const float *Y, long long n, int epochs, int batch, float lr,
+                   uint64_t seed, float *loss_out);
+/* Runs `sets` without the network (every full test; leave-one-out tests up to
+ * the exact cutoff) and returns one training row per gene pair: X[n][12] features, Y[n] fraction of (permutation > 0, class)
+ * full-test edges.  *n_out = number of pairs (EDDY_ERR_CAPACITY if > cap). */
+int eddy_learn_collect(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *params, float *X,
+                       float *Y, long long cap, long long *n_out);
+
+/* The learned job: the first train_sets sets run without the network (every
+ * full test) and their outcomes are logged; the network is trained on the GPU (Adam, `epochs`
+ * passes of `batch`-row mini-batches, learning rate lr, initialised from
+ * seed); the other sets run the learned schedule with window margin `margin`.
+ * Results, networks and the permutation stream equal eddy_run_sets over all
+ * sets. */
+typedef struct {
+  int train_sets;
+  int epochs;
+  int batch;
+  float lr;
+  float margin;
+  uint64_t seed;
+} eddy_learn_cfg;
+
+typedef struct {
+  double collect_ms, train_ms, apply_ms;
+  long long train_rows;
+  float loss_initial, loss_final;
+  long long tests_collect, tests_apply;
+  int trained; /* 0 when too few rows were logged (< 64): the rest ran unlearned */
+} eddy_learn_report;
+
+int eddy_run_learned(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *params,
+                     const eddy_learn_cfg *cfg, eddy_set_result *results, eddy_learn_report *report);
 
 /* ---- synthetic workloads (seeded DAG data in the benchmark shapes) ------- */
 typedef struct {
