@@ -102,7 +102,8 @@ def main():
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--nsets", type=int, default=None)
     ap.add_argument("--train-sets", type=int, default=100)
-    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--train-rows", type=int, default=131072)
     ap.add_argument("--margin", type=float, default=1.0)
     ap.add_argument("--plain-sets", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -123,8 +124,8 @@ def main():
     setup_s = time.perf_counter() - t_setup
     nsets = len(wl.sets)
     prm = wl.params(mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=1)
-    cfg = eddy_amd.learn_cfg(train_sets=args.train_sets, epochs=args.epochs, batch=4096, lr=3e-3,
-                             margin=args.margin, seed=1)
+    cfg = eddy_amd.learn_cfg(train_sets=args.train_sets, epochs=args.epochs, batch=8192, lr=3e-3,
+                             margin=args.margin, seed=1, max_rows=args.train_rows)
 
     print("bench: setup %.1fs, %d sets, N=%d" % (setup_s, nsets, wl.N), file=sys.stderr, flush=True)
     for w in range(args.warmup):
@@ -165,7 +166,9 @@ def main():
     js_head = [eng.js(s).copy() for s in range(nsets)]
     rep = reps[-1]
     learned = {
-        "train_sets": min(args.train_sets, nsets), "train_rows": rep.train_rows, "epochs": args.epochs,
+        "train_sets": min(args.train_sets, nsets), "logged_rows": rep.train_rows,
+        "train_rows": min(rep.train_rows, args.train_rows) if args.train_rows > 0 else rep.train_rows,
+        "epochs": args.epochs,
         "loss_initial": rep.loss_initial, "loss_final": rep.loss_final, "margin": args.margin,
         "collect_ms": rep.collect_ms, "train_ms": rep.train_ms, "apply_ms": rep.apply_ms,
         "tests_applied_sets": rep.tests_apply,

@@ -19,6 +19,13 @@
 // and runs the exact computation everywhere else (within the margins of a
 // threshold, and for every other dof).  thr = max(alphaEdge, alphaEdgePrior
 // if the pair has a prior edge) / c with Bonferroni, without it thr = alpha.
+//
+// The fast forms (eddy_edge_decision_fast, eddy_loo_mask_fast) compute x from
+// the margins' reciprocals instead of the reference's 18 divisions per tally:
+// the same x within 1e-10 (1 + x) absolute (error analysis at
+// eddy_half_chi2_fast; measured in the tests), so they apply the intervals
+// above shrunk by a guard of 1e-9 (1 + x) and run the exact decision on the
+// tally itself wherever x falls inside a guard band.
 #pragma once
 #include "fpmath.h"
 
@@ -111,4 +118,123 @@ EDDY_HD int eddy_edge_decision(const int *t, int prior, const eddy_edge_params *
     if (x >= eb->xhi[pr][pi] && x < ep->xclo[pi]) return 1;
   }
   return eddy_edge_exact(chi, dof, prior, ep, cpairs);
+}
+
+// ---------------------------------------------------------------------------
+// fast forms
+// ---------------------------------------------------------------------------
+// x = chi2 / 2 and dof of tally t with row sums R, column sums K, total n,
+// from iR[i] = 1/R[i], iK[j] = 1/K[j] (0 for empty sums) and in = 1/n.
+// Yates' rule is decided exactly in integers: (double)K*R/n < 5 holds iff
+// R*K < 5n, since a quotient of integers <= 2^22 that is not 5 differs from 5
+// by at least 1/n, far more than one rounding.  Each cell term
+// (|O - E| - y)^2 n iR iK differs from the reference's d*d/E by a few ulp of
+// itself plus 2|d| * 3 ulp(E) <= 2 n 6.7e-16, so |x - x_ref| <= 9 n 6.7e-16 +
+// 1e-15 x < 1.3e-11 + 1e-15 x for n <= 2046.
+EDDY_HD double eddy_half_chi2_fast(const int *t, const int *R, const int *K, int n, const double *iR,
+                                   const double *iK, double in, int *dof) {
+  int zr = 0, zk = 0, yates = 0;
+  for (int i = 0; i < 3; i++) {
+    zr += R[i] == 0;
+    zk += K[i] == 0;
+  }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      if (R[i] && K[j] && R[i] * K[j] < 5 * n) yates = 1;
+  const double y = yates ? 0.5 : 0.0, dn = (double)n;
+  double chi = 0.0;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      if (R[i] && K[j]) {
+        const double rk = (double)(R[i] * K[j]);
+        const double d = eddy_fabs((double)t[i * 3 + j] - rk * in) - y;
+        chi += d * d * dn * (iR[i] * iK[j]);
+      }
+  *dof = (2 - zk) * (2 - zr);
+  return 0.5 * chi;
+}
+
+// certain outcome of x (1 / 0) from the intervals shrunk by the guard, or -1
+EDDY_HD int eddy_edge_certain(double x, int dof, int prior, const eddy_edge_params *ep, const eddy_edge_bounds *eb) {
+  if (!ep->shortcut || !(dof == 1 || dof == 2 || dof == 4)) return -1;
+  const int pi = dof == 1 ? 0 : dof == 2 ? 1 : 2;
+  const int pr = prior == 1 ? 1 : 0;
+  const double gd = 1e-9 * (1.0 + x);
+  if (x <= eb->xlo[pr][pi] - gd) return 0;
+  if (x > ep->xchi[pi] + gd) return 0;
+  if (x >= eb->xhi[pr][pi] + gd && x < ep->xclo[pi] - gd) return 1;
+  return -1;
+}
+
+EDDY_HD double eddy_recip_or0(int v) { return v ? 1.0 / (double)v : 0.0; }
+
+// eddy_edge_decision() with the fast x
+EDDY_HD int eddy_edge_decision_fast(const int *t, int prior, const eddy_edge_params *ep, const eddy_edge_bounds *eb,
+                                    double cpairs) {
+  int R[3], K[3];
+  for (int i = 0; i < 3; i++) {
+    R[i] = t[i * 3] + t[i * 3 + 1] + t[i * 3 + 2];
+    K[i] = t[i] + t[3 + i] + t[6 + i];
+  }
+  const int n = R[0] + R[1] + R[2];
+  if (n > 0) {
+    double iR[3], iK[3];
+    for (int i = 0; i < 3; i++) {
+      iR[i] = eddy_recip_or0(R[i]);
+      iK[i] = eddy_recip_or0(K[i]);
+    }
+    int dof;
+    const double x = eddy_half_chi2_fast(t, R, K, n, iR, iK, 1.0 / (double)n, &dof);
+    const int c = eddy_edge_certain(x, dof, prior, ep, eb);
+    if (c >= 0) return c;
+  }
+  return eddy_edge_decision(t, prior, ep, eb, cpairs);
+}
+
+// Leave-one-out decisions of one tally: bit `cell` is the decision of t minus
+// one count in `cell`; empty cells copy `full`.  Thirteen reciprocals per
+// tally serve all nine leave-one-out tallies.
+EDDY_HD uint32_t eddy_loo_mask_fast(const int *t, int full, int prior, const eddy_edge_params *ep,
+                                    const eddy_edge_bounds *eb, double cpairs, int *ntests) {
+  int R[3], K[3];
+  for (int i = 0; i < 3; i++) {
+    R[i] = t[i * 3] + t[i * 3 + 1] + t[i * 3 + 2];
+    K[i] = t[i] + t[3 + i] + t[6 + i];
+  }
+  const int n = R[0] + R[1] + R[2];
+  double iR[3], iK[3], iRm[3], iKm[3];
+  for (int i = 0; i < 3; i++) {
+    iR[i] = eddy_recip_or0(R[i]);
+    iK[i] = eddy_recip_or0(K[i]);
+    iRm[i] = eddy_recip_or0(R[i] - 1);
+    iKm[i] = eddy_recip_or0(K[i] - 1);
+  }
+  const double in1 = eddy_recip_or0(n - 1);
+  uint32_t m = 0;
+  int tt[9];
+  for (int i = 0; i < 9; i++) tt[i] = t[i];
+  for (int cell = 0; cell < 9; cell++) {
+    int bit = full;
+    if (t[cell] > 0) {
+      const int a = cell / 3, b = cell % 3;
+      tt[cell] = t[cell] - 1;
+      int c = -1;
+      if (n > 1) {
+        int R1[3] = {R[0], R[1], R[2]}, K1[3] = {K[0], K[1], K[2]};
+        double iR1[3] = {iR[0], iR[1], iR[2]}, iK1[3] = {iK[0], iK[1], iK[2]};
+        R1[a] -= 1;
+        K1[b] -= 1;
+        iR1[a] = iRm[a];
+        iK1[b] = iKm[b];
+        int dof;
+        const double x = eddy_half_chi2_fast(tt, R1, K1, n - 1, iR1, iK1, in1, &dof);
+        c = eddy_edge_certain(x, dof, prior, ep, eb);
+      }
+      bit = c >= 0 ? c : eddy_edge_decision(tt, prior, ep, eb, cpairs);
+      tt[cell] = t[cell];
+      (*ntests)++;
+    }
+    m |= (uint32_t)bit << cell;
+  }
+  return m;
 }

@@ -29,6 +29,7 @@ __global__ void k_pvalue(const double *js, int nsets, int perms, const int *nan0
 __global__ void k_gather(const uint8_t *x, int N, int NP, const int *genes, const long long *rowoff, int nrows,
                          uint8_t *xs);
 __global__ void k_window(const SetMeta *sets, int nsets, const float *score, int mp, float kappa, int *win);
+__global__ void k_copy_slices(const Slice *sl, int n);
 
 namespace {
 
@@ -49,6 +50,15 @@ struct Perm0 {
   std::vector<uint32_t> info;   // [c]
   std::vector<uint64_t> famkey; // distinct families (t, first <= 3 parents)
   std::vector<double> famscore; // [f*2 + g] BDeu of family f on class g
+};
+
+struct P0Store {
+  DevBuf info, lcnt, lfull, rep, lval, ev, famkey, famscore, slices;
+  std::vector<SetMeta> meta;
+  std::vector<long long> evoff, evn, famoff, famn;
+  std::vector<int> U, done;
+  long long evtot = 0, famtot = 0;
+  std::vector<Slice> hslices;
 };
 
 }  // namespace
@@ -78,7 +88,8 @@ struct eddy_ctx {
   int last_perms = 0;
   std::vector<int> last_G;
   std::vector<std::vector<double>> last_js;
-  std::vector<Perm0> last_p0;
+  std::vector<Perm0> last_p0;  // host copies, fetched from p0 on request
+  P0Store p0;
   eddy_timing timing{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // learned proposal: network parameters + Adam state, per-run feature,
@@ -162,9 +173,12 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
                    &ctx->b_info, &ctx->b_tab, &ctx->b_lfull, &ctx->b_lcnt, &ctx->b_sflags, &ctx->b_ev,
                    &ctx->b_evcount, &ctx->b_hnet, &ctx->b_rep, &ctx->b_clsg, &ctx->b_evorder, &ctx->b_evfam,
                    &ctx->b_famkey, &ctx->b_famscore, &ctx->b_lval, &ctx->b_uniq, &ctx->b_js, &ctx->b_status,
-                   &ctx->b_tests, &ctx->b_genes, &ctx->b_rowoff, &ctx->b_nan0, &ctx->b_pout, &ctx->b_famcount, &ctx->b_counters, &ctx->b_famht, &ctx->b_famhv, &ctx->b_mlp, &ctx->b_mlpm,
-                   &ctx->b_mlpv, &ctx->b_mlpg, &ctx->b_mlploss, &ctx->b_feat, &ctx->b_labhits, &ctx->b_score,
-                   &ctx->b_win, &ctx->b_mx, &ctx->b_my, &ctx->b_mxb, &ctx->b_myb, &ctx->b_midx, &ctx->b_mscr, &ctx->b_mlbase, &ctx->b_mlinv};
+                   &ctx->b_tests, &ctx->b_genes, &ctx->b_rowoff, &ctx->b_nan0, &ctx->b_pout, &ctx->b_famcount,
+                   &ctx->b_counters, &ctx->b_famht, &ctx->b_famhv, &ctx->b_mlp, &ctx->b_mlpm, &ctx->b_mlpv,
+                   &ctx->b_mlpg, &ctx->b_mlploss, &ctx->b_feat, &ctx->b_labhits, &ctx->b_score, &ctx->b_win,
+                   &ctx->b_mx, &ctx->b_my, &ctx->b_mxb, &ctx->b_myb, &ctx->b_midx, &ctx->b_mscr, &ctx->b_mlbase,
+                   &ctx->b_mlinv, &ctx->p0.info, &ctx->p0.lcnt, &ctx->p0.lfull, &ctx->p0.rep, &ctx->p0.lval,
+                   &ctx->p0.ev, &ctx->p0.famkey, &ctx->p0.famscore, &ctx->p0.slices};
   for (DevBuf *b : all)
     if (b->p) hipFree(b->p);
   if (ctx->dX) hipFree(ctx->dX);
@@ -353,9 +367,13 @@ void sub_chunk(const Chunk &src, const std::vector<int> &idx, const std::vector<
 }
 
 // one launch sequence over a chunk; per-item event capacity `evcap`
+struct ChunkOut {  // per-item results read back after a launch
+  std::vector<int> status, evc, fc, uq;
+  std::vector<long long> tests;
+};
+
 int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, const eddy_params *p, int perms,
-                 const std::vector<int> &evcap, int attempt, std::vector<int> &status, std::vector<int> &evc,
-                 std::vector<long long> &tests) {
+                 const std::vector<int> &evcap, int attempt, ChunkOut &out) {
   auto hl0 = std::chrono::steady_clock::now();
   const int N = ctx->N, S = N + 2;
   const int nitems = (int)ch.items.size();
@@ -421,6 +439,7 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
   a.mp = p->mp;
   a.p_adjust = p->p_adjust;
   a.plain = plain_tests ? 1 : 0;
+  a.feat_only = 0;
   a.ht_size = ht;
   a.plan_extra = nullptr;
   a.feat = ctx->k_feat;
@@ -475,12 +494,17 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
   k_score<<<nitems, 256, score_lds, st>>>(a);
   hipEventRecord(ev[4], st);
   HIPCHK(ctx, hipGetLastError());
-  status.assign(nitems, 0);
-  evc.assign(nitems, 0);
-  tests.assign(nitems, 0);
-  HIPCHK(ctx, hipMemcpyAsync(status.data(), ctx->b_status.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
-  HIPCHK(ctx, hipMemcpyAsync(evc.data(), ctx->b_evcount.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
-  HIPCHK(ctx, hipMemcpyAsync(tests.data(), ctx->b_tests.p, sizeof(long long) * nitems, hipMemcpyDeviceToHost, st));
+  out.status.assign(nitems, 0);
+  out.evc.assign(nitems, 0);
+  out.fc.assign(nitems, 0);
+  out.uq.assign(nitems, 0);
+  out.tests.assign(nitems, 0);
+  HIPCHK(ctx, hipMemcpyAsync(out.status.data(), ctx->b_status.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(out.evc.data(), ctx->b_evcount.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(out.fc.data(), ctx->b_famcount.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(out.uq.data(), ctx->b_uniq.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(out.tests.data(), ctx->b_tests.p, sizeof(long long) * nitems, hipMemcpyDeviceToHost,
+                             st));
   auto hl1 = std::chrono::steady_clock::now();
   HIPCHK(ctx, hipStreamSynchronize(st));
   auto hl2 = std::chrono::steady_clock::now();
@@ -498,9 +522,9 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
   if (getenv("EDDY_DEBUG")) {
     int mx = 0, bad = 0, nb = 0;
     for (int i = 0; i < nitems; i++) {
-      mx = std::max(mx, evc[i]);
-      bad |= status[i];
-      nb += status[i] != 0;
+      mx = std::max(mx, out.evc[i]);
+      bad |= out.status[i];
+      nb += out.status[i] != 0;
     }
     fprintf(stderr, "[eddy] launch items=%d pairs=%lld attempt=%d max_events=%d failing=%d status=%d\n", nitems,
             ch.pairs, attempt, mx, nb, bad);
@@ -508,65 +532,148 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
   return EDDY_OK;
 }
 
-// Permutation-0 state of the finished items `idx` of chunk `ch` (one item per
-// set): the device buffers are copied back in bulk, then sliced per set.
-int capture_items(eddy_ctx *ctx, const std::vector<SetMeta> &sets, const Chunk &ch, const std::vector<int> &idx,
-                  const std::vector<int> &evc, std::vector<Perm0> *p0) {
+// Grow a device buffer keeping its first `keep` bytes (stream-ordered copy).
+int grow_keep(eddy_ctx *ctx, DevBuf &b, size_t bytes, size_t keep) {
+  if (b.n >= bytes) return EDDY_OK;
+  DevBuf nb;
+  size_t want = bytes + bytes / 2;
+  HIPCHK(ctx, hipMalloc(&nb.p, want));
+  nb.n = want;
+  if (b.p && keep) HIPCHK(ctx, hipMemcpyAsync(nb.p, b.p, keep, hipMemcpyDeviceToDevice, ctx->stream));
+  if (b.p) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipFree(b.p));
+  }
+  b = nb;
+  return EDDY_OK;
+}
+
+// Device-resident store of what permutation 0 of every set leaves behind:
+// fixed-size parts at offsets of the set layout, events and families
+// appended.  Filled by device-to-device slice copies, read back per set only
+// when the caller asks for networks or edge lists.
+int p0_begin(eddy_ctx *ctx, const std::vector<SetMeta> &meta) {
+  P0Store &q = ctx->p0;
+  const int nsets = (int)meta.size(), S = ctx->N + 2;
+  long long sumc = 0, sumg = 0;
+  for (const SetMeta &m : meta) {
+    sumc += m.c;
+    sumg += m.G;
+  }
+  CHK(grow(ctx, q.info, sizeof(uint32_t) * std::max(1LL, sumc)));
+  CHK(grow(ctx, q.lcnt, 2 * std::max(1LL, sumg)));
+  CHK(grow(ctx, q.lfull, sizeof(uint16_t) * EDDY_MAX_MP * 2 * std::max(1LL, sumg)));
+  CHK(grow(ctx, q.rep, sizeof(int) * (size_t)S * std::max(1, nsets)));
+  CHK(grow(ctx, q.lval, sizeof(double) * 2 * (size_t)S * std::max(1, nsets)));
+  q.meta = meta;
+  q.evoff.assign(nsets, 0);
+  q.evn.assign(nsets, 0);
+  q.famoff.assign(nsets, 0);
+  q.famn.assign(nsets, 0);
+  q.U.assign(nsets, 0);
+  q.done.assign(nsets, 0);
+  q.evtot = q.famtot = 0;
+  ctx->last_p0.assign(nsets, Perm0());
+  return EDDY_OK;
+}
+
+// Copy the permutation-0 state of the finished items `idx` of a chunk into
+// the store (one slice-copy launch, stream-ordered before the next chunk).
+int capture_items(eddy_ctx *ctx, const Chunk &ch, const std::vector<int> &idx, const ChunkOut &o) {
   auto hc0 = std::chrono::steady_clock::now();
+  P0Store &q = ctx->p0;
   const int S = ctx->N + 2;
-  const int nitems = (int)ch.items.size();
-  hipStream_t st = ctx->stream;
-  std::vector<int> uq(nitems), fc(nitems);
-  std::vector<uint8_t> lcnt(2 * ch.gslots);
-  std::vector<uint16_t> lfull((size_t)2 * ch.gslots * EDDY_MAX_MP);
-  std::vector<int> rep(ch.sslots);
-  std::vector<double> lval(2 * ch.sslots);
-  std::vector<uint32_t> info(ch.pairs);
-  HIPCHK(ctx, hipMemcpyAsync(uq.data(), ctx->b_uniq.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
-  HIPCHK(ctx, hipMemcpyAsync(fc.data(), ctx->b_famcount.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
-  HIPCHK(ctx, hipMemcpyAsync(lcnt.data(), ctx->b_lcnt.p, lcnt.size(), hipMemcpyDeviceToHost, st));
-  HIPCHK(ctx, hipMemcpyAsync(lfull.data(), ctx->b_lfull.p, sizeof(uint16_t) * lfull.size(), hipMemcpyDeviceToHost, st));
-  HIPCHK(ctx, hipMemcpyAsync(rep.data(), ctx->b_rep.p, sizeof(int) * rep.size(), hipMemcpyDeviceToHost, st));
-  HIPCHK(ctx, hipMemcpyAsync(lval.data(), ctx->b_lval.p, sizeof(double) * lval.size(), hipMemcpyDeviceToHost, st));
-  if (!info.empty())
-    HIPCHK(ctx, hipMemcpyAsync(info.data(), ctx->b_info.p, sizeof(uint32_t) * info.size(), hipMemcpyDeviceToHost, st));
-  HIPCHK(ctx, hipStreamSynchronize(st));
+  long long add_ev = 0, add_fam = 0;
+  for (int k : idx) {
+    add_ev += std::min(o.evc[k], ch.items[k].evcap);
+    add_fam += o.fc[k];
+  }
+  CHK(grow_keep(ctx, q.ev, sizeof(Event) * std::max(1LL, q.evtot + add_ev), sizeof(Event) * q.evtot));
+  CHK(grow_keep(ctx, q.famkey, sizeof(uint64_t) * std::max(1LL, q.famtot + add_fam), sizeof(uint64_t) * q.famtot));
+  CHK(grow_keep(ctx, q.famscore, sizeof(double) * 2 * std::max(1LL, q.famtot + add_fam),
+                sizeof(double) * 2 * q.famtot));
+  std::vector<Slice> &sl = q.hslices;
+  sl.clear();
+  auto add = [&](const void *src, void *dst, long long bytes) {
+    if (bytes > 0) sl.push_back(Slice{src, dst, bytes});
+  };
   for (int k : idx) {
     const ItemMeta &im = ch.items[k];
-    const SetMeta &sm = sets[im.set];
-    Perm0 &q = (*p0)[im.set];
-    q.U = uq[k];
-    q.lcnt.assign(lcnt.begin() + im.goff * 2, lcnt.begin() + im.goff * 2 + 2 * sm.G);
-    q.lfull.assign(lfull.begin() + im.goff * 2 * EDDY_MAX_MP, lfull.begin() + (im.goff * 2 + 2 * sm.G) * EDDY_MAX_MP);
-    q.rep.assign(rep.begin() + im.soff, rep.begin() + im.soff + S);
-    q.lval.assign(lval.begin() + im.soff * 2, lval.begin() + im.soff * 2 + 2 * S);
-    q.info.assign(info.begin() + im.infooff, info.begin() + im.infooff + sm.c);
-    q.ev.resize(std::min(evc[k], im.evcap));
-    q.famkey.resize(fc[k]);
-    q.famscore.resize(2 * (size_t)fc[k]);
-    if (!q.ev.empty())
-      HIPCHK(ctx, hipMemcpyAsync(q.ev.data(), (Event *)ctx->b_ev.p + im.evoff, sizeof(Event) * q.ev.size(),
-                                 hipMemcpyDeviceToHost, st));
-    if (fc[k]) {
-      HIPCHK(ctx, hipMemcpyAsync(q.famkey.data(), (uint64_t *)ctx->b_famkey.p + im.foff, sizeof(uint64_t) * fc[k],
-                                 hipMemcpyDeviceToHost, st));
-      HIPCHK(ctx, hipMemcpyAsync(q.famscore.data(), (double *)ctx->b_famscore.p + im.foff * 2,
-                                 sizeof(double) * 2 * fc[k], hipMemcpyDeviceToHost, st));
-    }
+    const SetMeta &sm = q.meta[im.set];
+    const int set = im.set;
+    add((const uint32_t *)ctx->b_info.p + im.infooff, (uint32_t *)q.info.p + sm.pairbase, 4LL * sm.c);
+    add((const uint8_t *)ctx->b_lcnt.p + im.goff * 2, (uint8_t *)q.lcnt.p + sm.genebase * 2, 2LL * sm.G);
+    add((const uint16_t *)ctx->b_lfull.p + im.goff * 2 * EDDY_MAX_MP,
+        (uint16_t *)q.lfull.p + sm.genebase * 2 * EDDY_MAX_MP, 2LL * 2 * sm.G * EDDY_MAX_MP);
+    add((const int *)ctx->b_rep.p + im.soff, (int *)q.rep.p + (size_t)set * S, 4LL * S);
+    add((const double *)ctx->b_lval.p + im.soff * 2, (double *)q.lval.p + (size_t)set * 2 * S, 16LL * S);
+    const int ne = std::min(o.evc[k], im.evcap);
+    q.evoff[set] = q.evtot;
+    q.evn[set] = ne;
+    add((const Event *)ctx->b_ev.p + im.evoff, (Event *)q.ev.p + q.evtot, (long long)sizeof(Event) * ne);
+    q.evtot += ne;
+    q.famoff[set] = q.famtot;
+    q.famn[set] = o.fc[k];
+    add((const uint64_t *)ctx->b_famkey.p + im.foff, (uint64_t *)q.famkey.p + q.famtot, 8LL * o.fc[k]);
+    add((const double *)ctx->b_famscore.p + im.foff * 2, (double *)q.famscore.p + q.famtot * 2, 16LL * o.fc[k]);
+    q.famtot += o.fc[k];
+    q.U[set] = o.uq[k];
+    q.done[set] = 1;
+    ctx->last_p0[set] = Perm0();
   }
-  HIPCHK(ctx, hipStreamSynchronize(st));
+  if (!sl.empty()) {
+    CHK(grow(ctx, q.slices, sizeof(Slice) * sl.size()));
+    HIPCHK(ctx, hipMemcpyAsync(q.slices.p, sl.data(), sizeof(Slice) * sl.size(), hipMemcpyHostToDevice, ctx->stream));
+    k_copy_slices<<<(unsigned)std::min<size_t>(sl.size(), 2048), 256, 0, ctx->stream>>>((const Slice *)q.slices.p,
+                                                                                        (int)sl.size());
+    HIPCHK(ctx, hipGetLastError());
+  }
   ctx->hp.capture += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hc0).count();
   return EDDY_OK;
 }
 
+// Host copy of one set's permutation-0 state (on first request).
+int p0_fetch(eddy_ctx *ctx, int set) {
+  P0Store &q = ctx->p0;
+  if (set < 0 || set >= (int)q.done.size() || !q.done[set]) return EDDY_ERR_STATE;
+  Perm0 &h = ctx->last_p0[set];
+  if (!h.rep.empty()) return EDDY_OK;
+  const SetMeta &sm = q.meta[set];
+  const int S = ctx->N + 2;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  h.U = q.U[set];
+  h.info.resize(sm.c);
+  h.lcnt.resize(2 * sm.G);
+  h.lfull.resize((size_t)2 * sm.G * EDDY_MAX_MP);
+  h.rep.resize(S);
+  h.lval.resize(2 * S);
+  h.ev.resize(q.evn[set]);
+  h.famkey.resize(q.famn[set]);
+  h.famscore.resize(2 * q.famn[set]);
+  auto get = [&](void *dst, const void *src, size_t bytes) -> int {
+    if (bytes) HIPCHK(ctx, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return EDDY_OK;
+  };
+  CHK(get(h.info.data(), (const uint32_t *)q.info.p + sm.pairbase, 4 * (size_t)sm.c));
+  CHK(get(h.lcnt.data(), (const uint8_t *)q.lcnt.p + sm.genebase * 2, 2 * (size_t)sm.G));
+  CHK(get(h.lfull.data(), (const uint16_t *)q.lfull.p + sm.genebase * 2 * EDDY_MAX_MP, 2 * h.lfull.size()));
+  CHK(get(h.rep.data(), (const int *)q.rep.p + (size_t)set * S, 4 * (size_t)S));
+  CHK(get(h.lval.data(), (const double *)q.lval.p + (size_t)set * 2 * S, 16 * (size_t)S));
+  CHK(get(h.ev.data(), (const Event *)q.ev.p + q.evoff[set], sizeof(Event) * h.ev.size()));
+  CHK(get(h.famkey.data(), (const uint64_t *)q.famkey.p + q.famoff[set], 8 * h.famkey.size()));
+  CHK(get(h.famscore.data(), (const double *)q.famscore.p + q.famoff[set] * 2, 8 * h.famscore.size()));
+  return EDDY_OK;
+}
+
+// Items of a chunk through the kernels; an item whose events or families
+// overflow its capacity, or whose network hashes collide, runs again alone
+// with its exact count and a new hash seed.  Finished permutation-0 items are
+// captured when `capture`.
 int run_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, const eddy_params *p, int perms,
-              bool capture, std::vector<Perm0> *p0, std::vector<long long> &tests_out) {
+              bool capture, std::vector<long long> &tests_out) {
   const int N = ctx->N, S = N + 2;
   const int nitems = (int)ch.items.size();
   if (nitems == 0) return EDDY_OK;
-  // first guess of the events per item (rows of leave-one-out networks that
-  // differ from the full network); an item that overflows is re-run alone
-  // with its exact count
   std::vector<int> cap(nitems);
   for (int i = 0; i < nitems; i++) cap[i] = std::max(512, 2 * S + 24 * sets[ch.items[i].set].G);
   std::vector<int> todo(nitems);
@@ -580,28 +687,27 @@ int run_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, const 
     }
     std::vector<int> subcap(todo.size());
     for (size_t k = 0; k < todo.size(); k++) subcap[k] = cap[todo[k]];
-    std::vector<int> status, evc;
-    std::vector<long long> tests;
-    CHK(launch_chunk(ctx, sets, *cur, p, perms, subcap, attempt, status, evc, tests));
+    ChunkOut o;
+    CHK(launch_chunk(ctx, sets, *cur, p, perms, subcap, attempt, o));
     std::vector<int> next, done_idx;
     for (size_t k = 0; k < todo.size(); k++) {
       const int i = todo[k];
-      if (status[k] & ST_NOT_EVALUATED) {
+      if (o.status[k] & ST_NOT_EVALUATED) {
         ctx->err = "internal: a row read a leave-one-out test that was not evaluated";
         return EDDY_ERR_STATE;
       }
-      if (status[k] & (ST_EV_OVERFLOW | ST_FAM_OVERFLOW | ST_COLLISION)) {
-        if (status[k] & ST_EV_OVERFLOW) cap[i] = evc[k] + evc[k] / 8 + 64;
+      if (o.status[k] & (ST_EV_OVERFLOW | ST_FAM_OVERFLOW | ST_COLLISION)) {
+        if (o.status[k] & ST_EV_OVERFLOW) cap[i] = o.evc[k] + o.evc[k] / 8 + 64;
         next.push_back(i);
         continue;
       }
-      tests_out[i] = tests[k];
-      ctx->timing.tests += tests[k];
+      tests_out[i] = o.tests[k];
+      ctx->timing.tests += o.tests[k];
       ctx->timing.items += 1;
       ctx->timing.pairs += sets[ch.items[i].set].c;
-      if (capture) done_idx.push_back((int)k);
+      if (capture && ch.items[i].perm == 0) done_idx.push_back((int)k);
     }
-    if (capture && !done_idx.empty()) CHK(capture_items(ctx, sets, *cur, done_idx, evc, p0));
+    if (!done_idx.empty()) CHK(capture_items(ctx, *cur, done_idx, o));
     todo.swap(next);
   }
   if (!todo.empty()) {
@@ -718,9 +824,146 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
 }  // namespace
 
 // ---------------------------------------------------------------------------
-extern "C" int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *p,
-                             eddy_set_result *results) {
+static int mlp_train_dev(eddy_ctx *ctx, const float *dX, const float *dY, long long n, long long max_rows,
+                         int epochs, int batch, float lr, uint64_t seed);
+static int mlp_eval_loss(eddy_ctx *ctx, const float *dX, const float *dY, long long n, float *out);
+
+namespace {
+
+// Replays the permutation stream on its own thread in the reference's order
+// (main.cu:538-573): sets in order, permutation 0's draws, then, unless the
+// set stops after permutation 0, the draws of permutations 1..r-1, whose
+// class-2 membership bits it stores.  A set stops after permutation 0 when
+// its JS there is NaN (main.cu:1400-1460), which is only known once that
+// permutation ran; the producer takes it from `stop0` (a set of fewer than
+// two genes always stops) and records the stream state after each set's
+// permutation-0 draws, so a set found to stop restarts the replay there.
+struct Producer {
+  eddy_ctx *ctx = nullptr;
+  int nsets = 0, perms = 0, N = 0, n1 = 0;
+  std::vector<char> stop0;
+  std::vector<std::vector<uint32_t>> lab;
+  std::vector<PermStream::State> after0;
+  std::atomic<int> ready{0};
+  std::atomic<bool> quit{false};
+  std::thread th;
+  double ms = 0;
+
+  void start(int from) {
+    quit.store(false);
+    ready.store(from);
+    th = std::thread([this, from]() {
+      auto h0 = std::chrono::steady_clock::now();
+      std::vector<int> ord;
+      const int words = (N + 31) / 32;
+      PermStream &ps = ctx->stream_gen;
+      for (int s = from; s < nsets; s++) {
+        if (quit.load(std::memory_order_relaxed)) break;
+        ps.permutation(N, ord);  // permutation 0's draws
+        after0[s] = ps.save();
+        if (!stop0[s]) {
+          std::vector<uint32_t> &L = lab[s];
+          L.assign((size_t)(perms - 1) * words, 0u);
+          for (int pm = 1; pm < perms; pm++) {
+            ps.permutation(N, ord);
+            uint32_t *w = &L[(size_t)(pm - 1) * words];
+            for (int i = n1; i < N; i++) w[ord[i] >> 5] |= 1u << (ord[i] & 31);  // class 2 (main.cu:561-570)
+          }
+        }
+        ready.store(s + 1, std::memory_order_release);
+      }
+      ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    });
+  }
+  void stop() {
+    quit.store(true);
+    if (th.joinable()) th.join();
+  }
+  ~Producer() { stop(); }
+};
+
+// Pooled-table features of every pair of every set (one permutation-0 item
+// per set through the count kernel, no tests) into b_feat at pairbase.
+int compute_features(eddy_ctx *ctx, const std::vector<SetMeta> &meta, const std::vector<uint32_t> &truebits) {
+  const int N = ctx->N, nsets = (int)meta.size();
+  const long long budget = 64LL << 20;
+  Chunk ch;
+  auto launch = [&]() -> int {
+    if (ch.items.empty()) return EDDY_OK;
+    GROW(b_items, sizeof(ItemMeta) * ch.items.size());
+    GROW(b_labels, sizeof(uint32_t) * ch.labels.size());
+    GROW(b_tiles, sizeof(uint2) * ch.tiles.size());
+    hipStream_t st = ctx->stream;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->b_items.p, ch.items.data(), sizeof(ItemMeta) * ch.items.size(),
+                               hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->b_labels.p, ch.labels.data(), sizeof(uint32_t) * ch.labels.size(),
+                               hipMemcpyHostToDevice, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->b_tiles.p, ch.tiles.data(), sizeof(uint2) * ch.tiles.size(),
+                               hipMemcpyHostToDevice, st));
+    KArgs a{};
+    a.N = N;
+    a.n1 = ctx->n1;
+    a.n2 = ctx->n2;
+    a.S = N + 2;
+    a.feat_only = 1;
+    a.sets = (const SetMeta *)ctx->b_sets.p;
+    a.xs = (const uint8_t *)ctx->b_xs.p;
+    a.prior = (const uint32_t *)ctx->b_prior.p;
+    a.items = (const ItemMeta *)ctx->b_items.p;
+    a.labels = (const uint32_t *)ctx->b_labels.p;
+    a.tiles = (const uint32_t *)ctx->b_tiles.p;
+    a.feat = (float *)ctx->b_feat.p;
+    k_count_test<<<(unsigned)ch.tiles.size(), 256, 0, st>>>(a);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    ch = Chunk();
+    return EDDY_OK;
+  };
+  for (int s = 0; s < nsets; s++) {
+    if (meta[s].G < 2) continue;
+    add_item(ch, meta, s, 0, truebits, N);
+    if (ch.pairs > budget) CHK(launch());
+  }
+  return launch();
+}
+
+// Window of every row of sets [from, nsets): network scores of the pairs,
+// then the first column where their running sum reaches mp + margin.
+int compute_windows(eddy_ctx *ctx, const std::vector<SetMeta> &meta, int from, int mp) {
+  const int nsets = (int)meta.size();
+  if (from >= nsets) return EDDY_OK;
+  long long sumc = 0, sumg = 0;
+  for (const SetMeta &m : meta) {
+    sumc += m.c;
+    sumg += m.G;
+  }
+  GROW(b_score, sizeof(float) * std::max(1LL, sumc));
+  GROW(b_win, sizeof(int) * std::max(1LL, sumg));
+  const long long r0 = meta[from].pairbase, nr = sumc - r0;
+  if (nr > INT32_MAX) {
+    ctx->err = "too many pairs for one scoring launch";
+    return EDDY_ERR_CAPACITY;
+  }
+  HIPCHK(ctx, mlp_forward((const float *)ctx->b_feat.p + r0 * MLP_IN, (int)nr, (const float *)ctx->b_mlp.p,
+                          (float *)ctx->b_score.p + r0, ctx->stream));
+  k_window<<<nsets - from, 256, 0, ctx->stream>>>((const SetMeta *)ctx->b_sets.p + from, nsets - from,
+                                                 (const float *)ctx->b_score.p, mp, ctx->mlp_kappa,
+                                                 (int *)ctx->b_win.p);
+  HIPCHK(ctx, hipGetLastError());
+  return EDDY_OK;
+}
+
+}  // namespace
+
+// Run all permutations of all sets.  Batched: chunks of (set, permutation)
+// items in the reference's set order, the stream replayed ahead on the
+// producer thread; with `lc` the first lc->train_sets sets log their full-test
+// outcomes, the network is trained from them on the GPU, and the other sets
+// run with its windows (the learned job).
+static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *p,
+                   const eddy_learn_cfg *lc, eddy_learn_report *rep, eddy_set_result *results) {
   if (!ctx || (nsets && (!sets || !results)) || nsets < 0) return EDDY_ERR_ARG;
+  if (ctx->device < 0) return EDDY_ERR_NODEV;
   if (!ctx->dX) return EDDY_ERR_STATE;
   CHK(check_params(ctx, p));
   hipSetDevice(ctx->device);
@@ -730,167 +973,237 @@ extern "C" int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
   const int N = ctx->N, perms = p->perms, S = N + 2;
   std::vector<SetMeta> meta;
   CHK(prepare_sets(ctx, sets, nsets, p, meta));
+  CHK(p0_begin(ctx, meta));
   GROW(b_js, sizeof(double) * (size_t)std::max(1, nsets) * perms);
   HIPCHK(ctx, hipMemsetAsync(ctx->b_js.p, 0, sizeof(double) * (size_t)std::max(1, nsets) * perms, ctx->stream));
   GROW(b_counters, sizeof(unsigned long long) * 32);
   HIPCHK(ctx, hipMemsetAsync(ctx->b_counters.p, 0, sizeof(unsigned long long) * 32, ctx->stream));
   ctx->last_perms = perms;
   ctx->last_G.assign(nsets, 0);
+  for (int s = 0; s < nsets; s++) ctx->last_G[s] = meta[s].G;
   ctx->last_js.assign(nsets, std::vector<double>());
-  ctx->last_p0.assign(nsets, Perm0());
   std::vector<long long> tests_eval(nsets, 0);
-  std::vector<int> nan0(nsets, 0);
-  std::vector<int> ran(nsets, 1);
+  std::vector<int> nan0(nsets, 0), ran(nsets, 1);
   const std::vector<uint32_t> truebits = label_bits(ctx->classOf);
   const bool batched = p->mode == EDDY_MODE_BATCHED;
-  // budget of pairs per chunk (device memory ~ 40 B/pair + per-item scratch)
-  const long long pair_budget = 64LL << 20;
-
-  auto flush = [&](Chunk &ch, bool capture) -> int {
-    std::vector<long long> t(ch.items.size(), 0);
-    CHK(run_chunk(ctx, meta, ch, p, perms, capture, &ctx->last_p0, t));
-    for (size_t i = 0; i < ch.items.size(); i++) tests_eval[ch.items[i].set] += t[i];
-    ch = Chunk();
-    return EDDY_OK;
-  };
-
-  std::vector<int> order;
-  long long sumc = 0, sumg = 0;
-  for (const SetMeta &m : meta) {
-    sumc += m.c;
-    sumg += m.G;
-  }
-  const bool learned = batched && p->learned == 1 && p->schedule == EDDY_TESTS_CUTOFF;
-  const bool collect = batched && p->learned == 2;  // training-data collection (eddy_learn_collect)
+  long long sumc = 0;
+  for (const SetMeta &m : meta) sumc += m.c;
   ctx->k_feat = nullptr;
   ctx->k_labhits = nullptr;
   ctx->k_win = nullptr;
-  if (learned && !ctx->mlp_ready) {
-    ctx->err = "learned schedule needs a trained network (eddy_mlp_train or eddy_mlp_set_params)";
-    return EDDY_ERR_STATE;
+  eddy_learn_report r{};
+
+  // which sets log labels, which run with windows
+  int collect_upto = 0, win_from = nsets;
+  if (batched && lc) {
+    collect_upto = std::min(lc->train_sets, nsets);
+    win_from = collect_upto;
+  } else if (batched && p->learned == 2) {
+    collect_upto = nsets;
+  } else if (batched && p->learned == 1 && p->schedule == EDDY_TESTS_CUTOFF) {
+    if (!ctx->mlp_ready) {
+      ctx->err = "learned schedule needs a trained network (eddy_mlp_train or eddy_mlp_set_params)";
+      return EDDY_ERR_STATE;
+    }
+    win_from = 0;
   }
-  if (learned || collect) {
-    GROW(b_feat, sizeof(float) * 12 * std::max(1LL, sumc));
-    ctx->k_feat = (float *)ctx->b_feat.p;
+  if (p->schedule != EDDY_TESTS_CUTOFF) win_from = nsets;
+  if (collect_upto > 0 || win_from < nsets) {
+    GROW(b_feat, sizeof(float) * MLP_IN * std::max(1LL, sumc));
+    CHK(compute_features(ctx, meta, truebits));
   }
-  if (collect) {
+  if (collect_upto > 0) {
     GROW(b_labhits, sizeof(unsigned) * std::max(1LL, sumc));
     HIPCHK(ctx, hipMemsetAsync(ctx->b_labhits.p, 0, sizeof(unsigned) * std::max(1LL, sumc), ctx->stream));
-    ctx->k_labhits = (unsigned *)ctx->b_labhits.p;
   }
-  auto tp1 = std::chrono::steady_clock::now();
+  bool windows_ready = false;
+  if (win_from == 0 && nsets > 0) {
+    CHK(compute_windows(ctx, meta, 0, p->mp));
+    windows_ready = true;
+  }
+
   if (batched) {
-    // phase 1: permutation 0 (true labels) of every set
-    Chunk ch;
-    for (int s = 0; s < nsets; s++) {
-      ctx->last_G[s] = meta[s].G;
-      if (meta[s].G < 2) {  // a single (empty) network: likelihoods are NaN (main.cu:1345-1397)
-        nan0[s] = 1;
-        continue;
-      }
-      add_item(ch, meta, s, 0, truebits, N);
-      if (ch.pairs > pair_budget) CHK(flush(ch, true));
-    }
-    CHK(flush(ch, true));
-    ctx->timing.phase1_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp1).count();
-    if (learned && sumc > 0) {
-      // score every pair once per set from its permutation-0 features, then
-      // the per-row windows the count kernel uses for permutations > 0
-      GROW(b_score, sizeof(float) * sumc);
-      GROW(b_win, sizeof(int) * std::max(1LL, sumg));
-      HIPCHK(ctx, mlp_forward((const float *)ctx->b_feat.p, (int)sumc, (const float *)ctx->b_mlp.p,
-                              (float *)ctx->b_score.p, ctx->stream));
-      k_window<<<nsets, 256, 0, ctx->stream>>>((const SetMeta *)ctx->b_sets.p, nsets, (const float *)ctx->b_score.p,
-                                              p->mp, ctx->mlp_kappa, (int *)ctx->b_win.p);
-      HIPCHK(ctx, hipGetLastError());
-      ctx->k_win = (const int *)ctx->b_win.p;
-    }
-    ctx->k_feat = nullptr;
-    {
-      auto hj = std::chrono::steady_clock::now();
-      // JS of permutation 0 of every set: one strided copy
-      std::vector<double> js0(std::max(1, nsets));
-      if (nsets)
-        HIPCHK(ctx, hipMemcpy2D(js0.data(), sizeof(double), ctx->b_js.p, sizeof(double) * perms, sizeof(double), nsets,
-                                hipMemcpyDeviceToHost));
-      for (int s = 0; s < nsets; s++)
-        if (!nan0[s] && std::isnan(js0[s])) nan0[s] = 1;
-      ctx->hp.js0 += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hj).count();
-    }
-    // phase 2: a producer thread replays the stream in the reference's order
-    // (sets in order; a NaN-at-permutation-0 set consumes one permutation's
-    // draws) while this thread batches the items and runs the GPU chunks
-    std::vector<std::vector<uint32_t>> setlab(nsets);
-    std::atomic<int> ready{0};
-    double stream_ms = 0.0;
-    std::thread producer([&]() {
-      auto h0 = std::chrono::steady_clock::now();
-      std::vector<int> ord;
-      std::vector<uint8_t> cls(N);
-      const int words = (N + 31) / 32;
-      for (int s = 0; s < nsets; s++) {
-        ctx->stream_gen.permutation(N, ord);  // permutation 0's draws
-        if (!nan0[s]) {
-          std::vector<uint32_t> &L = setlab[s];
-          L.assign((size_t)(perms - 1) * words, 0u);
-          for (int pm = 1; pm < perms; pm++) {
-            ctx->stream_gen.permutation(N, ord);
-            uint32_t *w = &L[(size_t)(pm - 1) * words];
-            for (int i = ctx->n1; i < N; i++) w[ord[i] >> 5] |= 1u << (ord[i] & 31);  // class 2 (main.cu:561-570)
-          }
-        }
-        ready.store(s + 1, std::memory_order_release);
-      }
-      stream_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
-    });
-    int rc2 = EDDY_OK;
+    Producer prod;
+    prod.ctx = ctx;
+    prod.nsets = nsets;
+    prod.perms = perms;
+    prod.N = N;
+    prod.n1 = ctx->n1;
+    prod.stop0.assign(nsets, 0);
+    for (int s = 0; s < nsets; s++) prod.stop0[s] = meta[s].G < 2;
+    prod.lab.assign(nsets, std::vector<uint32_t>());
+    prod.after0.resize(nsets);
+    prod.start(0);
+    // chunks grow from 8M to 64M pairs: the GPU starts early, later
+    // launches amortise their fixed costs
+    long long budget = 8LL << 20;
+    const long long budget_max = 64LL << 20;
     const int words = (N + 31) / 32;
     std::vector<uint32_t> one(words);
-    for (int s = 0; s < nsets && rc2 == EDDY_OK; s++) {
+    Chunk ch;
+    int s = 0, chunk_first = 0;
+    bool trained = false;
+    // runs the chunk of sets [chunk_first, s); returns the first set whose
+    // permutation 0 gave NaN unexpectedly (the stream must restart after
+    // it), or -1
+    auto flush = [&](int &bad) -> int {
+      bad = -1;
+      if (ch.items.empty()) {
+        chunk_first = s;
+        return EDDY_OK;
+      }
+      ctx->k_labhits = chunk_first < collect_upto ? (unsigned *)ctx->b_labhits.p : nullptr;
+      ctx->k_win = (windows_ready && chunk_first >= win_from) ? (const int *)ctx->b_win.p : nullptr;
+      std::vector<long long> t(ch.items.size(), 0);
+      CHK(run_chunk(ctx, meta, ch, p, perms, true, t));
+      for (size_t i = 0; i < ch.items.size(); i++) tests_eval[ch.items[i].set] += t[i];
+      ctx->k_labhits = nullptr;
+      ctx->k_win = nullptr;
+      // JS of permutation 0 of the chunk's sets: one strided copy
+      auto hj = std::chrono::steady_clock::now();
+      const int n = s - chunk_first;
+      std::vector<double> js0(n);
+      HIPCHK(ctx, hipMemcpy2DAsync(js0.data(), sizeof(double), (double *)ctx->b_js.p + (size_t)chunk_first * perms,
+                                   sizeof(double) * perms, sizeof(double), n, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      for (int q = chunk_first; q < s; q++) {
+        if (meta[q].G < 2) continue;
+        if (std::isnan(js0[q - chunk_first])) {
+          nan0[q] = 1;
+          if (!prod.stop0[q] && bad < 0) bad = q;
+        }
+      }
+      ctx->hp.js0 += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hj).count();
+      ch = Chunk();
+      chunk_first = s;
+      budget = std::min(budget * 2, budget_max);
+      return EDDY_OK;
+    };
+    auto rollback = [&](int q, int last) -> int {
+      // set q stops after permutation 0: drop what its later permutations
+      // left, and replay the stream for the sets after it
+      prod.stop();
+      prod.stop0[q] = 1;
+      const size_t rest = (size_t)(perms - 1);
+      if (rest)
+        HIPCHK(ctx, hipMemsetAsync((double *)ctx->b_js.p + (size_t)q * perms + 1, 0, sizeof(double) * rest,
+                                   ctx->stream));
+      if (q < collect_upto) {
+        const long long b0 = meta[q].pairbase, b1 = meta[last].pairbase + meta[last].c;
+        HIPCHK(ctx, hipMemsetAsync((unsigned *)ctx->b_labhits.p + b0, 0, sizeof(unsigned) * (b1 - b0),
+                                   ctx->stream));
+      }
+      for (int k = q + 1; k <= last; k++) nan0[k] = meta[k].G < 2;
+      ctx->stream_gen.restore(prod.after0[q]);
+      for (int k = q + 1; k < nsets; k++) std::vector<uint32_t>().swap(prod.lab[k]);
+      s = q + 1;
+      chunk_first = s;
+      prod.start(q + 1);
+      return EDDY_OK;
+    };
+    while (s < nsets || !ch.items.empty()) {
+      const bool boundary = s == nsets || s == collect_upto || s == win_from;
+      if (!ch.items.empty() && (ch.pairs >= budget || boundary)) {
+        const int last = s - 1;
+        int bad;
+        CHK(flush(bad));
+        if (bad >= 0) CHK(rollback(bad, last));
+        continue;
+      }
+      if (s == nsets) break;
+      if (lc && s == collect_upto && !trained) {
+        // all logged sets are done: train, then windows for the rest
+        trained = true;
+        auto tt = std::chrono::steady_clock::now();
+        r.collect_ms = std::chrono::duration<double, std::milli>(tt - t0).count();
+        long long n = 0;
+        std::vector<long long> base(std::max(1, collect_upto));
+        std::vector<float> inv(std::max(1, collect_upto));
+        for (int q = 0; q < collect_upto; q++) {
+          base[q] = n;
+          n += meta[q].c;
+          inv[q] = (!nan0[q] && perms > 1) ? (float)(1.0 / (2.0 * (perms - 1))) : 0.f;
+          r.tests_collect += tests_eval[q];
+        }
+        r.train_rows = n;
+        if (n >= 64) {
+          GROW(b_my, sizeof(float) * n);
+          GROW(b_mlbase, sizeof(long long) * collect_upto);
+          GROW(b_mlinv, sizeof(float) * collect_upto);
+          HIPCHK(ctx, hipMemcpyAsync(ctx->b_mlbase.p, base.data(), sizeof(long long) * collect_upto,
+                                     hipMemcpyHostToDevice, ctx->stream));
+          HIPCHK(ctx, hipMemcpyAsync(ctx->b_mlinv.p, inv.data(), sizeof(float) * collect_upto, hipMemcpyHostToDevice,
+                                     ctx->stream));
+          HIPCHK(ctx, mlp_labels((const unsigned *)ctx->b_labhits.p, (const float *)ctx->b_mlinv.p,
+                                 (const long long *)ctx->b_mlbase.p, collect_upto, n, (float *)ctx->b_my.p,
+                                 ctx->stream));
+          CHK(eddy_mlp_init(ctx, lc->seed));
+          const float *fx = (const float *)ctx->b_feat.p, *fy = (const float *)ctx->b_my.p;
+          CHK(mlp_eval_loss(ctx, fx, fy, n, &r.loss_initial));
+          CHK(mlp_train_dev(ctx, fx, fy, n, lc->max_rows, lc->epochs, lc->batch, lc->lr, lc->seed));
+          CHK(mlp_eval_loss(ctx, fx, fy, n, &r.loss_final));
+          const float kappa = ctx->mlp_kappa;
+          ctx->mlp_kappa = lc->margin;
+          int rc = compute_windows(ctx, meta, collect_upto, p->mp);
+          ctx->mlp_kappa = kappa;
+          CHK(rc);
+          windows_ready = true;
+          r.trained = 1;
+        }
+        r.train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tt).count();
+      }
+      // next set's items
       auto hw = std::chrono::steady_clock::now();
-      while (ready.load(std::memory_order_acquire) <= s) std::this_thread::yield();
+      while (prod.ready.load(std::memory_order_acquire) <= s) std::this_thread::yield();
       auto hb = std::chrono::steady_clock::now();
       ctx->hp.wait += std::chrono::duration<double, std::milli>(hb - hw).count();
-      if (nan0[s]) continue;
-      ran[s] = perms;
-      for (int pm = 1; pm < perms; pm++) {
-        std::copy(setlab[s].begin() + (size_t)(pm - 1) * words, setlab[s].begin() + (size_t)pm * words, one.begin());
-        add_item(ch, meta, s, pm, one, N);
+      if (meta[s].G < 2) {
+        nan0[s] = 1;
+      } else {
+        add_item(ch, meta, s, 0, truebits, N);
+        if (!prod.stop0[s]) {
+          const std::vector<uint32_t> &L = prod.lab[s];
+          for (int pm = 1; pm < perms; pm++) {
+            std::copy(L.begin() + (size_t)(pm - 1) * words, L.begin() + (size_t)pm * words, one.begin());
+            add_item(ch, meta, s, pm, one, N);
+          }
+        }
+        std::vector<uint32_t>().swap(prod.lab[s]);
       }
-      std::vector<uint32_t>().swap(setlab[s]);
+      s++;
       ctx->hp.build += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hb).count();
-      if (ch.pairs > pair_budget) rc2 = flush(ch, false);
     }
-    producer.join();
-    ctx->timing.host_stream_ms += stream_ms;
-    CHK(rc2);
-    CHK(flush(ch, false));
+    prod.stop();
+    ctx->timing.host_stream_ms += prod.ms;
+    for (int q = 0; q < nsets; q++) ran[q] = (nan0[q] || prod.stop0[q]) ? 1 : perms;
   } else {
     // plain: one set and one permutation at a time, like main.cu:538-1460
-    for (int s = 0; s < nsets; s++) {
-      ctx->last_G[s] = meta[s].G;
+    std::vector<int> order;
+    for (int q = 0; q < nsets; q++) {
       for (int pm = 0; pm < perms; pm++) {
         ctx->stream_gen.permutation(N, order);
-        if (meta[s].G < 2) {
-          nan0[s] = 1;
+        if (meta[q].G < 2) {
+          nan0[q] = 1;
           break;
         }
         Chunk ch;
-        add_item(ch, meta, s, pm, pm == 0 ? truebits : label_bits(perm_labels(order, ctx->n1)), N);
-        CHK(flush(ch, pm == 0));
+        add_item(ch, meta, q, pm, pm == 0 ? truebits : label_bits(perm_labels(order, ctx->n1)), N);
+        std::vector<long long> t(1, 0);
+        CHK(run_chunk(ctx, meta, ch, p, perms, pm == 0, t));
+        tests_eval[q] += t[0];
         if (pm == 0) {
           double v;
-          HIPCHK(ctx, hipMemcpy(&v, (double *)ctx->b_js.p + (size_t)s * perms, sizeof(double), hipMemcpyDeviceToHost));
+          HIPCHK(ctx, hipMemcpy(&v, (double *)ctx->b_js.p + (size_t)q * perms, sizeof(double), hipMemcpyDeviceToHost));
           if (std::isnan(v)) {
-            nan0[s] = 1;
+            nan0[q] = 1;
             break;
           }
         }
-        ran[s] = pm + 1;
+        ran[q] = pm + 1;
       }
     }
   }
-  auto htail = std::chrono::steady_clock::now();
+  auto t3 = std::chrono::steady_clock::now();
   // p-values on the GPU
   GROW(b_nan0, sizeof(int) * std::max(1, nsets));
   GROW(b_pout, sizeof(double) * 6 * std::max(1, nsets));
@@ -920,25 +1233,22 @@ extern "C" int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
     HIPCHK(ctx, hipMemcpyAsync(alljs.data(), ctx->b_js.p, sizeof(double) * alljs.size(), hipMemcpyDeviceToHost,
                                ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  for (int s = 0; s < nsets; s++) {
-    eddy_set_result &r = results[s];
-    const double *o = &pout[(size_t)6 * s];
-    r.status = (int)o[0];
-    r.perms_run = nan0[s] ? 1 : ran[s];
-    r.U0 = ctx->last_p0[s].U;
-    r.js0 = nan0[s] ? -999.0 : alljs[(size_t)s * perms];
-    r.pDDN = o[2];
-    r.alpha = o[3];
-    r.beta = o[4];
-    long long c = meta[s].c;
-    r.tests_ref = (long long)r.perms_run * S * c;
-    r.tests_eval = tests_eval[s];
-    ctx->last_js[s].assign(alljs.begin() + (size_t)s * perms, alljs.begin() + (size_t)(s + 1) * perms);
-    if (nan0[s]) ctx->last_js[s][0] = -999.0;
+  for (int q = 0; q < nsets; q++) {
+    eddy_set_result &res = results[q];
+    const double *o = &pout[(size_t)6 * q];
+    res.status = (int)o[0];
+    res.perms_run = nan0[q] ? 1 : ran[q];
+    res.U0 = ctx->p0.U[q];
+    res.js0 = nan0[q] ? -999.0 : alljs[(size_t)q * perms];
+    res.pDDN = o[2];
+    res.alpha = o[3];
+    res.beta = o[4];
+    res.tests_ref = (long long)res.perms_run * S * meta[q].c;
+    res.tests_eval = tests_eval[q];
+    ctx->last_js[q].assign(alljs.begin() + (size_t)q * perms, alljs.begin() + (size_t)(q + 1) * perms);
+    if (nan0[q]) ctx->last_js[q][0] = -999.0;
+    if (q >= collect_upto && q >= win_from) r.tests_apply += tests_eval[q];
   }
-  ctx->k_win = nullptr;
-  ctx->k_labhits = nullptr;
-  ctx->k_feat = nullptr;
   unsigned long long cnt[32];
   HIPCHK(ctx, hipMemcpy(cnt, ctx->b_counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
   for (int k = 0; k < 4; k++) ctx->timing.sched[k] = (long long)cnt[k];
@@ -946,16 +1256,26 @@ extern "C" int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
   int wcr = 0;
   if (hipDeviceGetAttribute(&wcr, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess || wcr <= 0) wcr = 100000;
   for (int k = 0; k < 10; k++) ctx->timing.phase_ms[k] = (double)cnt[16 + k] / (double)wcr;  // kHz -> ms
-  ctx->timing.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  ctx->hp.tail = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - htail).count();
+  auto t4 = std::chrono::steady_clock::now();
+  ctx->timing.total_ms = std::chrono::duration<double, std::milli>(t4 - t0).count();
+  ctx->hp.tail = std::chrono::duration<double, std::milli>(t4 - t3).count();
+  if (lc) {
+    r.apply_ms = std::chrono::duration<double, std::milli>(t4 - t0).count() - r.collect_ms - r.train_ms;
+    if (rep) *rep = r;
+  }
   if (getenv("EDDY_PROFILE"))
     fprintf(stderr,
-            "[eddy] run_sets %d sets: total %.1f ms, phase1 %.1f | launch %.1f sync %.1f capture %.1f wait %.1f "
-            "build %.1f js0 %.1f tail %.1f | kernels %.1f %.1f %.1f %.1f\n",
-            nsets, ctx->timing.total_ms, ctx->timing.phase1_ms, ctx->hp.launch, ctx->hp.sync, ctx->hp.capture,
-            ctx->hp.wait, ctx->hp.build, ctx->hp.js0, ctx->hp.tail, ctx->timing.kernel_ms[0],
-            ctx->timing.kernel_ms[1], ctx->timing.kernel_ms[2], ctx->timing.kernel_ms[3]);
+            "[eddy] run %d sets: total %.1f ms | launch %.1f sync %.1f capture %.1f wait %.1f build %.1f js0 %.1f "
+            "tail %.1f | kernels %.1f %.1f %.1f %.1f | train %.1f\n",
+            nsets, ctx->timing.total_ms, ctx->hp.launch, ctx->hp.sync, ctx->hp.capture, ctx->hp.wait, ctx->hp.build,
+            ctx->hp.js0, ctx->hp.tail, ctx->timing.kernel_ms[0], ctx->timing.kernel_ms[1], ctx->timing.kernel_ms[2],
+            ctx->timing.kernel_ms[3], r.train_ms);
   return EDDY_OK;
+}
+
+extern "C" int eddy_run_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *p,
+                             eddy_set_result *results) {
+  return run_job(ctx, sets, nsets, p, nullptr, nullptr, results);
 }
 
 extern "C" int eddy_get_js(eddy_ctx *ctx, int set, double *out, int cap) {
@@ -972,9 +1292,9 @@ extern "C" int eddy_get_js(eddy_ctx *ctx, int set, double *out, int cap) {
 extern "C" int eddy_get_perm0_networks(eddy_ctx *ctx, int set, int *ids, int *nodes, int nodes_cap, int *epn,
                                        int *edges, int edges_cap, double *lval, int *n_edges) {
   if (!ctx || set < 0 || set >= (int)ctx->last_p0.size()) return EDDY_ERR_ARG;
+  CHK(p0_fetch(ctx, set));
   const Perm0 &q = ctx->last_p0[set];
   const int G = ctx->last_G[set], S = ctx->N + 2, n1 = ctx->n1;
-  if (q.rep.empty()) return EDDY_ERR_STATE;
   // events by network, sorted by row
   std::vector<std::vector<const Event *>> evs(S);
   for (const Event &e : q.ev) evs[e.s].push_back(&e);
@@ -1024,9 +1344,9 @@ extern "C" int eddy_get_perm0_networks(eddy_ctx *ctx, int set, int *ids, int *no
 
 extern "C" int eddy_get_edgelist(eddy_ctx *ctx, int set, int *nodes, int *epn, int *edges, int edges_cap) {
   if (!ctx || set < 0 || set >= (int)ctx->last_p0.size()) return EDDY_ERR_ARG;
-  const Perm0 &q = ctx->last_p0[set];
   const int G = ctx->last_G[set];
-  if (q.rep.empty() && G >= 2) return EDDY_ERR_STATE;
+  if (G >= 2) CHK(p0_fetch(ctx, set));
+  const Perm0 &q = ctx->last_p0[set];
   int ne = 0;
   if (epn) epn[0] = 0;
   for (int g = 0; g < 2; g++) {
@@ -1074,11 +1394,11 @@ extern "C" int eddy_run_perm_detail(eddy_ctx *ctx, const eddy_set_desc *set, con
   }
   std::vector<SetMeta> meta;
   CHK(prepare_sets(ctx, set, 1, p, meta));
+  CHK(p0_begin(ctx, meta));
   GROW(b_js, sizeof(double) * p->perms);
   HIPCHK(ctx, hipMemsetAsync(ctx->b_js.p, 0, sizeof(double) * p->perms, ctx->stream));
   GROW(b_counters, sizeof(unsigned long long) * 32);
   HIPCHK(ctx, hipMemsetAsync(ctx->b_counters.p, 0, sizeof(unsigned long long) * 32, ctx->stream));
-  ctx->last_p0.assign(1, Perm0());
   ctx->last_G.assign(1, meta[0].G);
   ctx->last_perms = p->perms;
   const int G = meta[0].G;
@@ -1090,9 +1410,10 @@ extern "C" int eddy_run_perm_detail(eddy_ctx *ctx, const eddy_set_desc *set, con
   std::vector<uint8_t> cls(labels, labels + N);
   add_item(ch, meta, 0, 0, label_bits(cls), N);
   std::vector<long long> t(1);
-  CHK(run_chunk(ctx, meta, ch, p, p->perms, true, &ctx->last_p0, t));
+  CHK(run_chunk(ctx, meta, ch, p, p->perms, true, t));
   double js;
   HIPCHK(ctx, hipMemcpy(&js, ctx->b_js.p, sizeof(double), hipMemcpyDeviceToHost));
+  CHK(p0_fetch(ctx, 0));
   const Perm0 &q = ctx->last_p0[0];
   out->js = js;
   out->U = q.U;
@@ -1180,7 +1501,43 @@ extern "C" int eddy_edge_decision_host(const int *tally9, int c, int prior, cons
   edge_params(&tmp, p, ep);
   eddy_edge_bounds eb;
   edge_bounds(p, c, eb);
-  return eddy_edge_decision(tally9, prior, &ep, &eb, (double)c);
+  return eddy_edge_decision_fast(tally9, prior, &ep, &eb, (double)c);
+}
+
+extern "C" int eddy_edge_loo_mask_host(const int *tally9, int c, int prior, int full, const eddy_params *p) {
+  if (!tally9 || !p) return -1;
+  eddy_edge_params ep;
+  eddy_ctx tmp;
+  tmp.lg15 = std::lgamma(1.5);
+  tmp.lg2 = std::lgamma(2.0);
+  tmp.lg3 = std::lgamma(3.0);
+  edge_params(&tmp, p, ep);
+  eddy_edge_bounds eb;
+  edge_bounds(p, c, eb);
+  int nt = 0;
+  return (int)eddy_loo_mask_fast(tally9, full, prior, &ep, &eb, (double)c, &nt);
+}
+
+extern "C" int eddy_chi2_host(const int *tally9, int fast, int *dof, double *chi2) {
+  if (!tally9 || !dof || !chi2) return EDDY_ERR_ARG;
+  if (!fast) {
+    *chi2 = eddy_chi2(tally9, dof);
+    return EDDY_OK;
+  }
+  int R[3], K[3];
+  for (int i = 0; i < 3; i++) {
+    R[i] = tally9[i * 3] + tally9[i * 3 + 1] + tally9[i * 3 + 2];
+    K[i] = tally9[i] + tally9[3 + i] + tally9[6 + i];
+  }
+  const int n = R[0] + R[1] + R[2];
+  if (n <= 0) return EDDY_ERR_ARG;
+  double iR[3], iK[3];
+  for (int i = 0; i < 3; i++) {
+    iR[i] = eddy_recip_or0(R[i]);
+    iK[i] = eddy_recip_or0(K[i]);
+  }
+  *chi2 = 2.0 * eddy_half_chi2_fast(tally9, R, K, n, iR, iK, 1.0 / (double)n, dof);
+  return EDDY_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -1261,11 +1618,10 @@ extern "C" int eddy_mlp_forward(eddy_ctx *ctx, const float *X, int n, float *pro
 }
 
 // gradient and loss over device rows (idx[b0 .. b0 + n) or rows b0 ..)
-static int mlp_grad_dev(eddy_ctx *ctx, const int *dIdx, int b0, int n) {
+static int mlp_grad_dev(eddy_ctx *ctx, const float *dX, const float *dY, const int *dIdx, int b0, int n) {
   GROW(b_mscr, sizeof(float) * mlp_part_floats(n));
-  HIPCHK(ctx, mlp_gradient((const float *)ctx->b_mx.p, (const float *)ctx->b_my.p, dIdx, b0, n,
-                           (const float *)ctx->b_mlp.p, (float *)ctx->b_mscr.p, (float *)ctx->b_mlpg.p,
-                           (float *)ctx->b_mlploss.p, ctx->stream));
+  HIPCHK(ctx, mlp_gradient(dX, dY, dIdx, b0, n, (const float *)ctx->b_mlp.p, (float *)ctx->b_mscr.p,
+                           (float *)ctx->b_mlpg.p, (float *)ctx->b_mlploss.p, ctx->stream));
   return EDDY_OK;
 }
 
@@ -1274,33 +1630,40 @@ extern "C" int eddy_mlp_gradient(eddy_ctx *ctx, const float *X, const float *Y, 
   if (!ctx->mlp_ready) return EDDY_ERR_STATE;
   hipSetDevice(ctx->device);
   CHK(mlp_upload(ctx, X, Y, n));
-  CHK(mlp_grad_dev(ctx, nullptr, 0, n));
+  CHK(mlp_grad_dev(ctx, (const float *)ctx->b_mx.p, (const float *)ctx->b_my.p, nullptr, 0, n));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   HIPCHK(ctx, hipMemcpy(grad, ctx->b_mlpg.p, sizeof(float) * MLP_NPARAMS, hipMemcpyDeviceToHost));
   if (loss) HIPCHK(ctx, hipMemcpy(loss, ctx->b_mlploss.p, sizeof(float), hipMemcpyDeviceToHost));
   return EDDY_OK;
 }
 
-// mean loss over (up to) the first 65536 training rows with the current network
-static int mlp_eval_loss(eddy_ctx *ctx, long long n, float *out) {
-  CHK(mlp_grad_dev(ctx, nullptr, 0, (int)std::min<long long>(n, 65536)));
+// mean loss over (up to) the first 65536 rows with the current network
+static int mlp_eval_loss(eddy_ctx *ctx, const float *dX, const float *dY, long long n, float *out) {
+  CHK(mlp_grad_dev(ctx, dX, dY, nullptr, 0, (int)std::min<long long>(n, 65536)));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   HIPCHK(ctx, hipMemcpy(out, ctx->b_mlploss.p, sizeof(float), hipMemcpyDeviceToHost));
   return EDDY_OK;
 }
 
-// Adam over shuffled mini-batches of the n rows already in b_mx / b_my; the
-// last partial batch of each epoch is dropped (n >= batch)
-static int mlp_train_dev(eddy_ctx *ctx, long long n, int epochs, int batch, float lr, uint64_t seed) {
-  batch = (int)std::min<long long>(batch, n);
-  GROW(b_mscr, sizeof(float) * mlp_part_floats(batch));
-  GROW(b_midx, sizeof(int) * n);
+// Adam over shuffled mini-batches of device rows dX/dY: all n rows, or a
+// uniform sample of max_rows of them (max_rows > 0); the last partial batch
+// of each epoch is dropped
+static int mlp_train_dev(eddy_ctx *ctx, const float *dX, const float *dY, long long n, long long max_rows,
+                         int epochs, int batch, float lr, uint64_t seed) {
   std::mt19937_64 rng(seed ^ 0xADA3ULL);
   // two host copies of the order: epoch e shuffles into h[e & 1] while the
   // GPU still runs epoch e - 1 (whose upload read h[(e - 1) & 1])
   std::vector<int> h[2];
   h[1].resize(n);
   for (long long i = 0; i < n; i++) h[1][i] = (int)i;
+  if (max_rows > 0 && max_rows < n) {  // partial Fisher-Yates: a uniform sample
+    for (long long i = 0; i < max_rows; i++) std::swap(h[1][i], h[1][i + (long long)(rng() % (uint64_t)(n - i))]);
+    h[1].resize(max_rows);
+    n = max_rows;
+  }
+  batch = (int)std::min<long long>(batch, n);
+  GROW(b_mscr, sizeof(float) * mlp_part_floats(batch));
+  GROW(b_midx, sizeof(int) * n);
   hipStream_t st = ctx->stream;
   for (int ep = 0; ep < epochs; ep++) {
     std::vector<int> &idx = h[ep & 1];
@@ -1310,10 +1673,9 @@ static int mlp_train_dev(eddy_ctx *ctx, long long n, int epochs, int batch, floa
     HIPCHK(ctx, hipMemcpyAsync(ctx->b_midx.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
     for (long long b0 = 0; b0 + batch <= n; b0 += batch) {
       ctx->mlp_step++;
-      HIPCHK(ctx, mlp_train_step((const float *)ctx->b_mx.p, (const float *)ctx->b_my.p, (const int *)ctx->b_midx.p,
-                                 (int)b0, batch, (float *)ctx->b_mlp.p, (float *)ctx->b_mscr.p,
-                                 (float *)ctx->b_mlpg.p, (float *)ctx->b_mlploss.p, (float *)ctx->b_mlpm.p,
-                                 (float *)ctx->b_mlpv.p, lr, ctx->mlp_step, st));
+      HIPCHK(ctx, mlp_train_step(dX, dY, (const int *)ctx->b_midx.p, (int)b0, batch, (float *)ctx->b_mlp.p,
+                                 (float *)ctx->b_mscr.p, (float *)ctx->b_mlpg.p, (float *)ctx->b_mlploss.p,
+                                 (float *)ctx->b_mlpm.p, (float *)ctx->b_mlpv.p, lr, ctx->mlp_step, st));
     }
   }
   HIPCHK(ctx, hipStreamSynchronize(st));
@@ -1327,35 +1689,9 @@ extern "C" int eddy_mlp_train(eddy_ctx *ctx, const float *X, const float *Y, lon
   hipSetDevice(ctx->device);
   if (!ctx->mlp_ready) CHK(eddy_mlp_init(ctx, seed));
   CHK(mlp_upload(ctx, X, Y, n));
-  CHK(mlp_train_dev(ctx, n, epochs, batch, lr, seed));
-  if (loss_out) CHK(mlp_eval_loss(ctx, n, loss_out));
-  return EDDY_OK;
-}
-
-// Labels of the sets of the last logged run: fraction of (permutation > 0,
-// class) full-test edges per pair, written to b_my; the features move to b_mx.
-static int mlp_take_logged(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_set_result *res,
-                           long long *n_out) {
-  std::vector<long long> base(std::max(1, nsets));
-  std::vector<float> inv(std::max(1, nsets));
-  long long n = 0;
-  for (int s = 0; s < nsets; s++) {
-    base[s] = n;
-    n += (long long)sets[s].n_genes * (sets[s].n_genes - 1) / 2;
-    inv[s] = res[s].perms_run > 1 ? (float)(1.0 / (2.0 * (res[s].perms_run - 1))) : 0.f;
-  }
-  *n_out = n;
-  if (n == 0) return EDDY_OK;
-  std::swap(ctx->b_mx, ctx->b_feat);
-  GROW(b_my, sizeof(float) * n);
-  GROW(b_mlbase, sizeof(long long) * nsets);
-  GROW(b_mlinv, sizeof(float) * nsets);
-  HIPCHK(ctx, hipMemcpyAsync(ctx->b_mlbase.p, base.data(), sizeof(long long) * nsets, hipMemcpyHostToDevice,
-                             ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(ctx->b_mlinv.p, inv.data(), sizeof(float) * nsets, hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(ctx, mlp_labels((const unsigned *)ctx->b_labhits.p, (const float *)ctx->b_mlinv.p,
-                         (const long long *)ctx->b_mlbase.p, nsets, n, (float *)ctx->b_my.p, ctx->stream));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  const float *dX = (const float *)ctx->b_mx.p, *dY = (const float *)ctx->b_my.p;
+  CHK(mlp_train_dev(ctx, dX, dY, n, 0, epochs, batch, lr, seed));
+  if (loss_out) CHK(mlp_eval_loss(ctx, dX, dY, n, loss_out));
   return EDDY_OK;
 }
 
@@ -1372,95 +1708,47 @@ extern "C" int eddy_learn_collect(eddy_ctx *ctx, const eddy_set_desc *sets, int 
   q.learned = 2;
   std::vector<eddy_set_result> res(std::max(1, nsets));
   CHK(eddy_run_sets(ctx, sets, nsets, &q, res.data()));
+  std::vector<long long> base(std::max(1, nsets));
+  std::vector<float> inv(std::max(1, nsets));
   long long n = 0;
-  CHK(mlp_take_logged(ctx, sets, nsets, res.data(), &n));
+  for (int s = 0; s < nsets; s++) {
+    base[s] = n;
+    n += (long long)sets[s].n_genes * (sets[s].n_genes - 1) / 2;
+    inv[s] = res[s].perms_run > 1 ? (float)(1.0 / (2.0 * (res[s].perms_run - 1))) : 0.f;
+  }
   *n_out = n;
   if (n > cap) return EDDY_ERR_CAPACITY;
   if (n == 0) return EDDY_OK;
-  HIPCHK(ctx, hipMemcpy(X, ctx->b_mx.p, sizeof(float) * MLP_IN * n, hipMemcpyDeviceToHost));
+  GROW(b_my, sizeof(float) * n);
+  GROW(b_mlbase, sizeof(long long) * nsets);
+  GROW(b_mlinv, sizeof(float) * nsets);
+  HIPCHK(ctx, hipMemcpy(ctx->b_mlbase.p, base.data(), sizeof(long long) * nsets, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(ctx->b_mlinv.p, inv.data(), sizeof(float) * nsets, hipMemcpyHostToDevice));
+  HIPCHK(ctx, mlp_labels((const unsigned *)ctx->b_labhits.p, (const float *)ctx->b_mlinv.p,
+                         (const long long *)ctx->b_mlbase.p, nsets, n, (float *)ctx->b_my.p, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipMemcpy(X, ctx->b_feat.p, sizeof(float) * MLP_IN * n, hipMemcpyDeviceToHost));
   HIPCHK(ctx, hipMemcpy(Y, ctx->b_my.p, sizeof(float) * n, hipMemcpyDeviceToHost));
   return EDDY_OK;
 }
 
-static void timing_add(eddy_timing &a, const eddy_timing &b) {
-  for (int k = 0; k < EDDY_NKERNELS; k++) {
-    a.kernel_ms[k] += b.kernel_ms[k];
-    a.launches[k] += b.launches[k];
-  }
-  a.items += b.items;
-  a.pairs += b.pairs;
-  a.tests += b.tests;
-  a.host_stream_ms += b.host_stream_ms;
-  for (int k = 0; k < 4; k++) {
-    a.sched[k] += b.sched[k];
-    a.stats[k] += b.stats[k];
-  }
-  a.phase1_ms += b.phase1_ms;
-  for (int k = 0; k < 10; k++) a.phase_ms[k] += b.phase_ms[k];
-  a.total_ms += b.total_ms;
-}
-
 // The learned job: the first cfg->train_sets sets run without the network
-// (every full test) with logging, the network is trained on the GPU from those logs, and
-// the remaining sets run the learned schedule.  Results, networks and the
-// permutation stream are exactly those of eddy_run_sets over all sets.
+// (every full test) with logging, the network is trained on the GPU from those
+// logs while the stream replay runs on, and the remaining sets run the learned
+// schedule.  Results, networks and the permutation stream are exactly those of
+// eddy_run_sets over all sets.
 extern "C" int eddy_run_learned(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *p,
                                 const eddy_learn_cfg *cfg, eddy_set_result *results, eddy_learn_report *rep) {
   if (!ctx || !p || !cfg || nsets < 0 || (nsets && (!sets || !results))) return EDDY_ERR_ARG;
-  if (cfg->train_sets < 0 || cfg->epochs < 0 || cfg->batch <= 0 || !(cfg->lr > 0) || !(cfg->margin >= 0))
+  if (cfg->train_sets < 0 || cfg->epochs < 0 || cfg->batch <= 0 || !(cfg->lr > 0) || !(cfg->margin >= 0) ||
+      cfg->max_rows < 0)
     return EDDY_ERR_ARG;
   if (ctx->device < 0) return EDDY_ERR_NODEV;
-  eddy_learn_report r{};
-  const int k = std::min(cfg->train_sets, nsets);
-  auto t0 = std::chrono::steady_clock::now();
-  eddy_params q = *p;
-  q.mode = EDDY_MODE_BATCHED;
-  q.schedule = EDDY_TESTS_CUTOFF;
-  q.learned = 2;
-  CHK(eddy_run_sets(ctx, sets, k, &q, results));
-  eddy_timing tcol = ctx->timing;
-  std::vector<int> G0 = std::move(ctx->last_G);
-  std::vector<std::vector<double>> js0 = std::move(ctx->last_js);
-  std::vector<Perm0> p00 = std::move(ctx->last_p0);
-  long long n = 0;
-  CHK(mlp_take_logged(ctx, sets, k, results, &n));
-  for (int s = 0; s < k; s++) r.tests_collect += results[s].tests_eval;
-  auto t1 = std::chrono::steady_clock::now();
-  r.train_rows = n;
-  const bool trained = n >= 64;
-  if (trained) {
-    CHK(eddy_mlp_init(ctx, cfg->seed));
-    CHK(mlp_eval_loss(ctx, n, &r.loss_initial));
-    CHK(mlp_train_dev(ctx, n, cfg->epochs, cfg->batch, cfg->lr, cfg->seed));
-    CHK(mlp_eval_loss(ctx, n, &r.loss_final));
-  }
-  auto t2 = std::chrono::steady_clock::now();
   eddy_params a = *p;
   a.mode = EDDY_MODE_BATCHED;
   a.schedule = EDDY_TESTS_CUTOFF;
-  a.learned = trained ? 1 : 0;
-  const float kappa = ctx->mlp_kappa;
-  ctx->mlp_kappa = cfg->margin;
-  int rc = eddy_run_sets(ctx, sets + k, nsets - k, &a, results + k);
-  ctx->mlp_kappa = kappa;
-  CHK(rc);
-  for (int s = k; s < nsets; s++) r.tests_apply += results[s].tests_eval;
-  auto t3 = std::chrono::steady_clock::now();
-  // one record over all sets, in set order
-  G0.insert(G0.end(), ctx->last_G.begin(), ctx->last_G.end());
-  for (auto &v : ctx->last_js) js0.push_back(std::move(v));
-  for (auto &v : ctx->last_p0) p00.push_back(std::move(v));
-  ctx->last_G = std::move(G0);
-  ctx->last_js = std::move(js0);
-  ctx->last_p0 = std::move(p00);
-  timing_add(ctx->timing, tcol);
-  ctx->timing.total_ms = std::chrono::duration<double, std::milli>(t3 - t0).count();
-  r.collect_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  r.train_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
-  r.apply_ms = std::chrono::duration<double, std::milli>(t3 - t2).count();
-  r.trained = trained ? 1 : 0;
-  if (rep) *rep = r;
-  return EDDY_OK;
+  a.learned = 0;
+  return run_job(ctx, sets, nsets, &a, cfg, rep, results);
 }
 
 extern "C" int eddy_mlp_set_margin(eddy_ctx *ctx, float kappa) {

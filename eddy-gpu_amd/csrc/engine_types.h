@@ -52,10 +52,18 @@ struct Event {        // a row of one leave-one-out network that differs from it
   uint16_t par[EDDY_MAX_MP];
 };
 
+// one device-to-device copy of the permutation-0 capture
+struct Slice {
+  const void *src;
+  void *dst;
+  long long bytes;
+};
+
 struct KArgs {
   // dataset-level
   int N, n1, n2, S;
   int mp, p_adjust, plain;
+  int feat_only;           // count kernel: pooled-table features only (no tests, no outputs)
   int ht_size;             // network hash table entries (power of two >= 2S)
   eddy_edge_params ep;     // thresholds, lgamma(p+1) constants, underflow-cut interval
   const double *lgtab;     // [82][lgstride]: lgamma(n + 4/d)

@@ -33,28 +33,14 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 // 420-431 with an exact shortcut of the AS 147 series)
 __device__ __forceinline__ int d_edge_test(const int *t, int prior, const KArgs &a, const SetMeta &sm,
                                            double cpairs) {
-  return eddy_edge_decision(t, prior, &a.ep, &sm.eb, cpairs);
+  return eddy_edge_decision_fast(t, prior, &a.ep, &sm.eb, cpairs);
 }
 
 // Leave-one-out decisions of one (pair, class): bit cell set when the tally
 // minus one count in `cell` is an edge; empty cells copy the full decision.
 __device__ __forceinline__ uint32_t d_loo_mask(const int *t, int full, int prior, const KArgs &a,
                                                const SetMeta &sm, double cpairs, int *ntests) {
-  uint32_t m = 0;
-  int tt[9];
-#pragma unroll
-  for (int i = 0; i < 9; i++) tt[i] = t[i];
-  for (int cell = 0; cell < 9; cell++) {
-    int bit = full;
-    if (t[cell] > 0) {
-      tt[cell] = t[cell] - 1;
-      bit = d_edge_test(tt, prior, a, sm, cpairs);
-      tt[cell] = t[cell];
-      (*ntests)++;
-    }
-    m |= (uint32_t)bit << cell;
-  }
-  return m;
+  return eddy_loo_mask_fast(t, full, prior, &a.ep, &sm.eb, cpairs, ntests);
 }
 
 __device__ __forceinline__ long long pair_index(int t, int col) { return (long long)t * (t - 1) / 2 + col; }
@@ -252,6 +238,7 @@ __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
     long long p = pair_index(gi, gj);
     int pr = prior_bit(a, sm, p);
     if (a.feat && im.perm == 0) pair_features(t1, t2, pr, sm.c, a.feat + (sm.pairbase + p) * 12);
+    if (a.feat_only) continue;
     uint32_t info = 0;
     int f1 = 0, f2 = 0;
     if (!a.win || im.perm == 0 || gj <= a.win[sm.genebase + gi]) {
@@ -277,7 +264,7 @@ __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
   }
   atomicAdd(&tcount, ntests);
   __syncthreads();
-  if (tid == 0) atomicAdd((unsigned long long *)&a.tests[item], (unsigned long long)tcount);
+  if (tid == 0 && a.tests) atomicAdd((unsigned long long *)&a.tests[item], (unsigned long long)tcount);
 }
 
 // --------------------------------------------------------------------------
@@ -324,6 +311,8 @@ __device__ __forceinline__ int full_of(const KArgs &a, const ItemMeta &im, const
   return f;
 }
 
+__device__ int block_exscan(int *v, int n, int *wsum);
+
 __global__ __launch_bounds__(256) void k_loo(KArgs a) {
   const int item = blockIdx.x;
   const ItemMeta im = a.items[item];
@@ -333,11 +322,40 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
   const int tid = threadIdx.x;
   __shared__ int plan[K2_MAXROWS];       // planned last column per (row, class) entry
   __shared__ int off[K2_MAXROWS + 1];    // exclusive prefix of entry counts
-  __shared__ int wsum[256];
+  __shared__ int wsum[257];
   __shared__ int tcount, nfb, nfbcols;
   if (tid == 0) tcount = nfb = nfbcols = 0;
   const int R = 2 * G;  // entries r = 2t + g
   int ntests0 = 0;
+  // ---- A0 (learned window): the count kernel ran the full tests of row t
+  // only up to column win[t]; a row whose window holds fewer full edges than
+  // the plan needs gets all of its later full tests here, packed over the
+  // item's rows (the tests the count kernel would have run)
+  const int *win = (a.win && im.perm > 0) ? a.win + sm.genebase : nullptr;
+  if (win) {
+    for (int r = tid; r < R; r += 256) {
+      const int t = r >> 1, g = r & 1;
+      const int want = mp + (a.plan_extra ? a.plan_extra[(im.goff * 2 + g * G + t)] : 0);
+      const int W = min(win[t], t - 1);
+      int hits = 0;
+      for (int col = 0; col <= W && hits < want; col++) hits += (a.info[im.infooff + pair_index(t, col)] >> g) & 1;
+      off[r] = hits < want ? t - 1 - W : 0;
+    }
+    __syncthreads();
+    const int ext = block_exscan(off, R, wsum);
+    for (int e = tid; e < ext; e += 256) {
+      int lo = 0, hi = R - 1;  // last r with off[r] <= e
+      while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= e) lo = mid; else hi = mid - 1;
+      }
+      const int t = lo >> 1, g = lo & 1;
+      const int col = min(win[t], t - 1) + 1 + (e - off[lo]);
+      const long long p = pair_index(t, col);
+      full_of(a, im, sm, p, g, a.info[im.infooff + p], cpairs, &ntests0);
+    }
+    __syncthreads();
+  }
   // ---- A: plan ----------------------------------------------------------
   for (int r = tid; r < R; r += 256) {
     const int t = r >> 1, g = r & 1;
@@ -1133,6 +1151,24 @@ __global__ void k_gather(const uint8_t *x, int N, int NP, const int *genes, cons
 // Learned window per (set, row): the first column at which the expected number
 // of full-test edges (the network's scores, summed in column order) reaches
 // mp + kappa; the whole row when it never does.  One block per set.
+// permutation-0 captures: one workgroup per slice at a time, 4-byte words
+// when source, destination and length allow it, else bytes
+__global__ __launch_bounds__(256) void k_copy_slices(const Slice *sl, int n) {
+  for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    const Slice q = sl[i];
+    const uintptr_t al = (uintptr_t)q.src | (uintptr_t)q.dst | (uintptr_t)q.bytes;
+    if ((al & 3) == 0) {
+      const uint32_t *s = (const uint32_t *)q.src;
+      uint32_t *d = (uint32_t *)q.dst;
+      for (long long k = threadIdx.x; k < q.bytes / 4; k += blockDim.x) d[k] = s[k];
+    } else {
+      const uint8_t *s = (const uint8_t *)q.src;
+      uint8_t *d = (uint8_t *)q.dst;
+      for (long long k = threadIdx.x; k < q.bytes; k += blockDim.x) d[k] = s[k];
+    }
+  }
+}
+
 __global__ void k_window(const SetMeta *sets, int nsets, const float *score, int mp, float kappa, int *win) {
   const int s = blockIdx.x;
   if (s >= nsets) return;

@@ -41,6 +41,20 @@ class PermStream {
   // next rand() value in [0, 2^31)
   inline uint32_t next() { return next_raw() >> 1; }
 
+  // The recurrence's whole state: the 34 values before the next one.
+  struct State {
+    uint32_t h[34];
+  };
+  State save() const {
+    State st;
+    for (int i = 0; i < kHist; i++) st.h[i] = buf_[pos_ - kHist + i];
+    return st;
+  }
+  void restore(const State &st) {
+    for (int i = 0; i < kHist; i++) buf_[i] = st.h[i];
+    pos_ = end_ = kHist;  // empty block: the next draw refills from the history
+  }
+
   // One permutation's consumption (main.cu:541-558): `n` discarded draws,
   // then rejection sampling of a permutation of [0, n).  order receives the
   // accepted values in acceptance order.  `rand() % n` is computed with an

@@ -27,6 +27,7 @@ EXPORTS = [
     "eddy_edge_decision_host", "eddy_stream_permutation",
     "eddy_mlp_init", "eddy_mlp_get_params", "eddy_mlp_set_params", "eddy_mlp_set_margin", "eddy_mlp_forward",
     "eddy_mlp_gradient", "eddy_mlp_train", "eddy_learn_collect", "eddy_run_learned",
+    "eddy_edge_loo_mask_host", "eddy_chi2_host",
 ]
 MLP_FEATURES = 12
 MLP_NPARAMS = 5057
@@ -68,7 +69,7 @@ class Timing(C.Structure):
 
 class LearnCfg(C.Structure):
     _fields_ = [("train_sets", C.c_int), ("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_float),
-                ("margin", C.c_float), ("seed", C.c_uint64)]
+                ("margin", C.c_float), ("seed", C.c_uint64), ("max_rows", C.c_longlong)]
 
 
 class LearnReport(C.Structure):
@@ -77,8 +78,8 @@ class LearnReport(C.Structure):
                 ("tests_collect", C.c_longlong), ("tests_apply", C.c_longlong), ("trained", C.c_int)]
 
 
-def learn_cfg(train_sets=100, epochs=4, batch=4096, lr=3e-3, margin=1.0, seed=1):
-    return LearnCfg(train_sets, epochs, batch, lr, margin, seed)
+def learn_cfg(train_sets=100, epochs=3, batch=8192, lr=3e-3, margin=1.0, seed=1, max_rows=131072):
+    return LearnCfg(train_sets, epochs, batch, lr, margin, seed, max_rows)
 
 
 class SynthCfg(C.Structure):
@@ -122,6 +123,8 @@ def lib():
                                       C.c_longlong]
         L.eddy_stream_permutation.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
         L.eddy_edge_decision_host.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(Params)]
+        L.eddy_edge_loo_mask_host.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(Params)]
+        L.eddy_chi2_host.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.eddy_mlp_init.argtypes = [C.c_void_p, C.c_uint64]
         L.eddy_mlp_get_params.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         L.eddy_mlp_set_params.argtypes = [C.c_void_p, C.c_void_p, C.c_int]

@@ -201,6 +201,15 @@ int eddy_get_timing(eddy_ctx *ctx, eddy_timing *t);
  * (tally9[r*3+k], r = row-gene state): 1 edge, 0 none.  Tests compare it
  * with the CPU oracle; it runs on the CPU and needs no context. */
 int eddy_edge_decision_host(const int *tally9, int c, int prior, const eddy_params *p);
+/* Leave-one-out decision mask of one 3x3 class tally (bit k: the tally minus
+ * one count in cell k; empty cells copy `full`), as the kernels compute it
+ * (kernel1.cu:236-419 tests each leave-one-out tally in full); host build,
+ * for tests. */
+int eddy_edge_loo_mask_host(const int *tally9, int c, int prior, int full, const eddy_params *p);
+/* Chi-square and dof of a 3x3 tally: the reference's formula (fast = 0,
+ * kernel1.cu:121-203) or the kernels' reciprocal form (fast = 1); host build,
+ * for tests. */
+int eddy_chi2_host(const int *tally9, int fast, int *dof, double *chi2);
 
 /* ---- learned proposal ------------------------------------------------------
  * A 12 -> 64 -> 64 -> 1 network (ReLU, ReLU, sigmoid; FP32) scores each gene
@@ -233,9 +242,10 @@ int eddy_learn_collect(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, cons
                        float *Y, long long cap, long long *n_out);
 
 /* The learned job: the first train_sets sets run without the network (every
- * full test) and their outcomes are logged; the network is trained on the GPU (Adam, `epochs`
- * passes of `batch`-row mini-batches, learning rate lr, initialised from
- * seed); the other sets run the learned schedule with window margin `margin`.
+ * full test) and their outcomes are logged; the network is trained on the GPU
+ * (Adam, `epochs` passes of `batch`-row mini-batches over a uniform sample of
+ * max_rows logged rows (0: all), learning rate lr, initialised from seed);
+ * the other sets run the learned schedule with window margin `margin`.
  * Results, networks and the permutation stream equal eddy_run_sets over all
  * sets. */
 typedef struct {
@@ -245,6 +255,7 @@ typedef struct {
   float lr;
   float margin;
   uint64_t seed;
+  long long max_rows;
 } eddy_learn_cfg;
 
 typedef struct {

@@ -81,3 +81,97 @@ def test_unusual_thresholds_fall_back_to_exact():
             t = rng.multinomial(int(rng.integers(10, 400)), rng.dirichlet(np.ones(9)))
             pr = int(rng.integers(0, 2))
             assert engine_decision(t, 100, pr, ep) == O.edge_decision(t, 100, pr, op)
+
+
+def engine_chi2(t, fast):
+    t = np.ascontiguousarray(np.asarray(t, np.int32).reshape(9))
+    dof = C.c_int()
+    x = C.c_double()
+    rc = eddy_amd.lib().eddy_chi2_host(t.ctypes.data_as(C.c_void_p), fast, C.byref(dof), C.byref(x))
+    assert rc == 0
+    return x.value, dof.value
+
+
+def _random_tables(rng, count, nmax=2046):
+    for _ in range(count):
+        n = int(rng.integers(2, nmax + 1))
+        kind = rng.integers(0, 4)
+        if kind == 0:
+            w = rng.dirichlet(np.ones(9))
+        elif kind == 1:
+            w = np.eye(3).reshape(9) * rng.uniform(1, 20) + rng.dirichlet(np.ones(9))
+        elif kind == 2:
+            w = rng.dirichlet(np.ones(9))
+            w.reshape(3, 3)[rng.integers(0, 3), :] = 0
+        else:  # very skewed margins: Yates' rule in play
+            w = rng.dirichlet(np.full(9, 0.2))
+        yield rng.multinomial(n, w / w.sum())
+
+
+def test_fast_chi2_error_bound():
+    """The reciprocal form of chi-square stays within 1e-10 (1 + x) of the
+    reference formula (x = chi2 / 2), the guard the fast decisions assume
+    with a factor 10 to spare; dof is identical."""
+    rng = np.random.default_rng(17)
+    worst = 0.0
+    for t in _random_tables(rng, 20000):
+        for k in [-1] + [int(i) for i in np.nonzero(t)[0]][:3]:
+            u = t.copy()
+            if k >= 0:
+                u[k] -= 1
+            if u.sum() == 0:
+                continue
+            xe, de = engine_chi2(u, 0)
+            xf, df = engine_chi2(u, 1)
+            assert de == df
+            worst = max(worst, abs(xf - xe) / 2 / (1 + xe / 2))
+    assert worst < 1e-10, worst
+
+
+def test_planted_yates_mistake_is_caught():
+    # a table whose Yates decision sits at E = R K / n = 5 exactly: a float
+    # comparison off by one rounding would flip Yates and change chi2 by O(1)
+    t = np.array([5, 5, 5, 5, 5, 5, 5, 5, 5])  # E = 15 * 15 / 45 = 5
+    xe, _ = engine_chi2(t, 0)
+    xf, _ = engine_chi2(t, 1)
+    assert xe == 0.0 and xf == 0.0
+    t = np.array([6, 4, 5, 4, 6, 5, 5, 5, 5])
+    xe, _ = engine_chi2(t, 0)
+    xf, _ = engine_chi2(t, 1)
+    assert abs(xe - xf) < 1e-12
+    # Yates applies only below 5: chi2 without the 0.5 correction is larger
+    d = np.array([[6, 4, 5], [4, 6, 5], [5, 5, 5]], float)
+    E = 5.0
+    assert abs(xe - ((np.abs(d - E)) ** 2 / E).sum()) < 1e-12
+
+
+@pytest.mark.parametrize("c", [45, 11175, 124750])
+def test_loo_mask_matches_oracle(c):
+    """Every bit of the leave-one-out mask equals the oracle's decision of
+    that leave-one-out tally (empty cells copy the full decision)."""
+    rng = np.random.default_rng(c + 1)
+    ep, op = params(1)
+    L = eddy_amd.lib()
+    checked = 0
+    tables = list(_random_tables(rng, 1500, 700))
+    # tallies just around the significance threshold, where the guard bands matter
+    for n in (30, 120, 600):
+        k = n // 3
+        a = k // 3
+        for shift in range(0, a):
+            tables.append(np.array([a + shift, a, a - shift, a, a + shift, a - shift, a - shift, a, a + shift]))
+    for t in tables:
+        pr = int(rng.integers(0, 2))
+        full = O.edge_decision(t, c, pr, op)
+        t32 = np.ascontiguousarray(t, np.int32)
+        m = L.eddy_edge_loo_mask_host(t32.ctypes.data_as(C.c_void_p), c, pr, full, C.byref(ep))
+        for k in range(9):
+            if t[k] > 0:
+                u = t.copy()
+                u[k] -= 1
+                want = O.edge_decision(u, c, pr, op)
+            else:
+                want = full
+            assert (m >> k) & 1 == want, (t, k, c, pr)
+            checked += 1
+    assert checked > 10000

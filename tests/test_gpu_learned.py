@@ -138,7 +138,7 @@ def test_learned_job_is_exact_and_saves_tests(engine):
     base_js = [engine.js(s).copy() for s in range(len(wl.sets))]
     base_net = [engine.perm0_networks(s, base[s].U0, prm.mp) for s in (0, 20, 47)]
     engine.reset_stream()
-    cfg = eddy_amd.learn_cfg(train_sets=16, epochs=4, batch=2048, lr=3e-3, margin=1.0, seed=1)
+    cfg = eddy_amd.learn_cfg(train_sets=16, epochs=4, batch=2048, lr=3e-3, margin=1.0, seed=1, max_rows=0)
     res, rep = engine.run_learned(wl.sets, prm, cfg)
     assert rep.trained == 1 and rep.train_rows > 1024
     assert rep.loss_final < rep.loss_initial

@@ -68,11 +68,17 @@ def test_sets_stream_parity(engine):
     sets = [(genes[:12], pairs[:0]), (genes[12:14], pairs[:0]), (genes[14:30], pairs[:0]), (genes[:1], pairs[:0]),
             (genes[30:40], pairs[:0])]
     data_ref = wl.X.astype(np.int32) - 1
-    for mode, sched in [(eddy_amd.MODE_BATCHED, eddy_amd.TESTS_CUTOFF), (eddy_amd.MODE_PLAIN, eddy_amd.TESTS_ALL)]:
+    runs = [(eddy_amd.MODE_BATCHED, eddy_amd.TESTS_CUTOFF, None), (eddy_amd.MODE_PLAIN, eddy_amd.TESTS_ALL, None),
+            (eddy_amd.MODE_BATCHED, eddy_amd.TESTS_CUTOFF, 2), (eddy_amd.MODE_BATCHED, eddy_amd.TESTS_CUTOFF, 3)]
+    for mode, sched, learn in runs:
         prm = wl.params(perms=8, mode=mode, schedule=sched)
-        print("case: stream parity mode", mode, flush=True)
+        print("case: stream parity mode", mode, "learned job" if learn else "", learn, flush=True)
         engine.reset_stream()
-        res = engine.run_sets(sets, prm)
+        if learn is None:
+            res = engine.run_sets(sets, prm)
+        else:  # the learned job: the NaN set falls in the logged part (2) or right after it (3)
+            res, rep = engine.run_learned(sets, prm, eddy_amd.learn_cfg(train_sets=learn, epochs=2, batch=64))
+            assert rep.trained == 1
         oracle_ref.reset_stream()
         for s, (g, _) in enumerate(sets):
             prior = np.zeros((len(g), len(g)), np.int32)
