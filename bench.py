@@ -139,7 +139,7 @@ def main():
     host_stream_ms = 0.0
     sched = [0, 0, 0, 0]
     stats = [0, 0, 0, 0]
-    phase = [0.0] * 10
+    phase = [0.0] * 16
     reps = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -154,7 +154,7 @@ def main():
         for k in range(4):
             sched[k] += tm.sched[k]
             stats[k] += tm.stats[k]
-        for k in range(10):
+        for k in range(16):
             phase[k] += tm.phase_ms[k]
         print("bench: step done, total %.2fs (collect %.0f ms, train %.0f ms, apply %.0f ms)"
               % (time.perf_counter() - t0, rep.collect_ms, rep.train_ms, rep.apply_ms), file=sys.stderr, flush=True)

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <map>
 #include <random>
 #include <thread>
 #include <chrono>
@@ -63,6 +64,12 @@ struct P0Store {
 
 }  // namespace
 
+struct BoundsKey {
+  int p_adjust = -1;
+  double aE = -1, aEP = -1;
+  bool operator==(const BoundsKey &o) const { return p_adjust == o.p_adjust && aE == o.aE && aEP == o.aEP; }
+};
+
 struct HostProf {  // host-side time split of one eddy_run_sets call (ms)
   double launch = 0, sync = 0, capture = 0, wait = 0, build = 0, js0 = 0, tail = 0;
 };
@@ -90,6 +97,8 @@ struct eddy_ctx {
   std::vector<std::vector<double>> last_js;
   std::vector<Perm0> last_p0;  // host copies, fetched from p0 on request
   P0Store p0;
+  BoundsKey bounds_key;
+  std::map<int, eddy_edge_bounds> bounds_cache;
   eddy_timing timing{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // learned proposal: network parameters + Adam state, per-run feature,
@@ -273,12 +282,18 @@ long double q_tail(int pi, long double x) {
 }
 
 // boundary x where the decreasing tail crosses target: returns (lo, hi)
-// bracketing it to long-double precision
+// bracketing it to long-double precision (bisection until the midpoint
+// equals an end)
 void tail_root(int pi, long double target, long double &lo, long double &hi) {
   lo = 0.0L;
-  hi = 4000.0L;
-  for (int i = 0; i < 400; i++) {
+  hi = 1.0L;
+  while (hi < 4096.0L && q_tail(pi, hi) > target) {
+    lo = hi;
+    hi *= 2.0L;
+  }
+  for (int i = 0; i < 256; i++) {
     long double mid = 0.5L * (lo + hi);
+    if (!(mid > lo && mid < hi)) break;
     if (q_tail(pi, mid) > target) lo = mid; else hi = mid;
   }
 }
@@ -299,8 +314,9 @@ void edge_params(eddy_ctx *ctx, const eddy_params *p, eddy_edge_params &ep) {
   for (int pi = 0; pi < 3; pi++) {
     // AS 147 underflow cut: p ln x - lgamma(p+1) - x = ln 1e-37, decreasing for x > p
     long double lo = kPs[pi] + 1.0L, hi = 4000.0L;
-    for (int i = 0; i < 400; i++) {
+    for (int i = 0; i < 256; i++) {
       long double mid = 0.5L * (lo + hi);
+      if (!(mid > lo && mid < hi)) break;
       long double g = (long double)kPs[pi] * logl(mid) - (long double)lg[pi] - mid - L;
       if (g > 0) lo = mid; else hi = mid;
     }
@@ -760,6 +776,32 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
   std::vector<long long> rowoff;
   std::vector<uint32_t> prior;
   long long xoff = 0, pairbase = 0, genebase = 0;
+  // certainty intervals per distinct pair count c, cached across runs with
+  // the same thresholds, new ones computed on several threads
+  {
+    BoundsKey bk{p->p_adjust, p->alpha_edge, p->alpha_edge_prior};
+    if (!(bk == ctx->bounds_key)) {
+      ctx->bounds_cache.clear();
+      ctx->bounds_key = bk;
+    }
+    std::vector<int> todo;
+    for (int s = 0; s < nsets; s++) {
+      const int c = sets[s].n_genes > 1 ? sets[s].n_genes * (sets[s].n_genes - 1) / 2 : 0;
+      if (!ctx->bounds_cache.count(c)) {
+        ctx->bounds_cache[c] = eddy_edge_bounds{};
+        todo.push_back(c);
+      }
+    }
+    std::vector<eddy_edge_bounds> out(todo.size());
+    const int nth = (int)std::min<size_t>(16, std::max<size_t>(1, todo.size() / 4));
+    std::vector<std::thread> pool;
+    for (int k = 0; k < nth; k++)
+      pool.emplace_back([&, k]() {
+        for (size_t i = k; i < todo.size(); i += nth) edge_bounds(p, todo[i], out[i]);
+      });
+    for (auto &t : pool) t.join();
+    for (size_t i = 0; i < todo.size(); i++) ctx->bounds_cache[todo[i]] = out[i];
+  }
   for (int s = 0; s < nsets; s++) {
     const eddy_set_desc &d = sets[s];
     if (d.n_genes < 0 || d.n_genes > EDDY_MAX_G || (d.n_genes && !d.genes) || d.n_prior < 0 ||
@@ -776,7 +818,7 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
     m.genebase = genebase;
     pairbase += m.c;
     genebase += m.G;
-    edge_bounds(p, m.c, m.eb);
+    m.eb = ctx->bounds_cache[m.c];
     for (int i = 0; i < d.n_genes; i++) {
       if (d.genes[i] < 0 || d.genes[i] >= ctx->genes) {
         ctx->err = "gene index out of range";
@@ -1021,6 +1063,8 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     CHK(compute_windows(ctx, meta, 0, p->mp));
     windows_ready = true;
   }
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->timing.phase1_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 
   if (batched) {
     Producer prod;
@@ -1255,7 +1299,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
   for (int k = 0; k < 4; k++) ctx->timing.stats[k] = (long long)cnt[4 + k];
   int wcr = 0;
   if (hipDeviceGetAttribute(&wcr, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess || wcr <= 0) wcr = 100000;
-  for (int k = 0; k < 10; k++) ctx->timing.phase_ms[k] = (double)cnt[16 + k] / (double)wcr;  // kHz -> ms
+  for (int k = 0; k < 16; k++) ctx->timing.phase_ms[k] = (double)cnt[16 + k] / (double)wcr;  // kHz -> ms
   auto t4 = std::chrono::steady_clock::now();
   ctx->timing.total_ms = std::chrono::duration<double, std::milli>(t4 - t0).count();
   ctx->hp.tail = std::chrono::duration<double, std::milli>(t4 - t3).count();
@@ -1265,9 +1309,9 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
   }
   if (getenv("EDDY_PROFILE"))
     fprintf(stderr,
-            "[eddy] run %d sets: total %.1f ms | launch %.1f sync %.1f capture %.1f wait %.1f build %.1f js0 %.1f "
-            "tail %.1f | kernels %.1f %.1f %.1f %.1f | train %.1f\n",
-            nsets, ctx->timing.total_ms, ctx->hp.launch, ctx->hp.sync, ctx->hp.capture, ctx->hp.wait, ctx->hp.build,
+            "[eddy] run %d sets: total %.1f ms | setup %.1f launch %.1f sync %.1f capture %.1f wait %.1f build %.1f "
+            "js0 %.1f tail %.1f | kernels %.1f %.1f %.1f %.1f | train %.1f\n",
+            nsets, ctx->timing.total_ms, ctx->timing.phase1_ms, ctx->hp.launch, ctx->hp.sync, ctx->hp.capture, ctx->hp.wait, ctx->hp.build,
             ctx->hp.js0, ctx->hp.tail, ctx->timing.kernel_ms[0], ctx->timing.kernel_ms[1], ctx->timing.kernel_ms[2],
             ctx->timing.kernel_ms[3], r.train_ms);
   return EDDY_OK;

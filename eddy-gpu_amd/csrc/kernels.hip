@@ -327,6 +327,8 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
   if (tid == 0) tcount = nfb = nfbcols = 0;
   const int R = 2 * G;  // entries r = 2t + g
   int ntests0 = 0;
+  unsigned long long tph = 0;
+  phase_mark(a, -1, tph);
   // ---- A0 (learned window): the count kernel ran the full tests of row t
   // only up to column win[t]; a row whose window holds fewer full edges than
   // the plan needs gets all of its later full tests here, packed over the
@@ -374,6 +376,7 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
     plan[r] = P;
   }
   __syncthreads();
+  phase_mark(a, 10, tph);
   // exclusive scan of the entry counts (P + 1 columns per entry)
   {
     const int per = (R + 255) / 256;
@@ -414,6 +417,7 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
     atomicOr(&a.info[im.infooff + p], (m << INFO_LOO_SHIFT(g)) | INFO_EVAL(g));
   }
   __syncthreads();
+  phase_mark(a, 11, tph);
   // ---- C: exact cutoff check, fallback past the plan ---------------------
   int fb = 0, fbc = 0;
   for (int r = tid; r < R; r += 256) {
@@ -438,6 +442,7 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
   atomicAdd(&nfb, fb);
   atomicAdd(&nfbcols, fbc);
   __syncthreads();
+  phase_mark(a, 12, tph);
   if (tid == 0) {
     atomicAdd((unsigned long long *)&a.tests[item], (unsigned long long)tcount);
     if (a.counters) {

@@ -189,11 +189,14 @@ typedef struct {
                                         (pair, class) entries, rows */
   long long stats[4];                /* BDeu histogram bytes read, families
                                         scored, unique networks, events */
-  double phase1_ms;                  /* host wall time of permutation 0 of every set */
-  double phase_ms[10];               /* workgroup time per phase summed over
+  double phase1_ms;                  /* host wall time before the first chunk
+                                        (set preparation, features, windows) */
+  double phase_ms[16];               /* workgroup time per phase summed over
                                         items: score 0 dedup, 1 confirm, 2
                                         unique list, 3 families, 4 BDeu, 5 sums,
-                                        6 likelihood/JS; rows 7 setup, 8 rows */
+                                        6 likelihood/JS; rows 7 setup, 8 rows;
+                                        loo 10 plan, 11 packed tests, 12 cutoff
+                                        check and fallback */
 } eddy_timing;
 int eddy_get_timing(eddy_ctx *ctx, eddy_timing *t);
 
