@@ -105,17 +105,22 @@ EDDY_HD int eddy_edge_exact(double chi, int dof, int prior, const eddy_edge_para
   return (pv < ep->aE || (prior == 1 && pv < ep->aEP)) ? 1 : 0;
 }
 
+// v[0], v[1] or v[2] by constant indices and selects (no dynamic indexing,
+// so a caller's local copy of the bounds stays in registers)
+EDDY_HD double eddy_pick3(const double *v, bool p1, bool p2) { return p1 ? v[0] : p2 ? v[1] : v[2]; }
+
 EDDY_HD int eddy_edge_decision(const int *t, int prior, const eddy_edge_params *ep, const eddy_edge_bounds *eb,
                                double cpairs) {
   int dof;
   double chi = eddy_chi2(t, &dof);
   if (ep->shortcut && (dof == 1 || dof == 2 || dof == 4)) {
-    const int pi = dof == 1 ? 0 : dof == 2 ? 1 : 2;
+    const bool p1 = dof == 1, p2 = dof == 2, pr = prior == 1;
     const double x = chi / 2.0;
-    const int pr = prior == 1 ? 1 : 0;
-    if (x <= eb->xlo[pr][pi]) return 0;
-    if (x > ep->xchi[pi]) return 0;
-    if (x >= eb->xhi[pr][pi] && x < ep->xclo[pi]) return 1;
+    const double xlo = pr ? eddy_pick3(eb->xlo[1], p1, p2) : eddy_pick3(eb->xlo[0], p1, p2);
+    const double xhi = pr ? eddy_pick3(eb->xhi[1], p1, p2) : eddy_pick3(eb->xhi[0], p1, p2);
+    if (x <= xlo) return 0;
+    if (x > eddy_pick3(ep->xchi, p1, p2)) return 0;
+    if (x >= xhi && x < eddy_pick3(ep->xclo, p1, p2)) return 1;
   }
   return eddy_edge_exact(chi, dof, prior, ep, cpairs);
 }
@@ -157,16 +162,22 @@ EDDY_HD double eddy_half_chi2_fast(const int *t, const int *R, const int *K, int
 // certain outcome of x (1 / 0) from the intervals shrunk by the guard, or -1
 EDDY_HD int eddy_edge_certain(double x, int dof, int prior, const eddy_edge_params *ep, const eddy_edge_bounds *eb) {
   if (!ep->shortcut || !(dof == 1 || dof == 2 || dof == 4)) return -1;
-  const int pi = dof == 1 ? 0 : dof == 2 ? 1 : 2;
-  const int pr = prior == 1 ? 1 : 0;
+  // constant indices and selects only: the bounds stay in registers when
+  // the caller holds them in a local copy
+  const bool p1 = dof == 1, p2 = dof == 2, pr = prior == 1;
+  const double xlo = pr ? eddy_pick3(eb->xlo[1], p1, p2) : eddy_pick3(eb->xlo[0], p1, p2);
+  const double xhi = pr ? eddy_pick3(eb->xhi[1], p1, p2) : eddy_pick3(eb->xhi[0], p1, p2);
+  const double xchi = eddy_pick3(ep->xchi, p1, p2);
+  const double xclo = eddy_pick3(ep->xclo, p1, p2);
   const double gd = 1e-9 * (1.0 + x);
-  if (x <= eb->xlo[pr][pi] - gd) return 0;
-  if (x > ep->xchi[pi] + gd) return 0;
-  if (x >= eb->xhi[pr][pi] + gd && x < ep->xclo[pi] - gd) return 1;
+  if (x <= xlo - gd) return 0;
+  if (x > xchi + gd) return 0;
+  if (x >= xhi + gd && x < xclo - gd) return 1;
   return -1;
 }
 
 EDDY_HD double eddy_recip_or0(int v) { return v ? 1.0 / (double)v : 0.0; }
+
 
 // eddy_edge_decision() with the fast x
 EDDY_HD int eddy_edge_decision_fast(const int *t, int prior, const eddy_edge_params *ep, const eddy_edge_bounds *eb,
@@ -210,31 +221,45 @@ EDDY_HD uint32_t eddy_loo_mask_fast(const int *t, int full, int prior, const edd
     iKm[i] = eddy_recip_or0(K[i] - 1);
   }
   const double in1 = eddy_recip_or0(n - 1);
-  uint32_t m = 0;
-  int tt[9];
-  for (int i = 0; i < 9; i++) tt[i] = t[i];
+  uint32_t m = 0, unsure = 0;
+  // unrolled: every array index below is a constant, so all tallies stay in
+  // registers
+#pragma unroll
   for (int cell = 0; cell < 9; cell++) {
-    int bit = full;
     if (t[cell] > 0) {
-      const int a = cell / 3, b = cell % 3;
-      tt[cell] = t[cell] - 1;
+      (*ntests)++;
       int c = -1;
       if (n > 1) {
-        int R1[3] = {R[0], R[1], R[2]}, K1[3] = {K[0], K[1], K[2]};
-        double iR1[3] = {iR[0], iR[1], iR[2]}, iK1[3] = {iK[0], iK[1], iK[2]};
-        R1[a] -= 1;
-        K1[b] -= 1;
-        iR1[a] = iRm[a];
-        iK1[b] = iKm[b];
+        const int a = cell / 3, b = cell % 3;
+        int u[9], R1[3], K1[3];
+        double iR1[3], iK1[3];
+#pragma unroll
+        for (int i = 0; i < 9; i++) u[i] = t[i] - (i == cell ? 1 : 0);
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          R1[i] = R[i] - (i == a ? 1 : 0);
+          K1[i] = K[i] - (i == b ? 1 : 0);
+          iR1[i] = i == a ? iRm[i] : iR[i];
+          iK1[i] = i == b ? iKm[i] : iK[i];
+        }
         int dof;
-        const double x = eddy_half_chi2_fast(tt, R1, K1, n - 1, iR1, iK1, in1, &dof);
+        const double x = eddy_half_chi2_fast(u, R1, K1, n - 1, iR1, iK1, in1, &dof);
         c = eddy_edge_certain(x, dof, prior, ep, eb);
       }
-      bit = c >= 0 ? c : eddy_edge_decision(tt, prior, ep, eb, cpairs);
-      tt[cell] = t[cell];
-      (*ntests)++;
+      if (c >= 0) m |= (uint32_t)c << cell;
+      else unsure |= 1u << cell;
+    } else {
+      m |= (uint32_t)full << cell;
     }
-    m |= (uint32_t)bit << cell;
+  }
+  // tallies within a guard band of a threshold: the exact decision
+  while (unsure) {
+    const int cell = __builtin_ctz(unsure);
+    unsure &= unsure - 1;
+    int u[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) u[i] = t[i] - (i == cell ? 1 : 0);
+    m |= (uint32_t)eddy_edge_decision(u, prior, ep, eb, cpairs) << cell;
   }
   return m;
 }

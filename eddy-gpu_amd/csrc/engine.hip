@@ -90,7 +90,7 @@ struct eddy_ctx {
   // device buffers, grown on demand
   DevBuf b_sets, b_xs, b_prior, b_items, b_labels, b_tiles, b_info, b_tab, b_lfull, b_lcnt, b_sflags, b_ev,
       b_evcount, b_hnet, b_rep, b_clsg, b_evorder, b_evfam, b_famkey, b_famscore, b_lval, b_uniq, b_js,
-      b_status, b_tests, b_genes, b_rowoff, b_nan0, b_pout, b_famcount, b_counters, b_famht, b_famhv;
+      b_status, b_tests, b_genes, b_rowoff, b_nan0, b_pout, b_famcount, b_counters, b_famht, b_famhv, b_bits;
   // last run
   int last_perms = 0;
   std::vector<int> last_G;
@@ -187,7 +187,7 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
                    &ctx->b_mlpg, &ctx->b_mlploss, &ctx->b_feat, &ctx->b_labhits, &ctx->b_score, &ctx->b_win,
                    &ctx->b_mx, &ctx->b_my, &ctx->b_mxb, &ctx->b_myb, &ctx->b_midx, &ctx->b_mscr, &ctx->b_mlbase,
                    &ctx->b_mlinv, &ctx->p0.info, &ctx->p0.lcnt, &ctx->p0.lfull, &ctx->p0.rep, &ctx->p0.lval,
-                   &ctx->p0.ev, &ctx->p0.famkey, &ctx->p0.famscore, &ctx->p0.slices};
+                   &ctx->p0.ev, &ctx->p0.famkey, &ctx->p0.famscore, &ctx->p0.slices, &ctx->b_bits};
   for (DevBuf *b : all)
     if (b->p) hipFree(b->p);
   if (ctx->dX) hipFree(ctx->dX);
@@ -421,6 +421,8 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
   GROW(b_lfull, sizeof(uint16_t) * EDDY_MAX_MP * 2 * ch.gslots);
   GROW(b_lcnt, 2 * ch.gslots);
   GROW(b_sflags, 2 * ch.gslots);
+  const int bitw = (std::max(ctx->n1, ctx->n2) + 63) / 64;
+  GROW(b_bits, sizeof(uint64_t) * 2 * 3 * (size_t)bitw * ch.gslots);
   GROW(b_ev, sizeof(Event) * evoff);
   GROW(b_evcount, sizeof(int) * nitems);
   GROW(b_hnet, sizeof(uint64_t) * ch.sslots);
@@ -462,6 +464,8 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
   a.labhits = ctx->k_labhits;
   a.win = ctx->k_win;
   a.counters = (unsigned long long *)ctx->b_counters.p;
+  a.bits = (uint64_t *)ctx->b_bits.p;
+  a.bitw = bitw;
   edge_params(ctx, p, a.ep);
   if (getenv("EDDY_NO_SHORTCUT")) a.ep.shortcut = 0;
   a.lgtab = ctx->dLG;

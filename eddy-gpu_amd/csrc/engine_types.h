@@ -107,6 +107,8 @@ struct KArgs {
   unsigned *labhits;       // [pairbase + p] full-test edges over permutations > 0 (training labels), or null
   const int *win;          // [genebase + t] last column whose full test the count kernel runs, or null
   unsigned long long *counters;  // schedule statistics (optional)
+  uint64_t *bits;          // [((goff*2 + g*G + gene)*3 + v)*bitw + w]: class-g samples (class-list order) in state v
+  int bitw;                // words per mask: ceil(max(n1, n2) / 64)
   long long *tests;        // [item] evaluated tests
 };
 

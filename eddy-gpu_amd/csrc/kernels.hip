@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
   const int item = (int)w.x;
   const int bi = (int)(w.y >> 16), bj = (int)(w.y & 0xffff);
   const ItemMeta im = a.items[item];
-  const SetMeta sm = a.sets[im.set];
+  const SetMeta &sm = a.sets[im.set];  // read in place: the bounds are indexed per tally
   const int G = sm.G, N = a.N;
   const int NP = (N + 63) & ~63;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -316,7 +316,7 @@ __device__ int block_exscan(int *v, int n, int *wsum);
 __global__ __launch_bounds__(256) void k_loo(KArgs a) {
   const int item = blockIdx.x;
   const ItemMeta im = a.items[item];
-  const SetMeta sm = a.sets[im.set];
+  const SetMeta &sm = a.sets[im.set];  // read in place: the bounds are indexed per tally
   const int G = sm.G, mp = a.mp;
   const double cpairs = (double)sm.c;
   const int tid = threadIdx.x;
@@ -546,17 +546,33 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
   if (tid == 0) hfull[0] = hfull[1] = 0ULL;
   __syncthreads();
 
-  // observed states per gene per class on the class's full data (kernel1.cu:338-387)
-  for (int i = tid; i < G * 2; i += 256) {
-    int gene = i >> 1, g = i & 1;
-    uint32_t f = 0;
+  // observed states per gene per class on the class's full data
+  // (kernel1.cu:338-387), and the class's state masks: one wave per (gene,
+  // class), lanes over the class list, a ballot per state
+  for (int i = wave; i < G * 2; i += EDDY_NWAVES) {
+    const int gene = i >> 1, g = i & 1;
     const uint8_t *row = xs + (size_t)gene * NP;
-    for (int k = 0; k < ncls[g]; k++) {
-      uint8_t v = row[cls[g][k]];
-      if (v < 3) f |= 1u << v;
+    const int ng = ncls[g];
+    uint64_t *mk = a.bits + ((size_t)(im.goff * 2 + g * G + gene) * 3) * a.bitw;
+    uint64_t any0 = 0, any1 = 0, any2 = 0;
+    for (int w = 0; w < a.bitw; w++) {
+      const int k = w * 64 + lane;
+      const int v = k < ng ? row[cls[g][k]] : 3;
+      const uint64_t m0 = __ballot(v == 0), m1 = __ballot(v == 1), m2 = __ballot(v == 2);
+      if (lane == 0) {
+        mk[w] = m0;
+        mk[a.bitw + w] = m1;
+        mk[2 * a.bitw + w] = m2;
+      }
+      any0 |= m0;
+      any1 |= m1;
+      any2 |= m2;
     }
-    sfl[g][gene] = (uint8_t)f;
-    a.sflags[im.goff * 2 + g * G + gene] = (uint8_t)f;
+    if (lane == 0) {
+      const uint32_t f = (any0 ? 1u : 0u) | (any1 ? 2u : 0u) | (any2 ? 4u : 0u);
+      sfl[g][gene] = (uint8_t)f;
+      a.sflags[im.goff * 2 + g * G + gene] = (uint8_t)f;
+    }
   }
   __syncthreads();
 
@@ -673,6 +689,11 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
 // --------------------------------------------------------------------------
 
 __device__ __forceinline__ int net_class(int s, int n1) { return s <= n1 ? 0 : 1; }
+
+// the c-th (0-based) state present in the 3-bit state set f
+__device__ __forceinline__ int nth_state(uint32_t f, int c) {
+  return c == 0 ? __builtin_ctz(f) : c == 1 ? __builtin_ctz(f & (f - 1)) : 2;
+}
 
 // parent list of row t in network s (events of s sorted by t: es[0..ne))
 __device__ __forceinline__ int row_list(const KArgs &a, const ItemMeta &im, int G, int s, int t, const int *es,
@@ -829,26 +850,59 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   __syncthreads();
   phase_mark(a, 0, tph);
   // ---- 2. exact confirmation of every merge -----------------------------
+  // rows where the two classes' full networks differ, ascending (famfull is
+  // free until the families are numbered)
+  int *drows = famfull, *dflag = famfull + G;
+  __shared__ int ndiff;
+  for (int t = tid; t < G; t += 256) {
+    const size_t l0 = (size_t)(im.goff * 2 + t), l1 = l0 + G;
+    const int c0 = a.lcnt[l0];
+    bool d = c0 != a.lcnt[l1];
+    for (int i = 0; i < c0 && !d; i++) d = a.lfull[l0 * EDDY_MAX_MP + i] != a.lfull[l1 * EDDY_MAX_MP + i];
+    dflag[t] = d ? 1 : 0;
+  }
+  if (tid == 0) ndiff = 0;
+  __syncthreads();
+  {
+    // compact in rounds of 256 rows (ballot per wave, wave totals in wsum)
+    for (int t0 = 0; t0 < G; t0 += 256) {
+      const int t = t0 + tid;
+      const int d = t < G ? dflag[t] : 0;
+      const uint64_t m = __ballot(d);
+      if (lane == 0) wsum[wave] = __popcll(m);
+      __syncthreads();
+      int base = ndiff;
+      for (int w = 0; w < wave; w++) base += wsum[w];
+      if (d) drows[base + __popcll(m & ((1ULL << lane) - 1))] = t;
+      __syncthreads();
+      if (tid == 0) ndiff += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+      __syncthreads();
+    }
+  }
+  // a merge is confirmed row by row over the rows where the two networks can
+  // differ: their event rows, and, across classes, the rows of drows
   for (int s = tid; s < S; s += 256) {
     int r = rep[s];
     if (r == s) continue;
     int gs = net_class(s, n1), gr = net_class(r, n1);
     int ns = evst[s + 1] - evst[s], nr = evst[r + 1] - evst[r];
     if (gs == gr && ns == 0 && nr == 0) continue;
+    const int nd = gs == gr ? 0 : ndiff;
     bool eq = true;
-    int ps = 0, pr = 0;
-    for (int t = 0; t < G && eq; t++) {
-      if (gs == gr) {
-        // rows without events are the same full row; compare event rows only
-        while (ps < ns && ev[es[evst[s] + ps]].t < t) ps++;
-        while (pr < nr && ev[es[evst[r] + pr]].t < t) pr++;
-        bool hs = ps < ns && ev[es[evst[s] + ps]].t == t;
-        bool hr = pr < nr && ev[es[evst[r] + pr]].t == t;
-        if (!hs && !hr) continue;
-      }
+    int ps = 0, pr = 0, ia = 0, ib = 0, id = 0;
+    const int *esS = es + evst[s], *esR = es + evst[r];
+    while (eq) {
+      const int ta = ia < ns ? ev[esS[ia]].t : 0x7fffffff;
+      const int tb = ib < nr ? ev[esR[ib]].t : 0x7fffffff;
+      const int td = id < nd ? drows[id] : 0x7fffffff;
+      const int t = min(ta, min(tb, td));
+      if (t == 0x7fffffff) break;
+      ia += ta == t;
+      ib += tb == t;
+      id += td == t;
       const uint16_t *ls, *lr;
-      int cs = row_list(a, im, G, s, t, es + evst[s], ns, &ps, &ls);
-      int cr = row_list(a, im, G, r, t, es + evst[r], nr, &pr, &lr);
+      int cs = row_list(a, im, G, s, t, esS, ns, &ps, &ls);
+      int cr = row_list(a, im, G, r, t, esR, nr, &pr, &lr);
       if (cs != cr) eq = false;
       for (int i = 0; i < cs && eq; i++) eq = ls[i] == lr[i];
     }
@@ -959,28 +1013,35 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
       radix[i] = q;
       q *= __popc(sf[par[i]]);
     }
-    for (int i = lane; i < 84; i += 64) hist[wave][i] = 0;
-    wave_sync();
+    // histogram cell (j, k) = samples of class g with the child in its k-th
+    // observed state and the parents in the states that compose j: the
+    // popcount of the AND of their state masks
     if (q > 0 && r > 0) {
-      const uint8_t *xt = xs + (size_t)t * NP;
-      const uint16_t *cl = a.clsg + im.soff + (g ? a.n1 : 0);
-      const int ng = g ? a.n2 : a.n1;
-      for (int k = lane; k < ng; k += 64) {
-        int sid = cl[k];
-        int v = xt[sid];
-        if (v >= 3) continue;
-        int kk = __popc(sf[t] & ((1u << v) - 1));
-        int j = 0;
-        bool ok = true;
-        for (int i = 0; i < np; i++) {
-          int pv = xs[(size_t)par[i] * NP + sid];
-          if (pv >= 3) {
-            ok = false;
-            break;
-          }
-          j += __popc(sf[par[i]] & ((1u << pv) - 1)) * radix[i];
+      const uint64_t *mk = a.bits + (size_t)(im.goff * 2 + g * G) * 3 * a.bitw;
+      const int W = a.bitw;
+      const uint32_t f0 = sf[t], f1 = np > 0 ? sf[par[0]] : 1u, f2 = np > 1 ? sf[par[1]] : 1u,
+                     f3 = np > 2 ? sf[par[2]] : 1u;
+      const int n1s = __popc(f1), n2s = __popc(f2);
+      for (int c = lane; c < q * r; c += 64) {
+        const int kk = c % r;
+        int j = c / r;
+        const int i1 = j % n1s;
+        j /= n1s;
+        const int i2 = j % n2s;
+        const int i3 = j / n2s;
+        const uint64_t *m0 = mk + (t * 3 + nth_state(f0, kk)) * W;
+        const uint64_t *m1 = np > 0 ? mk + (par[0] * 3 + nth_state(f1, i1)) * W : nullptr;
+        const uint64_t *m2 = np > 1 ? mk + (par[1] * 3 + nth_state(f2, i2)) * W : nullptr;
+        const uint64_t *m3 = np > 2 ? mk + (par[2] * 3 + nth_state(f3, i3)) * W : nullptr;
+        int cnt = 0;
+        for (int w = 0; w < W; w++) {
+          uint64_t x = m0[w];
+          if (m1) x &= m1[w];
+          if (m2) x &= m2[w];
+          if (m3) x &= m3[w];
+          cnt += __popcll(x);
         }
-        if (ok) atomicAdd(&hist[wave][j * r + kk], 1);
+        hist[wave][c] = cnt;
       }
     }
     wave_sync();
