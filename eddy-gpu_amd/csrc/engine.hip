@@ -215,10 +215,10 @@ extern "C" int eddy_stream_draw(eddy_ctx *ctx, int n, int32_t *out) {
 
 extern "C" int eddy_stream_permutation(eddy_ctx *ctx, int samples, int n1, int perm, uint8_t *labels) {
   if (!ctx || samples < 1 || n1 < 0 || n1 > samples || !labels) return EDDY_ERR_ARG;
-  std::vector<int> ord;
-  ctx->stream_gen.permutation(samples, ord);
+  std::vector<uint32_t> c2((samples + 31) / 32);
+  ctx->stream_gen.class2_bits(samples, n1, c2.data());
   if (perm > 0)
-    for (int i = 0; i < samples; i++) labels[ord[i]] = i < n1 ? 0 : 1;  // main.cu:561-570
+    for (int i = 0; i < samples; i++) labels[i] = (uint8_t)((c2[i >> 5] >> (i & 31)) & 1);  // main.cu:561-570
   return EDDY_OK;
 }
 
@@ -765,12 +765,6 @@ std::vector<uint32_t> label_bits(const std::vector<uint8_t> &cls) {
   return b;
 }
 
-// class labels of one permutation from its accepted order (main.cu:561-570)
-std::vector<uint8_t> perm_labels(const std::vector<int> &order, int n1) {
-  std::vector<uint8_t> cls(order.size());
-  for (size_t i = 0; i < order.size(); i++) cls[order[i]] = (int)i < n1 ? 0 : 1;
-  return cls;
-}
 
 int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *p,
                  std::vector<SetMeta> &meta) {
@@ -900,21 +894,16 @@ struct Producer {
     ready.store(from);
     th = std::thread([this, from]() {
       auto h0 = std::chrono::steady_clock::now();
-      std::vector<int> ord;
       const int words = (N + 31) / 32;
       PermStream &ps = ctx->stream_gen;
       for (int s = from; s < nsets; s++) {
         if (quit.load(std::memory_order_relaxed)) break;
-        ps.permutation(N, ord);  // permutation 0's draws
+        ps.class2_bits(N, n1, nullptr);  // permutation 0's draws
         after0[s] = ps.save();
         if (!stop0[s]) {
           std::vector<uint32_t> &L = lab[s];
-          L.assign((size_t)(perms - 1) * words, 0u);
-          for (int pm = 1; pm < perms; pm++) {
-            ps.permutation(N, ord);
-            uint32_t *w = &L[(size_t)(pm - 1) * words];
-            for (int i = n1; i < N; i++) w[ord[i] >> 5] |= 1u << (ord[i] & 31);  // class 2 (main.cu:561-570)
-          }
+          L.resize((size_t)(perms - 1) * words);
+          for (int pm = 1; pm < perms; pm++) ps.class2_bits(N, n1, &L[(size_t)(pm - 1) * words]);
         }
         ready.store(s + 1, std::memory_order_release);
       }
@@ -1226,16 +1215,16 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     for (int q = 0; q < nsets; q++) ran[q] = (nan0[q] || prod.stop0[q]) ? 1 : perms;
   } else {
     // plain: one set and one permutation at a time, like main.cu:538-1460
-    std::vector<int> order;
+    std::vector<uint32_t> c2((N + 31) / 32);
     for (int q = 0; q < nsets; q++) {
       for (int pm = 0; pm < perms; pm++) {
-        ctx->stream_gen.permutation(N, order);
+        ctx->stream_gen.class2_bits(N, ctx->n1, c2.data());
         if (meta[q].G < 2) {
           nan0[q] = 1;
           break;
         }
         Chunk ch;
-        add_item(ch, meta, q, pm, pm == 0 ? truebits : label_bits(perm_labels(order, ctx->n1)), N);
+        add_item(ch, meta, q, pm, pm == 0 ? truebits : c2, N);
         std::vector<long long> t(1, 0);
         CHK(run_chunk(ctx, meta, ch, p, perms, pm == 0, t));
         tests_eval[q] += t[0];

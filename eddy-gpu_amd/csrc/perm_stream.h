@@ -56,47 +56,85 @@ class PermStream {
   }
 
   // One permutation's consumption (main.cu:541-558): `n` discarded draws,
-  // then rejection sampling of a permutation of [0, n).  order receives the
-  // accepted values in acceptance order.  `rand() % n` is computed with an
-  // exact multiply-shift reduction (Lemire, "Faster remainder by direct
-  // computation", 2019), which equals % for every 32-bit input.
-  void permutation(int n, std::vector<int> &order) {
-    order.resize(n);
+  // then rejection sampling of a permutation of [0, n).  `rand() % n` is
+  // computed with an exact multiply-shift reduction (Lemire, "Faster
+  // remainder by direct computation", 2019), which equals % for every 32-bit
+  // input.  Only what the class labels need is kept: the first n1 accepted
+  // values are class 1 (main.cu:561-570), so the scan records those, then only
+  // counts distinct values until all n are seen (where the next permutation's
+  // draws begin).  c2 (when given) receives the class-2 membership bits of
+  // samples 0..n-1.
+  void class2_bits(int n, int n1, uint32_t *c2) {
     if ((int)stamp_.size() < n) stamp_.assign(n, 0);
     if (++gen_ == 0) {
       std::fill(stamp_.begin(), stamp_.end(), 0);
       gen_ = 1;
     }
+    const int words = (n + 31) / 32;
+    if (c2)
+      for (int i = 0; i < words; i++) c2[i] = 0u;
     const uint64_t M = UINT64_C(0xFFFFFFFFFFFFFFFF) / (uint32_t)n + 1;
     skip(n);
     uint32_t *stamp = stamp_.data();
     const uint32_t gen = gen_;
     int k = 0;
+    // class 1: the first n1 distinct values (marked in c2, inverted below)
+    while (k < n1) {
+      if (pos_ == end_) refill();
+      const int avail = end_ - pos_;
+      const uint32_t *src = buf_ + pos_;
+      int i = 0;
+      for (; i < avail && k < n1; i++) {
+        const uint32_t x = src[i] >> 1;
+        const uint32_t v = (uint32_t)(((__uint128_t)(M * x) * (uint32_t)n) >> 64);
+        const uint32_t fresh = stamp[v] != gen;
+        stamp[v] = gen;
+        if (c2) c2[v >> 5] |= fresh << (v & 31);
+        k += (int)fresh;
+      }
+      pos_ += i;
+    }
+    // the rest: count until every value has appeared
     while (k < n) {
       if (pos_ == end_) refill();
       const int avail = end_ - pos_;
       const uint32_t *src = buf_ + pos_;
       int i = 0;
       for (; i < avail && k < n; i++) {
-        uint32_t x = src[i] >> 1;
-        uint32_t v = (uint32_t)(((__uint128_t)(M * x) * (uint32_t)n) >> 64);
-        int fresh = stamp[v] != gen;
+        const uint32_t x = src[i] >> 1;
+        const uint32_t v = (uint32_t)(((__uint128_t)(M * x) * (uint32_t)n) >> 64);
+        k += (int)(stamp[v] != gen);
         stamp[v] = gen;
-        order[k] = (int)v;
-        k += fresh;
       }
       pos_ += i;
+    }
+    if (c2) {
+      for (int i = 0; i < words; i++) c2[i] = ~c2[i];
+      if (n & 31) c2[words - 1] &= (1u << (n & 31)) - 1u;
     }
   }
 
  private:
   static const int kHist = 34;
-  static const int kBlock = 8192;
+  static const int kBlock = 8190;  // a multiple of 3 (the refill loop's step)
   // next block: r[i] = r[i-31] + r[i-3] over a linear array (3-way independent)
   void refill() {
     for (int i = 0; i < kHist; i++) buf_[i] = buf_[end_ - kHist + i];
     uint32_t *b = buf_;
-    for (int i = kHist; i < kHist + kBlock; i++) b[i] = b[i - 31] + b[i - 3];
+    // the three most recent values ride in registers: three independent
+    // add chains instead of a store-to-load round trip per value
+    uint32_t a0 = b[kHist - 3], a1 = b[kHist - 2], a2 = b[kHist - 1];
+    for (int i = kHist; i < kHist + kBlock; i += 3) {
+      const uint32_t v0 = b[i - 31] + a0;
+      const uint32_t v1 = b[i - 30] + a1;
+      const uint32_t v2 = b[i - 29] + a2;
+      b[i] = v0;
+      b[i + 1] = v1;
+      b[i + 2] = v2;
+      a0 = v0;
+      a1 = v1;
+      a2 = v2;
+    }
     pos_ = kHist;
     end_ = kHist + kBlock;
   }
