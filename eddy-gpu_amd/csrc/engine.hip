@@ -510,7 +510,7 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
   hipEventRecord(ev[2], st);
   k_rows<<<nitems, 256, 0, st>>>(a);
   hipEventRecord(ev[3], st);
-  const size_t score_lds = (size_t)ht * 12 + (size_t)(4 * S + 1 + 2 * maxG) * 4;
+  const size_t score_lds = (size_t)EDDY_NWAVES * 12 * bitw * 8 + (size_t)ht * 12 + (size_t)(4 * S + 1 + 2 * maxG) * 4;
   k_score<<<nitems, 256, score_lds, st>>>(a);
   hipEventRecord(ev[4], st);
   HIPCHK(ctx, hipGetLastError());

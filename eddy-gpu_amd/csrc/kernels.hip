@@ -690,6 +690,10 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
 
 __device__ __forceinline__ int net_class(int s, int n1) { return s <= n1 ? 0 : 1; }
 
+// c / d for 0 <= c < 128 and d in {1, 2, 3} (43/128 = 1/3 + 1/384 is
+// exact over that range)
+__device__ __forceinline__ int div_small(int c, int d) { return d == 1 ? c : d == 2 ? c >> 1 : (c * 43) >> 7; }
+
 // the c-th (0-based) state present in the 3-bit state set f
 __device__ __forceinline__ int nth_state(uint32_t f, int c) {
   return c == 0 ? __builtin_ctz(f) : c == 1 ? __builtin_ctz(f & (f - 1)) : 2;
@@ -781,7 +785,8 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   const int HT = a.ht_size;
 
   extern __shared__ __align__(16) unsigned char dyn[];
-  unsigned long long *hkeys = (unsigned long long *)dyn;  // [HT]
+  // [4 waves][12 * bitw] u64 BDeu mask stage, then the network tables
+  unsigned long long *hkeys = (unsigned long long *)dyn + (size_t)EDDY_NWAVES * 12 * a.bitw;  // [HT]
   int *hvals = (int *)(hkeys + HT);                       // [HT]
   int *rep = hvals + HT;                                  // [S]
   int *evst = rep + S;                                    // [S + 1]
@@ -791,8 +796,7 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   __shared__ int wsum[257];
   __shared__ int U, F, flag;
   __shared__ int hist[EDDY_NWAVES][84];
-  __shared__ double lgv[EDDY_NWAVES][84];
-  __shared__ double lgj[EDDY_NWAVES][28];
+  __shared__ double lgv[EDDY_NWAVES][28];  // per parent configuration j: (lq0 - lq[n_j]) + inner sum
 
   const int E = min(a.evcount[item], im.evcap);
   int *es = a.evorder + im.evoff;
@@ -999,81 +1003,71 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
     atomicAdd(&a.counters[7], (unsigned long long)E);       // events
   }
   // ---- 4. BDeu per (family, scoring class): kernel5.cu:4-236 -------------
-  // each wave scores its own (family, class) pairs with a private histogram
+  // each wave scores its own (family, class) pairs: the family's state masks
+  // staged in LDS, one histogram cell per lane, the inner sums of the
+  // reference's double loop per lane (one lane per parent configuration j),
+  // the outer sum over j by lane 0, both in the reference's order
+  uint64_t *mst = (uint64_t *)dyn + (size_t)wave * 12 * a.bitw;  // [4 genes][3 states][W]
   for (int fg = wave; fg < 2 * NF; fg += EDDY_NWAVES) {
     const int f = fg >> 1, g = fg & 1;
     const uint64_t key = a.famkey[im.foff + f] - 1;
     const int t = (int)(key & 1023), np = (int)((key >> 10) & 3);
-    int par[3] = {0, 0, 0};
-    for (int i = 0; i < np; i++) par[i] = (int)((key >> (12 + 10 * i)) & 1023);
+    const int p0 = (int)((key >> 12) & 1023), p1 = (int)((key >> 22) & 1023), p2 = (int)((key >> 32) & 1023);
     const uint8_t *sf = a.sflags + im.goff * 2 + g * G;
-    const int r = __popc(sf[t]);
-    int q = 1, radix[3] = {1, 1, 1};
-    for (int i = 0; i < np; i++) {
-      radix[i] = q;
-      q *= __popc(sf[par[i]]);
-    }
-    // histogram cell (j, k) = samples of class g with the child in its k-th
-    // observed state and the parents in the states that compose j: the
-    // popcount of the AND of their state masks
-    if (q > 0 && r > 0) {
-      const uint64_t *mk = a.bits + (size_t)(im.goff * 2 + g * G) * 3 * a.bitw;
+    const uint32_t f0 = sf[t], f1 = np > 0 ? sf[p0] : 1u, f2 = np > 1 ? sf[p1] : 1u, f3 = np > 2 ? sf[p2] : 1u;
+    const int r = __popc(f0), n1s = __popc(f1), n2s = __popc(f2), n3s = __popc(f3);
+    const int q = n1s * n2s * n3s;  // parents' observed-state combinations (1 without parents)
+    double sum2 = 0.0;
+    if (r > 0 && (np == 0 || q > 0)) {
       const int W = a.bitw;
-      const uint32_t f0 = sf[t], f1 = np > 0 ? sf[par[0]] : 1u, f2 = np > 1 ? sf[par[1]] : 1u,
-                     f3 = np > 2 ? sf[par[2]] : 1u;
-      const int n1s = __popc(f1), n2s = __popc(f2);
+      // stage the masks: slot i*3 + v holds gene i's state-v mask
+      {
+        const uint64_t *mk = a.bits + (size_t)(im.goff * 2 + g * G) * 3 * W;
+        const int gl0 = t, gl1 = p0, gl2 = p1, gl3 = p2;
+        for (int x = lane; x < (np + 1) * 3 * W; x += 64) {
+          const int slot = x / W, w = x - slot * W, gi = slot / 3, v = slot - gi * 3;
+          const int gene = gi == 0 ? gl0 : gi == 1 ? gl1 : gi == 2 ? gl2 : gl3;
+          mst[x] = mk[(size_t)(gene * 3 + v) * W + w];
+        }
+      }
+      wave_sync();
       for (int c = lane; c < q * r; c += 64) {
-        const int kk = c % r;
-        int j = c / r;
-        const int i1 = j % n1s;
-        j /= n1s;
-        const int i2 = j % n2s;
-        const int i3 = j / n2s;
-        const uint64_t *m0 = mk + (t * 3 + nth_state(f0, kk)) * W;
-        const uint64_t *m1 = np > 0 ? mk + (par[0] * 3 + nth_state(f1, i1)) * W : nullptr;
-        const uint64_t *m2 = np > 1 ? mk + (par[1] * 3 + nth_state(f2, i2)) * W : nullptr;
-        const uint64_t *m3 = np > 2 ? mk + (par[2] * 3 + nth_state(f3, i3)) * W : nullptr;
+        const int j = div_small(c, r), kk = c - j * r;
+        const int j1 = div_small(j, n1s), i1 = j - j1 * n1s;
+        const int i3 = div_small(j1, n2s), i2 = j1 - i3 * n2s;
+        const uint64_t *m0 = mst + (0 * 3 + nth_state(f0, kk)) * W;
+        const uint64_t *m1 = mst + (1 * 3 + nth_state(f1, i1)) * W;
+        const uint64_t *m2 = mst + (2 * 3 + nth_state(f2, i2)) * W;
+        const uint64_t *m3 = mst + (3 * 3 + nth_state(f3, i3)) * W;
         int cnt = 0;
         for (int w = 0; w < W; w++) {
           uint64_t x = m0[w];
-          if (m1) x &= m1[w];
-          if (m2) x &= m2[w];
-          if (m3) x &= m3[w];
+          if (np > 0) x &= m1[w];
+          if (np > 1) x &= m2[w];
+          if (np > 2) x &= m3[w];
           cnt += __popcll(x);
         }
         hist[wave][c] = cnt;
       }
-    }
-    wave_sync();
-    // lanes gather the lgamma terms; lane 0 adds them in the reference's
-    // order (j outer, k inner: kernel5.cu:200-218)
-    if (q > 0 && r > 0) {
+      wave_sync();
       const double *lq = a.lgtab + (size_t)q * a.lgstride;
       const double *lqr = a.lgtab + (size_t)(q * r) * a.lgstride;
-      for (int i = lane; i < q * r; i += 64) lgv[wave][i] = lqr[hist[wave][i]];
+      const double lq0 = lq[0], lqr0 = lqr[0];
       for (int j = lane; j < q; j += 64) {
         int nij = 0;
-        for (int k = 0; k < r; k++) nij += hist[wave][j * r + k];
-        lgj[wave][j] = lq[nij];
-      }
-      if (lane == 0) {
-        lgj[wave][27] = lq[0];
-        lgv[wave][81] = lqr[0];
-      }
-    }
-    wave_sync();
-    if (lane == 0) {
-      double sum2 = 0.0;
-      if (q > 0 && r > 0) {
-        const double lq0 = lgj[wave][27], lqr0 = lgv[wave][81];
-        for (int j = 0; j < q; j++) {
-          double sum1 = 0.0;
-          for (int k = 0; k < r; k++) sum1 = sum1 + (lgv[wave][j * r + k] - lqr0);
-          sum2 = sum2 + ((lq0 - lgj[wave][j]) + sum1);
+        double sum1 = 0.0;
+        for (int k = 0; k < r; k++) {
+          const int h = hist[wave][j * r + k];
+          nij += h;
+          sum1 = sum1 + (lqr[h] - lqr0);
         }
+        lgv[wave][j] = (lq0 - lq[nij]) + sum1;
       }
-      fscore[f * 2 + g] = sum2;
+      wave_sync();
+      if (lane == 0)
+        for (int j = 0; j < q; j++) sum2 = sum2 + lgv[wave][j];
     }
+    if (lane == 0) fscore[f * 2 + g] = sum2;
     wave_sync();
   }
   __syncthreads();
