@@ -1140,7 +1140,11 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     };
     while (s < nsets || !ch.items.empty()) {
       const bool boundary = s == nsets || s == collect_upto || s == win_from;
-      if (!ch.items.empty() && (ch.pairs >= budget || boundary)) {
+      // flush when the chunk is full, at a boundary, or when the next set's
+      // labels are not ready yet: the GPU then works while the replay
+      // catches up
+      const bool starved = s < nsets && prod.ready.load(std::memory_order_acquire) <= s;
+      if (!ch.items.empty() && (ch.pairs >= budget || boundary || (starved && ch.pairs >= (budget >> 3)))) {
         const int last = s - 1;
         int bad;
         CHK(flush(bad));
