@@ -466,6 +466,7 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
   a.counters = (unsigned long long *)ctx->b_counters.p;
   a.bits = (uint64_t *)ctx->b_bits.p;
   a.bitw = bitw;
+  a.rows_maxg = maxG;
   edge_params(ctx, p, a.ep);
   if (getenv("EDDY_NO_SHORTCUT")) a.ep.shortcut = 0;
   a.lgtab = ctx->dLG;
@@ -508,7 +509,8 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
   hipEventRecord(ev[1], st);
   if (!plain_tests) k_loo<<<nitems, 256, 0, st>>>(a);
   hipEventRecord(ev[2], st);
-  k_rows<<<nitems, 256, 0, st>>>(a);
+  const size_t rows_lds = (size_t)S * 8 + (size_t)2 * EDDY_NWAVES * maxG * 4 + (size_t)N * 2 + (size_t)2 * maxG + 16;
+  k_rows<<<nitems, 256, rows_lds, st>>>(a);
   hipEventRecord(ev[3], st);
   const size_t score_lds = (size_t)EDDY_NWAVES * 12 * bitw * 8 + (size_t)ht * 12 + (size_t)(4 * S + 1 + 2 * maxG) * 4;
   k_score<<<nitems, 256, score_lds, st>>>(a);

@@ -109,6 +109,7 @@ struct KArgs {
   unsigned long long *counters;  // schedule statistics (optional)
   uint64_t *bits;          // [((goff*2 + g*G + gene)*3 + v)*bitw + w]: class-g samples (class-list order) in state v
   int bitw;                // words per mask: ceil(max(n1, n2) / 64)
+  int rows_maxg;           // largest set of the launch (row kernel LDS layout)
   long long *tests;        // [item] evaluated tests
 };
 

@@ -319,14 +319,15 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
   const SetMeta &sm = a.sets[im.set];  // read in place: the bounds are indexed per tally
   const int G = sm.G, mp = a.mp;
   const double cpairs = (double)sm.c;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   __shared__ int plan[K2_MAXROWS];       // planned last column per (row, class) entry
   __shared__ int off[K2_MAXROWS + 1];    // exclusive prefix of entry counts
+  __shared__ int solid[K2_MAXROWS];      // solid edges found up to the plan
   __shared__ int wsum[257];
   __shared__ int tcount, nfb, nfbcols;
   if (tid == 0) tcount = nfb = nfbcols = 0;
   const int R = 2 * G;  // entries r = 2t + g
-  int ntests0 = 0;
+  int ntests = 0, fb = 0, fbc = 0;
   unsigned long long tph = 0;
   phase_mark(a, -1, tph);
   // ---- A0 (learned window): the count kernel ran the full tests of row t
@@ -335,13 +336,17 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
   // item's rows (the tests the count kernel would have run)
   const int *win = (a.win && im.perm > 0) ? a.win + sm.genebase : nullptr;
   if (win) {
-    for (int r = tid; r < R; r += 256) {
+    for (int r = wave; r < R; r += EDDY_NWAVES) {  // a wave per entry, 64 columns at a time
       const int t = r >> 1, g = r & 1;
       const int want = mp + (a.plan_extra ? a.plan_extra[(im.goff * 2 + g * G + t)] : 0);
       const int W = min(win[t], t - 1);
       int hits = 0;
-      for (int col = 0; col <= W && hits < want; col++) hits += (a.info[im.infooff + pair_index(t, col)] >> g) & 1;
-      off[r] = hits < want ? t - 1 - W : 0;
+      for (int base = 0; base <= W && hits < want; base += 64) {
+        const int col = base + lane;
+        const int f = col <= W ? (int)((a.info[im.infooff + pair_index(t, col)] >> g) & 1) : 0;
+        hits += __popcll(__ballot(f));
+      }
+      if (lane == 0) off[r] = hits < want ? t - 1 - W : 0;
     }
     __syncthreads();
     const int ext = block_exscan(off, R, wsum);
@@ -354,56 +359,45 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
       const int t = lo >> 1, g = lo & 1;
       const int col = min(win[t], t - 1) + 1 + (e - off[lo]);
       const long long p = pair_index(t, col);
-      full_of(a, im, sm, p, g, a.info[im.infooff + p], cpairs, &ntests0);
+      full_of(a, im, sm, p, g, a.info[im.infooff + p], cpairs, &ntests);
     }
     __syncthreads();
   }
-  // ---- A: plan ----------------------------------------------------------
-  for (int r = tid; r < R; r += 256) {
+  // ---- A: plan P = column of the (mp + extra)-th full-test edge, a wave per
+  // entry scanning 64 columns per ballot ------------------------------------
+  for (int r = wave; r < R; r += EDDY_NWAVES) {
     const int t = r >> 1, g = r & 1;
     const int want = mp + (a.plan_extra ? a.plan_extra[(im.goff * 2 + g * G + t)] : 0);
     int hits = 0, P = t - 1;
-    for (int col = 0; col < t; col++) {
-      const long long pc = pair_index(t, col);
-      uint32_t w = a.info[im.infooff + pc];
-      if (full_of(a, im, sm, pc, g, w, cpairs, &ntests0)) {
-        if (++hits >= want) {
-          P = col;
-          break;
-        }
+    for (int base = 0; base < t; base += 64) {
+      const int col = base + lane;
+      int f = 0;
+      if (col < t) {
+        const long long pc = pair_index(t, col);
+        f = full_of(a, im, sm, pc, g, a.info[im.infooff + pc], cpairs, &ntests);
       }
+      const uint64_t b = __ballot(f);
+      const int nb = __popcll(b);
+      if (hits + nb >= want) {  // wave-uniform
+        uint64_t x = b;
+        for (int i = 1; i < want - hits; i++) x &= x - 1;
+        P = base + __ffsll((long long)x) - 1;
+        break;
+      }
+      hits += nb;
     }
-    plan[r] = P;
+    if (lane == 0) {
+      plan[r] = P;
+      off[r] = P + 1;
+      solid[r] = 0;
+    }
   }
   __syncthreads();
   phase_mark(a, 10, tph);
-  // exclusive scan of the entry counts (P + 1 columns per entry)
-  {
-    const int per = (R + 255) / 256;
-    int b = tid * per, e = min(R, b + per), acc = 0;
-    for (int r = b; r < e; r++) acc += plan[r] + 1;
-    wsum[tid] = acc;
-    __syncthreads();
-    if (tid == 0) {
-      int run = 0;
-      for (int i = 0; i < 256; i++) {
-        int v = wsum[i];
-        wsum[i] = run;
-        run += v;
-      }
-      off[R] = run;
-    }
-    __syncthreads();
-    acc = wsum[tid];
-    for (int r = b; r < e; r++) {
-      off[r] = acc;
-      acc += plan[r] + 1;
-    }
-  }
+  const int total = block_exscan(off, R, wsum);
+  if (tid == 0) off[R] = total;
   __syncthreads();
   // ---- B: packed leave-one-out tests of every planned (pair, class) ------
-  const int total = off[R];
-  int ntests = ntests0;
   for (int e = tid; e < total; e += 256) {
     int lo = 0, hi = R - 1;  // last r with off[r] <= e
     while (lo < hi) {
@@ -412,30 +406,34 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
     }
     const int t = lo >> 1, g = lo & 1, col = e - off[lo];
     const long long p = pair_index(t, col);
-    uint32_t w = a.info[im.infooff + p];
-    uint32_t m = loo_of_pair(a, im, sm, p, g, (w >> g) & 1, cpairs, &ntests);
+    const uint32_t w = a.info[im.infooff + p];
+    const int full = (w >> g) & 1;
+    const uint32_t m = loo_of_pair(a, im, sm, p, g, full, cpairs, &ntests);
     atomicOr(&a.info[im.infooff + p], (m << INFO_LOO_SHIFT(g)) | INFO_EVAL(g));
+    if (full && m == 0x1FFu) atomicAdd(&solid[lo], 1);
   }
   __syncthreads();
   phase_mark(a, 11, tph);
-  // ---- C: exact cutoff check, fallback past the plan ---------------------
-  int fb = 0, fbc = 0;
-  for (int r = tid; r < R; r += 256) {
+  // ---- C: a row with fewer than mp solid edges up to its plan continues
+  // past it (a wave per row, 64 columns per step) until mp solid edges or
+  // the row's end.  Correctness never depends on the plan. ----------------
+  for (int r = wave; r < R; r += EDDY_NWAVES) {
     const int t = r >> 1, g = r & 1;
-    int solid = 0, col = 0;
-    for (; col <= plan[r] && col < t && solid < mp; col++) {
-      uint32_t w = a.info[im.infooff + pair_index(t, col)];
-      if (((w >> g) & 1) && ((w >> INFO_LOO_SHIFT(g)) & 0x1FFu) == 0x1FFu) solid++;
-    }
-    if (solid < mp && col < t) fb++;
-    for (; col < t && solid < mp; col++) {
-      const long long p = pair_index(t, col);
-      uint32_t w = a.info[im.infooff + p];
-      const int full = full_of(a, im, sm, p, g, w, cpairs, &ntests);
-      uint32_t m = loo_of_pair(a, im, sm, p, g, full, cpairs, &ntests);
-      atomicOr(&a.info[im.infooff + p], (m << INFO_LOO_SHIFT(g)) | INFO_EVAL(g));
-      if (full && m == 0x1FFu) solid++;
-      fbc++;
+    int have = solid[r];
+    if (have >= mp || plan[r] >= t - 1) continue;  // wave-uniform
+    fb += lane == 0;
+    for (int base = plan[r] + 1; base < t && have < mp; base += 64) {
+      const int col = base + lane;
+      int sol = 0;
+      if (col < t) {
+        const long long p = pair_index(t, col);
+        const int full = full_of(a, im, sm, p, g, a.info[im.infooff + p], cpairs, &ntests);
+        const uint32_t m = loo_of_pair(a, im, sm, p, g, full, cpairs, &ntests);
+        atomicOr(&a.info[im.infooff + p], (m << INFO_LOO_SHIFT(g)) | INFO_EVAL(g));
+        sol = full && m == 0x1FFu;
+        fbc++;
+      }
+      have += __popcll(__ballot(sol));
     }
   }
   atomicAdd(&tcount, ntests);
@@ -527,22 +525,29 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint8_t *xs = a.xs + sm.xoff;
 
-  __shared__ uint16_t cls[2][EDDY_MAX_N];
+  // LDS sized by the launch (rows_lds_bytes): S network deltas, per-wave
+  // flip and info rows of the largest set, the class lists, state flags
+  extern __shared__ __align__(16) unsigned char rdyn[];
+  const int GM = a.rows_maxg;
+  unsigned long long *delta = (unsigned long long *)rdyn;                  // [S]
+  uint32_t *flipsb = (uint32_t *)(delta + S);                              // [4][GM]: col | mask << 16
+  uint32_t *rinfob = flipsb + EDDY_NWAVES * GM;                            // [4][GM]: info words up to the cutoff
+  uint16_t *clsb = (uint16_t *)(rinfob + EDDY_NWAVES * GM);                // [N]: class 1, then class 2
+  uint8_t *sflb = (uint8_t *)(clsb + N);                                   // [2][GM]
+  uint16_t *cls0 = clsb, *cls1 = clsb + a.n1;
+  uint32_t *flips_w = flipsb + wave * GM;
+  uint32_t *rinfo_w = rinfob + wave * GM;
   __shared__ int ncls[2];
-  __shared__ unsigned long long delta[EDDY_MAX_N + 2];
   __shared__ unsigned long long hfull[2];
-  __shared__ uint32_t flips[EDDY_NWAVES][EDDY_MAX_G];  // col | mask << 16
   __shared__ uint16_t lrow[EDDY_NWAVES][EDDY_MAX_MP];
-  __shared__ uint32_t rinfo[EDDY_NWAVES][EDDY_MAX_G];  // this row's info words up to its cutoff
-  __shared__ uint8_t sfl[2][EDDY_MAX_G];
 
   unsigned long long tph = 0;
   phase_mark(a, -1, tph);
-  build_class_lists(a.labels + im.laboff, N, cls[0], cls[1], ncls);
-  for (int k = tid; k < ncls[0]; k += 256) a.clsg[im.soff + k] = cls[0][k];
-  for (int k = tid; k < ncls[1]; k += 256) a.clsg[im.soff + a.n1 + k] = cls[1][k];
+  build_class_lists(a.labels + im.laboff, N, cls0, cls1, ncls);
+  for (int k = tid; k < ncls[0]; k += 256) a.clsg[im.soff + k] = cls0[k];
+  for (int k = tid; k < ncls[1]; k += 256) a.clsg[im.soff + a.n1 + k] = cls1[k];
   for (int s = tid; s < S; s += 256) delta[s] = 0ULL;
-  for (int i = tid; i < 2 * G; i += 256) sfl[i / G][i % G] = 0;
+  for (int i = tid; i < 2 * GM; i += 256) sflb[i] = 0;
   if (tid == 0) hfull[0] = hfull[1] = 0ULL;
   __syncthreads();
 
@@ -557,7 +562,7 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
     uint64_t any0 = 0, any1 = 0, any2 = 0;
     for (int w = 0; w < a.bitw; w++) {
       const int k = w * 64 + lane;
-      const int v = k < ng ? row[cls[g][k]] : 3;
+      const int v = k < ng ? row[(g ? cls1 : cls0)[k]] : 3;
       const uint64_t m0 = __ballot(v == 0), m1 = __ballot(v == 1), m2 = __ballot(v == 2);
       if (lane == 0) {
         mk[w] = m0;
@@ -570,7 +575,7 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
     }
     if (lane == 0) {
       const uint32_t f = (any0 ? 1u : 0u) | (any1 ? 2u : 0u) | (any2 ? 4u : 0u);
-      sfl[g][gene] = (uint8_t)f;
+      sflb[g * GM + gene] = (uint8_t)f;
       a.sflags[im.goff * 2 + g * G + gene] = (uint8_t)f;
     }
   }
@@ -601,7 +606,7 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
         done = true;
       }
       bool inRange = valid && lane <= limit;
-      if (inRange) rinfo[wave][col] = w;
+      if (inRange) rinfo_w[col] = w;
       if (__any(inRange && !ev)) {
         if (lane == 0) atomicOr(&a.status[item], ST_NOT_EVALUATED);
       }
@@ -613,7 +618,7 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
       uint32_t flip = loo ^ (full ? 0x1FFu : 0u);
       bool isf = inRange && flip != 0;
       uint64_t fm = __ballot(isf);
-      if (isf) flips[wave][nflip + __popcll(fm & lt_mask)] = (uint32_t)col | (flip << 16);
+      if (isf) flips_w[nflip + __popcll(fm & lt_mask)] = (uint32_t)col | (flip << 16);
       nflip += __popcll(fm);
       nSolid += __popcll(sm_ & (limit == 63 ? ~0ULL : ((2ULL << limit) - 1)));
       wave_sync();
@@ -633,12 +638,12 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
     const int ng = ncls[g];
     const uint8_t *xt = xs + (size_t)t * NP;
     for (int k = lane; k < ng; k += 64) {
-      int sid = cls[g][k];
+      int sid = (g ? cls1 : cls0)[k];
       int vt = xt[sid];
       if (vt >= 3) continue;
       bool changed = false;
       for (int f = 0; f < nflip && !changed; f++) {
-        uint32_t e = flips[wave][f];
+        uint32_t e = flips_w[f];
         int vc = xs[(size_t)(e & 0xffff) * NP + sid];
         if (vc < 3 && ((e >> 16) >> (vt * 3 + vc)) & 1) changed = true;
       }
@@ -646,10 +651,10 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
       uint16_t nl[EDDY_MAX_MP];
       int cnt = 0, fi = 0;
       for (int col = 0; col <= C && cnt < mp; col++) {
-        uint32_t w = rinfo[wave][col];
+        uint32_t w = rinfo_w[col];
         int bit = (w >> g) & 1;
-        if (fi < nflip && (int)(flips[wave][fi] & 0xffff) == col) {
-          uint32_t e = flips[wave][fi];
+        if (fi < nflip && (int)(flips_w[fi] & 0xffff) == col) {
+          uint32_t e = flips_w[fi];
           int vc = xs[(size_t)col * NP + sid];
           if (vc < 3) bit ^= ((e >> 16) >> (vt * 3 + vc)) & 1;
           fi++;
