@@ -102,7 +102,7 @@ def main():
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--nsets", type=int, default=None)
     ap.add_argument("--train-sets", type=int, default=100)
-    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--epochs", type=int, default=2)
     ap.add_argument("--train-rows", type=int, default=131072)
     ap.add_argument("--margin", type=float, default=1.0)
     ap.add_argument("--plain-sets", type=int, default=10)
@@ -124,7 +124,7 @@ def main():
     setup_s = time.perf_counter() - t_setup
     nsets = len(wl.sets)
     prm = wl.params(mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=1)
-    cfg = eddy_amd.learn_cfg(train_sets=args.train_sets, epochs=args.epochs, batch=8192, lr=3e-3,
+    cfg = eddy_amd.learn_cfg(train_sets=args.train_sets, epochs=args.epochs, batch=16384, lr=3e-3,
                              margin=args.margin, seed=1, max_rows=args.train_rows)
 
     print("bench: setup %.1fs, %d sets, N=%d" % (setup_s, nsets, wl.N), file=sys.stderr, flush=True)

@@ -296,6 +296,13 @@ __device__ __forceinline__ uint32_t loo_of_pair(const KArgs &a, const ItemMeta &
   return d_loo_mask(tt, full, prior_bit(a, sm, p), a, sm, cpairs, ntests);
 }
 
+// Info words that this kernel's own atomics update are read at device scope
+// (past the CU's vector cache), so a read after another wave's atomicOr sees
+// its bits.
+__device__ __forceinline__ uint32_t info_ld(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // full-test decision of (pair, class), evaluated here when the count kernel
 // skipped it (outside the learned window)
 __device__ __forceinline__ int full_of(const KArgs &a, const ItemMeta &im, const SetMeta &sm, long long p, int g,
@@ -330,39 +337,9 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
   int ntests = 0, fb = 0, fbc = 0;
   unsigned long long tph = 0;
   phase_mark(a, -1, tph);
-  // ---- A0 (learned window): the count kernel ran the full tests of row t
-  // only up to column win[t]; a row whose window holds fewer full edges than
-  // the plan needs gets all of its later full tests here, packed over the
-  // item's rows (the tests the count kernel would have run)
-  const int *win = (a.win && im.perm > 0) ? a.win + sm.genebase : nullptr;
-  if (win) {
-    for (int r = wave; r < R; r += EDDY_NWAVES) {  // a wave per entry, 64 columns at a time
-      const int t = r >> 1, g = r & 1;
-      const int want = mp + (a.plan_extra ? a.plan_extra[(im.goff * 2 + g * G + t)] : 0);
-      const int W = min(win[t], t - 1);
-      int hits = 0;
-      for (int base = 0; base <= W && hits < want; base += 64) {
-        const int col = base + lane;
-        const int f = col <= W ? (int)((a.info[im.infooff + pair_index(t, col)] >> g) & 1) : 0;
-        hits += __popcll(__ballot(f));
-      }
-      if (lane == 0) off[r] = hits < want ? t - 1 - W : 0;
-    }
-    __syncthreads();
-    const int ext = block_exscan(off, R, wsum);
-    for (int e = tid; e < ext; e += 256) {
-      int lo = 0, hi = R - 1;  // last r with off[r] <= e
-      while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= e) lo = mid; else hi = mid - 1;
-      }
-      const int t = lo >> 1, g = lo & 1;
-      const int col = min(win[t], t - 1) + 1 + (e - off[lo]);
-      const long long p = pair_index(t, col);
-      full_of(a, im, sm, p, g, a.info[im.infooff + p], cpairs, &ntests);
-    }
-    __syncthreads();
-  }
+  // With the learned window the count kernel ran the full tests of row t
+  // only up to column win[t]; the plan below runs the missing ones it needs
+  // (64 columns per wave step).
   // ---- A: plan P = column of the (mp + extra)-th full-test edge, a wave per
   // entry scanning 64 columns per ballot ------------------------------------
   for (int r = wave; r < R; r += EDDY_NWAVES) {
@@ -374,7 +351,7 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
       int f = 0;
       if (col < t) {
         const long long pc = pair_index(t, col);
-        f = full_of(a, im, sm, pc, g, a.info[im.infooff + pc], cpairs, &ntests);
+        f = full_of(a, im, sm, pc, g, info_ld(&a.info[im.infooff + pc]), cpairs, &ntests);
       }
       const uint64_t b = __ballot(f);
       const int nb = __popcll(b);
@@ -406,8 +383,7 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
     }
     const int t = lo >> 1, g = lo & 1, col = e - off[lo];
     const long long p = pair_index(t, col);
-    const uint32_t w = a.info[im.infooff + p];
-    const int full = (w >> g) & 1;
+    const int full = full_of(a, im, sm, p, g, info_ld(&a.info[im.infooff + p]), cpairs, &ntests);
     const uint32_t m = loo_of_pair(a, im, sm, p, g, full, cpairs, &ntests);
     atomicOr(&a.info[im.infooff + p], (m << INFO_LOO_SHIFT(g)) | INFO_EVAL(g));
     if (full && m == 0x1FFu) atomicAdd(&solid[lo], 1);
@@ -415,25 +391,50 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
   __syncthreads();
   phase_mark(a, 11, tph);
   // ---- C: a row with fewer than mp solid edges up to its plan continues
-  // past it (a wave per row, 64 columns per step) until mp solid edges or
-  // the row's end.  Correctness never depends on the plan. ----------------
+  // past it (a wave per row) until mp solid edges or the row's end: the next
+  // columns up to the (mp - solid)-th further full-test edge all lie before
+  // the cutoff (solid edges are full edges), so they are tested together,
+  // 64 per step.  Correctness never depends on the plan. ------------------
   for (int r = wave; r < R; r += EDDY_NWAVES) {
     const int t = r >> 1, g = r & 1;
     int have = solid[r];
     if (have >= mp || plan[r] >= t - 1) continue;  // wave-uniform
     fb += lane == 0;
-    for (int base = plan[r] + 1; base < t && have < mp; base += 64) {
-      const int col = base + lane;
-      int sol = 0;
-      if (col < t) {
-        const long long p = pair_index(t, col);
-        const int full = full_of(a, im, sm, p, g, a.info[im.infooff + p], cpairs, &ntests);
-        const uint32_t m = loo_of_pair(a, im, sm, p, g, full, cpairs, &ntests);
-        atomicOr(&a.info[im.infooff + p], (m << INFO_LOO_SHIFT(g)) | INFO_EVAL(g));
-        sol = full && m == 0x1FFu;
-        fbc++;
+    int base = plan[r] + 1;
+    while (have < mp && base < t) {
+      // end of this round: the (mp - have)-th full edge at or after base
+      int need = mp - have, P2 = t - 1;
+      for (int b0 = base; b0 < t; b0 += 64) {
+        const int col = b0 + lane;
+        int f = 0;
+        if (col < t) {
+          const long long pc = pair_index(t, col);
+          f = full_of(a, im, sm, pc, g, info_ld(&a.info[im.infooff + pc]), cpairs, &ntests);
+        }
+        const uint64_t b = __ballot(f);
+        const int nb = __popcll(b);
+        if (nb >= need) {
+          uint64_t x = b;
+          for (int i = 1; i < need; i++) x &= x - 1;
+          P2 = b0 + __ffsll((long long)x) - 1;
+          break;
+        }
+        need -= nb;
       }
-      have += __popcll(__ballot(sol));
+      for (int b0 = base; b0 <= P2; b0 += 64) {
+        const int col = b0 + lane;
+        int sol = 0;
+        if (col <= P2) {
+          const long long p = pair_index(t, col);
+          const int full = full_of(a, im, sm, p, g, info_ld(&a.info[im.infooff + p]), cpairs, &ntests);
+          const uint32_t m = loo_of_pair(a, im, sm, p, g, full, cpairs, &ntests);
+          atomicOr(&a.info[im.infooff + p], (m << INFO_LOO_SHIFT(g)) | INFO_EVAL(g));
+          sol = full && m == 0x1FFu;
+          fbc++;
+        }
+        have += __popcll(__ballot(sol));
+      }
+      base = P2 + 1;
     }
   }
   atomicAdd(&tcount, ntests);

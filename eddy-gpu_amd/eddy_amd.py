@@ -78,7 +78,7 @@ class LearnReport(C.Structure):
                 ("tests_collect", C.c_longlong), ("tests_apply", C.c_longlong), ("trained", C.c_int)]
 
 
-def learn_cfg(train_sets=100, epochs=3, batch=8192, lr=3e-3, margin=1.0, seed=1, max_rows=131072):
+def learn_cfg(train_sets=100, epochs=2, batch=16384, lr=3e-3, margin=1.0, seed=1, max_rows=131072):
     return LearnCfg(train_sets, epochs, batch, lr, margin, seed, max_rows)
 
 
