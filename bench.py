@@ -109,6 +109,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run one device-to-device copy of MIB MiB (a kernel of known traffic for "
+                         "scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU sharding is not built (see DESIGN.md)",
@@ -116,6 +119,15 @@ def main():
         sys.exit(2)
     ensure_built()
     import eddy_amd
+
+    if args.calibrate_copy > 0:
+        import torch
+
+        src = torch.empty(args.calibrate_copy << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print("bench: calibration copy of %d MiB done" % args.calibrate_copy, file=sys.stderr, flush=True)
 
     t_setup = time.perf_counter()
     wl = eddy_amd.Workload(args.config, nsets=args.nsets)
@@ -138,7 +150,7 @@ def main():
     items = 0
     host_stream_ms = 0.0
     sched = [0, 0, 0, 0]
-    stats = [0, 0, 0, 0]
+    stats = [0] * 6
     phase = [0.0] * 16
     reps = []
     t0 = time.perf_counter()
@@ -153,6 +165,7 @@ def main():
         host_stream_ms += tm.host_stream_ms
         for k in range(4):
             sched[k] += tm.sched[k]
+        for k in range(6):
             stats[k] += tm.stats[k]
         for k in range(16):
             phase[k] += tm.phase_ms[k]
@@ -223,8 +236,9 @@ def main():
         roof = {"kernel": "k_count_test", "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_INT8_OPS / 1e12,
                 "unit": "TOP/s", "frac": achieved / PEAK_INT8_OPS, "traffic": None}
     elif dom == 3:
-        # BDeu: every (family, sample) reads the child's and parents' states
-        achieved = per_launch(stats[0]) / dom_avg_s
+        # BDeu histograms: (1 + |pa|) state bytes per sample and class, plus
+        # 4 q r bytes of bins, per distinct family and scoring class
+        achieved = per_launch(stats[0] + stats[4]) / dom_avg_s
         roof = {"kernel": "k_score", "bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
                 "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None}
     elif dom == 1:
@@ -269,8 +283,9 @@ def main():
         "host_stream_ms_per_step": host_stream_ms / args.steps,
         "schedule": {"rows_past_plan": sched[0] // args.steps, "cols_tested_past_plan": sched[1] // args.steps,
                      "planned_entries": sched[2] // args.steps, "rows": sched[3] // args.steps},
-        "score_stats": {"hist_bytes": stats[0] // args.steps, "families": stats[1] // args.steps,
-                        "unique_networks": stats[2] // args.steps, "events": stats[3] // args.steps},
+        "score_stats": {"hist_bytes": stats[0] // args.steps, "bin_bytes": stats[4] // args.steps,
+                        "families": stats[1] // args.steps, "unique_networks": stats[2] // args.steps,
+                        "events": stats[3] // args.steps},
         "phase_wg_ms": [round(x / args.steps, 1) for x in phase],
         "setup_s": setup_s,
         "learned": learned,

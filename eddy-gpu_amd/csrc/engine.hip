@@ -1295,7 +1295,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
   unsigned long long cnt[32];
   HIPCHK(ctx, hipMemcpy(cnt, ctx->b_counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
   for (int k = 0; k < 4; k++) ctx->timing.sched[k] = (long long)cnt[k];
-  for (int k = 0; k < 4; k++) ctx->timing.stats[k] = (long long)cnt[4 + k];
+  for (int k = 0; k < 6; k++) ctx->timing.stats[k] = (long long)cnt[4 + k];
   int wcr = 0;
   if (hipDeviceGetAttribute(&wcr, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess || wcr <= 0) wcr = 100000;
   for (int k = 0; k < 16; k++) ctx->timing.phase_ms[k] = (double)cnt[16 + k] / (double)wcr;  // kHz -> ms

@@ -998,12 +998,6 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   double *fscore = a.famscore + im.foff * 2;
   const int NF = F;
   if (a.counters && tid == 0) {
-    unsigned long long hb = 0;
-    for (int f = 0; f < NF; f++) {
-      uint64_t key = a.famkey[im.foff + f] - 1;
-      hb += (1 + ((key >> 10) & 3)) * (unsigned long long)(a.n1 + a.n2);
-    }
-    atomicAdd(&a.counters[4], hb);                          // histogram bytes read (both classes)
     atomicAdd(&a.counters[5], (unsigned long long)NF);      // families
     atomicAdd(&a.counters[6], (unsigned long long)UU);      // unique networks
     atomicAdd(&a.counters[7], (unsigned long long)E);       // events
@@ -1014,6 +1008,7 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   // reference's double loop per lane (one lane per parent configuration j),
   // the outer sum over j by lane 0, both in the reference's order
   uint64_t *mst = (uint64_t *)dyn + (size_t)wave * 12 * a.bitw;  // [4 genes][3 states][W]
+  unsigned long long st_hist = 0, st_bins = 0;                    // algorithmic bytes (lane 0)
   for (int fg = wave; fg < 2 * NF; fg += EDDY_NWAVES) {
     const int f = fg >> 1, g = fg & 1;
     const uint64_t key = a.famkey[im.foff + f] - 1;
@@ -1073,8 +1068,16 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
       if (lane == 0)
         for (int j = 0; j < q; j++) sum2 = sum2 + lgv[wave][j];
     }
-    if (lane == 0) fscore[f * 2 + g] = sum2;
+    if (lane == 0) {
+      fscore[f * 2 + g] = sum2;
+      st_hist += (unsigned long long)(1 + np) * (unsigned long long)(g ? a.n2 : a.n1);
+      if (r > 0 && q > 0) st_bins += (unsigned long long)(4 * q * r);
+    }
     wave_sync();
+  }
+  if (a.counters && lane == 0) {
+    atomicAdd(&a.counters[4], st_hist);  // histogram state bytes read (reference algorithm)
+    atomicAdd(&a.counters[8], st_bins);  // histogram bin bytes
   }
   __syncthreads();
   phase_mark(a, 4, tph);

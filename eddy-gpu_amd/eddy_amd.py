@@ -63,7 +63,7 @@ KERNELS = ["count_test", "loo", "rows", "score", "pvalue", "gather"]
 class Timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("kernel_ms", C.c_double * 6), ("launches", C.c_longlong * 6),
                 ("items", C.c_longlong), ("pairs", C.c_longlong), ("tests", C.c_longlong),
-                ("host_stream_ms", C.c_double), ("sched", C.c_longlong * 4), ("stats", C.c_longlong * 4),
+                ("host_stream_ms", C.c_double), ("sched", C.c_longlong * 4), ("stats", C.c_longlong * 6),
                 ("phase1_ms", C.c_double), ("phase_ms", C.c_double * 16)]
 
 

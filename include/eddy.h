@@ -187,8 +187,11 @@ typedef struct {
   long long sched[4];                /* cutoff schedule: rows past their plan,
                                         columns tested past plans, planned
                                         (pair, class) entries, rows */
-  long long stats[4];                /* BDeu histogram bytes read, families
-                                        scored, unique networks, events */
+  long long stats[6];                /* BDeu histogram bytes read (child and
+                                        parent states per sample, both
+                                        classes), families scored, unique
+                                        networks, events, histogram bin bytes
+                                        (4 q r per family and class), 0 */
   double phase1_ms;                  /* host wall time before the first chunk
                                         (set preparation, features, windows) */
   double phase_ms[16];               /* workgroup time per phase summed over
