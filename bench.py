@@ -251,6 +251,14 @@ def main():
                 "unit": "GB/s", "frac": None, "traffic": None}
     roof["avg_launch_ms"] = dom_avg_s * 1000.0
     roof["launches_per_step"] = launches[dom] / args.steps
+    # HBM bytes per launch of the same kernel from the committed byte-counter
+    # profile of this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_learned_job_5000x600.json")
+    if args.config == 3 and args.nsets is None and os.path.exists(tpath):
+        prof = json.load(open(tpath))["per_dispatch"].get(roof["kernel"])
+        if prof:
+            roof["traffic"] = prof["read_bytes"] + prof["write_bytes"]
+            roof["traffic_source"] = os.path.relpath(tpath, ROOT)
 
     cpu = None
     if not args.no_cpu:
