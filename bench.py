@@ -251,13 +251,15 @@ def main():
                 "unit": "GB/s", "frac": None, "traffic": None}
     roof["avg_launch_ms"] = dom_avg_s * 1000.0
     roof["launches_per_step"] = launches[dom] / args.steps
-    # HBM bytes per launch of the same kernel from the committed byte-counter
-    # profile of this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+    # HBM bytes of the same kernel from the committed byte-counter profile of
+    # this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated): its bytes per
+    # step over this run's launches per step (chunk boundaries follow the
+    # stream replay's timing, the work per step does not)
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_learned_job_5000x600.json")
     if args.config == 3 and args.nsets is None and os.path.exists(tpath):
-        prof = json.load(open(tpath))["per_dispatch"].get(roof["kernel"])
+        prof = json.load(open(tpath))["kernels"].get(roof["kernel"])
         if prof:
-            roof["traffic"] = prof["read_bytes"] + prof["write_bytes"]
+            roof["traffic"] = prof["bytes_per_step"] / max(1e-9, roof["launches_per_step"])
             roof["traffic_source"] = os.path.relpath(tpath, ROOT)
 
     cpu = None
