@@ -137,7 +137,7 @@ def main():
     nsets = len(wl.sets)
     prm = wl.params(mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=1)
     cfg = eddy_amd.learn_cfg(train_sets=args.train_sets, epochs=args.epochs, batch=16384, lr=3e-3,
-                             margin=args.margin, seed=1, max_rows=args.train_rows)
+                             margin=args.margin, seed=1, max_rows=args.train_rows, label_perms=10)
 
     print("bench: setup %.1fs, %d sets, N=%d" % (setup_s, nsets, wl.N), file=sys.stderr, flush=True)
     for w in range(args.warmup):
@@ -169,8 +169,9 @@ def main():
             stats[k] += tm.stats[k]
         for k in range(16):
             phase[k] += tm.phase_ms[k]
-        print("bench: step done, total %.2fs (collect %.0f ms, train %.0f ms, apply %.0f ms)"
-              % (time.perf_counter() - t0, rep.collect_ms, rep.train_ms, rep.apply_ms), file=sys.stderr, flush=True)
+        print("bench: step done, total %.2fs (collect %.0f ms, then %.0f ms; training %.1f ms on a second stream, "
+              "windows from set %d)" % (time.perf_counter() - t0, rep.collect_ms, rep.apply_ms, rep.train_ms,
+                                       rep.first_windowed_set), file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     ms_per_step = 1000.0 * elapsed / args.steps
     value = nsets * args.steps / elapsed
@@ -183,7 +184,8 @@ def main():
         "train_rows": min(rep.train_rows, args.train_rows) if args.train_rows > 0 else rep.train_rows,
         "epochs": args.epochs,
         "loss_initial": rep.loss_initial, "loss_final": rep.loss_final, "margin": args.margin,
-        "collect_ms": rep.collect_ms, "train_ms": rep.train_ms, "apply_ms": rep.apply_ms,
+        "collect_ms": rep.collect_ms, "train_gpu_ms_overlapped": rep.train_ms, "apply_ms": rep.apply_ms,
+        "first_windowed_set": rep.first_windowed_set, "label_perms": 10,
         "tests_applied_sets": rep.tests_apply,
     }
 
@@ -199,7 +201,18 @@ def main():
         same = same and all(bool((eng.js(s).view("u8") == js_head[s].view("u8")).all()) for s in range(nsets))
         k = min(args.train_sets, nsets)
         tests_u = sum(r.tests_eval for r in ures[k:])
+        # and with every full test deferred to the plan (no window at all)
+        zprm = wl.params(mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=3)
+        eng.reset_stream()
+        tz = time.perf_counter()
+        zres = eng.run_sets(wl.sets, zprm)
+        dtz = time.perf_counter() - tz
+        same = same and all((a.status, a.perms_run, a.U0) == (b.status, b.perms_run, b.U0) for a, b in zip(zres, res))
+        same = same and all(bool((eng.js(s).view("u8") == js_head[s].view("u8")).all()) for s in range(nsets))
+        tests_z = sum(r.tests_eval for r in zres[k:])
         compare = {"unlearned_sets_per_s": nsets / dtu, "unlearned_ms": dtu * 1000.0,
+                   "deferred_full_tests_sets_per_s": nsets / dtz, "deferred_full_tests_ms": dtz * 1000.0,
+                   "tests_applied_sets_deferred": tests_z,
                    "learned_ms_per_step": ms_per_step,
                    "tests_applied_sets_unlearned": tests_u, "tests_applied_sets_learned": rep.tests_apply,
                    "test_reduction_applied_sets": 1.0 - rep.tests_apply / max(1, tests_u),

@@ -135,6 +135,7 @@ int main(int argc, char **argv) {
     lc.epochs = 2;
     lc.batch = 16384;
     lc.max_rows = 131072;
+    lc.label_perms = 10;
     lc.lr = 3e-3f;
     lc.margin = 1.0f;
     lc.seed = 1;

@@ -101,16 +101,21 @@ struct eddy_ctx {
   std::map<int, eddy_edge_bounds> bounds_cache;
   eddy_timing timing{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // the learned job trains on a second stream while chunks keep running
+  hipStream_t stream2 = nullptr;
+  hipEvent_t train_ev = nullptr;
+  std::vector<int> train_order;  // host copy of the epochs' row orders (alive until the upload ran)
   // learned proposal: network parameters + Adam state, per-run feature,
   // label and window arrays (device pointers handed to the kernels)
   DevBuf b_mlp, b_mlpm, b_mlpv, b_mlpg, b_mlploss, b_feat, b_labhits, b_score, b_win, b_mx, b_my, b_mxb, b_myb,
-      b_midx, b_mscr, b_mlbase, b_mlinv;
+      b_midx, b_mscr, b_mlbase, b_mlinv, b_win2;
   bool mlp_ready = false;
   int mlp_step = 0;
   float mlp_kappa = 1.0f;
   float *k_feat = nullptr;
   unsigned *k_labhits = nullptr;
   const int *k_win = nullptr;
+  int k_label_perms = 0;
 };
 
 #define HIPCHK(ctx, x)                                                                  \
@@ -163,6 +168,8 @@ extern "C" int eddy_ctx_create(int device, eddy_ctx **out) {
   eddy_ctx *ctx = new eddy_ctx();
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->train_ev, hipEventDisableTiming) != hipSuccess ||
       hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
     delete ctx;
     return EDDY_ERR_HIP;
@@ -187,13 +194,16 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
                    &ctx->b_mlpg, &ctx->b_mlploss, &ctx->b_feat, &ctx->b_labhits, &ctx->b_score, &ctx->b_win,
                    &ctx->b_mx, &ctx->b_my, &ctx->b_mxb, &ctx->b_myb, &ctx->b_midx, &ctx->b_mscr, &ctx->b_mlbase,
                    &ctx->b_mlinv, &ctx->p0.info, &ctx->p0.lcnt, &ctx->p0.lfull, &ctx->p0.rep, &ctx->p0.lval,
-                   &ctx->p0.ev, &ctx->p0.famkey, &ctx->p0.famscore, &ctx->p0.slices, &ctx->b_bits};
+                   &ctx->p0.ev, &ctx->p0.famkey, &ctx->p0.famscore, &ctx->p0.slices, &ctx->b_bits,
+                   &ctx->b_win2};
   for (DevBuf *b : all)
     if (b->p) hipFree(b->p);
   if (ctx->dX) hipFree(ctx->dX);
   if (ctx->dLG) hipFree(ctx->dLG);
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
+  if (ctx->train_ev) hipEventDestroy(ctx->train_ev);
+  if (ctx->stream2) hipStreamDestroy(ctx->stream2);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
   return EDDY_OK;
@@ -463,6 +473,7 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
   a.feat = ctx->k_feat;
   a.labhits = ctx->k_labhits;
   a.win = ctx->k_win;
+  a.label_perms = ctx->k_label_perms;
   a.counters = (unsigned long long *)ctx->b_counters.p;
   a.bits = (uint64_t *)ctx->b_bits.p;
   a.bitw = bitw;
@@ -867,8 +878,9 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
 
 // ---------------------------------------------------------------------------
 static int mlp_train_dev(eddy_ctx *ctx, const float *dX, const float *dY, long long n, long long max_rows,
-                         int epochs, int batch, float lr, uint64_t seed);
-static int mlp_eval_loss(eddy_ctx *ctx, const float *dX, const float *dY, long long n, float *out);
+                         int epochs, int batch, float lr, uint64_t seed, hipStream_t st);
+static int mlp_grad_dev(eddy_ctx *ctx, const float *dX, const float *dY, const int *dIdx, int b0, int n,
+                        float *dLoss, hipStream_t st);
 
 namespace {
 
@@ -966,7 +978,8 @@ int compute_features(eddy_ctx *ctx, const std::vector<SetMeta> &meta, const std:
 
 // Window of every row of sets [from, nsets): network scores of the pairs,
 // then the first column where their running sum reaches mp + margin.
-int compute_windows(eddy_ctx *ctx, const std::vector<SetMeta> &meta, int from, int mp) {
+int compute_windows(eddy_ctx *ctx, const std::vector<SetMeta> &meta, int from, int mp, DevBuf &out,
+                    hipStream_t st) {
   const int nsets = (int)meta.size();
   if (from >= nsets) return EDDY_OK;
   long long sumc = 0, sumg = 0;
@@ -975,17 +988,16 @@ int compute_windows(eddy_ctx *ctx, const std::vector<SetMeta> &meta, int from, i
     sumg += m.G;
   }
   GROW(b_score, sizeof(float) * std::max(1LL, sumc));
-  GROW(b_win, sizeof(int) * std::max(1LL, sumg));
+  CHK(grow(ctx, out, sizeof(int) * std::max(1LL, sumg)));
   const long long r0 = meta[from].pairbase, nr = sumc - r0;
   if (nr > INT32_MAX) {
     ctx->err = "too many pairs for one scoring launch";
     return EDDY_ERR_CAPACITY;
   }
   HIPCHK(ctx, mlp_forward((const float *)ctx->b_feat.p + r0 * MLP_IN, (int)nr, (const float *)ctx->b_mlp.p,
-                          (float *)ctx->b_score.p + r0, ctx->stream));
-  k_window<<<nsets - from, 256, 0, ctx->stream>>>((const SetMeta *)ctx->b_sets.p + from, nsets - from,
-                                                 (const float *)ctx->b_score.p, mp, ctx->mlp_kappa,
-                                                 (int *)ctx->b_win.p);
+                          (float *)ctx->b_score.p + r0, st));
+  k_window<<<nsets - from, 256, 0, st>>>((const SetMeta *)ctx->b_sets.p + from, nsets - from,
+                                        (const float *)ctx->b_score.p, mp, ctx->mlp_kappa, (int *)out.p);
   HIPCHK(ctx, hipGetLastError());
   return EDDY_OK;
 }
@@ -1029,6 +1041,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
   ctx->k_labhits = nullptr;
   ctx->k_win = nullptr;
   eddy_learn_report r{};
+  r.first_windowed_set = -1;
 
   // which sets log labels, which run with windows
   int collect_upto = 0, win_from = nsets;
@@ -1044,8 +1057,12 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     }
     win_from = 0;
   }
+  // learned == 3: no full tests in the count kernel past permutation 0; the
+  // loo kernel runs each one its plan reaches
+  const bool lazy = batched && !lc && p->learned == 3 && p->schedule == EDDY_TESTS_CUTOFF;
+  if (lazy) win_from = 0;
   if (p->schedule != EDDY_TESTS_CUTOFF) win_from = nsets;
-  if (collect_upto > 0 || win_from < nsets) {
+  if (collect_upto > 0 || (win_from < nsets && !lazy)) {
     GROW(b_feat, sizeof(float) * MLP_IN * std::max(1LL, sumc));
     CHK(compute_features(ctx, meta, truebits));
   }
@@ -1054,9 +1071,29 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     HIPCHK(ctx, hipMemsetAsync(ctx->b_labhits.p, 0, sizeof(unsigned) * std::max(1LL, sumc), ctx->stream));
   }
   bool windows_ready = false;
-  if (win_from == 0 && nsets > 0) {
-    CHK(compute_windows(ctx, meta, 0, p->mp));
+  if (lazy && nsets > 0) {
+    long long sumg = 0;
+    for (const SetMeta &m : meta) sumg += m.G;
+    GROW(b_win, sizeof(int) * std::max(1LL, sumg));
+    HIPCHK(ctx, hipMemsetAsync(ctx->b_win.p, 0xFF, sizeof(int) * std::max(1LL, sumg), ctx->stream));  // -1
     windows_ready = true;
+  } else if (win_from == 0 && nsets > 0) {
+    CHK(compute_windows(ctx, meta, 0, p->mp, ctx->b_win, ctx->stream));
+    windows_ready = true;
+  }
+  // the learned job: every set runs with deferred full tests (windows of -1)
+  // until the network is trained; logged sets also run every full test of
+  // their first label_perms permutations
+  bool training = false, trained_windows = false;
+  const int label_perms = lc ? (lc->label_perms > 0 ? std::min(lc->label_perms, perms - 1) : perms - 1) : 0;
+  hipEvent_t tr0 = nullptr, tr1 = nullptr;
+  if (lc && nsets > 0) {
+    long long sumg = 0;
+    for (const SetMeta &m : meta) sumg += m.G;
+    GROW(b_win, sizeof(int) * std::max(1LL, sumg));
+    HIPCHK(ctx, hipMemsetAsync(ctx->b_win.p, 0xFF, sizeof(int) * std::max(1LL, sumg), ctx->stream));  // -1
+    windows_ready = true;
+    win_from = 0;
   }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->timing.phase1_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1092,11 +1129,18 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
         return EDDY_OK;
       }
       ctx->k_labhits = chunk_first < collect_upto ? (unsigned *)ctx->b_labhits.p : nullptr;
+      ctx->k_label_perms = chunk_first < collect_upto ? label_perms : 0;
       ctx->k_win = (windows_ready && chunk_first >= win_from) ? (const int *)ctx->b_win.p : nullptr;
+      if (training && !trained_windows && hipEventQuery(ctx->train_ev) == hipSuccess) trained_windows = true;
+      if (trained_windows && chunk_first >= collect_upto) {
+        ctx->k_win = (const int *)ctx->b_win2.p;
+        if (r.first_windowed_set < 0) r.first_windowed_set = chunk_first;
+      }
       std::vector<long long> t(ch.items.size(), 0);
       CHK(run_chunk(ctx, meta, ch, p, perms, true, t));
       for (size_t i = 0; i < ch.items.size(); i++) tests_eval[ch.items[i].set] += t[i];
       ctx->k_labhits = nullptr;
+      ctx->k_label_perms = 0;
       ctx->k_win = nullptr;
       // JS of permutation 0 of the chunk's sets: one strided copy
       auto hj = std::chrono::steady_clock::now();
@@ -1141,7 +1185,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       return EDDY_OK;
     };
     while (s < nsets || !ch.items.empty()) {
-      const bool boundary = s == nsets || s == collect_upto || s == win_from;
+      const bool boundary = s == nsets || s == collect_upto || (!lc && s == win_from);
       // flush when the chunk is full, at a boundary, or when the next set's
       // labels are not ready yet: the GPU then works while the replay
       // catches up
@@ -1155,45 +1199,49 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       }
       if (s == nsets) break;
       if (lc && s == collect_upto && !trained) {
-        // all logged sets are done: train, then windows for the rest
+        // all logged sets are done: train on the second stream while the
+        // next chunks run with deferred full tests; chunks launched after the
+        // training's event use its windows
         trained = true;
-        auto tt = std::chrono::steady_clock::now();
-        r.collect_ms = std::chrono::duration<double, std::milli>(tt - t0).count();
+        r.collect_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         long long n = 0;
         std::vector<long long> base(std::max(1, collect_upto));
         std::vector<float> inv(std::max(1, collect_upto));
         for (int q = 0; q < collect_upto; q++) {
           base[q] = n;
           n += meta[q].c;
-          inv[q] = (!nan0[q] && perms > 1) ? (float)(1.0 / (2.0 * (perms - 1))) : 0.f;
+          inv[q] = (!nan0[q] && label_perms > 0) ? (float)(1.0 / (2.0 * label_perms)) : 0.f;
           r.tests_collect += tests_eval[q];
         }
         r.train_rows = n;
         if (n >= 64) {
+          hipStream_t st2 = ctx->stream2;
           GROW(b_my, sizeof(float) * n);
           GROW(b_mlbase, sizeof(long long) * collect_upto);
           GROW(b_mlinv, sizeof(float) * collect_upto);
-          HIPCHK(ctx, hipMemcpyAsync(ctx->b_mlbase.p, base.data(), sizeof(long long) * collect_upto,
-                                     hipMemcpyHostToDevice, ctx->stream));
-          HIPCHK(ctx, hipMemcpyAsync(ctx->b_mlinv.p, inv.data(), sizeof(float) * collect_upto, hipMemcpyHostToDevice,
-                                     ctx->stream));
-          HIPCHK(ctx, mlp_labels((const unsigned *)ctx->b_labhits.p, (const float *)ctx->b_mlinv.p,
-                                 (const long long *)ctx->b_mlbase.p, collect_upto, n, (float *)ctx->b_my.p,
-                                 ctx->stream));
+          HIPCHK(ctx, hipMemcpy(ctx->b_mlbase.p, base.data(), sizeof(long long) * collect_upto, hipMemcpyHostToDevice));
+          HIPCHK(ctx, hipMemcpy(ctx->b_mlinv.p, inv.data(), sizeof(float) * collect_upto, hipMemcpyHostToDevice));
           CHK(eddy_mlp_init(ctx, lc->seed));
+          hipEventCreate(&tr0);
+          hipEventCreate(&tr1);
+          hipEventRecord(tr0, st2);
+          HIPCHK(ctx, mlp_labels((const unsigned *)ctx->b_labhits.p, (const float *)ctx->b_mlinv.p,
+                                 (const long long *)ctx->b_mlbase.p, collect_upto, n, (float *)ctx->b_my.p, st2));
           const float *fx = (const float *)ctx->b_feat.p, *fy = (const float *)ctx->b_my.p;
-          CHK(mlp_eval_loss(ctx, fx, fy, n, &r.loss_initial));
-          CHK(mlp_train_dev(ctx, fx, fy, n, lc->max_rows, lc->epochs, lc->batch, lc->lr, lc->seed));
-          CHK(mlp_eval_loss(ctx, fx, fy, n, &r.loss_final));
+          const int nl = (int)std::min<long long>(n, 65536);
+          CHK(mlp_grad_dev(ctx, fx, fy, nullptr, 0, nl, (float *)ctx->b_mlploss.p + 1, st2));
+          CHK(mlp_train_dev(ctx, fx, fy, n, lc->max_rows, lc->epochs, lc->batch, lc->lr, lc->seed, st2));
+          CHK(mlp_grad_dev(ctx, fx, fy, nullptr, 0, nl, (float *)ctx->b_mlploss.p + 2, st2));
           const float kappa = ctx->mlp_kappa;
           ctx->mlp_kappa = lc->margin;
-          int rc = compute_windows(ctx, meta, collect_upto, p->mp);
+          int rc = compute_windows(ctx, meta, collect_upto, p->mp, ctx->b_win2, st2);
           ctx->mlp_kappa = kappa;
           CHK(rc);
-          windows_ready = true;
+          hipEventRecord(tr1, st2);
+          HIPCHK(ctx, hipEventRecord(ctx->train_ev, st2));
+          training = true;
           r.trained = 1;
         }
-        r.train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tt).count();
       }
       // next set's items
       auto hw = std::chrono::steady_clock::now();
@@ -1218,6 +1266,18 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     }
     prod.stop();
     ctx->timing.host_stream_ms += prod.ms;
+    if (training) {
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream2));
+      float lv[3];
+      HIPCHK(ctx, hipMemcpy(lv, ctx->b_mlploss.p, sizeof(lv), hipMemcpyDeviceToHost));
+      r.loss_initial = lv[1];
+      r.loss_final = lv[2];
+      float tms = 0;
+      hipEventElapsedTime(&tms, tr0, tr1);
+      r.train_ms = tms;
+    }
+    if (tr0) hipEventDestroy(tr0);
+    if (tr1) hipEventDestroy(tr1);
     for (int q = 0; q < nsets; q++) ran[q] = (nan0[q] || prod.stop0[q]) ? 1 : perms;
   } else {
     // plain: one set and one permutation at a time, like main.cu:538-1460
@@ -1660,11 +1720,13 @@ extern "C" int eddy_mlp_forward(eddy_ctx *ctx, const float *X, int n, float *pro
   return EDDY_OK;
 }
 
-// gradient and loss over device rows (idx[b0 .. b0 + n) or rows b0 ..)
-static int mlp_grad_dev(eddy_ctx *ctx, const float *dX, const float *dY, const int *dIdx, int b0, int n) {
+// gradient and loss over device rows (idx[b0 .. b0 + n) or rows b0 ..); the
+// mean loss goes to dLoss
+static int mlp_grad_dev(eddy_ctx *ctx, const float *dX, const float *dY, const int *dIdx, int b0, int n,
+                        float *dLoss, hipStream_t st) {
   GROW(b_mscr, sizeof(float) * mlp_part_floats(n));
   HIPCHK(ctx, mlp_gradient(dX, dY, dIdx, b0, n, (const float *)ctx->b_mlp.p, (float *)ctx->b_mscr.p,
-                           (float *)ctx->b_mlpg.p, (float *)ctx->b_mlploss.p, ctx->stream));
+                           (float *)ctx->b_mlpg.p, dLoss, st));
   return EDDY_OK;
 }
 
@@ -1673,7 +1735,8 @@ extern "C" int eddy_mlp_gradient(eddy_ctx *ctx, const float *X, const float *Y, 
   if (!ctx->mlp_ready) return EDDY_ERR_STATE;
   hipSetDevice(ctx->device);
   CHK(mlp_upload(ctx, X, Y, n));
-  CHK(mlp_grad_dev(ctx, (const float *)ctx->b_mx.p, (const float *)ctx->b_my.p, nullptr, 0, n));
+  CHK(mlp_grad_dev(ctx, (const float *)ctx->b_mx.p, (const float *)ctx->b_my.p, nullptr, 0, n,
+                   (float *)ctx->b_mlploss.p, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   HIPCHK(ctx, hipMemcpy(grad, ctx->b_mlpg.p, sizeof(float) * MLP_NPARAMS, hipMemcpyDeviceToHost));
   if (loss) HIPCHK(ctx, hipMemcpy(loss, ctx->b_mlploss.p, sizeof(float), hipMemcpyDeviceToHost));
@@ -1682,7 +1745,8 @@ extern "C" int eddy_mlp_gradient(eddy_ctx *ctx, const float *X, const float *Y, 
 
 // mean loss over (up to) the first 65536 rows with the current network
 static int mlp_eval_loss(eddy_ctx *ctx, const float *dX, const float *dY, long long n, float *out) {
-  CHK(mlp_grad_dev(ctx, dX, dY, nullptr, 0, (int)std::min<long long>(n, 65536)));
+  CHK(mlp_grad_dev(ctx, dX, dY, nullptr, 0, (int)std::min<long long>(n, 65536), (float *)ctx->b_mlploss.p,
+                   ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   HIPCHK(ctx, hipMemcpy(out, ctx->b_mlploss.p, sizeof(float), hipMemcpyDeviceToHost));
   return EDDY_OK;
@@ -1690,38 +1754,36 @@ static int mlp_eval_loss(eddy_ctx *ctx, const float *dX, const float *dY, long l
 
 // Adam over shuffled mini-batches of device rows dX/dY: all n rows, or a
 // uniform sample of max_rows of them (max_rows > 0); the last partial batch
-// of each epoch is dropped
+// of each epoch is dropped.  Every epoch's order is drawn on the host and
+// uploaded at once; nothing here waits for the stream.
 static int mlp_train_dev(eddy_ctx *ctx, const float *dX, const float *dY, long long n, long long max_rows,
-                         int epochs, int batch, float lr, uint64_t seed) {
+                         int epochs, int batch, float lr, uint64_t seed, hipStream_t st) {
   std::mt19937_64 rng(seed ^ 0xADA3ULL);
-  // two host copies of the order: epoch e shuffles into h[e & 1] while the
-  // GPU still runs epoch e - 1 (whose upload read h[(e - 1) & 1])
-  std::vector<int> h[2];
-  h[1].resize(n);
-  for (long long i = 0; i < n; i++) h[1][i] = (int)i;
+  std::vector<int> rows(n);
+  for (long long i = 0; i < n; i++) rows[i] = (int)i;
   if (max_rows > 0 && max_rows < n) {  // partial Fisher-Yates: a uniform sample
-    for (long long i = 0; i < max_rows; i++) std::swap(h[1][i], h[1][i + (long long)(rng() % (uint64_t)(n - i))]);
-    h[1].resize(max_rows);
+    for (long long i = 0; i < max_rows; i++) std::swap(rows[i], rows[i + (long long)(rng() % (uint64_t)(n - i))]);
+    rows.resize(max_rows);
     n = max_rows;
   }
   batch = (int)std::min<long long>(batch, n);
+  if (epochs <= 0 || batch <= 0) return EDDY_OK;
   GROW(b_mscr, sizeof(float) * mlp_part_floats(batch));
-  GROW(b_midx, sizeof(int) * n);
-  hipStream_t st = ctx->stream;
+  GROW(b_midx, sizeof(int) * n * epochs);
+  std::vector<int> &ord = ctx->train_order;
+  ord.resize((size_t)n * epochs);
   for (int ep = 0; ep < epochs; ep++) {
-    std::vector<int> &idx = h[ep & 1];
-    idx = h[(ep + 1) & 1];
-    for (long long i = n - 1; i > 0; i--) std::swap(idx[i], idx[rng() % (uint64_t)(i + 1)]);
-    HIPCHK(ctx, hipStreamSynchronize(st));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->b_midx.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    for (long long i = n - 1; i > 0; i--) std::swap(rows[i], rows[rng() % (uint64_t)(i + 1)]);
+    std::copy(rows.begin(), rows.end(), ord.begin() + (size_t)ep * n);
+  }
+  HIPCHK(ctx, hipMemcpyAsync(ctx->b_midx.p, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice, st));
+  for (int ep = 0; ep < epochs; ep++)
     for (long long b0 = 0; b0 + batch <= n; b0 += batch) {
       ctx->mlp_step++;
-      HIPCHK(ctx, mlp_train_step(dX, dY, (const int *)ctx->b_midx.p, (int)b0, batch, (float *)ctx->b_mlp.p,
+      HIPCHK(ctx, mlp_train_step(dX, dY, (const int *)ctx->b_midx.p, (int)(ep * n + b0), batch, (float *)ctx->b_mlp.p,
                                  (float *)ctx->b_mscr.p, (float *)ctx->b_mlpg.p, (float *)ctx->b_mlploss.p,
                                  (float *)ctx->b_mlpm.p, (float *)ctx->b_mlpv.p, lr, ctx->mlp_step, st));
     }
-  }
-  HIPCHK(ctx, hipStreamSynchronize(st));
   return EDDY_OK;
 }
 
@@ -1733,7 +1795,8 @@ extern "C" int eddy_mlp_train(eddy_ctx *ctx, const float *X, const float *Y, lon
   if (!ctx->mlp_ready) CHK(eddy_mlp_init(ctx, seed));
   CHK(mlp_upload(ctx, X, Y, n));
   const float *dX = (const float *)ctx->b_mx.p, *dY = (const float *)ctx->b_my.p;
-  CHK(mlp_train_dev(ctx, dX, dY, n, 0, epochs, batch, lr, seed));
+  CHK(mlp_train_dev(ctx, dX, dY, n, 0, epochs, batch, lr, seed, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   if (loss_out) CHK(mlp_eval_loss(ctx, dX, dY, n, loss_out));
   return EDDY_OK;
 }

@@ -105,6 +105,7 @@ struct KArgs {
   const uint8_t *plan_extra;  // [goff*2 + g*G + t] extra full edges in the plan (learned), or null
   float *feat;             // [(pairbase + p) * 12] learned-proposal features of permutation 0, or null
   unsigned *labhits;       // [pairbase + p] full-test edges over permutations > 0 (training labels), or null
+  int label_perms;         // > 0: permutations 1..label_perms run every full test (and log it) despite win
   const int *win;          // [genebase + t] last column whose full test the count kernel runs, or null
   unsigned long long *counters;  // schedule statistics (optional)
   uint64_t *bits;          // [((goff*2 + g*G + gene)*3 + v)*bitw + w]: class-g samples (class-list order) in state v

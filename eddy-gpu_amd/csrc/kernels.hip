@@ -241,12 +241,14 @@ __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
     if (a.feat_only) continue;
     uint32_t info = 0;
     int f1 = 0, f2 = 0;
-    if (!a.win || im.perm == 0 || gj <= a.win[sm.genebase + gi]) {
+    const bool logged = im.perm > 0 && im.perm <= a.label_perms;
+    if (!a.win || im.perm == 0 || logged || gj <= a.win[sm.genebase + gi]) {
       f1 = d_edge_test(t1, pr, a, sm, cpairs);
       f2 = d_edge_test(t2, pr, a, sm, cpairs);
       ntests += 2;
       info = (uint32_t)f1 | ((uint32_t)f2 << 1) | INFO_FEVAL(0) | INFO_FEVAL(1);
-      if (a.labhits && im.perm > 0) atomicAdd(&a.labhits[sm.pairbase + p], (unsigned)(f1 + f2));
+      if (a.labhits && im.perm > 0 && (a.label_perms == 0 || logged))
+        atomicAdd(&a.labhits[sm.pairbase + p], (unsigned)(f1 + f2));
     }
     if (a.plain) {
       info |= d_loo_mask(t1, f1, pr, a, sm, cpairs, &ntests) << INFO_LOO_SHIFT(0);

@@ -69,17 +69,18 @@ class Timing(C.Structure):
 
 class LearnCfg(C.Structure):
     _fields_ = [("train_sets", C.c_int), ("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_float),
-                ("margin", C.c_float), ("seed", C.c_uint64), ("max_rows", C.c_longlong)]
+                ("margin", C.c_float), ("seed", C.c_uint64), ("max_rows", C.c_longlong), ("label_perms", C.c_int)]
 
 
 class LearnReport(C.Structure):
     _fields_ = [("collect_ms", C.c_double), ("train_ms", C.c_double), ("apply_ms", C.c_double),
                 ("train_rows", C.c_longlong), ("loss_initial", C.c_float), ("loss_final", C.c_float),
-                ("tests_collect", C.c_longlong), ("tests_apply", C.c_longlong), ("trained", C.c_int)]
+                ("tests_collect", C.c_longlong), ("tests_apply", C.c_longlong), ("trained", C.c_int),
+                ("first_windowed_set", C.c_int)]
 
 
-def learn_cfg(train_sets=100, epochs=2, batch=16384, lr=3e-3, margin=1.0, seed=1, max_rows=131072):
-    return LearnCfg(train_sets, epochs, batch, lr, margin, seed, max_rows)
+def learn_cfg(train_sets=100, epochs=2, batch=16384, lr=3e-3, margin=1.0, seed=1, max_rows=131072, label_perms=10):
+    return LearnCfg(train_sets, epochs, batch, lr, margin, seed, max_rows, label_perms)
 
 
 class SynthCfg(C.Structure):

@@ -78,7 +78,8 @@ typedef struct {
   double alpha_ddn;      /* -pD (main.cu:150-153) */
   int mode;              /* eddy_mode */
   int schedule;          /* eddy_schedule */
-  int learned;           /* 1: learned proposal orders the cutoff schedule's tests */
+  int learned;           /* 1: learned proposal windows the cutoff schedule's full tests (needs a
+                            trained network); 3: no full tests before the plan asks for them */
 } eddy_params;
 
 /* One gene set after name lookup (main.cu:400-430) and prior loading
@@ -247,13 +248,15 @@ int eddy_mlp_train(eddy_ctx *ctx, const float *X, const float *Y, long long n, i
 int eddy_learn_collect(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *params, float *X,
                        float *Y, long long cap, long long *n_out);
 
-/* The learned job: the first train_sets sets run without the network (every
- * full test) and their outcomes are logged; the network is trained on the GPU
- * (Adam, `epochs` passes of `batch`-row mini-batches over a uniform sample of
- * max_rows logged rows (0: all), learning rate lr, initialised from seed);
- * the other sets run the learned schedule with window margin `margin`.
+/* The learned job: the first train_sets sets run without the network and
+ * log the full-test outcomes of their first label_perms permutations (0: all);
+ * the network is then trained on the GPU (Adam, `epochs` passes of
+ * `batch`-row mini-batches over a uniform sample of max_rows logged rows (0:
+ * all), learning rate lr, initialised from seed) on a second stream while the
+ * next sets run with every full test deferred to the plan; sets launched after
+ * the training finished use the network's windows (margin `margin`).
  * Results, networks and the permutation stream equal eddy_run_sets over all
- * sets. */
+ * sets, whichever sets use the windows. */
 typedef struct {
   int train_sets;
   int epochs;
@@ -262,14 +265,16 @@ typedef struct {
   float margin;
   uint64_t seed;
   long long max_rows;
+  int label_perms;
 } eddy_learn_cfg;
 
 typedef struct {
-  double collect_ms, train_ms, apply_ms;
+  double collect_ms, train_ms, apply_ms; /* train_ms: GPU time of training (second stream) */
   long long train_rows;
   float loss_initial, loss_final;
   long long tests_collect, tests_apply;
-  int trained; /* 0 when too few rows were logged (< 64): the rest ran unlearned */
+  int trained;            /* 0 when too few rows were logged (< 64): no windows were used */
+  int first_windowed_set; /* first set run with the network's windows, -1 if none */
 } eddy_learn_report;
 
 int eddy_run_learned(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *params,

@@ -128,7 +128,56 @@ def test_logged_features_and_labels_match_oracle(engine):
     assert np.array_equal(Y, (hits / (2 * (r - 1))).astype(np.float32))
 
 
-def test_learned_job_is_exact_and_saves_tests(engine):
+def _same_results(run_a, run_b):
+    (a_res, a_js), (b_res, b_js) = run_a, run_b
+    assert len(a_res) == len(b_res)
+    for s in range(len(a_res)):
+        a, b = a_res[s], b_res[s]
+        assert (a.status, a.perms_run, a.U0) == (b.status, b.perms_run, b.U0), s
+        for f in ("js0", "pDDN", "alpha", "beta"):
+            x, y = getattr(a, f), getattr(b, f)
+            assert (np.isnan(x) and np.isnan(y)) or np.float64(x).view(np.uint64) == np.float64(y).view(np.uint64)
+        assert np.array_equal(a_js[s].view(np.uint64), b_js[s].view(np.uint64)), s
+
+
+def test_learned_windows_are_exact_and_save_tests(engine):
+    """The trained network's windows against the method without it: identical
+    results; fewer tests than running every full test in the count kernel;
+    close to the fewest possible (every full test deferred to the plan); and
+    on held-out sets the network predicts the edge rates better than their
+    training mean."""
+    wl = eddy_amd.Workload(3, nsets=48)
+    engine.upload(wl.X, wl.classes)
+    prm = wl.params(perms=40, mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=0)
+    print("case: learned windows, 48 sets of config 3 at r=40", flush=True)
+    engine.reset_stream()
+    Xtr, Ytr = engine.learn_collect(wl.sets[:16], prm)
+    Xte, Yte = engine.learn_collect(wl.sets[16:], prm)
+    engine.mlp_init(1)
+    engine.mlp_train(Xtr, Ytr, epochs=4, batch=2048, lr=3e-3, seed=1)
+    P = engine.mlp_params()
+    yte = np.clip(Yte.astype(np.float64), 0, 1)
+    net_loss = M.loss(P, Xte, yte)
+    m = float(np.clip(Ytr.mean(), 1e-6, 1 - 1e-6))
+    const_loss = float(np.mean(-(yte * np.log(m) + (1 - yte) * np.log(1 - m))))
+    print("held-out loss: network %.4f, training-mean predictor %.4f" % (net_loss, const_loss), flush=True)
+    assert net_loss < 0.8 * const_loss
+    runs = {}
+    for name, learned in (("all", 0), ("deferred", 3), ("learned", 1)):
+        engine.reset_stream()
+        res = engine.run_sets(wl.sets, wl.params(perms=40, mode=eddy_amd.MODE_BATCHED,
+                                                   schedule=eddy_amd.TESTS_CUTOFF, learned=learned))
+        runs[name] = (res, [engine.js(s).copy() for s in range(len(wl.sets))])
+    for name in ("deferred", "learned"):
+        _same_results(runs[name], runs["all"])
+    tests = {k: sum(r.tests_eval for r in v[0]) for k, v in runs.items()}
+    print("tests: every full test %d, deferred %d, learned %d" % (tests["all"], tests["deferred"], tests["learned"]),
+          flush=True)
+    assert tests["learned"] < 0.97 * tests["all"]
+    assert tests["learned"] <= 1.05 * tests["deferred"]
+
+
+def test_learned_job_is_exact(engine):
     wl = eddy_amd.Workload(3, nsets=48)
     engine.upload(wl.X, wl.classes)
     prm = wl.params(perms=40, mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=0)
@@ -138,23 +187,14 @@ def test_learned_job_is_exact_and_saves_tests(engine):
     base_js = [engine.js(s).copy() for s in range(len(wl.sets))]
     base_net = [engine.perm0_networks(s, base[s].U0, prm.mp) for s in (0, 20, 47)]
     engine.reset_stream()
-    cfg = eddy_amd.learn_cfg(train_sets=16, epochs=4, batch=2048, lr=3e-3, margin=1.0, seed=1, max_rows=0)
+    cfg = eddy_amd.learn_cfg(train_sets=16, epochs=4, batch=2048, lr=3e-3, margin=1.0, seed=1, max_rows=0,
+                             label_perms=10)
     res, rep = engine.run_learned(wl.sets, prm, cfg)
     assert rep.trained == 1 and rep.train_rows > 1024
     assert rep.loss_final < rep.loss_initial
-    for s in range(len(wl.sets)):
-        a, b = base[s], res[s]
-        assert (a.status, a.perms_run, a.U0) == (b.status, b.perms_run, b.U0), s
-        for f in ("js0", "pDDN", "alpha", "beta"):
-            x, y = getattr(a, f), getattr(b, f)
-            assert (np.isnan(x) and np.isnan(y)) or np.float64(x).view(np.uint64) == np.float64(y).view(np.uint64)
-        assert np.array_equal(base_js[s].view(np.uint64), engine.js(s).view(np.uint64)), s
+    assert rep.first_windowed_set == -1 or rep.first_windowed_set >= 16
+    _same_results((base, base_js), (res, [engine.js(s).copy() for s in range(len(wl.sets))]))
     for k, s in enumerate((0, 20, 47)):
         got = engine.perm0_networks(s, res[s].U0, prm.mp)
         for x, y in zip(base_net[k], got):
             assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
-    applied_base = sum(base[s].tests_eval for s in range(16, len(wl.sets)))
-    applied = sum(res[s].tests_eval for s in range(16, len(wl.sets)))
-    print("tests on the applied sets: without %d, learned %d (%.1f%% fewer)"
-          % (applied_base, applied, 100.0 * (1 - applied / applied_base)), flush=True)
-    assert applied < applied_base
