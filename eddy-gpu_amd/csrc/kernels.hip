@@ -1010,11 +1010,49 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   // reference's double loop per lane (one lane per parent configuration j),
   // the outer sum over j by lane 0, both in the reference's order
   uint64_t *mst = (uint64_t *)dyn + (size_t)wave * 12 * a.bitw;  // [4 genes][3 states][W]
-  unsigned long long st_hist = 0, st_bins = 0;                    // algorithmic bytes (lane 0)
+  unsigned long long st_hist = 0, st_bins = 0;                    // algorithmic bytes
+  // (a) families with at most one parent (at most 9 cells): one lane each,
+  // the same j-outer / k-inner order
+  for (int fg = tid; fg < 2 * NF; fg += 256) {
+    const int f = fg >> 1, g = fg & 1;
+    const uint64_t key = a.famkey[im.foff + f] - 1;
+    const int t = (int)(key & 1023), np = (int)((key >> 10) & 3);
+    if (np > 1) continue;
+    const int p0 = (int)((key >> 12) & 1023);
+    const uint8_t *sf = a.sflags + im.goff * 2 + g * G;
+    const uint32_t f0 = sf[t], f1 = np > 0 ? sf[p0] : 1u;
+    const int r = __popc(f0), q = __popc(f1);
+    double sum2 = 0.0;
+    if (r > 0 && q > 0) {
+      const int W = a.bitw;
+      const uint64_t *mk = a.bits + (size_t)(im.goff * 2 + g * G) * 3 * W;
+      const double *lq = a.lgtab + (size_t)q * a.lgstride;
+      const double *lqr = a.lgtab + (size_t)(q * r) * a.lgstride;
+      const double lq0 = lq[0], lqr0 = lqr[0];
+      for (int j = 0; j < q; j++) {
+        const uint64_t *mp_ = np > 0 ? mk + (size_t)(p0 * 3 + nth_state(f1, j)) * W : nullptr;
+        int nij = 0;
+        double sum1 = 0.0;
+        for (int k = 0; k < r; k++) {
+          const uint64_t *mc = mk + (size_t)(t * 3 + nth_state(f0, k)) * W;
+          int cnt = 0;
+          for (int w = 0; w < W; w++) cnt += __popcll(np > 0 ? (mc[w] & mp_[w]) : mc[w]);
+          nij += cnt;
+          sum1 = sum1 + (lqr[cnt] - lqr0);
+        }
+        sum2 = sum2 + ((lq0 - lq[nij]) + sum1);
+      }
+      st_bins += (unsigned long long)(4 * q * r);
+    }
+    fscore[f * 2 + g] = sum2;
+    st_hist += (unsigned long long)(1 + np) * (unsigned long long)(g ? a.n2 : a.n1);
+  }
+  // (b) families with two or three parents: a wave each
   for (int fg = wave; fg < 2 * NF; fg += EDDY_NWAVES) {
     const int f = fg >> 1, g = fg & 1;
     const uint64_t key = a.famkey[im.foff + f] - 1;
     const int t = (int)(key & 1023), np = (int)((key >> 10) & 3);
+    if (np <= 1) continue;  // wave-uniform
     const int p0 = (int)((key >> 12) & 1023), p1 = (int)((key >> 22) & 1023), p2 = (int)((key >> 32) & 1023);
     const uint8_t *sf = a.sflags + im.goff * 2 + g * G;
     const uint32_t f0 = sf[t], f1 = np > 0 ? sf[p0] : 1u, f2 = np > 1 ? sf[p1] : 1u, f3 = np > 2 ? sf[p2] : 1u;
@@ -1077,9 +1115,17 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
     }
     wave_sync();
   }
-  if (a.counters && lane == 0) {
-    atomicAdd(&a.counters[4], st_hist);  // histogram state bytes read (reference algorithm)
-    atomicAdd(&a.counters[8], st_bins);  // histogram bin bytes
+  if (a.counters) {
+    __shared__ unsigned long long sh_st[2];
+    if (tid == 0) sh_st[0] = sh_st[1] = 0ULL;
+    __syncthreads();
+    if (st_hist) atomicAdd(&sh_st[0], st_hist);
+    if (st_bins) atomicAdd(&sh_st[1], st_bins);
+    __syncthreads();
+    if (tid == 0) {
+      atomicAdd(&a.counters[4], sh_st[0]);  // histogram state bytes read (reference algorithm)
+      atomicAdd(&a.counters[8], sh_st[1]);  // histogram bin bytes
+    }
   }
   __syncthreads();
   phase_mark(a, 4, tph);
