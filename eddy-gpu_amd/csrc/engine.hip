@@ -427,7 +427,7 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
   GROW(b_tiles, sizeof(uint2) * ch.tiles.size());
   GROW(b_info, sizeof(uint32_t) * ch.pairs);
   const bool plain_tests = p->schedule == EDDY_TESTS_ALL;
-  if (!plain_tests) GROW(b_tab, sizeof(uint16_t) * 18 * ch.pairs);
+  if (!plain_tests) GROW(b_tab, sizeof(uint16_t) * EDDY_TAB_STRIDE * ch.pairs);
   GROW(b_lfull, sizeof(uint16_t) * EDDY_MAX_MP * 2 * ch.gslots);
   GROW(b_lcnt, 2 * ch.gslots);
   GROW(b_sflags, 2 * ch.gslots);

@@ -59,6 +59,10 @@ struct Slice {
   long long bytes;
 };
 
+// per-pair tallies in the cutoff schedule: two classes of 12 u16 (9 counts + padding)
+#define EDDY_TAB_CLASS 12
+#define EDDY_TAB_STRIDE 24
+
 struct KArgs {
   // dataset-level
   int N, n1, n2, S;
@@ -80,7 +84,7 @@ struct KArgs {
   const uint32_t *tiles;   // K1 work list: item<<16 | bi<<8 | bj
   // per-pair outputs
   uint32_t *info;
-  uint16_t *tab;           // 18 counts per pair (cutoff mode)
+  uint16_t *tab;           // [pair][2][EDDY_TAB_CLASS] counts (cutoff mode)
   // per-item scratch
   uint16_t *lfull;         // [goff*2 + g*G + t][mp]
   uint8_t *lcnt;           // [goff*2 + g*G + t]

@@ -45,6 +45,27 @@ __device__ __forceinline__ uint32_t d_loo_mask(const int *t, int full, int prior
 
 __device__ __forceinline__ long long pair_index(int t, int col) { return (long long)t * (t - 1) / 2 + col; }
 
+// A class's 3x3 tally as 12 u16 (9 counts + 3 padding): three 8-byte
+// stores / loads instead of nine 2-byte ones.
+__device__ __forceinline__ void store_tally(uint16_t *dst, const int *t) {
+  uint64_t *d = (uint64_t *)dst;
+  d[0] = (uint64_t)(uint16_t)t[0] | ((uint64_t)(uint16_t)t[1] << 16) | ((uint64_t)(uint16_t)t[2] << 32) |
+         ((uint64_t)(uint16_t)t[3] << 48);
+  d[1] = (uint64_t)(uint16_t)t[4] | ((uint64_t)(uint16_t)t[5] << 16) | ((uint64_t)(uint16_t)t[6] << 32) |
+         ((uint64_t)(uint16_t)t[7] << 48);
+  d[2] = (uint64_t)(uint16_t)t[8];
+}
+__device__ __forceinline__ void load_tally(const uint16_t *src, int *t) {
+  const uint64_t *s = (const uint64_t *)src;
+  const uint64_t a0 = s[0], a1 = s[1], a2 = s[2];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    t[i] = (int)((a0 >> (16 * i)) & 0xffff);
+    t[4 + i] = (int)((a1 >> (16 * i)) & 0xffff);
+  }
+  t[8] = (int)(a2 & 0xffff);
+}
+
 __device__ __forceinline__ int prior_bit(const KArgs &a, const SetMeta &sm, long long p) {
   return (a.prior[sm.poff + (p >> 5)] >> (p & 31)) & 1;
 }
@@ -255,12 +276,8 @@ __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
       info |= d_loo_mask(t2, f2, pr, a, sm, cpairs, &ntests) << INFO_LOO_SHIFT(1);
       info |= INFO_EVAL(0) | INFO_EVAL(1);
     } else {
-      uint16_t *tb = a.tab + (im.infooff + p) * 18;
-#pragma unroll
-      for (int i = 0; i < 9; i++) {
-        tb[i] = (uint16_t)t1[i];
-        tb[9 + i] = (uint16_t)t2[i];
-      }
+      store_tally(a.tab + (im.infooff + p) * EDDY_TAB_STRIDE, t1);
+      store_tally(a.tab + (im.infooff + p) * EDDY_TAB_STRIDE + EDDY_TAB_CLASS, t2);
     }
     a.info[im.infooff + p] = info;
   }
@@ -291,10 +308,8 @@ __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
 
 __device__ __forceinline__ uint32_t loo_of_pair(const KArgs &a, const ItemMeta &im, const SetMeta &sm, long long p,
                                                 int g, int full, double cpairs, int *ntests) {
-  const uint16_t *tb = a.tab + (im.infooff + p) * 18 + 9 * g;
   int tt[9];
-#pragma unroll
-  for (int i = 0; i < 9; i++) tt[i] = tb[i];
+  load_tally(a.tab + (im.infooff + p) * EDDY_TAB_STRIDE + EDDY_TAB_CLASS * g, tt);
   return d_loo_mask(tt, full, prior_bit(a, sm, p), a, sm, cpairs, ntests);
 }
 
@@ -310,10 +325,8 @@ __device__ __forceinline__ uint32_t info_ld(const uint32_t *p) {
 __device__ __forceinline__ int full_of(const KArgs &a, const ItemMeta &im, const SetMeta &sm, long long p, int g,
                                        uint32_t w, double cpairs, int *ntests) {
   if (w & INFO_FEVAL(g)) return (w >> g) & 1;
-  const uint16_t *tb = a.tab + (im.infooff + p) * 18 + 9 * g;
   int tt[9];
-#pragma unroll
-  for (int i = 0; i < 9; i++) tt[i] = tb[i];
+  load_tally(a.tab + (im.infooff + p) * EDDY_TAB_STRIDE + EDDY_TAB_CLASS * g, tt);
   int f = d_edge_test(tt, prior_bit(a, sm, p), a, sm, cpairs);
   (*ntests)++;
   atomicOr(&a.info[im.infooff + p], ((uint32_t)f << g) | INFO_FEVAL(g));
@@ -377,12 +390,17 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
   if (tid == 0) off[R] = total;
   __syncthreads();
   // ---- B: packed leave-one-out tests of every planned (pair, class) ------
-  for (int e = tid; e < total; e += 256) {
-    int lo = 0, hi = R - 1;  // last r with off[r] <= e
+  // 64 consecutive entries per wave step: one (wave-uniform) binary search
+  // for the step's first entry, then each lane walks to its own row
+  for (int e0 = wave * 64; e0 < total; e0 += 256) {
+    int lo = 0, hi = R - 1;  // last r with off[r] <= e0
     while (lo < hi) {
       int mid = (lo + hi + 1) >> 1;
-      if (off[mid] <= e) lo = mid; else hi = mid - 1;
+      if (off[mid] <= e0) lo = mid; else hi = mid - 1;
     }
+    const int e = e0 + lane;
+    if (e >= total) continue;
+    while (off[lo + 1] <= e) lo++;  // off[R] = total > e ends the walk
     const int t = lo >> 1, g = lo & 1, col = e - off[lo];
     const long long p = pair_index(t, col);
     const int full = full_of(a, im, sm, p, g, info_ld(&a.info[im.infooff + p]), cpairs, &ntests);
