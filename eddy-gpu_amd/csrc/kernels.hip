@@ -153,6 +153,12 @@ __device__ void pair_features(const int *t1, const int *t2, int prior, int c, fl
 #define K1_TB 32
 #define K1_KCH 512
 #define K1_ROWS (2 * K1_TB)
+// LDS strides padded off multiples of the bank width: staged gene rows
+// K1_KCH + 16 bytes apart (16 rows read by one MFMA operand load land on
+// distinct banks), tally rows K1_TS u16 apart (the 4 row groups of an
+// accumulator store land on distinct banks)
+#define K1_RS (K1_KCH + 16)
+#define K1_TS 112
 
 __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
   const uint2 w = ((const uint2 *)a.tiles)[blockIdx.x];
@@ -164,7 +170,7 @@ __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
   const int NP = (N + 63) & ~63;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
-  __shared__ __align__(16) uint8_t rows[K1_ROWS * K1_KCH > 2 * 96 * 96 * 2 ? K1_ROWS * K1_KCH : 2 * 96 * 96 * 2];
+  __shared__ __align__(16) uint8_t rows[K1_ROWS * K1_RS > 2 * 96 * K1_TS * 2 ? K1_ROWS * K1_RS : 2 * 96 * K1_TS * 2];
   __shared__ __align__(16) uint8_t clsb[EDDY_MAX_N + 66];
   __shared__ int tcount;
 
@@ -196,7 +202,7 @@ __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
       int gene = (r < K1_TB) ? bi * K1_TB + r : bj * K1_TB + (r - K1_TB);
       uint4 val = make_uint4(0x03030303u, 0x03030303u, 0x03030303u, 0x03030303u);
       if (gene < G) val = *(const uint4 *)(xs + (size_t)gene * NP + k0 + c * 16);
-      *(uint4 *)(rows + r * K1_KCH + c * 16) = val;
+      *(uint4 *)(rows + r * K1_RS + c * 16) = val;
     }
     __syncthreads();
     for (int kk = 0; kk < kch; kk += 64) {
@@ -207,14 +213,14 @@ __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
       for (int i = 0; i < 3; i++) {
         int R = (rt0 + i) * 16 + (lane & 15);
         int gl = R / 3, st = R - gl * 3;
-        uint4 x = *(const uint4 *)(rows + gl * K1_KCH + kofs);
+        uint4 x = *(const uint4 *)(rows + gl * K1_RS + kofs);
         uint32_t o0 = eq_bytes(x.x, st), o1 = eq_bytes(x.y, st), o2 = eq_bytes(x.z, st), o3 = eq_bytes(x.w, st);
         A1[i] = (v4i){(int)(o0 & m.x), (int)(o1 & m.y), (int)(o2 & m.z), (int)(o3 & m.w)};
         A2[i] = (v4i){(int)(o0 & (m.x ^ 0x01010101u)), (int)(o1 & (m.y ^ 0x01010101u)),
                       (int)(o2 & (m.z ^ 0x01010101u)), (int)(o3 & (m.w ^ 0x01010101u))};
         int C = (ct0 + i) * 16 + (lane & 15);
         int gc = C / 3, sc = C - gc * 3;
-        uint4 y = *(const uint4 *)(rows + (K1_TB + gc) * K1_KCH + kofs);
+        uint4 y = *(const uint4 *)(rows + (K1_TB + gc) * K1_RS + kofs);
         B[i] = (v4i){(int)eq_bytes(y.x, sc), (int)eq_bytes(y.y, sc), (int)eq_bytes(y.z, sc), (int)eq_bytes(y.w, sc)};
       }
 #pragma unroll
@@ -237,8 +243,8 @@ __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
       for (int e = 0; e < 4; e++) {
         int R = (rt0 + i) * 16 + 4 * (lane >> 4) + e;
         int C = (ct0 + j) * 16 + (lane & 15);
-        T[R * 96 + C] = (uint16_t)acc1[i][j][e];
-        T[96 * 96 + R * 96 + C] = (uint16_t)acc2[i][j][e];
+        T[R * K1_TS + C] = (uint16_t)acc1[i][j][e];
+        T[96 * K1_TS + R * K1_TS + C] = (uint16_t)acc2[i][j][e];
       }
   __syncthreads();
 
@@ -253,8 +259,8 @@ __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
     for (int r = 0; r < 3; r++)
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        t1[r * 3 + k] = T[(li * 3 + r) * 96 + lj * 3 + k];
-        t2[r * 3 + k] = T[96 * 96 + (li * 3 + r) * 96 + lj * 3 + k];
+        t1[r * 3 + k] = T[(li * 3 + r) * K1_TS + lj * 3 + k];
+        t2[r * 3 + k] = T[96 * K1_TS + (li * 3 + r) * K1_TS + lj * 3 + k];
       }
     long long p = pair_index(gi, gj);
     int pr = prior_bit(a, sm, p);
