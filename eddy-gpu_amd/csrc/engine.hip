@@ -1458,7 +1458,7 @@ extern "C" int eddy_get_edgelist(eddy_ctx *ctx, int set, int *nodes, int *epn, i
     for (int t = 0; t < G; t++) {
       if (nodes) nodes[g * G + t] = acc;
       for (int col = 0; col < t; col++, pidx++)
-        if ((q.info[pidx] >> g) & 1) {
+        if (INFO_FULL_BIT(q.info[pidx], g)) {
           if (edges) {
             if (ne >= edges_cap) return EDDY_ERR_CAPACITY;
             edges[ne] = col;
@@ -1550,7 +1550,7 @@ extern "C" int eddy_run_perm_detail(eddy_ctx *ctx, const eddy_set_desc *set, con
           int g = s <= n1 ? 0 : 1;
           int k = g == 0 ? s - 1 : s - n1 - 1;
           int nk = g == 0 ? n1 : N - n1;
-          int bit = (w >> g) & 1;
+          int bit = INFO_FULL_BIT(w, g);
           if (k >= 0 && k < nk) {
             int sid = g == 0 ? c1[k] : c2[k];
             int a = xrow[(size_t)r * N + sid], b = xrow[(size_t)col * N + sid];

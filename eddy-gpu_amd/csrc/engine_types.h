@@ -12,15 +12,24 @@
 #define EDDY_NWAVES 4       // waves per workgroup in the per-item kernels
 
 // Per-pair info word written by the test kernels and read by the network
-// kernel.  Class g in {0,1} = class 1 / class 2 of the reference.
-//   bit g            full-data test of class g is an edge
-//   bits 2+9g..10+9g leave-one-out test per cell (r*3+k) of class g; a cell
-//                    with no sample holds the full bit (it never occurs)
-//   bit 20+g         the leave-one-out tests of class g were evaluated
-#define INFO_FULL(g) (1u << (g))
-#define INFO_LOO_SHIFT(g) (2 + 9 * (g))
-#define INFO_EVAL(g) (1u << (20 + (g)))
-#define INFO_FEVAL(g) (1u << (22 + (g)))  // the full test of class g was evaluated
+// kernel: class g in {0,1} (class 1 / class 2 of the reference) owns the 16
+// bits at 16g, so each class's half is written by one thread with a plain
+// 16-bit store.
+//   bit 16g          full-data test of class g is an edge
+//   bit 16g+1        the full test of class g was evaluated
+//   bit 16g+2        the leave-one-out tests of class g were evaluated
+//   bits 16g+3..+11  leave-one-out test per cell (r*3+k) of class g; a cell
+//                    with no sample holds the full bit
+#define INFO_FULL(g) (1u << (16 * (g)))
+#define INFO_FEVAL(g) (1u << (16 * (g) + 1))
+#define INFO_EVAL(g) (1u << (16 * (g) + 2))
+#define INFO_LOO_SHIFT(g) (16 * (g) + 3)
+#define INFO_FULL_BIT(w, g) (((w) >> (16 * (g))) & 1u)
+// bits within one class's 16-bit half
+#define IH_FULL 1u
+#define IH_FEVAL 2u
+#define IH_EVAL 4u
+#define IH_LOO_SHIFT 3
 
 struct SetMeta {
   int G;            // genes in the set

@@ -273,7 +273,7 @@ __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
       f1 = d_edge_test(t1, pr, a, sm, cpairs);
       f2 = d_edge_test(t2, pr, a, sm, cpairs);
       ntests += 2;
-      info = (uint32_t)f1 | ((uint32_t)f2 << 1) | INFO_FEVAL(0) | INFO_FEVAL(1);
+      info = (f1 ? INFO_FULL(0) : 0u) | (f2 ? INFO_FULL(1) : 0u) | INFO_FEVAL(0) | INFO_FEVAL(1);
       if (a.labhits && im.perm > 0 && (a.label_perms == 0 || logged))
         atomicAdd(&a.labhits[sm.pairbase + p], (unsigned)(f1 + f2));
     }
@@ -319,24 +319,32 @@ __device__ __forceinline__ uint32_t loo_of_pair(const KArgs &a, const ItemMeta &
   return d_loo_mask(tt, full, prior_bit(a, sm, p), a, sm, cpairs, ntests);
 }
 
-// Info words that this kernel's own atomics update are read at device scope
-// (past the CU's vector cache), so a read after another wave's atomicOr sees
-// its bits.
-__device__ __forceinline__ uint32_t info_ld(const uint32_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Class g's 16-bit half of a pair's info word: only the thread handling
+// (pair, class) writes it, with plain stores; a __syncthreads separates the
+// phases that read what another thread wrote.
+__device__ __forceinline__ uint16_t *info_half(const KArgs &a, const ItemMeta &im, long long p, int g) {
+  return (uint16_t *)(a.info + im.infooff + p) + g;
 }
 
 // full-test decision of (pair, class), evaluated here when the count kernel
-// skipped it (outside the learned window)
+// skipped it (outside the learned window or deferred)
 __device__ __forceinline__ int full_of(const KArgs &a, const ItemMeta &im, const SetMeta &sm, long long p, int g,
-                                       uint32_t w, double cpairs, int *ntests) {
-  if (w & INFO_FEVAL(g)) return (w >> g) & 1;
+                                       double cpairs, int *ntests) {
+  uint16_t *ih = info_half(a, im, p, g);
+  const uint16_t h = *ih;
+  if (h & IH_FEVAL) return h & IH_FULL;
   int tt[9];
   load_tally(a.tab + (im.infooff + p) * EDDY_TAB_STRIDE + EDDY_TAB_CLASS * g, tt);
-  int f = d_edge_test(tt, prior_bit(a, sm, p), a, sm, cpairs);
+  const int f = d_edge_test(tt, prior_bit(a, sm, p), a, sm, cpairs);
   (*ntests)++;
-  atomicOr(&a.info[im.infooff + p], ((uint32_t)f << g) | INFO_FEVAL(g));
+  *ih = (uint16_t)(h | IH_FEVAL | (f ? IH_FULL : 0u));
   return f;
+}
+
+// records the leave-one-out mask of (pair, class)
+__device__ __forceinline__ void set_loo(const KArgs &a, const ItemMeta &im, long long p, int g, uint32_t m) {
+  uint16_t *ih = info_half(a, im, p, g);
+  *ih = (uint16_t)(*ih | IH_EVAL | (m << IH_LOO_SHIFT));
 }
 
 __device__ int block_exscan(int *v, int n, int *wsum);
@@ -372,7 +380,7 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
       int f = 0;
       if (col < t) {
         const long long pc = pair_index(t, col);
-        f = full_of(a, im, sm, pc, g, info_ld(&a.info[im.infooff + pc]), cpairs, &ntests);
+        f = full_of(a, im, sm, pc, g, cpairs, &ntests);
       }
       const uint64_t b = __ballot(f);
       const int nb = __popcll(b);
@@ -409,9 +417,9 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
     while (off[lo + 1] <= e) lo++;  // off[R] = total > e ends the walk
     const int t = lo >> 1, g = lo & 1, col = e - off[lo];
     const long long p = pair_index(t, col);
-    const int full = full_of(a, im, sm, p, g, info_ld(&a.info[im.infooff + p]), cpairs, &ntests);
+    const int full = full_of(a, im, sm, p, g, cpairs, &ntests);
     const uint32_t m = loo_of_pair(a, im, sm, p, g, full, cpairs, &ntests);
-    atomicOr(&a.info[im.infooff + p], (m << INFO_LOO_SHIFT(g)) | INFO_EVAL(g));
+    set_loo(a, im, p, g, m);
     if (full && m == 0x1FFu) atomicAdd(&solid[lo], 1);
   }
   __syncthreads();
@@ -435,7 +443,7 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
         int f = 0;
         if (col < t) {
           const long long pc = pair_index(t, col);
-          f = full_of(a, im, sm, pc, g, info_ld(&a.info[im.infooff + pc]), cpairs, &ntests);
+          f = full_of(a, im, sm, pc, g, cpairs, &ntests);
         }
         const uint64_t b = __ballot(f);
         const int nb = __popcll(b);
@@ -452,9 +460,9 @@ __global__ __launch_bounds__(256) void k_loo(KArgs a) {
         int sol = 0;
         if (col <= P2) {
           const long long p = pair_index(t, col);
-          const int full = full_of(a, im, sm, p, g, info_ld(&a.info[im.infooff + p]), cpairs, &ntests);
+          const int full = full_of(a, im, sm, p, g, cpairs, &ntests);
           const uint32_t m = loo_of_pair(a, im, sm, p, g, full, cpairs, &ntests);
-          atomicOr(&a.info[im.infooff + p], (m << INFO_LOO_SHIFT(g)) | INFO_EVAL(g));
+          set_loo(a, im, p, g, m);
           sol = full && m == 0x1FFu;
           fbc++;
         }
@@ -618,7 +626,7 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
       int col = base + lane;
       bool valid = col < t;
       uint32_t w = valid ? a.info[im.infooff + pair_index(t, col)] : 0u;
-      int full = (w >> g) & 1;
+      int full = INFO_FULL_BIT(w, g);
       uint32_t loo = (w >> INFO_LOO_SHIFT(g)) & 0x1FFu;
       bool ev = (w & INFO_EVAL(g)) != 0 && (w & INFO_FEVAL(g)) != 0;  // both tests of this cell ran
       bool solid = valid && full && ev && loo == 0x1FFu;
@@ -679,7 +687,7 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
       int cnt = 0, fi = 0;
       for (int col = 0; col <= C && cnt < mp; col++) {
         uint32_t w = rinfo_w[col];
-        int bit = (w >> g) & 1;
+        int bit = INFO_FULL_BIT(w, g);
         if (fi < nflip && (int)(flips_w[fi] & 0xffff) == col) {
           uint32_t e = flips_w[fi];
           int vc = xs[(size_t)col * NP + sid];
