@@ -90,7 +90,7 @@ struct eddy_ctx {
   // device buffers, grown on demand
   DevBuf b_sets, b_xs, b_prior, b_items, b_labels, b_tiles, b_info, b_tab, b_lfull, b_lcnt, b_sflags, b_ev,
       b_evcount, b_hnet, b_rep, b_clsg, b_evorder, b_evfam, b_famkey, b_famscore, b_lval, b_uniq, b_js,
-      b_status, b_tests, b_genes, b_rowoff, b_nan0, b_pout, b_famcount, b_counters, b_famht, b_famhv, b_bits;
+      b_status, b_tests, b_genes, b_rowoff, b_nan0, b_pout, b_famcount, b_counters, b_famht, b_famhv, b_bits, b_order;
   // last run
   int last_perms = 0;
   std::vector<int> last_G;
@@ -195,7 +195,7 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
                    &ctx->b_mx, &ctx->b_my, &ctx->b_mxb, &ctx->b_myb, &ctx->b_midx, &ctx->b_mscr, &ctx->b_mlbase,
                    &ctx->b_mlinv, &ctx->p0.info, &ctx->p0.lcnt, &ctx->p0.lfull, &ctx->p0.rep, &ctx->p0.lval,
                    &ctx->p0.ev, &ctx->p0.famkey, &ctx->p0.famscore, &ctx->p0.slices, &ctx->b_bits,
-                   &ctx->b_win2};
+                   &ctx->b_win2, &ctx->b_order};
   for (DevBuf *b : all)
     if (b->p) hipFree(b->p);
   if (ctx->dX) hipFree(ctx->dX);
@@ -449,12 +449,20 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
   GROW(b_famcount, sizeof(int) * nitems);
   GROW(b_status, sizeof(int) * nitems);
   GROW(b_tests, sizeof(long long) * nitems);
+  GROW(b_order, sizeof(int) * nitems);
+  // workgroups take items largest set first: the longest items start early
+  // instead of finishing last
+  std::vector<int> order(nitems);
+  for (int i = 0; i < nitems; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return sets[ch.items[x].set].G > sets[ch.items[y].set].G; });
   hipStream_t st = ctx->stream;
   HIPCHK(ctx, hipMemcpyAsync(ctx->b_items.p, ch.items.data(), sizeof(ItemMeta) * nitems, hipMemcpyHostToDevice, st));
   HIPCHK(ctx, hipMemcpyAsync(ctx->b_labels.p, ch.labels.data(), sizeof(uint32_t) * ch.labels.size(),
                              hipMemcpyHostToDevice, st));
   HIPCHK(ctx, hipMemcpyAsync(ctx->b_tiles.p, ch.tiles.data(), sizeof(uint2) * ch.tiles.size(), hipMemcpyHostToDevice,
                              st));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->b_order.p, order.data(), sizeof(int) * nitems, hipMemcpyHostToDevice, st));
   HIPCHK(ctx, hipMemsetAsync(ctx->b_evcount.p, 0, sizeof(int) * nitems, st));
   HIPCHK(ctx, hipMemsetAsync(ctx->b_status.p, 0, sizeof(int) * nitems, st));
   HIPCHK(ctx, hipMemsetAsync(ctx->b_tests.p, 0, sizeof(long long) * nitems, st));
@@ -487,6 +495,7 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
   a.xs = (const uint8_t *)ctx->b_xs.p;
   a.prior = (const uint32_t *)ctx->b_prior.p;
   a.items = (const ItemMeta *)ctx->b_items.p;
+  a.order = (const int *)ctx->b_order.p;
   a.labels = (const uint32_t *)ctx->b_labels.p;
   a.tiles = (const uint32_t *)ctx->b_tiles.p;
   a.info = (uint32_t *)ctx->b_info.p;

@@ -88,6 +88,7 @@ struct KArgs {
   const uint32_t *prior;   // prior bits
   // items
   const ItemMeta *items;
+  const int *order;        // per-item kernels: workgroup b runs item order[b]
   const uint32_t *labels;
   // work lists
   const uint32_t *tiles;   // K1 work list: item<<16 | bi<<8 | bj

@@ -349,8 +349,8 @@ __device__ __forceinline__ void set_loo(const KArgs &a, const ItemMeta &im, long
 
 __device__ int block_exscan(int *v, int n, int *wsum);
 
-__global__ __launch_bounds__(256) void k_loo(KArgs a) {
-  const int item = blockIdx.x;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_loo(KArgs a) {
+  const int item = a.order[blockIdx.x];  // largest items first (shorter tail)
   const ItemMeta im = a.items[item];
   const SetMeta &sm = a.sets[im.set];  // read in place: the bounds are indexed per tally
   const int G = sm.G, mp = a.mp;
@@ -552,7 +552,7 @@ __device__ void build_class_lists(const uint32_t *lab, int N, uint16_t *cls0, ui
 // One workgroup (4 waves) per item; a wave owns one row at a time.
 // --------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_rows(KArgs a) {
-  const int item = blockIdx.x;
+  const int item = a.order[blockIdx.x];  // largest items first (shorter tail)
   const ItemMeta im = a.items[item];
   const SetMeta sm = a.sets[im.set];
   const int G = sm.G, N = a.N, S = a.S, mp = a.mp;
@@ -815,7 +815,7 @@ __device__ int block_exscan(int *v, int n, int *wsum) {
 }
 
 __global__ __launch_bounds__(256) void k_score(KArgs a) {
-  const int item = blockIdx.x;
+  const int item = a.order[blockIdx.x];  // largest items first (shorter tail)
   const ItemMeta im = a.items[item];
   const SetMeta sm = a.sets[im.set];
   const int G = sm.G, N = a.N, S = a.S, n1 = a.n1;
