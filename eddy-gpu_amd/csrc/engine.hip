@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <deque>
 #include <map>
 #include <random>
 #include <thread>
@@ -53,6 +54,33 @@ struct Perm0 {
   std::vector<double> famscore; // [f*2 + g] BDeu of family f on class g
 };
 
+// one chunk of items through K1..K4 (host side)
+struct Chunk {
+  std::vector<ItemMeta> items;
+  std::vector<uint32_t> labels;
+  std::vector<uint2> tiles;
+  long long pairs = 0, gslots = 0, sslots = 0;
+};
+
+// Device buffers, stream and pinned readback of one chunk in flight.  Two
+// slots let one chunk's kernels run while the other's tail drains and the
+// host prepares the next.
+struct Slot {
+  DevBuf items, labels, tiles, info, tab, lfull, lcnt, sflags, bits, ev, evcount, hnet, rep, clsg, evorder, evfam,
+      famkey, famscore, famht, famhv, lval, uniq, famcount, status, tests, order, slices;
+  hipStream_t st = nullptr;
+  hipEvent_t tev[5] = {};
+  int *h_int = nullptr;  // pinned: status, events, families, unique networks (4 x h_cap)
+  long long *h_tests = nullptr;
+  int h_cap = 0;
+  bool busy = false;
+  bool plain_tests = false, has_tiles = false;
+  Chunk ch;             // the chunk in flight
+  std::vector<int> cap;  // its event capacities
+  std::vector<int> horder;
+  int first = 0, last = -1;  // its sets (batched schedule)
+};
+
 struct P0Store {
   DevBuf info, lcnt, lfull, rep, lval, ev, famkey, famscore, slices;
   std::vector<SetMeta> meta;
@@ -97,6 +125,7 @@ struct eddy_ctx {
   std::vector<std::vector<double>> last_js;
   std::vector<Perm0> last_p0;  // host copies, fetched from p0 on request
   P0Store p0;
+  Slot slot[2];
   BoundsKey bounds_key;
   std::map<int, eddy_edge_bounds> bounds_cache;
   eddy_timing timing{};
@@ -169,6 +198,8 @@ extern "C" int eddy_ctx_create(int device, eddy_ctx **out) {
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->slot[0].st, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&ctx->slot[1].st, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->train_ev, hipEventDisableTiming) != hipSuccess ||
       hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
     delete ctx;
@@ -198,6 +229,19 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
                    &ctx->b_win2, &ctx->b_order};
   for (DevBuf *b : all)
     if (b->p) hipFree(b->p);
+  for (Slot &sl : ctx->slot) {
+    DevBuf *sb[] = {&sl.items, &sl.labels, &sl.tiles, &sl.info, &sl.tab, &sl.lfull, &sl.lcnt, &sl.sflags, &sl.bits,
+                    &sl.ev, &sl.evcount, &sl.hnet, &sl.rep, &sl.clsg, &sl.evorder, &sl.evfam, &sl.famkey,
+                    &sl.famscore, &sl.famht, &sl.famhv, &sl.lval, &sl.uniq, &sl.famcount, &sl.status, &sl.tests,
+                    &sl.order, &sl.slices};
+    for (DevBuf *b : sb)
+      if (b->p) hipFree(b->p);
+    if (sl.h_int) hipHostFree(sl.h_int);
+    if (sl.h_tests) hipHostFree(sl.h_tests);
+    for (hipEvent_t e : sl.tev)
+      if (e) hipEventDestroy(e);
+    if (sl.st) hipStreamDestroy(sl.st);
+  }
   if (ctx->dX) hipFree(ctx->dX);
   if (ctx->dLG) hipFree(ctx->dLG);
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
@@ -369,14 +413,6 @@ int check_params(eddy_ctx *ctx, const eddy_params *p) {
   return EDDY_OK;
 }
 
-// one chunk of items through K1..K4
-struct Chunk {
-  std::vector<ItemMeta> items;
-  std::vector<uint32_t> labels;
-  std::vector<uint2> tiles;
-  long long pairs = 0, gslots = 0, sslots = 0;
-};
-
 void add_item(Chunk &ch, const std::vector<SetMeta> &sets, int set, int perm, const std::vector<uint32_t> &labbits,
               int N);
 
@@ -398,9 +434,13 @@ struct ChunkOut {  // per-item results read back after a launch
   std::vector<long long> tests;
 };
 
-int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, const eddy_params *p, int perms,
-                 const std::vector<int> &evcap, int attempt, ChunkOut &out) {
+// Launches the slot's chunk (sl.ch) with per-item event capacities evcap:
+// uploads, K1..K4, and the readback of per-item results into the slot's
+// pinned buffers, all on the slot's stream; returns without waiting.
+int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, const eddy_params *p, int perms,
+                 const std::vector<int> &evcap, int attempt) {
   auto hl0 = std::chrono::steady_clock::now();
+  Chunk &ch = sl.ch;
   const int N = ctx->N, S = N + 2;
   const int nitems = (int)ch.items.size();
   int maxG = 0;
@@ -422,50 +462,61 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
     foff += im.fcap;
     fhoff += im.fhsize;
   }
-  GROW(b_items, sizeof(ItemMeta) * nitems);
-  GROW(b_labels, sizeof(uint32_t) * ch.labels.size());
-  GROW(b_tiles, sizeof(uint2) * ch.tiles.size());
-  GROW(b_info, sizeof(uint32_t) * ch.pairs);
   const bool plain_tests = p->schedule == EDDY_TESTS_ALL;
-  if (!plain_tests) GROW(b_tab, sizeof(uint16_t) * EDDY_TAB_STRIDE * ch.pairs);
-  GROW(b_lfull, sizeof(uint16_t) * EDDY_MAX_MP * 2 * ch.gslots);
-  GROW(b_lcnt, 2 * ch.gslots);
-  GROW(b_sflags, 2 * ch.gslots);
   const int bitw = (std::max(ctx->n1, ctx->n2) + 63) / 64;
-  GROW(b_bits, sizeof(uint64_t) * 2 * 3 * (size_t)bitw * ch.gslots);
-  GROW(b_ev, sizeof(Event) * evoff);
-  GROW(b_evcount, sizeof(int) * nitems);
-  GROW(b_hnet, sizeof(uint64_t) * ch.sslots);
-  GROW(b_rep, sizeof(int) * ch.sslots);
-  GROW(b_clsg, sizeof(uint16_t) * ch.sslots);
-  GROW(b_evorder, sizeof(int) * evoff);
-  GROW(b_evfam, sizeof(int) * evoff);
-  GROW(b_famkey, sizeof(uint64_t) * foff);
-  GROW(b_famscore, sizeof(double) * 2 * foff);
-  GROW(b_famht, sizeof(unsigned long long) * fhoff);
-  GROW(b_famhv, sizeof(int) * fhoff);
-  GROW(b_lval, sizeof(double) * 2 * ch.sslots);
-  GROW(b_uniq, sizeof(int) * nitems);
-  GROW(b_famcount, sizeof(int) * nitems);
-  GROW(b_status, sizeof(int) * nitems);
-  GROW(b_tests, sizeof(long long) * nitems);
-  GROW(b_order, sizeof(int) * nitems);
+  CHK(grow(ctx, sl.items, sizeof(ItemMeta) * nitems));
+  CHK(grow(ctx, sl.labels, sizeof(uint32_t) * ch.labels.size()));
+  CHK(grow(ctx, sl.tiles, sizeof(uint2) * std::max<size_t>(1, ch.tiles.size())));
+  CHK(grow(ctx, sl.info, sizeof(uint32_t) * ch.pairs));
+  if (!plain_tests) CHK(grow(ctx, sl.tab, sizeof(uint16_t) * EDDY_TAB_STRIDE * ch.pairs));
+  CHK(grow(ctx, sl.lfull, sizeof(uint16_t) * EDDY_MAX_MP * 2 * ch.gslots));
+  CHK(grow(ctx, sl.lcnt, 2 * ch.gslots));
+  CHK(grow(ctx, sl.sflags, 2 * ch.gslots));
+  CHK(grow(ctx, sl.bits, sizeof(uint64_t) * 2 * 3 * (size_t)bitw * ch.gslots));
+  CHK(grow(ctx, sl.ev, sizeof(Event) * evoff));
+  CHK(grow(ctx, sl.evcount, sizeof(int) * nitems));
+  CHK(grow(ctx, sl.hnet, sizeof(uint64_t) * ch.sslots));
+  CHK(grow(ctx, sl.rep, sizeof(int) * ch.sslots));
+  CHK(grow(ctx, sl.clsg, sizeof(uint16_t) * ch.sslots));
+  CHK(grow(ctx, sl.evorder, sizeof(int) * evoff));
+  CHK(grow(ctx, sl.evfam, sizeof(int) * evoff));
+  CHK(grow(ctx, sl.famkey, sizeof(uint64_t) * foff));
+  CHK(grow(ctx, sl.famscore, sizeof(double) * 2 * foff));
+  CHK(grow(ctx, sl.famht, sizeof(unsigned long long) * fhoff));
+  CHK(grow(ctx, sl.famhv, sizeof(int) * fhoff));
+  CHK(grow(ctx, sl.lval, sizeof(double) * 2 * ch.sslots));
+  CHK(grow(ctx, sl.uniq, sizeof(int) * nitems));
+  CHK(grow(ctx, sl.famcount, sizeof(int) * nitems));
+  CHK(grow(ctx, sl.status, sizeof(int) * nitems));
+  CHK(grow(ctx, sl.tests, sizeof(long long) * nitems));
+  CHK(grow(ctx, sl.order, sizeof(int) * nitems));
+  if (sl.h_cap < nitems) {
+    if (sl.h_int) hipHostFree(sl.h_int);
+    if (sl.h_tests) hipHostFree(sl.h_tests);
+    sl.h_int = nullptr;
+    sl.h_tests = nullptr;
+    sl.h_cap = 0;
+    const int want = nitems + nitems / 4;
+    HIPCHK(ctx, hipHostMalloc((void **)&sl.h_int, sizeof(int) * 4 * (size_t)want, hipHostMallocDefault));
+    HIPCHK(ctx, hipHostMalloc((void **)&sl.h_tests, sizeof(long long) * (size_t)want, hipHostMallocDefault));
+    sl.h_cap = want;
+  }
   // workgroups take items largest set first: the longest items start early
   // instead of finishing last
-  std::vector<int> order(nitems);
-  for (int i = 0; i < nitems; i++) order[i] = i;
-  std::stable_sort(order.begin(), order.end(),
+  sl.horder.resize(nitems);
+  for (int i = 0; i < nitems; i++) sl.horder[i] = i;
+  std::stable_sort(sl.horder.begin(), sl.horder.end(),
                    [&](int x, int y) { return sets[ch.items[x].set].G > sets[ch.items[y].set].G; });
-  hipStream_t st = ctx->stream;
-  HIPCHK(ctx, hipMemcpyAsync(ctx->b_items.p, ch.items.data(), sizeof(ItemMeta) * nitems, hipMemcpyHostToDevice, st));
-  HIPCHK(ctx, hipMemcpyAsync(ctx->b_labels.p, ch.labels.data(), sizeof(uint32_t) * ch.labels.size(),
-                             hipMemcpyHostToDevice, st));
-  HIPCHK(ctx, hipMemcpyAsync(ctx->b_tiles.p, ch.tiles.data(), sizeof(uint2) * ch.tiles.size(), hipMemcpyHostToDevice,
+  hipStream_t st = sl.st;
+  HIPCHK(ctx, hipMemcpyAsync(sl.items.p, ch.items.data(), sizeof(ItemMeta) * nitems, hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, hipMemcpyAsync(sl.labels.p, ch.labels.data(), sizeof(uint32_t) * ch.labels.size(), hipMemcpyHostToDevice,
                              st));
-  HIPCHK(ctx, hipMemcpyAsync(ctx->b_order.p, order.data(), sizeof(int) * nitems, hipMemcpyHostToDevice, st));
-  HIPCHK(ctx, hipMemsetAsync(ctx->b_evcount.p, 0, sizeof(int) * nitems, st));
-  HIPCHK(ctx, hipMemsetAsync(ctx->b_status.p, 0, sizeof(int) * nitems, st));
-  HIPCHK(ctx, hipMemsetAsync(ctx->b_tests.p, 0, sizeof(long long) * nitems, st));
+  if (!ch.tiles.empty())
+    HIPCHK(ctx, hipMemcpyAsync(sl.tiles.p, ch.tiles.data(), sizeof(uint2) * ch.tiles.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, hipMemcpyAsync(sl.order.p, sl.horder.data(), sizeof(int) * nitems, hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, hipMemsetAsync(sl.evcount.p, 0, sizeof(int) * nitems, st));
+  HIPCHK(ctx, hipMemsetAsync(sl.status.p, 0, sizeof(int) * nitems, st));
+  HIPCHK(ctx, hipMemsetAsync(sl.tests.p, 0, sizeof(long long) * nitems, st));
 
   KArgs a{};
   a.N = N;
@@ -483,7 +534,7 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
   a.win = ctx->k_win;
   a.label_perms = ctx->k_label_perms;
   a.counters = (unsigned long long *)ctx->b_counters.p;
-  a.bits = (uint64_t *)ctx->b_bits.p;
+  a.bits = (uint64_t *)sl.bits.p;
   a.bitw = bitw;
   a.rows_maxg = maxG;
   edge_params(ctx, p, a.ep);
@@ -494,73 +545,83 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
   a.sets = (const SetMeta *)ctx->b_sets.p;
   a.xs = (const uint8_t *)ctx->b_xs.p;
   a.prior = (const uint32_t *)ctx->b_prior.p;
-  a.items = (const ItemMeta *)ctx->b_items.p;
-  a.order = (const int *)ctx->b_order.p;
-  a.labels = (const uint32_t *)ctx->b_labels.p;
-  a.tiles = (const uint32_t *)ctx->b_tiles.p;
-  a.info = (uint32_t *)ctx->b_info.p;
-  a.tab = (uint16_t *)ctx->b_tab.p;
-  a.lfull = (uint16_t *)ctx->b_lfull.p;
-  a.lcnt = (uint8_t *)ctx->b_lcnt.p;
-  a.sflags = (uint8_t *)ctx->b_sflags.p;
-  a.ev = (Event *)ctx->b_ev.p;
-  a.evcount = (int *)ctx->b_evcount.p;
-  a.hnet = (uint64_t *)ctx->b_hnet.p;
-  a.rep = (int *)ctx->b_rep.p;
-  a.clsg = (uint16_t *)ctx->b_clsg.p;
-  a.evorder = (int *)ctx->b_evorder.p;
-  a.evfam = (int *)ctx->b_evfam.p;
-  a.famkey = (uint64_t *)ctx->b_famkey.p;
-  a.famht = (unsigned long long *)ctx->b_famht.p;
-  a.famhv = (int *)ctx->b_famhv.p;
-  a.famscore = (double *)ctx->b_famscore.p;
-  a.lval = (double *)ctx->b_lval.p;
-  a.uniq_u = (int *)ctx->b_uniq.p;
-  a.famcount = (int *)ctx->b_famcount.p;
+  a.items = (const ItemMeta *)sl.items.p;
+  a.order = (const int *)sl.order.p;
+  a.labels = (const uint32_t *)sl.labels.p;
+  a.tiles = (const uint32_t *)sl.tiles.p;
+  a.info = (uint32_t *)sl.info.p;
+  a.tab = (uint16_t *)sl.tab.p;
+  a.lfull = (uint16_t *)sl.lfull.p;
+  a.lcnt = (uint8_t *)sl.lcnt.p;
+  a.sflags = (uint8_t *)sl.sflags.p;
+  a.ev = (Event *)sl.ev.p;
+  a.evcount = (int *)sl.evcount.p;
+  a.hnet = (uint64_t *)sl.hnet.p;
+  a.rep = (int *)sl.rep.p;
+  a.clsg = (uint16_t *)sl.clsg.p;
+  a.evorder = (int *)sl.evorder.p;
+  a.evfam = (int *)sl.evfam.p;
+  a.famkey = (uint64_t *)sl.famkey.p;
+  a.famht = (unsigned long long *)sl.famht.p;
+  a.famhv = (int *)sl.famhv.p;
+  a.famscore = (double *)sl.famscore.p;
+  a.lval = (double *)sl.lval.p;
+  a.uniq_u = (int *)sl.uniq.p;
+  a.famcount = (int *)sl.famcount.p;
   a.js = (double *)ctx->b_js.p;
   a.perms = perms;
-  a.status = (int *)ctx->b_status.p;
-  a.tests = (long long *)ctx->b_tests.p;
+  a.status = (int *)sl.status.p;
+  a.tests = (long long *)sl.tests.p;
 
-  hipEvent_t ev[5];
-  for (int k = 0; k < 5; k++) hipEventCreate(&ev[k]);
-  hipEventRecord(ev[0], st);
+  for (int k = 0; k < 5; k++)
+    if (!sl.tev[k]) HIPCHK(ctx, hipEventCreate(&sl.tev[k]));
+  hipEventRecord(sl.tev[0], st);
   if (!ch.tiles.empty()) k_count_test<<<(unsigned)ch.tiles.size(), 256, 0, st>>>(a);
-  hipEventRecord(ev[1], st);
+  hipEventRecord(sl.tev[1], st);
   if (!plain_tests) k_loo<<<nitems, 256, 0, st>>>(a);
-  hipEventRecord(ev[2], st);
+  hipEventRecord(sl.tev[2], st);
   const size_t rows_lds = (size_t)S * 8 + (size_t)2 * EDDY_NWAVES * maxG * 4 + (size_t)N * 2 + (size_t)2 * maxG + 16;
   k_rows<<<nitems, 256, rows_lds, st>>>(a);
-  hipEventRecord(ev[3], st);
+  hipEventRecord(sl.tev[3], st);
   const size_t score_lds = (size_t)EDDY_NWAVES * 12 * bitw * 8 + (size_t)ht * 12 + (size_t)(4 * S + 1 + 2 * maxG) * 4;
   k_score<<<nitems, 256, score_lds, st>>>(a);
-  hipEventRecord(ev[4], st);
+  hipEventRecord(sl.tev[4], st);
   HIPCHK(ctx, hipGetLastError());
-  out.status.assign(nitems, 0);
-  out.evc.assign(nitems, 0);
-  out.fc.assign(nitems, 0);
-  out.uq.assign(nitems, 0);
-  out.tests.assign(nitems, 0);
-  HIPCHK(ctx, hipMemcpyAsync(out.status.data(), ctx->b_status.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
-  HIPCHK(ctx, hipMemcpyAsync(out.evc.data(), ctx->b_evcount.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
-  HIPCHK(ctx, hipMemcpyAsync(out.fc.data(), ctx->b_famcount.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
-  HIPCHK(ctx, hipMemcpyAsync(out.uq.data(), ctx->b_uniq.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
-  HIPCHK(ctx, hipMemcpyAsync(out.tests.data(), ctx->b_tests.p, sizeof(long long) * nitems, hipMemcpyDeviceToHost,
-                             st));
+  int *h = sl.h_int;
+  const int hc = sl.h_cap;
+  HIPCHK(ctx, hipMemcpyAsync(h, sl.status.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(h + hc, sl.evcount.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(h + 2 * hc, sl.famcount.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(h + 3 * hc, sl.uniq.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipMemcpyAsync(sl.h_tests, sl.tests.p, sizeof(long long) * nitems, hipMemcpyDeviceToHost, st));
+  sl.busy = true;
+  sl.plain_tests = plain_tests;
+  sl.has_tiles = !ch.tiles.empty();
+  ctx->hp.launch += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hl0).count();
+  return EDDY_OK;
+}
+
+// Waits for the slot's launch and copies its per-item results out.
+int wait_chunk(eddy_ctx *ctx, Slot &sl, ChunkOut &out) {
   auto hl1 = std::chrono::steady_clock::now();
-  HIPCHK(ctx, hipStreamSynchronize(st));
-  auto hl2 = std::chrono::steady_clock::now();
-  ctx->hp.launch += std::chrono::duration<double, std::milli>(hl1 - hl0).count();
-  ctx->hp.sync += std::chrono::duration<double, std::milli>(hl2 - hl1).count();
+  HIPCHK(ctx, hipStreamSynchronize(sl.st));
+  ctx->hp.sync += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hl1).count();
+  sl.busy = false;
+  const int nitems = (int)sl.ch.items.size();
+  const int hc = sl.h_cap;
+  out.status.assign(sl.h_int, sl.h_int + nitems);
+  out.evc.assign(sl.h_int + hc, sl.h_int + hc + nitems);
+  out.fc.assign(sl.h_int + 2 * hc, sl.h_int + 2 * hc + nitems);
+  out.uq.assign(sl.h_int + 3 * hc, sl.h_int + 3 * hc + nitems);
+  out.tests.assign(sl.h_tests, sl.h_tests + nitems);
   for (int k = 0; k < 4; k++) {
-    if (k == 1 && plain_tests) continue;
-    if (k == 0 && ch.tiles.empty()) continue;
+    if (k == 1 && sl.plain_tests) continue;
+    if (k == 0 && !sl.has_tiles) continue;
     float ms = 0;
-    hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+    hipEventElapsedTime(&ms, sl.tev[k], sl.tev[k + 1]);
     ctx->timing.kernel_ms[k] += ms;
     ctx->timing.launches[k] += 1;
   }
-  for (int k = 0; k < 5; k++) hipEventDestroy(ev[k]);
   if (getenv("EDDY_DEBUG")) {
     int mx = 0, bad = 0, nb = 0;
     for (int i = 0; i < nitems; i++) {
@@ -568,8 +629,8 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
       bad |= out.status[i];
       nb += out.status[i] != 0;
     }
-    fprintf(stderr, "[eddy] launch items=%d pairs=%lld attempt=%d max_events=%d failing=%d status=%d\n", nitems,
-            ch.pairs, attempt, mx, nb, bad);
+    fprintf(stderr, "[eddy] chunk items=%d pairs=%lld max_events=%d failing=%d status=%d\n", nitems, sl.ch.pairs, mx,
+            nb, bad);
   }
   return EDDY_OK;
 }
@@ -577,6 +638,7 @@ int launch_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, con
 // Grow a device buffer keeping its first `keep` bytes (stream-ordered copy).
 int grow_keep(eddy_ctx *ctx, DevBuf &b, size_t bytes, size_t keep) {
   if (b.n >= bytes) return EDDY_OK;
+  HIPCHK(ctx, hipDeviceSynchronize());  // both slots may be copying into the store
   DevBuf nb;
   size_t want = bytes + bytes / 2;
   HIPCHK(ctx, hipMalloc(&nb.p, want));
@@ -619,10 +681,12 @@ int p0_begin(eddy_ctx *ctx, const std::vector<SetMeta> &meta) {
   return EDDY_OK;
 }
 
-// Copy the permutation-0 state of the finished items `idx` of a chunk into
-// the store (one slice-copy launch, stream-ordered before the next chunk).
-int capture_items(eddy_ctx *ctx, const Chunk &ch, const std::vector<int> &idx, const ChunkOut &o) {
+// Copy the permutation-0 state of the finished items `idx` of the slot's
+// chunk into the store (one slice-copy launch on the slot's stream, ordered
+// before the slot's next chunk).
+int capture_items(eddy_ctx *ctx, Slot &sl, const std::vector<int> &idx, const ChunkOut &o) {
   auto hc0 = std::chrono::steady_clock::now();
+  const Chunk &ch = sl.ch;
   P0Store &q = ctx->p0;
   const int S = ctx->N + 2;
   long long add_ev = 0, add_fam = 0;
@@ -634,40 +698,42 @@ int capture_items(eddy_ctx *ctx, const Chunk &ch, const std::vector<int> &idx, c
   CHK(grow_keep(ctx, q.famkey, sizeof(uint64_t) * std::max(1LL, q.famtot + add_fam), sizeof(uint64_t) * q.famtot));
   CHK(grow_keep(ctx, q.famscore, sizeof(double) * 2 * std::max(1LL, q.famtot + add_fam),
                 sizeof(double) * 2 * q.famtot));
-  std::vector<Slice> &sl = q.hslices;
-  sl.clear();
+  std::vector<Slice> sv;
   auto add = [&](const void *src, void *dst, long long bytes) {
-    if (bytes > 0) sl.push_back(Slice{src, dst, bytes});
+    if (bytes > 0) sv.push_back(Slice{src, dst, bytes});
   };
   for (int k : idx) {
     const ItemMeta &im = ch.items[k];
     const SetMeta &sm = q.meta[im.set];
     const int set = im.set;
-    add((const uint32_t *)ctx->b_info.p + im.infooff, (uint32_t *)q.info.p + sm.pairbase, 4LL * sm.c);
-    add((const uint8_t *)ctx->b_lcnt.p + im.goff * 2, (uint8_t *)q.lcnt.p + sm.genebase * 2, 2LL * sm.G);
-    add((const uint16_t *)ctx->b_lfull.p + im.goff * 2 * EDDY_MAX_MP,
-        (uint16_t *)q.lfull.p + sm.genebase * 2 * EDDY_MAX_MP, 2LL * 2 * sm.G * EDDY_MAX_MP);
-    add((const int *)ctx->b_rep.p + im.soff, (int *)q.rep.p + (size_t)set * S, 4LL * S);
-    add((const double *)ctx->b_lval.p + im.soff * 2, (double *)q.lval.p + (size_t)set * 2 * S, 16LL * S);
+    add((const uint32_t *)sl.info.p + im.infooff, (uint32_t *)q.info.p + sm.pairbase, 4LL * sm.c);
+    add((const uint8_t *)sl.lcnt.p + im.goff * 2, (uint8_t *)q.lcnt.p + sm.genebase * 2, 2LL * sm.G);
+    add((const uint16_t *)sl.lfull.p + im.goff * 2 * EDDY_MAX_MP, (uint16_t *)q.lfull.p + sm.genebase * 2 * EDDY_MAX_MP,
+        2LL * 2 * sm.G * EDDY_MAX_MP);
+    add((const int *)sl.rep.p + im.soff, (int *)q.rep.p + (size_t)set * S, 4LL * S);
+    add((const double *)sl.lval.p + im.soff * 2, (double *)q.lval.p + (size_t)set * 2 * S, 16LL * S);
     const int ne = std::min(o.evc[k], im.evcap);
     q.evoff[set] = q.evtot;
     q.evn[set] = ne;
-    add((const Event *)ctx->b_ev.p + im.evoff, (Event *)q.ev.p + q.evtot, (long long)sizeof(Event) * ne);
+    add((const Event *)sl.ev.p + im.evoff, (Event *)q.ev.p + q.evtot, (long long)sizeof(Event) * ne);
     q.evtot += ne;
     q.famoff[set] = q.famtot;
     q.famn[set] = o.fc[k];
-    add((const uint64_t *)ctx->b_famkey.p + im.foff, (uint64_t *)q.famkey.p + q.famtot, 8LL * o.fc[k]);
-    add((const double *)ctx->b_famscore.p + im.foff * 2, (double *)q.famscore.p + q.famtot * 2, 16LL * o.fc[k]);
+    add((const uint64_t *)sl.famkey.p + im.foff, (uint64_t *)q.famkey.p + q.famtot, 8LL * o.fc[k]);
+    add((const double *)sl.famscore.p + im.foff * 2, (double *)q.famscore.p + q.famtot * 2, 16LL * o.fc[k]);
     q.famtot += o.fc[k];
     q.U[set] = o.uq[k];
     q.done[set] = 1;
     ctx->last_p0[set] = Perm0();
   }
-  if (!sl.empty()) {
-    CHK(grow(ctx, q.slices, sizeof(Slice) * sl.size()));
-    HIPCHK(ctx, hipMemcpyAsync(q.slices.p, sl.data(), sizeof(Slice) * sl.size(), hipMemcpyHostToDevice, ctx->stream));
-    k_copy_slices<<<(unsigned)std::min<size_t>(sl.size(), 2048), 256, 0, ctx->stream>>>((const Slice *)q.slices.p,
-                                                                                        (int)sl.size());
+  if (!sv.empty()) {
+    // the slice list travels in the slot's own buffer (its copy is ordered on
+    // the slot's stream; the host vector is pinned by a synchronous copy)
+    CHK(grow(ctx, sl.slices, sizeof(Slice) * sv.size()));
+    HIPCHK(ctx, hipStreamSynchronize(sl.st));
+    HIPCHK(ctx, hipMemcpy(sl.slices.p, sv.data(), sizeof(Slice) * sv.size(), hipMemcpyHostToDevice));
+    k_copy_slices<<<(unsigned)std::min<size_t>(sv.size(), 2048), 256, 0, sl.st>>>((const Slice *)sl.slices.p,
+                                                                                 (int)sv.size());
     HIPCHK(ctx, hipGetLastError());
   }
   ctx->hp.capture += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hc0).count();
@@ -682,7 +748,7 @@ int p0_fetch(eddy_ctx *ctx, int set) {
   if (!h.rep.empty()) return EDDY_OK;
   const SetMeta &sm = q.meta[set];
   const int S = ctx->N + 2;
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  HIPCHK(ctx, hipDeviceSynchronize());  // captures run on the slots' streams
   h.U = q.U[set];
   h.info.resize(sm.c);
   h.lcnt.resize(2 * sm.G);
@@ -707,30 +773,50 @@ int p0_fetch(eddy_ctx *ctx, int set) {
   return EDDY_OK;
 }
 
-// Items of a chunk through the kernels; an item whose events or families
-// overflow its capacity, or whose network hashes collide, runs again alone
-// with its exact count and a new hash seed.  Finished permutation-0 items are
-// captured when `capture`.
-int run_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, const eddy_params *p, int perms,
-              bool capture, std::vector<long long> &tests_out) {
-  const int N = ctx->N, S = N + 2;
-  const int nitems = (int)ch.items.size();
+// Starts a chunk on a slot (the chunk moves into it): first launch, with
+// per-item event capacities from the set sizes.
+int start_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, Chunk &&ch, const eddy_params *p,
+                int perms) {
+  const int S = ctx->N + 2;
+  sl.ch = std::move(ch);
+  const int nitems = (int)sl.ch.items.size();
+  sl.cap.resize(nitems);
+  for (int i = 0; i < nitems; i++) sl.cap[i] = std::max(512, 2 * S + 24 * sets[sl.ch.items[i].set].G);
   if (nitems == 0) return EDDY_OK;
-  std::vector<int> cap(nitems);
-  for (int i = 0; i < nitems; i++) cap[i] = std::max(512, 2 * S + 24 * sets[ch.items[i].set].G);
+  return launch_chunk(ctx, sl, sets, p, perms, sl.cap, 0);
+}
+
+// Finishes the slot's chunk: waits for it; an item whose events or families
+// overflowed, or whose network hashes collided, runs again alone on the same
+// slot with its exact count and a new hash seed; finished permutation-0 items
+// are captured when `capture`.  tests_out[i] per item of the chunk, whose
+// items stay in sl.ch.
+int finish_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, const eddy_params *p, int perms,
+                 bool capture, std::vector<long long> &tests_out) {
+  const int N = ctx->N;
+  const int nitems = (int)sl.ch.items.size();
+  tests_out.assign(nitems, 0);
+  if (nitems == 0 || !sl.busy) return EDDY_OK;
+  std::vector<int> cap = sl.cap;
   std::vector<int> todo(nitems);
   for (int i = 0; i < nitems; i++) todo[i] = i;
-  Chunk sub;
+  Chunk orig;
+  bool replaced = false;
   for (int attempt = 0; attempt < 6 && !todo.empty(); attempt++) {
-    Chunk *cur = &ch;
     if (attempt > 0) {
-      sub_chunk(ch, todo, sets, N, sub);
-      cur = &sub;
+      if (!replaced) {
+        orig = std::move(sl.ch);
+        replaced = true;
+      }
+      Chunk sub;
+      sub_chunk(orig, todo, sets, N, sub);
+      sl.ch = std::move(sub);
+      std::vector<int> subcap(todo.size());
+      for (size_t k = 0; k < todo.size(); k++) subcap[k] = cap[todo[k]];
+      CHK(launch_chunk(ctx, sl, sets, p, perms, subcap, attempt));
     }
-    std::vector<int> subcap(todo.size());
-    for (size_t k = 0; k < todo.size(); k++) subcap[k] = cap[todo[k]];
     ChunkOut o;
-    CHK(launch_chunk(ctx, sets, *cur, p, perms, subcap, attempt, o));
+    CHK(wait_chunk(ctx, sl, o));
     std::vector<int> next, done_idx;
     for (size_t k = 0; k < todo.size(); k++) {
       const int i = todo[k];
@@ -746,17 +832,26 @@ int run_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, const 
       tests_out[i] = o.tests[k];
       ctx->timing.tests += o.tests[k];
       ctx->timing.items += 1;
-      ctx->timing.pairs += sets[ch.items[i].set].c;
-      if (capture && ch.items[i].perm == 0) done_idx.push_back((int)k);
+      ctx->timing.pairs += sets[sl.ch.items[k].set].c;
+      if (capture && sl.ch.items[k].perm == 0) done_idx.push_back((int)k);
     }
-    if (!done_idx.empty()) CHK(capture_items(ctx, *cur, done_idx, o));
+    if (!done_idx.empty()) CHK(capture_items(ctx, sl, done_idx, o));
     todo.swap(next);
   }
   if (!todo.empty()) {
     ctx->err = "event capacity or hash collisions persisted after retries";
     return EDDY_ERR_CAPACITY;
   }
+  if (replaced) sl.ch = std::move(orig);
   return EDDY_OK;
+}
+
+// One chunk start to end on slot 0 (plain schedule, single permutations).
+int run_chunk(eddy_ctx *ctx, const std::vector<SetMeta> &sets, Chunk &ch, const eddy_params *p, int perms,
+              bool capture, std::vector<long long> &tests_out) {
+  Slot &sl = ctx->slot[0];
+  CHK(start_chunk(ctx, sl, sets, std::move(ch), p, perms));
+  return finish_chunk(ctx, sl, sets, p, perms, capture, tests_out);
 }
 
 void add_item(Chunk &ch, const std::vector<SetMeta> &sets, int set, int perm, const std::vector<uint32_t> &labbits,
@@ -1128,15 +1223,12 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     Chunk ch;
     int s = 0, chunk_first = 0;
     bool trained = false;
-    // runs the chunk of sets [chunk_first, s); returns the first set whose
-    // permutation 0 gave NaN unexpectedly (the stream must restart after
-    // it), or -1
-    auto flush = [&](int &bad) -> int {
-      bad = -1;
-      if (ch.items.empty()) {
-        chunk_first = s;
-        return EDDY_OK;
-      }
+    // Two chunks may be in flight (one per slot): the next one is launched
+    // before the older one is finished, so one's kernels overlap the other's
+    // tail and the host's bookkeeping.
+    std::deque<int> inflight;
+    int next_slot = 0, launched_last = -1;
+    auto launch = [&]() -> int {
       ctx->k_labhits = chunk_first < collect_upto ? (unsigned *)ctx->b_labhits.p : nullptr;
       ctx->k_label_perms = chunk_first < collect_upto ? label_perms : 0;
       ctx->k_win = (windows_ready && chunk_first >= win_from) ? (const int *)ctx->b_win.p : nullptr;
@@ -1145,68 +1237,121 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
         ctx->k_win = (const int *)ctx->b_win2.p;
         if (r.first_windowed_set < 0) r.first_windowed_set = chunk_first;
       }
-      std::vector<long long> t(ch.items.size(), 0);
-      CHK(run_chunk(ctx, meta, ch, p, perms, true, t));
-      for (size_t i = 0; i < ch.items.size(); i++) tests_eval[ch.items[i].set] += t[i];
+      Slot &sl = ctx->slot[next_slot];
+      sl.first = chunk_first;
+      sl.last = s - 1;
+      int rc = start_chunk(ctx, sl, meta, std::move(ch), p, perms);
       ctx->k_labhits = nullptr;
       ctx->k_label_perms = 0;
       ctx->k_win = nullptr;
-      // JS of permutation 0 of the chunk's sets: one strided copy
-      auto hj = std::chrono::steady_clock::now();
-      const int n = s - chunk_first;
-      std::vector<double> js0(n);
-      HIPCHK(ctx, hipMemcpy2DAsync(js0.data(), sizeof(double), (double *)ctx->b_js.p + (size_t)chunk_first * perms,
-                                   sizeof(double) * perms, sizeof(double), n, hipMemcpyDeviceToHost, ctx->stream));
-      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-      for (int q = chunk_first; q < s; q++) {
-        if (meta[q].G < 2) continue;
-        if (std::isnan(js0[q - chunk_first])) {
-          nan0[q] = 1;
-          if (!prod.stop0[q] && bad < 0) bad = q;
-        }
-      }
-      ctx->hp.js0 += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hj).count();
+      CHK(rc);
+      inflight.push_back(next_slot);
+      next_slot ^= 1;
+      launched_last = s - 1;
       ch = Chunk();
       chunk_first = s;
       budget = std::min(budget * 2, budget_max);
       return EDDY_OK;
     };
-    auto rollback = [&](int q, int last) -> int {
+    // finishes the oldest chunk in flight; bad = its first set whose
+    // permutation 0 gave NaN unexpectedly (the stream must restart after it)
+    auto finish_oldest = [&](int &bad) -> int {
+      bad = -1;
+      Slot &sl = ctx->slot[inflight.front()];
+      inflight.pop_front();
+      std::vector<long long> t;
+      CHK(finish_chunk(ctx, sl, meta, p, perms, true, t));
+      for (size_t i = 0; i < sl.ch.items.size(); i++) tests_eval[sl.ch.items[i].set] += t[i];
+      // JS of permutation 0 of the chunk's sets: one strided copy
+      auto hj = std::chrono::steady_clock::now();
+      const int n = sl.last - sl.first + 1;
+      if (n > 0) {
+        std::vector<double> js0(n);
+        HIPCHK(ctx, hipMemcpy2D(js0.data(), sizeof(double), (double *)ctx->b_js.p + (size_t)sl.first * perms,
+                                sizeof(double) * perms, sizeof(double), n, hipMemcpyDeviceToHost));
+        for (int q = sl.first; q <= sl.last; q++) {
+          if (meta[q].G < 2) continue;
+          if (std::isnan(js0[q - sl.first])) {
+            nan0[q] = 1;
+            if (!prod.stop0[q] && bad < 0) bad = q;
+          }
+        }
+      }
+      ctx->hp.js0 += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hj).count();
+      return EDDY_OK;
+    };
+    auto drain = [&](int &bad) -> int {
+      bad = -1;
+      while (!inflight.empty()) {
+        int b;
+        CHK(finish_oldest(b));
+        if (b >= 0 && (bad < 0 || b < bad)) bad = b;
+      }
+      return EDDY_OK;
+    };
+    auto rollback = [&](int q) -> int {
       // set q stops after permutation 0: drop what its later permutations
-      // left, and replay the stream for the sets after it
+      // left, and replay the stream for the sets after it (every chunk has
+      // been finished; the chunk being built is dropped)
+      int b;
+      CHK(drain(b));
+      ch = Chunk();
+      const int last = launched_last;
       prod.stop();
       prod.stop0[q] = 1;
       const size_t rest = (size_t)(perms - 1);
       if (rest)
         HIPCHK(ctx, hipMemsetAsync((double *)ctx->b_js.p + (size_t)q * perms + 1, 0, sizeof(double) * rest,
                                    ctx->stream));
-      if (q < collect_upto) {
+      if (q < collect_upto && last >= q) {
         const long long b0 = meta[q].pairbase, b1 = meta[last].pairbase + meta[last].c;
         HIPCHK(ctx, hipMemsetAsync((unsigned *)ctx->b_labhits.p + b0, 0, sizeof(unsigned) * (b1 - b0),
                                    ctx->stream));
       }
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
       for (int k = q + 1; k <= last; k++) nan0[k] = meta[k].G < 2;
       ctx->stream_gen.restore(prod.after0[q]);
       for (int k = q + 1; k < nsets; k++) std::vector<uint32_t>().swap(prod.lab[k]);
       s = q + 1;
       chunk_first = s;
+      launched_last = q;
       prod.start(q + 1);
       return EDDY_OK;
     };
-    while (s < nsets || !ch.items.empty()) {
+    while (true) {
       const bool boundary = s == nsets || s == collect_upto || (!lc && s == win_from);
-      // flush when the chunk is full, at a boundary, or when the next set's
+      // launch when the chunk is full, at a boundary, or when the next set's
       // labels are not ready yet: the GPU then works while the replay
       // catches up
       const bool starved = s < nsets && prod.ready.load(std::memory_order_acquire) <= s;
       if (!ch.items.empty() && (ch.pairs >= budget || boundary || (starved && ch.pairs >= (budget >> 3)))) {
-        const int last = s - 1;
-        int bad;
-        CHK(flush(bad));
-        if (bad >= 0) CHK(rollback(bad, last));
+        if (inflight.size() == 2) {
+          int bad;
+          CHK(finish_oldest(bad));
+          if (bad >= 0) {
+            CHK(rollback(bad));
+            continue;
+          }
+        }
+        CHK(launch());
         continue;
       }
-      if (s == nsets) break;
+      if (s == nsets) {
+        if (inflight.empty()) break;
+        int bad;
+        CHK(finish_oldest(bad));
+        if (bad >= 0) CHK(rollback(bad));
+        continue;
+      }
+      if (lc && s == collect_upto && !trained) {
+        // the logged sets' chunks must be finished before training
+        int bad;
+        CHK(drain(bad));
+        if (bad >= 0) {
+          CHK(rollback(bad));
+          continue;
+        }
+      }
       if (lc && s == collect_upto && !trained) {
         // all logged sets are done: train on the second stream while the
         // next chunks run with deferred full tests; chunks launched after the
@@ -1315,6 +1460,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       }
     }
   }
+  HIPCHK(ctx, hipDeviceSynchronize());  // the slots' last captures
   auto t3 = std::chrono::steady_clock::now();
   // p-values on the GPU
   GROW(b_nan0, sizeof(int) * std::max(1, nsets));
