@@ -73,6 +73,8 @@ struct Slot {
   int *h_int = nullptr;  // pinned: status, events, families, unique networks (4 x h_cap)
   long long *h_tests = nullptr;
   int h_cap = 0;
+  Slice *h_slices = nullptr;  // pinned staging of the capture slice list
+  size_t h_slices_cap = 0;
   bool busy = false;
   bool plain_tests = false, has_tiles = false;
   Chunk ch;             // the chunk in flight
@@ -238,6 +240,7 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
       if (b->p) hipFree(b->p);
     if (sl.h_int) hipHostFree(sl.h_int);
     if (sl.h_tests) hipHostFree(sl.h_tests);
+    if (sl.h_slices) hipHostFree(sl.h_slices);
     for (hipEvent_t e : sl.tev)
       if (e) hipEventDestroy(e);
     if (sl.st) hipStreamDestroy(sl.st);
@@ -727,11 +730,21 @@ int capture_items(eddy_ctx *ctx, Slot &sl, const std::vector<int> &idx, const Ch
     ctx->last_p0[set] = Perm0();
   }
   if (!sv.empty()) {
-    // the slice list travels in the slot's own buffer (its copy is ordered on
-    // the slot's stream; the host vector is pinned by a synchronous copy)
+    // the slice list goes through the slot's pinned staging buffer, copied
+    // and consumed in order on the slot's stream (the slot's previous use of
+    // the buffer is ordered before it on the same stream, and the host only
+    // refills it after waiting for that stream)
     CHK(grow(ctx, sl.slices, sizeof(Slice) * sv.size()));
-    HIPCHK(ctx, hipStreamSynchronize(sl.st));
-    HIPCHK(ctx, hipMemcpy(sl.slices.p, sv.data(), sizeof(Slice) * sv.size(), hipMemcpyHostToDevice));
+    if (sl.h_slices_cap < sv.size()) {
+      if (sl.h_slices) HIPCHK(ctx, hipHostFree(sl.h_slices));
+      sl.h_slices = nullptr;
+      sl.h_slices_cap = 0;
+      const size_t want = sv.size() + sv.size() / 2;
+      HIPCHK(ctx, hipHostMalloc((void **)&sl.h_slices, sizeof(Slice) * want, hipHostMallocDefault));
+      sl.h_slices_cap = want;
+    }
+    std::copy(sv.begin(), sv.end(), sl.h_slices);
+    HIPCHK(ctx, hipMemcpyAsync(sl.slices.p, sl.h_slices, sizeof(Slice) * sv.size(), hipMemcpyHostToDevice, sl.st));
     k_copy_slices<<<(unsigned)std::min<size_t>(sv.size(), 2048), 256, 0, sl.st>>>((const Slice *)sl.slices.p,
                                                                                  (int)sv.size());
     HIPCHK(ctx, hipGetLastError());
