@@ -8,6 +8,7 @@
 #include <deque>
 #include <map>
 #include <random>
+#include <mutex>
 #include <thread>
 #include <chrono>
 #include <cmath>
@@ -32,6 +33,7 @@ __global__ void k_gather(const uint8_t *x, int N, int NP, const int *genes, cons
                          uint8_t *xs);
 __global__ void k_window(const SetMeta *sets, int nsets, const float *score, int mp, float kappa, int *win);
 __global__ void k_copy_slices(const Slice *sl, int n);
+__global__ void k_js0(const double *js, int perms, int first, int n, double *out);
 
 namespace {
 
@@ -120,7 +122,7 @@ struct eddy_ctx {
   // device buffers, grown on demand
   DevBuf b_sets, b_xs, b_prior, b_items, b_labels, b_tiles, b_info, b_tab, b_lfull, b_lcnt, b_sflags, b_ev,
       b_evcount, b_hnet, b_rep, b_clsg, b_evorder, b_evfam, b_famkey, b_famscore, b_lval, b_uniq, b_js,
-      b_status, b_tests, b_genes, b_rowoff, b_nan0, b_pout, b_famcount, b_counters, b_famht, b_famhv, b_bits, b_order;
+      b_status, b_tests, b_genes, b_rowoff, b_nan0, b_pout, b_famcount, b_counters, b_famht, b_famhv, b_bits, b_order, b_js0;
   // last run
   int last_perms = 0;
   std::vector<int> last_G;
@@ -228,7 +230,7 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
                    &ctx->b_mx, &ctx->b_my, &ctx->b_mxb, &ctx->b_myb, &ctx->b_midx, &ctx->b_mscr, &ctx->b_mlbase,
                    &ctx->b_mlinv, &ctx->p0.info, &ctx->p0.lcnt, &ctx->p0.lfull, &ctx->p0.rep, &ctx->p0.lval,
                    &ctx->p0.ev, &ctx->p0.famkey, &ctx->p0.famscore, &ctx->p0.slices, &ctx->b_bits,
-                   &ctx->b_win2, &ctx->b_order};
+                   &ctx->b_win2, &ctx->b_order, &ctx->b_js0};
   for (DevBuf *b : all)
     if (b->p) hipFree(b->p);
   for (Slot &sl : ctx->slot) {
@@ -1012,26 +1014,35 @@ namespace {
 struct Producer {
   eddy_ctx *ctx = nullptr;
   int nsets = 0, perms = 0, N = 0, n1 = 0;
-  std::vector<char> stop0;
+  std::vector<char> stop0;  // guarded by mu while the thread runs
   std::vector<std::vector<uint32_t>> lab;
   std::vector<PermStream::State> after0;
   std::atomic<int> ready{0};
   std::atomic<bool> quit{false};
+  std::mutex mu;
+  int pos = 0;  // sets whose stop0 the thread has read (guarded by mu)
   std::thread th;
   double ms = 0;
 
   void start(int from) {
     quit.store(false);
     ready.store(from);
+    pos = from;
     th = std::thread([this, from]() {
       auto h0 = std::chrono::steady_clock::now();
       const int words = (N + 31) / 32;
       PermStream &ps = ctx->stream_gen;
       for (int s = from; s < nsets; s++) {
         if (quit.load(std::memory_order_relaxed)) break;
+        bool stop;
+        {
+          std::lock_guard<std::mutex> g(mu);
+          stop = stop0[s] != 0;
+          pos = s + 1;
+        }
         ps.class2_bits(N, n1, nullptr);  // permutation 0's draws
         after0[s] = ps.save();
-        if (!stop0[s]) {
+        if (!stop) {
           std::vector<uint32_t> &L = lab[s];
           L.resize((size_t)(perms - 1) * words);
           for (int pm = 1; pm < perms; pm++) ps.class2_bits(N, n1, &L[(size_t)(pm - 1) * words]);
@@ -1044,6 +1055,26 @@ struct Producer {
   void stop() {
     quit.store(true);
     if (th.joinable()) th.join();
+  }
+  // Marks sets `qs` as stopping after permutation 0; returns the first of
+  // them whose flag the thread has already read (its later draws went to the
+  // wrong sets), or -1.
+  int mark_stop0(const std::vector<int> &qs) {
+    std::lock_guard<std::mutex> g(mu);
+    int first = -1;
+    for (int q : qs) {
+      stop0[q] = 1;
+      if (q < pos && (first < 0 || q < first)) first = q;
+    }
+    return first;
+  }
+  // Restarts the replay after set q's permutation 0 (no set after q may
+  // have been consumed yet).
+  void replay_after(int q) {
+    stop();
+    ctx->stream_gen.restore(after0[q]);
+    for (int k = q; k < nsets; k++) std::vector<uint32_t>().swap(lab[k]);
+    start(q + 1);
   }
   ~Producer() { stop(); }
 };
@@ -1227,20 +1258,157 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     prod.lab.assign(nsets, std::vector<uint32_t>());
     prod.after0.resize(nsets);
     prod.start(0);
-    // chunks grow from 8M to 64M pairs: the GPU starts early, later
-    // launches amortise their fixed costs
-    long long budget = 8LL << 20;
-    const long long budget_max = 64LL << 20;
+    for (int q = 0; q < nsets; q++) nan0[q] = meta[q].G < 2;
+    const char *pe = getenv("EDDY_PROFILE");
+    const bool trace = pe && pe[0] == '2';
+    auto now_ms = [&]() {
+      return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    };
+    const long long budget_min = 8LL << 20, budget_max = 64LL << 20;
+    long long budget = budget_min;
     const int words = (N + 31) / 32;
     std::vector<uint32_t> one(words);
     Chunk ch;
-    int s = 0, chunk_first = 0;
+    int s = 0, chunk_first = 0, launched_last = -1;
     bool trained = false;
+    hipEvent_t slot_ev[2] = {nullptr, nullptr};
+    std::vector<long long> tests0(nsets, 0);  // permutation 0's share of tests_eval
     // Two chunks may be in flight (one per slot): the next one is launched
     // before the older one is finished, so one's kernels overlap the other's
     // tail and the host's bookkeeping.
     std::deque<int> inflight;
-    int next_slot = 0, launched_last = -1;
+    int next_slot = 0, p0_pending = 0;
+    bool slot_p0[2] = {false, false};  // the slot holds a permutation-0 chunk
+    auto start_on_slot = [&](Chunk &&c, int first, int last, bool p0) -> int {
+      Slot &sl = ctx->slot[next_slot];
+      sl.first = first;
+      sl.last = last;
+      CHK(start_chunk(ctx, sl, meta, std::move(c), p, perms));
+      inflight.push_back(next_slot);
+      slot_p0[next_slot] = p0;
+      if (p0) p0_pending++;
+      if (trace)
+        fprintf(stderr, "[eddy] t=%.1f launch slot %d sets %d..%d pairs %lld%s (replay at %d)\n", now_ms(), next_slot,
+                first, last, sl.ch.pairs, p0 ? " permutation 0" : "", prod.ready.load());
+      next_slot ^= 1;
+      return EDDY_OK;
+    };
+    // finishes the oldest chunk; a permutation-0 chunk adds its sets whose
+    // JS there is NaN to `stops`
+    auto finish_oldest = [&](std::vector<int> &stops) -> int {
+      const int k = inflight.front();
+      Slot &sl = ctx->slot[k];
+      inflight.pop_front();
+      std::vector<long long> t;
+      CHK(finish_chunk(ctx, sl, meta, p, perms, true, t));
+      if (trace) fprintf(stderr, "[eddy] t=%.1f finished sets %d..%d\n", now_ms(), sl.first, sl.last);
+      for (size_t i = 0; i < sl.ch.items.size(); i++) {
+        tests_eval[sl.ch.items[i].set] += t[i];
+        if (slot_p0[k]) tests0[sl.ch.items[i].set] += t[i];
+      }
+      if (slot_p0[k]) {
+        p0_pending--;
+        auto hj = std::chrono::steady_clock::now();
+        const int n = sl.last - sl.first + 1;
+        std::vector<double> js0(n);
+        CHK(grow(ctx, ctx->b_js0, sizeof(double) * n));
+        k_js0<<<(n + 255) / 256, 256, 0, sl.st>>>((const double *)ctx->b_js.p, perms, sl.first, n,
+                                                   (double *)ctx->b_js0.p);
+        HIPCHK(ctx, hipGetLastError());
+        HIPCHK(ctx, hipMemcpyAsync(js0.data(), ctx->b_js0.p, sizeof(double) * n, hipMemcpyDeviceToHost, sl.st));
+        HIPCHK(ctx, hipStreamSynchronize(sl.st));
+        for (int q = sl.first; q <= sl.last; q++)
+          if (meta[q].G >= 2 && std::isnan(js0[q - sl.first])) stops.push_back(q);
+        ctx->hp.js0 += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hj).count();
+      }
+      return EDDY_OK;
+    };
+    // Sets in `stops` stop after permutation 0 (main.cu:1429-1434) and draw
+    // no further permutations, which moves every later set's draws.  The
+    // replay and the chunks run ahead assuming no such stop; a stop the
+    // replay has passed restarts it after the stop, and one among the sets
+    // already taken into chunks also drops and re-runs everything after it.
+    // Permutation-0 chunks are launched before any other, so this happens
+    // while only the first chunk or two have been taken.
+    auto apply_stops = [&](std::vector<int> stops, bool &rolled) -> int {
+      rolled = false;
+      if (stops.empty()) return EDDY_OK;
+      int q = -1;
+      for (int k : stops)
+        if (k < s && (q < 0 || k < q)) q = k;
+      if (q < 0) {
+        for (int k : stops) nan0[k] = 1;
+        const int passed = prod.mark_stop0(stops);
+        if (passed >= 0) prod.replay_after(passed);
+        if (trace) fprintf(stderr, "[eddy] t=%.1f %zu stops, replay from %d\n", now_ms(), stops.size(), passed);
+        return EDDY_OK;
+      }
+      // finish everything in flight (the permutation-0 chunks among it may
+      // report earlier stops), then restart after the first stop taken
+      while (!inflight.empty()) CHK(finish_oldest(stops));
+      q = -1;
+      for (int k : stops)
+        if (k < s && (q < 0 || k < q)) q = k;
+      for (int k : stops) nan0[k] = 1;
+      prod.mark_stop0(stops);
+      const int last = launched_last;
+      if (q <= last) {
+        // launched: the results of q's later permutations and of every set
+        // after it go; the chunk being built goes too
+        for (int k = q; k <= last; k++) {
+          tests_eval[k] = tests0[k];
+          if (perms > 1)
+            HIPCHK(ctx, hipMemsetAsync((double *)ctx->b_js.p + (size_t)k * perms + 1, 0,
+                                       sizeof(double) * (perms - 1), ctx->stream));
+        }
+        if (q < collect_upto) {
+          const int e = std::min(last, collect_upto - 1);
+          const long long b0 = meta[q].pairbase, b1 = meta[e].pairbase + meta[e].c;
+          HIPCHK(ctx, hipMemsetAsync((unsigned *)ctx->b_labhits.p + b0, 0, sizeof(unsigned) * (b1 - b0),
+                                     ctx->stream));
+        }
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        ch = Chunk();
+        chunk_first = q + 1;
+        launched_last = q;
+      } else {
+        // only in the chunk being built: its sets before q stay
+        std::vector<int> keep;
+        for (int i = 0; i < (int)ch.items.size(); i++)
+          if (ch.items[i].set < q) keep.push_back(i);
+        Chunk kept;
+        sub_chunk(ch, keep, meta, N, kept);
+        ch = std::move(kept);
+      }
+      s = q + 1;
+      prod.replay_after(q);
+      rolled = true;
+      if (trace) fprintf(stderr, "[eddy] t=%.1f stop at set %d: sets %d..%d run again\n", now_ms(), q, q + 1, last);
+      return EDDY_OK;
+    };
+
+    // permutation 0 of every set first (which sets stop there is then known
+    // before their later permutations have gone far)
+    {
+      Chunk c0;
+      int f0 = 0;
+      std::vector<int> stops;
+      for (int q = 0; q <= nsets; q++) {
+        if (q < nsets && meta[q].G >= 2) add_item(c0, meta, q, 0, truebits, N);
+        if (!c0.items.empty() && (q == nsets || c0.pairs >= budget_max)) {
+          if (inflight.size() == 2) CHK(finish_oldest(stops));
+          CHK(start_on_slot(std::move(c0), f0, std::min(q, nsets - 1), true));
+          c0 = Chunk();
+          f0 = q + 1;
+        }
+      }
+      bool rolled;
+      CHK(apply_stops(stops, rolled));
+    }
+
+    // chunks of the later permutations grow from 8M to 64M pairs (again after
+    // the training boundary): the GPU starts early, later launches amortise
+    // their fixed costs
     auto launch = [&]() -> int {
       ctx->k_labhits = chunk_first < collect_upto ? (unsigned *)ctx->b_labhits.p : nullptr;
       ctx->k_label_perms = chunk_first < collect_upto ? label_perms : 0;
@@ -1250,85 +1418,15 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
         ctx->k_win = (const int *)ctx->b_win2.p;
         if (r.first_windowed_set < 0) r.first_windowed_set = chunk_first;
       }
-      Slot &sl = ctx->slot[next_slot];
-      sl.first = chunk_first;
-      sl.last = s - 1;
-      int rc = start_chunk(ctx, sl, meta, std::move(ch), p, perms);
+      int rc = start_on_slot(std::move(ch), chunk_first, s - 1, false);
       ctx->k_labhits = nullptr;
       ctx->k_label_perms = 0;
       ctx->k_win = nullptr;
       CHK(rc);
-      inflight.push_back(next_slot);
-      next_slot ^= 1;
       launched_last = s - 1;
       ch = Chunk();
       chunk_first = s;
       budget = std::min(budget * 2, budget_max);
-      return EDDY_OK;
-    };
-    // finishes the oldest chunk in flight; bad = its first set whose
-    // permutation 0 gave NaN unexpectedly (the stream must restart after it)
-    auto finish_oldest = [&](int &bad) -> int {
-      bad = -1;
-      Slot &sl = ctx->slot[inflight.front()];
-      inflight.pop_front();
-      std::vector<long long> t;
-      CHK(finish_chunk(ctx, sl, meta, p, perms, true, t));
-      for (size_t i = 0; i < sl.ch.items.size(); i++) tests_eval[sl.ch.items[i].set] += t[i];
-      // JS of permutation 0 of the chunk's sets: one strided copy
-      auto hj = std::chrono::steady_clock::now();
-      const int n = sl.last - sl.first + 1;
-      if (n > 0) {
-        std::vector<double> js0(n);
-        HIPCHK(ctx, hipMemcpy2D(js0.data(), sizeof(double), (double *)ctx->b_js.p + (size_t)sl.first * perms,
-                                sizeof(double) * perms, sizeof(double), n, hipMemcpyDeviceToHost));
-        for (int q = sl.first; q <= sl.last; q++) {
-          if (meta[q].G < 2) continue;
-          if (std::isnan(js0[q - sl.first])) {
-            nan0[q] = 1;
-            if (!prod.stop0[q] && bad < 0) bad = q;
-          }
-        }
-      }
-      ctx->hp.js0 += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hj).count();
-      return EDDY_OK;
-    };
-    auto drain = [&](int &bad) -> int {
-      bad = -1;
-      while (!inflight.empty()) {
-        int b;
-        CHK(finish_oldest(b));
-        if (b >= 0 && (bad < 0 || b < bad)) bad = b;
-      }
-      return EDDY_OK;
-    };
-    auto rollback = [&](int q) -> int {
-      // set q stops after permutation 0: drop what its later permutations
-      // left, and replay the stream for the sets after it (every chunk has
-      // been finished; the chunk being built is dropped)
-      int b;
-      CHK(drain(b));
-      ch = Chunk();
-      const int last = launched_last;
-      prod.stop();
-      prod.stop0[q] = 1;
-      const size_t rest = (size_t)(perms - 1);
-      if (rest)
-        HIPCHK(ctx, hipMemsetAsync((double *)ctx->b_js.p + (size_t)q * perms + 1, 0, sizeof(double) * rest,
-                                   ctx->stream));
-      if (q < collect_upto && last >= q) {
-        const long long b0 = meta[q].pairbase, b1 = meta[last].pairbase + meta[last].c;
-        HIPCHK(ctx, hipMemsetAsync((unsigned *)ctx->b_labhits.p + b0, 0, sizeof(unsigned) * (b1 - b0),
-                                   ctx->stream));
-      }
-      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-      for (int k = q + 1; k <= last; k++) nan0[k] = meta[k].G < 2;
-      ctx->stream_gen.restore(prod.after0[q]);
-      for (int k = q + 1; k < nsets; k++) std::vector<uint32_t>().swap(prod.lab[k]);
-      s = q + 1;
-      chunk_first = s;
-      launched_last = q;
-      prod.start(q + 1);
       return EDDY_OK;
     };
     while (true) {
@@ -1339,38 +1437,37 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       const bool starved = s < nsets && prod.ready.load(std::memory_order_acquire) <= s;
       if (!ch.items.empty() && (ch.pairs >= budget || boundary || (starved && ch.pairs >= (budget >> 3)))) {
         if (inflight.size() == 2) {
-          int bad;
-          CHK(finish_oldest(bad));
-          if (bad >= 0) {
-            CHK(rollback(bad));
-            continue;
-          }
+          std::vector<int> stops;
+          bool rolled;
+          CHK(finish_oldest(stops));
+          CHK(apply_stops(stops, rolled));
+          if (rolled) continue;
         }
         CHK(launch());
         continue;
       }
       if (s == nsets) {
         if (inflight.empty()) break;
-        int bad;
-        CHK(finish_oldest(bad));
-        if (bad >= 0) CHK(rollback(bad));
+        std::vector<int> stops;
+        bool rolled;
+        CHK(finish_oldest(stops));
+        CHK(apply_stops(stops, rolled));
         continue;
       }
       if (lc && s == collect_upto && !trained) {
-        // the logged sets' chunks must be finished before training
-        int bad;
-        CHK(drain(bad));
-        if (bad >= 0) {
-          CHK(rollback(bad));
+        // every logged set has been taken: once the stops are all known, train
+        // on the second stream after the slots' queued work (their label
+        // counts), while the next chunks run with deferred full tests; chunks
+        // launched after the training's event use its windows
+        if (p0_pending > 0) {
+          std::vector<int> stops;
+          bool rolled;
+          CHK(finish_oldest(stops));
+          CHK(apply_stops(stops, rolled));
           continue;
         }
-      }
-      if (lc && s == collect_upto && !trained) {
-        // all logged sets are done: train on the second stream while the
-        // next chunks run with deferred full tests; chunks launched after the
-        // training's event use its windows
         trained = true;
-        r.collect_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        r.collect_ms = now_ms();
         long long n = 0;
         std::vector<long long> base(std::max(1, collect_upto));
         std::vector<float> inv(std::max(1, collect_upto));
@@ -1378,11 +1475,15 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
           base[q] = n;
           n += meta[q].c;
           inv[q] = (!nan0[q] && label_perms > 0) ? (float)(1.0 / (2.0 * label_perms)) : 0.f;
-          r.tests_collect += tests_eval[q];
         }
         r.train_rows = n;
         if (n >= 64) {
           hipStream_t st2 = ctx->stream2;
+          for (int k = 0; k < 2; k++) {
+            HIPCHK(ctx, hipEventCreateWithFlags(&slot_ev[k], hipEventDisableTiming));
+            HIPCHK(ctx, hipEventRecord(slot_ev[k], ctx->slot[k].st));
+            HIPCHK(ctx, hipStreamWaitEvent(st2, slot_ev[k], 0));
+          }
           GROW(b_my, sizeof(float) * n);
           GROW(b_mlbase, sizeof(long long) * collect_upto);
           GROW(b_mlinv, sizeof(float) * collect_upto);
@@ -1408,6 +1509,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
           HIPCHK(ctx, hipEventRecord(ctx->train_ev, st2));
           training = true;
           r.trained = 1;
+          budget = budget_min;
         }
       }
       // next set's items
@@ -1415,24 +1517,22 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       while (prod.ready.load(std::memory_order_acquire) <= s) std::this_thread::yield();
       auto hb = std::chrono::steady_clock::now();
       ctx->hp.wait += std::chrono::duration<double, std::milli>(hb - hw).count();
-      if (meta[s].G < 2) {
-        nan0[s] = 1;
-      } else {
-        add_item(ch, meta, s, 0, truebits, N);
-        if (!prod.stop0[s]) {
-          const std::vector<uint32_t> &L = prod.lab[s];
-          for (int pm = 1; pm < perms; pm++) {
-            std::copy(L.begin() + (size_t)(pm - 1) * words, L.begin() + (size_t)pm * words, one.begin());
-            add_item(ch, meta, s, pm, one, N);
-          }
+      if (!nan0[s]) {
+        const std::vector<uint32_t> &L = prod.lab[s];
+        for (int pm = 1; pm < perms; pm++) {
+          std::copy(L.begin() + (size_t)(pm - 1) * words, L.begin() + (size_t)pm * words, one.begin());
+          add_item(ch, meta, s, pm, one, N);
         }
-        std::vector<uint32_t>().swap(prod.lab[s]);
       }
+      std::vector<uint32_t>().swap(prod.lab[s]);
       s++;
       ctx->hp.build += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hb).count();
     }
     prod.stop();
     ctx->timing.host_stream_ms += prod.ms;
+    for (int k = 0; k < 2; k++)
+      if (slot_ev[k]) hipEventDestroy(slot_ev[k]);
+    for (int q = 0; q < collect_upto; q++) r.tests_collect += tests_eval[q];
     if (training) {
       HIPCHK(ctx, hipStreamSynchronize(ctx->stream2));
       float lv[3];
@@ -1445,7 +1545,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     }
     if (tr0) hipEventDestroy(tr0);
     if (tr1) hipEventDestroy(tr1);
-    for (int q = 0; q < nsets; q++) ran[q] = (nan0[q] || prod.stop0[q]) ? 1 : perms;
+    for (int q = 0; q < nsets; q++) ran[q] = nan0[q] ? 1 : perms;
   } else {
     // plain: one set and one permutation at a time, like main.cu:538-1460
     std::vector<uint32_t> c2((N + 31) / 32);

@@ -1297,9 +1297,6 @@ __global__ void k_gather(const uint8_t *x, int N, int NP, const int *genes, cons
   for (int i = threadIdx.x; i < NP / 16; i += blockDim.x) dst[i] = src[i];
 }
 
-// Learned window per (set, row): the first column at which the expected number
-// of full-test edges (the network's scores, summed in column order) reaches
-// mp + kappa; the whole row when it never does.  One block per set.
 // permutation-0 captures: one workgroup per slice at a time, 4-byte words
 // when source, destination and length allow it, else bytes
 __global__ __launch_bounds__(256) void k_copy_slices(const Slice *sl, int n) {
@@ -1318,6 +1315,15 @@ __global__ __launch_bounds__(256) void k_copy_slices(const Slice *sl, int n) {
   }
 }
 
+// permutation-0 JS of sets first..first+n-1 (one value per row of js)
+__global__ void k_js0(const double *js, int perms, int first, int n, double *out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = js[(size_t)(first + i) * perms];
+}
+
+// Learned window per (set, row): the first column at which the expected number
+// of full-test edges (the network's scores, summed in column order) reaches
+// mp + kappa; the whole row when it never does.  One block per set.
 __global__ void k_window(const SetMeta *sets, int nsets, const float *score, int mp, float kappa, int *win) {
   const int s = blockIdx.x;
   if (s >= nsets) return;
