@@ -21,6 +21,7 @@
 #include "eddy.h"
 #include "engine_types.h"
 #include "mlp.h"
+#include "perm_index.h"
 #include "perm_stream.h"
 
 // kernels (kernels.hip)
@@ -278,6 +279,22 @@ extern "C" int eddy_stream_permutation(eddy_ctx *ctx, int samples, int n1, int p
   ctx->stream_gen.class2_bits(samples, n1, c2.data());
   if (perm > 0)
     for (int i = 0; i < samples; i++) labels[i] = (uint8_t)((c2[i >> 5] >> (i & 31)) & 1);  // main.cu:561-570
+  return EDDY_OK;
+}
+
+// worker threads for the stream index: half the CPUs, 2..8
+static int index_workers() {
+  const int hw = (int)std::thread::hardware_concurrency();
+  return std::max(2, std::min(8, hw / 2));
+}
+
+extern "C" int eddy_stream_class2_batch(eddy_ctx *ctx, int samples, int n1, int count, uint32_t *c2) {
+  if (!ctx || samples < 1 || n1 < 0 || n1 > samples || count < 0 || (count && !c2)) return EDDY_ERR_ARG;
+  if (samples > EDDY_MAX_N) return EDDY_ERR_ARG;
+  const int words = (samples + 31) / 32;
+  PermIndex idx(ctx->stream_gen.save(), samples, n1, index_workers());
+  for (int k = 0; k < count; k++) idx.class2_bits(c2 + (size_t)k * words);
+  ctx->stream_gen.restore(idx.save());
   return EDDY_OK;
 }
 
@@ -1022,16 +1039,18 @@ struct Producer {
   std::mutex mu;
   int pos = 0;  // sets whose stop0 the thread has read (guarded by mu)
   std::thread th;
+  std::unique_ptr<PermIndex> idx;  // the stream while the thread runs
   double ms = 0;
 
   void start(int from) {
     quit.store(false);
     ready.store(from);
     pos = from;
+    idx.reset(new PermIndex(ctx->stream_gen.save(), N, n1, index_workers()));
     th = std::thread([this, from]() {
       auto h0 = std::chrono::steady_clock::now();
       const int words = (N + 31) / 32;
-      PermStream &ps = ctx->stream_gen;
+      PermIndex &ps = *idx;
       for (int s = from; s < nsets; s++) {
         if (quit.load(std::memory_order_relaxed)) break;
         bool stop;
@@ -1040,21 +1059,26 @@ struct Producer {
           stop = stop0[s] != 0;
           pos = s + 1;
         }
-        ps.class2_bits(N, n1, nullptr);  // permutation 0's draws
+        ps.class2_bits(nullptr);  // permutation 0's draws
         after0[s] = ps.save();
         if (!stop) {
           std::vector<uint32_t> &L = lab[s];
           L.resize((size_t)(perms - 1) * words);
-          for (int pm = 1; pm < perms; pm++) ps.class2_bits(N, n1, &L[(size_t)(pm - 1) * words]);
+          for (int pm = 1; pm < perms; pm++) ps.class2_bits(&L[(size_t)(pm - 1) * words]);
         }
         ready.store(s + 1, std::memory_order_release);
       }
       ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
     });
   }
+  // stops the thread; the context's stream continues where it stopped
   void stop() {
     quit.store(true);
     if (th.joinable()) th.join();
+    if (idx) {
+      ctx->stream_gen.restore(idx->save());
+      idx.reset();
+    }
   }
   // Marks sets `qs` as stopping after permutation 0; returns the first of
   // them whose flag the thread has already read (its later draws went to the

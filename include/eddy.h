@@ -126,6 +126,12 @@ int eddy_stream_draw(eddy_ctx *ctx, int n, int32_t *out);
 /* One permutation's draws (main.cu:541-570): for perm > 0 labels[samples]
  * receives the relabelling (0 = class 1); for perm 0 labels is untouched. */
 int eddy_stream_permutation(eddy_ctx *ctx, int samples, int n1, int perm, uint8_t *labels);
+/* The next `count` permutations' draws at once, as `count` calls of
+ * eddy_stream_permutation would make them, through the stream index that
+ * eddy_run_sets uses (worker threads scan the stream ahead).  c2 receives the
+ * class-2 bits (1 = class 2) of each permutation: count * ceil(samples / 32)
+ * words, sample i at bit i % 32 of word i / 32. */
+int eddy_stream_class2_batch(eddy_ctx *ctx, int samples, int n1, int count, uint32_t *c2);
 
 /* ---- the hot path ----------------------------------------------------------
  * Runs every set's permutation loop (main.cu:538-1460) and p-value

@@ -102,3 +102,33 @@ def test_stream_permutations_match_oracle():
             assert np.array_equal(lab_e, lab_o.astype(np.uint8)), (N, perm)
             assert (lab_e == 0).sum() == n1
     L.eddy_ctx_destroy(h)
+
+
+def test_stream_index_matches_stream():
+    """The stream index (worker threads, block-wise scan) gives the same
+    class-2 bits and leaves the stream at the same place as one permutation at
+    a time, across many superblocks and for odd, even, small and maximal n."""
+    L = eddy_amd.lib()
+    for N, n1, count in ((600, 300, 3000), (601, 17, 200), (2046, 1023, 40), (64, 63, 500), (65, 1, 500),
+                         (2, 1, 2000), (1200, 300, 50)):
+        ha, hb = C.c_void_p(), C.c_void_p()
+        assert L.eddy_ctx_create(-1, C.byref(ha)) == 0 and L.eddy_ctx_create(-1, C.byref(hb)) == 0
+        words = (N + 31) // 32
+        # start both a little way into the stream
+        pre = np.zeros(777, np.int32)
+        for h in (ha, hb):
+            assert L.eddy_stream_draw(h, len(pre), pre.ctypes.data_as(C.c_void_p)) == 0
+        got = np.zeros((count, words), np.uint32)
+        assert L.eddy_stream_class2_batch(ha, N, n1, count, got.ctypes.data_as(C.c_void_p)) == 0
+        lab = np.zeros(N, np.uint8)
+        for k in range(count):
+            assert L.eddy_stream_permutation(hb, N, n1, 1, lab.ctypes.data_as(C.c_void_p)) == 0
+            bits = (got[k][np.arange(N) >> 5] >> (np.arange(N) & 31)) & 1
+            assert np.array_equal(bits.astype(np.uint8), lab), (N, n1, k)
+        # same position afterwards
+        a, b = np.zeros(100, np.int32), np.zeros(100, np.int32)
+        assert L.eddy_stream_draw(ha, 100, a.ctypes.data_as(C.c_void_p)) == 0
+        assert L.eddy_stream_draw(hb, 100, b.ctypes.data_as(C.c_void_p)) == 0
+        assert np.array_equal(a, b), (N, n1)
+        L.eddy_ctx_destroy(ha)
+        L.eddy_ctx_destroy(hb)
