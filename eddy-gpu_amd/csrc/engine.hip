@@ -138,7 +138,8 @@ struct eddy_ctx {
   // the learned job trains on a second stream while chunks keep running
   hipStream_t stream2 = nullptr;
   hipEvent_t train_ev = nullptr;
-  std::vector<int> train_order;  // host copy of the epochs' row orders (alive until the upload ran)
+  void *h_train = nullptr;  // pinned staging of the learned job's uploads (read by stream2)
+  size_t h_train_cap = 0;
   // learned proposal: network parameters + Adam state, per-run feature,
   // label and window arrays (device pointers handed to the kernels)
   DevBuf b_mlp, b_mlpm, b_mlpv, b_mlpg, b_mlploss, b_feat, b_labhits, b_score, b_win, b_mx, b_my, b_mxb, b_myb,
@@ -253,6 +254,7 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
   if (ctx->ev0) hipEventDestroy(ctx->ev0);
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
   if (ctx->train_ev) hipEventDestroy(ctx->train_ev);
+  if (ctx->h_train) hipHostFree(ctx->h_train);
   if (ctx->stream2) hipStreamDestroy(ctx->stream2);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -1017,6 +1019,8 @@ static int mlp_train_dev(eddy_ctx *ctx, const float *dX, const float *dY, long l
                          int epochs, int batch, float lr, uint64_t seed, hipStream_t st);
 static int mlp_grad_dev(eddy_ctx *ctx, const float *dX, const float *dY, const int *dIdx, int b0, int n,
                         float *dLoss, hipStream_t st);
+static void mlp_initial_params(uint64_t seed, float *P);
+static int mlp_init_async(eddy_ctx *ctx, const float *P_pinned, hipStream_t st);
 
 namespace {
 
@@ -1307,13 +1311,14 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       Slot &sl = ctx->slot[next_slot];
       sl.first = first;
       sl.last = last;
+      const double tl = trace ? now_ms() : 0.0;
       CHK(start_chunk(ctx, sl, meta, std::move(c), p, perms));
       inflight.push_back(next_slot);
       slot_p0[next_slot] = p0;
       if (p0) p0_pending++;
       if (trace)
-        fprintf(stderr, "[eddy] t=%.1f launch slot %d sets %d..%d pairs %lld%s (replay at %d)\n", now_ms(), next_slot,
-                first, last, sl.ch.pairs, p0 ? " permutation 0" : "", prod.ready.load());
+        fprintf(stderr, "[eddy] t=%.1f launch slot %d sets %d..%d pairs %lld%s (%.1f ms, replay at %d)\n", now_ms(),
+                next_slot, first, last, sl.ch.pairs, p0 ? " permutation 0" : "", now_ms() - tl, prod.ready.load());
       next_slot ^= 1;
       return EDDY_OK;
     };
@@ -1502,18 +1507,38 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
         }
         r.train_rows = n;
         if (n >= 64) {
+          // everything below is queued on stream2 and nothing waits for it:
+          // the small uploads go through pinned staging (stream2 finished
+          // its previous use at the end of the previous job), and the
+          // default stream is not touched (it may share a hardware queue
+          // with a slot's stream)
           hipStream_t st2 = ctx->stream2;
+          const size_t need = (sizeof(long long) + sizeof(float)) * collect_upto + sizeof(float) * (MLP_NPARAMS + 1);
+          if (ctx->h_train_cap < need) {
+            if (ctx->h_train) HIPCHK(ctx, hipHostFree(ctx->h_train));
+            ctx->h_train = nullptr;
+            ctx->h_train_cap = 0;
+            HIPCHK(ctx, hipHostMalloc(&ctx->h_train, need, hipHostMallocDefault));
+            ctx->h_train_cap = need;
+          }
+          float *hP = (float *)ctx->h_train;
+          long long *hbase = (long long *)(hP + MLP_NPARAMS + (MLP_NPARAMS & 1));
+          float *hinv = (float *)(hbase + collect_upto);
+          std::copy(base.begin(), base.begin() + collect_upto, hbase);
+          std::copy(inv.begin(), inv.begin() + collect_upto, hinv);
+          mlp_initial_params(lc->seed, hP);
+          GROW(b_my, sizeof(float) * n);
+          GROW(b_mlbase, sizeof(long long) * collect_upto);
+          GROW(b_mlinv, sizeof(float) * collect_upto);
+          HIPCHK(ctx, hipMemcpyAsync(ctx->b_mlbase.p, hbase, sizeof(long long) * collect_upto, hipMemcpyHostToDevice,
+                                     st2));
+          HIPCHK(ctx, hipMemcpyAsync(ctx->b_mlinv.p, hinv, sizeof(float) * collect_upto, hipMemcpyHostToDevice, st2));
+          CHK(mlp_init_async(ctx, hP, st2));
           for (int k = 0; k < 2; k++) {
             HIPCHK(ctx, hipEventCreateWithFlags(&slot_ev[k], hipEventDisableTiming));
             HIPCHK(ctx, hipEventRecord(slot_ev[k], ctx->slot[k].st));
             HIPCHK(ctx, hipStreamWaitEvent(st2, slot_ev[k], 0));
           }
-          GROW(b_my, sizeof(float) * n);
-          GROW(b_mlbase, sizeof(long long) * collect_upto);
-          GROW(b_mlinv, sizeof(float) * collect_upto);
-          HIPCHK(ctx, hipMemcpy(ctx->b_mlbase.p, base.data(), sizeof(long long) * collect_upto, hipMemcpyHostToDevice));
-          HIPCHK(ctx, hipMemcpy(ctx->b_mlinv.p, inv.data(), sizeof(float) * collect_upto, hipMemcpyHostToDevice));
-          CHK(eddy_mlp_init(ctx, lc->seed));
           hipEventCreate(&tr0);
           hipEventCreate(&tr1);
           hipEventRecord(tr0, st2);
@@ -1534,6 +1559,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
           training = true;
           r.trained = 1;
           budget = budget_min;
+          if (trace) fprintf(stderr, "[eddy] t=%.1f training queued (%.1f ms)\n", now_ms(), now_ms() - r.collect_ms);
         }
       }
       // next set's items
@@ -1947,24 +1973,37 @@ static int mlp_alloc(eddy_ctx *ctx) {
   return EDDY_OK;
 }
 
-extern "C" int eddy_mlp_init(eddy_ctx *ctx, uint64_t seed) {
-  if (!ctx || ctx->device < 0) return ctx ? EDDY_ERR_NODEV : EDDY_ERR_ARG;
-  hipSetDevice(ctx->device);
-  CHK(mlp_alloc(ctx));
-  // He initialisation (normal, variance 2/fan_in), zero biases; Box-Muller on
-  // a seeded mt19937_64 so the initial network is reproducible
+// He initialisation (normal, variance 2/fan_in), zero biases; Box-Muller on a
+// seeded mt19937_64 so the initial network is reproducible
+static void mlp_initial_params(uint64_t seed, float *P) {
   std::mt19937_64 rng(seed);
   auto u01 = [&]() { return (double)(rng() >> 11) * 0x1p-53; };
   auto normal = [&]() { return std::sqrt(-2.0 * std::log(1.0 - u01())) * std::cos(6.283185307179586 * u01()); };
-  std::vector<float> P(MLP_NPARAMS, 0.f);
+  for (int i = 0; i < MLP_NPARAMS; i++) P[i] = 0.f;
   for (int i = 0; i < MLP_H * MLP_IN; i++) P[MLP_OFF_W1 + i] = (float)(normal() * std::sqrt(2.0 / MLP_IN));
   for (int i = 0; i < MLP_H * MLP_H; i++) P[MLP_OFF_W2 + i] = (float)(normal() * std::sqrt(2.0 / MLP_H));
   for (int i = 0; i < MLP_H; i++) P[MLP_OFF_W3 + i] = (float)(normal() * std::sqrt(1.0 / MLP_H));
-  HIPCHK(ctx, hipMemcpy(ctx->b_mlp.p, P.data(), sizeof(float) * MLP_NPARAMS, hipMemcpyHostToDevice));
-  HIPCHK(ctx, hipMemset(ctx->b_mlpm.p, 0, sizeof(float) * MLP_NPARAMS));
-  HIPCHK(ctx, hipMemset(ctx->b_mlpv.p, 0, sizeof(float) * MLP_NPARAMS));
+}
+
+// Initial network and cleared Adam moments, copied from `P_pinned` (host
+// memory that stays untouched until the stream has run the copy) on `st`.
+static int mlp_init_async(eddy_ctx *ctx, const float *P_pinned, hipStream_t st) {
+  CHK(mlp_alloc(ctx));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->b_mlp.p, P_pinned, sizeof(float) * MLP_NPARAMS, hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, hipMemsetAsync(ctx->b_mlpm.p, 0, sizeof(float) * MLP_NPARAMS, st));
+  HIPCHK(ctx, hipMemsetAsync(ctx->b_mlpv.p, 0, sizeof(float) * MLP_NPARAMS, st));
   ctx->mlp_step = 0;
   ctx->mlp_ready = true;
+  return EDDY_OK;
+}
+
+extern "C" int eddy_mlp_init(eddy_ctx *ctx, uint64_t seed) {
+  if (!ctx || ctx->device < 0) return ctx ? EDDY_ERR_NODEV : EDDY_ERR_ARG;
+  hipSetDevice(ctx->device);
+  std::vector<float> P(MLP_NPARAMS);
+  mlp_initial_params(seed, P.data());
+  CHK(mlp_init_async(ctx, P.data(), ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return EDDY_OK;
 }
 
@@ -2045,30 +2084,22 @@ static int mlp_eval_loss(eddy_ctx *ctx, const float *dX, const float *dY, long l
 }
 
 // Adam over shuffled mini-batches of device rows dX/dY: all n rows, or a
-// uniform sample of max_rows of them (max_rows > 0); the last partial batch
-// of each epoch is dropped.  Every epoch's order is drawn on the host and
-// uploaded at once; nothing here waits for the stream.
+// pseudo-random sample of max_rows of them (max_rows > 0); the last partial
+// batch of each epoch is dropped.  The epochs' orders are computed on the
+// stream by keyed bijections; nothing here waits for the stream.
 static int mlp_train_dev(eddy_ctx *ctx, const float *dX, const float *dY, long long n, long long max_rows,
                          int epochs, int batch, float lr, uint64_t seed, hipStream_t st) {
-  std::mt19937_64 rng(seed ^ 0xADA3ULL);
-  std::vector<int> rows(n);
-  for (long long i = 0; i < n; i++) rows[i] = (int)i;
-  if (max_rows > 0 && max_rows < n) {  // partial Fisher-Yates: a uniform sample
-    for (long long i = 0; i < max_rows; i++) std::swap(rows[i], rows[i + (long long)(rng() % (uint64_t)(n - i))]);
-    rows.resize(max_rows);
-    n = max_rows;
-  }
-  batch = (int)std::min<long long>(batch, n);
   if (epochs <= 0 || batch <= 0) return EDDY_OK;
+  if (epochs > MLP_MAX_EPOCHS || n > (1LL << 30)) {
+    ctx->err = "training: at most 64 epochs over at most 2^30 rows";
+    return EDDY_ERR_ARG;
+  }
+  const long long n_all = n;
+  if (max_rows > 0 && max_rows < n) n = max_rows;  // a pseudo-random sample of the rows
+  batch = (int)std::min<long long>(batch, n);
   GROW(b_mscr, sizeof(float) * mlp_part_floats(batch));
   GROW(b_midx, sizeof(int) * n * epochs);
-  std::vector<int> &ord = ctx->train_order;
-  ord.resize((size_t)n * epochs);
-  for (int ep = 0; ep < epochs; ep++) {
-    for (long long i = n - 1; i > 0; i--) std::swap(rows[i], rows[rng() % (uint64_t)(i + 1)]);
-    std::copy(rows.begin(), rows.end(), ord.begin() + (size_t)ep * n);
-  }
-  HIPCHK(ctx, hipMemcpyAsync(ctx->b_midx.p, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, mlp_order(seed, n_all, n, epochs, (int *)ctx->b_midx.p, st));
   for (int ep = 0; ep < epochs; ep++)
     for (long long b0 = 0; b0 + batch <= n; b0 += batch) {
       ctx->mlp_step++;
@@ -2077,6 +2108,11 @@ static int mlp_train_dev(eddy_ctx *ctx, const float *dX, const float *dY, long l
                                  (float *)ctx->b_mlpm.p, (float *)ctx->b_mlpv.p, lr, ctx->mlp_step, st));
     }
   return EDDY_OK;
+}
+
+extern "C" int eddy_mlp_order_host(uint64_t seed, long long n, long long m, int epochs, int *out) {
+  if (!out) return EDDY_ERR_ARG;
+  return mlp_order_host(seed, n, m, epochs, out) ? EDDY_OK : EDDY_ERR_ARG;
 }
 
 extern "C" int eddy_mlp_train(eddy_ctx *ctx, const float *X, const float *Y, long long n, int epochs, int batch,

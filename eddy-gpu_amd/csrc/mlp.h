@@ -13,6 +13,12 @@
 #define MLP_OFF_B3 (MLP_OFF_W3 + MLP_H)
 #define MLP_NPARAMS (MLP_OFF_B3 + 1)
 
+#define MLP_MAX_EPOCHS 64
+struct MlpOrderKeys {
+  uint32_t sample[4], epoch[4 * MLP_MAX_EPOCHS];
+  int half_n, half_m;
+};
+
 // floats of block-partial scratch a batch of n samples needs
 size_t mlp_part_floats(int n);
 // out[i] = sigmoid(net(X[i]))
@@ -23,6 +29,12 @@ hipError_t mlp_gradient(const float *dX, const float *dY, const int *dIdx, int b
 // the same gradient followed by one Adam step (beta 0.9 / 0.999, eps 1e-8)
 hipError_t mlp_train_step(const float *dX, const float *dY, const int *dIdx, int b0, int n, float *dP, float *dPart,
                           float *dGrad, float *dLoss, float *dM, float *dV, float lr, int step, hipStream_t st);
+// Training order of `epochs` epochs over a sample of m of the n rows (keyed
+// by seed): out[e * m + i] = the i-th row epoch e visits.  n <= 2^30,
+// epochs <= MLP_MAX_EPOCHS.
+hipError_t mlp_order(uint64_t seed, long long n, long long m, int epochs, int *dOut, hipStream_t st);
+// the same order computed on the host (false for arguments mlp_order refuses)
+bool mlp_order_host(uint64_t seed, long long n, long long m, int epochs, int *out);
 // Y[i] = hits[i] * inv_den[set of i], sets starting at setbase[] (ascending)
 hipError_t mlp_labels(const unsigned *dHits, const float *dInvDen, const long long *dSetBase, int nsets, long long n,
                       float *dY, hipStream_t st);

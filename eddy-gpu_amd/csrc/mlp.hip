@@ -190,7 +190,86 @@ __global__ void k_mlp_labels(const unsigned *hits, const float *inv_den, const l
   Y[i] = (float)hits[i] * inv_den[lo];
 }
 
+// Training order.  Epoch e visits the m sampled rows in the order
+// perm_e(0), perm_e(1), ..., and sampled row j is row perm_s(j) of the n
+// logged rows, where perm_s (over [0, n)) and perm_e (over [0, m)) are keyed
+// pseudo-random bijections: a four-round balanced Feistel network on the
+// smallest even bit width covering the range, with cycle walking back into
+// the range (Black & Rogaway, "Ciphers with arbitrary finite domains", 2002).
+// The round function is MurmurHash3's 32-bit finaliser of the half and the
+// round key.
+__host__ __device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+__host__ __device__ uint32_t feistel_perm(uint32_t x, uint32_t range, int half, const uint32_t *key) {
+  const uint32_t mask = (1u << half) - 1u;
+  do {
+    uint32_t l = x >> half, r = x & mask;
+    for (int k = 0; k < 4; k++) {
+      const uint32_t t = l ^ (fmix32(r ^ key[k]) & mask);
+      l = r;
+      r = t;
+    }
+    x = (l << half) | r;
+  } while (x >= range);
+  return x;
+}
+__global__ void k_mlp_order(MlpOrderKeys keys, uint32_t n, uint32_t m, int epochs, int *out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)m * epochs) return;
+  const int e = (int)(i / m);
+  const uint32_t j = feistel_perm((uint32_t)(i - (long long)e * m), m, keys.half_m, keys.epoch + 4 * e);
+  out[i] = (int)feistel_perm(j, n, keys.half_n, keys.sample);
+}
+
 // ---- host helpers (device pointers; stream-ordered) -------------------------
+static int half_bits(uint32_t range) {
+  int b = 2;
+  while (b < 32 && ((uint64_t)1 << b) < range) b += 2;
+  return b / 2;
+}
+static uint64_t splitmix64(uint64_t &s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+static bool order_keys(uint64_t seed, long long n, long long m, int epochs, MlpOrderKeys &k) {
+  if (n <= 0 || m <= 0 || m > n || epochs <= 0 || epochs > MLP_MAX_EPOCHS || n > (1LL << 30) ||
+      m * epochs > 0x7fffffffLL)
+    return false;
+  uint64_t s = seed ^ 0xADA3ULL;
+  for (int i = 0; i < 4; i++) k.sample[i] = (uint32_t)splitmix64(s);
+  for (int i = 0; i < 4 * epochs; i++) k.epoch[i] = (uint32_t)splitmix64(s);
+  k.half_n = half_bits((uint32_t)n);
+  k.half_m = half_bits((uint32_t)m);
+  return true;
+}
+
+hipError_t mlp_order(uint64_t seed, long long n, long long m, int epochs, int *dOut, hipStream_t st) {
+  MlpOrderKeys k;
+  if (!order_keys(seed, n, m, epochs, k)) return hipErrorInvalidValue;
+  const long long total = m * epochs;
+  k_mlp_order<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(k, (uint32_t)n, (uint32_t)m, epochs, dOut);
+  return hipGetLastError();
+}
+
+bool mlp_order_host(uint64_t seed, long long n, long long m, int epochs, int *out) {
+  MlpOrderKeys k;
+  if (!order_keys(seed, n, m, epochs, k)) return false;
+  for (long long i = 0; i < m * epochs; i++) {
+    const int e = (int)(i / m);
+    const uint32_t j = feistel_perm((uint32_t)(i - (long long)e * m), (uint32_t)m, k.half_m, k.epoch + 4 * e);
+    out[i] = (int)feistel_perm(j, (uint32_t)n, k.half_n, k.sample);
+  }
+  return true;
+}
 size_t mlp_part_floats(int n) { return (size_t)((n + MB - 1) / MB) * (MLP_NPARAMS + 1); }
 
 hipError_t mlp_forward(const float *dX, int n, const float *dP, float *dOut, hipStream_t st) {

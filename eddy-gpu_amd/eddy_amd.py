@@ -28,6 +28,7 @@ EXPORTS = [
     "eddy_mlp_init", "eddy_mlp_get_params", "eddy_mlp_set_params", "eddy_mlp_set_margin", "eddy_mlp_forward",
     "eddy_mlp_gradient", "eddy_mlp_train", "eddy_learn_collect", "eddy_run_learned",
     "eddy_edge_loo_mask_host", "eddy_chi2_host", "eddy_stream_class2_batch",
+    "eddy_mlp_order_host",
 ]
 MLP_FEATURES = 12
 MLP_NPARAMS = 5057
@@ -124,6 +125,7 @@ def lib():
                                       C.c_longlong]
         L.eddy_stream_permutation.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
         L.eddy_stream_class2_batch.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        L.eddy_mlp_order_host.argtypes = [C.c_uint64, C.c_longlong, C.c_longlong, C.c_int, C.c_void_p]
         L.eddy_edge_decision_host.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(Params)]
         L.eddy_edge_loo_mask_host.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(Params)]
         L.eddy_chi2_host.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]

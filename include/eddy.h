@@ -223,6 +223,10 @@ int eddy_edge_loo_mask_host(const int *tally9, int c, int prior, int full, const
  * kernel1.cu:121-203) or the kernels' reciprocal form (fast = 1); host build,
  * for tests. */
 int eddy_chi2_host(const int *tally9, int fast, int *dof, double *chi2);
+/* The learned proposal's training order (no counterpart in the reference):
+ * out[e * m + i] = the i-th of the m sampled rows (of n) that epoch e visits,
+ * as the training kernels draw it for `seed`; host build, for tests. */
+int eddy_mlp_order_host(uint64_t seed, long long n, long long m, int epochs, int *out);
 
 /* ---- learned proposal ------------------------------------------------------
  * A 12 -> 64 -> 64 -> 1 network (ReLU, ReLU, sigmoid; FP32) scores each gene

@@ -132,3 +132,34 @@ def test_stream_index_matches_stream():
         assert np.array_equal(a, b), (N, n1)
         L.eddy_ctx_destroy(ha)
         L.eddy_ctx_destroy(hb)
+
+
+def test_mlp_training_order_is_a_sample_and_permutations():
+    """Each epoch visits the same m distinct sampled rows, each exactly once, in
+    a different order; the sample and the orders depend on the seed; with
+    m = n every row is visited."""
+    L = eddy_amd.lib()
+    for n, m, epochs in ((614123, 131072, 2), (1000, 1000, 3), (5, 3, 4), (1, 1, 1), (70000, 65536, 1)):
+        out = np.zeros(m * epochs, np.int32)
+        assert L.eddy_mlp_order_host(1, n, m, epochs, out.ctypes.data_as(C.c_void_p)) == 0
+        ep = out.reshape(epochs, m)
+        sample = np.sort(ep[0])
+        assert sample[0] >= 0 and sample[-1] < n and len(np.unique(sample)) == m
+        for e in range(1, epochs):
+            assert np.array_equal(np.sort(ep[e]), sample)
+            if m > 8:
+                assert not np.array_equal(ep[e], ep[0])
+        if m == n:
+            assert np.array_equal(sample, np.arange(n))
+        other = np.zeros_like(out)
+        assert L.eddy_mlp_order_host(2, n, m, epochs, other.ctypes.data_as(C.c_void_p)) == 0
+        if m > 8:
+            assert not np.array_equal(other, out)
+    # the sample of 131072 of 614123 rows is spread evenly over the rows
+    out = np.zeros(131072, np.int32)
+    assert L.eddy_mlp_order_host(7, 614123, 131072, 1, out.ctypes.data_as(C.c_void_p)) == 0
+    hist = np.bincount(out // 61413, minlength=10)[:10]
+    assert hist.min() > 0.9 * 131072 / 10
+    # refused arguments
+    assert L.eddy_mlp_order_host(1, 10, 11, 1, out.ctypes.data_as(C.c_void_p)) != 0
+    assert L.eddy_mlp_order_host(1, 10, 5, 65, out.ctypes.data_as(C.c_void_p)) != 0
