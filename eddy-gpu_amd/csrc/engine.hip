@@ -469,8 +469,9 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   const int nitems = (int)ch.items.size();
   int maxG = 0;
   for (const ItemMeta &im : ch.items) maxG = std::max(maxG, sets[im.set].G);
+  // network hash table of the score kernel: load factor at most 2/3
   int ht = 1024;
-  while (ht < 2 * S) ht *= 2;
+  while (2 * ht < 3 * S) ht *= 2;
   long long evoff = 0, foff = 0, fhoff = 0;
   for (int i = 0; i < nitems; i++) {
     ItemMeta &im = ch.items[i];
@@ -607,7 +608,12 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   const size_t rows_lds = (size_t)S * 8 + (size_t)2 * EDDY_NWAVES * maxG * 4 + (size_t)N * 2 + (size_t)2 * maxG + 16;
   k_rows<<<nitems, 256, rows_lds, st>>>(a);
   hipEventRecord(sl.tev[3], st);
-  const size_t score_lds = (size_t)EDDY_NWAVES * 12 * bitw * 8 + (size_t)ht * 12 + (size_t)(4 * S + 1 + 2 * maxG) * 4;
+  // the score kernel's phase-shared region: the network hash table, then the
+  // BDeu mask stage, then the base family scores (at least ht * 12 bytes)
+  const size_t score_shared = std::max<size_t>({(size_t)ht * 12, (size_t)EDDY_NWAVES * 12 * bitw * 8,
+                                                 (size_t)4 * maxG * 8});
+  a.score_shared = (int)score_shared;
+  const size_t score_lds = score_shared + (size_t)(4 * S + 1 + 2 * maxG) * 4;
   k_score<<<nitems, 256, score_lds, st>>>(a);
   hipEventRecord(sl.tev[4], st);
   HIPCHK(ctx, hipGetLastError());

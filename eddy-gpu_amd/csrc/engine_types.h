@@ -77,7 +77,8 @@ struct KArgs {
   int N, n1, n2, S;
   int mp, p_adjust, plain;
   int feat_only;           // count kernel: pooled-table features only (no tests, no outputs)
-  int ht_size;             // network hash table entries (power of two >= 2S)
+  int ht_size;             // network hash table entries (power of two >= 1.5 S)
+  int score_shared;        // bytes of the score kernel's phase-shared LDS region
   eddy_edge_params ep;     // thresholds, lgamma(p+1) constants, underflow-cut interval
   const double *lgtab;     // [82][lgstride]: lgamma(n + 4/d)
   int lgstride;

@@ -825,10 +825,12 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   const int HT = a.ht_size;
 
   extern __shared__ __align__(16) unsigned char dyn[];
-  // [4 waves][12 * bitw] u64 BDeu mask stage, then the network tables
-  unsigned long long *hkeys = (unsigned long long *)dyn + (size_t)EDDY_NWAVES * 12 * a.bitw;  // [HT]
+  // a region shared by phases (the network hash table in phase 1, the BDeu
+  // mask stage in phase 4, the base family scores in phase 5, the KL terms
+  // in phase 6), then the network tables
+  unsigned long long *hkeys = (unsigned long long *)dyn;  // [HT]
   int *hvals = (int *)(hkeys + HT);                       // [HT]
-  int *rep = hvals + HT;                                  // [S]
+  int *rep = (int *)(dyn + a.score_shared);               // [S]
   int *evst = rep + S;                                    // [S + 1]
   int *work = evst + S + 1;                               // [S]: scratch counts
   int *ulist = work + S;                                  // [S]: unique networks in order
@@ -1041,7 +1043,7 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   // staged in LDS, one histogram cell per lane, the inner sums of the
   // reference's double loop per lane (one lane per parent configuration j),
   // the outer sum over j by lane 0, both in the reference's order
-  uint64_t *mst = (uint64_t *)dyn + (size_t)wave * 12 * a.bitw;  // [4 genes][3 states][W]
+  uint64_t *mst = (uint64_t *)dyn + (size_t)wave * 12 * a.bitw;  // [4 genes][3 states][W] (hash table is dead)
   unsigned long long st_hist = 0, st_bins = 0;                    // algorithmic bytes
   // (a) families with at most one parent (at most 9 cells): one lane each,
   // the same j-outer / k-inner order
@@ -1162,18 +1164,33 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   __syncthreads();
   phase_mark(a, 4, tph);
   // ---- 5. network scores, in node order (main.cu:1335-1342) -------------
+  // base[(gs * 2 + g) * G + t]: the score in class g of row t's family in
+  // the full network of class gs (every network's value outside its event
+  // rows), staged in LDS; event rows read their own family's score
+  double *base = (double *)dyn;
+  for (int i = tid; i < 4 * G; i += 256) {
+    const int q = i / G, t = i - q * G, gs = q >> 1, g = q & 1;
+    const int f = famfull[gs * G + t];
+    base[i] = f >= 0 ? fscore[f * 2 + g] : eddy_nan();
+  }
+  __syncthreads();
   double *dist = a.lval + im.soff * 2;  // [g*S + u] (overwritten by the likelihoods below)
   for (int ug = tid; ug < 2 * UU; ug += 256) {
     const int u = ug >> 1, g = ug & 1;
     const int s = ulist[u];
     const int b = evst[s], ne = evst[s + 1] - b;
-    const int gs = net_class(s, n1);
-    int ep = 0;
+    const double *bs = base + (net_class(s, n1) * 2 + g) * G;
+    int ep = 0, te = ne > 0 ? ev[es[b]].t : G;
     double d = 0.0;
     for (int t = 0; t < G; t++) {
-      int f = famfull[gs * G + t];
-      if (ep < ne && ev[es[b + ep]].t == t) f = a.evfam[im.evoff + es[b + ep++]];
-      d = d + (f >= 0 ? fscore[f * 2 + g] : eddy_nan());
+      double v = bs[t];
+      if (t == te) {
+        const int f = a.evfam[im.evoff + es[b + ep]];
+        v = f >= 0 ? fscore[f * 2 + g] : eddy_nan();
+        ep++;
+        te = ep < ne ? ev[es[b + ep]].t : G;
+      }
+      d = d + v;
     }
     dist[g * S + u] = d;
   }
