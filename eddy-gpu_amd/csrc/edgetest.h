@@ -202,54 +202,143 @@ EDDY_HD int eddy_edge_decision_fast(const int *t, int prior, const eddy_edge_par
   return eddy_edge_decision(t, prior, ep, eb, cpairs);
 }
 
-// Leave-one-out decisions of one tally: bit `cell` is the decision of t minus
-// one count in `cell`; empty cells copy `full`.  Thirteen reciprocals per
-// tally serve all nine leave-one-out tallies.
-EDDY_HD uint32_t eddy_loo_mask_fast(const int *t, int full, int prior, const eddy_edge_params *ep,
-                                    const eddy_edge_bounds *eb, double cpairs, int *ntests) {
-  int R[3], K[3];
+// 1/v and 1/(v - 1) from one division (0 for a zero denominator): with
+// inv = 1/(v (v - 1)) rounded once, (v - 1) inv and v inv are within 1.5 ulp
+// of the true reciprocals (v (v - 1) <= 2^22 is exact).
+EDDY_HD void eddy_recip_pair(int v, double *r, double *rm) {
+  if (v >= 2) {
+    const double inv = 1.0 / ((double)v * (double)(v - 1));
+    *r = (double)(v - 1) * inv;
+    *rm = (double)v * inv;
+  } else {
+    *r = v == 1 ? 1.0 : 0.0;
+    *rm = 0.0;
+  }
+}
+
+EDDY_HD double eddy_sq(int v) { return (double)(v * v); }
+
+// x = chi2 / 2 of the tally t minus one count in cell (a, b), t[a*3+b] > 0,
+// without Yates' correction, from the identity
+//   sum (O - E)^2 / E = n (sum O^2 / (R K) - 1)     (E = R K / n, empty rows
+//                                                    and columns left out):
+//   x = (n - 1) / 2 * (sum_i 1/R'_i sum_j O'_ij^2 / K'_j - 1).
+// T[i] = sum_{j != b} t_ij^2 / K_j and U[i] = T[i] + t_ib^2 / (K_b - 1) are
+// shared by the three cells of column b; row a takes 1/(R_a - 1) and
+// (t_ab - 1)^2.  Error: every product is within 5 ulp, the sum S of at most
+// nine nonnegative terms (S <= 3) within 12 ulp of itself, so
+// |x - x_exact| <= 0.5 (n - 1) 36 u + 2 u x < 4.1e-12 + 2.3e-16 x for
+// n <= 2046 (u = 2^-53): with the reference formula's own rounding this stays
+// far inside the 1e-9 (1 + x) guard of eddy_edge_certain.
+EDDY_HD double eddy_loo_half_chi2(const int *t, int n, int a, int b, const double *T, const double *U,
+                                  const double *iR, const double *iRm, const double *iKm) {
+  const double W = T[a] + eddy_sq(t[a * 3 + b] - 1) * iKm[b];
+  double S = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; i++) S = S + (i == a ? iRm[i] * W : iR[i] * U[i]);
+  return 0.5 * (double)(n - 1) * (S - 1.0);
+}
+
+// x and dof of the tally t minus one count in cell (a, b) (t[a*3+b] > 0,
+// n > 1), given t's margins R, K, their reciprocal pairs and column b's T, U:
+// the identity above, or, when an expected count of the reduced tally is
+// below 5 (Yates' rule, decided exactly in integers), the per-cell form.
+EDDY_HD double eddy_loo_x(const int *t, const int *R, const int *K, int n, int a, int b, const double *T,
+                          const double *U, const double *iR, const double *iRm, const double *iK,
+                          const double *iKm, int *dof) {
+  int R1[3], K1[3], zr = 0, zk = 0, mr = 1 << 30, mk = 1 << 30;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    R1[i] = R[i] - (i == a ? 1 : 0);
+    K1[i] = K[i] - (i == b ? 1 : 0);
+    zr += R1[i] == 0;
+    zk += K1[i] == 0;
+    if (R1[i] > 0 && R1[i] < mr) mr = R1[i];
+    if (K1[i] > 0 && K1[i] < mk) mk = K1[i];
+  }
+  *dof = (2 - zk) * (2 - zr);
+  if (mr * mk >= 5 * (n - 1)) return eddy_loo_half_chi2(t, n, a, b, T, U, iR, iRm, iKm);
+  int u[9], d;
+  double iR1[3], iK1[3];
+#pragma unroll
+  for (int i = 0; i < 9; i++) u[i] = t[i] - (i == a * 3 + b ? 1 : 0);
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    iR1[i] = i == a ? iRm[i] : iR[i];
+    iK1[i] = i == b ? iKm[i] : iK[i];
+  }
+  return eddy_half_chi2_fast(u, R1, K1, n - 1, iR1, iK1, 1.0 / (double)(n - 1), &d);
+}
+
+// margins of t, their reciprocal pairs, n
+EDDY_HD int eddy_loo_margins(const int *t, int *R, int *K, double *iR, double *iRm, double *iK, double *iKm) {
   for (int i = 0; i < 3; i++) {
     R[i] = t[i * 3] + t[i * 3 + 1] + t[i * 3 + 2];
     K[i] = t[i] + t[3 + i] + t[6 + i];
   }
-  const int n = R[0] + R[1] + R[2];
-  double iR[3], iK[3], iRm[3], iKm[3];
   for (int i = 0; i < 3; i++) {
-    iR[i] = eddy_recip_or0(R[i]);
-    iK[i] = eddy_recip_or0(K[i]);
-    iRm[i] = eddy_recip_or0(R[i] - 1);
-    iKm[i] = eddy_recip_or0(K[i] - 1);
+    eddy_recip_pair(R[i], &iR[i], &iRm[i]);
+    eddy_recip_pair(K[i], &iK[i], &iKm[i]);
   }
-  const double in1 = eddy_recip_or0(n - 1);
+  return R[0] + R[1] + R[2];
+}
+
+// T[i] = sum_{j != b} t_ij^2 / K_j and U[i] = T[i] + t_ib^2 / (K_b - 1)
+EDDY_HD void eddy_loo_column(const int *t, int b, const double *iK, const double *iKm, double *T, double *U) {
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      if (j != b) s = s + eddy_sq(t[i * 3 + j]) * iK[j];
+    T[i] = s;
+    U[i] = s + eddy_sq(t[i * 3 + b]) * iKm[b];
+  }
+}
+
+// Leave-one-out decisions of one tally: bit `cell` is the decision of t minus
+// one count in `cell`; empty cells copy `full`.  Six divisions per tally serve
+// all nine leave-one-out tallies.  A reduced tally under Yates' rule (rare:
+// an expected count below 5) or within a guard band of a threshold takes the
+// exact decision after the unrolled pass.
+EDDY_HD uint32_t eddy_loo_mask_fast(const int *t, int full, int prior, const eddy_edge_params *ep,
+                                    const eddy_edge_bounds *eb, double cpairs, int *ntests) {
+  int R[3], K[3];
+  double iR[3], iK[3], iRm[3], iKm[3];
+  const int n = eddy_loo_margins(t, R, K, iR, iRm, iK, iKm);
   uint32_t m = 0, unsure = 0;
   // unrolled: every array index below is a constant, so all tallies stay in
   // registers
 #pragma unroll
-  for (int cell = 0; cell < 9; cell++) {
-    if (t[cell] > 0) {
-      (*ntests)++;
-      int c = -1;
-      if (n > 1) {
-        const int a = cell / 3, b = cell % 3;
-        int u[9], R1[3], K1[3];
-        double iR1[3], iK1[3];
+  for (int b = 0; b < 3; b++) {
+    double T[3], U[3];
+    eddy_loo_column(t, b, iK, iKm, T, U);
 #pragma unroll
-        for (int i = 0; i < 9; i++) u[i] = t[i] - (i == cell ? 1 : 0);
+    for (int a = 0; a < 3; a++) {
+      const int cell = a * 3 + b;
+      if (t[cell] > 0) {
+        (*ntests)++;
+        int c = -1;
+        if (n > 1) {
+          int zr = 0, zk = 0, mr = 1 << 30, mk = 1 << 30;
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-          R1[i] = R[i] - (i == a ? 1 : 0);
-          K1[i] = K[i] - (i == b ? 1 : 0);
-          iR1[i] = i == a ? iRm[i] : iR[i];
-          iK1[i] = i == b ? iKm[i] : iK[i];
+          for (int i = 0; i < 3; i++) {
+            const int r1 = R[i] - (i == a ? 1 : 0), k1 = K[i] - (i == b ? 1 : 0);
+            zr += r1 == 0;
+            zk += k1 == 0;
+            if (r1 > 0 && r1 < mr) mr = r1;
+            if (k1 > 0 && k1 < mk) mk = k1;
+          }
+          if (mr * mk >= 5 * (n - 1)) {  // no Yates correction: the identity
+            const double x = eddy_loo_half_chi2(t, n, a, b, T, U, iR, iRm, iKm);
+            c = eddy_edge_certain(x, (2 - zk) * (2 - zr), prior, ep, eb);
+          }
         }
-        int dof;
-        const double x = eddy_half_chi2_fast(u, R1, K1, n - 1, iR1, iK1, in1, &dof);
-        c = eddy_edge_certain(x, dof, prior, ep, eb);
+        if (c >= 0) m |= (uint32_t)c << cell;
+        else unsure |= 1u << cell;
+      } else {
+        m |= (uint32_t)full << cell;
       }
-      if (c >= 0) m |= (uint32_t)c << cell;
-      else unsure |= 1u << cell;
-    } else {
-      m |= (uint32_t)full << cell;
     }
   }
   // tallies within a guard band of a threshold: the exact decision

@@ -1951,6 +1951,16 @@ extern "C" int eddy_chi2_host(const int *tally9, int fast, int *dof, double *chi
     *chi2 = eddy_chi2(tally9, dof);
     return EDDY_OK;
   }
+  if (fast >= 2 && fast <= 10) {  // the leave-one-out form for the tally minus one count in cell fast - 2
+    const int cell = fast - 2, a = cell / 3, b = cell % 3;
+    int R[3], K[3];
+    double iR[3], iK[3], iRm[3], iKm[3], T[3], U[3];
+    const int n = eddy_loo_margins(tally9, R, K, iR, iRm, iK, iKm);
+    if (tally9[cell] <= 0 || n < 2) return EDDY_ERR_ARG;
+    eddy_loo_column(tally9, b, iK, iKm, T, U);
+    *chi2 = 2.0 * eddy_loo_x(tally9, R, K, n, a, b, T, U, iR, iRm, iK, iKm, dof);
+    return EDDY_OK;
+  }
   int R[3], K[3];
   for (int i = 0; i < 3; i++) {
     R[i] = tally9[i * 3] + tally9[i * 3 + 1] + tally9[i * 3 + 2];

@@ -349,7 +349,7 @@ __device__ __forceinline__ void set_loo(const KArgs &a, const ItemMeta &im, long
 
 __device__ int block_exscan(int *v, int n, int *wsum);
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_loo(KArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_loo(KArgs a) {
   const int item = a.order[blockIdx.x];  // largest items first (shorter tail)
   const ItemMeta im = a.items[item];
   const SetMeta &sm = a.sets[im.set];  // read in place: the bounds are indexed per tally

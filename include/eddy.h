@@ -220,8 +220,10 @@ int eddy_edge_decision_host(const int *tally9, int c, int prior, const eddy_para
  * for tests. */
 int eddy_edge_loo_mask_host(const int *tally9, int c, int prior, int full, const eddy_params *p);
 /* Chi-square and dof of a 3x3 tally: the reference's formula (fast = 0,
- * kernel1.cu:121-203) or the kernels' reciprocal form (fast = 1); host build,
- * for tests. */
+ * kernel1.cu:121-203), the kernels' reciprocal form (fast = 1), or the
+ * kernels' leave-one-out form for the tally minus one count in cell k
+ * (fast = 2 + k, k = 0..8; the cell must be nonempty); host build, for
+ * tests. */
 int eddy_chi2_host(const int *tally9, int fast, int *dof, double *chi2);
 /* The learned proposal's training order (no counterpart in the reference):
  * out[e * m + i] = the i-th of the m sampled rows (of n) that epoch e visits,

@@ -128,6 +128,48 @@ def test_fast_chi2_error_bound():
     assert worst < 1e-10, worst
 
 
+def test_loo_chi2_error_bound():
+    """The leave-one-out form (the identity n (sum O^2 / (R K) - 1) with the
+    column partial sums shared by a column's three cells, or the per-cell
+    form under Yates' rule) stays within 1e-10 (1 + x) of the reference
+    formula on the reduced tally; dof is identical."""
+    rng = np.random.default_rng(23)
+    worst = 0.0
+    checked = 0
+    for t in _random_tables(rng, 12000):
+        if t.sum() < 2:
+            continue
+        for k in [int(i) for i in np.nonzero(t)[0]]:
+            u = t.copy()
+            u[k] -= 1
+            xe, de = engine_chi2(u, 0)
+            xl, dl = engine_chi2(t, 2 + k)
+            assert de == dl, (t, k)
+            worst = max(worst, abs(xl - xe) / 2 / (1 + xe / 2))
+            checked += 1
+    assert checked > 50000
+    assert worst < 1e-10, worst
+
+
+def test_loo_form_edge_cases():
+    # the identity drops the empty rows and columns; a reduced tally whose
+    # row empties out (R_a = 1) must match the reference with that row gone
+    t = np.array([1, 0, 0, 30, 40, 35, 20, 33, 41])
+    u = t.copy()
+    u[0] -= 1
+    xe, de = engine_chi2(u, 0)
+    xl, dl = engine_chi2(t, 2)
+    assert de == dl == 2 and abs(xe - xl) < 1e-10
+    # a column with a small count: 1/(K_b - 1) rather than 1/K_b for the
+    # reduced column matters far beyond the guard here
+    v = np.array([30, 2, 25, 28, 1, 33, 35, 3, 27])
+    w = v.copy()
+    w[4] -= 1
+    xe, _ = engine_chi2(w, 0)
+    xl, _ = engine_chi2(v, 2 + 4)
+    assert abs(xe - xl) < 1e-10 * (1 + xe)
+
+
 def test_planted_yates_mistake_is_caught():
     # a table whose Yates decision sits at E = R K / n = 5 exactly: a float
     # comparison off by one rounding would flip Yates and change chi2 by O(1)
