@@ -128,14 +128,34 @@ EDDY_HD int eddy_edge_decision(const int *t, int prior, const eddy_edge_params *
 // ---------------------------------------------------------------------------
 // fast forms
 // ---------------------------------------------------------------------------
+// 1/v and 1/(v - 1) from one division (0 for a zero denominator): with
+// inv = 1/(v (v - 1)) rounded once, (v - 1) inv and v inv are within 1.5 ulp
+// of the true reciprocals (v (v - 1) <= 2^22 is exact).
+EDDY_HD void eddy_recip_pair(int v, double *r, double *rm) {
+  if (v >= 2) {
+    const double inv = 1.0 / ((double)v * (double)(v - 1));
+    *r = (double)(v - 1) * inv;
+    *rm = (double)v * inv;
+  } else {
+    *r = v == 1 ? 1.0 : 0.0;
+    *rm = 0.0;
+  }
+}
+
+EDDY_HD double eddy_sq(int v) { return (double)(v * v); }
+
 // x = chi2 / 2 and dof of tally t with row sums R, column sums K, total n,
-// from iR[i] = 1/R[i], iK[j] = 1/K[j] (0 for empty sums) and in = 1/n.
-// Yates' rule is decided exactly in integers: (double)K*R/n < 5 holds iff
-// R*K < 5n, since a quotient of integers <= 2^22 that is not 5 differs from 5
-// by at least 1/n, far more than one rounding.  Each cell term
-// (|O - E| - y)^2 n iR iK differs from the reference's d*d/E by a few ulp of
-// itself plus 2|d| * 3 ulp(E) <= 2 n 6.7e-16, so |x - x_ref| <= 9 n 6.7e-16 +
-// 1e-15 x < 1.3e-11 + 1e-15 x for n <= 2046.
+// from iR[i] = 1/R[i], iK[j] = 1/K[j] (0 for empty sums, within 1.5 ulp) and
+// in = 1/n.  Yates' rule is decided exactly in integers: (double)K*R/n < 5
+// holds iff R*K < 5n, since a quotient of integers <= 2^22 that is not 5
+// differs from 5 by at least 1/n, far more than one rounding.
+//  - Without Yates' correction, the identity sum (O - E)^2 / E =
+//    n (sum O^2 / (R K) - 1) (empty rows and columns left out): each product
+//    O^2 iK iR is within 5 ulp, their sum S <= 3 within 12 ulp of itself, so
+//    |x - x_exact| <= 0.5 n 36 u + 2 u x < 4.1e-12 + 2.3e-16 x (n <= 2046).
+//  - With it, per cell (|O - E| - y)^2 n iR iK, which differs from the
+//    reference's d*d/E by a few ulp of itself plus 2|d| * 3 ulp(E) <=
+//    2 n 6.7e-16, so |x - x_ref| <= 9 n 6.7e-16 + 1e-15 x < 1.3e-11 + 1e-15 x.
 EDDY_HD double eddy_half_chi2_fast(const int *t, const int *R, const int *K, int n, const double *iR,
                                    const double *iK, double in, int *dof) {
   int zr = 0, zk = 0, yates = 0;
@@ -146,7 +166,17 @@ EDDY_HD double eddy_half_chi2_fast(const int *t, const int *R, const int *K, int
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++)
       if (R[i] && K[j] && R[i] * K[j] < 5 * n) yates = 1;
-  const double y = yates ? 0.5 : 0.0, dn = (double)n;
+  *dof = (2 - zk) * (2 - zr);
+  if (!yates) {
+    double S = 0.0;
+    for (int i = 0; i < 3; i++) {
+      double r = 0.0;
+      for (int j = 0; j < 3; j++) r = r + eddy_sq(t[i * 3 + j]) * iK[j];
+      S = S + iR[i] * r;
+    }
+    return 0.5 * (double)n * (S - 1.0);
+  }
+  const double y = 0.5, dn = (double)n;
   double chi = 0.0;
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++)
@@ -155,7 +185,6 @@ EDDY_HD double eddy_half_chi2_fast(const int *t, const int *R, const int *K, int
         const double d = eddy_fabs((double)t[i * 3 + j] - rk * in) - y;
         chi += d * d * dn * (iR[i] * iK[j]);
       }
-  *dof = (2 - zk) * (2 - zr);
   return 0.5 * chi;
 }
 
@@ -201,22 +230,6 @@ EDDY_HD int eddy_edge_decision_fast(const int *t, int prior, const eddy_edge_par
   }
   return eddy_edge_decision(t, prior, ep, eb, cpairs);
 }
-
-// 1/v and 1/(v - 1) from one division (0 for a zero denominator): with
-// inv = 1/(v (v - 1)) rounded once, (v - 1) inv and v inv are within 1.5 ulp
-// of the true reciprocals (v (v - 1) <= 2^22 is exact).
-EDDY_HD void eddy_recip_pair(int v, double *r, double *rm) {
-  if (v >= 2) {
-    const double inv = 1.0 / ((double)v * (double)(v - 1));
-    *r = (double)(v - 1) * inv;
-    *rm = (double)v * inv;
-  } else {
-    *r = v == 1 ? 1.0 : 0.0;
-    *rm = 0.0;
-  }
-}
-
-EDDY_HD double eddy_sq(int v) { return (double)(v * v); }
 
 // x = chi2 / 2 of the tally t minus one count in cell (a, b), t[a*3+b] > 0,
 // without Yates' correction, from the identity

@@ -26,6 +26,7 @@
 
 // kernels (kernels.hip)
 __global__ void k_count_test(KArgs a);
+__global__ void k_count_test_all(KArgs a);
 __global__ void k_loo(KArgs a);
 __global__ void k_rows(KArgs a);
 __global__ void k_score(KArgs a);
@@ -601,7 +602,10 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   for (int k = 0; k < 5; k++)
     if (!sl.tev[k]) HIPCHK(ctx, hipEventCreate(&sl.tev[k]));
   hipEventRecord(sl.tev[0], st);
-  if (!ch.tiles.empty()) k_count_test<<<(unsigned)ch.tiles.size(), 256, 0, st>>>(a);
+  if (!ch.tiles.empty()) {
+    if (plain_tests) k_count_test_all<<<(unsigned)ch.tiles.size(), 256, 0, st>>>(a);
+    else k_count_test<<<(unsigned)ch.tiles.size(), 256, 0, st>>>(a);
+  }
   hipEventRecord(sl.tev[1], st);
   if (!plain_tests) k_loo<<<nitems, 256, 0, st>>>(a);
   hipEventRecord(sl.tev[2], st);

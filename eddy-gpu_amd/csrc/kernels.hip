@@ -160,7 +160,10 @@ __device__ void pair_features(const int *t1, const int *t2, int prior, int c, fl
 #define K1_RS (K1_KCH + 16)
 #define K1_TS 112
 
-__global__ __launch_bounds__(256) void k_count_test(KArgs a) {
+// PLAIN: the all-tests schedule (every leave-one-out test here too); the
+// cutoff schedule's instance leaves them out and so holds fewer registers
+template <bool PLAIN>
+__device__ __forceinline__ void count_test_body(const KArgs &a) {
   const uint2 w = ((const uint2 *)a.tiles)[blockIdx.x];
   const int item = (int)w.x;
   const int bi = (int)(w.y >> 16), bj = (int)(w.y & 0xffff);
@@ -277,7 +280,7 @@ __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
       if (a.labhits && im.perm > 0 && (a.label_perms == 0 || logged))
         atomicAdd(&a.labhits[sm.pairbase + p], (unsigned)(f1 + f2));
     }
-    if (a.plain) {
+    if (PLAIN) {
       info |= d_loo_mask(t1, f1, pr, a, sm, cpairs, &ntests) << INFO_LOO_SHIFT(0);
       info |= d_loo_mask(t2, f2, pr, a, sm, cpairs, &ntests) << INFO_LOO_SHIFT(1);
       info |= INFO_EVAL(0) | INFO_EVAL(1);
@@ -291,6 +294,9 @@ __global__ __launch_bounds__(256) void k_count_test(KArgs a) {
   __syncthreads();
   if (tid == 0 && a.tests) atomicAdd((unsigned long long *)&a.tests[item], (unsigned long long)tcount);
 }
+
+__global__ __launch_bounds__(256) void k_count_test(KArgs a) { count_test_body<false>(a); }
+__global__ __launch_bounds__(256) void k_count_test_all(KArgs a) { count_test_body<true>(a); }
 
 // --------------------------------------------------------------------------
 // K2 (cutoff schedule): leave-one-out tests of each row, class g, up to the
