@@ -609,7 +609,7 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   hipEventRecord(sl.tev[1], st);
   if (!plain_tests) k_loo<<<nitems, 256, 0, st>>>(a);
   hipEventRecord(sl.tev[2], st);
-  const size_t rows_lds = (size_t)S * 8 + (size_t)2 * EDDY_NWAVES * maxG * 4 + (size_t)N * 2 + (size_t)2 * maxG + 16;
+  const size_t rows_lds = (size_t)S * 8 + (size_t)EDDY_NWAVES * maxG * 4 + (size_t)N * 2 + (size_t)2 * maxG + 16;
   k_rows<<<nitems, 256, rows_lds, st>>>(a);
   hipEventRecord(sl.tev[3], st);
   // the score kernel's phase-shared region: the network hash table, then the

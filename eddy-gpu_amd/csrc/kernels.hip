@@ -567,17 +567,15 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
   const uint8_t *xs = a.xs + sm.xoff;
 
   // LDS sized by the launch (rows_lds_bytes): S network deltas, per-wave
-  // flip and info rows of the largest set, the class lists, state flags
+  // candidate columns of the largest set, the class lists, state flags
   extern __shared__ __align__(16) unsigned char rdyn[];
   const int GM = a.rows_maxg;
   unsigned long long *delta = (unsigned long long *)rdyn;                  // [S]
-  uint32_t *flipsb = (uint32_t *)(delta + S);                              // [4][GM]: col | mask << 16
-  uint32_t *rinfob = flipsb + EDDY_NWAVES * GM;                            // [4][GM]: info words up to the cutoff
-  uint16_t *clsb = (uint16_t *)(rinfob + EDDY_NWAVES * GM);                // [N]: class 1, then class 2
+  uint32_t *candb = (uint32_t *)(delta + S);                               // [4][GM]: col | flip << 16 | full << 25
+  uint16_t *clsb = (uint16_t *)(candb + EDDY_NWAVES * GM);                 // [N]: class 1, then class 2
   uint8_t *sflb = (uint8_t *)(clsb + N);                                   // [2][GM]
   uint16_t *cls0 = clsb, *cls1 = clsb + a.n1;
-  uint32_t *flips_w = flipsb + wave * GM;
-  uint32_t *rinfo_w = rinfob + wave * GM;
+  uint32_t *cand_w = candb + wave * GM;
   __shared__ int ncls[2];
   __shared__ unsigned long long hfull[2];
   __shared__ uint16_t lrow[EDDY_NWAVES][EDDY_MAX_MP];
@@ -626,7 +624,10 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
   uint64_t lt_mask = (lane == 0) ? 0ULL : (~0ULL >> (64 - lane));
   for (int rg = wave; rg < 2 * G; rg += EDDY_NWAVES) {
     const int t = rg >> 1, g = rg & 1;
-    int nL = 0, nSolid = 0, nflip = 0, C = t - 1;
+    // candidates: the columns up to the cutoff that are a full edge or whose
+    // leave-one-out decisions differ from the full one (flip mask); a
+    // leave-one-out network's row list is drawn from these alone
+    int nL = 0, nSolid = 0, nflip = 0, ncand = 0, C = t - 1;
     bool done = (t == 0);
     for (int base = 0; base < t && !done; base += 64) {
       int col = base + lane;
@@ -647,7 +648,6 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
         done = true;
       }
       bool inRange = valid && lane <= limit;
-      if (inRange) rinfo_w[col] = w;
       if (__any(inRange && !ev)) {
         if (lane == 0) atomicOr(&a.status[item], ST_NOT_EVALUATED);
       }
@@ -657,10 +657,11 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
       nL += __popcll(hm);
       if (nL > mp) nL = mp;
       uint32_t flip = loo ^ (full ? 0x1FFu : 0u);
-      bool isf = inRange && flip != 0;
-      uint64_t fm = __ballot(isf);
-      if (isf) flips_w[nflip + __popcll(fm & lt_mask)] = (uint32_t)col | (flip << 16);
-      nflip += __popcll(fm);
+      const bool isc = inRange && (full || flip != 0);
+      const uint64_t cm = __ballot(isc);
+      if (isc) cand_w[ncand + __popcll(cm & lt_mask)] = (uint32_t)col | (flip << 16) | ((uint32_t)full << 25);
+      ncand += __popcll(cm);
+      nflip += __popcll(__ballot(inRange && flip != 0));
       nSolid += __popcll(sm_ & (limit == 63 ? ~0ULL : ((2ULL << limit) - 1)));
       wave_sync();
     }
@@ -683,22 +684,24 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
       int vt = xt[sid];
       if (vt >= 3) continue;
       bool changed = false;
-      for (int f = 0; f < nflip && !changed; f++) {
-        uint32_t e = flips_w[f];
-        int vc = xs[(size_t)(e & 0xffff) * NP + sid];
-        if (vc < 3 && ((e >> 16) >> (vt * 3 + vc)) & 1) changed = true;
+      for (int i = 0; i < ncand && !changed; i++) {
+        const uint32_t e = cand_w[i], fm = (e >> 16) & 0x1FFu;
+        if (!fm) continue;
+        const int vc = xs[(size_t)(e & 0xffff) * NP + sid];
+        if (vc < 3 && (fm >> (vt * 3 + vc)) & 1) changed = true;
       }
       if (!changed) continue;
+      // the sample's row list: the candidates in column order, each a full
+      // edge toggled where its flip mask covers the sample's (vt, vc) cell
       uint16_t nl[EDDY_MAX_MP];
-      int cnt = 0, fi = 0;
-      for (int col = 0; col <= C && cnt < mp; col++) {
-        uint32_t w = rinfo_w[col];
-        int bit = INFO_FULL_BIT(w, g);
-        if (fi < nflip && (int)(flips_w[fi] & 0xffff) == col) {
-          uint32_t e = flips_w[fi];
-          int vc = xs[(size_t)col * NP + sid];
-          if (vc < 3) bit ^= ((e >> 16) >> (vt * 3 + vc)) & 1;
-          fi++;
+      int cnt = 0;
+      for (int i = 0; i < ncand && cnt < mp; i++) {
+        const uint32_t e = cand_w[i], fm = (e >> 16) & 0x1FFu;
+        const int col = (int)(e & 0xffff);
+        int bit = (int)((e >> 25) & 1u);
+        if (fm) {
+          const int vc = xs[(size_t)col * NP + sid];
+          if (vc < 3) bit ^= (fm >> (vt * 3 + vc)) & 1;
         }
         if (bit) nl[cnt++] = (uint16_t)col;
       }
