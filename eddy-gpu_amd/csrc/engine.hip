@@ -614,8 +614,14 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   hipEventRecord(sl.tev[3], st);
   // the score kernel's phase-shared region: the network hash table, then the
   // BDeu mask stage, then the base family scores (at least ht * 12 bytes)
-  const size_t score_shared = std::max<size_t>({(size_t)ht * 12, (size_t)EDDY_NWAVES * 12 * bitw * 8,
-                                                 (size_t)4 * maxG * 8});
+  // BDeu of families with 2-3 parents: G4 lane groups per wave (fewer when
+  // the masks are long), each staging 12 masks, 84 histogram cells (u16)
+  // and 28 configuration terms
+  const int G4 = std::max(1, std::min(4, 32 / std::max(1, bitw)));
+  a.score_groups = G4;
+  const size_t bdeu_lds = (size_t)EDDY_NWAVES * G4 * (12 * bitw * 8 + 28 * 8) +
+                          (((size_t)EDDY_NWAVES * G4 * 84 * 2 + 7) & ~(size_t)7);
+  const size_t score_shared = std::max<size_t>({(size_t)ht * 12, bdeu_lds, (size_t)4 * maxG * 8});
   a.score_shared = (int)score_shared;
   const size_t score_lds = score_shared + (size_t)(4 * S + 1 + 2 * maxG) * 4;
   k_score<<<nitems, 256, score_lds, st>>>(a);

@@ -79,6 +79,7 @@ struct KArgs {
   int feat_only;           // count kernel: pooled-table features only (no tests, no outputs)
   int ht_size;             // network hash table entries (power of two >= 1.5 S)
   int score_shared;        // bytes of the score kernel's phase-shared LDS region
+  int score_groups;        // lane groups per wave in the score kernel's BDeu of 2-3 parent families
   eddy_edge_params ep;     // thresholds, lgamma(p+1) constants, underflow-cut interval
   const double *lgtab;     // [82][lgstride]: lgamma(n + 4/d)
   int lgstride;

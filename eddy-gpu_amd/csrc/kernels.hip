@@ -846,8 +846,6 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   int *famfull = ulist + S;                               // [2G]
   __shared__ int wsum[257];
   __shared__ int U, F, flag;
-  __shared__ int hist[EDDY_NWAVES][84];
-  __shared__ double lgv[EDDY_NWAVES][28];  // per parent configuration j: (lq0 - lq[n_j]) + inner sum
 
   const int E = min(a.evcount[item], im.evcap);
   int *es = a.evorder + im.evoff;
@@ -1052,7 +1050,6 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   // staged in LDS, one histogram cell per lane, the inner sums of the
   // reference's double loop per lane (one lane per parent configuration j),
   // the outer sum over j by lane 0, both in the reference's order
-  uint64_t *mst = (uint64_t *)dyn + (size_t)wave * 12 * a.bitw;  // [4 genes][3 states][W] (hash table is dead)
   unsigned long long st_hist = 0, st_bins = 0;                    // algorithmic bytes
   // (a) families with at most one parent (at most 9 cells): one lane each,
   // the same j-outer / k-inner order
@@ -1090,73 +1087,101 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
     fscore[f * 2 + g] = sum2;
     st_hist += (unsigned long long)(1 + np) * (unsigned long long)(g ? a.n2 : a.n1);
   }
-  // (b) families with two or three parents: a wave each
-  for (int fg = wave; fg < 2 * NF; fg += EDDY_NWAVES) {
-    const int f = fg >> 1, g = fg & 1;
-    const uint64_t key = a.famkey[im.foff + f] - 1;
-    const int t = (int)(key & 1023), np = (int)((key >> 10) & 3);
-    if (np <= 1) continue;  // wave-uniform
-    const int p0 = (int)((key >> 12) & 1023), p1 = (int)((key >> 22) & 1023), p2 = (int)((key >> 32) & 1023);
-    const uint8_t *sf = a.sflags + im.goff * 2 + g * G;
-    const uint32_t f0 = sf[t], f1 = np > 0 ? sf[p0] : 1u, f2 = np > 1 ? sf[p1] : 1u, f3 = np > 2 ? sf[p2] : 1u;
-    const int r = __popc(f0), n1s = __popc(f1), n2s = __popc(f2), n3s = __popc(f3);
-    const int q = n1s * n2s * n3s;  // parents' observed-state combinations (1 without parents)
-    double sum2 = 0.0;
-    if (r > 0 && (np == 0 || q > 0)) {
-      const int W = a.bitw;
-      // stage the masks: slot i*3 + v holds gene i's state-v mask
-      {
+  // (b) families with two or three parents: a group of 64/G4 lanes each, G4
+  // groups per wave working on different families side by side (their
+  // memory latencies overlap); a group stages its family's masks, counts one
+  // histogram cell per lane, sums each parent configuration's inner terms
+  // per lane, and its first lane adds the configurations in order
+  {
+    const int G4 = a.score_groups, GS = 64 / G4, gi = lane / GS, gl = lane - gi * GS;
+    const int W = a.bitw;
+    uint64_t *gmst = (uint64_t *)dyn + (size_t)(wave * G4 + gi) * 12 * W;  // [4 genes][3 states][W]
+    uint16_t *ghist = (uint16_t *)((uint64_t *)dyn + (size_t)EDDY_NWAVES * G4 * 12 * W) + (wave * G4 + gi) * 84;
+    double *glgv = (double *)((uint64_t *)dyn + (size_t)EDDY_NWAVES * G4 * 12 * W +
+                              ((size_t)EDDY_NWAVES * G4 * 84 * 2 + 7) / 8) +
+                   (wave * G4 + gi) * 28;
+    for (int fg0 = wave * G4; fg0 < 2 * NF; fg0 += EDDY_NWAVES * G4) {
+      const int fg = fg0 + gi;
+      int np = 0, t = 0, p0 = 0, p1 = 0, p2 = 0, g = 0, f = 0;
+      if (fg < 2 * NF) {
+        f = fg >> 1;
+        g = fg & 1;
+        const uint64_t key = a.famkey[im.foff + f] - 1;
+        t = (int)(key & 1023);
+        np = (int)((key >> 10) & 3);
+        p0 = (int)((key >> 12) & 1023);
+        p1 = (int)((key >> 22) & 1023);
+        p2 = (int)((key >> 32) & 1023);
+      }
+      const bool mine = np > 1;  // group-uniform
+      uint32_t f0 = 1u, f1 = 1u, f2 = 1u, f3 = 1u;
+      if (mine) {
+        const uint8_t *sf = a.sflags + im.goff * 2 + g * G;
+        f0 = sf[t];
+        f1 = sf[p0];
+        f2 = sf[p1];
+        f3 = np > 2 ? sf[p2] : 1u;
+      }
+      const int r = __popc(f0), n1s = __popc(f1), n2s = __popc(f2), n3s = __popc(f3);
+      const int q = n1s * n2s * n3s;  // parents' observed-state combinations
+      const bool live = mine && r > 0 && q > 0;
+      if (live) {
         const uint64_t *mk = a.bits + (size_t)(im.goff * 2 + g * G) * 3 * W;
-        const int gl0 = t, gl1 = p0, gl2 = p1, gl3 = p2;
-        for (int x = lane; x < (np + 1) * 3 * W; x += 64) {
-          const int slot = x / W, w = x - slot * W, gi = slot / 3, v = slot - gi * 3;
-          const int gene = gi == 0 ? gl0 : gi == 1 ? gl1 : gi == 2 ? gl2 : gl3;
-          mst[x] = mk[(size_t)(gene * 3 + v) * W + w];
+        for (int x = gl; x < (np + 1) * 3 * W; x += GS) {
+          const int slot = x / W, w = x - slot * W, gix = slot / 3, v = slot - gix * 3;
+          const int gene = gix == 0 ? t : gix == 1 ? p0 : gix == 2 ? p1 : p2;
+          gmst[x] = mk[(size_t)(gene * 3 + v) * W + w];
         }
       }
       wave_sync();
-      for (int c = lane; c < q * r; c += 64) {
-        const int j = div_small(c, r), kk = c - j * r;
-        const int j1 = div_small(j, n1s), i1 = j - j1 * n1s;
-        const int i3 = div_small(j1, n2s), i2 = j1 - i3 * n2s;
-        const uint64_t *m0 = mst + (0 * 3 + nth_state(f0, kk)) * W;
-        const uint64_t *m1 = mst + (1 * 3 + nth_state(f1, i1)) * W;
-        const uint64_t *m2 = mst + (2 * 3 + nth_state(f2, i2)) * W;
-        const uint64_t *m3 = mst + (3 * 3 + nth_state(f3, i3)) * W;
-        int cnt = 0;
-        for (int w = 0; w < W; w++) {
-          uint64_t x = m0[w];
-          if (np > 0) x &= m1[w];
-          if (np > 1) x &= m2[w];
-          if (np > 2) x &= m3[w];
-          cnt += __popcll(x);
+      if (live) {
+        for (int c = gl; c < q * r; c += GS) {
+          const int j = div_small(c, r), kk = c - j * r;
+          const int j1 = div_small(j, n1s), i1 = j - j1 * n1s;
+          const int i3 = div_small(j1, n2s), i2 = j1 - i3 * n2s;
+          const uint64_t *m0 = gmst + (0 * 3 + nth_state(f0, kk)) * W;
+          const uint64_t *m1 = gmst + (1 * 3 + nth_state(f1, i1)) * W;
+          const uint64_t *m2 = gmst + (2 * 3 + nth_state(f2, i2)) * W;
+          const uint64_t *m3 = gmst + (3 * 3 + nth_state(f3, i3)) * W;
+          int cnt = 0;
+          for (int w = 0; w < W; w++) {
+            uint64_t x = m0[w] & m1[w] & m2[w];
+            if (np > 2) x &= m3[w];
+            cnt += __popcll(x);
+          }
+          ghist[c] = (uint16_t)cnt;
         }
-        hist[wave][c] = cnt;
       }
       wave_sync();
-      const double *lq = a.lgtab + (size_t)q * a.lgstride;
-      const double *lqr = a.lgtab + (size_t)(q * r) * a.lgstride;
-      const double lq0 = lq[0], lqr0 = lqr[0];
-      for (int j = lane; j < q; j += 64) {
-        int nij = 0;
-        double sum1 = 0.0;
-        for (int k = 0; k < r; k++) {
-          const int h = hist[wave][j * r + k];
-          nij += h;
-          sum1 = sum1 + (lqr[h] - lqr0);
+      double lq0 = 0.0;
+      const double *lq = a.lgtab, *lqr = a.lgtab;
+      if (live) {
+        lq = a.lgtab + (size_t)q * a.lgstride;
+        lqr = a.lgtab + (size_t)(q * r) * a.lgstride;
+        lq0 = lq[0];
+        const double lqr0 = lqr[0];
+        for (int j = gl; j < q; j += GS) {
+          int nij = 0;
+          double sum1 = 0.0;
+          for (int k = 0; k < r; k++) {
+            const int h = ghist[j * r + k];
+            nij += h;
+            sum1 = sum1 + (lqr[h] - lqr0);
+          }
+          glgv[j] = (lq0 - lq[nij]) + sum1;
         }
-        lgv[wave][j] = (lq0 - lq[nij]) + sum1;
       }
       wave_sync();
-      if (lane == 0)
-        for (int j = 0; j < q; j++) sum2 = sum2 + lgv[wave][j];
+      if (mine && gl == 0) {
+        double sum2 = 0.0;
+        if (live)
+          for (int j = 0; j < q; j++) sum2 = sum2 + glgv[j];
+        fscore[f * 2 + g] = sum2;
+        st_hist += (unsigned long long)(1 + np) * (unsigned long long)(g ? a.n2 : a.n1);
+        if (live) st_bins += (unsigned long long)(4 * q * r);
+      }
+      wave_sync();
     }
-    if (lane == 0) {
-      fscore[f * 2 + g] = sum2;
-      st_hist += (unsigned long long)(1 + np) * (unsigned long long)(g ? a.n2 : a.n1);
-      if (r > 0 && q > 0) st_bins += (unsigned long long)(4 * q * r);
-    }
-    wave_sync();
   }
   if (a.counters) {
     __shared__ unsigned long long sh_st[2];
