@@ -377,31 +377,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   // (64 columns per wave step).
   // ---- A: plan P = column of the (mp + extra)-th full-test edge, a wave per
   // entry scanning 64 columns per ballot ------------------------------------
-  for (int r = wave; r < R; r += EDDY_NWAVES) {
-    const int t = r >> 1, g = r & 1;
-    const int want = mp + (a.plan_extra ? a.plan_extra[(im.goff * 2 + g * G + t)] : 0);
-    int hits = 0, P = t - 1;
-    for (int base = 0; base < t; base += 64) {
-      const int col = base + lane;
-      int f = 0;
-      if (col < t) {
-        const long long pc = pair_index(t, col);
-        f = full_of(a, im, sm, pc, g, cpairs, &ntests);
-      }
-      const uint64_t b = __ballot(f);
-      const int nb = __popcll(b);
-      if (hits + nb >= want) {  // wave-uniform
-        uint64_t x = b;
-        for (int i = 1; i < want - hits; i++) x &= x - 1;
-        P = base + __ffsll((long long)x) - 1;
-        break;
-      }
-      hits += nb;
+  // Four entries per wave step: their first 64 columns' info words are
+  // loaded together (their latencies overlap), then each entry continues
+  // alone only if it has fewer than `want` full edges there.
+  for (int r0 = wave; r0 < R; r0 += 4 * EDDY_NWAVES) {
+    uint16_t h0[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int r = r0 + u * EDDY_NWAVES, t = r >> 1;
+      h0[u] = (r < R && lane < t) ? *info_half(a, im, pair_index(t, lane), r & 1) : (uint16_t)IH_FEVAL;
     }
-    if (lane == 0) {
-      plan[r] = P;
-      off[r] = P + 1;
-      solid[r] = 0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int r = r0 + u * EDDY_NWAVES;
+      if (r >= R) break;  // wave-uniform
+      const int t = r >> 1, g = r & 1;
+      const int want = mp + (a.plan_extra ? a.plan_extra[(im.goff * 2 + g * G + t)] : 0);
+      int hits = 0, P = t - 1;
+      for (int base = 0; base < t; base += 64) {
+        const int col = base + lane;
+        int f = 0;
+        if (col < t) {
+          if (base == 0 && (h0[u] & IH_FEVAL)) f = h0[u] & IH_FULL;
+          else f = full_of(a, im, sm, pair_index(t, col), g, cpairs, &ntests);
+        }
+        const uint64_t b = __ballot(f);
+        const int nb = __popcll(b);
+        if (hits + nb >= want) {  // wave-uniform
+          uint64_t x = b;
+          for (int i = 1; i < want - hits; i++) x &= x - 1;
+          P = base + __ffsll((long long)x) - 1;
+          break;
+        }
+        hits += nb;
+      }
+      if (lane == 0) {
+        plan[r] = P;
+        off[r] = P + 1;
+        solid[r] = 0;
+      }
     }
   }
   __syncthreads();
@@ -597,20 +611,31 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
     const int gene = i >> 1, g = i & 1;
     const uint8_t *row = xs + (size_t)gene * NP;
     const int ng = ncls[g];
+    const uint16_t *cl = g ? cls1 : cls0;
     uint64_t *mk = a.bits + ((size_t)(im.goff * 2 + g * G + gene) * 3) * a.bitw;
     uint64_t any0 = 0, any1 = 0, any2 = 0;
-    for (int w = 0; w < a.bitw; w++) {
-      const int k = w * 64 + lane;
-      const int v = k < ng ? row[(g ? cls1 : cls0)[k]] : 3;
-      const uint64_t m0 = __ballot(v == 0), m1 = __ballot(v == 1), m2 = __ballot(v == 2);
-      if (lane == 0) {
-        mk[w] = m0;
-        mk[a.bitw + w] = m1;
-        mk[2 * a.bitw + w] = m2;
+    // four words' gathers in flight at once
+    for (int w0 = 0; w0 < a.bitw; w0 += 4) {
+      int v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int k = (w0 + u) * 64 + lane;
+        v[u] = k < ng ? row[cl[k]] : 3;
       }
-      any0 |= m0;
-      any1 |= m1;
-      any2 |= m2;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int w = w0 + u;
+        if (w >= a.bitw) break;  // wave-uniform
+        const uint64_t m0 = __ballot(v[u] == 0), m1 = __ballot(v[u] == 1), m2 = __ballot(v[u] == 2);
+        if (lane == 0) {
+          mk[w] = m0;
+          mk[a.bitw + w] = m1;
+          mk[2 * a.bitw + w] = m2;
+        }
+        any0 |= m0;
+        any1 |= m1;
+        any2 |= m2;
+      }
     }
     if (lane == 0) {
       const uint32_t f = (any0 ? 1u : 0u) | (any1 ? 2u : 0u) | (any2 ? 4u : 0u);
