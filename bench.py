@@ -102,9 +102,10 @@ def main():
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--nsets", type=int, default=None)
     ap.add_argument("--train-sets", type=int, default=100)
-    ap.add_argument("--epochs", type=int, default=2)
-    ap.add_argument("--train-rows", type=int, default=131072)
+    ap.add_argument("--epochs", type=int, default=1)
+    ap.add_argument("--train-rows", type=int, default=65536)
     ap.add_argument("--margin", type=float, default=1.0)
+    ap.add_argument("--label-perms", type=int, default=10)
     ap.add_argument("--plain-sets", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -137,7 +138,8 @@ def main():
     nsets = len(wl.sets)
     prm = wl.params(mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=1)
     cfg = eddy_amd.learn_cfg(train_sets=args.train_sets, epochs=args.epochs, batch=16384, lr=3e-3,
-                             margin=args.margin, seed=1, max_rows=args.train_rows, label_perms=10)
+                             margin=args.margin, seed=1, max_rows=args.train_rows,
+                             label_perms=args.label_perms)
 
     print("bench: setup %.1fs, %d sets, N=%d" % (setup_s, nsets, wl.N), file=sys.stderr, flush=True)
     for w in range(args.warmup):
@@ -185,7 +187,7 @@ def main():
         "epochs": args.epochs,
         "loss_initial": rep.loss_initial, "loss_final": rep.loss_final, "margin": args.margin,
         "collect_ms": rep.collect_ms, "train_gpu_ms_overlapped": rep.train_ms, "apply_ms": rep.apply_ms,
-        "first_windowed_set": rep.first_windowed_set, "label_perms": 10,
+        "first_windowed_set": rep.first_windowed_set, "label_perms": args.label_perms,
         "tests_applied_sets": rep.tests_apply,
     }
 

@@ -132,9 +132,9 @@ int main(int argc, char **argv) {
   if (learned && mode == EDDY_MODE_BATCHED && schedule == EDDY_TESTS_CUTOFF && learn_sets >= 0) {
     eddy_learn_cfg lc{};
     lc.train_sets = learn_sets;
-    lc.epochs = 2;
+    lc.epochs = 1;
     lc.batch = 16384;
-    lc.max_rows = 131072;
+    lc.max_rows = 65536;
     lc.label_perms = 10;
     lc.lr = 3e-3f;
     lc.margin = 1.0f;

@@ -1273,20 +1273,13 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     CHK(compute_windows(ctx, meta, 0, p->mp, ctx->b_win, ctx->stream));
     windows_ready = true;
   }
-  // the learned job: every set runs with deferred full tests (windows of -1)
-  // until the network is trained; logged sets also run every full test of
-  // their first label_perms permutations
+  // the learned job: sets run every full test in the count kernel until the
+  // network is trained, then only those inside its windows (the loo kernel's
+  // plan runs any other it reaches); logged sets also record the full-test
+  // outcomes of their first label_perms permutations
   bool training = false, trained_windows = false;
   const int label_perms = lc ? (lc->label_perms > 0 ? std::min(lc->label_perms, perms - 1) : perms - 1) : 0;
   hipEvent_t tr0 = nullptr, tr1 = nullptr;
-  if (lc && nsets > 0) {
-    long long sumg = 0;
-    for (const SetMeta &m : meta) sumg += m.G;
-    GROW(b_win, sizeof(int) * std::max(1LL, sumg));
-    HIPCHK(ctx, hipMemsetAsync(ctx->b_win.p, 0xFF, sizeof(int) * std::max(1LL, sumg), ctx->stream));  // -1
-    windows_ready = true;
-    win_from = 0;
-  }
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->timing.phase1_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 
@@ -1457,6 +1450,9 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     auto launch = [&]() -> int {
       ctx->k_labhits = chunk_first < collect_upto ? (unsigned *)ctx->b_labhits.p : nullptr;
       ctx->k_label_perms = chunk_first < collect_upto ? label_perms : 0;
+      // the count kernel's full-test windows: the deferred schedule's (-1),
+      // the given network's, or in the learned job the trained network's
+      // once its event has passed (until then every full test runs there)
       ctx->k_win = (windows_ready && chunk_first >= win_from) ? (const int *)ctx->b_win.p : nullptr;
       if (training && !trained_windows && hipEventQuery(ctx->train_ev) == hipSuccess) trained_windows = true;
       if (trained_windows && chunk_first >= collect_upto) {
