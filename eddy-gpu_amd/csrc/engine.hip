@@ -30,6 +30,8 @@ __global__ void k_count_test_all(KArgs a);
 __global__ void k_loo(KArgs a);
 __global__ void k_rows(KArgs a);
 __global__ void k_score(KArgs a);
+__global__ void k_bdeu(KArgs a);
+__global__ void k_score_tail(KArgs a);
 __global__ void k_pvalue(const double *js, int nsets, int perms, const int *nan0, double *out);
 __global__ void k_gather(const uint8_t *x, int N, int NP, const int *genes, const long long *rowoff, int nrows,
                          uint8_t *xs);
@@ -71,7 +73,8 @@ struct Chunk {
 // host prepares the next.
 struct Slot {
   DevBuf items, labels, tiles, info, tab, lfull, lcnt, sflags, bits, ev, evcount, hnet, rep, clsg, evorder, evfam,
-      famkey, famscore, famht, famhv, lval, uniq, famcount, status, tests, order, slices;
+      famkey, famscore, famht, famhv, lval, uniq, famcount, status, tests, order, slices, bdtask, bdcount, sculist,
+      scevst, scfamfull;
   hipStream_t st = nullptr;
   hipEvent_t tev[5] = {};
   int *h_int = nullptr;  // pinned: status, events, families, unique networks (4 x h_cap)
@@ -240,7 +243,7 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
     DevBuf *sb[] = {&sl.items, &sl.labels, &sl.tiles, &sl.info, &sl.tab, &sl.lfull, &sl.lcnt, &sl.sflags, &sl.bits,
                     &sl.ev, &sl.evcount, &sl.hnet, &sl.rep, &sl.clsg, &sl.evorder, &sl.evfam, &sl.famkey,
                     &sl.famscore, &sl.famht, &sl.famhv, &sl.lval, &sl.uniq, &sl.famcount, &sl.status, &sl.tests,
-                    &sl.order, &sl.slices};
+                    &sl.order, &sl.slices, &sl.bdtask, &sl.bdcount, &sl.sculist, &sl.scevst, &sl.scfamfull};
     for (DevBuf *b : sb)
       if (b->p) hipFree(b->p);
     if (sl.h_int) hipHostFree(sl.h_int);
@@ -516,6 +519,14 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   CHK(grow(ctx, sl.status, sizeof(int) * nitems));
   CHK(grow(ctx, sl.tests, sizeof(long long) * nitems));
   CHK(grow(ctx, sl.order, sizeof(int) * nitems));
+  // BDeu task list: room for 2 x the families the items can hold, at most 4M
+  // tasks (an item that finds the list full scores its families in place)
+  const long long bd_cap = std::max(1LL, std::min(1LL << 22, 2 * foff));
+  CHK(grow(ctx, sl.bdtask, sizeof(int2) * bd_cap));
+  CHK(grow(ctx, sl.bdcount, sizeof(unsigned)));
+  CHK(grow(ctx, sl.sculist, sizeof(int) * ch.sslots));
+  CHK(grow(ctx, sl.scevst, sizeof(int) * ch.sslots));
+  CHK(grow(ctx, sl.scfamfull, sizeof(int) * 2 * ch.gslots));
   if (sl.h_cap < nitems) {
     if (sl.h_int) hipHostFree(sl.h_int);
     if (sl.h_tests) hipHostFree(sl.h_tests);
@@ -543,6 +554,7 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   HIPCHK(ctx, hipMemsetAsync(sl.evcount.p, 0, sizeof(int) * nitems, st));
   HIPCHK(ctx, hipMemsetAsync(sl.status.p, 0, sizeof(int) * nitems, st));
   HIPCHK(ctx, hipMemsetAsync(sl.tests.p, 0, sizeof(long long) * nitems, st));
+  HIPCHK(ctx, hipMemsetAsync(sl.bdcount.p, 0, sizeof(unsigned), st));
 
   KArgs a{};
   a.N = N;
@@ -598,6 +610,12 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   a.perms = perms;
   a.status = (int *)sl.status.p;
   a.tests = (long long *)sl.tests.p;
+  a.bd_tasks = (int2 *)sl.bdtask.p;
+  a.bd_count = (unsigned *)sl.bdcount.p;
+  a.bd_cap = (int)bd_cap;
+  a.sc_ulist = (int *)sl.sculist.p;
+  a.sc_evst = (int *)sl.scevst.p;
+  a.sc_famfull = (int *)sl.scfamfull.p;
 
   for (int k = 0; k < 5; k++)
     if (!sl.tev[k]) HIPCHK(ctx, hipEventCreate(&sl.tev[k]));
@@ -625,6 +643,9 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   a.score_shared = (int)score_shared;
   const size_t score_lds = score_shared + (size_t)(4 * S + 1 + 2 * maxG) * 4;
   k_score<<<nitems, 256, score_lds, st>>>(a);
+  k_bdeu<<<2048, 256, bdeu_lds, st>>>(a);
+  const size_t tail_lds = std::max<size_t>((size_t)4 * maxG * 8, (size_t)2 * S * 8);
+  k_score_tail<<<nitems, 256, tail_lds, st>>>(a);
   hipEventRecord(sl.tev[4], st);
   HIPCHK(ctx, hipGetLastError());
   int *h = sl.h_int;

@@ -128,6 +128,13 @@ struct KArgs {
   int bitw;                // words per mask: ceil(max(n1, n2) / 64)
   int rows_maxg;           // largest set of the launch (row kernel LDS layout)
   long long *tests;        // [item] evaluated tests
+  // score kernel -> BDeu kernel -> score tail kernel
+  int2 *bd_tasks;          // (item, family * 2 + class) of every 2-3 parent family to score
+  unsigned *bd_count;      // tasks listed (may exceed bd_cap: the rest were scored in place)
+  int bd_cap;
+  int *sc_ulist;           // [soff + u] unique networks in order
+  int *sc_evst;            // [soff + s] first event of network s (network order)
+  int *sc_famfull;         // [goff*2 + g*G + t] family of row t in class g's full network
 };
 
 enum {

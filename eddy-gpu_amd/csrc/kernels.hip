@@ -760,6 +760,9 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
 
 // --------------------------------------------------------------------------
 // K4: dedup, BDeu, likelihood, JS.  One workgroup per item.
+
+
+
 // --------------------------------------------------------------------------
 
 __device__ __forceinline__ int net_class(int s, int n1) { return s <= n1 ? 0 : 1; }
@@ -846,6 +849,105 @@ __device__ int block_exscan(int *v, int n, int *wsum) {
   int total = wsum[256];
   __syncthreads();
   return total;
+}
+
+// BDeu of one family with two or three parents in one scoring class
+// (kernel5.cu:4-236), by a group of GS lanes of one wave (gl = lane within the
+// group): the family's state masks staged in the group's LDS, one histogram
+// cell per lane, each parent configuration's inner terms per lane, and the
+// group's first lane adds the configurations in order (j outer, k inner).
+// Every lane of the wave calls it together (it synchronises the wave); a
+// group with no family (f < 0) or a family with fewer than two parents only
+// keeps step.
+__device__ __forceinline__ void bdeu_group(const KArgs &a, const ItemMeta &im, int G, int f, int g, int gl, int GS,
+                                           uint64_t *gmst, uint16_t *ghist, double *glgv,
+                                           unsigned long long &st_hist, unsigned long long &st_bins) {
+  const int W = a.bitw;
+  int np = 0, t = 0, p0 = 0, p1 = 0, p2 = 0;
+  if (f >= 0) {
+    const uint64_t key = a.famkey[im.foff + f] - 1;
+    t = (int)(key & 1023);
+    np = (int)((key >> 10) & 3);
+    p0 = (int)((key >> 12) & 1023);
+    p1 = (int)((key >> 22) & 1023);
+    p2 = (int)((key >> 32) & 1023);
+  }
+  const bool mine = np > 1;  // group-uniform
+  uint32_t f0 = 1u, f1 = 1u, f2 = 1u, f3 = 1u;
+  if (mine) {
+    const uint8_t *sf = a.sflags + im.goff * 2 + g * G;
+    f0 = sf[t];
+    f1 = sf[p0];
+    f2 = sf[p1];
+    f3 = np > 2 ? sf[p2] : 1u;
+  }
+  const int r = __popc(f0), n1s = __popc(f1), n2s = __popc(f2), n3s = __popc(f3);
+  const int q = n1s * n2s * n3s;  // parents' observed-state combinations
+  const bool live = mine && r > 0 && q > 0;
+  if (live) {
+    const uint64_t *mk = a.bits + (size_t)(im.goff * 2 + g * G) * 3 * W;
+    for (int x = gl; x < (np + 1) * 3 * W; x += GS) {
+      const int slot = x / W, w = x - slot * W, gix = slot / 3, v = slot - gix * 3;
+      const int gene = gix == 0 ? t : gix == 1 ? p0 : gix == 2 ? p1 : p2;
+      gmst[x] = mk[(size_t)(gene * 3 + v) * W + w];
+    }
+  }
+  wave_sync();
+  if (live) {
+    for (int c = gl; c < q * r; c += GS) {
+      const int j = div_small(c, r), kk = c - j * r;
+      const int j1 = div_small(j, n1s), i1 = j - j1 * n1s;
+      const int i3 = div_small(j1, n2s), i2 = j1 - i3 * n2s;
+      const uint64_t *m0 = gmst + (0 * 3 + nth_state(f0, kk)) * W;
+      const uint64_t *m1 = gmst + (1 * 3 + nth_state(f1, i1)) * W;
+      const uint64_t *m2 = gmst + (2 * 3 + nth_state(f2, i2)) * W;
+      const uint64_t *m3 = gmst + (3 * 3 + nth_state(f3, i3)) * W;
+      int cnt = 0;
+      for (int w = 0; w < W; w++) {
+        uint64_t x = m0[w] & m1[w] & m2[w];
+        if (np > 2) x &= m3[w];
+        cnt += __popcll(x);
+      }
+      ghist[c] = (uint16_t)cnt;
+    }
+  }
+  wave_sync();
+  if (live) {
+    const double *lq = a.lgtab + (size_t)q * a.lgstride;
+    const double *lqr = a.lgtab + (size_t)(q * r) * a.lgstride;
+    const double lq0 = lq[0], lqr0 = lqr[0];
+    for (int j = gl; j < q; j += GS) {
+      int nij = 0;
+      double sum1 = 0.0;
+      for (int k = 0; k < r; k++) {
+        const int h = ghist[j * r + k];
+        nij += h;
+        sum1 = sum1 + (lqr[h] - lqr0);
+      }
+      glgv[j] = (lq0 - lq[nij]) + sum1;
+    }
+  }
+  wave_sync();
+  if (mine && gl == 0) {
+    double sum2 = 0.0;
+    if (live)
+      for (int j = 0; j < q; j++) sum2 = sum2 + glgv[j];
+    a.famscore[(im.foff + f) * 2 + g] = sum2;
+    st_hist += (unsigned long long)(1 + np) * (unsigned long long)(g ? a.n2 : a.n1);
+    if (live) st_bins += (unsigned long long)(4 * q * r);
+  }
+  wave_sync();
+}
+
+// the BDeu group stage of one wave in an LDS region laid out as
+// [waves x G4 groups][12 W] masks, [waves x G4][84] u16 histogram cells,
+// [waves x G4][28] configuration terms
+__device__ __forceinline__ void bdeu_stage(unsigned char *base, int G4, int W, int wave, int gi, uint64_t **gmst,
+                                           uint16_t **ghist, double **glgv) {
+  *gmst = (uint64_t *)base + (size_t)(wave * G4 + gi) * 12 * W;
+  *ghist = (uint16_t *)((uint64_t *)base + (size_t)EDDY_NWAVES * G4 * 12 * W) + (wave * G4 + gi) * 84;
+  *glgv = (double *)((uint64_t *)base + (size_t)EDDY_NWAVES * G4 * 12 * W + ((size_t)EDDY_NWAVES * G4 * 84 * 2 + 7) / 8) +
+          (wave * G4 + gi) * 28;
 }
 
 __global__ __launch_bounds__(256) void k_score(KArgs a) {
@@ -1112,100 +1214,43 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
     fscore[f * 2 + g] = sum2;
     st_hist += (unsigned long long)(1 + np) * (unsigned long long)(g ? a.n2 : a.n1);
   }
-  // (b) families with two or three parents: a group of 64/G4 lanes each, G4
-  // groups per wave working on different families side by side (their
-  // memory latencies overlap); a group stages its family's masks, counts one
-  // histogram cell per lane, sums each parent configuration's inner terms
-  // per lane, and its first lane adds the configurations in order
+  // (b) families with two or three parents: appended to the launch's task
+  // list for the BDeu kernel (its lane groups over all items of the chunk),
+  // or, when the list is full, scored here by this block's lane groups
   {
-    const int G4 = a.score_groups, GS = 64 / G4, gi = lane / GS, gl = lane - gi * GS;
-    const int W = a.bitw;
-    uint64_t *gmst = (uint64_t *)dyn + (size_t)(wave * G4 + gi) * 12 * W;  // [4 genes][3 states][W]
-    uint16_t *ghist = (uint16_t *)((uint64_t *)dyn + (size_t)EDDY_NWAVES * G4 * 12 * W) + (wave * G4 + gi) * 84;
-    double *glgv = (double *)((uint64_t *)dyn + (size_t)EDDY_NWAVES * G4 * 12 * W +
-                              ((size_t)EDDY_NWAVES * G4 * 84 * 2 + 7) / 8) +
-                   (wave * G4 + gi) * 28;
-    for (int fg0 = wave * G4; fg0 < 2 * NF; fg0 += EDDY_NWAVES * G4) {
-      const int fg = fg0 + gi;
-      int np = 0, t = 0, p0 = 0, p1 = 0, p2 = 0, g = 0, f = 0;
-      if (fg < 2 * NF) {
-        f = fg >> 1;
-        g = fg & 1;
-        const uint64_t key = a.famkey[im.foff + f] - 1;
-        t = (int)(key & 1023);
-        np = (int)((key >> 10) & 3);
-        p0 = (int)((key >> 12) & 1023);
-        p1 = (int)((key >> 22) & 1023);
-        p2 = (int)((key >> 32) & 1023);
+    __shared__ int nb, bbase, bpos;
+    if (tid == 0) nb = bpos = 0;
+    __syncthreads();
+    int mine_n = 0;
+    for (int fg = tid; fg < 2 * NF; fg += 256) {
+      const uint64_t key = a.famkey[im.foff + (fg >> 1)] - 1;
+      mine_n += ((key >> 10) & 3) > 1;
+    }
+    if (mine_n) atomicAdd(&nb, mine_n);
+    __syncthreads();
+    if (tid == 0) {
+      bbase = -1;
+      if (a.bd_tasks && nb > 0) {
+        const unsigned b0 = atomicAdd(a.bd_count, (unsigned)nb);
+        if ((long long)b0 + nb <= (long long)a.bd_cap) bbase = (int)b0;
       }
-      const bool mine = np > 1;  // group-uniform
-      uint32_t f0 = 1u, f1 = 1u, f2 = 1u, f3 = 1u;
-      if (mine) {
-        const uint8_t *sf = a.sflags + im.goff * 2 + g * G;
-        f0 = sf[t];
-        f1 = sf[p0];
-        f2 = sf[p1];
-        f3 = np > 2 ? sf[p2] : 1u;
+    }
+    __syncthreads();
+    if (bbase >= 0) {
+      for (int fg = tid; fg < 2 * NF; fg += 256) {
+        const uint64_t key = a.famkey[im.foff + (fg >> 1)] - 1;
+        if (((key >> 10) & 3) > 1) a.bd_tasks[bbase + atomicAdd(&bpos, 1)] = make_int2(item, fg);
       }
-      const int r = __popc(f0), n1s = __popc(f1), n2s = __popc(f2), n3s = __popc(f3);
-      const int q = n1s * n2s * n3s;  // parents' observed-state combinations
-      const bool live = mine && r > 0 && q > 0;
-      if (live) {
-        const uint64_t *mk = a.bits + (size_t)(im.goff * 2 + g * G) * 3 * W;
-        for (int x = gl; x < (np + 1) * 3 * W; x += GS) {
-          const int slot = x / W, w = x - slot * W, gix = slot / 3, v = slot - gix * 3;
-          const int gene = gix == 0 ? t : gix == 1 ? p0 : gix == 2 ? p1 : p2;
-          gmst[x] = mk[(size_t)(gene * 3 + v) * W + w];
-        }
+    } else if (nb > 0) {
+      const int G4 = a.score_groups, GS = 64 / G4, gi = lane / GS, gl = lane - gi * GS;
+      uint64_t *gmst;
+      uint16_t *ghist;
+      double *glgv;
+      bdeu_stage(dyn, G4, a.bitw, wave, gi, &gmst, &ghist, &glgv);
+      for (int fg0 = wave * G4; fg0 < 2 * NF; fg0 += EDDY_NWAVES * G4) {
+        const int fg = fg0 + gi;
+        bdeu_group(a, im, G, fg < 2 * NF ? (fg >> 1) : -1, fg & 1, gl, GS, gmst, ghist, glgv, st_hist, st_bins);
       }
-      wave_sync();
-      if (live) {
-        for (int c = gl; c < q * r; c += GS) {
-          const int j = div_small(c, r), kk = c - j * r;
-          const int j1 = div_small(j, n1s), i1 = j - j1 * n1s;
-          const int i3 = div_small(j1, n2s), i2 = j1 - i3 * n2s;
-          const uint64_t *m0 = gmst + (0 * 3 + nth_state(f0, kk)) * W;
-          const uint64_t *m1 = gmst + (1 * 3 + nth_state(f1, i1)) * W;
-          const uint64_t *m2 = gmst + (2 * 3 + nth_state(f2, i2)) * W;
-          const uint64_t *m3 = gmst + (3 * 3 + nth_state(f3, i3)) * W;
-          int cnt = 0;
-          for (int w = 0; w < W; w++) {
-            uint64_t x = m0[w] & m1[w] & m2[w];
-            if (np > 2) x &= m3[w];
-            cnt += __popcll(x);
-          }
-          ghist[c] = (uint16_t)cnt;
-        }
-      }
-      wave_sync();
-      double lq0 = 0.0;
-      const double *lq = a.lgtab, *lqr = a.lgtab;
-      if (live) {
-        lq = a.lgtab + (size_t)q * a.lgstride;
-        lqr = a.lgtab + (size_t)(q * r) * a.lgstride;
-        lq0 = lq[0];
-        const double lqr0 = lqr[0];
-        for (int j = gl; j < q; j += GS) {
-          int nij = 0;
-          double sum1 = 0.0;
-          for (int k = 0; k < r; k++) {
-            const int h = ghist[j * r + k];
-            nij += h;
-            sum1 = sum1 + (lqr[h] - lqr0);
-          }
-          glgv[j] = (lq0 - lq[nij]) + sum1;
-        }
-      }
-      wave_sync();
-      if (mine && gl == 0) {
-        double sum2 = 0.0;
-        if (live)
-          for (int j = 0; j < q; j++) sum2 = sum2 + glgv[j];
-        fscore[f * 2 + g] = sum2;
-        st_hist += (unsigned long long)(1 + np) * (unsigned long long)(g ? a.n2 : a.n1);
-        if (live) st_bins += (unsigned long long)(4 * q * r);
-      }
-      wave_sync();
     }
   }
   if (a.counters) {
@@ -1220,8 +1265,36 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
       atomicAdd(&a.counters[8], sh_st[1]);  // histogram bin bytes
     }
   }
+  // what the tail kernel needs besides the family scores
+  for (int u = tid; u < UU; u += 256) a.sc_ulist[im.soff + u] = ulist[u];
+  for (int x = tid; x < S; x += 256) a.sc_evst[im.soff + x] = evst[x];
+  for (int i = tid; i < 2 * G; i += 256) a.sc_famfull[im.goff * 2 + i] = famfull[i];
   __syncthreads();
   phase_mark(a, 4, tph);
+}
+
+// Network scores, likelihoods and JS of every item whose score kernel ran to
+// the end, once every BDeu task of the launch has finished.  One workgroup per
+// item; the unique-network list, event ranges and full families come from
+// the score kernel through global memory.
+__global__ __launch_bounds__(256) void k_score_tail(KArgs a) {
+  const int item = a.order[blockIdx.x];
+  if (a.status[item] & (ST_FAM_OVERFLOW | ST_COLLISION)) return;
+  const ItemMeta im = a.items[item];
+  const SetMeta sm = a.sets[im.set];
+  const int G = sm.G, S = a.S, n1 = a.n1;
+  const int tid = threadIdx.x;
+  extern __shared__ __align__(16) unsigned char dyn[];
+  const int UU = a.uniq_u[item];
+  const int E = min(a.evcount[item], im.evcap);
+  const int *es = a.evorder + im.evoff;
+  const Event *ev = a.ev + im.evoff;
+  const int *ulist = a.sc_ulist + im.soff;
+  const int *evst = a.sc_evst + im.soff;
+  const int *famfull = a.sc_famfull + im.goff * 2;
+  const double *fscore = a.famscore + im.foff * 2;
+  unsigned long long tph = 0;
+  phase_mark(a, -1, tph);
   // ---- 5. network scores, in node order (main.cu:1335-1342) -------------
   // base[(gs * 2 + g) * G + t]: the score in class g of row t's family in
   // the full network of class gs (every network's value outside its event
@@ -1237,7 +1310,7 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   for (int ug = tid; ug < 2 * UU; ug += 256) {
     const int u = ug >> 1, g = ug & 1;
     const int s = ulist[u];
-    const int b = evst[s], ne = evst[s + 1] - b;
+    const int b = evst[s], ne = (s + 1 < S ? evst[s + 1] : E) - b;
     const double *bs = base + (net_class(s, n1) * 2 + g) * G;
     int ep = 0, te = ne > 0 ? ev[es[b]].t : G;
     double d = 0.0;
@@ -1316,6 +1389,46 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
     a.js[im.set * a.perms + im.perm] = js / eddy_log(2.0);
   }
   phase_mark(a, 6, tph);
+}
+
+// BDeu of every (family, class) task the launch's score kernels listed, in
+// lane groups (G4 per wave, as in the score kernel), the blocks striding over
+// the list.  Each task writes its own family score.
+__global__ __launch_bounds__(256) void k_bdeu(KArgs a) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int G4 = a.score_groups, GS = 64 / G4, gi = lane / GS, gl = lane - gi * GS;
+  extern __shared__ __align__(16) unsigned char dyn[];
+  uint64_t *gmst;
+  uint16_t *ghist;
+  double *glgv;
+  bdeu_stage(dyn, G4, a.bitw, wave, gi, &gmst, &ghist, &glgv);
+  const long long ntask = min((long long)*a.bd_count, (long long)a.bd_cap);
+  unsigned long long st_hist = 0, st_bins = 0;
+  const long long per_wave = G4, stride = (long long)gridDim.x * EDDY_NWAVES * G4;
+  for (long long t0 = ((long long)blockIdx.x * EDDY_NWAVES + wave) * per_wave; t0 < ntask; t0 += stride) {
+    const long long tk = t0 + gi;
+    int item = 0, fg = -2;
+    if (tk < ntask) {
+      const int2 w = a.bd_tasks[tk];
+      item = w.x;
+      fg = w.y;
+    }
+    const ItemMeta im = a.items[item];
+    const int G = a.sets[im.set].G;
+    bdeu_group(a, im, G, fg >= 0 ? (fg >> 1) : -1, fg & 1, gl, GS, gmst, ghist, glgv, st_hist, st_bins);
+  }
+  if (a.counters) {
+    __shared__ unsigned long long sh_st[2];
+    if (tid == 0) sh_st[0] = sh_st[1] = 0ULL;
+    __syncthreads();
+    if (st_hist) atomicAdd(&sh_st[0], st_hist);
+    if (st_bins) atomicAdd(&sh_st[1], st_bins);
+    __syncthreads();
+    if (tid == 0) {
+      atomicAdd(&a.counters[4], sh_st[0]);
+      atomicAdd(&a.counters[8], sh_st[1]);
+    }
+  }
 }
 
 // --------------------------------------------------------------------------
