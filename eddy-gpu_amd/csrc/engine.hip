@@ -616,6 +616,10 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   a.sc_ulist = (int *)sl.sculist.p;
   a.sc_evst = (int *)sl.scevst.p;
   a.sc_famfull = (int *)sl.scfamfull.p;
+  // tail kernel LDS: base scores (4 maxG), then the KL terms (2 S), in one
+  // region; the network scores / likelihoods (2 S) after it
+  a.tail_shared = (int)(std::max<size_t>((size_t)4 * maxG, (size_t)2 * S) * 8);
+  const size_t tail_lds = (size_t)a.tail_shared + (size_t)2 * S * 8;
 
   for (int k = 0; k < 5; k++)
     if (!sl.tev[k]) HIPCHK(ctx, hipEventCreate(&sl.tev[k]));
@@ -644,7 +648,6 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   const size_t score_lds = score_shared + (size_t)(4 * S + 1 + 2 * maxG) * 4;
   k_score<<<nitems, 256, score_lds, st>>>(a);
   k_bdeu<<<2048, 256, bdeu_lds, st>>>(a);
-  const size_t tail_lds = std::max<size_t>((size_t)4 * maxG * 8, (size_t)2 * S * 8);
   k_score_tail<<<nitems, 256, tail_lds, st>>>(a);
   hipEventRecord(sl.tev[4], st);
   HIPCHK(ctx, hipGetLastError());

@@ -135,6 +135,7 @@ struct KArgs {
   int *sc_ulist;           // [soff + u] unique networks in order
   int *sc_evst;            // [soff + s] first event of network s (network order)
   int *sc_famfull;         // [goff*2 + g*G + t] family of row t in class g's full network
+  int tail_shared;         // bytes of the tail kernel's region for base scores, then KL terms
 };
 
 enum {

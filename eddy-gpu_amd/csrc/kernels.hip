@@ -1306,7 +1306,9 @@ __global__ __launch_bounds__(256) void k_score_tail(KArgs a) {
     base[i] = f >= 0 ? fscore[f * 2 + g] : eddy_nan();
   }
   __syncthreads();
-  double *dist = a.lval + im.soff * 2;  // [g*S + u] (overwritten by the likelihoods below)
+  // network scores, then (in place) the likelihoods, in LDS after the
+  // phase-shared region: [g*S + u]
+  double *dist = (double *)(dyn + a.tail_shared);
   for (int ug = tid; ug < 2 * UU; ug += 256) {
     const int u = ug >> 1, g = ug & 1;
     const int s = ulist[u];
@@ -1343,7 +1345,7 @@ __global__ __launch_bounds__(256) void k_score_tail(KArgs a) {
     sh_scale[g] = 10 / (mx - mn);
   }
   __syncthreads();
-  double *lk = a.lval + im.soff * 2;  // in place: dist -> exp(adjusted)
+  double *lk = dist;  // in place: dist -> exp(adjusted)
   for (int ug = tid; ug < 2 * UU; ug += 256) {
     int g = ug / UU, u = ug - g * UU;
     lk[g * S + u] = eddy_exp((dist[g * S + u] + sh_inalpha[g]) * sh_scale[g]);
@@ -1379,6 +1381,12 @@ __global__ __launch_bounds__(256) void k_score_tail(KArgs a) {
     double m = (p1 + p2) / 2;
     double pp = g ? p2 : p1;
     kt[ug] = (m == 0 || pp == 0) ? 0.0 : pp * eddy_log(pp / m);
+  }
+  // the likelihoods for the permutation-0 captures
+  double *lv = a.lval + im.soff * 2;
+  for (int ug = tid; ug < 2 * UU; ug += 256) {
+    const int g = ug / UU, u = ug - g * UU;
+    lv[g * S + u] = lk[g * S + u];
   }
   __syncthreads();
   if (tid == 0) {
