@@ -74,7 +74,7 @@ struct Chunk {
 struct Slot {
   DevBuf items, labels, tiles, info, tab, lfull, lcnt, sflags, bits, ev, evcount, hnet, rep, clsg, evorder, evfam,
       famkey, famscore, famht, famhv, lval, uniq, famcount, status, tests, order, slices, bdtask, bdcount, sculist,
-      scevst, scfamfull;
+      scevst, scfamfull, fbits;
   hipStream_t st = nullptr;
   hipEvent_t tev[5] = {};
   int *h_int = nullptr;  // pinned: status, events, families, unique networks (4 x h_cap)
@@ -243,7 +243,8 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
     DevBuf *sb[] = {&sl.items, &sl.labels, &sl.tiles, &sl.info, &sl.tab, &sl.lfull, &sl.lcnt, &sl.sflags, &sl.bits,
                     &sl.ev, &sl.evcount, &sl.hnet, &sl.rep, &sl.clsg, &sl.evorder, &sl.evfam, &sl.famkey,
                     &sl.famscore, &sl.famht, &sl.famhv, &sl.lval, &sl.uniq, &sl.famcount, &sl.status, &sl.tests,
-                    &sl.order, &sl.slices, &sl.bdtask, &sl.bdcount, &sl.sculist, &sl.scevst, &sl.scfamfull};
+                    &sl.order, &sl.slices, &sl.bdtask, &sl.bdcount, &sl.sculist, &sl.scevst, &sl.scfamfull,
+                    &sl.fbits};
     for (DevBuf *b : sb)
       if (b->p) hipFree(b->p);
     if (sl.h_int) hipHostFree(sl.h_int);
@@ -527,6 +528,10 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   CHK(grow(ctx, sl.sculist, sizeof(int) * ch.sslots));
   CHK(grow(ctx, sl.scevst, sizeof(int) * ch.sslots));
   CHK(grow(ctx, sl.scfamfull, sizeof(int) * 2 * ch.gslots));
+  const int fbw = std::max(1, (maxG + 31) / 32);
+  int fbwp = 1;
+  while (fbwp < fbw) fbwp *= 2;
+  CHK(grow(ctx, sl.fbits, sizeof(uint32_t) * 2 * (size_t)ch.gslots * fbw));
   if (sl.h_cap < nitems) {
     if (sl.h_int) hipHostFree(sl.h_int);
     if (sl.h_tests) hipHostFree(sl.h_tests);
@@ -619,6 +624,9 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   // tail kernel LDS: base scores (4 maxG), then the KL terms (2 S), in one
   // region; the network scores / likelihoods (2 S) after it
   a.tail_shared = (int)(std::max<size_t>((size_t)4 * maxG, (size_t)2 * S) * 8);
+  a.fbits = (uint32_t *)sl.fbits.p;
+  a.fbw = fbw;
+  a.fbwp = fbwp;
   const size_t tail_lds = (size_t)a.tail_shared + (size_t)2 * S * 8;
 
   for (int k = 0; k < 5; k++)

@@ -136,6 +136,8 @@ struct KArgs {
   int *sc_evst;            // [soff + s] first event of network s (network order)
   int *sc_famfull;         // [goff*2 + g*G + t] family of row t in class g's full network
   int tail_shared;         // bytes of the tail kernel's region for base scores, then KL terms
+  uint32_t *fbits;         // [(goff*2 + g*G + t) * fbw + w]: full edges of row t, class g, columns 32w..32w+31
+  int fbw, fbwp;           // words per row (ceil(largest G / 32)) and the power of two >= it
 };
 
 enum {

@@ -253,10 +253,14 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
 
   const double cpairs = (double)sm.c;
   int ntests = 0;
+  // idx = li * 32 + lj: a wave covers rows li, li + 1 of the block, lanes
+  // 0..31 and 32..63; its ballots of the full decisions are the two rows'
+  // full-edge words of column block bj (the loo kernel's plan reads them)
   for (int idx = tid; idx < K1_TB * K1_TB; idx += 256) {
     int li = idx >> 5, lj = idx & 31;
     int gi = bi * K1_TB + li, gj = bj * K1_TB + lj;
-    if (gi >= G || gj >= gi) continue;
+    int f1 = 0, f2 = 0;
+    if (gi < G && gj < gi) {
     int t1[9], t2[9];
 #pragma unroll
     for (int r = 0; r < 3; r++)
@@ -268,9 +272,8 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
     long long p = pair_index(gi, gj);
     int pr = prior_bit(a, sm, p);
     if (a.feat && im.perm == 0) pair_features(t1, t2, pr, sm.c, a.feat + (sm.pairbase + p) * 12);
-    if (a.feat_only) continue;
+    if (!a.feat_only) {
     uint32_t info = 0;
-    int f1 = 0, f2 = 0;
     const bool logged = im.perm > 0 && im.perm <= a.label_perms;
     if (!a.win || im.perm == 0 || logged || gj <= a.win[sm.genebase + gi]) {
       f1 = d_edge_test(t1, pr, a, sm, cpairs);
@@ -289,6 +292,15 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
       store_tally(a.tab + (im.infooff + p) * EDDY_TAB_STRIDE + EDDY_TAB_CLASS, t2);
     }
     a.info[im.infooff + p] = info;
+    }
+    }
+    const uint64_t b1 = __ballot(f1), b2 = __ballot(f2);
+    if (a.fbits && (lane & 31) == 0 && gi < G) {
+      const int h = lane >> 5;  // 0: row li (lanes 0..31), 1: row li + 1
+      const uint32_t w1 = (uint32_t)(b1 >> (32 * h)), w2 = (uint32_t)(b2 >> (32 * h));
+      a.fbits[(size_t)(im.goff * 2 + gi) * a.fbw + bj] = w1;
+      a.fbits[(size_t)(im.goff * 2 + G + gi) * a.fbw + bj] = w2;
+    }
   }
   atomicAdd(&tcount, ntests);
   __syncthreads();
@@ -377,45 +389,83 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   // (64 columns per wave step).
   // ---- A: plan P = column of the (mp + extra)-th full-test edge, a wave per
   // entry scanning 64 columns per ballot ------------------------------------
-  // Four entries per wave step: their first 64 columns' info words are
-  // loaded together (their latencies overlap), then each entry continues
-  // alone only if it has fewer than `want` full edges there.
-  for (int r0 = wave; r0 < R; r0 += 4 * EDDY_NWAVES) {
-    uint16_t h0[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int r = r0 + u * EDDY_NWAVES, t = r >> 1;
-      h0[u] = (r < R && lane < t) ? *info_half(a, im, pair_index(t, lane), r & 1) : (uint16_t)IH_FEVAL;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int r = r0 + u * EDDY_NWAVES;
-      if (r >= R) break;  // wave-uniform
-      const int t = r >> 1, g = r & 1;
-      const int want = mp + (a.plan_extra ? a.plan_extra[(im.goff * 2 + g * G + t)] : 0);
-      int hits = 0, P = t - 1;
-      for (int base = 0; base < t; base += 64) {
-        const int col = base + lane;
-        int f = 0;
-        if (col < t) {
-          if (base == 0 && (h0[u] & IH_FEVAL)) f = h0[u] & IH_FULL;
-          else f = full_of(a, im, sm, pair_index(t, col), g, cpairs, &ntests);
-        }
-        const uint64_t b = __ballot(f);
-        const int nb = __popcll(b);
-        if (hits + nb >= want) {  // wave-uniform
-          uint64_t x = b;
-          for (int i = 1; i < want - hits; i++) x &= x - 1;
-          P = base + __ffsll((long long)x) - 1;
-          break;
-        }
-        hits += nb;
+  // The count kernel's full-edge words (32 columns of a row and class per
+  // word, valid over the columns whose full test it ran: all of them, or
+  // those up to the row's window) give the plan without reading the info
+  // words: a group of NBWP lanes per entry, one word per lane, a prefix count
+  // of their edges, and the want-th edge's bit.  An entry whose evaluated
+  // columns hold fewer than `want` full edges but do not cover the row
+  // continues below, column by column.
+  {
+    const int NBW = a.fbw, NBWP = a.fbwp, RPW = 64 / NBWP;
+    const int gidx = lane / NBWP, gl = lane - gidx * NBWP;
+    const bool logged = im.perm > 0 && im.perm <= a.label_perms;
+    const bool allev = !a.win || im.perm == 0 || logged;
+    for (int r0 = wave * RPW; r0 < R; r0 += EDDY_NWAVES * RPW) {
+      const int r = r0 + gidx;
+      const bool act = r < R;
+      const int t = act ? (r >> 1) : 0, g = r & 1;
+      const int want = act ? mp + (a.plan_extra ? a.plan_extra[(im.goff * 2 + g * G + t)] : 0) : 1;
+      const int L = !act ? 0 : allev ? t : min(t, max(0, a.win[sm.genebase + t] + 1));
+      const int nw = (L + 31) >> 5;
+      uint32_t wd = 0;
+      if (gl < nw) {
+        wd = a.fbits[(size_t)(im.goff * 2 + g * G + t) * NBW + gl];
+        if (gl == nw - 1 && (L & 31)) wd &= (1u << (L & 31)) - 1u;
       }
-      if (lane == 0) {
+      const int c = __popc(wd);
+      int inc = c;
+      for (int d = 1; d < NBWP; d <<= 1) {
+        const int y = __shfl_up(inc, d, NBWP);
+        if (gl >= d) inc += y;
+      }
+      const int total = __shfl(inc, NBWP - 1, NBWP);
+      if (act && inc >= want && inc - c < want) {
+        uint32_t x = wd;
+        for (int i = 1; i < want - (inc - c); i++) x &= x - 1;
+        const int P = gl * 32 + __ffs(x) - 1;
         plan[r] = P;
         off[r] = P + 1;
         solid[r] = 0;
       }
+      if (act && gl == 0 && total < want) {
+        if (L >= t) {  // the whole row
+          plan[r] = t - 1;
+          off[r] = t;
+          solid[r] = 0;
+        } else {  // continue from column L with `total` full edges so far
+          plan[r] = -2 - L;
+          solid[r] = total;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int r = wave; r < R; r += EDDY_NWAVES) {
+    if (plan[r] > -2) continue;  // wave-uniform
+    const int t = r >> 1, g = r & 1;
+    const int want = mp + (a.plan_extra ? a.plan_extra[(im.goff * 2 + g * G + t)] : 0);
+    const int L = -2 - plan[r];
+    int hits = solid[r], P = t - 1;
+    for (int base = L; base < t; base += 64) {
+      const int col = base + lane;
+      int f = 0;
+      if (col < t) f = full_of(a, im, sm, pair_index(t, col), g, cpairs, &ntests);
+      const uint64_t b = __ballot(f);
+      const int nb = __popcll(b);
+      if (hits + nb >= want) {  // wave-uniform
+        uint64_t x = b;
+        for (int i = 1; i < want - hits; i++) x &= x - 1;
+        P = base + __ffsll((long long)x) - 1;
+        break;
+      }
+      hits += nb;
+    }
+    wave_sync();
+    if (lane == 0) {
+      plan[r] = P;
+      off[r] = P + 1;
+      solid[r] = 0;
     }
   }
   __syncthreads();
