@@ -2239,6 +2239,31 @@ extern "C" int eddy_run_learned(eddy_ctx *ctx, const eddy_set_desc *sets, int ns
   return run_job(ctx, sets, nsets, &a, cfg, rep, results);
 }
 
+extern "C" int eddy_run_repermute(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *params,
+                                  const eddy_set_result *first, int top, int perms, int *chosen,
+                                  eddy_set_result *results, int *nchosen) {
+  if (!ctx || !params || !nchosen || nsets < 0 || top < 0 || perms < 1 || (nsets && (!sets || !first)) ||
+      (top > 0 && (!chosen || !results)))
+    return EDDY_ERR_ARG;
+  std::vector<int> cand;
+  for (int q = 0; q < nsets; q++)
+    if (first[q].status == EDDY_SET_OK && !std::isnan(first[q].pDDN)) cand.push_back(q);
+  std::stable_sort(cand.begin(), cand.end(), [&](int x, int y) { return first[x].pDDN < first[y].pDDN; });
+  if ((int)cand.size() > top) cand.resize(top);
+  std::sort(cand.begin(), cand.end());
+  *nchosen = (int)cand.size();
+  if (cand.empty()) return EDDY_OK;
+  std::vector<eddy_set_desc> sub(cand.size());
+  for (size_t k = 0; k < cand.size(); k++) {
+    sub[k] = sets[cand[k]];
+    chosen[k] = cand[k];
+  }
+  eddy_params p2 = *params;
+  p2.perms = perms;
+  ctx->stream_gen.reset(1);
+  return eddy_run_sets(ctx, sub.data(), (int)sub.size(), &p2, results);
+}
+
 extern "C" int eddy_mlp_set_margin(eddy_ctx *ctx, float kappa) {
   if (!ctx || !(kappa >= 0)) return EDDY_ERR_ARG;
   ctx->mlp_kappa = kappa;

@@ -28,7 +28,7 @@ EXPORTS = [
     "eddy_mlp_init", "eddy_mlp_get_params", "eddy_mlp_set_params", "eddy_mlp_set_margin", "eddy_mlp_forward",
     "eddy_mlp_gradient", "eddy_mlp_train", "eddy_learn_collect", "eddy_run_learned",
     "eddy_edge_loo_mask_host", "eddy_chi2_host", "eddy_stream_class2_batch",
-    "eddy_mlp_order_host",
+    "eddy_mlp_order_host", "eddy_run_repermute",
 ]
 MLP_FEATURES = 12
 MLP_NPARAMS = 5057
@@ -141,6 +141,9 @@ def lib():
                                          C.c_void_p, C.c_longlong, C.POINTER(C.c_longlong)]
         L.eddy_run_learned.argtypes = [C.c_void_p, C.POINTER(SetDesc), C.c_int, C.POINTER(Params),
                                        C.POINTER(LearnCfg), C.POINTER(SetResult), C.POINTER(LearnReport)]
+        L.eddy_run_repermute.argtypes = [C.c_void_p, C.POINTER(SetDesc), C.c_int, C.POINTER(Params),
+                                         C.POINTER(SetResult), C.c_int, C.c_int, C.c_void_p, C.POINTER(SetResult),
+                                         C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -277,6 +280,21 @@ class Engine:
         self.last_perms = params.perms
         self.last_G = [len(g) for g, _ in sets]
         return [res[i] for i in range(len(sets))], rep
+
+    def repermute(self, sets, first, params, top, perms):
+        """eddy_run_repermute: the `top` most significant sets of `first` run
+        again with `perms` permutations on a fresh stream.  Returns
+        (chosen set indices, their results)."""
+        arr, keep = self._descs(sets)
+        fr = (SetResult * max(1, len(sets)))(*first) if len(first) else (SetResult * 1)()
+        chosen = np.zeros(max(1, top), np.int32)
+        res = (SetResult * max(1, top))()
+        n = C.c_int()
+        self._chk(self.L.eddy_run_repermute(self.h, arr, len(sets), C.byref(params), fr, top, perms, _ptr(chosen),
+                                            res, C.byref(n)), "run_repermute")
+        self.last_perms = perms
+        self.last_G = [len(sets[i][0]) for i in chosen[:n.value]]
+        return chosen[:n.value].tolist(), [res[i] for i in range(n.value)]
 
     # ---- learned proposal network
     def mlp_init(self, seed):

@@ -292,6 +292,17 @@ typedef struct {
 int eddy_run_learned(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *params,
                      const eddy_learn_cfg *cfg, eddy_set_result *results, eddy_learn_report *report);
 
+/* Re-permutation driver (no counterpart in the reference, which would be run
+ * again by hand on the chosen sets): the `top` sets with the smallest pDDN in
+ * `first` (status EDDY_SET_OK; ties by set index) run again with `perms`
+ * permutations on a fresh permutation stream (as a new `eddy` process would
+ * draw them), in set order.  chosen[k] receives the k-th re-run set's index,
+ * results[k] its result, *nchosen their count (<= top).  The stream is left
+ * after the re-run's draws. */
+int eddy_run_repermute(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *params,
+                       const eddy_set_result *first, int top, int perms, int *chosen, eddy_set_result *results,
+                       int *nchosen);
+
 /* ---- synthetic workloads (seeded DAG data in the benchmark shapes) ------- */
 typedef struct {
   uint64_t seed;

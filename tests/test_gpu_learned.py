@@ -198,3 +198,25 @@ def test_learned_job_is_exact(engine):
         got = engine.perm0_networks(s, res[s].U0, prm.mp)
         for x, y in zip(base_net[k], got):
             assert np.array_equal(np.asarray(x).view(np.uint8), np.asarray(y).view(np.uint8))
+
+
+def test_repermute_driver_matches_direct_run(engine):
+    """The re-permutation driver reruns the most significant sets (smallest
+    pDDN, status OK) with more permutations on a fresh stream, in set order:
+    the same results as resetting the stream and running those sets directly."""
+    wl = eddy_amd.Workload(3, nsets=30)
+    engine.upload(wl.X, wl.classes)
+    prm = wl.params(perms=20, mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF)
+    print("case: repermute top 6 of 30 sets at r=40", flush=True)
+    engine.reset_stream()
+    first = engine.run_sets(wl.sets, prm)
+    ok = [q for q in range(len(first)) if first[q].status == eddy_amd.SET_OK and not np.isnan(first[q].pDDN)]
+    want = sorted(sorted(ok, key=lambda q: (first[q].pDDN, q))[:6])
+    chosen, res = engine.repermute(wl.sets, first, prm, top=6, perms=40)
+    assert chosen == want
+    got_js = [engine.js(k).copy() for k in range(len(chosen))]
+    engine.reset_stream()
+    direct = engine.run_sets([wl.sets[q] for q in chosen], wl.params(perms=40, mode=eddy_amd.MODE_BATCHED,
+                                                                      schedule=eddy_amd.TESTS_CUTOFF))
+    _same_results((res, got_js), (direct, [engine.js(k).copy() for k in range(len(chosen))]))
+    assert all(r.perms_run in (1, 40) for r in res)
