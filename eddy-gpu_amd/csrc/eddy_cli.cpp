@@ -38,7 +38,7 @@ int main(int argc, char **argv) {
   double priorWeight = 0.5, alphaDDN = 0.05, theta = -1, lambda = 2.0, alphaEdge = 0.05;
   bool pAdjust = true;
   int mode = EDDY_MODE_BATCHED, schedule = EDDY_TESTS_CUTOFF, learned = 1, learn_sets = 100;
-  bool parse_only = false;
+  bool parse_only = false, use_cache = false;
   for (int i = 1; i < argc; i++) {
     std::string a = argv[i];
     const char *nx = i + 1 < argc ? argv[i + 1] : "";
@@ -61,6 +61,7 @@ int main(int argc, char **argv) {
     else if (a == "--no-learned") learned = 0;
     else if (a == "--learn-sets") learn_sets = atoi(nx);
     else if (a == "--parse-only") parse_only = true;
+    else if (a == "--cache") use_cache = true;
   }
   if (input.empty()) fail("Invalid input file entered. Exiting...");
   if (classfile.empty()) fail("Invalid class file entered. Exiting...");
@@ -83,8 +84,14 @@ int main(int argc, char **argv) {
   printf("Multiple testing correction (edge detection) : %s\n", pAdjust ? "true" : "false");
 
   ExprData expr;
-  std::string e = load_expression(input, expr);
-  if (!e.empty()) fail(e);
+  std::string e;
+  if (use_cache && load_expression_cache(input, expr)) {
+    printf("expression cache %s.eddybin\n", input.c_str());
+  } else {
+    e = load_expression(input, expr);
+    if (!e.empty()) fail(e);
+    if (use_cache) save_expression_cache(input, expr);
+  }
   ClassData cls;
   e = load_classes(classfile, expr.samples, cls);
   if (!e.empty()) fail(e);
@@ -97,6 +104,13 @@ int main(int argc, char **argv) {
   printf("Classes: %d %s, %d %s\n", cls.n1, cls.class1.c_str(), cls.n2, cls.class2.c_str());
   for (auto &s : sets) load_prior("PRIORS/", s);
   if (parse_only) {  // inputs as the engine would receive them; no GPU needed
+    uint64_t h = 1469598103934665603ULL;  // FNV-1a of the states and the gene names
+    for (uint8_t v : expr.x) h = (h ^ v) * 1099511628211ULL;
+    for (const auto &gname : expr.genes) {
+      for (char ch : gname) h = (h ^ (uint8_t)ch) * 1099511628211ULL;
+      h = (h ^ 0xFFu) * 1099511628211ULL;
+    }
+    printf("expression hash %016llx\n", (unsigned long long)h);
     for (auto &s : sets) {
       printf("set %s genes %d prior %d :", s.name.c_str(), (int)s.rows.size(), (int)s.prior.size() / 2);
       for (int r : s.rows) printf(" %d", r);

@@ -24,6 +24,8 @@
 #include <fstream>
 #include <unordered_map>
 
+#include <sys/stat.h>
+
 namespace {
 
 // strtok semantics on a private copy: runs of delimiters separate tokens,
@@ -54,45 +56,164 @@ bool read_line(FILE *f, std::string &line) {
   return any;
 }
 
-// sscanf("%d") of a token: leading whitespace, optional sign, digits
-bool parse_int(const std::string &tok, int &v) {
-  return sscanf(tok.c_str(), "%d", &v) == 1;
-}
-
 }  // namespace
 
-std::string load_expression(const std::string &path, ExprData &out) {
-  FILE *f = fopen(path.c_str(), "r");
-  if (!f) return std::string("Expression file is NULL: ") + strerror(errno);
-  std::string line;
-  if (!read_line(f, line)) {
-    fclose(f);
-    return "Expression file is empty";
+// sscanf("%d") on the bytes [p, e) of one token, as the C library reads it:
+// leading white space, an optional sign, at least one digit (the rest of the
+// token is ignored); the number is read as a long (saturating like strtol)
+// and stored into the int
+static bool scan_int(const char *p, const char *e, int &v) {
+  while (p < e && (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t' || *p == '\v' || *p == '\f')) p++;
+  bool neg = false;
+  if (p < e && (*p == '+' || *p == '-')) neg = *p++ == '-';
+  if (p >= e || *p < '0' || *p > '9') return false;
+  unsigned long long x = 0;
+  bool over = false;
+  while (p < e && *p >= '0' && *p <= '9') {
+    const unsigned d = (unsigned)(*p - '0');
+    if (!over && x > (0x7FFFFFFFFFFFFFFFULL - d) / 10) over = true;
+    if (!over) x = x * 10 + d;
+    p++;
   }
-  std::vector<std::string> head = split_tokens(line, '\t');
-  int ns = head.empty() ? 0 : (int)head.size() - 1;
-  std::vector<std::string> snames(head.begin() + (head.empty() ? 0 : 1), head.end());
-  for (int i = 0; i < ns; i++)
-    if (snames[i].size() == 1) ns--;
-  out.samples = ns;
-  std::vector<std::vector<std::string>> rows;
-  while (read_line(f, line)) rows.push_back(split_tokens(line, '\t'));
+  long long l;
+  if (over) l = neg ? (-0x7FFFFFFFFFFFFFFFLL - 1) : 0x7FFFFFFFFFFFFFFFLL;
+  else l = neg ? -(long long)x : (long long)x;
+  v = (int)(uint32_t)(uint64_t)l;
+  return true;
+}
+
+// The whole file in memory, scanned once: lines end at '\n' (kept in the
+// line, as fgets-style reading keeps it), tokens are runs of non-tab bytes
+// (strtok on '\t').  Same rows, values and errors as tokenising each line.
+std::string load_expression(const std::string &path, ExprData &out) {
+  FILE *f = fopen(path.c_str(), "rb");
+  if (!f) return std::string("Expression file is NULL: ") + strerror(errno);
+  std::vector<char> buf;
+  {
+    char tmp[1 << 16];
+    size_t n;
+    while ((n = fread(tmp, 1, sizeof(tmp), f)) > 0) buf.insert(buf.end(), tmp, tmp + n);
+  }
   fclose(f);
-  int ng = (int)rows.size() - 1;
-  if (ng < 0) ng = 0;
-  out.genes.resize(ng);
+  if (buf.empty()) return "Expression file is empty";
+  const char *b0 = buf.data(), *be = b0 + buf.size();
+  // line starts
+  std::vector<size_t> ls;
+  ls.push_back(0);
+  for (size_t i = 0; i < buf.size(); i++)
+    if (b0[i] == '\n' && i + 1 < buf.size()) ls.push_back(i + 1);
+  const size_t nlines = ls.size();
+  auto line_end = [&](size_t k) { return k + 1 < nlines ? b0 + ls[k + 1] : be; };
+  // header: every token after the first is a sample, except 1-byte tokens
+  int ns = 0, ntok = 0;
+  for (const char *p = b0, *e = line_end(0); p < e;) {
+    while (p < e && *p == '\t') p++;
+    if (p >= e) break;
+    const char *q = p;
+    while (q < e && *q != '\t') q++;
+    if (ntok++ > 0 && q - p != 1) ns++;
+    p = q;
+  }
+  out.samples = ns;
+  // data rows: lines 1 .. nlines-1, of which the last is dropped
+  const int ng = nlines >= 2 ? (int)nlines - 2 : 0;
+  out.genes.assign(ng, std::string());
   out.x.assign((size_t)ng * ns, 3);
   for (int g = 0; g < ng; g++) {
-    const auto &r = rows[g];
-    out.genes[g] = r.empty() ? std::string() : r[0];
-    if ((int)r.size() < ns + 1) return "Expression row " + std::to_string(g + 2) + " has too few values";
-    for (int j = 0; j < ns; j++) {
-      int v;
-      if (!parse_int(r[j + 1], v)) return "Expression row " + std::to_string(g + 2) + " has a non-integer value";
-      out.x[(size_t)g * ns + j] = (v >= -1 && v <= 1) ? (uint8_t)(v + 1) : 3;
+    const char *p = b0 + ls[g + 1], *e = line_end(g + 1);
+    int k = 0;  // tokens seen
+    for (const char *c = p; c < e && k <= ns;) {  // enough tokens? (checked before any value)
+      while (c < e && *c == '\t') c++;
+      if (c >= e) break;
+      while (c < e && *c != '\t') c++;
+      k++;
+    }
+    if (k < ns + 1) return "Expression row " + std::to_string(g + 2) + " has too few values";
+    k = 0;
+    uint8_t *row = out.x.data() + (size_t)g * ns;
+    while (p < e && k <= ns) {
+      while (p < e && *p == '\t') p++;
+      if (p >= e) break;
+      const char *q = p;
+      while (q < e && *q != '\t') q++;
+      if (k == 0) {
+        out.genes[g].assign(p, q);
+      } else {
+        int v;
+        if (!scan_int(p, q, v)) return "Expression row " + std::to_string(g + 2) + " has a non-integer value";
+        row[k - 1] = (v >= -1 && v <= 1) ? (uint8_t)(v + 1) : 3;
+      }
+      k++;
+      p = q;
     }
   }
   return "";
+}
+
+// ---- binary cache of a parsed expression file -----------------------------
+// <input>.eddybin: "EDDYBIN1", the source's size and modification time, the
+// sample and gene counts, the gene names (length-prefixed), the states.
+static const char kCacheMagic[8] = {'E', 'D', 'D', 'Y', 'B', 'I', 'N', '1'};
+
+static bool source_stamp(const std::string &path, long long &size, long long &mtime_ns) {
+  struct stat st;
+  if (stat(path.c_str(), &st) != 0) return false;
+  size = (long long)st.st_size;
+  mtime_ns = (long long)st.st_mtim.tv_sec * 1000000000LL + st.st_mtim.tv_nsec;
+  return true;
+}
+
+bool load_expression_cache(const std::string &path, ExprData &out) {
+  long long size, mt;
+  if (!source_stamp(path, size, mt)) return false;
+  FILE *f = fopen((path + ".eddybin").c_str(), "rb");
+  if (!f) return false;
+  char magic[8];
+  long long hdr[4];
+  bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, kCacheMagic, 8) == 0 && fread(hdr, sizeof(long long), 4, f) == 4 &&
+            hdr[0] == size && hdr[1] == mt && hdr[2] >= 0 && hdr[3] >= 0;
+  ExprData d;
+  if (ok) {
+    d.samples = (int)hdr[2];
+    d.genes.resize((size_t)hdr[3]);
+    for (auto &g : d.genes) {
+      uint32_t n;
+      if (fread(&n, 4, 1, f) != 1 || n > (1u << 20)) {
+        ok = false;
+        break;
+      }
+      g.resize(n);
+      if (n && fread(&g[0], 1, n, f) != n) {
+        ok = false;
+        break;
+      }
+    }
+  }
+  if (ok) {
+    d.x.resize((size_t)d.samples * d.genes.size());
+    ok = d.x.empty() || fread(d.x.data(), 1, d.x.size(), f) == d.x.size();
+  }
+  fclose(f);
+  if (ok) out = std::move(d);
+  return ok;
+}
+
+void save_expression_cache(const std::string &path, const ExprData &d) {
+  long long size, mt;
+  if (!source_stamp(path, size, mt)) return;
+  const std::string tmp = path + ".eddybin.tmp";
+  FILE *f = fopen(tmp.c_str(), "wb");
+  if (!f) return;
+  long long hdr[4] = {size, mt, (long long)d.samples, (long long)d.genes.size()};
+  bool ok = fwrite(kCacheMagic, 1, 8, f) == 8 && fwrite(hdr, sizeof(long long), 4, f) == 4;
+  for (const auto &g : d.genes) {
+    const uint32_t n = (uint32_t)g.size();
+    ok = ok && fwrite(&n, 4, 1, f) == 1 && (n == 0 || fwrite(g.data(), 1, n, f) == n);
+  }
+  ok = ok && (d.x.empty() || fwrite(d.x.data(), 1, d.x.size(), f) == d.x.size());
+  ok = fclose(f) == 0 && ok;
+  if (ok) ok = rename(tmp.c_str(), (path + ".eddybin").c_str()) == 0;
+  if (!ok) remove(tmp.c_str());
 }
 
 std::string load_classes(const std::string &path, int samples, ClassData &out) {

@@ -28,6 +28,11 @@ struct GeneSet {
 // Each loader returns an empty string on success, else the reference's error
 // message (the CLI prints it and exits with EXIT_FAILURE).
 std::string load_expression(const std::string &path, ExprData &out);
+// Binary cache of a parsed expression file at <path>.eddybin, valid while the
+// source's size and modification time are unchanged (no counterpart in the
+// reference; the CLI uses it only with --cache).
+bool load_expression_cache(const std::string &path, ExprData &out);
+void save_expression_cache(const std::string &path, const ExprData &d);
 std::string load_classes(const std::string &path, int samples, ClassData &out);
 std::string load_gene_sets(const std::string &path, const ExprData &expr, std::vector<GeneSet> &out);
 void load_prior(const std::string &dir, GeneSet &set);

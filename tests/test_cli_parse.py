@@ -79,3 +79,91 @@ def test_oracle_cli_golden(tmp_path):
     exp = os.path.join(CASE, "expected")
     for fn in os.listdir(exp):
         assert open(tmp_path / fn).read() == open(os.path.join(exp, fn)).read(), fn
+
+
+def _ref_parse(text):
+    """The reference's expression reading (main.cu:255-301) restated: lines
+    keep their newline, tokens are strtok runs of non-tabs, values are
+    sscanf("%d") read as a long and stored into an int, 1-byte header tokens
+    do not count as samples, the last data row is dropped."""
+    import re
+
+    parts = text.split("\n")  # only '\n' ends a line (a '\r' stays in its token)
+    lines = [x + "\n" for x in parts[:-1]] + ([parts[-1]] if parts[-1] else [])
+    head = [t for t in lines[0].split("\t") if t]
+    ns = max(0, len(head) - 1) - sum(1 for t in head[1:] if len(t) == 1)
+    rows = lines[1:]
+    genes, states = [], []
+    for r in rows[:-1]:
+        toks = [t for t in r.split("\t") if t]
+        if len(toks) < ns + 1:
+            return None
+        genes.append(toks[0])
+        for t in toks[1:ns + 1]:
+            m = re.match(r"[ \t\n\r\v\f]*([+-]?)(\d+)", t)
+            if not m:
+                return None
+            v = int(m.group(2)) * (-1 if m.group(1) == "-" else 1)
+            v = max(-(1 << 63), min((1 << 63) - 1, v))
+            v = (v & 0xFFFFFFFF) - (1 << 32 if v & 0x80000000 else 0)
+            states.append(v + 1 if -1 <= v <= 1 else 3)
+    h = 1469598103934665603
+    for s in states:
+        h = ((h ^ s) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    for g in genes:
+        for ch in g.encode():
+            h = ((h ^ ch) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+        h = ((h ^ 0xFF) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return ns, len(genes), h
+
+
+def _hash_of(out):
+    for line in out.splitlines():
+        if line.startswith("expression hash "):
+            return int(line.split()[2], 16)
+    return None
+
+
+def test_expression_reader_matches_reference_semantics(tmp_path):
+    """Random files with the awkward cases (runs of tabs, trailing tabs, CRLF,
+    signs, leading blanks, decimals, out-of-range and huge values, extra
+    columns, a last line without newline) read into the same states and gene
+    names as the restated reference reader."""
+    import random
+
+    rnd = random.Random(5)
+    vals = ["1", "0", "-1", "+1", " -1", "1.7", "2", "-3", "007", "4294967296", "99999999999999999999999", "1\r"]
+    for case in range(25):
+        ns = rnd.randint(2, 9)
+        head = "id" + "".join("\t" * rnd.randint(1, 2) + "S%d" % i for i in range(ns))
+        head += rnd.choice(["", "\t", "\t\t"]) + "\n"
+        body = []
+        for g in range(rnd.randint(2, 12)):
+            toks = ["G%d" % rnd.randint(0, 5)] + [rnd.choice(vals) for _ in range(ns + rnd.randint(0, 2))]
+            body.append("\t".join(t if rnd.random() > 0.1 else t + "\t" for t in toks))
+        text = head + "\n".join(body) + rnd.choice(["", "\n"])
+        want = _ref_parse(text)
+        r = parse(tmp_path, text, "A\tB" + "\tA" * 20 + "\n", "S\tu\tG0\n")
+        if want is None:
+            assert r.returncode != 0, (case, text)
+            continue
+        assert f"{want[0]} samples" in r.stdout and f"{want[1]} genes" in r.stdout, (case, r.stdout)
+        if r.returncode == 0:
+            assert _hash_of(r.stdout) == want[2], (case, text)
+
+
+def test_expression_cache(tmp_path):
+    expr = "id\tS1\tS2\tS3\tS4\nG0\t1\t0\t-1\t1\nG1\t0\t0\t1\t-1\nG2\t1\t1\t1\t0\nG3\t0\t1\t0\t1\n"
+    r1 = parse(tmp_path, expr, "A\tB\tA\tB\n", "S\tu\tG0\tG1\n", extra=("--cache",))
+    assert r1.returncode == 0 and "expression cache" not in r1.stdout
+    assert (tmp_path / "e.txt.eddybin").exists()
+    r2 = subprocess.run([EXE, "-d", "e.txt", "-c", "c.txt", "-g", "g.gmt", "--parse-only", "--cache"],
+                        cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r2.returncode == 0 and "expression cache e.txt.eddybin" in r2.stdout
+    assert _hash_of(r2.stdout) == _hash_of(r1.stdout) and _hash_of(r1.stdout) is not None
+    # a changed source is read again
+    (tmp_path / "e.txt").write_text(expr.replace("G3\t0\t1\t0\t1", "G3\t1\t1\t1\t1") + "G4\t0\t0\t0\t0\n")
+    r3 = subprocess.run([EXE, "-d", "e.txt", "-c", "c.txt", "-g", "g.gmt", "--parse-only", "--cache"],
+                        cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r3.returncode == 0 and "expression cache" not in r3.stdout
+    assert _hash_of(r3.stdout) != _hash_of(r1.stdout)
