@@ -156,6 +156,10 @@ def main():
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--workflow4", action="store_true",
                     help="run config 4's workflow (r=100 over 2000 sets, then r=1000 on the top 200) instead")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of running the autotuner in setup")
+    ap.add_argument("--launch", default="",
+                    help="fixed launch shapes after tuning, e.g. graph=1,loo_waves=3 (results never change)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a kernel of known traffic for "
                          "scaling hardware byte counters)")
@@ -191,6 +195,14 @@ def main():
                              label_perms=args.label_perms)
 
     print("bench: setup %.1fs, %d sets, N=%d" % (setup_s, nsets, wl.N), file=sys.stderr, flush=True)
+    tune = None
+    if not args.no_autotune:
+        chosen, tune = eng.autotune(wl.sets, prm)
+        print("bench: autotune %.0f ms, %d trials: %s" % (tune["tune_ms"], tune["trials"], chosen), file=sys.stderr,
+              flush=True)
+    if args.launch:
+        eng.set_launch_cfg(**{k: int(v) for k, v in (kv.split("=") for kv in args.launch.split(","))})
+    launch = eng.launch_cfg()
     for w in range(args.warmup):
         tw = time.perf_counter()
         run_step(eng, wl, prm, cfg)
@@ -365,6 +377,8 @@ def main():
                         "events": stats[3] // args.steps},
         "phase_wg_ms": [round(x / args.steps, 1) for x in phase],
         "setup_s": setup_s,
+        "launch": launch,
+        "autotune": tune,
         "learned": learned,
         "plain": plain,
         "roofline": roof,

@@ -13,6 +13,9 @@
 //     the first `--learn-sets K` (100) sets train the learned proposal's
 //     network and the rest use it;
 //     `--parse-only` prints the parsed inputs and stops (no GPU);
+//   * batched runs pick their launch shapes with the autotuner first
+//     (`--no-autotune` keeps the defaults; `--tune-cache FILE` keeps the
+//     choice per device and dataset shape across runs);
 //   * a degenerate beta fit (e.g. -r 1), where the reference's Boost call
 //     throws and aborts, prints pDDN as nan and writes no files;
 //   * inputs the reference would read out of bounds (missing class labels,
@@ -38,7 +41,8 @@ int main(int argc, char **argv) {
   double priorWeight = 0.5, alphaDDN = 0.05, theta = -1, lambda = 2.0, alphaEdge = 0.05;
   bool pAdjust = true;
   int mode = EDDY_MODE_BATCHED, schedule = EDDY_TESTS_CUTOFF, learned = 1, learn_sets = 100;
-  bool parse_only = false, use_cache = false;
+  bool parse_only = false, use_cache = false, autotune = true;
+  std::string tune_cache;
   for (int i = 1; i < argc; i++) {
     std::string a = argv[i];
     const char *nx = i + 1 < argc ? argv[i + 1] : "";
@@ -62,6 +66,8 @@ int main(int argc, char **argv) {
     else if (a == "--learn-sets") learn_sets = atoi(nx);
     else if (a == "--parse-only") parse_only = true;
     else if (a == "--cache") use_cache = true;
+    else if (a == "--no-autotune") autotune = false;
+    else if (a == "--tune-cache") tune_cache = nx;
   }
   if (input.empty()) fail("Invalid input file entered. Exiting...");
   if (classfile.empty()) fail("Invalid class file entered. Exiting...");
@@ -143,6 +149,16 @@ int main(int argc, char **argv) {
     desc[i].prior = sets[i].prior.data();
   }
   std::vector<eddy_set_result> out(sets.size());
+  if (autotune && mode == EDDY_MODE_BATCHED && !desc.empty()) {
+    eddy_launch_cfg lcfg{};
+    eddy_tune_report trep{};
+    rc = eddy_autotune(ctx, desc.data(), (int)desc.size(), &p, tune_cache.empty() ? nullptr : tune_cache.c_str(), &lcfg,
+                       &trep);
+    if (rc != EDDY_OK) fail(std::string("autotune failed: ") + eddy_last_error(ctx));
+    fprintf(stderr, "launch shapes (%s, %.0f ms): graph %d tile_order %d item_order %d loo_waves %d bdeu_groups %d "
+            "bdeu_blocks %d\n", trep.from_cache ? "cached" : "tuned", trep.tune_ms, lcfg.graph, lcfg.tile_order,
+            lcfg.item_order, lcfg.loo_waves, lcfg.bdeu_groups, lcfg.bdeu_blocks);
+  }
   if (learned && mode == EDDY_MODE_BATCHED && schedule == EDDY_TESTS_CUTOFF && learn_sets >= 0) {
     eddy_learn_cfg lc{};
     lc.train_sets = learn_sets;

@@ -138,6 +138,7 @@ struct KArgs {
   int tail_shared;         // bytes of the tail kernel's region for base scores, then KL terms
   uint32_t *fbits;         // [(goff*2 + g*G + t) * fbw + w]: full edges of row t, class g, columns 32w..32w+31
   int fbw, fbwp;           // words per row (ceil(largest G / 32)) and the power of two >= it
+  int tile_order;          // count kernel: 1 = each XCD takes a contiguous run of the tile list
 };
 
 enum {

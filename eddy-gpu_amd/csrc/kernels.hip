@@ -162,9 +162,20 @@ __device__ void pair_features(const int *t1, const int *t2, int prior, int c, fl
 
 // PLAIN: the all-tests schedule (every leave-one-out test here too); the
 // cutoff schedule's instance leaves them out and so holds fewer registers
+// Tile of workgroup `bid` of `n`.  Workgroups are dispatched to the 8 XCDs in
+// turn (bid mod 8); with `grouped`, XCD x takes the x-th contiguous run of the
+// tile list instead of every 8th tile, so the tiles of one item (and of the
+// items of one set, which share gene rows) meet in one XCD's L2.  A bijection
+// of 0..n-1: XCD x has q + (x < r) workgroups, q = n / 8, r = n mod 8.
+__device__ __forceinline__ unsigned tile_of(unsigned bid, unsigned n, int grouped) {
+  if (!grouped) return bid;
+  const unsigned x = bid & 7u, k = bid >> 3, q = n >> 3, r = n & 7u;
+  return x * q + (x < r ? x : r) + k;
+}
+
 template <bool PLAIN>
 __device__ __forceinline__ void count_test_body(const KArgs &a) {
-  const uint2 w = ((const uint2 *)a.tiles)[blockIdx.x];
+  const uint2 w = ((const uint2 *)a.tiles)[tile_of(blockIdx.x, gridDim.x, a.tile_order)];
   const int item = (int)w.x;
   const int bi = (int)(w.y >> 16), bj = (int)(w.y & 0xffff);
   const ItemMeta im = a.items[item];
@@ -307,7 +318,10 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
   if (tid == 0 && a.tests) atomicAdd((unsigned long long *)&a.tests[item], (unsigned long long)tcount);
 }
 
-__global__ __launch_bounds__(256) void k_count_test(KArgs a) { count_test_body<false>(a); }
+// 3 waves per SIMD: the MFMA accumulators (72 AGPRs) plus the epilogue fit in 168 registers
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_count_test(KArgs a) {
+  count_test_body<false>(a);
+}
 __global__ __launch_bounds__(256) void k_count_test_all(KArgs a) { count_test_body<true>(a); }
 
 // --------------------------------------------------------------------------
@@ -367,7 +381,7 @@ __device__ __forceinline__ void set_loo(const KArgs &a, const ItemMeta &im, long
 
 __device__ int block_exscan(int *v, int n, int *wsum);
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_loo(KArgs a) {
+__device__ __forceinline__ void loo_body(const KArgs &a) {
   const int item = a.order[blockIdx.x];  // largest items first (shorter tail)
   const ItemMeta im = a.items[item];
   const SetMeta &sm = a.sets[im.set];  // read in place: the bounds are indexed per tally
@@ -555,6 +569,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       atomicAdd((unsigned long long *)&a.counters[3], (unsigned long long)R);        // rows
     }
   }
+}
+
+// two register budgets of the same code: 4 waves per SIMD (128 VGPRs, a few
+// bytes spilled) or 3 (no spills); the launch configuration picks one
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_loo(KArgs a) { loo_body(a); }
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_loo_w3(KArgs a) { loo_body(a); }
+
+// zeroes a slot's per-item counters before its kernels (a kernel rather than
+// memsets, so the launch sequence can be one graph)
+__global__ void k_clear_slot(int *evcount, int *status, long long *tests, unsigned *bdcount, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    evcount[i] = 0;
+    status[i] = 0;
+    tests[i] = 0;
+  }
+  if (i == 0) *bdcount = 0u;
 }
 
 // --------------------------------------------------------------------------

@@ -28,7 +28,7 @@ EXPORTS = [
     "eddy_mlp_init", "eddy_mlp_get_params", "eddy_mlp_set_params", "eddy_mlp_set_margin", "eddy_mlp_forward",
     "eddy_mlp_gradient", "eddy_mlp_train", "eddy_learn_collect", "eddy_run_learned",
     "eddy_edge_loo_mask_host", "eddy_chi2_host", "eddy_stream_class2_batch",
-    "eddy_mlp_order_host", "eddy_run_repermute",
+    "eddy_mlp_order_host", "eddy_run_repermute", "eddy_get_launch_cfg", "eddy_set_launch_cfg", "eddy_autotune",
 ]
 MLP_FEATURES = 12
 MLP_NPARAMS = 5057
@@ -66,6 +66,20 @@ class Timing(C.Structure):
                 ("items", C.c_longlong), ("pairs", C.c_longlong), ("tests", C.c_longlong),
                 ("host_stream_ms", C.c_double), ("sched", C.c_longlong * 4), ("stats", C.c_longlong * 6),
                 ("phase1_ms", C.c_double), ("phase_ms", C.c_double * 16)]
+
+
+class LaunchCfg(C.Structure):
+    """Launch shapes (include/eddy.h eddy_launch_cfg); results never depend on them."""
+    _fields_ = [("graph", C.c_int), ("tile_order", C.c_int), ("item_order", C.c_int), ("loo_waves", C.c_int),
+                ("bdeu_groups", C.c_int), ("bdeu_blocks", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("from_cache", C.c_int), ("trials", C.c_int), ("tune_ms", C.c_double), ("ms_default", C.c_double),
+                ("ms_best", C.c_double), ("items", C.c_longlong), ("pairs", C.c_longlong)]
 
 
 class LearnCfg(C.Structure):
@@ -118,6 +132,10 @@ def lib():
         L.eddy_run_perm_detail.argtypes = [C.c_void_p, C.POINTER(SetDesc), C.c_void_p, C.POINTER(Params),
                                            C.POINTER(PermDetail)]
         L.eddy_get_timing.argtypes = [C.c_void_p, C.POINTER(Timing)]
+        L.eddy_get_launch_cfg.argtypes = [C.c_void_p, C.POINTER(LaunchCfg)]
+        L.eddy_set_launch_cfg.argtypes = [C.c_void_p, C.POINTER(LaunchCfg)]
+        L.eddy_autotune.argtypes = [C.c_void_p, C.POINTER(SetDesc), C.c_int, C.POINTER(Params), C.c_char_p,
+                                    C.POINTER(LaunchCfg), C.POINTER(TuneReport)]
         L.eddy_synth_config.argtypes = [C.c_int, C.POINTER(SynthCfg)]
         L.eddy_synth_dataset.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p]
         L.eddy_synth_sets.argtypes = [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
@@ -405,3 +423,28 @@ class Engine:
         t = Timing()
         self._chk(self.L.eddy_get_timing(self.h, C.byref(t)), "timing")
         return t
+
+    # ---- launch shapes and the autotuner
+    def launch_cfg(self):
+        c = LaunchCfg()
+        self._chk(self.L.eddy_get_launch_cfg(self.h, C.byref(c)), "get_launch_cfg")
+        return c.as_dict()
+
+    def set_launch_cfg(self, **kw):
+        """Changes the named fields of the launch configuration."""
+        c = LaunchCfg()
+        self._chk(self.L.eddy_get_launch_cfg(self.h, C.byref(c)), "get_launch_cfg")
+        for k, v in kw.items():
+            if k not in dict(LaunchCfg._fields_):
+                raise KeyError(k)
+            setattr(c, k, int(v))
+        self._chk(self.L.eddy_set_launch_cfg(self.h, C.byref(c)), "set_launch_cfg")
+
+    def autotune(self, sets, params, cache=None):
+        """eddy_autotune: returns (chosen configuration, report dict)."""
+        arr, keep = self._descs(sets)
+        c, rep = LaunchCfg(), TuneReport()
+        path = None if cache is None else os.fsencode(cache)
+        self._chk(self.L.eddy_autotune(self.h, arr, len(sets), C.byref(params), path, C.byref(c), C.byref(rep)),
+                  "autotune")
+        return c.as_dict(), {k: getattr(rep, k) for k, _ in TuneReport._fields_}

@@ -210,6 +210,48 @@ typedef struct {
 } eddy_timing;
 int eddy_get_timing(eddy_ctx *ctx, eddy_timing *t);
 
+/* ---- launch shapes and the autotuner (no counterpart in the reference,
+ * whose launch shapes are fixed in main.cu:737-1290) ----------------------
+ * Every field changes only how the work is laid on the GPU, never a result:
+ * every configuration gives bit-identical outputs. */
+typedef struct {
+  int graph;        /* 1: each chunk's kernel sequence (clear, count, loo, rows,
+                       score, BDeu, tail, timing events) is one hipGraph per
+                       slot, instantiated once and updated in place for every
+                       chunk; 0: the kernels are launched one by one */
+  int tile_order;   /* count kernel: 0 = workgroup i takes tile i, 1 = each XCD
+                       takes a contiguous run of the tile list */
+  int item_order;   /* per-item kernels: 0 = largest set first, 1 = chunk order */
+  int loo_waves;    /* loo kernel instance: 4 or 3 waves per SIMD */
+  int bdeu_groups;  /* lane groups per wave in the BDeu of 2-3 parent families:
+                       1, 2 or 4; 0 = from the mask length */
+  int bdeu_blocks;  /* persistent workgroups of the BDeu kernel, 64..16384 */
+} eddy_launch_cfg;
+int eddy_get_launch_cfg(eddy_ctx *ctx, eddy_launch_cfg *cfg);
+/* EDDY_ERR_ARG (and no change) for a value outside the ranges above */
+int eddy_set_launch_cfg(eddy_ctx *ctx, const eddy_launch_cfg *cfg);
+
+typedef struct {
+  int from_cache;      /* 1: the configuration came from cache_path */
+  int trials;          /* calibration launches timed */
+  double tune_ms;      /* wall time of the call */
+  double ms_default;   /* calibration chunk, kernels of the starting configuration */
+  double ms_best;      /* the same chunk with the chosen configuration */
+  long long items, pairs; /* the calibration chunk */
+} eddy_tune_report;
+/* Picks the launch configuration for this dataset and these sets: one
+ * calibration chunk (sets sampled across the list, permutations drawn from a
+ * private generator, so the rand() stream is untouched) runs serially under
+ * candidate configurations, knob by knob; each knob keeps the candidate whose
+ * kernels' HIP-event time is at least 2% below the current choice.  The choice
+ * is installed in the context (as eddy_set_launch_cfg) and written to *out.
+ * cache_path (may be NULL): a text file of earlier choices keyed by device,
+ * samples, classes, mp, schedule and largest set; a matching line is used
+ * without calibrating, a new choice is appended.  Results of later runs do not
+ * depend on the choice.  Does not touch the last run's retrievable results. */
+int eddy_autotune(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *params,
+                  const char *cache_path, eddy_launch_cfg *out, eddy_tune_report *report);
+
 /* Host build of the engine's own edge decision for one 3x3 tally
  * (tally9[r*3+k], r = row-gene state): 1 edge, 0 none.  Tests compare it
  * with the CPU oracle; it runs on the CPU and needs no context. */

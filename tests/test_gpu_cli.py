@@ -14,7 +14,8 @@ CASE = os.path.join(ROOT, "tests", "golden", "cli_small")
 EXE = os.path.join(ROOT, "eddy-gpu_amd", "eddy")
 
 
-@pytest.mark.parametrize("extra", [[], ["--mode", "plain", "--tests", "all"], ["--no-learned"], ["--learn-sets", "2"]])
+@pytest.mark.parametrize("extra", [[], ["--mode", "plain", "--tests", "all"], ["--no-learned"], ["--learn-sets", "2"],
+                                   ["--no-autotune"], ["--tune-cache", "shapes.txt"]])
 def test_cli_golden(tmp_path, extra):
     for fn in ("expr.txt", "class.txt", "sets.gmt"):
         shutil.copy(os.path.join(CASE, fn), tmp_path / fn)
