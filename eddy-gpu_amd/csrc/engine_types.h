@@ -107,6 +107,8 @@ struct KArgs {
   int *rep;                // [soff + s] first occurrence of each network
   uint16_t *clsg;          // [soff + k]: class-1 samples, then class-2 samples
   int *evorder;            // [evoff + e]
+  uint32_t *evkey;         // [evoff + e]: s | t << 16 of event e (written with it by the rows kernel)
+  uint16_t *evt;           // [evoff + k]: row of the k-th event in network order (score kernel)
   int *evfam;              // [evoff + e]
   uint64_t *famkey;        // [foff + f]
   unsigned long long *famht;  // [fhoff + i] family hash table keys

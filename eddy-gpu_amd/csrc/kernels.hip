@@ -733,7 +733,7 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
     // candidates: the columns up to the cutoff that are a full edge or whose
     // leave-one-out decisions differ from the full one (flip mask); a
     // leave-one-out network's row list is drawn from these alone
-    int nL = 0, nSolid = 0, nflip = 0, ncand = 0, C = t - 1;
+    int nL = 0, nSolid = 0, nflip = 0, ncand = 0;
     bool done = (t == 0);
     for (int base = 0; base < t && !done; base += 64) {
       int col = base + lane;
@@ -750,7 +750,6 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
         uint64_t x = sm_;
         for (int i = 1; i < need; i++) x &= x - 1;
         limit = __ffsll((long long)x) - 1;
-        C = base + limit;
         done = true;
       }
       bool inRange = valid && lane <= limit;
@@ -824,6 +823,7 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
         ev.pad = 0;
         for (int i = 0; i < EDDY_MAX_MP; i++) ev.par[i] = i < cnt ? nl[i] : 0;
         a.ev[im.evoff + e] = ev;
+        a.evkey[im.evoff + e] = (uint32_t)s | ((uint32_t)t << 16);
       } else {
         atomicOr(&a.status[item], ST_EV_OVERFLOW);
       }
@@ -857,11 +857,12 @@ __device__ __forceinline__ int nth_state(uint32_t f, int c) {
   return c == 0 ? __builtin_ctz(f) : c == 1 ? __builtin_ctz(f & (f - 1)) : 2;
 }
 
-// parent list of row t in network s (events of s sorted by t: es[0..ne))
+// parent list of row t in network s (events of s sorted by row: es[0..ne),
+// their rows et[0..ne))
 __device__ __forceinline__ int row_list(const KArgs &a, const ItemMeta &im, int G, int s, int t, const int *es,
-                                        int ne, int *ep, const uint16_t **out) {
-  while (*ep < ne && a.ev[im.evoff + es[*ep]].t < t) (*ep)++;
-  if (*ep < ne && a.ev[im.evoff + es[*ep]].t == t) {
+                                        const uint16_t *et, int ne, int *ep, const uint16_t **out) {
+  while (*ep < ne && et[*ep] < t) (*ep)++;
+  if (*ep < ne && et[*ep] == t) {
     const Event *e = &a.ev[im.evoff + es[*ep]];
     *out = e->par;
     return e->n;
@@ -1035,10 +1036,8 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   const int item = a.order[blockIdx.x];  // largest items first (shorter tail)
   const ItemMeta im = a.items[item];
   const SetMeta sm = a.sets[im.set];
-  const int G = sm.G, N = a.N, S = a.S, n1 = a.n1;
-  const int NP = (N + 63) & ~63;
+  const int G = sm.G, S = a.S, n1 = a.n1;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint8_t *xs = a.xs + sm.xoff;
   const int HT = a.ht_size;
 
   extern __shared__ __align__(16) unsigned char dyn[];
@@ -1057,6 +1056,8 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
 
   const int E = min(a.evcount[item], im.evcap);
   int *es = a.evorder + im.evoff;
+  uint16_t *et = a.evt + im.evoff;              // rows of es[], in the same order
+  const uint32_t *ek = a.evkey + im.evoff;      // s | t << 16 per event
   const Event *ev = a.ev + im.evoff;
   unsigned long long tph = 0;
   phase_mark(a, -1, tph);
@@ -1074,7 +1075,7 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
     if (h == 0ULL) h = 1ULL;
     if (ht_insert_min(hkeys, hvals, h, s, HT) < 0) atomicOr(&flag, 1);
   }
-  for (int e = tid; e < E; e += 256) atomicAdd(&work[ev[e].s], 1);
+  for (int e = tid; e < E; e += 256) atomicAdd(&work[ek[e] & 0xffffu], 1);
   __syncthreads();
   for (int s = tid; s < S; s += 256) {
     uint64_t h = a.hnet[im.soff + s];
@@ -1091,21 +1092,28 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   for (int s = tid; s < S; s += 256) work[s] = 0;
   __syncthreads();
   for (int e = tid; e < E; e += 256) {
-    int s = ev[e].s;
-    es[evst[s] + atomicAdd(&work[s], 1)] = e;
+    const uint32_t k = ek[e];
+    const int s = (int)(k & 0xffffu);
+    const int pos = evst[s] + atomicAdd(&work[s], 1);
+    es[pos] = e;
+    et[pos] = (uint16_t)(k >> 16);
   }
   __syncthreads();
-  for (int s = tid; s < S; s += 256) {  // sort each network's events by row
+  // sort each network's events by row (a network has at most one event per
+  // row, so the order does not depend on the atomics above)
+  for (int s = tid; s < S; s += 256) {
     int b = evst[s], n = evst[s + 1] - b;
     for (int i = 1; i < n; i++) {
-      int v = es[b + i];
-      int tv = ev[v].t;
+      const int v = es[b + i];
+      const uint16_t tv = et[b + i];
       int j = i - 1;
-      while (j >= 0 && ev[es[b + j]].t > tv) {
+      while (j >= 0 && et[b + j] > tv) {
         es[b + j + 1] = es[b + j];
+        et[b + j + 1] = et[b + j];
         j--;
       }
       es[b + j + 1] = v;
+      et[b + j + 1] = tv;
     }
   }
   __syncthreads();
@@ -1152,9 +1160,10 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
     bool eq = true;
     int ps = 0, pr = 0, ia = 0, ib = 0, id = 0;
     const int *esS = es + evst[s], *esR = es + evst[r];
+    const uint16_t *etS = et + evst[s], *etR = et + evst[r];
     while (eq) {
-      const int ta = ia < ns ? ev[esS[ia]].t : 0x7fffffff;
-      const int tb = ib < nr ? ev[esR[ib]].t : 0x7fffffff;
+      const int ta = ia < ns ? (int)etS[ia] : 0x7fffffff;
+      const int tb = ib < nr ? (int)etR[ib] : 0x7fffffff;
       const int td = id < nd ? drows[id] : 0x7fffffff;
       const int t = min(ta, min(tb, td));
       if (t == 0x7fffffff) break;
@@ -1162,8 +1171,8 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
       ib += tb == t;
       id += td == t;
       const uint16_t *ls, *lr;
-      int cs = row_list(a, im, G, s, t, esS, ns, &ps, &ls);
-      int cr = row_list(a, im, G, r, t, esR, nr, &pr, &lr);
+      int cs = row_list(a, im, G, s, t, esS, etS, ns, &ps, &ls);
+      int cr = row_list(a, im, G, r, t, esR, etR, nr, &pr, &lr);
       if (cs != cr) eq = false;
       for (int i = 0; i < cs && eq; i++) eq = ls[i] == lr[i];
     }
@@ -1369,7 +1378,7 @@ __global__ __launch_bounds__(256) void k_score_tail(KArgs a) {
   const int UU = a.uniq_u[item];
   const int E = min(a.evcount[item], im.evcap);
   const int *es = a.evorder + im.evoff;
-  const Event *ev = a.ev + im.evoff;
+  const uint16_t *et = a.evt + im.evoff;  // rows of es[], sorted per network
   const int *ulist = a.sc_ulist + im.soff;
   const int *evst = a.sc_evst + im.soff;
   const int *famfull = a.sc_famfull + im.goff * 2;
@@ -1395,7 +1404,7 @@ __global__ __launch_bounds__(256) void k_score_tail(KArgs a) {
     const int s = ulist[u];
     const int b = evst[s], ne = (s + 1 < S ? evst[s + 1] : E) - b;
     const double *bs = base + (net_class(s, n1) * 2 + g) * G;
-    int ep = 0, te = ne > 0 ? ev[es[b]].t : G;
+    int ep = 0, te = ne > 0 ? (int)et[b] : G;
     double d = 0.0;
     for (int t = 0; t < G; t++) {
       double v = bs[t];
@@ -1403,7 +1412,7 @@ __global__ __launch_bounds__(256) void k_score_tail(KArgs a) {
         const int f = a.evfam[im.evoff + es[b + ep]];
         v = f >= 0 ? fscore[f * 2 + g] : eddy_nan();
         ep++;
-        te = ep < ne ? ev[es[b + ep]].t : G;
+        te = ep < ne ? (int)et[b + ep] : G;
       }
       d = d + v;
     }
