@@ -2148,6 +2148,7 @@ extern "C" int eddy_autotune(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
     }
   }
   rep.ms_best = total(base);
+  for (int i = 0; i < 6; i++) rep.kernel_ms[i] = base.k[i];
   ctx->timing = keep_timing;
   ctx->hp = keep_hp;
   ctx->k_feat = keep_feat;

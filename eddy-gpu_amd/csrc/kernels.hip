@@ -66,6 +66,11 @@ __device__ __forceinline__ void load_tally(const uint16_t *src, int *t) {
   t[8] = (int)(a2 & 0xffff);
 }
 
+// class g's tally of pair p of the item
+__device__ __forceinline__ uint16_t *tally_at(const KArgs &a, const ItemMeta &im, long long p, int g) {
+  return a.tab + (size_t)(im.infooff + p) * EDDY_TAB_STRIDE + EDDY_TAB_CLASS * g;
+}
+
 __device__ __forceinline__ int prior_bit(const KArgs &a, const SetMeta &sm, long long p) {
   return (a.prior[sm.poff + (p >> 5)] >> (p & 31)) & 1;
 }
@@ -299,8 +304,8 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
       info |= d_loo_mask(t2, f2, pr, a, sm, cpairs, &ntests) << INFO_LOO_SHIFT(1);
       info |= INFO_EVAL(0) | INFO_EVAL(1);
     } else {
-      store_tally(a.tab + (im.infooff + p) * EDDY_TAB_STRIDE, t1);
-      store_tally(a.tab + (im.infooff + p) * EDDY_TAB_STRIDE + EDDY_TAB_CLASS, t2);
+      store_tally(tally_at(a, im, p, 0), t1);
+      store_tally(tally_at(a, im, p, 1), t2);
     }
     a.info[im.infooff + p] = info;
     }
@@ -347,7 +352,7 @@ __global__ __launch_bounds__(256) void k_count_test_all(KArgs a) { count_test_bo
 __device__ __forceinline__ uint32_t loo_of_pair(const KArgs &a, const ItemMeta &im, const SetMeta &sm, long long p,
                                                 int g, int full, double cpairs, int *ntests) {
   int tt[9];
-  load_tally(a.tab + (im.infooff + p) * EDDY_TAB_STRIDE + EDDY_TAB_CLASS * g, tt);
+  load_tally(tally_at(a, im, p, g), tt);
   return d_loo_mask(tt, full, prior_bit(a, sm, p), a, sm, cpairs, ntests);
 }
 
@@ -366,7 +371,7 @@ __device__ __forceinline__ int full_of(const KArgs &a, const ItemMeta &im, const
   const uint16_t h = *ih;
   if (h & IH_FEVAL) return h & IH_FULL;
   int tt[9];
-  load_tally(a.tab + (im.infooff + p) * EDDY_TAB_STRIDE + EDDY_TAB_CLASS * g, tt);
+  load_tally(tally_at(a, im, p, g), tt);
   const int f = d_edge_test(tt, prior_bit(a, sm, p), a, sm, cpairs);
   (*ntests)++;
   *ih = (uint16_t)(h | IH_FEVAL | (f ? IH_FULL : 0u));

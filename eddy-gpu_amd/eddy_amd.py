@@ -79,7 +79,8 @@ class LaunchCfg(C.Structure):
 
 class TuneReport(C.Structure):
     _fields_ = [("from_cache", C.c_int), ("trials", C.c_int), ("tune_ms", C.c_double), ("ms_default", C.c_double),
-                ("ms_best", C.c_double), ("items", C.c_longlong), ("pairs", C.c_longlong)]
+                ("ms_best", C.c_double), ("items", C.c_longlong), ("pairs", C.c_longlong),
+                ("kernel_ms", C.c_double * 6)]
 
 
 class LearnCfg(C.Structure):
@@ -447,4 +448,6 @@ class Engine:
         path = None if cache is None else os.fsencode(cache)
         self._chk(self.L.eddy_autotune(self.h, arr, len(sets), C.byref(params), path, C.byref(c), C.byref(rep)),
                   "autotune")
-        return c.as_dict(), {k: getattr(rep, k) for k, _ in TuneReport._fields_}
+        out = {k: getattr(rep, k) for k, _ in TuneReport._fields_}
+        out["kernel_ms"] = dict(zip(["count", "loo", "rows", "score", "bdeu", "tail"], list(rep.kernel_ms)))
+        return c.as_dict(), out

@@ -238,6 +238,8 @@ typedef struct {
   double ms_default;   /* calibration chunk, kernels of the starting configuration */
   double ms_best;      /* the same chunk with the chosen configuration */
   long long items, pairs; /* the calibration chunk */
+  double kernel_ms[6];  /* the chosen configuration on the calibration chunk (min of
+                           3 serial runs): count, loo, rows, score, BDeu, tail */
 } eddy_tune_report;
 /* Picks the launch configuration for this dataset and these sets: one
  * calibration chunk (sets sampled across the list, permutations drawn from a
