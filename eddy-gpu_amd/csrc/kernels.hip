@@ -75,13 +75,6 @@ __device__ __forceinline__ int prior_bit(const KArgs &a, const SetMeta &sm, long
   return (a.prior[sm.poff + (p >> 5)] >> (p & 31)) & 1;
 }
 
-// bytes equal to v -> 0x01, others 0x00 (byte values 0..3)
-__device__ __forceinline__ uint32_t eq_bytes(uint32_t x, uint32_t v) {
-  uint32_t t = x ^ (v * 0x01010101u);
-  uint32_t z = ~(((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t) & 0x80808080u;
-  return z >> 7;
-}
-
 // per-phase workgroup time (constant-rate wall clock), summed over items into
 // counters[16 + phase] when the engine collects statistics
 __device__ __forceinline__ void phase_mark(const KArgs &a, int phase, unsigned long long &t0) {
@@ -156,17 +149,17 @@ __device__ void pair_features(const int *t1, const int *t2, int prior, int c, fl
 // over all N samples accumulates both classes' tables.
 // --------------------------------------------------------------------------
 #define K1_TB 32
-#define K1_KCH 512
+#define K1_KCH 128
 #define K1_ROWS (2 * K1_TB)
-// LDS strides padded off multiples of the bank width: staged gene rows
+// LDS strides padded off multiples of the bank width: staged operand rows
 // K1_KCH + 16 bytes apart (16 rows read by one MFMA operand load land on
 // distinct banks), tally rows K1_TS u16 apart (the 4 row groups of an
 // accumulator store land on distinct banks)
 #define K1_RS (K1_KCH + 16)
 #define K1_TS 112
+#define K1_OPS (3 * 96 * K1_RS)        // staged operands: A class 1, A class 2, B
+#define K1_TABS (2 * 96 * K1_TS * 2)   // the two classes' tables (u16)
 
-// PLAIN: the all-tests schedule (every leave-one-out test here too); the
-// cutoff schedule's instance leaves them out and so holds fewer registers
 // Tile of workgroup `bid` of `n`.  Workgroups are dispatched to the 8 XCDs in
 // turn (bid mod 8); with `grouped`, XCD x takes the x-th contiguous run of the
 // tile list instead of every 8th tile, so the tiles of one item (and of the
@@ -178,6 +171,18 @@ __device__ __forceinline__ unsigned tile_of(unsigned bid, unsigned n, int groupe
   return x * q + (x < r ? x : r) + k;
 }
 
+// one-hot bytes of four samples' states (0, 1, 2; 3 = missing matches none)
+// from the two low bits of each byte
+__device__ __forceinline__ void one_hot3(uint32_t x, uint32_t &o0, uint32_t &o1, uint32_t &o2) {
+  const uint32_t b0 = x & 0x01010101u, b1 = (x >> 1) & 0x01010101u;
+  const uint32_t n0 = b0 ^ 0x01010101u, n1 = b1 ^ 0x01010101u;
+  o0 = n0 & n1;
+  o1 = b0 & n1;
+  o2 = n0 & b1;
+}
+
+// PLAIN: the all-tests schedule (every leave-one-out test here too); the
+// cutoff schedule's instance leaves them out and so holds fewer registers
 template <bool PLAIN>
 __device__ __forceinline__ void count_test_body(const KArgs &a) {
   const uint2 w = ((const uint2 *)a.tiles)[tile_of(blockIdx.x, gridDim.x, a.tile_order)];
@@ -189,14 +194,48 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
   const int NP = (N + 63) & ~63;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
-  __shared__ __align__(16) uint8_t rows[K1_ROWS * K1_RS > 2 * 96 * K1_TS * 2 ? K1_ROWS * K1_RS : 2 * 96 * K1_TS * 2];
+  // the staged MFMA operands of one sample chunk, then (after the last chunk)
+  // the tables: opA1[96][K1_RS] (row block, class 1), opA2 (class 2), opB
+  // (column block), row = gene * 3 + state
+  __shared__ __align__(16) uint8_t rows[K1_OPS > K1_TABS ? K1_OPS : K1_TABS];
   __shared__ __align__(16) uint8_t clsb[EDDY_MAX_N + 66];
   __shared__ int tcount;
+  uint8_t *opA1 = rows, *opA2 = rows + 96 * K1_RS, *opB = rows + 2 * 96 * K1_RS;
 
   // class-1 membership byte per sample (label bit 0 = class 1)
   const uint32_t *lab = a.labels + im.laboff;
   for (int s = tid; s < NP; s += 256) clsb[s] = (s < N) ? (uint8_t)(((lab[s >> 5] >> (s & 31)) & 1) ^ 1) : 0;
   if (tid == 0) tcount = 0;
+  unsigned long long tph = 0;
+  phase_mark(a, -1, tph);
+  // what the tests read besides the tallies, loaded once while the operands
+  // stage: the tile's prior bits (bit lj of row li = pair (gi, bj * 32 + lj)),
+  // the rows' learned windows and the set's certainty intervals
+  __shared__ uint32_t sh_pb[K1_TB];
+  __shared__ int sh_win[K1_TB];
+  __shared__ eddy_edge_bounds sh_eb;
+  if (tid < K1_TB) {
+    const int gi = bi * K1_TB + tid;
+    uint32_t pb = 0u;
+    int wv = 0x7fffffff;
+    if (gi < G) {
+      const int nval = min(K1_TB, gi - bj * K1_TB);  // columns gj < gi of the tile
+      if (nval > 0) {
+        // bits base .. base + nval - 1, all of them pairs of the set
+        const long long base = pair_index(gi, bj * K1_TB);
+        const uint32_t *pw = a.prior + sm.poff + (base >> 5);
+        const int sh = (int)(base & 31);
+        uint64_t v = pw[0];
+        if (sh + nval > 32) v |= (uint64_t)pw[1] << 32;
+        pb = (uint32_t)(v >> sh);
+      }
+      if (a.win) wv = a.win[sm.genebase + gi];
+    }
+    sh_pb[tid] = pb;
+    sh_win[tid] = wv;
+  } else if (tid >= 64 && tid < 64 + 12) {
+    ((double *)&sh_eb)[tid - 64] = ((const double *)&sm.eb)[tid - 64];
+  }
 
   v4i acc1[3][3], acc2[3][3];
 #pragma unroll
@@ -214,33 +253,47 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
   for (int k0 = 0; k0 < NP; k0 += K1_KCH) {
     const int kch = (NP - k0) < K1_KCH ? (NP - k0) : K1_KCH;
     __syncthreads();
-    // stage 64 gene rows x kch samples (rows 0..31 = block bi, 32..63 = block bj)
+    // stage the one-hot operands of 64 genes x kch samples (genes 0..31 =
+    // block bi, split by class; 32..63 = block bj), 16 samples per task
     const int vecs = kch >> 4;
     for (int v = tid; v < K1_ROWS * vecs; v += 256) {
-      int r = v / vecs, c = v - r * vecs;
-      int gene = (r < K1_TB) ? bi * K1_TB + r : bj * K1_TB + (r - K1_TB);
-      uint4 val = make_uint4(0x03030303u, 0x03030303u, 0x03030303u, 0x03030303u);
-      if (gene < G) val = *(const uint4 *)(xs + (size_t)gene * NP + k0 + c * 16);
-      *(uint4 *)(rows + r * K1_RS + c * 16) = val;
+      const int r = v / vecs, c = v - r * vecs;
+      const int gene = (r < K1_TB) ? bi * K1_TB + r : bj * K1_TB + (r - K1_TB);
+      uint4 x = make_uint4(0x03030303u, 0x03030303u, 0x03030303u, 0x03030303u);
+      if (gene < G) x = *(const uint4 *)(xs + (size_t)gene * NP + k0 + c * 16);
+      uint4 h0, h1, h2;
+      one_hot3(x.x, h0.x, h1.x, h2.x);
+      one_hot3(x.y, h0.y, h1.y, h2.y);
+      one_hot3(x.z, h0.z, h1.z, h2.z);
+      one_hot3(x.w, h0.w, h1.w, h2.w);
+      if (r < K1_TB) {
+        const uint4 m = *(const uint4 *)(clsb + k0 + c * 16);
+        const uint4 m2 = make_uint4(m.x ^ 0x01010101u, m.y ^ 0x01010101u, m.z ^ 0x01010101u, m.w ^ 0x01010101u);
+        uint8_t *d1 = opA1 + (r * 3) * K1_RS + c * 16, *d2 = opA2 + (r * 3) * K1_RS + c * 16;
+        *(uint4 *)(d1) = make_uint4(h0.x & m.x, h0.y & m.y, h0.z & m.z, h0.w & m.w);
+        *(uint4 *)(d1 + K1_RS) = make_uint4(h1.x & m.x, h1.y & m.y, h1.z & m.z, h1.w & m.w);
+        *(uint4 *)(d1 + 2 * K1_RS) = make_uint4(h2.x & m.x, h2.y & m.y, h2.z & m.z, h2.w & m.w);
+        *(uint4 *)(d2) = make_uint4(h0.x & m2.x, h0.y & m2.y, h0.z & m2.z, h0.w & m2.w);
+        *(uint4 *)(d2 + K1_RS) = make_uint4(h1.x & m2.x, h1.y & m2.y, h1.z & m2.z, h1.w & m2.w);
+        *(uint4 *)(d2 + 2 * K1_RS) = make_uint4(h2.x & m2.x, h2.y & m2.y, h2.z & m2.z, h2.w & m2.w);
+      } else {
+        uint8_t *d = opB + ((r - K1_TB) * 3) * K1_RS + c * 16;
+        *(uint4 *)(d) = h0;
+        *(uint4 *)(d + K1_RS) = h1;
+        *(uint4 *)(d + 2 * K1_RS) = h2;
+      }
     }
     __syncthreads();
     for (int kk = 0; kk < kch; kk += 64) {
       const int kofs = kk + (lane >> 4) * 16;
-      const uint4 m = *(const uint4 *)(clsb + k0 + kofs);
       v4i A1[3], A2[3], B[3];
 #pragma unroll
       for (int i = 0; i < 3; i++) {
-        int R = (rt0 + i) * 16 + (lane & 15);
-        int gl = R / 3, st = R - gl * 3;
-        uint4 x = *(const uint4 *)(rows + gl * K1_RS + kofs);
-        uint32_t o0 = eq_bytes(x.x, st), o1 = eq_bytes(x.y, st), o2 = eq_bytes(x.z, st), o3 = eq_bytes(x.w, st);
-        A1[i] = (v4i){(int)(o0 & m.x), (int)(o1 & m.y), (int)(o2 & m.z), (int)(o3 & m.w)};
-        A2[i] = (v4i){(int)(o0 & (m.x ^ 0x01010101u)), (int)(o1 & (m.y ^ 0x01010101u)),
-                      (int)(o2 & (m.z ^ 0x01010101u)), (int)(o3 & (m.w ^ 0x01010101u))};
-        int C = (ct0 + i) * 16 + (lane & 15);
-        int gc = C / 3, sc = C - gc * 3;
-        uint4 y = *(const uint4 *)(rows + (K1_TB + gc) * K1_RS + kofs);
-        B[i] = (v4i){(int)eq_bytes(y.x, sc), (int)eq_bytes(y.y, sc), (int)eq_bytes(y.z, sc), (int)eq_bytes(y.w, sc)};
+        const int R = (rt0 + i) * 16 + (lane & 15);
+        const int C = (ct0 + i) * 16 + (lane & 15);
+        A1[i] = *(const v4i *)(opA1 + R * K1_RS + kofs);
+        A2[i] = *(const v4i *)(opA2 + R * K1_RS + kofs);
+        B[i] = *(const v4i *)(opB + C * K1_RS + kofs);
       }
 #pragma unroll
       for (int i = 0; i < 3; i++)
@@ -252,6 +305,7 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
     }
   }
   __syncthreads();
+  phase_mark(a, 13, tph);
   // accumulators -> LDS tables T[g][R][C] (u16)
   uint16_t *T = (uint16_t *)rows;
 #pragma unroll
@@ -286,22 +340,22 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
         t2[r * 3 + k] = T[96 * K1_TS + (li * 3 + r) * K1_TS + lj * 3 + k];
       }
     long long p = pair_index(gi, gj);
-    int pr = prior_bit(a, sm, p);
+    int pr = (int)((sh_pb[li] >> lj) & 1u);
     if (a.feat && im.perm == 0) pair_features(t1, t2, pr, sm.c, a.feat + (sm.pairbase + p) * 12);
     if (!a.feat_only) {
     uint32_t info = 0;
     const bool logged = im.perm > 0 && im.perm <= a.label_perms;
-    if (!a.win || im.perm == 0 || logged || gj <= a.win[sm.genebase + gi]) {
-      f1 = d_edge_test(t1, pr, a, sm, cpairs);
-      f2 = d_edge_test(t2, pr, a, sm, cpairs);
+    if (!a.win || im.perm == 0 || logged || gj <= sh_win[li]) {
+      f1 = eddy_edge_decision_fast(t1, pr, &a.ep, &sh_eb, cpairs);
+      f2 = eddy_edge_decision_fast(t2, pr, &a.ep, &sh_eb, cpairs);
       ntests += 2;
       info = (f1 ? INFO_FULL(0) : 0u) | (f2 ? INFO_FULL(1) : 0u) | INFO_FEVAL(0) | INFO_FEVAL(1);
       if (a.labhits && im.perm > 0 && (a.label_perms == 0 || logged))
         atomicAdd(&a.labhits[sm.pairbase + p], (unsigned)(f1 + f2));
     }
     if (PLAIN) {
-      info |= d_loo_mask(t1, f1, pr, a, sm, cpairs, &ntests) << INFO_LOO_SHIFT(0);
-      info |= d_loo_mask(t2, f2, pr, a, sm, cpairs, &ntests) << INFO_LOO_SHIFT(1);
+      info |= eddy_loo_mask_fast(t1, f1, pr, &a.ep, &sh_eb, cpairs, &ntests) << INFO_LOO_SHIFT(0);
+      info |= eddy_loo_mask_fast(t2, f2, pr, &a.ep, &sh_eb, cpairs, &ntests) << INFO_LOO_SHIFT(1);
       info |= INFO_EVAL(0) | INFO_EVAL(1);
     } else {
       store_tally(tally_at(a, im, p, 0), t1);
@@ -321,6 +375,7 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
   atomicAdd(&tcount, ntests);
   __syncthreads();
   if (tid == 0 && a.tests) atomicAdd((unsigned long long *)&a.tests[item], (unsigned long long)tcount);
+  phase_mark(a, 14, tph);
 }
 
 // 3 waves per SIMD: the MFMA accumulators (72 AGPRs) plus the epilogue fit in 168 registers

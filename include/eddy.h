@@ -206,7 +206,8 @@ typedef struct {
                                         unique list, 3 families, 4 BDeu, 5 sums,
                                         6 likelihood/JS; rows 7 setup, 8 rows;
                                         loo 10 plan, 11 packed tests, 12 cutoff
-                                        check and fallback */
+                                        check and fallback; count 13 staging
+                                        and MFMA, 14 tests and stores */
 } eddy_timing;
 int eddy_get_timing(eddy_ctx *ctx, eddy_timing *t);
 
