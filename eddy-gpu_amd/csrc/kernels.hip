@@ -250,17 +250,38 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
   const int rt0 = (wave >> 1) * 3;  // this wave's row tiles (of 6)
   const int ct0 = (wave & 1) * 3;   // this wave's column tiles (of 6)
 
+  // raw states of this thread's staging tasks (at most 2 x 16 samples: 64
+  // genes x K1_KCH / 16 tasks over 256 threads) for chunk k0, loaded one chunk
+  // ahead so the loads run under the previous chunk's MFMAs
+  static_assert(K1_ROWS * (K1_KCH / 16) <= 2 * 256, "two staging tasks per thread");
+  auto fetch = [&](int k0n, uint4 *dst) {
+    const int vecsn = ((NP - k0n) < K1_KCH ? (NP - k0n) : K1_KCH) >> 4;
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int v = tid + 256 * j;
+      dst[j] = make_uint4(0x03030303u, 0x03030303u, 0x03030303u, 0x03030303u);
+      if (v < K1_ROWS * vecsn) {
+        const int r = v / vecsn, c = v - r * vecsn;
+        const int gene = (r < K1_TB) ? bi * K1_TB + r : bj * K1_TB + (r - K1_TB);
+        if (gene < G) dst[j] = *(const uint4 *)(xs + (size_t)gene * NP + k0n + c * 16);
+      }
+    }
+  };
+  uint4 nxt[2];
+  fetch(0, nxt);
   for (int k0 = 0; k0 < NP; k0 += K1_KCH) {
     const int kch = (NP - k0) < K1_KCH ? (NP - k0) : K1_KCH;
+    const uint4 cur[2] = {nxt[0], nxt[1]};
     __syncthreads();
     // stage the one-hot operands of 64 genes x kch samples (genes 0..31 =
     // block bi, split by class; 32..63 = block bj), 16 samples per task
     const int vecs = kch >> 4;
-    for (int v = tid; v < K1_ROWS * vecs; v += 256) {
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int v = tid + 256 * j;
+      if (v >= K1_ROWS * vecs) continue;
       const int r = v / vecs, c = v - r * vecs;
-      const int gene = (r < K1_TB) ? bi * K1_TB + r : bj * K1_TB + (r - K1_TB);
-      uint4 x = make_uint4(0x03030303u, 0x03030303u, 0x03030303u, 0x03030303u);
-      if (gene < G) x = *(const uint4 *)(xs + (size_t)gene * NP + k0 + c * 16);
+      const uint4 x = cur[j];
       uint4 h0, h1, h2;
       one_hot3(x.x, h0.x, h1.x, h2.x);
       one_hot3(x.y, h0.y, h1.y, h2.y);
@@ -284,6 +305,7 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
       }
     }
     __syncthreads();
+    if (k0 + K1_KCH < NP) fetch(k0 + K1_KCH, nxt);
     for (int kk = 0; kk < kch; kk += 64) {
       const int kofs = kk + (lane >> 4) * 16;
       v4i A1[3], A2[3], B[3];
