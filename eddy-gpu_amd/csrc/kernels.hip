@@ -1285,11 +1285,29 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   for (int s = tid; s < S; s += 256) a.rep[im.soff + s] = rep[s];
   const int UU = U;
   phase_mark(a, 2, tph);
-  // ---- 3. families of the unique networks (hash table in global memory) --
+  // ---- 3. families of the unique networks (hash table of at least 2x the
+  // families an item can hold: in the phase-shared LDS region, which is free
+  // between the network table and the BDeu stage, when it fits; else in
+  // global memory)
+  // keys inserted: the 2G full-network rows and the events of unique
+  // networks (EU of them)
+  __shared__ int EU;
+  if (tid == 0) EU = 0;
+  __syncthreads();
+  {
+    int eu = 0;
+    for (int u = tid; u < UU; u += 256) eu += evst[ulist[u] + 1] - evst[ulist[u]];
+    if (eu) atomicAdd(&EU, eu);
+  }
+  __syncthreads();
+  int FH = 1024;
+  while (FH < 2 * (2 * G + EU) && FH < im.fhsize) FH *= 2;
   unsigned long long *fk = a.famht + im.fhoff;
   int *fv = a.famhv + im.fhoff;
-  int FH = 1024;
-  while (FH < 2 * (2 * G + E) && FH < im.fhsize) FH *= 2;
+  if ((long long)FH * 12 <= (long long)a.score_shared) {
+    fk = (unsigned long long *)dyn;
+    fv = (int *)(fk + FH);
+  }
   for (int i = tid; i < FH; i += 256) {
     fk[i] = 0ULL;
     fv[i] = 0x7fffffff;
