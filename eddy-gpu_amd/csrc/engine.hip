@@ -86,7 +86,7 @@ struct SlotGraph {
 struct Slot {
   DevBuf items, labels, tiles, info, tab, lfull, lcnt, sflags, bits, ev, evcount, hnet, rep, clsg, evorder, evfam,
       famkey, famscore, famht, famhv, lval, uniq, famcount, status, tests, order, slices, bdtask, bdcount, sculist,
-      scevst, scfamfull, fbits, evkey, evt;
+      scevst, scfamfull, fbits, evkey, evt, evpar;
   hipStream_t st = nullptr;
   // timing events: 0 start, 1 after count, 2 after loo, 3 after rows,
   // 5 after score, 6 after BDeu, 4 after the tail
@@ -262,7 +262,7 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
                     &sl.ev, &sl.evcount, &sl.hnet, &sl.rep, &sl.clsg, &sl.evorder, &sl.evfam, &sl.famkey,
                     &sl.famscore, &sl.famht, &sl.famhv, &sl.lval, &sl.uniq, &sl.famcount, &sl.status, &sl.tests,
                     &sl.order, &sl.slices, &sl.bdtask, &sl.bdcount, &sl.sculist, &sl.scevst, &sl.scfamfull,
-                    &sl.fbits, &sl.evkey, &sl.evt};
+                    &sl.fbits, &sl.evkey, &sl.evt, &sl.evpar};
     for (DevBuf *b : sb)
       if (b->p) hipFree(b->p);
     if (sl.h_int) hipHostFree(sl.h_int);
@@ -533,6 +533,7 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   CHK(grow(ctx, sl.evorder, sizeof(int) * evoff));
   CHK(grow(ctx, sl.evkey, sizeof(uint32_t) * evoff));
   CHK(grow(ctx, sl.evt, sizeof(uint16_t) * evoff));
+  CHK(grow(ctx, sl.evpar, sizeof(uint16_t) * (size_t)(p->mp + 1) * evoff));
   CHK(grow(ctx, sl.evfam, sizeof(int) * evoff));
   CHK(grow(ctx, sl.famkey, sizeof(uint64_t) * foff));
   CHK(grow(ctx, sl.famscore, sizeof(double) * 2 * foff));
@@ -626,6 +627,8 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   a.evorder = (int *)sl.evorder.p;
   a.evkey = (uint32_t *)sl.evkey.p;
   a.evt = (uint16_t *)sl.evt.p;
+  a.evpar = (uint16_t *)sl.evpar.p;
+  a.evw = p->mp + 1;
   a.evfam = (int *)sl.evfam.p;
   a.famkey = (uint64_t *)sl.famkey.p;
   a.famht = (unsigned long long *)sl.famht.p;

@@ -109,6 +109,8 @@ struct KArgs {
   int *evorder;            // [evoff + e]
   uint32_t *evkey;         // [evoff + e]: s | t << 16 of event e (written with it by the rows kernel)
   uint16_t *evt;           // [evoff + k]: row of the k-th event in network order (score kernel)
+  uint16_t *evpar;         // [(evoff + e) * evw]: parent count n, then the n parents of event e
+  int evw;                 // u16 per event in evpar: mp + 1
   int *evfam;              // [evoff + e]
   uint64_t *famkey;        // [foff + f]
   unsigned long long *famht;  // [fhoff + i] family hash table keys

@@ -898,14 +898,28 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
       int s = g == 0 ? 1 + k : a.n1 + 1 + k;
       int e = atomicAdd(&a.evcount[item], 1);
       if (e < im.evcap) {
-        Event ev;
-        ev.s = (uint16_t)s;
-        ev.t = (uint16_t)t;
-        ev.n = (uint16_t)cnt;
-        ev.pad = 0;
-        for (int i = 0; i < EDDY_MAX_MP; i++) ev.par[i] = i < cnt ? nl[i] : 0;
-        a.ev[im.evoff + e] = ev;
+        // compact form for the score kernels; the full record only where
+        // permutation 0's events are captured
         a.evkey[im.evoff + e] = (uint32_t)s | ((uint32_t)t << 16);
+        uint16_t *pp = a.evpar + (size_t)(im.evoff + e) * a.evw;
+        if (a.evw == 4) {  // mp = 3: n and three parents in one 8-byte store
+          uint64_t v = (uint64_t)cnt;
+          for (int i = 0; i < 3; i++)
+            if (i < cnt) v |= (uint64_t)nl[i] << (16 * (i + 1));
+          *(uint64_t *)pp = v;
+        } else {
+          pp[0] = (uint16_t)cnt;
+          for (int i = 0; i < cnt; i++) pp[1 + i] = nl[i];
+        }
+        if (im.perm == 0) {
+          Event ev;
+          ev.s = (uint16_t)s;
+          ev.t = (uint16_t)t;
+          ev.n = (uint16_t)cnt;
+          ev.pad = 0;
+          for (int i = 0; i < EDDY_MAX_MP; i++) ev.par[i] = i < cnt ? nl[i] : 0;
+          a.ev[im.evoff + e] = ev;
+        }
       } else {
         atomicOr(&a.status[item], ST_EV_OVERFLOW);
       }
@@ -945,9 +959,9 @@ __device__ __forceinline__ int row_list(const KArgs &a, const ItemMeta &im, int 
                                         const uint16_t *et, int ne, int *ep, const uint16_t **out) {
   while (*ep < ne && et[*ep] < t) (*ep)++;
   if (*ep < ne && et[*ep] == t) {
-    const Event *e = &a.ev[im.evoff + es[*ep]];
-    *out = e->par;
-    return e->n;
+    const uint16_t *pp = a.evpar + (size_t)(im.evoff + es[*ep]) * a.evw;
+    *out = pp + 1;
+    return pp[0];
   }
   int g = net_class(s, a.n1);
   size_t lb = (size_t)(im.goff * 2 + g * G + t);
@@ -1140,7 +1154,7 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   int *es = a.evorder + im.evoff;
   uint16_t *et = a.evt + im.evoff;              // rows of es[], in the same order
   const uint32_t *ek = a.evkey + im.evoff;      // s | t << 16 per event
-  const Event *ev = a.ev + im.evoff;
+  const uint16_t *epar = a.evpar + (size_t)im.evoff * a.evw;  // n, parents per event
   unsigned long long tph = 0;
   phase_mark(a, -1, tph);
 
@@ -1321,9 +1335,12 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
       size_t lb = (size_t)(im.goff * 2 + g * G + t);
       key = fam_key(t, a.lcnt[lb], a.lfull + lb * EDDY_MAX_MP);
     } else {
-      const Event *e = &ev[i - 2 * G];
-      if (rep[e->s] != e->s) continue;
-      key = fam_key(e->t, e->n, e->par);
+      const int e = i - 2 * G;
+      const uint32_t k = ek[e];
+      const int s = (int)(k & 0xffffu);
+      if (rep[s] != s) continue;
+      const uint16_t *pp = epar + (size_t)e * a.evw;
+      key = fam_key((int)(k >> 16), pp[0], pp + 1);
     }
     if (ht_insert_min(fk, fv, key, 0, FH) < 0) atomicOr(&flag, 4);
   }
@@ -1348,8 +1365,10 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
     famfull[i] = slot >= 0 ? fv[slot] : -1;
   }
   for (int e = tid; e < E; e += 256) {
-    const Event *x = &ev[e];
-    int slot = rep[x->s] == x->s ? ht_find(fk, fam_key(x->t, x->n, x->par), FH) : -1;
+    const uint32_t k = ek[e];
+    const int s = (int)(k & 0xffffu);
+    const uint16_t *pp = epar + (size_t)e * a.evw;
+    int slot = rep[s] == s ? ht_find(fk, fam_key((int)(k >> 16), pp[0], pp + 1), FH) : -1;
     a.evfam[im.evoff + e] = slot >= 0 ? fv[slot] : -1;
   }
   __syncthreads();
