@@ -571,22 +571,66 @@ __device__ __forceinline__ void loo_body(const KArgs &a) {
   __syncthreads();
   // ---- B: packed leave-one-out tests of every planned (pair, class) ------
   // 64 consecutive entries per wave step: one (wave-uniform) binary search
-  // for the step's first entry, then each lane walks to its own row
-  for (int e0 = wave * 64; e0 < total; e0 += 256) {
+  // for the step's first entry, then each lane walks to its own row.  The
+  // next step's entry is located and its info half, prior word and tally
+  // loaded before the current entry's tests, so the loads run under them
+  // (an entry's info half is written only by its own lane in this phase).
+  struct LooEntry {
+    int lo, g, ok;
+    long long p;
+    uint32_t ih, pw;
+    uint64_t q0, q1, q2;
+  };
+  auto locate = [&](int e0, LooEntry &x) {
+    x.ok = 0;
+    if (e0 >= total) return;
     int lo = 0, hi = R - 1;  // last r with off[r] <= e0
     while (lo < hi) {
       int mid = (lo + hi + 1) >> 1;
       if (off[mid] <= e0) lo = mid; else hi = mid - 1;
     }
     const int e = e0 + lane;
-    if (e >= total) continue;
+    if (e >= total) return;
     while (off[lo + 1] <= e) lo++;  // off[R] = total > e ends the walk
     const int t = lo >> 1, g = lo & 1, col = e - off[lo];
     const long long p = pair_index(t, col);
-    const int full = full_of(a, im, sm, p, g, cpairs, &ntests);
-    const uint32_t m = loo_of_pair(a, im, sm, p, g, full, cpairs, &ntests);
-    set_loo(a, im, p, g, m);
-    if (full && m == 0x1FFu) atomicAdd(&solid[lo], 1);
+    x.ok = 1;
+    x.lo = lo;
+    x.g = g;
+    x.p = p;
+    x.ih = *info_half(a, im, p, g);
+    x.pw = a.prior[sm.poff + (p >> 5)];
+    const uint64_t *q = (const uint64_t *)tally_at(a, im, p, g);
+    x.q0 = q[0];
+    x.q1 = q[1];
+    x.q2 = q[2];
+  };
+  LooEntry nx;
+  locate(wave * 64, nx);
+  for (int e0 = wave * 64; e0 < total; e0 += 256) {
+    const LooEntry cu = nx;
+    locate(e0 + 256, nx);
+    if (!cu.ok) continue;
+    int tt[9];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      tt[i] = (int)((cu.q0 >> (16 * i)) & 0xffff);
+      tt[4 + i] = (int)((cu.q1 >> (16 * i)) & 0xffff);
+    }
+    tt[8] = (int)(cu.q2 & 0xffff);
+    const int pr = (int)((cu.pw >> (cu.p & 31)) & 1u);
+    uint32_t h = cu.ih;
+    int full;
+    if (h & IH_FEVAL) {
+      full = (int)(h & IH_FULL);
+    } else {  // outside the learned window or deferred: the full test here
+      full = d_edge_test(tt, pr, a, sm, cpairs);
+      ntests++;
+      h |= IH_FEVAL | (full ? IH_FULL : 0u);
+    }
+    const uint32_t m = d_loo_mask(tt, full, pr, a, sm, cpairs, &ntests);
+    *info_half(a, im, cu.p, cu.g) = (uint16_t)(h | IH_EVAL | (m << IH_LOO_SHIFT));
+    if (full && m == 0x1FFu) atomicAdd(&solid[cu.lo], 1);
   }
   __syncthreads();
   phase_mark(a, 11, tph);
