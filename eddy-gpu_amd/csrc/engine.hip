@@ -83,6 +83,9 @@ struct SlotGraph {
 // Device buffers, stream and pinned readback of one chunk in flight.  Two
 // slots let one chunk's kernels run while the other's tail drains and the
 // host prepares the next.
+#ifndef EDDY_SLOTS
+#define EDDY_SLOTS 2
+#endif
 struct Slot {
   DevBuf items, labels, tiles, info, tab, lfull, lcnt, sflags, bits, ev, evcount, hnet, rep, clsg, evorder, evfam,
       famkey, famscore, famht, famhv, lval, uniq, famcount, status, tests, order, slices, bdtask, bdcount, sculist,
@@ -150,7 +153,7 @@ struct eddy_ctx {
   std::vector<std::vector<double>> last_js;
   std::vector<Perm0> last_p0;  // host copies, fetched from p0 on request
   P0Store p0;
-  Slot slot[2];
+  Slot slot[EDDY_SLOTS];
   BoundsKey bounds_key;
   std::map<int, eddy_edge_bounds> bounds_cache;
   eddy_timing timing{};
@@ -226,13 +229,16 @@ extern "C" int eddy_ctx_create(int device, eddy_ctx **out) {
   ctx->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&ctx->slot[0].st, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&ctx->slot[1].st, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->train_ev, hipEventDisableTiming) != hipSuccess ||
       hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
     delete ctx;
     return EDDY_ERR_HIP;
   }
+  for (Slot &sl : ctx->slot)
+    if (hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) != hipSuccess) {
+      eddy_ctx_destroy(ctx);
+      return EDDY_ERR_HIP;
+    }
   *out = ctx;
   return EDDY_OK;
 }
@@ -1447,14 +1453,14 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     Chunk ch;
     int s = 0, chunk_first = 0, launched_last = -1;
     bool trained = false;
-    hipEvent_t slot_ev[2] = {nullptr, nullptr};
+    hipEvent_t slot_ev[EDDY_SLOTS] = {};
     std::vector<long long> tests0(nsets, 0);  // permutation 0's share of tests_eval
     // Two chunks may be in flight (one per slot): the next one is launched
     // before the older one is finished, so one's kernels overlap the other's
     // tail and the host's bookkeeping.
     std::deque<int> inflight;
     int next_slot = 0, p0_pending = 0;
-    bool slot_p0[2] = {false, false};  // the slot holds a permutation-0 chunk
+    bool slot_p0[EDDY_SLOTS] = {};  // the slot holds a permutation-0 chunk
     auto start_on_slot = [&](Chunk &&c, int first, int last, bool p0) -> int {
       Slot &sl = ctx->slot[next_slot];
       sl.first = first;
@@ -1467,7 +1473,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       if (trace)
         fprintf(stderr, "[eddy] t=%.1f launch slot %d sets %d..%d pairs %lld%s (%.1f ms, replay at %d)\n", now_ms(),
                 next_slot, first, last, sl.ch.pairs, p0 ? " permutation 0" : "", now_ms() - tl, prod.ready.load());
-      next_slot ^= 1;
+      next_slot = (next_slot + 1) % EDDY_SLOTS;
       return EDDY_OK;
     };
     // finishes the oldest chunk; a permutation-0 chunk adds its sets whose
@@ -1573,7 +1579,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       for (int q = 0; q <= nsets; q++) {
         if (q < nsets && meta[q].G >= 2) add_item(c0, meta, q, 0, truebits, N);
         if (!c0.items.empty() && (q == nsets || c0.pairs >= budget_max)) {
-          if (inflight.size() == 2) CHK(finish_oldest(stops));
+          if (inflight.size() == EDDY_SLOTS) CHK(finish_oldest(stops));
           CHK(start_on_slot(std::move(c0), f0, std::min(q, nsets - 1), true));
           c0 = Chunk();
           f0 = q + 1;
@@ -1616,7 +1622,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       // catches up
       const bool starved = s < nsets && prod.ready.load(std::memory_order_acquire) <= s;
       if (!ch.items.empty() && (ch.pairs >= budget || boundary || (starved && ch.pairs >= (budget >> 3)))) {
-        if (inflight.size() == 2) {
+        if (inflight.size() == EDDY_SLOTS) {
           std::vector<int> stops;
           bool rolled;
           CHK(finish_oldest(stops));
@@ -1685,7 +1691,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
                                      st2));
           HIPCHK(ctx, hipMemcpyAsync(ctx->b_mlinv.p, hinv, sizeof(float) * collect_upto, hipMemcpyHostToDevice, st2));
           CHK(mlp_init_async(ctx, hP, st2));
-          for (int k = 0; k < 2; k++) {
+          for (int k = 0; k < EDDY_SLOTS; k++) {
             HIPCHK(ctx, hipEventCreateWithFlags(&slot_ev[k], hipEventDisableTiming));
             HIPCHK(ctx, hipEventRecord(slot_ev[k], ctx->slot[k].st));
             HIPCHK(ctx, hipStreamWaitEvent(st2, slot_ev[k], 0));
@@ -1731,7 +1737,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     }
     prod.stop();
     ctx->timing.host_stream_ms += prod.ms;
-    for (int k = 0; k < 2; k++)
+    for (int k = 0; k < EDDY_SLOTS; k++)
       if (slot_ev[k]) hipEventDestroy(slot_ev[k]);
     for (int q = 0; q < collect_upto; q++) r.tests_collect += tests_eval[q];
     if (training) {

@@ -57,8 +57,9 @@ typedef enum {
 typedef enum {
   EDDY_MODE_PLAIN = 0,   /* one gene set and one permutation per launch
                             sequence, as the reference's loop is structured */
-  EDDY_MODE_BATCHED = 1  /* all sets x permutations of a batch per launch,
-                            replayed from a captured HIP graph */
+  EDDY_MODE_BATCHED = 1  /* all sets x permutations of a batch in chunks of
+                            thousands of items, each chunk's kernels one
+                            HIP graph launch (eddy_launch_cfg.graph) */
 } eddy_mode;
 
 typedef enum {

@@ -163,3 +163,36 @@ def test_mlp_training_order_is_a_sample_and_permutations():
     # refused arguments
     assert L.eddy_mlp_order_host(1, 10, 11, 1, out.ctypes.data_as(C.c_void_p)) != 0
     assert L.eddy_mlp_order_host(1, 10, 5, 65, out.ctypes.data_as(C.c_void_p)) != 0
+
+
+def test_launch_config_is_validated_without_a_gpu():
+    """eddy_set_launch_cfg accepts every documented value and rejects the rest
+    (leaving the configuration as it was); eddy_autotune needs a device."""
+    L, h = host_ctx()
+    try:
+        c = eddy_amd.LaunchCfg()
+        assert L.eddy_get_launch_cfg(h, C.byref(c)) == 0
+        default = c.as_dict()
+        assert default == {"graph": 1, "tile_order": 0, "item_order": 0, "loo_waves": 4, "bdeu_groups": 0,
+                           "bdeu_blocks": 2048}
+        good = [dict(graph=0), dict(tile_order=1), dict(item_order=1), dict(loo_waves=3), dict(bdeu_groups=1),
+                dict(bdeu_groups=2), dict(bdeu_groups=4), dict(bdeu_blocks=64), dict(bdeu_blocks=16384)]
+        bad = [dict(graph=2), dict(tile_order=2), dict(item_order=-1), dict(loo_waves=2), dict(loo_waves=5),
+               dict(bdeu_groups=3), dict(bdeu_groups=8), dict(bdeu_blocks=63), dict(bdeu_blocks=16385)]
+        for kw in good + bad:
+            x = eddy_amd.LaunchCfg(**default)
+            for k, v in kw.items():
+                setattr(x, k, v)
+            rc = L.eddy_set_launch_cfg(h, C.byref(x))
+            got = eddy_amd.LaunchCfg()
+            assert L.eddy_get_launch_cfg(h, C.byref(got)) == 0
+            if kw in good:
+                assert rc == eddy_amd.EDDY_OK and got.as_dict() == dict(default, **kw), kw
+            else:
+                assert rc == eddy_amd.EDDY_ERR_ARG and got.as_dict() == default, kw
+            assert L.eddy_set_launch_cfg(h, C.byref(eddy_amd.LaunchCfg(**default))) == 0
+        prm = eddy_amd.Params(3, 10, 1, 0.05, 0.05, 0.05, eddy_amd.MODE_BATCHED, eddy_amd.TESTS_CUTOFF, 0)
+        rep = eddy_amd.TuneReport()
+        assert L.eddy_autotune(h, None, 0, C.byref(prm), None, C.byref(c), C.byref(rep)) == eddy_amd.EDDY_ERR_NODEV
+    finally:
+        L.eddy_ctx_destroy(h)
