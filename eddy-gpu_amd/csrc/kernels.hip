@@ -812,43 +812,51 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
   __syncthreads();
 
   // observed states per gene per class on the class's full data
-  // (kernel1.cu:338-387), and the class's state masks: one wave per (gene,
-  // class), lanes over the class list, a ballot per state
-  for (int i = wave; i < G * 2; i += EDDY_NWAVES) {
-    const int gene = i >> 1, g = i & 1;
-    const uint8_t *row = xs + (size_t)gene * NP;
+  // (kernel1.cu:338-387), and the class's state masks: per class and chunk of
+  // 8 mask words, each lane holds its samples' ids (the class list is the
+  // same for every gene), then a wave per gene gathers the 8 state bytes
+  // together and takes a ballot per state
+  for (int g = 0; g < 2; g++) {
     const int ng = ncls[g];
     const uint16_t *cl = g ? cls1 : cls0;
-    uint64_t *mk = a.bits + ((size_t)(im.goff * 2 + g * G + gene) * 3) * a.bitw;
-    uint64_t any0 = 0, any1 = 0, any2 = 0;
-    // four words' gathers in flight at once
-    for (int w0 = 0; w0 < a.bitw; w0 += 4) {
-      int v[4];
+    for (int w0 = 0; w0 < a.bitw; w0 += 8) {
+      int sid[8];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
+      for (int u = 0; u < 8; u++) {
         const int k = (w0 + u) * 64 + lane;
-        v[u] = k < ng ? row[cl[k]] : 3;
+        sid[u] = k < ng ? (int)cl[k] : -1;
       }
+      for (int gene = wave; gene < G; gene += EDDY_NWAVES) {
+        const uint8_t *row = xs + (size_t)gene * NP;
+        int v[8];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int w = w0 + u;
-        if (w >= a.bitw) break;  // wave-uniform
-        const uint64_t m0 = __ballot(v[u] == 0), m1 = __ballot(v[u] == 1), m2 = __ballot(v[u] == 2);
-        if (lane == 0) {
-          mk[w] = m0;
-          mk[a.bitw + w] = m1;
-          mk[2 * a.bitw + w] = m2;
+        for (int u = 0; u < 8; u++) v[u] = sid[u] >= 0 ? row[sid[u]] : 3;
+        uint64_t *mk = a.bits + ((size_t)(im.goff * 2 + g * G + gene) * 3) * a.bitw;
+        uint64_t any0 = 0, any1 = 0, any2 = 0;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int w = w0 + u;
+          if (w >= a.bitw) break;  // wave-uniform
+          const uint64_t m0 = __ballot(v[u] == 0), m1 = __ballot(v[u] == 1), m2 = __ballot(v[u] == 2);
+          if (lane == 0) {
+            mk[w] = m0;
+            mk[a.bitw + w] = m1;
+            mk[2 * a.bitw + w] = m2;
+          }
+          any0 |= m0;
+          any1 |= m1;
+          any2 |= m2;
         }
-        any0 |= m0;
-        any1 |= m1;
-        any2 |= m2;
+        // this wave owns the gene in every chunk: a plain read-modify-write
+        if (lane == 0)
+          sflb[g * GM + gene] |= (uint8_t)((any0 ? 1u : 0u) | (any1 ? 2u : 0u) | (any2 ? 4u : 0u));
       }
     }
-    if (lane == 0) {
-      const uint32_t f = (any0 ? 1u : 0u) | (any1 ? 2u : 0u) | (any2 ? 4u : 0u);
-      sflb[g * GM + gene] = (uint8_t)f;
-      a.sflags[im.goff * 2 + g * G + gene] = (uint8_t)f;
-    }
+  }
+  __syncthreads();
+  for (int i = tid; i < 2 * G; i += 256) {
+    const int g = i / G, gene = i - g * G;
+    a.sflags[im.goff * 2 + i] = sflb[g * GM + gene];
   }
   __syncthreads();
 
