@@ -23,10 +23,10 @@ def _labels(N, n1, seed):
     return lab
 
 
-def _compare_perm(engine, wl, s, labels, schedule, check_bits):
+def _compare_perm(engine, wl, s, labels, schedule, check_bits, mp=None):
     genes, pairs = wl.sets[s]
-    prm = wl.params(schedule=schedule, mode=eddy_amd.MODE_PLAIN)
-    print("case: set", s, "G", len(genes), "schedule", schedule, flush=True)
+    prm = wl.params(schedule=schedule, mode=eddy_amd.MODE_PLAIN, mp=mp)
+    print("case: set", s, "G", len(genes), "schedule", schedule, "mp", prm.mp, flush=True)
     g = engine.perm_detail(genes, pairs, labels, prm, bits=check_bits, out5=True)
     data_ref = wl.X.astype(np.int32) - 1
     t1, t2, n1, n2 = oracle_ref.class_split(data_ref, genes, labels)
@@ -51,6 +51,19 @@ def test_perm_parity_config0(engine, schedule):
     for seed in range(3):
         lab = wl.classes if seed == 0 else _labels(N, n1, seed)
         _compare_perm(engine, wl, 0, lab, schedule, check_bits=(schedule == eddy_amd.TESTS_ALL))
+
+
+@pytest.mark.parametrize("mp", [1, 2, 5, 8])
+@pytest.mark.parametrize("schedule", [eddy_amd.TESTS_ALL, eddy_amd.TESTS_CUTOFF])
+def test_perm_parity_other_parent_caps(engine, schedule, mp):
+    """Parent caps other than 3: row lists, events (n and mp parents per
+    event) and families with more parents than BDeu scores (at most 3,
+    kernel5.cu:119-130) against the oracle."""
+    wl = eddy_amd.Workload(0)
+    engine.upload(wl.X, wl.classes)
+    for seed in range(2):
+        lab = wl.classes if seed == 0 else _labels(wl.N, wl.cfg.nA, 10 + seed)
+        _compare_perm(engine, wl, 0, lab, schedule, check_bits=False, mp=mp)
 
 
 def _oracle_threads():
