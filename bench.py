@@ -37,14 +37,17 @@ def ensure_built():
         subprocess.run([sys.executable, os.path.join(ROOT, "__graft_entry__.py")], check=True)
 
 
-def algorithmic_work(sets, N, perms_run):
+def algorithmic_work(sets, N, nA, perms_run):
     """Per-step algorithmic quantities summed over all (set, permutation) items."""
-    ops = 0.0      # one-hot GEMM int8 MACs x2: both classes, lower triangle: 18 c N per item
+    # pairwise counting as a dense 0/1 contraction: 2 * 9 * c * n_A int8 ops per
+    # item (class A's nine one-hot products per pair; class B's table is the
+    # pair total minus class A's)
+    ops = 0.0
     ref_tests = 0  # reference-equivalent CI tests: S c per item
     for (genes, _), r in zip(sets, perms_run):
         G = len(genes)
         c = G * (G - 1) // 2
-        ops += r * 18.0 * c * N
+        ops += r * 18.0 * c * nA
         ref_tests += r * (N + 2) * c
     return ops, ref_tests
 
@@ -239,7 +242,7 @@ def main():
     ms_per_step = 1000.0 * elapsed / args.steps
     value = nsets * args.steps / elapsed
     perms_run = [r.perms_run for r in res]
-    ops, ref_tests = algorithmic_work(wl.sets, wl.N, perms_run)
+    ops, ref_tests = algorithmic_work(wl.sets, wl.N, wl.cfg.nA, perms_run)
     js_head = [eng.js(s).copy() for s in range(nsets)]
     rep = reps[-1]
     learned = {
