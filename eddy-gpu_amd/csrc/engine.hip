@@ -27,6 +27,7 @@
 // kernels (kernels.hip)
 __global__ void k_count_test(KArgs a);
 __global__ void k_count_test_all(KArgs a);
+__global__ void k_count_pooled(KArgs a);
 __global__ void k_loo(KArgs a);
 __global__ void k_loo_w3(KArgs a);
 __global__ void k_clear_slot(int *evcount, int *status, long long *tests, unsigned *bdcount, int n);
@@ -146,7 +147,8 @@ struct eddy_ctx {
   // device buffers, grown on demand
   DevBuf b_sets, b_xs, b_prior, b_items, b_labels, b_tiles, b_info, b_tab, b_lfull, b_lcnt, b_sflags, b_ev,
       b_evcount, b_hnet, b_rep, b_clsg, b_evorder, b_evfam, b_famkey, b_famscore, b_lval, b_uniq, b_js,
-      b_status, b_tests, b_genes, b_rowoff, b_nan0, b_pout, b_famcount, b_counters, b_famht, b_famhv, b_bits, b_order, b_js0;
+      b_status, b_tests, b_genes, b_rowoff, b_nan0, b_pout, b_famcount, b_counters, b_famht, b_famhv, b_bits, b_order, b_js0,
+      b_totals;
   // last run
   int last_perms = 0;
   std::vector<int> last_G;
@@ -260,7 +262,7 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
                    &ctx->b_mx, &ctx->b_my, &ctx->b_mxb, &ctx->b_myb, &ctx->b_midx, &ctx->b_mscr, &ctx->b_mlbase,
                    &ctx->b_mlinv, &ctx->p0.info, &ctx->p0.lcnt, &ctx->p0.lfull, &ctx->p0.rep, &ctx->p0.lval,
                    &ctx->p0.ev, &ctx->p0.famkey, &ctx->p0.famscore, &ctx->p0.slices, &ctx->b_bits,
-                   &ctx->b_win2, &ctx->b_order, &ctx->b_js0};
+                   &ctx->b_win2, &ctx->b_order, &ctx->b_js0, &ctx->b_totals};
   for (DevBuf *b : all)
     if (b->p) hipFree(b->p);
   for (Slot &sl : ctx->slot) {
@@ -622,6 +624,7 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   a.tiles = (const uint32_t *)sl.tiles.p;
   a.info = (uint32_t *)sl.info.p;
   a.tab = (uint16_t *)sl.tab.p;
+  a.totals = (uint16_t *)ctx->b_totals.p;
   a.lfull = (uint16_t *)sl.lfull.p;
   a.lcnt = (uint8_t *)sl.lcnt.p;
   a.sflags = (uint8_t *)sl.sflags.p;
@@ -1169,6 +1172,49 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
                                    nrows, (uint8_t *)ctx->b_xs.p);
     HIPCHK(ctx, hipGetLastError());
   }
+  // the pooled table (all samples, both classes) of every pair of every set:
+  // the cutoff schedule's count kernel takes class 2's table as this minus
+  // class 1's.  One item per set, launches of at most 64M pairs.
+  GROW(b_totals, sizeof(uint16_t) * EDDY_TAB_CLASS * (size_t)std::max(1LL, pairbase));
+  {
+    const std::vector<uint32_t> lab = label_bits(ctx->classOf);
+    Chunk ch;
+    auto run = [&]() -> int {
+      if (ch.items.empty()) return EDDY_OK;
+      GROW(b_items, sizeof(ItemMeta) * ch.items.size());
+      GROW(b_labels, sizeof(uint32_t) * ch.labels.size());
+      GROW(b_tiles, sizeof(uint2) * ch.tiles.size());
+      HIPCHK(ctx, hipMemcpyAsync(ctx->b_items.p, ch.items.data(), sizeof(ItemMeta) * ch.items.size(),
+                                 hipMemcpyHostToDevice, st));
+      HIPCHK(ctx, hipMemcpyAsync(ctx->b_labels.p, ch.labels.data(), sizeof(uint32_t) * ch.labels.size(),
+                                 hipMemcpyHostToDevice, st));
+      HIPCHK(ctx, hipMemcpyAsync(ctx->b_tiles.p, ch.tiles.data(), sizeof(uint2) * ch.tiles.size(),
+                                 hipMemcpyHostToDevice, st));
+      KArgs a{};
+      a.N = N;
+      a.n1 = ctx->n1;
+      a.n2 = ctx->n2;
+      a.S = N + 2;
+      a.sets = (const SetMeta *)ctx->b_sets.p;
+      a.xs = (const uint8_t *)ctx->b_xs.p;
+      a.prior = (const uint32_t *)ctx->b_prior.p;
+      a.items = (const ItemMeta *)ctx->b_items.p;
+      a.labels = (const uint32_t *)ctx->b_labels.p;
+      a.tiles = (const uint32_t *)ctx->b_tiles.p;
+      a.totals = (uint16_t *)ctx->b_totals.p;
+      k_count_pooled<<<(unsigned)ch.tiles.size(), 256, 0, st>>>(a);
+      HIPCHK(ctx, hipGetLastError());
+      HIPCHK(ctx, hipStreamSynchronize(st));
+      ch = Chunk();
+      return EDDY_OK;
+    };
+    for (int s = 0; s < nsets; s++) {
+      if (meta[s].G < 2) continue;
+      add_item(ch, meta, s, 0, lab, N);
+      if (ch.pairs > (64LL << 20)) CHK(run());
+    }
+    CHK(run());
+  }
   HIPCHK(ctx, hipStreamSynchronize(st));
   return EDDY_OK;
 }
@@ -1299,6 +1345,7 @@ int compute_features(eddy_ctx *ctx, const std::vector<SetMeta> &meta, const std:
     a.labels = (const uint32_t *)ctx->b_labels.p;
     a.tiles = (const uint32_t *)ctx->b_tiles.p;
     a.feat = (float *)ctx->b_feat.p;
+    a.totals = (uint16_t *)ctx->b_totals.p;
     k_count_test<<<(unsigned)ch.tiles.size(), 256, 0, st>>>(a);
     HIPCHK(ctx, hipGetLastError());
     HIPCHK(ctx, hipStreamSynchronize(st));

@@ -97,6 +97,7 @@ struct KArgs {
   // per-pair outputs
   uint32_t *info;
   uint16_t *tab;           // [pair][2][EDDY_TAB_CLASS] counts (cutoff mode)
+  uint16_t *totals;        // [set pairbase + p][EDDY_TAB_CLASS]: pooled table of each pair over all samples
   // per-item scratch
   uint16_t *lfull;         // [goff*2 + g*G + t][mp]
   uint8_t *lcnt;           // [goff*2 + g*G + t]

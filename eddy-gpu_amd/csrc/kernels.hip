@@ -181,9 +181,15 @@ __device__ __forceinline__ void one_hot3(uint32_t x, uint32_t &o0, uint32_t &o1,
   o2 = n0 & b1;
 }
 
-// PLAIN: the all-tests schedule (every leave-one-out test here too); the
-// cutoff schedule's instance leaves them out and so holds fewer registers
-template <bool PLAIN>
+// MODE: K1_BOTH both classes on MFMA and every leave-one-out test here too
+// (the all-tests schedule); K1_CLASS1 class 1 on MFMA, class 2's table = the
+// pair's pooled table minus class 1's (the cutoff schedule); K1_POOLED the
+// pooled table of every pair over all samples, written to a.totals (once per
+// set and run: it does not depend on the labels)
+#define K1_BOTH 0
+#define K1_CLASS1 1
+#define K1_POOLED 2
+template <int MODE>
 __device__ __forceinline__ void count_test_body(const KArgs &a) {
   const uint2 w = ((const uint2 *)a.tiles)[tile_of(blockIdx.x, gridDim.x, a.tile_order)];
   const int item = (int)w.x;
@@ -195,12 +201,14 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
   // the staged MFMA operands of one sample chunk, then (after the last chunk)
-  // the tables: opA1[96][K1_RS] (row block, class 1), opA2 (class 2), opB
-  // (column block), row = gene * 3 + state
-  __shared__ __align__(16) uint8_t rows[K1_OPS > K1_TABS ? K1_OPS : K1_TABS];
+  // the tables: opA1[96][K1_RS] (row block: class 1, or every sample), opA2
+  // (class 2, K1_BOTH only), opB (column block), row = gene * 3 + state
+  constexpr int NOPS = MODE == K1_BOTH ? 3 : 2, NTAB = MODE == K1_BOTH ? 2 : 1;
+  constexpr int OPS_B = NOPS * 96 * K1_RS, TAB_B = NTAB * 96 * K1_TS * 2;
+  __shared__ __align__(16) uint8_t rows[OPS_B > TAB_B ? OPS_B : TAB_B];
   __shared__ __align__(16) uint8_t clsb[EDDY_MAX_N + 66];
   __shared__ int tcount;
-  uint8_t *opA1 = rows, *opA2 = rows + 96 * K1_RS, *opB = rows + 2 * 96 * K1_RS;
+  uint8_t *opA1 = rows, *opA2 = rows + 96 * K1_RS, *opB = rows + (NOPS - 1) * 96 * K1_RS;
 
   // class-1 membership byte per sample (label bit 0 = class 1)
   const uint32_t *lab = a.labels + im.laboff;
@@ -288,15 +296,24 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
       one_hot3(x.z, h0.z, h1.z, h2.z);
       one_hot3(x.w, h0.w, h1.w, h2.w);
       if (r < K1_TB) {
-        const uint4 m = *(const uint4 *)(clsb + k0 + c * 16);
-        const uint4 m2 = make_uint4(m.x ^ 0x01010101u, m.y ^ 0x01010101u, m.z ^ 0x01010101u, m.w ^ 0x01010101u);
-        uint8_t *d1 = opA1 + (r * 3) * K1_RS + c * 16, *d2 = opA2 + (r * 3) * K1_RS + c * 16;
-        *(uint4 *)(d1) = make_uint4(h0.x & m.x, h0.y & m.y, h0.z & m.z, h0.w & m.w);
-        *(uint4 *)(d1 + K1_RS) = make_uint4(h1.x & m.x, h1.y & m.y, h1.z & m.z, h1.w & m.w);
-        *(uint4 *)(d1 + 2 * K1_RS) = make_uint4(h2.x & m.x, h2.y & m.y, h2.z & m.z, h2.w & m.w);
-        *(uint4 *)(d2) = make_uint4(h0.x & m2.x, h0.y & m2.y, h0.z & m2.z, h0.w & m2.w);
-        *(uint4 *)(d2 + K1_RS) = make_uint4(h1.x & m2.x, h1.y & m2.y, h1.z & m2.z, h1.w & m2.w);
-        *(uint4 *)(d2 + 2 * K1_RS) = make_uint4(h2.x & m2.x, h2.y & m2.y, h2.z & m2.z, h2.w & m2.w);
+        uint8_t *d1 = opA1 + (r * 3) * K1_RS + c * 16;
+        if (MODE == K1_POOLED) {
+          *(uint4 *)(d1) = h0;
+          *(uint4 *)(d1 + K1_RS) = h1;
+          *(uint4 *)(d1 + 2 * K1_RS) = h2;
+        } else {
+          const uint4 m = *(const uint4 *)(clsb + k0 + c * 16);
+          *(uint4 *)(d1) = make_uint4(h0.x & m.x, h0.y & m.y, h0.z & m.z, h0.w & m.w);
+          *(uint4 *)(d1 + K1_RS) = make_uint4(h1.x & m.x, h1.y & m.y, h1.z & m.z, h1.w & m.w);
+          *(uint4 *)(d1 + 2 * K1_RS) = make_uint4(h2.x & m.x, h2.y & m.y, h2.z & m.z, h2.w & m.w);
+          if (MODE == K1_BOTH) {
+            const uint4 m2 = make_uint4(m.x ^ 0x01010101u, m.y ^ 0x01010101u, m.z ^ 0x01010101u, m.w ^ 0x01010101u);
+            uint8_t *d2 = opA2 + (r * 3) * K1_RS + c * 16;
+            *(uint4 *)(d2) = make_uint4(h0.x & m2.x, h0.y & m2.y, h0.z & m2.z, h0.w & m2.w);
+            *(uint4 *)(d2 + K1_RS) = make_uint4(h1.x & m2.x, h1.y & m2.y, h1.z & m2.z, h1.w & m2.w);
+            *(uint4 *)(d2 + 2 * K1_RS) = make_uint4(h2.x & m2.x, h2.y & m2.y, h2.z & m2.z, h2.w & m2.w);
+          }
+        }
       } else {
         uint8_t *d = opB + ((r - K1_TB) * 3) * K1_RS + c * 16;
         *(uint4 *)(d) = h0;
@@ -314,7 +331,7 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
         const int R = (rt0 + i) * 16 + (lane & 15);
         const int C = (ct0 + i) * 16 + (lane & 15);
         A1[i] = *(const v4i *)(opA1 + R * K1_RS + kofs);
-        A2[i] = *(const v4i *)(opA2 + R * K1_RS + kofs);
+        if (MODE == K1_BOTH) A2[i] = *(const v4i *)(opA2 + R * K1_RS + kofs);
         B[i] = *(const v4i *)(opB + C * K1_RS + kofs);
       }
 #pragma unroll
@@ -322,10 +339,24 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
 #pragma unroll
         for (int j = 0; j < 3; j++) {
           acc1[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A1[i], B[j], acc1[i][j], 0, 0, 0);
-          acc2[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A2[i], B[j], acc2[i][j], 0, 0, 0);
+          if (MODE == K1_BOTH) acc2[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A2[i], B[j], acc2[i][j], 0, 0, 0);
         }
     }
   }
+  // class 2 = pooled - class 1: the pooled table of this thread's next pair
+  // is loaded one pair ahead (the first before the table barrier), so the
+  // loads' latency hides under the tests
+  auto tot_load = [&](int idx, uint64_t &q0, uint64_t &q1, uint64_t &q2) {
+    const int li = idx >> 5, lj = idx & 31, gi = bi * K1_TB + li, gj = bj * K1_TB + lj;
+    if (MODE == K1_CLASS1 && idx < K1_TB * K1_TB && gi < G && gj < gi) {
+      const uint64_t *q = (const uint64_t *)(a.totals + (size_t)(sm.pairbase + pair_index(gi, gj)) * EDDY_TAB_CLASS);
+      q0 = q[0];
+      q1 = q[1];
+      q2 = q[2];
+    }
+  };
+  uint64_t nq0 = 0, nq1 = 0, nq2 = 0;
+  tot_load(tid, nq0, nq1, nq2);
   __syncthreads();
   phase_mark(a, 13, tph);
   // accumulators -> LDS tables T[g][R][C] (u16)
@@ -339,7 +370,7 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
         int R = (rt0 + i) * 16 + 4 * (lane >> 4) + e;
         int C = (ct0 + j) * 16 + (lane & 15);
         T[R * K1_TS + C] = (uint16_t)acc1[i][j][e];
-        T[96 * K1_TS + R * K1_TS + C] = (uint16_t)acc2[i][j][e];
+        if (MODE == K1_BOTH) T[96 * K1_TS + R * K1_TS + C] = (uint16_t)acc2[i][j][e];
       }
   __syncthreads();
 
@@ -349,6 +380,8 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
   // 0..31 and 32..63; its ballots of the full decisions are the two rows'
   // full-edge words of column block bj (the loo kernel's plan reads them)
   for (int idx = tid; idx < K1_TB * K1_TB; idx += 256) {
+    const uint64_t q0 = nq0, q1 = nq1, q2 = nq2;
+    tot_load(idx + 256, nq0, nq1, nq2);
     int li = idx >> 5, lj = idx & 31;
     int gi = bi * K1_TB + li, gj = bj * K1_TB + lj;
     int f1 = 0, f2 = 0;
@@ -357,11 +390,25 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
 #pragma unroll
     for (int r = 0; r < 3; r++)
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
-        t1[r * 3 + k] = T[(li * 3 + r) * K1_TS + lj * 3 + k];
-        t2[r * 3 + k] = T[96 * K1_TS + (li * 3 + r) * K1_TS + lj * 3 + k];
-      }
+      for (int k = 0; k < 3; k++) t1[r * 3 + k] = T[(li * 3 + r) * K1_TS + lj * 3 + k];
     long long p = pair_index(gi, gj);
+    if (MODE == K1_POOLED) {
+      store_tally(a.totals + (size_t)(sm.pairbase + p) * EDDY_TAB_CLASS, t1);
+      continue;  // the loop's ballots below only serve the test modes (fbits is null here)
+    }
+    if (MODE == K1_BOTH) {
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) t2[r * 3 + k] = T[96 * K1_TS + (li * 3 + r) * K1_TS + lj * 3 + k];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        t2[i] = (int)((q0 >> (16 * i)) & 0xffff) - t1[i];
+        t2[4 + i] = (int)((q1 >> (16 * i)) & 0xffff) - t1[4 + i];
+      }
+      t2[8] = (int)(q2 & 0xffff) - t1[8];
+    }
     int pr = (int)((sh_pb[li] >> lj) & 1u);
     if (a.feat && im.perm == 0) pair_features(t1, t2, pr, sm.c, a.feat + (sm.pairbase + p) * 12);
     if (!a.feat_only) {
@@ -375,7 +422,7 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
       if (a.labhits && im.perm > 0 && (a.label_perms == 0 || logged))
         atomicAdd(&a.labhits[sm.pairbase + p], (unsigned)(f1 + f2));
     }
-    if (PLAIN) {
+    if (MODE == K1_BOTH) {
       info |= eddy_loo_mask_fast(t1, f1, pr, &a.ep, &sh_eb, cpairs, &ntests) << INFO_LOO_SHIFT(0);
       info |= eddy_loo_mask_fast(t2, f2, pr, &a.ep, &sh_eb, cpairs, &ntests) << INFO_LOO_SHIFT(1);
       info |= INFO_EVAL(0) | INFO_EVAL(1);
@@ -400,11 +447,11 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
   phase_mark(a, 14, tph);
 }
 
-// 3 waves per SIMD: the MFMA accumulators (72 AGPRs) plus the epilogue fit in 168 registers
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_count_test(KArgs a) {
-  count_test_body<false>(a);
+  count_test_body<K1_CLASS1>(a);
 }
-__global__ __launch_bounds__(256) void k_count_test_all(KArgs a) { count_test_body<true>(a); }
+__global__ __launch_bounds__(256) void k_count_test_all(KArgs a) { count_test_body<K1_BOTH>(a); }
+__global__ __launch_bounds__(256) void k_count_pooled(KArgs a) { count_test_body<K1_POOLED>(a); }
 
 // --------------------------------------------------------------------------
 // K2 (cutoff schedule): leave-one-out tests of each row, class g, up to the
