@@ -151,12 +151,14 @@ def main():
     ap.add_argument("--train-sets", type=int, default=100)
     ap.add_argument("--epochs", type=int, default=1)
     ap.add_argument("--train-rows", type=int, default=65536)
-    ap.add_argument("--margin", type=float, default=1.0)
+    ap.add_argument("--margin", type=float, default=3.0)
     ap.add_argument("--label-perms", type=int, default=10)
     ap.add_argument("--plain-sets", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
+    ap.add_argument("--compare-rounds", type=int, default=5,
+                    help="interleaved rounds of the learned job, the job without the network and the deferred form")
     ap.add_argument("--workflow4", action="store_true",
                     help="run config 4's workflow (r=100 over 2000 sets, then r=1000 on the top 200) instead")
     ap.add_argument("--no-autotune", action="store_true",
@@ -257,29 +259,45 @@ def main():
 
     compare = None
     if not args.no_compare:
-        # the same job without the network: identical results, tests and time beside the learned job
+        # the learned job beside the same job without the network and with every
+        # full test deferred to the plan (no window at all): identical results;
+        # the three forms interleaved over several rounds (medians), so drift
+        # over time does not fall on one form
         uprm = wl.params(mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=0)
-        eng.reset_stream()
-        tu = time.perf_counter()
-        ures = eng.run_sets(wl.sets, uprm)
-        dtu = time.perf_counter() - tu
-        same = all((a.status, a.perms_run, a.U0) == (b.status, b.perms_run, b.U0) for a, b in zip(ures, res))
-        same = same and all(bool((eng.js(s).view("u8") == js_head[s].view("u8")).all()) for s in range(nsets))
-        k = min(args.train_sets, nsets)
-        tests_u = sum(r.tests_eval for r in ures[k:])
-        # and with every full test deferred to the plan (no window at all)
         zprm = wl.params(mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=3)
-        eng.reset_stream()
-        tz = time.perf_counter()
-        zres = eng.run_sets(wl.sets, zprm)
-        dtz = time.perf_counter() - tz
-        same = same and all((a.status, a.perms_run, a.U0) == (b.status, b.perms_run, b.U0) for a, b in zip(zres, res))
-        same = same and all(bool((eng.js(s).view("u8") == js_head[s].view("u8")).all()) for s in range(nsets))
+        k = min(args.train_sets, nsets)
+        same = True
+
+        def same_as_learned(out):
+            ok = all((a.status, a.perms_run, a.U0) == (b.status, b.perms_run, b.U0) for a, b in zip(out, res))
+            return ok and all(bool((eng.js(s).view("u8") == js_head[s].view("u8")).all()) for s in range(nsets))
+
+        tl, tu, tz = [], [], []
+        for _ in range(args.compare_rounds):
+            t0 = time.perf_counter()
+            lres, _, _ = run_step(eng, wl, prm, cfg)
+            tl.append(time.perf_counter() - t0)
+            same = same and same_as_learned(lres)
+            eng.reset_stream()
+            t0 = time.perf_counter()
+            ures = eng.run_sets(wl.sets, uprm)
+            tu.append(time.perf_counter() - t0)
+            same = same and same_as_learned(ures)
+            eng.reset_stream()
+            t0 = time.perf_counter()
+            zres = eng.run_sets(wl.sets, zprm)
+            tz.append(time.perf_counter() - t0)
+            same = same and same_as_learned(zres)
+        med = lambda v: sorted(v)[len(v) // 2]
+        tests_u = sum(r.tests_eval for r in ures[k:])
         tests_z = sum(r.tests_eval for r in zres[k:])
-        compare = {"unlearned_sets_per_s": nsets / dtu, "unlearned_ms": dtu * 1000.0,
-                   "deferred_full_tests_sets_per_s": nsets / dtz, "deferred_full_tests_ms": dtz * 1000.0,
+        compare = {"rounds": args.compare_rounds,
+                   "learned_ms_median": med(tl) * 1000.0, "learned_sets_per_s_median": nsets / med(tl),
+                   "unlearned_ms_median": med(tu) * 1000.0, "unlearned_sets_per_s_median": nsets / med(tu),
+                   "deferred_full_tests_ms_median": med(tz) * 1000.0,
+                   "deferred_full_tests_sets_per_s_median": nsets / med(tz),
+                   "learned_speedup_vs_unlearned": med(tu) / med(tl),
                    "tests_applied_sets_deferred": tests_z,
-                   "learned_ms_per_step": ms_per_step,
                    "tests_applied_sets_unlearned": tests_u, "tests_applied_sets_learned": rep.tests_apply,
                    "test_reduction_applied_sets": 1.0 - rep.tests_apply / max(1, tests_u),
                    "results_bit_identical": same}

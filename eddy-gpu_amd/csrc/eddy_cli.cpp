@@ -167,7 +167,7 @@ int main(int argc, char **argv) {
     lc.max_rows = 65536;
     lc.label_perms = 10;
     lc.lr = 3e-3f;
-    lc.margin = 1.0f;
+    lc.margin = 3.0f;
     lc.seed = 1;
     rc = eddy_run_learned(ctx, desc.data(), (int)desc.size(), &p, &lc, out.data(), nullptr);
   } else {

@@ -95,7 +95,7 @@ class LearnReport(C.Structure):
                 ("first_windowed_set", C.c_int)]
 
 
-def learn_cfg(train_sets=100, epochs=1, batch=16384, lr=3e-3, margin=1.0, seed=1, max_rows=65536, label_perms=10):
+def learn_cfg(train_sets=100, epochs=1, batch=16384, lr=3e-3, margin=3.0, seed=1, max_rows=65536, label_perms=10):
     return LearnCfg(train_sets, epochs, batch, lr, margin, seed, max_rows, label_perms)
 
 
