@@ -127,3 +127,21 @@ def test_perm_parity_config1_cutoff(engine):
     assert np.array_equal(g["out5"].view(np.uint64), o["out5"].view(np.uint64))
     assert np.array_equal(g["lval"].view(np.uint64), o["lval"].view(np.uint64))
     assert g["js"] == o["js"]
+
+
+@pytest.mark.parametrize("nA,nB", [(1000, 1000), (300, 1700)])
+def test_perm_parity_large_sample_counts(engine, nA, nB):
+    """Near the engine's sample limit (2046): 16 sample chunks in the count
+    kernel, 16-27 mask words per class, the tail kernel's largest LDS use;
+    config 0's set of 40 genes on a seeded dataset of nA + nB samples."""
+    wl = eddy_amd.Workload(0)
+    L = eddy_amd.lib()
+    X = np.zeros((wl.cfg.genes, nA + nB), np.uint8)
+    assert L.eddy_synth_dataset(wl.cfg.seed + 7, wl.cfg.genes, nA, nB, wl.cfg.rewire,
+                                X.ctypes.data_as(eddy_amd.C.c_void_p)) == eddy_amd.EDDY_OK
+    wl.X, wl.N = X, nA + nB
+    wl.classes = np.array([0] * nA + [1] * nB, np.uint8)
+    wl.cfg.nA, wl.cfg.nB = nA, nB
+    engine.upload(wl.X, wl.classes)
+    lab = _labels(wl.N, nA, 21)
+    _compare_perm(engine, wl, 0, lab, eddy_amd.TESTS_CUTOFF, check_bits=False)
