@@ -314,11 +314,65 @@ EDDY_HD void eddy_loo_column(const int *t, int b, const double *iK, const double
 // all nine leave-one-out tallies.  A reduced tally under Yates' rule (rare:
 // an expected count below 5) or within a guard band of a threshold takes the
 // exact decision after the unrolled pass.
+// Every leave-one-out decision of a tally at once, when its statistic is far
+// enough from the thresholds: 1 all nine are "edge", 0 all are "no edge", -1
+// unknown.  With every nonzero margin >= 2, removing one sample from cell
+// (a, b) keeps the zero rows and columns (so the dof) and changes
+// S = sum O^2 / (R K) by at most e = 1/(R_a - 1) + 1/(K_b - 1): the other
+// cells of row a and of column b gain at most 1/(R_a - 1) and 1/(K_b - 1)
+// (O_aj^2 / K_j <= O_aj), and the cell itself loses at most e
+// ((O-1)/(R-1) = O/R - (1 - O/R)/(R - 1)).  With x = n (S - 1) / 2, the
+// leave-one-out x' = (n - 1)(S' - 1) / 2 then lies in
+// [x - (n - 1) e / 2 - x / n, x + (n - 1) e / 2], e taken at the smallest
+// margins.  (Rmin - 1)(Kmin - 1) >= 5 (n - 1) keeps Yates' rule out of every
+// leave-one-out tally.  The fast x is within 1e-9 (1 + x) of the exact value,
+// and each interval end gets the same guard, so an answer here is what the
+// exact decision of each leave-one-out tally gives.
+EDDY_HD int eddy_loo_all_certain(const int *R, const int *K, int n, const double *iR, const double *iK, const int *t,
+                                 int prior, const eddy_edge_params *ep, const eddy_edge_bounds *eb) {
+  if (!ep->shortcut || n < 2) return -1;
+  int rmin = 1 << 30, kmin = 1 << 30, zr = 0, zk = 0;
+  for (int i = 0; i < 3; i++) {
+    if (R[i] == 0) zr++;
+    else if (R[i] < rmin) rmin = R[i];
+    if (K[i] == 0) zk++;
+    else if (K[i] < kmin) kmin = K[i];
+  }
+  if (rmin < 2 || kmin < 2) return -1;
+  if ((long long)(rmin - 1) * (long long)(kmin - 1) < 5LL * (n - 1)) return -1;
+  const int dof = (2 - zk) * (2 - zr);
+  if (!(dof == 1 || dof == 2 || dof == 4)) return -1;
+  double S = 0.0;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) S = S + eddy_sq(t[i * 3 + j]) * iR[i] * iK[j];
+  const double x = 0.5 * (double)n * (S - 1.0);
+  const double e = 1.0 / (double)(rmin - 1) + 1.0 / (double)(kmin - 1);
+  const double d = 0.5 * (double)(n - 1) * e * (1.0 + 1e-12);
+  const double xg = x + 1e-9 * (1.0 + (x > 0.0 ? x : 0.0));  // >= the exact x
+  const double up = xg + d;                                   // >= every leave-one-out x
+  const double dn = x - 1e-9 * (1.0 + (x > 0.0 ? x : 0.0)) - d - xg / (double)n;  // <= every one
+  const bool p1 = dof == 1, p2 = dof == 2, pr = prior == 1;
+  const double xlo = pr ? eddy_pick3(eb->xlo[1], p1, p2) : eddy_pick3(eb->xlo[0], p1, p2);
+  const double xhi = pr ? eddy_pick3(eb->xhi[1], p1, p2) : eddy_pick3(eb->xhi[0], p1, p2);
+  const double xclo = eddy_pick3(ep->xclo, p1, p2);
+  if (up <= xlo - 1e-9 * (1.0 + (xlo > 0.0 ? xlo : 0.0))) return 0;
+  if (dn >= xhi + 1e-9 * (1.0 + xhi) && up <= xclo - 1e-9 * (1.0 + xclo)) return 1;
+  return -1;
+}
+
 EDDY_HD uint32_t eddy_loo_mask_fast(const int *t, int full, int prior, const eddy_edge_params *ep,
                                     const eddy_edge_bounds *eb, double cpairs, int *ntests) {
   int R[3], K[3];
   double iR[3], iK[3], iRm[3], iKm[3];
   const int n = eddy_loo_margins(t, R, K, iR, iRm, iK, iKm);
+  {
+    const int all = eddy_loo_all_certain(R, K, n, iR, iK, t, prior, ep, eb);
+    if (all >= 0) {  // nonzero cells take `all`, empty cells copy `full`
+      uint32_t nz = 0;
+      for (int c = 0; c < 9; c++) nz |= (t[c] > 0 ? 1u : 0u) << c;
+      return (all ? nz : 0u) | (full ? (~nz & 0x1FFu) : 0u);
+    }
+  }
   uint32_t m = 0, unsure = 0;
   // unrolled: every array index below is a constant, so all tallies stay in
   // registers

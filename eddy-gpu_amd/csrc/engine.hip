@@ -2348,6 +2348,22 @@ extern "C" int eddy_edge_decision_host(const int *tally9, int c, int prior, cons
   return eddy_edge_decision_fast(tally9, prior, &ep, &eb, (double)c);
 }
 
+extern "C" int eddy_loo_all_certain_host(const int *tally9, int c, int prior, const eddy_params *p) {
+  if (!tally9 || !p) return -2;
+  eddy_edge_params ep;
+  eddy_ctx tmp;
+  tmp.lg15 = std::lgamma(1.5);
+  tmp.lg2 = std::lgamma(2.0);
+  tmp.lg3 = std::lgamma(3.0);
+  edge_params(&tmp, p, ep);
+  eddy_edge_bounds eb;
+  edge_bounds(p, c, eb);
+  int R[3], K[3];
+  double iR[3], iK[3], iRm[3], iKm[3];
+  const int n = eddy_loo_margins(tally9, R, K, iR, iRm, iK, iKm);
+  return eddy_loo_all_certain(R, K, n, iR, iK, tally9, prior, &ep, &eb);
+}
+
 extern "C" int eddy_edge_loo_mask_host(const int *tally9, int c, int prior, int full, const eddy_params *p) {
   if (!tally9 || !p) return -1;
   eddy_edge_params ep;

@@ -256,6 +256,12 @@ typedef struct {
 int eddy_autotune(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *params,
                   const char *cache_path, eddy_launch_cfg *out, eddy_tune_report *report);
 
+/* Host build of the kernels' all-at-once leave-one-out check: 1 when every
+ * leave-one-out tally of tally9 is certainly an edge, 0 when certainly none is,
+ * -1 when the per-cell decisions are needed (tests compare it with the
+ * oracle's decision of each leave-one-out tally). */
+int eddy_loo_all_certain_host(const int *tally9, int c, int prior, const eddy_params *p);
+
 /* Host build of the engine's own edge decision for one 3x3 tally
  * (tally9[r*3+k], r = row-gene state): 1 edge, 0 none.  Tests compare it
  * with the CPU oracle; it runs on the CPU and needs no context. */

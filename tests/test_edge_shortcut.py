@@ -217,3 +217,98 @@ def test_loo_mask_matches_oracle(c):
             assert (m >> k) & 1 == want, (t, k, c, pr)
             checked += 1
     assert checked > 10000
+
+
+def _dependent_tables(rng, count):
+    """Tertile-like tables of 150..1200 samples, from independent to strongly
+    dependent, so the statistic sweeps through both thresholds."""
+    for _ in range(count):
+        n = int(rng.integers(150, 1201))
+        pr = rng.dirichlet([8, 8, 8])
+        pc = rng.dirichlet([8, 8, 8])
+        lam = float(rng.uniform(0.0, 0.35))
+        perm = rng.permutation(3)
+        dep = np.zeros((3, 3))
+        for i in range(3):
+            dep[i, perm[i]] = 1.0 / 3.0
+        prob = (1 - lam) * np.outer(pr, pc) + lam * dep
+        yield rng.multinomial(n, prob.ravel() / prob.sum()).astype(np.int64)
+
+
+@pytest.mark.parametrize("c", [45, 11175, 124750])
+def test_loo_all_certain_matches_oracle(c):
+    """Where the all-at-once check answers, every leave-one-out tally of a
+    nonzero cell gets that decision from the oracle (the reference formula);
+    it answers for most tables and both ways."""
+    rng = np.random.default_rng(7 * c + 3)
+    ep, op = params(1)
+    L = eddy_amd.lib()
+    seen = {-1: 0, 0: 0, 1: 0}
+    for t in _dependent_tables(rng, 2500):
+        pr = int(rng.integers(0, 2))
+        t32 = np.ascontiguousarray(t, np.int32)
+        a = L.eddy_loo_all_certain_host(t32.ctypes.data_as(C.c_void_p), c, pr, C.byref(ep))
+        seen[a] += 1
+        if a < 0:
+            continue
+        for k in range(9):
+            if t[k] > 0:
+                u = t.copy()
+                u[k] -= 1
+                assert O.edge_decision(u, c, pr, op) == a, (t, k, c, pr, a)
+    assert seen[0] > 150 and seen[1] > 150 and seen[0] + seen[1] > 1250, seen
+
+
+def _boundary_tables(rng, walks, c, op):
+    """Tables around a decision flip: from an independent table, move one
+    sample at a time onto a fixed diagonal until the full decision turns to an
+    edge, keeping the last 60 tables before the flip and the 20 after it."""
+    out = []
+    for _ in range(walks):
+        n = int(rng.integers(150, 1201))
+        pr = int(rng.integers(0, 2))
+        p_r = rng.dirichlet([8, 8, 8])
+        p_c = rng.dirichlet([8, 8, 8])
+        t = rng.multinomial(n, np.outer(p_r, p_c).ravel()).astype(np.int64)
+        diag = rng.permutation(3)
+        hist, after = [], -1
+        for _ in range(4 * n):
+            hist.append(t.copy())
+            if after < 0 and O.edge_decision(t, c, pr, op) == 1:
+                after = 0
+            if after >= 0:
+                after += 1
+                if after > 20:
+                    break
+            off = [k for k in range(9) if t[k] > 1 and k % 3 != diag[k // 3]]
+            if not off:
+                break
+            k = off[int(rng.integers(0, len(off)))]
+            t[k] -= 1
+            t[(k // 3) * 3 + diag[k // 3]] += 1
+        if after > 0:
+            out += [(h, pr) for h in hist[-80:]]
+    return out
+
+
+@pytest.mark.parametrize("c", [45, 11175])
+def test_loo_all_certain_at_decision_flips(c):
+    """Near a flip some leave-one-out tallies decide differently from the full
+    one; wherever the all-at-once check answers there, the oracle agrees on
+    every leave-one-out tally."""
+    rng = np.random.default_rng(11 * c + 5)
+    ep, op = params(1)
+    L = eddy_amd.lib()
+    answered = 0
+    for t, pr in _boundary_tables(rng, 120, c, op):
+        t32 = np.ascontiguousarray(t, np.int32)
+        a = L.eddy_loo_all_certain_host(t32.ctypes.data_as(C.c_void_p), c, pr, C.byref(ep))
+        if a < 0:
+            continue
+        answered += 1
+        for k in range(9):
+            if t[k] > 0:
+                u = t.copy()
+                u[k] -= 1
+                assert O.edge_decision(u, c, pr, op) == a, (t, k, c, pr, a)
+    assert answered > 1000
