@@ -873,26 +873,33 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
         const int k = (w0 + u) * 64 + lane;
         sid[u] = k < ng ? (int)cl[k] : -1;
       }
+      // lane u < 8 stores mask word w0 + u of each state
+      const int nw = min(8, a.bitw - w0);
       for (int gene = wave; gene < G; gene += EDDY_NWAVES) {
         const uint8_t *row = xs + (size_t)gene * NP;
         int v[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) v[u] = sid[u] >= 0 ? row[sid[u]] : 3;
         uint64_t *mk = a.bits + ((size_t)(im.goff * 2 + g * G + gene) * 3) * a.bitw;
-        uint64_t any0 = 0, any1 = 0, any2 = 0;
+        uint64_t any0 = 0, any1 = 0, any2 = 0, s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-          const int w = w0 + u;
-          if (w >= a.bitw) break;  // wave-uniform
           const uint64_t m0 = __ballot(v[u] == 0), m1 = __ballot(v[u] == 1), m2 = __ballot(v[u] == 2);
-          if (lane == 0) {
-            mk[w] = m0;
-            mk[a.bitw + w] = m1;
-            mk[2 * a.bitw + w] = m2;
+          if (lane == u) {
+            s0 = m0;
+            s1 = m1;
+            s2 = m2;
           }
-          any0 |= m0;
-          any1 |= m1;
-          any2 |= m2;
+          if (u < nw) {
+            any0 |= m0;
+            any1 |= m1;
+            any2 |= m2;
+          }
+        }
+        if (lane < nw) {
+          mk[w0 + lane] = s0;
+          mk[a.bitw + w0 + lane] = s1;
+          mk[2 * a.bitw + w0 + lane] = s2;
         }
         // this wave owns the gene in every chunk: a plain read-modify-write
         if (lane == 0)
