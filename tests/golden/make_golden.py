@@ -7,6 +7,12 @@
 * gammainc.npz         scipy.special.gammainc(p, x) for p in {1/2, 1, 2}
 * cli_small/           a tiny synthetic run (30 genes x 60 samples, 3 sets)
                        of the oracle CLI: its inputs and the four output files
+* reference_shapes/    the reference's own example inputs, as data: its class
+                       file (575 labels, 303 GBApPD then 272 GBApUnaffected)
+                       and its two GMT files (14 sets of 11..50 genes), copied
+                       from resources/ of the reference tree; its expression
+                       matrix is absent there, so the tests pair them with a
+                       seeded synthetic matrix over the GMT gene names
 """
 import ctypes
 import os
@@ -98,8 +104,17 @@ def cli_fixture():
             shutil.move(os.path.join(d, fn), os.path.join(d, "expected", fn))
 
 
+def reference_shapes_fixture():
+    src = "/root/reference/resources"
+    d = os.path.join(HERE, "reference_shapes")
+    os.makedirs(d, exist_ok=True)
+    for rel, dst in (("data/GBApPD_vs_GBApUnaffected_ClassFile.txt", "class.txt"), ("gmt/test.gmt", "test.gmt"),
+                     ("gmt/failed.gmt", "failed.gmt")):
+        shutil.copyfile(os.path.join(src, rel), os.path.join(d, dst))
+
+
 if __name__ == "__main__":
-    what = sys.argv[1:] or ["rand", "betaincc", "gammainc", "cli"]
+    what = sys.argv[1:] or ["rand", "betaincc", "gammainc", "cli", "reference_shapes"]
     if "rand" in what:
         rand_fixture()
     if "betaincc" in what:
@@ -108,3 +123,5 @@ if __name__ == "__main__":
         gammainc_fixture()
     if "cli" in what:
         cli_fixture()
+    if "reference_shapes" in what:
+        reference_shapes_fixture()
