@@ -5,20 +5,40 @@
  *
  * Each function cites the reference file:line it restates.  What pins each
  * part (the tests under tests/ that check it):
- *   eo_perm_labels   glibc rand() fixture + permutation invariants
+ *   eo_perm_labels   a statement-by-statement Python restatement of
+ *                    main.cu:541-570 over the committed glibc rand() fixture:
+ *                    labels of consecutive permutations and the stream
+ *                    position after them; a variant without the discarded
+ *                    draws fails (test_oracle_stream.py)
+ *   eo_run_set       stream consumption: one permutation's draws for a set
+ *                    that is NaN at permutation 0, r otherwise
+ *                    (test_oracle_stream.py)
+ *   eo_class_split   numpy's stable partition under permuted labels
  *                    (test_oracle_stream.py)
  *   eo_chi2          hand-worked 3x3 tables, 2x2 Yates closed form, scipy
- *                    Pearson chi2 when no Yates applies (test_oracle_tests.py)
+ *                    Pearson chi2 when no Yates applies, dof rule, transpose
+ *                    invariance (test_oracle_tests.py)
  *   eo_gammds        closed forms P(1/2,x)=erf(sqrt x), P(1,x)=1-e^-x,
- *                    P(2,x)=1-e^-x(1+x), underflow cut (test_oracle_tests.py)
- *   networks, dedup  brute force over explicit edge sets on small G
+ *                    P(2,x)=1-e^-x(1+x), scipy gammainc vectors, underflow
+ *                    cut (test_oracle_math.py)
+ *   edge bits        an independent Python leave-one-out tally per
+ *                    (network, pair) (test_oracle_networks.py)
+ *   networks, dedup  a direct Python construction from the edge bits:
+ *                    first mp columns, first occurrence, compaction
  *                    (test_oracle_networks.py)
- *   eo_bdeu_family   BDeu from its textbook formula with numpy counts and
- *                    math.lgamma (test_oracle_bdeu.py)
- *   likelihood / JS  scipy jensenshannon, P1=P2 -> 0, disjoint -> 1
- *   eo_pvalue        scipy betaincc golden vectors, I_x(a,1)=x^a,
- *                    I_x(1,b)=1-(1-x)^b (test_oracle_pvalue.py)
- *   eo_main          golden files of a small synthetic run (test_cli_golden)
+ *   BDeu             the textbook formula with numpy counts and math.lgamma,
+ *                    within 1e-12 relative (test_oracle_networks.py)
+ *   likelihood / JS  the scaled softmax in numpy, scipy jensenshannon
+ *                    (test_oracle_networks.py)
+ *   eo_pvalue        the method-of-moments fit restated in numpy; ibetac
+ *                    within 1e-12 relative of scipy betaincc (fixture and a
+ *                    live sweep of shapes 0.3..5000), I_x(a,1)=x^a,
+ *                    I_x(1,b)=1-(1-x)^b, symmetry (test_oracle_math.py,
+ *                    test_oracle_networks.py)
+ *   eo_uncapped_pair a direct construction from the edge bits, and the capped
+ *                    construction at mp = G (test_oracle_stream.py)
+ *   eo_main          parsing quirks and golden files of a small synthetic run
+ *                    (test_cli_parse.py)
  * Nothing here is "parity unpinned" except the agreement with the reference
  * binary's own FP64 bits, which come from closed CUDA device math and
  * Boost 1.51 and cannot be produced in this environment.

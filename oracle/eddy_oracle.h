@@ -85,6 +85,14 @@ int eo_run_set(const int *data, int numSamples, const int *trueClass, int n1,
                const eo_params *p, double *js_out, int *failed, double *pDDN,
                int *U0);
 
+/* H17: uncapped parent lists (PARENTS_LIMIT = INT_MAX) of permutation 0's
+ * two full-data networks for the EdgeList file (main.cu:1530-1612):
+ * bits0 / bitsL are out23 rows of networks 0 and S-1 ([c] each); nodes[2G]
+ * receives the CSR offsets, edges the parents (at most edges_cap), edgesPN
+ * {0, E0, E0 + E1}.  Returns the edge count, or -1 if edges_cap is short. */
+int eo_uncapped_pair(const uint8_t *bits0, const uint8_t *bitsL, int G, int *nodes, int *edges, int edges_cap,
+                     int edgesPN[3]);
+
 /* Reset the rand() stream to glibc's unseeded state (srand(1)). */
 void eo_reset_stream(void);
 

@@ -197,6 +197,20 @@ static void uncapped_pair(const uint8_t *bits0, const uint8_t *bitsL, int G,
   *edges_out = edges;
 }
 
+int eo_uncapped_pair(const uint8_t *bits0, const uint8_t *bitsL, int G, int *nodes, int *edges, int edges_cap,
+                     int edgesPN[3]) {
+  int *e = NULL;
+  uncapped_pair(bits0, bitsL, G, nodes, &e, edgesPN);
+  int n = edgesPN[2];
+  if (n > edges_cap) {
+    free(e);
+    return -1;
+  }
+  memcpy(edges, e, sizeof(int) * (size_t)n);
+  free(e);
+  return n;
+}
+
 int eo_main(int argc, char **argv) {
   char *inputFile = NULL, *classFile = NULL, *genesetFile = NULL;
   int parentCap = 0, perms = 100, flag_pAdjust = 1;

@@ -59,6 +59,8 @@ def lib():
         L.eo_run_set.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                  C.c_void_p, C.POINTER(EoParams), C.c_void_p, C.POINTER(C.c_int),
                                  C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.eo_uncapped_pair.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                       C.c_void_p]
         for f in ("eo_exp", "eo_log", "eo_lgamma_pos"):
             getattr(L, f).restype = C.c_double
             getattr(L, f).argtypes = [C.c_double]

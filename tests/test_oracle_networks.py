@@ -107,7 +107,7 @@ def test_networks_dedup_bdeu_js(G, n1, n2, mp, seed):
                     for v in rstates:
                         nijk = int((m & (node == v)).sum())
                         score += math.lgamma(nijk + 4 / (q * r)) - math.lgamma(4 / (q * r))
-                assert abs(o["out5"][g, u, t] - score) <= 1e-9 * max(1.0, abs(score))
+                assert abs(o["out5"][g, u, t] - score) <= 1e-12 * max(1.0, abs(score))
     # likelihoods + JS
     U = o["U"]
     P1, P2 = o["lval"][:U], o["lval"][U:]
