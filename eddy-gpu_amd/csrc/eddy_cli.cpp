@@ -20,6 +20,7 @@
 //     throws and aborts, prints pDDN as nan and writes no files;
 //   * inputs the reference would read out of bounds (missing class labels,
 //     a third class, short expression rows) are rejected with EXIT_FAILURE.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -88,6 +89,7 @@ int main(int argc, char **argv) {
   double alphaEdgePrior = 1 - (1 - alphaEdge) * pow(priorWeight, 1 / lambda);
   printf("alphaEdge : %f, alphaEdgePrior : %f\n", alphaEdge, alphaEdgePrior);
   printf("Multiple testing correction (edge detection) : %s\n", pAdjust ? "true" : "false");
+  printf("[to be deprecated] lambda : %f, theta : %f\n", lambda, theta);
 
   ExprData expr;
   std::string e;
@@ -180,8 +182,17 @@ int main(int argc, char **argv) {
     const GeneSet &s = sets[i];
     const eddy_set_result &r = out[i];
     const int G = (int)s.rows.size();
-    printf("%s\n%d genes in geneset\n", s.name.c_str(), G);
+    // the reference's per-set lines in its order (main.cu:390-537, 1124, 1432, 1462)
+    printf("%s\n%d genes in geneset\n", s.name.c_str(), s.listed);
+    printf("Adjusted genes : %d\n", G);
+    const std::string prior_path = "PRIORS/" + s.name + ".prior";
+    if (prior_path.find('\r') != std::string::npos) printf("File problem! Uses window endings!\n");
+    printf("file : %s\n", prior_path.c_str());
+    if (!s.prior_file) printf("No prior file exists. Computing without prior knowledge\n");
+    printf("Permutations begin:\n");
     if (r.U0 > 0) printf("Original Number of unique Networks : %d\n", r.U0);
+    if (r.status == EDDY_SET_FAILED_NAN && r.perms_run == 1) printf("jsVal[0] NAN--Breaking permutation loop\n");
+    printf("permutation loop (%d) finished\n", perms);
     if (r.status == EDDY_SET_FAILED_NAN) {
       fprintf(res, "%s failed with %d genes- one of the js scores was nan\n", s.name.c_str(), G);
       printf("nan value- quiting!\n\n\n");
@@ -198,7 +209,7 @@ int main(int argc, char **argv) {
       continue;
     }
     const int U = r.U0;
-    std::vector<int> ids(U), nodes((size_t)U * G), epn(U + 1), edges((size_t)U * G * parentCap + 1);
+    std::vector<int> ids(U), nodes((size_t)U * G), epn(U + 1), edges((size_t)U * G * std::min(parentCap, G) + 1);
     std::vector<double> lval(2 * U);
     int ne = 0;
     rc = eddy_get_perm0_networks(ctx, (int)i, ids.data(), nodes.data(), (int)nodes.size(), epn.data(), edges.data(),

@@ -32,9 +32,14 @@ __global__ void k_loo(KArgs a);
 __global__ void k_loo_w3(KArgs a);
 __global__ void k_clear_slot(int *evcount, int *status, long long *tests, unsigned *bdcount, int n);
 __global__ void k_rows(KArgs a);
+__global__ void k_rows_g(KArgs a);
+__global__ void k_rows_w(KArgs a);
+__global__ void k_rows_gw(KArgs a);
 __global__ void k_score(KArgs a);
+__global__ void k_score_g(KArgs a);
 __global__ void k_bdeu(KArgs a);
 __global__ void k_score_tail(KArgs a);
+__global__ void k_score_tail_g(KArgs a);
 __global__ void k_pvalue(const double *js, int nsets, int perms, const int *nan0, double *out);
 __global__ void k_gather(const uint8_t *x, int N, int NP, const int *genes, const long long *rowoff, int nrows,
                          uint8_t *xs);
@@ -53,10 +58,11 @@ struct DevBuf {
 // arrays and edgeListData1/2, main.cu:782-819, 1085-1195, 1406-1409).
 struct Perm0 {
   int U = 0;
-  std::vector<uint8_t> lcnt;    // [2G]
-  std::vector<uint16_t> lfull;  // [2G * EDDY_MAX_MP]
+  std::vector<uint16_t> lcnt;   // [2G]
+  std::vector<uint16_t> lfull;  // [2G * mpw]
   std::vector<int> rep;         // [S]
-  std::vector<Event> ev;        // events of the permutation
+  std::vector<uint32_t> evkey;  // events of the permutation: s | t << 16
+  std::vector<uint16_t> evpar;  // [evw] per event: n, then the parents
   std::vector<double> lval;     // [2S]: g*S + u
   std::vector<uint32_t> info;   // [c]
   std::vector<uint64_t> famkey; // distinct families (t, first <= 3 parents)
@@ -69,7 +75,21 @@ struct Chunk {
   std::vector<uint32_t> labels;
   std::vector<uint2> tiles;
   long long pairs = 0, gslots = 0, sslots = 0;
+  long long evslots = 0;  // sum of the items' first event capacities
 };
+
+// an item's first event capacity (grown on overflow)
+inline int item_evcap(int S, int G) { return std::max(512, 2 * S + 24 * G); }
+
+// device bytes a chunk's per-item buffers take, roughly: per network slot
+// (hashes, representatives, class lists, likelihoods, score tables and the
+// score kernels' global scratch), per event of capacity (key, order, row,
+// family, evw parent words) and per family slot
+inline long long chunk_bytes(const Chunk &ch, int evw) {
+  return ch.sslots * 160 + ch.evslots * (18 + 2LL * evw + 96);
+}
+// a chunk is cut between sets once its per-item buffers pass this
+const long long kChunkBytes = 4LL << 30;
 
 // A slot's launch sequence as a graph: built on the slot's first chunk of a
 // given shape (with or without the count kernel, all-tests or cutoff
@@ -88,15 +108,15 @@ struct SlotGraph {
 #define EDDY_SLOTS 2
 #endif
 struct Slot {
-  DevBuf items, labels, tiles, info, tab, lfull, lcnt, sflags, bits, ev, evcount, hnet, rep, clsg, evorder, evfam,
+  DevBuf items, labels, tiles, info, tab, lfull, lcnt, sflags, bits, evcount, hnet, rep, clsg, evorder, evfam,
       famkey, famscore, famht, famhv, lval, uniq, famcount, status, tests, order, slices, bdtask, bdcount, sculist,
-      scevst, scfamfull, fbits, evkey, evt, evpar;
+      scevst, scfamfull, fbits, evkey, evt, evpar, gscr;
   hipStream_t st = nullptr;
   // timing events: 0 start, 1 after count, 2 after loo, 3 after rows,
   // 5 after score, 6 after BDeu, 4 after the tail
   hipEvent_t tev[7] = {};
   float kms[6] = {};  // last launch: count, loo, rows, score, BDeu, tail (ms)
-  SlotGraph gr[8];
+  SlotGraph gr[64];
   int *h_int = nullptr;  // pinned: status, events, families, unique networks (4 x h_cap)
   long long *h_tests = nullptr;
   int h_cap = 0;
@@ -111,7 +131,8 @@ struct Slot {
 };
 
 struct P0Store {
-  DevBuf info, lcnt, lfull, rep, lval, ev, famkey, famscore, slices;
+  DevBuf info, lcnt, lfull, rep, lval, evkey, evpar, famkey, famscore, slices;
+  int mpw = 1, evw = 2;  // strides of the run's row lists and event records
   std::vector<SetMeta> meta;
   std::vector<long long> evoff, evn, famoff, famn;
   std::vector<int> U, done;
@@ -145,7 +166,7 @@ struct eddy_ctx {
   double lg15 = 0, lg2 = 0, lg3 = 0;
   PermStream stream_gen;
   // device buffers, grown on demand
-  DevBuf b_sets, b_xs, b_prior, b_items, b_labels, b_tiles, b_info, b_tab, b_lfull, b_lcnt, b_sflags, b_ev,
+  DevBuf b_sets, b_xs, b_prior, b_items, b_labels, b_tiles, b_info, b_tab, b_lfull, b_lcnt, b_sflags,
       b_evcount, b_hnet, b_rep, b_clsg, b_evorder, b_evfam, b_famkey, b_famscore, b_lval, b_uniq, b_js,
       b_status, b_tests, b_genes, b_rowoff, b_nan0, b_pout, b_famcount, b_counters, b_famht, b_famhv, b_bits, b_order, b_js0,
       b_totals;
@@ -178,6 +199,7 @@ struct eddy_ctx {
   int k_label_perms = 0;
   // launch shapes (eddy_set_launch_cfg / eddy_autotune)
   eddy_launch_cfg lc{1, 0, 0, 4, 0, 2048};
+  size_t lds_max = 65536;  // LDS bytes a workgroup may use on the device
 };
 
 #define HIPCHK(ctx, x)                                                                  \
@@ -241,6 +263,9 @@ extern "C" int eddy_ctx_create(int device, eddy_ctx **out) {
       eddy_ctx_destroy(ctx);
       return EDDY_ERR_HIP;
     }
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+    ctx->lds_max = (size_t)lds;
   *out = ctx;
   return EDDY_OK;
 }
@@ -253,7 +278,7 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
   }
   hipSetDevice(ctx->device);
   DevBuf *all[] = {&ctx->b_sets, &ctx->b_xs, &ctx->b_prior, &ctx->b_items, &ctx->b_labels, &ctx->b_tiles,
-                   &ctx->b_info, &ctx->b_tab, &ctx->b_lfull, &ctx->b_lcnt, &ctx->b_sflags, &ctx->b_ev,
+                   &ctx->b_info, &ctx->b_tab, &ctx->b_lfull, &ctx->b_lcnt, &ctx->b_sflags,
                    &ctx->b_evcount, &ctx->b_hnet, &ctx->b_rep, &ctx->b_clsg, &ctx->b_evorder, &ctx->b_evfam,
                    &ctx->b_famkey, &ctx->b_famscore, &ctx->b_lval, &ctx->b_uniq, &ctx->b_js, &ctx->b_status,
                    &ctx->b_tests, &ctx->b_genes, &ctx->b_rowoff, &ctx->b_nan0, &ctx->b_pout, &ctx->b_famcount,
@@ -261,13 +286,13 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
                    &ctx->b_mlpg, &ctx->b_mlploss, &ctx->b_feat, &ctx->b_labhits, &ctx->b_score, &ctx->b_win,
                    &ctx->b_mx, &ctx->b_my, &ctx->b_mxb, &ctx->b_myb, &ctx->b_midx, &ctx->b_mscr, &ctx->b_mlbase,
                    &ctx->b_mlinv, &ctx->p0.info, &ctx->p0.lcnt, &ctx->p0.lfull, &ctx->p0.rep, &ctx->p0.lval,
-                   &ctx->p0.ev, &ctx->p0.famkey, &ctx->p0.famscore, &ctx->p0.slices, &ctx->b_bits,
+                   &ctx->p0.evkey, &ctx->p0.evpar, &ctx->p0.famkey, &ctx->p0.famscore, &ctx->p0.slices, &ctx->b_bits,
                    &ctx->b_win2, &ctx->b_order, &ctx->b_js0, &ctx->b_totals};
   for (DevBuf *b : all)
     if (b->p) hipFree(b->p);
   for (Slot &sl : ctx->slot) {
     DevBuf *sb[] = {&sl.items, &sl.labels, &sl.tiles, &sl.info, &sl.tab, &sl.lfull, &sl.lcnt, &sl.sflags, &sl.bits,
-                    &sl.ev, &sl.evcount, &sl.hnet, &sl.rep, &sl.clsg, &sl.evorder, &sl.evfam, &sl.famkey,
+                    &sl.gscr, &sl.evcount, &sl.hnet, &sl.rep, &sl.clsg, &sl.evorder, &sl.evfam, &sl.famkey,
                     &sl.famscore, &sl.famht, &sl.famhv, &sl.lval, &sl.uniq, &sl.famcount, &sl.status, &sl.tests,
                     &sl.order, &sl.slices, &sl.bdtask, &sl.bdcount, &sl.sculist, &sl.scevst, &sl.scfamfull,
                     &sl.fbits, &sl.evkey, &sl.evt, &sl.evpar};
@@ -339,7 +364,7 @@ extern "C" int eddy_upload_dataset(eddy_ctx *ctx, const uint8_t *x, int genes, i
   if (!ctx || !x || !class_of || genes <= 0 || samples <= 1) return EDDY_ERR_ARG;
   if (ctx->device < 0) return EDDY_ERR_NODEV;
   if (samples > EDDY_MAX_N) {
-    ctx->err = "more than 2046 samples";
+    ctx->err = "more than 20000 samples";
     return EDDY_ERR_ARG;
   }
   hipSetDevice(ctx->device);
@@ -465,11 +490,20 @@ struct SetPrep {
 };
 
 int check_params(eddy_ctx *ctx, const eddy_params *p) {
-  if (!p || p->mp < 1 || p->mp > EDDY_MAX_MP || p->perms < 1) {
-    ctx->err = "invalid parameters (mp must be 1..16, perms >= 1)";
+  if (!p || p->mp < 1 || p->perms < 1) {
+    ctx->err = "invalid parameters (mp >= 1, perms >= 1)";
     return EDDY_ERR_ARG;
   }
   return EDDY_OK;
+}
+
+// The parent cap the kernels hold: row t has at most t <= G - 1 candidate
+// parents, so a cap above the largest set's G - 1 caps nothing and every
+// result equals the one at G - 1.
+int mp_eff(const eddy_params *p, const std::vector<SetMeta> &sets) {
+  int gmax = 2;
+  for (const SetMeta &m : sets) gmax = std::max(gmax, m.G);
+  return std::max(1, std::min(p->mp, gmax - 1));
 }
 
 void add_item(Chunk &ch, const std::vector<SetMeta> &sets, int set, int perm, const std::vector<uint32_t> &labbits,
@@ -524,16 +558,16 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   }
   const bool plain_tests = p->schedule == EDDY_TESTS_ALL;
   const int bitw = (std::max(ctx->n1, ctx->n2) + 63) / 64;
+  const int mp = mp_eff(p, sets), evw = mp + 1;
   CHK(grow(ctx, sl.items, sizeof(ItemMeta) * nitems));
   CHK(grow(ctx, sl.labels, sizeof(uint32_t) * ch.labels.size()));
   CHK(grow(ctx, sl.tiles, sizeof(uint2) * std::max<size_t>(1, ch.tiles.size())));
   CHK(grow(ctx, sl.info, sizeof(uint32_t) * ch.pairs));
   if (!plain_tests) CHK(grow(ctx, sl.tab, sizeof(uint16_t) * EDDY_TAB_STRIDE * ch.pairs));
-  CHK(grow(ctx, sl.lfull, sizeof(uint16_t) * EDDY_MAX_MP * 2 * ch.gslots));
-  CHK(grow(ctx, sl.lcnt, 2 * ch.gslots));
+  CHK(grow(ctx, sl.lfull, sizeof(uint16_t) * (size_t)mp * 2 * ch.gslots));
+  CHK(grow(ctx, sl.lcnt, sizeof(uint16_t) * 2 * ch.gslots));
   CHK(grow(ctx, sl.sflags, 2 * ch.gslots));
   CHK(grow(ctx, sl.bits, sizeof(uint64_t) * 2 * 3 * (size_t)bitw * ch.gslots));
-  CHK(grow(ctx, sl.ev, sizeof(Event) * evoff));
   CHK(grow(ctx, sl.evcount, sizeof(int) * nitems));
   CHK(grow(ctx, sl.hnet, sizeof(uint64_t) * ch.sslots));
   CHK(grow(ctx, sl.rep, sizeof(int) * ch.sslots));
@@ -541,7 +575,7 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   CHK(grow(ctx, sl.evorder, sizeof(int) * evoff));
   CHK(grow(ctx, sl.evkey, sizeof(uint32_t) * evoff));
   CHK(grow(ctx, sl.evt, sizeof(uint16_t) * evoff));
-  CHK(grow(ctx, sl.evpar, sizeof(uint16_t) * (size_t)(p->mp + 1) * evoff));
+  CHK(grow(ctx, sl.evpar, sizeof(uint16_t) * (size_t)evw * evoff));
   CHK(grow(ctx, sl.evfam, sizeof(int) * evoff));
   CHK(grow(ctx, sl.famkey, sizeof(uint64_t) * foff));
   CHK(grow(ctx, sl.famscore, sizeof(double) * 2 * foff));
@@ -596,7 +630,8 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   a.n1 = ctx->n1;
   a.n2 = ctx->n2;
   a.S = S;
-  a.mp = p->mp;
+  a.mp = mp;
+  a.mpw = mp;
   a.p_adjust = p->p_adjust;
   a.plain = plain_tests ? 1 : 0;
   a.feat_only = 0;
@@ -626,9 +661,8 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   a.tab = (uint16_t *)sl.tab.p;
   a.totals = (uint16_t *)ctx->b_totals.p;
   a.lfull = (uint16_t *)sl.lfull.p;
-  a.lcnt = (uint8_t *)sl.lcnt.p;
+  a.lcnt = (uint16_t *)sl.lcnt.p;
   a.sflags = (uint8_t *)sl.sflags.p;
-  a.ev = (Event *)sl.ev.p;
   a.evcount = (int *)sl.evcount.p;
   a.hnet = (uint64_t *)sl.hnet.p;
   a.rep = (int *)sl.rep.p;
@@ -637,7 +671,7 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   a.evkey = (uint32_t *)sl.evkey.p;
   a.evt = (uint16_t *)sl.evt.p;
   a.evpar = (uint16_t *)sl.evpar.p;
-  a.evw = p->mp + 1;
+  a.evw = evw;
   a.evfam = (int *)sl.evfam.p;
   a.famkey = (uint64_t *)sl.famkey.p;
   a.famht = (unsigned long long *)sl.famht.p;
@@ -656,31 +690,64 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   a.sc_ulist = (int *)sl.sculist.p;
   a.sc_evst = (int *)sl.scevst.p;
   a.sc_famfull = (int *)sl.scfamfull.p;
-  // tail kernel LDS: base scores (4 maxG), then the KL terms (2 S), in one
-  // region; the network scores / likelihoods (2 S) after it
-  a.tail_shared = (int)(std::max<size_t>((size_t)4 * maxG, (size_t)2 * S) * 8);
   a.fbits = (uint32_t *)sl.fbits.p;
   a.fbw = fbw;
   a.fbwp = fbwp;
   a.tile_order = ctx->lc.tile_order;
-  const size_t tail_lds = (size_t)a.tail_shared + (size_t)2 * S * 8;
 
   for (int k = 0; k < 7; k++)
     if (!sl.tev[k]) HIPCHK(ctx, hipEventCreate(&sl.tev[k]));
-  const size_t rows_lds = (size_t)S * 8 + (size_t)EDDY_NWAVES * maxG * 4 + (size_t)N * 2 + (size_t)2 * maxG + 16;
-  // the score kernel's phase-shared region: the network hash table, then the
-  // BDeu mask stage, then the base family scores (at least ht * 12 bytes)
+  // Where the S-sized tables live.  Up to a few thousand samples they fit in
+  // LDS (the rows kernel's deltas and class lists within 64 KB, so it keeps
+  // its occupancy; the score kernels' tables within the device's limit);
+  // beyond, the *_g kernels keep them in global memory (per-item scratch).
+  const size_t lmax = ctx->lds_max;
+  const size_t rows_fix = (size_t)EDDY_NWAVES * maxG * 4 + (size_t)EDDY_NWAVES * mp * 2 + (size_t)2 * maxG + 16;
+  const bool rows_gm = (size_t)S * 8 + (size_t)N * 2 + rows_fix > 65536;
+  const size_t rows_lds = rows_fix + (rows_gm ? 0 : (size_t)S * 8 + (size_t)N * 2);
   // BDeu of families with 2-3 parents: G4 lane groups per wave (a power of
   // two, so 64 / G4 lanes per group tile the wave; fewer groups when the
   // masks are long), each staging 12 masks, 84 histogram cells (u16) and 28
   // configuration terms
-  const int G4 = ctx->lc.bdeu_groups > 0 ? ctx->lc.bdeu_groups : bitw <= 8 ? 4 : bitw <= 16 ? 2 : 1;
+  int G4 = ctx->lc.bdeu_groups > 0 ? ctx->lc.bdeu_groups : bitw <= 8 ? 4 : bitw <= 16 ? 2 : 1;
+  auto bdeu_bytes = [&](int g4) {
+    return (size_t)EDDY_NWAVES * g4 * (12 * bitw * 8 + 28 * 8) + (((size_t)EDDY_NWAVES * g4 * 84 * 2 + 7) & ~(size_t)7);
+  };
+  while (G4 > 1 && bdeu_bytes(G4) + (size_t)8 * maxG > lmax) G4 >>= 1;
   a.score_groups = G4;
-  const size_t bdeu_lds = (size_t)EDDY_NWAVES * G4 * (12 * bitw * 8 + 28 * 8) +
-                          (((size_t)EDDY_NWAVES * G4 * 84 * 2 + 7) & ~(size_t)7);
-  const size_t score_shared = std::max<size_t>({(size_t)ht * 12, bdeu_lds, (size_t)4 * maxG * 8});
+  const size_t bdeu_lds = bdeu_bytes(G4);
+  // the score kernel's phase-shared region: the network hash table (not GM),
+  // the BDeu mask stage, the base family scores; then the network tables
+  // (not GM) and the full networks' families.  The tail kernel: base scores
+  // (4 maxG), then the KL terms (2 S, not GM) in one region; the network
+  // scores / likelihoods (2 S, not GM) after it.
+  const size_t sh_lds = std::max<size_t>({(size_t)ht * 12, bdeu_lds, (size_t)4 * maxG * 8});
+  const size_t score_lds_l = sh_lds + (size_t)(4 * S + 1 + 2 * maxG) * 4;
+  const size_t tail_lds_l = std::max<size_t>((size_t)4 * maxG, (size_t)2 * S) * 8 + (size_t)2 * S * 8;
+  const bool score_gm = score_lds_l > lmax || tail_lds_l > lmax;
+  const size_t score_shared = score_gm ? std::max<size_t>(bdeu_lds, (size_t)4 * maxG * 8) : sh_lds;
   a.score_shared = (int)score_shared;
-  const size_t score_lds = score_shared + (size_t)(4 * S + 1 + 2 * maxG) * 4;
+  const size_t score_lds = score_gm ? score_shared + (size_t)2 * maxG * 4 : score_lds_l;
+  a.tail_shared = (int)(score_gm ? (size_t)4 * maxG * 8 : std::max<size_t>((size_t)4 * maxG, (size_t)2 * S) * 8);
+  const size_t tail_lds = score_gm ? (size_t)a.tail_shared : tail_lds_l;
+  if (rows_lds > lmax || score_lds > lmax || tail_lds > lmax || bdeu_lds > lmax) {
+    ctx->err = "a set or sample count needs more LDS than the device has";
+    return EDDY_ERR_CAPACITY;
+  }
+  a.gscr = nullptr;
+  a.gscr_item = 0;
+  if (score_gm) {
+    // per item: the network hash table (12 B per entry), rep / evst / work /
+    // ulist (4 S + 1 ints); the tail's KL terms (2 S doubles) fit in it
+    a.gscr_item = (((long long)ht * 12 + (4LL * S + 1) * 4) + 255) & ~255LL;
+    CHK(grow(ctx, sl.gscr, (size_t)a.gscr_item * nitems));
+    a.gscr = (unsigned char *)sl.gscr.p;
+  }
+  const bool wide = mp > EDDY_NARROW_MP;
+  const void *f_rows = rows_gm ? (wide ? (const void *)k_rows_gw : (const void *)k_rows_g)
+                               : (wide ? (const void *)k_rows_w : (const void *)k_rows);
+  const void *f_score = score_gm ? (const void *)k_score_g : (const void *)k_score;
+  const void *f_tail = score_gm ? (const void *)k_score_tail_g : (const void *)k_score_tail;
 
   // the launch sequence: kernels, and the timing events between them
   struct Step {
@@ -701,13 +768,13 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   seq[ns++] = {1, nullptr, 0, 0};
   if (!plain_tests) seq[ns++] = {-1, loo3 ? (const void *)k_loo_w3 : (const void *)k_loo, (unsigned)nitems, 0};
   seq[ns++] = {2, nullptr, 0, 0};
-  seq[ns++] = {-1, (const void *)k_rows, (unsigned)nitems, rows_lds};
+  seq[ns++] = {-1, f_rows, (unsigned)nitems, rows_lds};
   seq[ns++] = {3, nullptr, 0, 0};
-  seq[ns++] = {-1, (const void *)k_score, (unsigned)nitems, score_lds};
+  seq[ns++] = {-1, f_score, (unsigned)nitems, score_lds};
   seq[ns++] = {5, nullptr, 0, 0};
   seq[ns++] = {-1, (const void *)k_bdeu, (unsigned)ctx->lc.bdeu_blocks, bdeu_lds};
   seq[ns++] = {6, nullptr, 0, 0};
-  seq[ns++] = {-1, (const void *)k_score_tail, (unsigned)nitems, tail_lds};
+  seq[ns++] = {-1, f_tail, (unsigned)nitems, tail_lds};
   seq[ns++] = {4, nullptr, 0, 0};
   int *c_ev = (int *)sl.evcount.p, *c_st = (int *)sl.status.p;
   long long *c_te = (long long *)sl.tests.p;
@@ -726,7 +793,8 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
     return kp;
   };
   if (ctx->lc.graph) {
-    SlotGraph &gr = sl.gr[(ch.tiles.empty() ? 0 : 1) | (plain_tests ? 2 : 0) | (loo3 ? 4 : 0)];
+    SlotGraph &gr = sl.gr[(ch.tiles.empty() ? 0 : 1) | (plain_tests ? 2 : 0) | (loo3 ? 4 : 0) | (rows_gm ? 8 : 0) |
+                          (wide ? 16 : 0) | (score_gm ? 32 : 0)];
     if (!gr.x) {
       if (!gr.g) {
         HIPCHK(ctx, hipGraphCreate(&gr.g, 0));
@@ -844,8 +912,10 @@ int grow_keep(eddy_ctx *ctx, DevBuf &b, size_t bytes, size_t keep) {
 // fixed-size parts at offsets of the set layout, events and families
 // appended.  Filled by device-to-device slice copies, read back per set only
 // when the caller asks for networks or edge lists.
-int p0_begin(eddy_ctx *ctx, const std::vector<SetMeta> &meta) {
+int p0_begin(eddy_ctx *ctx, const std::vector<SetMeta> &meta, const eddy_params *p) {
   P0Store &q = ctx->p0;
+  q.mpw = mp_eff(p, meta);
+  q.evw = q.mpw + 1;
   const int nsets = (int)meta.size(), S = ctx->N + 2;
   long long sumc = 0, sumg = 0;
   for (const SetMeta &m : meta) {
@@ -853,8 +923,8 @@ int p0_begin(eddy_ctx *ctx, const std::vector<SetMeta> &meta) {
     sumg += m.G;
   }
   CHK(grow(ctx, q.info, sizeof(uint32_t) * std::max(1LL, sumc)));
-  CHK(grow(ctx, q.lcnt, 2 * std::max(1LL, sumg)));
-  CHK(grow(ctx, q.lfull, sizeof(uint16_t) * EDDY_MAX_MP * 2 * std::max(1LL, sumg)));
+  CHK(grow(ctx, q.lcnt, sizeof(uint16_t) * 2 * std::max(1LL, sumg)));
+  CHK(grow(ctx, q.lfull, sizeof(uint16_t) * (size_t)q.mpw * 2 * std::max(1LL, sumg)));
   CHK(grow(ctx, q.rep, sizeof(int) * (size_t)S * std::max(1, nsets)));
   CHK(grow(ctx, q.lval, sizeof(double) * 2 * (size_t)S * std::max(1, nsets)));
   q.meta = meta;
@@ -882,7 +952,9 @@ int capture_items(eddy_ctx *ctx, Slot &sl, const std::vector<int> &idx, const Ch
     add_ev += std::min(o.evc[k], ch.items[k].evcap);
     add_fam += o.fc[k];
   }
-  CHK(grow_keep(ctx, q.ev, sizeof(Event) * std::max(1LL, q.evtot + add_ev), sizeof(Event) * q.evtot));
+  CHK(grow_keep(ctx, q.evkey, sizeof(uint32_t) * std::max(1LL, q.evtot + add_ev), sizeof(uint32_t) * q.evtot));
+  CHK(grow_keep(ctx, q.evpar, sizeof(uint16_t) * q.evw * std::max(1LL, q.evtot + add_ev),
+                sizeof(uint16_t) * q.evw * q.evtot));
   CHK(grow_keep(ctx, q.famkey, sizeof(uint64_t) * std::max(1LL, q.famtot + add_fam), sizeof(uint64_t) * q.famtot));
   CHK(grow_keep(ctx, q.famscore, sizeof(double) * 2 * std::max(1LL, q.famtot + add_fam),
                 sizeof(double) * 2 * q.famtot));
@@ -895,15 +967,16 @@ int capture_items(eddy_ctx *ctx, Slot &sl, const std::vector<int> &idx, const Ch
     const SetMeta &sm = q.meta[im.set];
     const int set = im.set;
     add((const uint32_t *)sl.info.p + im.infooff, (uint32_t *)q.info.p + sm.pairbase, 4LL * sm.c);
-    add((const uint8_t *)sl.lcnt.p + im.goff * 2, (uint8_t *)q.lcnt.p + sm.genebase * 2, 2LL * sm.G);
-    add((const uint16_t *)sl.lfull.p + im.goff * 2 * EDDY_MAX_MP, (uint16_t *)q.lfull.p + sm.genebase * 2 * EDDY_MAX_MP,
-        2LL * 2 * sm.G * EDDY_MAX_MP);
+    add((const uint16_t *)sl.lcnt.p + im.goff * 2, (uint16_t *)q.lcnt.p + sm.genebase * 2, 2LL * 2 * sm.G);
+    add((const uint16_t *)sl.lfull.p + im.goff * 2 * q.mpw, (uint16_t *)q.lfull.p + sm.genebase * 2 * q.mpw,
+        2LL * 2 * sm.G * q.mpw);
     add((const int *)sl.rep.p + im.soff, (int *)q.rep.p + (size_t)set * S, 4LL * S);
     add((const double *)sl.lval.p + im.soff * 2, (double *)q.lval.p + (size_t)set * 2 * S, 16LL * S);
     const int ne = std::min(o.evc[k], im.evcap);
     q.evoff[set] = q.evtot;
     q.evn[set] = ne;
-    add((const Event *)sl.ev.p + im.evoff, (Event *)q.ev.p + q.evtot, (long long)sizeof(Event) * ne);
+    add((const uint32_t *)sl.evkey.p + im.evoff, (uint32_t *)q.evkey.p + q.evtot, 4LL * ne);
+    add((const uint16_t *)sl.evpar.p + im.evoff * q.evw, (uint16_t *)q.evpar.p + q.evtot * q.evw, 2LL * q.evw * ne);
     q.evtot += ne;
     q.famoff[set] = q.famtot;
     q.famn[set] = o.fc[k];
@@ -950,10 +1023,11 @@ int p0_fetch(eddy_ctx *ctx, int set) {
   h.U = q.U[set];
   h.info.resize(sm.c);
   h.lcnt.resize(2 * sm.G);
-  h.lfull.resize((size_t)2 * sm.G * EDDY_MAX_MP);
+  h.lfull.resize((size_t)2 * sm.G * q.mpw);
   h.rep.resize(S);
   h.lval.resize(2 * S);
-  h.ev.resize(q.evn[set]);
+  h.evkey.resize(q.evn[set]);
+  h.evpar.resize((size_t)q.evw * q.evn[set]);
   h.famkey.resize(q.famn[set]);
   h.famscore.resize(2 * q.famn[set]);
   auto get = [&](void *dst, const void *src, size_t bytes) -> int {
@@ -961,11 +1035,12 @@ int p0_fetch(eddy_ctx *ctx, int set) {
     return EDDY_OK;
   };
   CHK(get(h.info.data(), (const uint32_t *)q.info.p + sm.pairbase, 4 * (size_t)sm.c));
-  CHK(get(h.lcnt.data(), (const uint8_t *)q.lcnt.p + sm.genebase * 2, 2 * (size_t)sm.G));
-  CHK(get(h.lfull.data(), (const uint16_t *)q.lfull.p + sm.genebase * 2 * EDDY_MAX_MP, 2 * h.lfull.size()));
+  CHK(get(h.lcnt.data(), (const uint16_t *)q.lcnt.p + sm.genebase * 2, 2 * h.lcnt.size()));
+  CHK(get(h.lfull.data(), (const uint16_t *)q.lfull.p + sm.genebase * 2 * q.mpw, 2 * h.lfull.size()));
   CHK(get(h.rep.data(), (const int *)q.rep.p + (size_t)set * S, 4 * (size_t)S));
   CHK(get(h.lval.data(), (const double *)q.lval.p + (size_t)set * 2 * S, 16 * (size_t)S));
-  CHK(get(h.ev.data(), (const Event *)q.ev.p + q.evoff[set], sizeof(Event) * h.ev.size()));
+  CHK(get(h.evkey.data(), (const uint32_t *)q.evkey.p + q.evoff[set], 4 * h.evkey.size()));
+  CHK(get(h.evpar.data(), (const uint16_t *)q.evpar.p + q.evoff[set] * q.evw, 2 * h.evpar.size()));
   CHK(get(h.famkey.data(), (const uint64_t *)q.famkey.p + q.famoff[set], 8 * h.famkey.size()));
   CHK(get(h.famscore.data(), (const double *)q.famscore.p + q.famoff[set] * 2, 8 * h.famscore.size()));
   return EDDY_OK;
@@ -979,7 +1054,7 @@ int start_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, Chunk
   sl.ch = std::move(ch);
   const int nitems = (int)sl.ch.items.size();
   sl.cap.resize(nitems);
-  for (int i = 0; i < nitems; i++) sl.cap[i] = std::max(512, 2 * S + 24 * sets[sl.ch.items[i].set].G);
+  for (int i = 0; i < nitems; i++) sl.cap[i] = item_evcap(S, sets[sl.ch.items[i].set].G);
   if (nitems == 0) return EDDY_OK;
   return launch_chunk(ctx, sl, sets, p, perms, sl.cap, 0);
 }
@@ -1071,6 +1146,7 @@ void add_item(Chunk &ch, const std::vector<SetMeta> &sets, int set, int perm, co
   ch.pairs += sm.c;
   ch.sslots += N + 2;
   ch.gslots += sm.G;
+  ch.evslots += item_evcap(N + 2, sm.G);
 }
 
 std::vector<uint32_t> label_bits(const std::vector<uint8_t> &cls) {
@@ -1406,7 +1482,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
   const int N = ctx->N, perms = p->perms, S = N + 2;
   std::vector<SetMeta> meta;
   CHK(prepare_sets(ctx, sets, nsets, p, meta));
-  CHK(p0_begin(ctx, meta));
+  CHK(p0_begin(ctx, meta, p));
   GROW(b_js, sizeof(double) * (size_t)std::max(1, nsets) * perms);
   HIPCHK(ctx, hipMemsetAsync(ctx->b_js.p, 0, sizeof(double) * (size_t)std::max(1, nsets) * perms, ctx->stream));
   GROW(b_counters, sizeof(unsigned long long) * 32);
@@ -1494,6 +1570,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     };
     const long long budget_min = 8LL << 20, budget_max = 64LL << 20;
+    const int evw = mp_eff(p, meta) + 1;
     long long budget = budget_min;
     const int words = (N + 31) / 32;
     std::vector<uint32_t> one(words);
@@ -1625,7 +1702,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       std::vector<int> stops;
       for (int q = 0; q <= nsets; q++) {
         if (q < nsets && meta[q].G >= 2) add_item(c0, meta, q, 0, truebits, N);
-        if (!c0.items.empty() && (q == nsets || c0.pairs >= budget_max)) {
+        if (!c0.items.empty() && (q == nsets || c0.pairs >= budget_max || chunk_bytes(c0, evw) >= kChunkBytes)) {
           if (inflight.size() == EDDY_SLOTS) CHK(finish_oldest(stops));
           CHK(start_on_slot(std::move(c0), f0, std::min(q, nsets - 1), true));
           c0 = Chunk();
@@ -1668,7 +1745,8 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       // labels are not ready yet: the GPU then works while the replay
       // catches up
       const bool starved = s < nsets && prod.ready.load(std::memory_order_acquire) <= s;
-      if (!ch.items.empty() && (ch.pairs >= budget || boundary || (starved && ch.pairs >= (budget >> 3)))) {
+      if (!ch.items.empty() && (ch.pairs >= budget || chunk_bytes(ch, evw) >= kChunkBytes || boundary ||
+                                (starved && ch.pairs >= (budget >> 3)))) {
         if (inflight.size() == EDDY_SLOTS) {
           std::vector<int> stops;
           bool rolled;
@@ -1919,11 +1997,12 @@ extern "C" int eddy_get_perm0_networks(eddy_ctx *ctx, int set, int *ids, int *no
   if (!ctx || set < 0 || set >= (int)ctx->last_p0.size()) return EDDY_ERR_ARG;
   CHK(p0_fetch(ctx, set));
   const Perm0 &q = ctx->last_p0[set];
-  const int G = ctx->last_G[set], S = ctx->N + 2, n1 = ctx->n1;
-  // events by network, sorted by row
-  std::vector<std::vector<const Event *>> evs(S);
-  for (const Event &e : q.ev) evs[e.s].push_back(&e);
-  for (auto &v : evs) std::sort(v.begin(), v.end(), [](const Event *x, const Event *y) { return x->t < y->t; });
+  const int G = ctx->last_G[set], S = ctx->N + 2, n1 = ctx->n1, mpw = ctx->p0.mpw, evw = ctx->p0.evw;
+  // events by network, sorted by row (a network has at most one per row)
+  std::vector<std::vector<int>> evs(S);
+  for (int e = 0; e < (int)q.evkey.size(); e++) evs[q.evkey[e] & 0xffffu].push_back(e);
+  auto row_of = [&](int e) { return (int)(q.evkey[e] >> 16); };
+  for (auto &v : evs) std::sort(v.begin(), v.end(), [&](int x, int y) { return row_of(x) < row_of(y); });
   int u = 0, ne = 0;
   if (epn) epn[0] = 0;
   for (int s = 0; s < S; s++) {
@@ -1938,12 +2017,13 @@ extern "C" int eddy_get_perm0_networks(eddy_ctx *ctx, int set, int *ids, int *no
       }
       const uint16_t *par;
       int cnt;
-      while (k < evs[s].size() && evs[s][k]->t < t) k++;
-      if (k < evs[s].size() && evs[s][k]->t == t) {
-        par = evs[s][k]->par;
-        cnt = evs[s][k]->n;
+      while (k < evs[s].size() && row_of(evs[s][k]) < t) k++;
+      if (k < evs[s].size() && row_of(evs[s][k]) == t) {
+        const uint16_t *pp = &q.evpar[(size_t)evs[s][k] * evw];
+        par = pp + 1;
+        cnt = pp[0];
       } else {
-        par = &q.lfull[((size_t)g * G + t) * EDDY_MAX_MP];
+        par = &q.lfull[((size_t)g * G + t) * mpw];
         cnt = q.lcnt[(size_t)g * G + t];
       }
       for (int i = 0; i < cnt; i++) {
@@ -2238,7 +2318,7 @@ extern "C" int eddy_run_perm_detail(eddy_ctx *ctx, const eddy_set_desc *set, con
   }
   std::vector<SetMeta> meta;
   CHK(prepare_sets(ctx, set, 1, p, meta));
-  CHK(p0_begin(ctx, meta));
+  CHK(p0_begin(ctx, meta, p));
   GROW(b_js, sizeof(double) * p->perms);
   HIPCHK(ctx, hipMemsetAsync(ctx->b_js.p, 0, sizeof(double) * p->perms, ctx->stream));
   GROW(b_counters, sizeof(unsigned long long) * 32);
@@ -2262,8 +2342,10 @@ extern "C" int eddy_run_perm_detail(eddy_ctx *ctx, const eddy_set_desc *set, con
   out->js = js;
   out->U = q.U;
   int ne = 0;
+  // at most min(mp, G - 1) parents per row
+  const long long ecap = (long long)S * G * std::min(p->mp, G - 1);
   CHK(eddy_get_perm0_networks(ctx, 0, out->uniq_ids, out->uni_nodes, S * G, out->uni_epn, out->uni_edges,
-                              S * G * p->mp, out->lval, &ne));
+                              (int)std::min<long long>(ecap, INT32_MAX), out->lval, &ne));
   out->edSum = ne;
   if (out->bits) {
     // out23 from the per-pair decisions: network s of class g takes the
@@ -2304,7 +2386,7 @@ extern "C" int eddy_run_perm_detail(eddy_ctx *ctx, const eddy_set_desc *set, con
   if (out->out5) {
     // out5[(g*U + u)*G + t] = BDeu of row t's family of unique network u on
     // class g (run4's layout, kernel5.cu:237-257), from the family scores
-    std::vector<int> ids(S), nodes((size_t)S * G), epn(S + 1), edges((size_t)S * G * p->mp);
+    std::vector<int> ids(S), nodes((size_t)S * G), epn(S + 1), edges((size_t)std::max(1, ne));
     int n_e = 0;
     CHK(eddy_get_perm0_networks(ctx, 0, ids.data(), nodes.data(), S * G, epn.data(), edges.data(),
                                 (int)edges.size(), nullptr, &n_e));

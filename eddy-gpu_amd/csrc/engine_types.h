@@ -4,11 +4,12 @@
 
 #include "edgetest.h"
 
-// Engine limits (the reference's are MAX_GENESET_SIZE 200 and a 1024-thread
-// block per network, definitions.cuh:19, main.cu:931).
+// Engine limits.  The reference's: MAX_GENESET_SIZE 200 genes per set (a
+// stack array, definitions.cuh:19, main.cu:239), MAX_NUM_SAMPLES 20000
+// samples (definitions.cuh:17), any positive parent cap (main.cu:170-172).
 #define EDDY_MAX_G 1024     // genes per set (10-bit family keys)
-#define EDDY_MAX_N 2046     // samples (S = N + 2 <= 2048 networks)
-#define EDDY_MAX_MP 16      // parent cap held per row
+#define EDDY_MAX_N 20000    // samples (S = N + 2 networks; sample and network ids fit 16 bits)
+#define EDDY_NARROW_MP 16   // parent caps up to this keep a row list in registers
 #define EDDY_NWAVES 4       // waves per workgroup in the per-item kernels
 
 // Per-pair info word written by the test kernels and read by the network
@@ -56,11 +57,6 @@ struct ItemMeta {   // one (gene set, permutation)
   int fhsize;         // power of two >= 2 * fcap
 };
 
-struct Event {        // a row of one leave-one-out network that differs from its class's full network
-  uint16_t s, t, n, pad;
-  uint16_t par[EDDY_MAX_MP];
-};
-
 // one device-to-device copy of the permutation-0 capture
 struct Slice {
   const void *src;
@@ -75,7 +71,8 @@ struct Slice {
 struct KArgs {
   // dataset-level
   int N, n1, n2, S;
-  int mp, p_adjust, plain;
+  int mp, p_adjust, plain;  // mp: the parent cap, at most the largest set's G - 1 (no row has more parents)
+  int mpw;                 // u16 per row in lfull: mp
   int feat_only;           // count kernel: pooled-table features only (no tests, no outputs)
   int ht_size;             // network hash table entries (power of two >= 1.5 S)
   int score_shared;        // bytes of the score kernel's phase-shared LDS region
@@ -99,10 +96,9 @@ struct KArgs {
   uint16_t *tab;           // [pair][2][EDDY_TAB_CLASS] counts (cutoff mode)
   uint16_t *totals;        // [set pairbase + p][EDDY_TAB_CLASS]: pooled table of each pair over all samples
   // per-item scratch
-  uint16_t *lfull;         // [goff*2 + g*G + t][mp]
-  uint8_t *lcnt;           // [goff*2 + g*G + t]
+  uint16_t *lfull;         // [goff*2 + g*G + t][mpw]
+  uint16_t *lcnt;          // [goff*2 + g*G + t]
   uint8_t *sflags;         // [goff*2 + g*G + t] observed-state flags (bit v)
-  Event *ev;
   int *evcount;            // [item]
   uint64_t *hnet;          // [soff + s]
   int *rep;                // [soff + s] first occurrence of each network
@@ -144,6 +140,12 @@ struct KArgs {
   uint32_t *fbits;         // [(goff*2 + g*G + t) * fbw + w]: full edges of row t, class g, columns 32w..32w+31
   int fbw, fbwp;           // words per row (ceil(largest G / 32)) and the power of two >= it
   int tile_order;          // count kernel: 1 = each XCD takes a contiguous run of the tile list
+  // per-item global scratch of the score kernels when their network tables do
+  // not fit LDS (sample counts in the thousands): item i owns bytes
+  // [i * gscr_item, (i + 1) * gscr_item): the network hash table, then
+  // rep / evst / work / ulist (score kernel), later the KL terms (tail)
+  unsigned char *gscr;
+  long long gscr_item;
 };
 
 enum {

@@ -206,13 +206,21 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
   constexpr int NOPS = MODE == K1_BOTH ? 3 : 2, NTAB = MODE == K1_BOTH ? 2 : 1;
   constexpr int OPS_B = NOPS * 96 * K1_RS, TAB_B = NTAB * 96 * K1_TS * 2;
   __shared__ __align__(16) uint8_t rows[OPS_B > TAB_B ? OPS_B : TAB_B];
-  __shared__ __align__(16) uint8_t clsb[EDDY_MAX_N + 66];
   __shared__ int tcount;
   uint8_t *opA1 = rows, *opA2 = rows + 96 * K1_RS, *opB = rows + (NOPS - 1) * 96 * K1_RS;
 
-  // class-1 membership byte per sample (label bit 0 = class 1)
+  // class-1 membership of samples k..k+15 (k a multiple of 16) as one byte
+  // per sample, from the label bits (bit = class 2); samples past N are in
+  // neither class (their states are 3, one-hot zero, anyway)
   const uint32_t *lab = a.labels + im.laboff;
-  for (int s = tid; s < NP; s += 256) clsb[s] = (s < N) ? (uint8_t)(((lab[s >> 5] >> (s & 31)) & 1) ^ 1) : 0;
+  auto class1_bytes = [&](int k) {
+    uint32_t b = k < N ? ~(lab[k >> 5] >> (k & 31)) & 0xffffu : 0u;
+    if (k + 16 > N && k < N) b &= (1u << (N - k)) - 1u;
+    // bit i of a nibble to byte i: the four shifted copies do not overlap
+    const uint32_t spread = 0x00204081u;
+    return make_uint4((b & 15u) * spread & 0x01010101u, ((b >> 4) & 15u) * spread & 0x01010101u,
+                      ((b >> 8) & 15u) * spread & 0x01010101u, ((b >> 12) & 15u) * spread & 0x01010101u);
+  };
   if (tid == 0) tcount = 0;
   unsigned long long tph = 0;
   phase_mark(a, -1, tph);
@@ -302,7 +310,7 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
           *(uint4 *)(d1 + K1_RS) = h1;
           *(uint4 *)(d1 + 2 * K1_RS) = h2;
         } else {
-          const uint4 m = *(const uint4 *)(clsb + k0 + c * 16);
+          const uint4 m = class1_bytes(k0 + c * 16);
           *(uint4 *)(d1) = make_uint4(h0.x & m.x, h0.y & m.y, h0.z & m.z, h0.w & m.w);
           *(uint4 *)(d1 + K1_RS) = make_uint4(h1.x & m.x, h1.y & m.y, h1.z & m.z, h1.w & m.w);
           *(uint4 *)(d1 + 2 * K1_RS) = make_uint4(h2.x & m.x, h2.y & m.y, h2.z & m.z, h2.w & m.w);
@@ -770,10 +778,18 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
+// hash of row t's parent list par[0..n): begin, one step per parent, end
+__device__ __forceinline__ uint64_t row_hash_begin(uint64_t seed, int t) {
+  return mix64(seed ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(t + 1)));
+}
+__device__ __forceinline__ uint64_t row_hash_step(uint64_t h, int par, int i) {
+  return mix64(h ^ (((uint64_t)par + 1) << 16) ^ (uint64_t)i);
+}
+__device__ __forceinline__ uint64_t row_hash_end(uint64_t h, int n) { return mix64(h ^ (uint64_t)n * 0xD6E8FEB86659FD93ULL); }
 __device__ __forceinline__ uint64_t row_hash(uint64_t seed, int t, int n, const uint16_t *par) {
-  uint64_t h = mix64(seed ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(t + 1)));
-  for (int i = 0; i < n; i++) h = mix64(h ^ (((uint64_t)par[i] + 1) << 16) ^ (uint64_t)i);
-  return mix64(h ^ (uint64_t)n * 0xD6E8FEB86659FD93ULL);
+  uint64_t h = row_hash_begin(seed, t);
+  for (int i = 0; i < n; i++) h = row_hash_step(h, par[i], i);
+  return row_hash_end(h, n);
 }
 
 // class lists: cls[g][k] = k-th sample of class g in ascending sample order
@@ -825,36 +841,61 @@ __device__ void build_class_lists(const uint32_t *lab, int N, uint16_t *cls0, ui
 // row differs (events).  Network hash = XOR over rows of row_hash.
 // One workgroup (4 waves) per item; a wave owns one row at a time.
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_rows(KArgs a) {
+// GM: the S network deltas and the class lists in global memory (the item's
+// hash words and class-list slots) instead of LDS, for sample counts whose
+// arrays do not fit; WIDE: parent caps above EDDY_NARROW_MP, where a
+// leave-one-out row list is compared with the full one as it is produced and
+// written straight to the event record (no per-lane copy).
+template <bool GM, bool WIDE>
+__device__ __forceinline__ void rows_body(const KArgs &a) {
   const int item = a.order[blockIdx.x];  // largest items first (shorter tail)
   const ItemMeta im = a.items[item];
   const SetMeta sm = a.sets[im.set];
-  const int G = sm.G, N = a.N, S = a.S, mp = a.mp;
+  const int G = sm.G, N = a.N, S = a.S, mp = a.mp, MPW = a.mpw;
   const int NP = (N + 63) & ~63;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint8_t *xs = a.xs + sm.xoff;
 
-  // LDS sized by the launch (rows_lds_bytes): S network deltas, per-wave
-  // candidate columns of the largest set, the class lists, state flags
+  // LDS sized by the launch (rows_lds_bytes): the S network deltas (not GM),
+  // per-wave candidate columns of the largest set, per-wave full row lists,
+  // the class lists (not GM), state flags
   extern __shared__ __align__(16) unsigned char rdyn[];
-  const int GM = a.rows_maxg;
-  unsigned long long *delta = (unsigned long long *)rdyn;                  // [S]
-  uint32_t *candb = (uint32_t *)(delta + S);                               // [4][GM]: col | flip << 16 | full << 25
-  uint16_t *clsb = (uint16_t *)(candb + EDDY_NWAVES * GM);                 // [N]: class 1, then class 2
-  uint8_t *sflb = (uint8_t *)(clsb + N);                                   // [2][GM]
+  const int GMX = a.rows_maxg;
+  unsigned char *lp = rdyn;
+  unsigned long long *delta;
+  if (GM) {
+    delta = (unsigned long long *)(a.hnet + im.soff);
+  } else {
+    delta = (unsigned long long *)lp;
+    lp += (size_t)S * 8;
+  }
+  uint32_t *candb = (uint32_t *)lp;                                        // [4][GMX]: col | flip << 16 | full << 25
+  lp += (size_t)EDDY_NWAVES * GMX * 4;
+  uint16_t *lrowb = (uint16_t *)lp;                                        // [4][MPW]
+  lp += (size_t)EDDY_NWAVES * MPW * 2;
+  uint16_t *clsb;                                                          // [N]: class 1, then class 2
+  if (GM) {
+    clsb = a.clsg + im.soff;
+  } else {
+    clsb = (uint16_t *)lp;
+    lp += (size_t)N * 2;
+  }
+  uint8_t *sflb = (uint8_t *)lp;                                           // [2][GMX]
   uint16_t *cls0 = clsb, *cls1 = clsb + a.n1;
-  uint32_t *cand_w = candb + wave * GM;
+  uint32_t *cand_w = candb + wave * GMX;
+  uint16_t *lrow = lrowb + wave * MPW;
   __shared__ int ncls[2];
   __shared__ unsigned long long hfull[2];
-  __shared__ uint16_t lrow[EDDY_NWAVES][EDDY_MAX_MP];
 
   unsigned long long tph = 0;
   phase_mark(a, -1, tph);
   build_class_lists(a.labels + im.laboff, N, cls0, cls1, ncls);
-  for (int k = tid; k < ncls[0]; k += 256) a.clsg[im.soff + k] = cls0[k];
-  for (int k = tid; k < ncls[1]; k += 256) a.clsg[im.soff + a.n1 + k] = cls1[k];
+  if (!GM) {
+    for (int k = tid; k < ncls[0]; k += 256) a.clsg[im.soff + k] = cls0[k];
+    for (int k = tid; k < ncls[1]; k += 256) a.clsg[im.soff + a.n1 + k] = cls1[k];
+  }
   for (int s = tid; s < S; s += 256) delta[s] = 0ULL;
-  for (int i = tid; i < 2 * GM; i += 256) sflb[i] = 0;
+  for (int i = tid; i < 2 * GMX; i += 256) sflb[i] = 0;
   if (tid == 0) hfull[0] = hfull[1] = 0ULL;
   __syncthreads();
 
@@ -903,14 +944,14 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
         }
         // this wave owns the gene in every chunk: a plain read-modify-write
         if (lane == 0)
-          sflb[g * GM + gene] |= (uint8_t)((any0 ? 1u : 0u) | (any1 ? 2u : 0u) | (any2 ? 4u : 0u));
+          sflb[g * GMX + gene] |= (uint8_t)((any0 ? 1u : 0u) | (any1 ? 2u : 0u) | (any2 ? 4u : 0u));
       }
     }
   }
   __syncthreads();
   for (int i = tid; i < 2 * G; i += 256) {
     const int g = i / G, gene = i - g * G;
-    a.sflags[im.goff * 2 + i] = sflb[g * GM + gene];
+    a.sflags[im.goff * 2 + i] = sflb[g * GMX + gene];
   }
   __syncthreads();
 
@@ -946,7 +987,7 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
       }
       uint64_t hm = __ballot(inRange && full);
       int rank = __popcll(hm & lt_mask);
-      if (inRange && full && nL + rank < mp) lrow[wave][nL + rank] = (uint16_t)col;
+      if (inRange && full && nL + rank < mp) lrow[nL + rank] = (uint16_t)col;
       nL += __popcll(hm);
       if (nL > mp) nL = mp;
       uint32_t flip = loo ^ (full ? 0x1FFu : 0u);
@@ -961,9 +1002,13 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
     wave_sync();
     // full row list
     const size_t lbase = (size_t)(im.goff * 2 + g * G + t);
-    if (lane < nL) a.lfull[lbase * EDDY_MAX_MP + lane] = lrow[wave][lane];
-    if (lane == 0) a.lcnt[lbase] = (uint8_t)nL;
-    uint64_t hrow = row_hash(a.hseed, t, nL, lrow[wave]);
+    if (!WIDE) {
+      if (lane < nL) a.lfull[lbase * MPW + lane] = lrow[lane];
+    } else {
+      for (int i = lane; i < nL; i += 64) a.lfull[lbase * MPW + i] = lrow[i];
+    }
+    if (lane == 0) a.lcnt[lbase] = (uint16_t)nL;
+    uint64_t hrow = row_hash(a.hseed, t, nL, lrow);
     if (lane == 0) atomicXor(&hfull[g], (unsigned long long)hrow);
     if (nflip == 0) {
       wave_sync();
@@ -986,60 +1031,94 @@ __global__ __launch_bounds__(256) void k_rows(KArgs a) {
       if (!changed) continue;
       // the sample's row list: the candidates in column order, each a full
       // edge toggled where its flip mask covers the sample's (vt, vc) cell
-      uint16_t nl[EDDY_MAX_MP];
-      int cnt = 0;
-      for (int i = 0; i < ncand && cnt < mp; i++) {
+      auto cand_bit = [cand_w, xs, NP, sid, vt](int i, int &col) -> int {
         const uint32_t e = cand_w[i], fm = (e >> 16) & 0x1FFu;
-        const int col = (int)(e & 0xffff);
+        col = (int)(e & 0xffff);
         int bit = (int)((e >> 25) & 1u);
         if (fm) {
           const int vc = xs[(size_t)col * NP + sid];
           if (vc < 3) bit ^= (fm >> (vt * 3 + vc)) & 1;
         }
-        if (bit) nl[cnt++] = (uint16_t)col;
-      }
-      bool same = cnt == nL;
-      for (int i = 0; i < cnt && same; i++) same = nl[i] == lrow[wave][i];
-      if (same) continue;
-      int s = g == 0 ? 1 + k : a.n1 + 1 + k;
-      int e = atomicAdd(&a.evcount[item], 1);
-      if (e < im.evcap) {
-        // compact form for the score kernels; the full record only where
-        // permutation 0's events are captured
-        a.evkey[im.evoff + e] = (uint32_t)s | ((uint32_t)t << 16);
-        uint16_t *pp = a.evpar + (size_t)(im.evoff + e) * a.evw;
-        if (a.evw == 4) {  // mp = 3: n and three parents in one 8-byte store
-          uint64_t v = (uint64_t)cnt;
-          for (int i = 0; i < 3; i++)
-            if (i < cnt) v |= (uint64_t)nl[i] << (16 * (i + 1));
-          *(uint64_t *)pp = v;
+        return bit;
+      };
+      const int s = g == 0 ? 1 + k : a.n1 + 1 + k;
+      if (!WIDE) {
+        uint16_t nl[EDDY_NARROW_MP];
+        int cnt = 0;
+        for (int i = 0; i < ncand && cnt < mp; i++) {
+          int col;
+          if (cand_bit(i, col)) nl[cnt++] = (uint16_t)col;
+        }
+        bool same = cnt == nL;
+        for (int i = 0; i < cnt && same; i++) same = nl[i] == lrow[i];
+        if (same) continue;
+        int e = atomicAdd(&a.evcount[item], 1);
+        if (e < im.evcap) {
+          a.evkey[im.evoff + e] = (uint32_t)s | ((uint32_t)t << 16);
+          uint16_t *pp = a.evpar + (size_t)(im.evoff + e) * a.evw;
+          if (a.evw == 4) {  // mp = 3: n and three parents in one 8-byte store
+            uint64_t v = (uint64_t)cnt;
+            for (int i = 0; i < 3; i++)
+              if (i < cnt) v |= (uint64_t)nl[i] << (16 * (i + 1));
+            *(uint64_t *)pp = v;
+          } else {
+            pp[0] = (uint16_t)cnt;
+            for (int i = 0; i < cnt; i++) pp[1 + i] = nl[i];
+          }
         } else {
-          pp[0] = (uint16_t)cnt;
-          for (int i = 0; i < cnt; i++) pp[1 + i] = nl[i];
+          atomicOr(&a.status[item], ST_EV_OVERFLOW);
         }
-        if (im.perm == 0) {
-          Event ev;
-          ev.s = (uint16_t)s;
-          ev.t = (uint16_t)t;
-          ev.n = (uint16_t)cnt;
-          ev.pad = 0;
-          for (int i = 0; i < EDDY_MAX_MP; i++) ev.par[i] = i < cnt ? nl[i] : 0;
-          a.ev[im.evoff + e] = ev;
-        }
+        atomicXor(&delta[s], (unsigned long long)(hrow ^ row_hash(a.hseed, t, cnt, nl)));
       } else {
-        atomicOr(&a.status[item], ST_EV_OVERFLOW);
+        // pass 1: compare with the full row list as the list is produced
+        int cnt = 0;
+        bool same = true;
+        for (int i = 0; i < ncand && cnt < mp; i++) {
+          int col;
+          if (cand_bit(i, col)) {
+            same = same && cnt < nL && lrow[cnt] == (uint16_t)col;
+            cnt++;
+          }
+        }
+        if (same && cnt == nL) continue;
+        // pass 2: the event record and the row's hash
+        int e = atomicAdd(&a.evcount[item], 1);
+        uint16_t *pp = nullptr;
+        if (e < im.evcap) {
+          a.evkey[im.evoff + e] = (uint32_t)s | ((uint32_t)t << 16);
+          pp = a.evpar + (size_t)(im.evoff + e) * a.evw;
+          pp[0] = (uint16_t)cnt;
+        } else {
+          atomicOr(&a.status[item], ST_EV_OVERFLOW);
+        }
+        uint64_t h = row_hash_begin(a.hseed, t);
+        int c2 = 0;
+        for (int i = 0; i < ncand && c2 < mp; i++) {
+          int col;
+          if (cand_bit(i, col)) {
+            if (pp) pp[1 + c2] = (uint16_t)col;
+            h = row_hash_step(h, col, c2);
+            c2++;
+          }
+        }
+        atomicXor(&delta[s], (unsigned long long)(hrow ^ row_hash_end(h, cnt)));
       }
-      atomicXor(&delta[s], (unsigned long long)(hrow ^ row_hash(a.hseed, t, cnt, nl)));
     }
     wave_sync();
   }
   __syncthreads();
   for (int s = tid; s < S; s += 256) {
     int g = (s <= a.n1) ? 0 : 1;
-    a.hnet[im.soff + s] = hfull[g] ^ delta[s];
+    if (GM) a.hnet[im.soff + s] ^= hfull[g];  // delta[s] is that word
+    else a.hnet[im.soff + s] = hfull[g] ^ delta[s];
   }
   phase_mark(a, 8, tph);
 }
+
+__global__ __launch_bounds__(256) void k_rows(KArgs a) { rows_body<false, false>(a); }
+__global__ __launch_bounds__(256) void k_rows_g(KArgs a) { rows_body<true, false>(a); }
+__global__ __launch_bounds__(256) void k_rows_w(KArgs a) { rows_body<false, true>(a); }
+__global__ __launch_bounds__(256) void k_rows_gw(KArgs a) { rows_body<true, true>(a); }
 
 // --------------------------------------------------------------------------
 // K4: dedup, BDeu, likelihood, JS.  One workgroup per item.
@@ -1071,7 +1150,7 @@ __device__ __forceinline__ int row_list(const KArgs &a, const ItemMeta &im, int 
   }
   int g = net_class(s, a.n1);
   size_t lb = (size_t)(im.goff * 2 + g * G + t);
-  *out = a.lfull + lb * EDDY_MAX_MP;
+  *out = a.lfull + lb * a.mpw;
   return a.lcnt[lb];
 }
 
@@ -1234,25 +1313,30 @@ __device__ __forceinline__ void bdeu_stage(unsigned char *base, int G4, int W, i
           (wave * G4 + gi) * 28;
 }
 
-__global__ __launch_bounds__(256) void k_score(KArgs a) {
+// GM: the network hash table and the S-sized network tables in the item's
+// global scratch (a.gscr) instead of LDS, for sample counts whose tables do
+// not fit
+template <bool GM>
+__device__ __forceinline__ void score_body(const KArgs &a) {
   const int item = a.order[blockIdx.x];  // largest items first (shorter tail)
   const ItemMeta im = a.items[item];
   const SetMeta sm = a.sets[im.set];
-  const int G = sm.G, S = a.S, n1 = a.n1;
+  const int G = sm.G, S = a.S, n1 = a.n1, MPW = a.mpw;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int HT = a.ht_size;
 
   extern __shared__ __align__(16) unsigned char dyn[];
-  // a region shared by phases (the network hash table in phase 1, the BDeu
-  // mask stage in phase 4, the base family scores in phase 5, the KL terms
-  // in phase 6), then the network tables
-  unsigned long long *hkeys = (unsigned long long *)dyn;  // [HT]
-  int *hvals = (int *)(hkeys + HT);                       // [HT]
-  int *rep = (int *)(dyn + a.score_shared);               // [S]
+  // a region shared by phases (the network hash table in phase 1 unless GM,
+  // the BDeu mask stage in phase 4), then the network tables (unless GM) and
+  // the full networks' families
+  unsigned char *gs = GM ? a.gscr + (size_t)item * a.gscr_item : nullptr;
+  unsigned long long *hkeys = GM ? (unsigned long long *)gs : (unsigned long long *)dyn;  // [HT]
+  int *hvals = (int *)(hkeys + HT);                                                         // [HT]
+  int *rep = GM ? hvals + HT : (int *)(dyn + a.score_shared);                               // [S]
   int *evst = rep + S;                                    // [S + 1]
   int *work = evst + S + 1;                               // [S]: scratch counts
   int *ulist = work + S;                                  // [S]: unique networks in order
-  int *famfull = ulist + S;                               // [2G]
+  int *famfull = GM ? (int *)(dyn + a.score_shared) : ulist + S;  // [2G]
   __shared__ int wsum[257];
   __shared__ int U, F, flag;
 
@@ -1329,7 +1413,7 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
     const size_t l0 = (size_t)(im.goff * 2 + t), l1 = l0 + G;
     const int c0 = a.lcnt[l0];
     bool d = c0 != a.lcnt[l1];
-    for (int i = 0; i < c0 && !d; i++) d = a.lfull[l0 * EDDY_MAX_MP + i] != a.lfull[l1 * EDDY_MAX_MP + i];
+    for (int i = 0; i < c0 && !d; i++) d = a.lfull[l0 * MPW + i] != a.lfull[l1 * MPW + i];
     dflag[t] = d ? 1 : 0;
   }
   if (tid == 0) ndiff = 0;
@@ -1439,7 +1523,7 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
     if (i < 2 * G) {
       int g = i / G, t = i - g * G;
       size_t lb = (size_t)(im.goff * 2 + g * G + t);
-      key = fam_key(t, a.lcnt[lb], a.lfull + lb * EDDY_MAX_MP);
+      key = fam_key(t, a.lcnt[lb], a.lfull + lb * MPW);
     } else {
       const int e = i - 2 * G;
       const uint32_t k = ek[e];
@@ -1467,7 +1551,7 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   for (int i = tid; i < 2 * G; i += 256) {
     int g = i / G, t = i - g * G;
     size_t lb = (size_t)(im.goff * 2 + g * G + t);
-    int slot = ht_find(fk, fam_key(t, a.lcnt[lb], a.lfull + lb * EDDY_MAX_MP), FH);
+    int slot = ht_find(fk, fam_key(t, a.lcnt[lb], a.lfull + lb * MPW), FH);
     famfull[i] = slot >= 0 ? fv[slot] : -1;
   }
   for (int e = tid; e < E; e += 256) {
@@ -1588,11 +1672,17 @@ __global__ __launch_bounds__(256) void k_score(KArgs a) {
   phase_mark(a, 4, tph);
 }
 
+__global__ __launch_bounds__(256) void k_score(KArgs a) { score_body<false>(a); }
+__global__ __launch_bounds__(256) void k_score_g(KArgs a) { score_body<true>(a); }
+
 // Network scores, likelihoods and JS of every item whose score kernel ran to
 // the end, once every BDeu task of the launch has finished.  One workgroup per
 // item; the unique-network list, event ranges and full families come from
 // the score kernel through global memory.
-__global__ __launch_bounds__(256) void k_score_tail(KArgs a) {
+// GM: the network scores / likelihoods in the item's likelihood slots
+// (a.lval) and the KL terms in its global scratch instead of LDS
+template <bool GM>
+__device__ __forceinline__ void score_tail_body(const KArgs &a) {
   const int item = a.order[blockIdx.x];
   if (a.status[item] & (ST_FAM_OVERFLOW | ST_COLLISION)) return;
   const ItemMeta im = a.items[item];
@@ -1623,7 +1713,7 @@ __global__ __launch_bounds__(256) void k_score_tail(KArgs a) {
   __syncthreads();
   // network scores, then (in place) the likelihoods, in LDS after the
   // phase-shared region: [g*S + u]
-  double *dist = (double *)(dyn + a.tail_shared);
+  double *dist = GM ? a.lval + im.soff * 2 : (double *)(dyn + a.tail_shared);
   for (int ug = tid; ug < 2 * UU; ug += 256) {
     const int u = ug >> 1, g = ug & 1;
     const int s = ulist[u];
@@ -1689,7 +1779,7 @@ __global__ __launch_bounds__(256) void k_score_tail(KArgs a) {
   // serially in network order.  A skipped term (M = 0 or P = 0,
   // routines.cu:47-53) adds +0.0, which leaves every partial sum unchanged:
   // the sums start at +0.0 and no term is -0.0.
-  double *kt = (double *)dyn;
+  double *kt = GM ? (double *)(a.gscr + (size_t)item * a.gscr_item) : (double *)dyn;
   for (int ug = tid; ug < 2 * UU; ug += 256) {
     int g = ug / UU, u = ug - g * UU;
     double p1 = lk[u], p2 = lk[S + u];
@@ -1697,11 +1787,13 @@ __global__ __launch_bounds__(256) void k_score_tail(KArgs a) {
     double pp = g ? p2 : p1;
     kt[ug] = (m == 0 || pp == 0) ? 0.0 : pp * eddy_log(pp / m);
   }
-  // the likelihoods for the permutation-0 captures
-  double *lv = a.lval + im.soff * 2;
-  for (int ug = tid; ug < 2 * UU; ug += 256) {
-    const int g = ug / UU, u = ug - g * UU;
-    lv[g * S + u] = lk[g * S + u];
+  // the likelihoods for the permutation-0 captures (with GM they are there)
+  if (!GM) {
+    double *lv = a.lval + im.soff * 2;
+    for (int ug = tid; ug < 2 * UU; ug += 256) {
+      const int g = ug / UU, u = ug - g * UU;
+      lv[g * S + u] = lk[g * S + u];
+    }
   }
   __syncthreads();
   if (tid == 0) {
@@ -1713,6 +1805,9 @@ __global__ __launch_bounds__(256) void k_score_tail(KArgs a) {
   }
   phase_mark(a, 6, tph);
 }
+
+__global__ __launch_bounds__(256) void k_score_tail(KArgs a) { score_tail_body<false>(a); }
+__global__ __launch_bounds__(256) void k_score_tail_g(KArgs a) { score_tail_body<true>(a); }
 
 // BDeu of every (family, class) task the launch's score kernels listed, in
 // lane groups (G4 per wave, as in the score kernel), the blocks striding over

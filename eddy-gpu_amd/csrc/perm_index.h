@@ -40,6 +40,8 @@ class PermIndex {
   // permutation has n samples of which n1 are class 1.
   PermIndex(const PermStream::State &st, int n, int n1, int workers) : n_(n), n1_(n1) {
     W_ = (n + 63) / 64;
+    c1_.assign(W_, 0);
+    miss_.assign(W_, 0);
     K_ = workers < 1 ? 1 : workers;
     R_ = 2 * K_;
     sh_ = 0;
@@ -68,7 +70,7 @@ class PermIndex {
   // 0..n-1, the position moves past the permutation's draws.
   void class2_bits(uint32_t *c2) {
     const int n = n_, n1 = n1_, W = W_;
-    uint64_t c1[32], miss[32];
+    uint64_t *c1 = c1_.data(), *miss = miss_.data();
     for (int w = 0; w < W; w++) c1[w] = 0;
     long long p = g_ + n;  // discarded draws
     int cnt = 0;
@@ -312,6 +314,7 @@ class PermIndex {
 #endif
 
   int n_, n1_, W_, K_, R_;
+  std::vector<uint64_t> c1_, miss_;  // class-1 residues and the residues still missing (W_ words each)
   int sh_ = 0;
   uint64_t m_ = 0;
   long long g_ = 0;  // draws consumed since the starting state

@@ -267,6 +267,7 @@ std::string load_gene_sets(const std::string &path, const ExprData &expr, std::v
     std::vector<std::string> t = split_tokens(body, '\t');
     GeneSet s;
     s.name = t.empty() ? std::string() : t[0];
+    s.listed = t.size() > 2 ? (int)t.size() - 2 : 0;
     for (size_t i = 2; i < t.size(); i++) {
       auto it = first.find(t[i]);
       if (it == first.end()) continue;
@@ -282,10 +283,8 @@ std::string load_gene_sets(const std::string &path, const ExprData &expr, std::v
 void load_prior(const std::string &dir, GeneSet &set) {
   std::string path = dir + set.name + ".prior";
   FILE *f = fopen(path.c_str(), "r");
-  if (!f) {
-    printf("No prior file exists. Computing without prior knowledge\n");
-    return;
-  }
+  set.prior_file = f != nullptr;
+  if (!f) return;
   const int G = (int)set.names.size();
   std::unordered_map<std::string, int> pos;
   for (int k = G - 1; k >= 0; k--) pos[set.names[k]] = k;  // first match wins

@@ -20,9 +20,11 @@ struct ClassData {
 
 struct GeneSet {
   std::string name;
+  int listed = 0;                   // genes named on the GMT line (before the lookup)
   std::vector<int> rows;            // dataset rows, set order (missing genes dropped)
   std::vector<std::string> names;   // names of the kept genes
   std::vector<int> prior;           // pairs (a, b) of set positions
+  bool prior_file = false;          // PRIORS/<name>.prior was found
 };
 
 // Each loader returns an empty string on success, else the reference's error

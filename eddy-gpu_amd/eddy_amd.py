@@ -378,7 +378,7 @@ class Engine:
         ids = np.zeros(U, np.int32)
         nodes = np.zeros(max(1, U * G), np.int32)
         epn = np.zeros(U + 1, np.int32)
-        edges = np.zeros(max(1, U * G * mp), np.int32)
+        edges = np.zeros(max(1, U * G * min(mp, max(1, G - 1))), np.int32)
         lval = np.zeros(2 * U, np.float64)
         ne = C.c_int()
         self._chk(self.L.eddy_get_perm0_networks(self.h, s, _ptr(ids), _ptr(nodes), len(nodes), _ptr(epn),
@@ -403,7 +403,7 @@ class Engine:
             "uniq_ids": np.zeros(S, np.int32),
             "uni_nodes": np.zeros(S * G, np.int32),
             "uni_epn": np.zeros(S + 1, np.int32),
-            "uni_edges": np.zeros(max(1, S * G * params.mp), np.int32),
+            "uni_edges": np.zeros(max(1, S * G * min(params.mp, max(1, G - 1))), np.int32),
             "out5": np.zeros(2 * S * G, np.float64) if out5 else None,
             "lval": np.zeros(2 * S, np.float64),
         }

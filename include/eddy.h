@@ -71,7 +71,8 @@ typedef enum {
 } eddy_schedule;
 
 typedef struct {
-  int mp;                /* parent cap, -mp (main.cu:158-163); 1..16 here */
+  int mp;                /* parent cap, -mp (main.cu:158-163): any positive
+                            value, as in the reference (main.cu:170-172) */
   int perms;             /* r, -r (main.cu:165-168) */
   int p_adjust;          /* Bonferroni (1) or -rawp (0) (kernel1.cu:421-424) */
   double alpha_edge;     /* -pE, or from -t/-l (main.cu:180-186) */
@@ -115,7 +116,8 @@ const char *eddy_version(void);
 /* Expression data: [genes x samples] row-major states 0,1,2 (reference values
  * -1,0,1; any other reference value is stored as 3 and never counted, F1 of
  * kernel1.cu:91-118).  class_of: [samples] true labels, 0 = class 1 (first
- * label of the class file), 1 = class 2 (main.cu:303-352).  */
+ * label of the class file), 1 = class 2 (main.cu:303-352).  At most 20000
+ * samples, the reference's MAX_NUM_SAMPLES (definitions.cuh:17). */
 int eddy_upload_dataset(eddy_ctx *ctx, const uint8_t *x, int genes, int samples,
                         const uint8_t *class_of);
 
@@ -168,7 +170,7 @@ typedef struct {
   int *uniq_ids;    /* [S]   */
   int *uni_nodes;   /* [S*G] */
   int *uni_epn;     /* [S+1] */
-  int *uni_edges;   /* [S*G*mp] */
+  int *uni_edges;   /* [S*G*min(mp, G-1)] */
   double *out5;     /* [2*S*G] */
   double *lval;     /* [2*S] */
   int U, edSum;

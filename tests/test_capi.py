@@ -91,7 +91,7 @@ def test_stream_permutations_match_oracle():
     import oracle_ref
 
     L, h = host_ctx()
-    for N, n1 in ((200, 100), (601, 300), (1200, 300)):
+    for N, n1 in ((200, 100), (601, 300), (1200, 300), (4000, 1000), (20000, 5000)):
         L.eddy_stream_reset(h)
         oracle_ref.reset_stream()
         lab_e = np.array([0] * n1 + [1] * (N - n1), np.uint8)
@@ -110,7 +110,7 @@ def test_stream_index_matches_stream():
     a time, across many superblocks and for odd, even, small and maximal n."""
     L = eddy_amd.lib()
     for N, n1, count in ((600, 300, 3000), (601, 17, 200), (2046, 1023, 40), (64, 63, 500), (65, 1, 500),
-                         (2, 1, 2000), (1200, 300, 50)):
+                         (2, 1, 2000), (1200, 300, 50), (4000, 1000, 12), (20000, 7001, 3)):
         ha, hb = C.c_void_p(), C.c_void_p()
         assert L.eddy_ctx_create(-1, C.byref(ha)) == 0 and L.eddy_ctx_create(-1, C.byref(hb)) == 0
         words = (N + 31) // 32

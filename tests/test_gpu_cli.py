@@ -39,11 +39,13 @@ def test_cli_golden(tmp_path, extra):
     assert produced == {f for f in os.listdir(exp_dir) if f.startswith("SET_")}
 
 
-@pytest.mark.parametrize("gmt,extra", [("failed.gmt", []), ("failed.gmt", ["-rawp"]), ("test.gmt", [])])
+@pytest.mark.parametrize("gmt,extra", [("failed.gmt", []), ("failed.gmt", ["-rawp"]), ("test.gmt", []),
+                                       ("test.gmt", ["-mp", "20"])])
 def test_cli_reference_shapes(tmp_path, gmt, extra):
     """The reference's class file (303/272 samples) and GMT files (sets of
-    11..50 genes, some genes absent from the matrix): every output file of the
-    GPU CLI equals the oracle CLI's, byte for byte."""
+    11..50 genes, some genes absent from the matrix), also with a parent cap
+    above 16: every output file of the GPU CLI equals the oracle CLI's, byte
+    for byte."""
     args = ["-d", "expr.txt", "-c", "class.txt", "-g", gmt, "-pE", "0.05", "-pw", "0.5", "-pD", "0.1", "-r",
             "20"] + extra
     runs = {}
