@@ -52,6 +52,7 @@ namespace {
 struct DevBuf {
   void *p = nullptr;
   size_t n = 0;
+  bool arena = false;  // carved from the no-allocation scratch array (never freed)
 };
 
 // Per-set copy of what permutation 0 leaves behind (the reference's first_*
@@ -200,7 +201,50 @@ struct eddy_ctx {
   // launch shapes (eddy_set_launch_cfg / eddy_autotune)
   eddy_launch_cfg lc{1, 0, 0, 4, 0, 2048};
   size_t lds_max = 65536;  // LDS bytes a workgroup may use on the device
+  // eddy_run_perm_detail_noalloc: the device address of the file-scope
+  // scratch array on this context's device, and while such a call runs, the
+  // bump offset into it (every device buffer of the call is carved there) and
+  // host staging that needs no pinned allocation
+  unsigned char *scratch = nullptr;
+  bool arena = false;
+  size_t arena_used = 0;
+  static const int kArenaItems = 4, kArenaSlices = 64;
+  int arena_h_int[4 * kArenaItems];
+  long long arena_h_tests[kArenaItems];
+  Slice arena_h_slices[kArenaSlices];
 };
+
+// One array per GPU, shared by every eddy_run_perm_detail_noalloc call on it
+// (calls that overlap on one GPU race): see include/eddy.h.
+static __device__ __align__(256) unsigned char scratch[EDDY_NOALLOC_SCRATCH_BYTES];
+
+// Every device buffer the context holds (the no-allocation call sets them
+// aside and restores them; the destructor frees them).
+static int list_bufs(eddy_ctx *ctx, DevBuf **out) {
+  DevBuf *all[] = {&ctx->b_sets, &ctx->b_xs, &ctx->b_prior, &ctx->b_items, &ctx->b_labels, &ctx->b_tiles,
+                   &ctx->b_info, &ctx->b_tab, &ctx->b_lfull, &ctx->b_lcnt, &ctx->b_sflags,
+                   &ctx->b_evcount, &ctx->b_hnet, &ctx->b_rep, &ctx->b_clsg, &ctx->b_evorder, &ctx->b_evfam,
+                   &ctx->b_famkey, &ctx->b_famscore, &ctx->b_lval, &ctx->b_uniq, &ctx->b_js, &ctx->b_status,
+                   &ctx->b_tests, &ctx->b_genes, &ctx->b_rowoff, &ctx->b_nan0, &ctx->b_pout, &ctx->b_famcount,
+                   &ctx->b_counters, &ctx->b_famht, &ctx->b_famhv, &ctx->b_mlp, &ctx->b_mlpm, &ctx->b_mlpv,
+                   &ctx->b_mlpg, &ctx->b_mlploss, &ctx->b_feat, &ctx->b_labhits, &ctx->b_score, &ctx->b_win,
+                   &ctx->b_mx, &ctx->b_my, &ctx->b_mxb, &ctx->b_myb, &ctx->b_midx, &ctx->b_mscr, &ctx->b_mlbase,
+                   &ctx->b_mlinv, &ctx->p0.info, &ctx->p0.lcnt, &ctx->p0.lfull, &ctx->p0.rep, &ctx->p0.lval,
+                   &ctx->p0.evkey, &ctx->p0.evpar, &ctx->p0.famkey, &ctx->p0.famscore, &ctx->p0.slices, &ctx->b_bits,
+                   &ctx->b_win2, &ctx->b_order, &ctx->b_js0, &ctx->b_totals};
+  int n = 0;
+  for (DevBuf *b : all) out[n++] = b;
+  for (Slot &sl : ctx->slot) {
+    DevBuf *sb[] = {&sl.items, &sl.labels, &sl.tiles, &sl.info, &sl.tab, &sl.lfull, &sl.lcnt, &sl.sflags, &sl.bits,
+                    &sl.gscr, &sl.evcount, &sl.hnet, &sl.rep, &sl.clsg, &sl.evorder, &sl.evfam, &sl.famkey,
+                    &sl.famscore, &sl.famht, &sl.famhv, &sl.lval, &sl.uniq, &sl.famcount, &sl.status, &sl.tests,
+                    &sl.order, &sl.slices, &sl.bdtask, &sl.bdcount, &sl.sculist, &sl.scevst, &sl.scfamfull,
+                    &sl.fbits, &sl.evkey, &sl.evt, &sl.evpar};
+    for (DevBuf *b : sb) out[n++] = b;
+  }
+  return n;
+}
+static const int kMaxBufs = 160;
 
 #define HIPCHK(ctx, x)                                                                  \
   do {                                                                                  \
@@ -211,10 +255,32 @@ struct eddy_ctx {
     }                                                                                   \
   } while (0)
 
+// a buffer carved from the scratch array (no-allocation calls): 256-byte
+// aligned, bump order, never freed; the same quarter of slack as grow()
+static int carve(eddy_ctx *ctx, DevBuf &b, size_t bytes) {
+  const size_t off = (ctx->arena_used + 255) & ~(size_t)255;
+  bytes += bytes / 4;
+  if (off > EDDY_NOALLOC_SCRATCH_BYTES || bytes > EDDY_NOALLOC_SCRATCH_BYTES - off) {
+    ctx->err = "this set and sample count need more device memory than the no-allocation scratch array holds (" +
+               std::to_string(EDDY_NOALLOC_SCRATCH_BYTES >> 20) + " MiB); use eddy_run_perm_detail";
+    return EDDY_ERR_CAPACITY;
+  }
+  b.p = ctx->scratch + off;
+  b.n = bytes;
+  b.arena = true;
+  ctx->arena_used = off + bytes;
+  return EDDY_OK;
+}
+
 static int grow(eddy_ctx *ctx, DevBuf &b, size_t bytes) {
   if (bytes == 0) bytes = 16;
+  if (ctx->arena) {
+    if (b.arena && b.n >= bytes) return EDDY_OK;
+    return carve(ctx, b, bytes);
+  }
   if (b.n >= bytes) return EDDY_OK;
-  if (b.p) HIPCHK(ctx, hipFree(b.p));
+  if (b.p && !b.arena) HIPCHK(ctx, hipFree(b.p));
+  b.arena = false;
   b.p = nullptr;
   b.n = 0;
   size_t want = bytes + bytes / 4;
@@ -258,11 +324,23 @@ extern "C" int eddy_ctx_create(int device, eddy_ctx **out) {
     delete ctx;
     return EDDY_ERR_HIP;
   }
-  for (Slot &sl : ctx->slot)
+  for (Slot &sl : ctx->slot) {
     if (hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) != hipSuccess) {
       eddy_ctx_destroy(ctx);
       return EDDY_ERR_HIP;
     }
+    for (hipEvent_t &e : sl.tev)
+      if (hipEventCreate(&e) != hipSuccess) {
+        eddy_ctx_destroy(ctx);
+        return EDDY_ERR_HIP;
+      }
+  }
+  // the no-allocation scratch array's address on this device (the name is
+  // only a host-side stand-in; kernels and copies take this pointer)
+  if (hipGetSymbolAddress((void **)&ctx->scratch, HIP_SYMBOL(scratch)) != hipSuccess) {
+    (void)hipGetLastError();
+    ctx->scratch = nullptr;
+  }
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
     ctx->lds_max = (size_t)lds;
@@ -277,27 +355,11 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
     return EDDY_OK;
   }
   hipSetDevice(ctx->device);
-  DevBuf *all[] = {&ctx->b_sets, &ctx->b_xs, &ctx->b_prior, &ctx->b_items, &ctx->b_labels, &ctx->b_tiles,
-                   &ctx->b_info, &ctx->b_tab, &ctx->b_lfull, &ctx->b_lcnt, &ctx->b_sflags,
-                   &ctx->b_evcount, &ctx->b_hnet, &ctx->b_rep, &ctx->b_clsg, &ctx->b_evorder, &ctx->b_evfam,
-                   &ctx->b_famkey, &ctx->b_famscore, &ctx->b_lval, &ctx->b_uniq, &ctx->b_js, &ctx->b_status,
-                   &ctx->b_tests, &ctx->b_genes, &ctx->b_rowoff, &ctx->b_nan0, &ctx->b_pout, &ctx->b_famcount,
-                   &ctx->b_counters, &ctx->b_famht, &ctx->b_famhv, &ctx->b_mlp, &ctx->b_mlpm, &ctx->b_mlpv,
-                   &ctx->b_mlpg, &ctx->b_mlploss, &ctx->b_feat, &ctx->b_labhits, &ctx->b_score, &ctx->b_win,
-                   &ctx->b_mx, &ctx->b_my, &ctx->b_mxb, &ctx->b_myb, &ctx->b_midx, &ctx->b_mscr, &ctx->b_mlbase,
-                   &ctx->b_mlinv, &ctx->p0.info, &ctx->p0.lcnt, &ctx->p0.lfull, &ctx->p0.rep, &ctx->p0.lval,
-                   &ctx->p0.evkey, &ctx->p0.evpar, &ctx->p0.famkey, &ctx->p0.famscore, &ctx->p0.slices, &ctx->b_bits,
-                   &ctx->b_win2, &ctx->b_order, &ctx->b_js0, &ctx->b_totals};
-  for (DevBuf *b : all)
-    if (b->p) hipFree(b->p);
+  DevBuf *bufs[kMaxBufs];
+  const int nb = list_bufs(ctx, bufs);
+  for (int i = 0; i < nb; i++)
+    if (bufs[i]->p && !bufs[i]->arena) hipFree(bufs[i]->p);
   for (Slot &sl : ctx->slot) {
-    DevBuf *sb[] = {&sl.items, &sl.labels, &sl.tiles, &sl.info, &sl.tab, &sl.lfull, &sl.lcnt, &sl.sflags, &sl.bits,
-                    &sl.gscr, &sl.evcount, &sl.hnet, &sl.rep, &sl.clsg, &sl.evorder, &sl.evfam, &sl.famkey,
-                    &sl.famscore, &sl.famht, &sl.famhv, &sl.lval, &sl.uniq, &sl.famcount, &sl.status, &sl.tests,
-                    &sl.order, &sl.slices, &sl.bdtask, &sl.bdcount, &sl.sculist, &sl.scevst, &sl.scfamfull,
-                    &sl.fbits, &sl.evkey, &sl.evt, &sl.evpar};
-    for (DevBuf *b : sb)
-      if (b->p) hipFree(b->p);
     if (sl.h_int) hipHostFree(sl.h_int);
     if (sl.h_tests) hipHostFree(sl.h_tests);
     if (sl.h_slices) hipHostFree(sl.h_slices);
@@ -599,7 +661,15 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   int fbwp = 1;
   while (fbwp < fbw) fbwp *= 2;
   CHK(grow(ctx, sl.fbits, sizeof(uint32_t) * 2 * (size_t)ch.gslots * fbw));
-  if (sl.h_cap < nitems) {
+  if (ctx->arena) {  // pageable staging held by the context (no pinned allocation)
+    if (nitems > eddy_ctx::kArenaItems) {
+      ctx->err = "too many items for a no-allocation call";
+      return EDDY_ERR_CAPACITY;
+    }
+    sl.h_int = ctx->arena_h_int;
+    sl.h_tests = ctx->arena_h_tests;
+    sl.h_cap = eddy_ctx::kArenaItems;
+  } else if (sl.h_cap < nitems) {
     if (sl.h_int) hipHostFree(sl.h_int);
     if (sl.h_tests) hipHostFree(sl.h_tests);
     sl.h_int = nullptr;
@@ -792,7 +862,7 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
     kp.extra = nullptr;
     return kp;
   };
-  if (ctx->lc.graph) {
+  if (ctx->lc.graph && !ctx->arena) {
     SlotGraph &gr = sl.gr[(ch.tiles.empty() ? 0 : 1) | (plain_tests ? 2 : 0) | (loo3 ? 4 : 0) | (rows_gm ? 8 : 0) |
                           (wide ? 16 : 0) | (score_gm ? 32 : 0)];
     if (!gr.x) {
@@ -893,6 +963,15 @@ int wait_chunk(eddy_ctx *ctx, Slot &sl, ChunkOut &out) {
 
 // Grow a device buffer keeping its first `keep` bytes (stream-ordered copy).
 int grow_keep(eddy_ctx *ctx, DevBuf &b, size_t bytes, size_t keep) {
+  if (ctx->arena) {
+    if (b.arena && b.n >= bytes) return EDDY_OK;
+    DevBuf nb;
+    CHK(carve(ctx, nb, bytes));
+    if (b.p && keep) HIPCHK(ctx, hipMemcpyAsync(nb.p, b.p, keep, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    b = nb;
+    return EDDY_OK;
+  }
   if (b.n >= bytes) return EDDY_OK;
   HIPCHK(ctx, hipDeviceSynchronize());  // both slots may be copying into the store
   DevBuf nb;
@@ -902,7 +981,7 @@ int grow_keep(eddy_ctx *ctx, DevBuf &b, size_t bytes, size_t keep) {
   if (b.p && keep) HIPCHK(ctx, hipMemcpyAsync(nb.p, b.p, keep, hipMemcpyDeviceToDevice, ctx->stream));
   if (b.p) {
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    HIPCHK(ctx, hipFree(b.p));
+    if (!b.arena) HIPCHK(ctx, hipFree(b.p));
   }
   b = nb;
   return EDDY_OK;
@@ -993,7 +1072,14 @@ int capture_items(eddy_ctx *ctx, Slot &sl, const std::vector<int> &idx, const Ch
     // the buffer is ordered before it on the same stream, and the host only
     // refills it after waiting for that stream)
     CHK(grow(ctx, sl.slices, sizeof(Slice) * sv.size()));
-    if (sl.h_slices_cap < sv.size()) {
+    if (ctx->arena) {  // pageable staging held by the context
+      if (sv.size() > (size_t)eddy_ctx::kArenaSlices) {
+        ctx->err = "too many capture slices for a no-allocation call";
+        return EDDY_ERR_CAPACITY;
+      }
+      sl.h_slices = ctx->arena_h_slices;
+      sl.h_slices_cap = eddy_ctx::kArenaSlices;
+    } else if (sl.h_slices_cap < sv.size()) {
       if (sl.h_slices) HIPCHK(ctx, hipHostFree(sl.h_slices));
       sl.h_slices = nullptr;
       sl.h_slices_cap = 0;
@@ -2303,9 +2389,10 @@ extern "C" int eddy_autotune(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
 // layouts (out23, the compacted unique CSR, out5 is not kept: BDeu per family
 // is folded into the network sums, so out5 is reconstructed per unique
 // network from the same family scores).
-extern "C" int eddy_run_perm_detail(eddy_ctx *ctx, const eddy_set_desc *set, const uint8_t *labels,
-                                    const eddy_params *p, eddy_perm_detail *out) {
+static int perm_detail_run(eddy_ctx *ctx, const eddy_set_desc *set, const uint8_t *labels, const eddy_params *p,
+                           eddy_perm_detail *out) {
   if (!ctx || !set || !labels || !out) return EDDY_ERR_ARG;
+  if (ctx->device < 0) return EDDY_ERR_NODEV;
   if (!ctx->dX) return EDDY_ERR_STATE;
   CHK(check_params(ctx, p));
   hipSetDevice(ctx->device);
@@ -2415,8 +2502,74 @@ extern "C" int eddy_run_perm_detail(eddy_ctx *ctx, const eddy_set_desc *set, con
   return EDDY_OK;
 }
 
+extern "C" int eddy_run_perm_detail(eddy_ctx *ctx, const eddy_set_desc *set, const uint8_t *labels,
+                                    const eddy_params *p, eddy_perm_detail *out) {
+  return perm_detail_run(ctx, set, labels, p, out);
+}
+
+// The context's device buffers and pinned staging set aside while a
+// no-allocation call carves its own from the scratch array, and restored
+// (with the carved ones forgotten) however the call ends.
+struct ArenaScope {
+  eddy_ctx *ctx;
+  DevBuf *bufs[kMaxBufs];
+  DevBuf saved[kMaxBufs];
+  int n = 0;
+  struct Staging {
+    int *h_int;
+    long long *h_tests;
+    int h_cap;
+    Slice *h_slices;
+    size_t h_slices_cap;
+  } st[EDDY_SLOTS];
+  explicit ArenaScope(eddy_ctx *c) : ctx(c) {
+    n = list_bufs(ctx, bufs);
+    for (int i = 0; i < n; i++) {
+      saved[i] = *bufs[i];
+      *bufs[i] = DevBuf();
+    }
+    for (int k = 0; k < EDDY_SLOTS; k++) {
+      Slot &sl = ctx->slot[k];
+      st[k] = {sl.h_int, sl.h_tests, sl.h_cap, sl.h_slices, sl.h_slices_cap};
+    }
+    ctx->arena = true;
+    ctx->arena_used = 0;
+  }
+  ~ArenaScope() {
+    // nothing may still read or write the scratch array when the next call
+    // carves it again
+    hipDeviceSynchronize();
+    for (int i = 0; i < n; i++) *bufs[i] = saved[i];
+    for (int k = 0; k < EDDY_SLOTS; k++) {
+      Slot &sl = ctx->slot[k];
+      sl.h_int = st[k].h_int;
+      sl.h_tests = st[k].h_tests;
+      sl.h_cap = st[k].h_cap;
+      sl.h_slices = st[k].h_slices;
+      sl.h_slices_cap = st[k].h_slices_cap;
+      sl.busy = false;
+    }
+    ctx->arena = false;
+    ctx->arena_used = 0;
+  }
+};
+
+extern "C" int eddy_run_perm_detail_noalloc(eddy_ctx *ctx, const eddy_set_desc *set, const uint8_t *labels,
+                                            const eddy_params *p, eddy_perm_detail *out) {
+  if (!ctx || !set || !labels || !out) return EDDY_ERR_ARG;
+  if (ctx->device < 0) return EDDY_ERR_NODEV;
+  if (!ctx->scratch) {
+    ctx->err = "the no-allocation scratch array has no device address on this device";
+    return EDDY_ERR_HIP;
+  }
+  if (hipSetDevice(ctx->device) != hipSuccess) return EDDY_ERR_HIP;
+  ArenaScope scope(ctx);
+  return perm_detail_run(ctx, set, labels, p, out);
+}
+
 // Host build of the engine's edge decision (edgetest.h), for the tests that
 // compare it with the oracle on many tables.  c: Bonferroni factor.
+
 extern "C" int eddy_edge_decision_host(const int *tally9, int c, int prior, const eddy_params *p) {
   if (!tally9 || !p) return -1;
   eddy_edge_params ep;

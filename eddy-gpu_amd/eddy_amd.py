@@ -29,7 +29,7 @@ EXPORTS = [
     "eddy_mlp_gradient", "eddy_mlp_train", "eddy_learn_collect", "eddy_run_learned",
     "eddy_edge_loo_mask_host", "eddy_chi2_host", "eddy_stream_class2_batch",
     "eddy_mlp_order_host", "eddy_run_repermute", "eddy_get_launch_cfg", "eddy_set_launch_cfg", "eddy_autotune",
-    "eddy_loo_all_certain_host",
+    "eddy_loo_all_certain_host", "eddy_run_perm_detail_noalloc",
 ]
 MLP_FEATURES = 12
 MLP_NPARAMS = 5057
@@ -133,6 +133,8 @@ def lib():
         L.eddy_get_edgelist.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
         L.eddy_run_perm_detail.argtypes = [C.c_void_p, C.POINTER(SetDesc), C.c_void_p, C.POINTER(Params),
                                            C.POINTER(PermDetail)]
+        L.eddy_run_perm_detail_noalloc.argtypes = [C.c_void_p, C.POINTER(SetDesc), C.c_void_p, C.POINTER(Params),
+                                                   C.POINTER(PermDetail)]
         L.eddy_get_timing.argtypes = [C.c_void_p, C.POINTER(Timing)]
         L.eddy_get_launch_cfg.argtypes = [C.c_void_p, C.POINTER(LaunchCfg)]
         L.eddy_set_launch_cfg.argtypes = [C.c_void_p, C.POINTER(LaunchCfg)]
@@ -393,7 +395,7 @@ class Engine:
         self._chk(self.L.eddy_get_edgelist(self.h, s, _ptr(nodes), _ptr(epn), _ptr(edges), len(edges)), "edgelist")
         return nodes[:2 * G], epn, edges[:epn[2]]
 
-    def perm_detail(self, genes, pairs, labels, params, bits=True, out5=True):
+    def perm_detail(self, genes, pairs, labels, params, bits=True, out5=True, noalloc=False):
         G = len(genes)
         S = self.N + 2
         c = G * (G - 1) // 2
@@ -411,7 +413,8 @@ class Engine:
             setattr(d, k, _ptr(v) if v is not None else None)
         arr, keep = self._descs([(genes, pairs)])
         lab = np.ascontiguousarray(labels, np.uint8)
-        self._chk(self.L.eddy_run_perm_detail(self.h, arr, _ptr(lab), C.byref(params), C.byref(d)), "perm_detail")
+        fn = self.L.eddy_run_perm_detail_noalloc if noalloc else self.L.eddy_run_perm_detail
+        self._chk(fn(self.h, arr, _ptr(lab), C.byref(params), C.byref(d)), "perm_detail")
         U = d.U
         return {
             "U": U, "js": d.js, "edSum": d.edSum,

@@ -181,6 +181,26 @@ int eddy_run_perm_detail(eddy_ctx *ctx, const eddy_set_desc *set,
                          const uint8_t *labels, const eddy_params *params,
                          eddy_perm_detail *out);
 
+/* eddy_run_perm_detail without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * device buffer of the call (the set's gathered rows and pooled tables, the
+ * item's tallies, networks, events, families and captures) is carved from one
+ * file-scope device array of EDDY_NOALLOC_SCRATCH_BYTES per GPU, which each
+ * GPU the library runs on holds until the process ends; host staging is
+ * pageable memory inside the context, and the kernels are launched one by
+ * one (no graph objects).  The context's own buffers are untouched.  A set
+ * and sample count whose buffers exceed the array is rejected with
+ * EDDY_ERR_CAPACITY (a 300-gene set at 600 samples takes about 6 MiB,
+ * 1024 genes at 20000 samples about 50 MiB).  Every call on one GPU shares
+ * that GPU's array and finds the last call's contents, which it never relies
+ * on; calls that overlap on one GPU (two contexts, two host threads) race.
+ * Needs the dataset uploaded before (eddy_upload_dataset allocates). */
+#define EDDY_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int eddy_run_perm_detail_noalloc(eddy_ctx *ctx, const eddy_set_desc *set,
+                                 const uint8_t *labels, const eddy_params *params,
+                                 eddy_perm_detail *out);
+
 /* ---- timing of the last run (HIP events on the engine stream) ----------
  * kernel index: 0 count_test (tables on MFMA + tests), 1 loo (cutoff
  * schedule), 2 rows (networks + hashes), 3 score (dedup, BDeu, JS),
