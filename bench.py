@@ -10,11 +10,14 @@ same work.  Prints one JSON line:
 
   value          gene sets/s (whole job, r = 100), batched + learned
   plain          the plain GPU form (one set and one permutation per launch
-                 sequence, every leave-one-out test) on a prefix of the sets
+                 sequence, every leave-one-out test) measured on every set
   ci_tests       reference-equivalent and evaluated chi-square tests/s
-  roofline       the dominant kernel against MI355X's published peak
-  cpu_baseline   the CPU oracle (reference algorithm) on a bounded sample,
-                 extrapolated to the same workload
+  roofline       the kernel with the most device time against MI355X's
+                 published peak; rooflines: the count kernel (int8 MFMA),
+                 the leave-one-out kernel (HBM) and BDeu (LDS)
+  cpu_baseline   the CPU oracle (reference algorithm) on one item of each
+                 set-size decile, on all of the GPU's host cores and on one
+                 thread, extrapolated decile by decile to the same workload
 """
 import argparse
 import json
@@ -30,6 +33,8 @@ sys.path.insert(0, PKG)
 PEAK_INT8_OPS = 5.0e15      # dense MFMA int8, OP/s
 PEAK_FP64 = 78.6e12         # FP64, FLOP/s
 PEAK_HBM = 8.0e12           # B/s
+PEAK_LDS = 256 * 128 * 2.4e9  # B/s: 256 CUs x 128 B/clk at 2.4 GHz
+HOST_CORE_SHARE = 16        # host cores one GPU's jobs may use on the GPU box
 
 
 def ensure_built():
@@ -52,10 +57,28 @@ def algorithmic_work(sets, N, nA, perms_run):
     return ops, ref_tests
 
 
-def cpu_baseline(wl, perms_run, seconds, threads):
-    """Oracle (reference algorithm, full LOO tally per network and pair) on the
-    first (set, permutation) items until `seconds` of CPU work, extrapolated by
-    the reference's cost S*c*n per item to the whole step."""
+def host_description():
+    """nproc and the CPU model of this host (lscpu), for the baseline's record."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_baseline(wl, perms_run, seconds_1t, threads):
+    """The oracle (reference algorithm: a full leave-one-out tally per network
+    and pair, AS 147 for every test, brute-force BDeu) on one permuted item of
+    each set-size decile of the workload, on `threads` threads and on one
+    thread.  Each decile's measured seconds per unit of the reference's cost
+    S*c*n extrapolate that decile's sets at their permutations; the one-thread
+    run stops after `seconds_1t` and takes the remaining deciles from the
+    threaded run scaled by the one-thread/threaded ratio measured before."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -63,31 +86,56 @@ def cpu_baseline(wl, perms_run, seconds, threads):
 
     data_ref = wl.X.astype(np.int32) - 1
     N = wl.N
+    cost = lambda G: (N + 2) * (G * (G - 1) // 2) * N
+    sizes = np.array([len(g) for g, _ in wl.sets])
+    order = np.argsort(sizes, kind="stable")
+    deciles = np.array_split(order, 10)
+    pick = [int(d[len(d) // 2]) for d in deciles]
     rng = np.random.default_rng(0)
-    done_cost, t0, items = 0.0, time.perf_counter(), 0
-    for s, (genes, _) in enumerate(wl.sets):
-        if len(genes) < 2:
-            continue
-        prior = wl.prior_matrix(s)
-        lab = wl.classes if items == 0 else rng.permutation(wl.classes)
-        t1, t2, n1, n2 = oracle_ref.class_split(data_ref, genes, lab)
-        prm = oracle_ref.params(mp=wl.cfg.mp, threads=threads)
-        oracle_ref.run_perm(t1, t2, n1, n2, prior, prm, want_bits=False)
-        G = len(genes)
-        done_cost += (N + 2) * (G * (G - 1) // 2) * N
-        items += 1
-        if time.perf_counter() - t0 > seconds and items >= 2:
-            break
-    dt = time.perf_counter() - t0
-    total_cost = sum(r * (N + 2) * (len(g) * (len(g) - 1) // 2) * N for (g, _), r in zip(wl.sets, perms_run))
-    t_total = dt * total_cost / done_cost
+    labs = [rng.permutation(wl.classes) for _ in pick]
+
+    def run(th, budget=None):
+        secs = []
+        t_start = time.perf_counter()
+        for s, lab in zip(pick, labs):
+            if budget is not None and time.perf_counter() - t_start > budget:
+                break
+            genes, _ = wl.sets[s]
+            t1, t2, n1, n2 = oracle_ref.class_split(data_ref, genes, lab)
+            t0 = time.perf_counter()
+            oracle_ref.run_perm(t1, t2, n1, n2, wl.prior_matrix(s), oracle_ref.params(mp=wl.cfg.mp, threads=th),
+                                want_bits=False)
+            secs.append(time.perf_counter() - t0)
+        return secs
+
+    def extrapolate(secs):
+        total = 0.0
+        for d, s, t in zip(deciles, pick, secs):
+            rate = t / max(1, cost(sizes[s]))
+            total += rate * sum(perms_run[q] * cost(sizes[q]) for q in d)
+        return total
+
+    sec_n = run(threads)
+    sec_1 = run(1, budget=seconds_1t)
+    measured_1 = len(sec_1)
+    if measured_1 < len(pick):
+        ratio = sum(sec_1) / max(1e-9, sum(sec_n[:measured_1]))
+        sec_1 = sec_1 + [t * ratio for t in sec_n[measured_1:]]
+    t_n, t_1 = extrapolate(sec_n), extrapolate(sec_1)
+    host = host_description()
+    sample = ("one permuted item of each set-size decile (G %s) of the same workload: %.1f s on %d threads, "
+              "%.1f s on one thread%s; each decile's seconds per S*c*n extrapolate its %d sets at r=%d"
+              % ("/".join(str(int(sizes[s])) for s in pick), sum(sec_n), threads, sum(sec_1[:measured_1]),
+                 "" if measured_1 == len(pick) else
+                 " (deciles %d..9 from the threaded times x the one-thread/threaded ratio %.1f of the others)"
+                 % (measured_1, sum(sec_1[:measured_1]) / max(1e-9, sum(sec_n[:measured_1]))),
+                 len(wl.sets) // 10, wl.cfg.perms))
     return {
-        "value": len(wl.sets) / t_total,
-        "unit": "gene sets/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": "%d (set, permutation) items of the same workload in %.1f s with %d OpenMP threads, "
-                  "extrapolated by S*c*n to %d sets at r=%d" % (items, dt, threads, len(wl.sets), wl.cfg.perms),
+        "value": len(wl.sets) / t_n, "unit": "gene sets/s", "cores": threads, "kind": "port",
+        "sample": sample, "host": host,
+        "single_thread": {"value": len(wl.sets) / t_1, "unit": "gene sets/s", "cores": 1,
+                          "deciles_measured": measured_1},
+        "decile_item_seconds": {"threads": [round(x, 3) for x in sec_n], "one_thread": [round(x, 3) for x in sec_1]},
     }
 
 
@@ -153,8 +201,10 @@ def main():
     ap.add_argument("--train-rows", type=int, default=65536)
     ap.add_argument("--margin", type=float, default=3.0)
     ap.add_argument("--label-perms", type=int, default=10)
-    ap.add_argument("--plain-sets", type=int, default=10)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--plain-sets", type=int, default=None,
+                    help="sets the plain GPU form runs (default: all of them); 0 skips it")
+    ap.add_argument("--cpu-seconds", type=float, default=40.0,
+                    help="time limit of the one-thread CPU baseline run (the deciles past it are scaled)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--compare-rounds", type=int, default=5,
@@ -212,8 +262,8 @@ def main():
         tw = time.perf_counter()
         run_step(eng, wl, prm, cfg)
         print("bench: warmup %d %.2fs" % (w, time.perf_counter() - tw), file=sys.stderr, flush=True)
-    kern_ms = [0.0] * 6
-    launches = [0] * 6
+    split_ms = [0.0] * 6
+    split_n = [0] * 6
     tests_eval = 0
     items = 0
     host_stream_ms = 0.0
@@ -226,8 +276,8 @@ def main():
         res, rep, tm = run_step(eng, wl, prm, cfg)
         reps.append(rep)
         for k in range(6):
-            kern_ms[k] += tm.kernel_ms[k]
-            launches[k] += tm.launches[k]
+            split_ms[k] += tm.split_ms[k]
+            split_n[k] += tm.split_launches[k]
         tests_eval += tm.tests
         items += tm.items
         host_stream_ms += tm.host_stream_ms
@@ -304,11 +354,11 @@ def main():
         learned["vs_unlearned"] = compare
 
     # plain GPU form (one set and one permutation per launch sequence, every
-    # test) on a prefix of the sets; its time is extrapolated to the whole
-    # workload by the reference's cost S*c*r per set
+    # test), measured on the same sets (all of them by default)
     plain = None
-    if args.plain_sets > 0:
-        sub = wl.sets[:args.plain_sets]
+    nplain = nsets if args.plain_sets is None else min(args.plain_sets, nsets)
+    if nplain > 0:
+        sub = wl.sets[:nplain]
         pprm = wl.params(mode=eddy_amd.MODE_PLAIN, schedule=eddy_amd.TESTS_ALL, learned=0)
         eng.reset_stream()
         tp = time.perf_counter()
@@ -317,56 +367,73 @@ def main():
         same = all((r.status, r.perms_run, r.U0) == (q.status, q.perms_run, q.U0) for r, q in zip(pres, res))
         for s in range(len(sub)):
             same = same and bool((eng.js(s).view("u8") == js_head[s].view("u8")).all())
-        cost = [(wl.N + 2) * len(g) * (len(g) - 1) // 2 * r for (g, _), r in zip(wl.sets, perms_run)]
-        frac = sum(cost[:len(sub)]) / max(1, sum(cost))
-        plain_rate = nsets / (dtp / max(frac, 1e-12))
-        plain = {"sets": len(sub), "seconds": dtp, "cost_fraction_of_workload": frac,
-                 "plain_sets_per_s_extrapolated": plain_rate, "speedup_batched_learned_vs_plain": value / plain_rate,
-                 "results_bit_identical": same}
+        plain = {"sets": len(sub), "seconds": dtp, "plain_sets_per_s": len(sub) / dtp,
+                 "measured": "every set" if nplain == nsets else "the first %d sets" % nplain,
+                 "speedup_batched_learned_vs_plain": value / (len(sub) / dtp), "results_bit_identical": same}
+        print("bench: plain form %d sets in %.1fs" % (len(sub), dtp), file=sys.stderr, flush=True)
 
-    # roofline of the dominant kernel
-    dom = max(range(6), key=lambda k: kern_ms[k])
-    dom_avg_s = kern_ms[dom] / max(1, launches[dom]) / 1000.0
-    per_launch = lambda total: total / max(1, launches[dom])
-    if dom == 0:
-        achieved = per_launch(ops * args.steps) / dom_avg_s
-        roof = {"kernel": "k_count_test", "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_INT8_OPS / 1e12,
-                "unit": "TOP/s", "frac": achieved / PEAK_INT8_OPS, "traffic": None}
-    elif dom == 3:
-        # BDeu histograms: (1 + |pa|) state bytes per sample and class, plus
-        # 4 q r bytes of bins, per distinct family and scoring class
-        achieved = per_launch(stats[0] + stats[4]) / dom_avg_s
-        # the score interval spans k_score, k_bdeu and k_score_tail
-        roof = {"kernel": "k_score+k_bdeu+k_score_tail", "bound": "hbm", "achieved": achieved / 1e9,
-                "peak": PEAK_HBM / 1e9,
-                "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None}
-    elif dom == 1:
-        # leave-one-out tests: each planned (pair, class) entry reads its 9 u16 cells and 4 B of info
-        achieved = per_launch(sched[2] * 22) / dom_avg_s
-        roof = {"kernel": "k_loo", "bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9,
-                "unit": "GB/s", "frac": achieved / PEAK_HBM, "traffic": None}
+    # Rooflines.  Algorithmic work per launch over the kernel's average launch
+    # time (HIP events on the slot's stream, summed over the timed steps):
+    #   k_count_test  2*9*c*n_A int8 ops per item (class A's nine one-hot
+    #                 products per pair; class B's table is the pooled table
+    #                 minus class A's) against dense int8 MFMA
+    #   k_loo         22 B per planned (pair, class) entry (its 9 u16 cells
+    #                 and 4 B of info) against HBM
+    #   BDeu          (1 + |pa|) state bytes per sample and class plus 4 q r
+    #                 bytes of bins, per distinct family and scoring class,
+    #                 over k_score + k_bdeu (which score them) against LDS
+    per_step = lambda x: x / args.steps
+    avg_s = [split_ms[k] / max(1, split_n[k]) / 1000.0 for k in range(6)]
+    lps = [split_n[k] / args.steps for k in range(6)]
+    roofs = {}
+    if split_n[0]:
+        a_cnt = per_step(ops * args.steps) / lps[0] / avg_s[0]
+        roofs["k_count_test"] = {"bound": "mfma", "achieved": a_cnt / 1e12, "peak": PEAK_INT8_OPS / 1e12,
+                                 "unit": "TOP/s", "frac": a_cnt / PEAK_INT8_OPS,
+                                 "work_per_launch": per_step(ops * args.steps) / lps[0],
+                                 "avg_launch_ms": avg_s[0] * 1e3, "launches_per_step": lps[0]}
+    if split_n[1]:
+        a_loo = per_step(sched[2] * 22) / lps[1] / avg_s[1]
+        roofs["k_loo"] = {"bound": "hbm", "achieved": a_loo / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                          "frac": a_loo / PEAK_HBM, "work_per_launch": per_step(sched[2] * 22) / lps[1],
+                          "avg_launch_ms": avg_s[1] * 1e3, "launches_per_step": lps[1]}
+    if split_n[3] and split_n[4]:
+        t_bd = (split_ms[3] + split_ms[4]) / split_n[3] / 1000.0
+        a_bd = per_step(stats[0] + stats[4]) / lps[3] / t_bd
+        roofs["bdeu"] = {"kernels": "k_score+k_bdeu", "bound": "lds", "achieved": a_bd / 1e9,
+                         "peak": PEAK_LDS / 1e9, "unit": "GB/s", "frac": a_bd / PEAK_LDS,
+                         "work_per_launch": per_step(stats[0] + stats[4]) / lps[3],
+                         "avg_launch_ms": t_bd * 1e3, "launches_per_step": lps[3]}
+    dom = max(range(6), key=lambda k: split_ms[k])
+    dname = eddy_amd.SPLIT[dom]
+    if dname in roofs:
+        roof = {"kernel": dname, **{k: roofs[dname][k] for k in ("bound", "achieved", "peak", "unit", "frac")}}
+    elif dom in (3, 4):
+        roof = {"kernel": dname, **{k: roofs["bdeu"][k] for k in ("bound", "achieved", "peak", "unit", "frac")},
+                "note": "BDeu bytes over k_score + k_bdeu"}
     else:
-        roof = {"kernel": eddy_amd.KERNELS[dom], "bound": "hbm", "achieved": None, "peak": PEAK_HBM / 1e9,
-                "unit": "GB/s", "frac": None, "traffic": None}
-    roof["avg_launch_ms"] = dom_avg_s * 1000.0
-    roof["launches_per_step"] = launches[dom] / args.steps
+        roof = {"kernel": dname, "bound": "hbm", "achieved": None, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": None}
+    roof["avg_launch_ms"] = avg_s[dom] * 1e3
+    roof["launches_per_step"] = lps[dom]
+    roof["traffic"] = None
     # HBM bytes of the same kernel from the committed byte-counter profile of
     # this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated): its bytes per
-    # step over this run's launches per step (chunk boundaries follow the
-    # stream replay's timing, the work per step does not)
+    # step over this run's launches per step
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_learned_job_5000x600.json")
     if args.config == 3 and args.nsets is None and os.path.exists(tpath):
         kern = json.load(open(tpath))["kernels"]
-        parts = [kern.get(k) for k in roof["kernel"].split("+")]
-        if all(parts):
-            roof["traffic"] = sum(q["bytes_per_step"] for q in parts) / max(1e-9, roof["launches_per_step"])
+        if dname in kern:
+            roof["traffic"] = kern[dname]["bytes_per_step"] / max(1e-9, lps[dom])
             roof["traffic_source"] = os.path.relpath(tpath, ROOT)
 
     cpu = None
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        threads = max(1, min(threads, 16))
+        # every host core this GPU's jobs may use (the box's share), and one
+        threads = max(1, min(HOST_CORE_SHARE, os.cpu_count() or 1))
+        tc = time.perf_counter()
         cpu = cpu_baseline(wl, perms_run, args.cpu_seconds, threads)
+        print("bench: cpu baseline %.1fs" % (time.perf_counter() - tc), file=sys.stderr, flush=True)
 
     line = {
         "metric": "gene sets/s at r=100 perms (batched+learned); CI tests/s",
@@ -389,7 +456,10 @@ def main():
         "ci_tests_per_s_ref_equiv": ref_tests * args.steps / elapsed,
         "ci_tests_per_s_evaluated": tests_eval / elapsed,
         "items_per_step": items // args.steps,
-        "kernel_ms_per_step": {eddy_amd.KERNELS[k]: kern_ms[k] / args.steps for k in range(6)},
+        "kernel_ms_per_step": {eddy_amd.SPLIT[k]: split_ms[k] / args.steps for k in range(6)},
+        "kernel_ms_per_step_note": "HIP events between the kernels of each chunk's sequence on its slot's stream; "
+                                   "two chunks are in flight, so intervals overlap; k_count_test includes the "
+                                   "class-1 gather launched just before it",
         "host_stream_ms_per_step": host_stream_ms / args.steps,
         "schedule": {"rows_past_plan": sched[0] // args.steps, "cols_tested_past_plan": sched[1] // args.steps,
                      "planned_entries": sched[2] // args.steps, "rows": sched[3] // args.steps},
@@ -403,6 +473,7 @@ def main():
         "learned": learned,
         "plain": plain,
         "roofline": roof,
+        "rooflines": roofs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -28,6 +28,8 @@
 __global__ void k_count_test(KArgs a);
 __global__ void k_count_test_all(KArgs a);
 __global__ void k_count_pooled(KArgs a);
+__global__ void k_gather_class1(KArgs a);
+__global__ void k_features(KArgs a, int nsets);
 __global__ void k_loo(KArgs a);
 __global__ void k_loo_w3(KArgs a);
 __global__ void k_clear_slot(int *evcount, int *status, long long *tests, unsigned *bdcount, int n);
@@ -111,7 +113,7 @@ struct SlotGraph {
 struct Slot {
   DevBuf items, labels, tiles, info, tab, lfull, lcnt, sflags, bits, evcount, hnet, rep, clsg, evorder, evfam,
       famkey, famscore, famht, famhv, lval, uniq, famcount, status, tests, order, slices, bdtask, bdcount, sculist,
-      scevst, scfamfull, fbits, evkey, evt, evpar, gscr;
+      scevst, scfamfull, fbits, evkey, evt, evpar, gscr, xc;
   hipStream_t st = nullptr;
   // timing events: 0 start, 1 after count, 2 after loo, 3 after rows,
   // 5 after score, 6 after BDeu, 4 after the tail
@@ -194,7 +196,6 @@ struct eddy_ctx {
   bool mlp_ready = false;
   int mlp_step = 0;
   float mlp_kappa = 1.0f;
-  float *k_feat = nullptr;
   unsigned *k_labhits = nullptr;
   const int *k_win = nullptr;
   int k_label_perms = 0;
@@ -239,7 +240,7 @@ static int list_bufs(eddy_ctx *ctx, DevBuf **out) {
                     &sl.gscr, &sl.evcount, &sl.hnet, &sl.rep, &sl.clsg, &sl.evorder, &sl.evfam, &sl.famkey,
                     &sl.famscore, &sl.famht, &sl.famhv, &sl.lval, &sl.uniq, &sl.famcount, &sl.status, &sl.tests,
                     &sl.order, &sl.slices, &sl.bdtask, &sl.bdcount, &sl.sculist, &sl.scevst, &sl.scfamfull,
-                    &sl.fbits, &sl.evkey, &sl.evt, &sl.evpar};
+                    &sl.fbits, &sl.evkey, &sl.evt, &sl.evpar, &sl.xc};
     for (DevBuf *b : sb) out[n++] = b;
   }
   return n;
@@ -625,7 +626,11 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   CHK(grow(ctx, sl.labels, sizeof(uint32_t) * ch.labels.size()));
   CHK(grow(ctx, sl.tiles, sizeof(uint2) * std::max<size_t>(1, ch.tiles.size())));
   CHK(grow(ctx, sl.info, sizeof(uint32_t) * ch.pairs));
+  // class 1's samples of every item, compacted (cutoff schedule's count kernel)
+  const int kp = ((ctx->n1 + 63) / 64) * 64;
+  const bool gather1 = !plain_tests && !ch.tiles.empty();
   if (!plain_tests) CHK(grow(ctx, sl.tab, sizeof(uint16_t) * EDDY_TAB_STRIDE * ch.pairs));
+  if (gather1) CHK(grow(ctx, sl.xc, (size_t)kp * ch.gslots));
   CHK(grow(ctx, sl.lfull, sizeof(uint16_t) * (size_t)mp * 2 * ch.gslots));
   CHK(grow(ctx, sl.lcnt, sizeof(uint16_t) * 2 * ch.gslots));
   CHK(grow(ctx, sl.sflags, 2 * ch.gslots));
@@ -704,10 +709,9 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   a.mpw = mp;
   a.p_adjust = p->p_adjust;
   a.plain = plain_tests ? 1 : 0;
-  a.feat_only = 0;
   a.ht_size = ht;
   a.plan_extra = nullptr;
-  a.feat = ctx->k_feat;
+  a.feat = nullptr;
   a.labhits = ctx->k_labhits;
   a.win = ctx->k_win;
   a.label_perms = ctx->k_label_perms;
@@ -722,6 +726,8 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   a.hseed = 0x243F6A8885A308D3ULL + (uint64_t)attempt * 0x9E3779B97F4A7C15ULL;
   a.sets = (const SetMeta *)ctx->b_sets.p;
   a.xs = (const uint8_t *)ctx->b_xs.p;
+  a.xc = gather1 ? (uint8_t *)sl.xc.p : nullptr;
+  a.kp = kp;
   a.prior = (const uint32_t *)ctx->b_prior.p;
   a.items = (const ItemMeta *)sl.items.p;
   a.order = (const int *)sl.order.p;
@@ -832,6 +838,8 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   int ns = 0;
   seq[ns++] = {-1, f_clear, (unsigned)((nitems + 255) / 256), 0};
   seq[ns++] = {0, nullptr, 0, 0};
+  if (gather1)
+    seq[ns++] = {-1, (const void *)k_gather_class1, (unsigned)nitems, (size_t)4 * ((N + 63) / 64 + 1) + 2 * (size_t)kp};
   if (!ch.tiles.empty())
     seq[ns++] = {-1, plain_tests ? (const void *)k_count_test_all : (const void *)k_count_test,
                  (unsigned)ch.tiles.size(), 0};
@@ -941,6 +949,11 @@ int wait_chunk(eddy_ctx *ctx, Slot &sl, ChunkOut &out) {
   sl.kms[3] = span(3, 5);
   sl.kms[4] = span(5, 6);
   sl.kms[5] = span(6, 4);
+  for (int k = 0; k < 6; k++) {
+    if ((k == 1 && sl.plain_tests) || (k == 0 && !sl.has_tiles)) continue;
+    ctx->timing.split_ms[k] += sl.kms[k];
+    ctx->timing.split_launches[k] += 1;
+  }
   for (int k = 0; k < 4; k++) {
     if (k == 1 && sl.plain_tests) continue;
     if (k == 0 && !sl.has_tiles) continue;
@@ -1478,48 +1491,18 @@ struct Producer {
 
 // Pooled-table features of every pair of every set (one permutation-0 item
 // per set through the count kernel, no tests) into b_feat at pairbase.
-int compute_features(eddy_ctx *ctx, const std::vector<SetMeta> &meta, const std::vector<uint32_t> &truebits) {
-  const int N = ctx->N, nsets = (int)meta.size();
-  const long long budget = 64LL << 20;
-  Chunk ch;
-  auto launch = [&]() -> int {
-    if (ch.items.empty()) return EDDY_OK;
-    GROW(b_items, sizeof(ItemMeta) * ch.items.size());
-    GROW(b_labels, sizeof(uint32_t) * ch.labels.size());
-    GROW(b_tiles, sizeof(uint2) * ch.tiles.size());
-    hipStream_t st = ctx->stream;
-    HIPCHK(ctx, hipMemcpyAsync(ctx->b_items.p, ch.items.data(), sizeof(ItemMeta) * ch.items.size(),
-                               hipMemcpyHostToDevice, st));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->b_labels.p, ch.labels.data(), sizeof(uint32_t) * ch.labels.size(),
-                               hipMemcpyHostToDevice, st));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->b_tiles.p, ch.tiles.data(), sizeof(uint2) * ch.tiles.size(),
-                               hipMemcpyHostToDevice, st));
-    KArgs a{};
-    a.N = N;
-    a.n1 = ctx->n1;
-    a.n2 = ctx->n2;
-    a.S = N + 2;
-    a.feat_only = 1;
-    a.sets = (const SetMeta *)ctx->b_sets.p;
-    a.xs = (const uint8_t *)ctx->b_xs.p;
-    a.prior = (const uint32_t *)ctx->b_prior.p;
-    a.items = (const ItemMeta *)ctx->b_items.p;
-    a.labels = (const uint32_t *)ctx->b_labels.p;
-    a.tiles = (const uint32_t *)ctx->b_tiles.p;
-    a.feat = (float *)ctx->b_feat.p;
-    a.totals = (uint16_t *)ctx->b_totals.p;
-    k_count_test<<<(unsigned)ch.tiles.size(), 256, 0, st>>>(a);
-    HIPCHK(ctx, hipGetLastError());
-    HIPCHK(ctx, hipStreamSynchronize(st));
-    ch = Chunk();
-    return EDDY_OK;
-  };
-  for (int s = 0; s < nsets; s++) {
-    if (meta[s].G < 2) continue;
-    add_item(ch, meta, s, 0, truebits, N);
-    if (ch.pairs > budget) CHK(launch());
-  }
-  return launch();
+int compute_features(eddy_ctx *ctx, const std::vector<SetMeta> &meta) {
+  const int nsets = (int)meta.size();
+  if (nsets == 0) return EDDY_OK;
+  KArgs a{};
+  a.sets = (const SetMeta *)ctx->b_sets.p;
+  a.prior = (const uint32_t *)ctx->b_prior.p;
+  a.totals = (uint16_t *)ctx->b_totals.p;
+  a.feat = (float *)ctx->b_feat.p;
+  k_features<<<nsets, 256, 0, ctx->stream>>>(a, nsets);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return EDDY_OK;
 }
 
 // Window of every row of sets [from, nsets): network scores of the pairs,
@@ -1583,7 +1566,6 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
   const bool batched = p->mode == EDDY_MODE_BATCHED;
   long long sumc = 0;
   for (const SetMeta &m : meta) sumc += m.c;
-  ctx->k_feat = nullptr;
   ctx->k_labhits = nullptr;
   ctx->k_win = nullptr;
   eddy_learn_report r{};
@@ -1610,7 +1592,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
   if (p->schedule != EDDY_TESTS_CUTOFF) win_from = nsets;
   if (collect_upto > 0 || (win_from < nsets && !lazy)) {
     GROW(b_feat, sizeof(float) * MLP_IN * std::max(1LL, sumc));
-    CHK(compute_features(ctx, meta, truebits));
+    CHK(compute_features(ctx, meta));
   }
   if (collect_upto > 0) {
     GROW(b_labhits, sizeof(unsigned) * std::max(1LL, sumc));
@@ -2298,11 +2280,9 @@ extern "C" int eddy_autotune(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
   // the calibration leaves the caller's run state as it was
   const eddy_timing keep_timing = ctx->timing;
   const HostProf keep_hp = ctx->hp;
-  float *keep_feat = ctx->k_feat;
   unsigned *keep_lab = ctx->k_labhits;
   const int *keep_win = ctx->k_win;
   const int keep_lp = ctx->k_label_perms;
-  ctx->k_feat = nullptr;
   ctx->k_labhits = nullptr;
   ctx->k_win = nullptr;
   ctx->k_label_perms = 0;
@@ -2373,7 +2353,6 @@ extern "C" int eddy_autotune(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
   for (int i = 0; i < 6; i++) rep.kernel_ms[i] = base.k[i];
   ctx->timing = keep_timing;
   ctx->hp = keep_hp;
-  ctx->k_feat = keep_feat;
   ctx->k_labhits = keep_lab;
   ctx->k_win = keep_win;
   ctx->k_label_perms = keep_lp;

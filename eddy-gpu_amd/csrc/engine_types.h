@@ -64,16 +64,17 @@ struct Slice {
   long long bytes;
 };
 
-// per-pair tallies in the cutoff schedule: two classes of 12 u16 (9 counts + padding)
+// per-pair tallies in the cutoff schedule: class 1's 3x3 tally as 12 u16 (9
+// counts + padding); class 2's is the pair's pooled table (same layout, once
+// per set) minus class 1's
 #define EDDY_TAB_CLASS 12
-#define EDDY_TAB_STRIDE 24
+#define EDDY_TAB_STRIDE 12
 
 struct KArgs {
   // dataset-level
   int N, n1, n2, S;
   int mp, p_adjust, plain;  // mp: the parent cap, at most the largest set's G - 1 (no row has more parents)
   int mpw;                 // u16 per row in lfull: mp
-  int feat_only;           // count kernel: pooled-table features only (no tests, no outputs)
   int ht_size;             // network hash table entries (power of two >= 1.5 S)
   int score_shared;        // bytes of the score kernel's phase-shared LDS region
   int score_groups;        // lane groups per wave in the score kernel's BDeu of 2-3 parent families
@@ -84,6 +85,8 @@ struct KArgs {
   // sets
   const SetMeta *sets;
   const uint8_t *xs;       // gathered set rows
+  uint8_t *xc;             // [(goff + g) * kp + k]: each item's rows at its class-1 samples (cutoff schedule)
+  int kp;                  // class-1 samples rounded up to 64
   const uint32_t *prior;   // prior bits
   // items
   const ItemMeta *items;
@@ -93,7 +96,7 @@ struct KArgs {
   const uint32_t *tiles;   // K1 work list: item<<16 | bi<<8 | bj
   // per-pair outputs
   uint32_t *info;
-  uint16_t *tab;           // [pair][2][EDDY_TAB_CLASS] counts (cutoff mode)
+  uint16_t *tab;           // [pair][EDDY_TAB_CLASS] class 1's counts (cutoff mode)
   uint16_t *totals;        // [set pairbase + p][EDDY_TAB_CLASS]: pooled table of each pair over all samples
   // per-item scratch
   uint16_t *lfull;         // [goff*2 + g*G + t][mpw]
@@ -120,7 +123,7 @@ struct KArgs {
   int perms;
   int *status;             // [item] error bits
   const uint8_t *plan_extra;  // [goff*2 + g*G + t] extra full edges in the plan (learned), or null
-  float *feat;             // [(pairbase + p) * 12] learned-proposal features of permutation 0, or null
+  float *feat;             // [(pairbase + p) * 12] learned-proposal features (k_features)
   unsigned *labhits;       // [pairbase + p] full-test edges over permutations > 0 (training labels), or null
   int label_perms;         // > 0: permutations 1..label_perms run every full test (and log it) despite win
   const int *win;          // [genebase + t] last column whose full test the count kernel runs, or null

@@ -55,20 +55,42 @@ __device__ __forceinline__ void store_tally(uint16_t *dst, const int *t) {
          ((uint64_t)(uint16_t)t[7] << 48);
   d[2] = (uint64_t)(uint16_t)t[8];
 }
-__device__ __forceinline__ void load_tally(const uint16_t *src, int *t) {
-  const uint64_t *s = (const uint64_t *)src;
-  const uint64_t a0 = s[0], a1 = s[1], a2 = s[2];
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    t[i] = (int)((a0 >> (16 * i)) & 0xffff);
-    t[4 + i] = (int)((a1 >> (16 * i)) & 0xffff);
-  }
-  t[8] = (int)(a2 & 0xffff);
+// class 1's tally of pair p of the item (cutoff schedule)
+__device__ __forceinline__ uint16_t *tally_at(const KArgs &a, const ItemMeta &im, long long p) {
+  return a.tab + (size_t)(im.infooff + p) * EDDY_TAB_STRIDE;
 }
 
-// class g's tally of pair p of the item
-__device__ __forceinline__ uint16_t *tally_at(const KArgs &a, const ItemMeta &im, long long p, int g) {
-  return a.tab + (size_t)(im.infooff + p) * EDDY_TAB_STRIDE + EDDY_TAB_CLASS * g;
+// the packed tally words of pair p, class g: class 1's as stored; class 2's
+// as the pair's pooled table minus class 1's, word by word (every pooled
+// count is at least class 1's, so no borrow crosses a 16-bit field)
+__device__ __forceinline__ void class_tally_words(const KArgs &a, const ItemMeta &im, const SetMeta &sm, long long p,
+                                                  int g, uint64_t &q0, uint64_t &q1, uint64_t &q2) {
+  const uint64_t *c = (const uint64_t *)tally_at(a, im, p);
+  q0 = c[0];
+  q1 = c[1];
+  q2 = c[2];
+  if (g) {
+    const uint64_t *t = (const uint64_t *)(a.totals + (size_t)(sm.pairbase + p) * EDDY_TAB_CLASS);
+    q0 = t[0] - q0;
+    q1 = t[1] - q1;
+    q2 = t[2] - q2;
+  }
+}
+
+__device__ __forceinline__ void unpack_tally(uint64_t q0, uint64_t q1, uint64_t q2, int *t) {
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    t[i] = (int)((q0 >> (16 * i)) & 0xffff);
+    t[4 + i] = (int)((q1 >> (16 * i)) & 0xffff);
+  }
+  t[8] = (int)(q2 & 0xffff);
+}
+
+__device__ __forceinline__ void load_class_tally(const KArgs &a, const ItemMeta &im, const SetMeta &sm, long long p,
+                                                 int g, int *t) {
+  uint64_t q0, q1, q2;
+  class_tally_words(a, im, sm, p, g, q0, q1, q2);
+  unpack_tally(q0, q1, q2, t);
 }
 
 __device__ __forceinline__ int prior_bit(const KArgs &a, const SetMeta &sm, long long p) {
@@ -181,11 +203,13 @@ __device__ __forceinline__ void one_hot3(uint32_t x, uint32_t &o0, uint32_t &o1,
   o2 = n0 & b1;
 }
 
-// MODE: K1_BOTH both classes on MFMA and every leave-one-out test here too
-// (the all-tests schedule); K1_CLASS1 class 1 on MFMA, class 2's table = the
-// pair's pooled table minus class 1's (the cutoff schedule); K1_POOLED the
-// pooled table of every pair over all samples, written to a.totals (once per
-// set and run: it does not depend on the labels)
+// MODE: K1_BOTH both classes on MFMA over all samples (each masked to its
+// class) and every leave-one-out test here too (the all-tests schedule);
+// K1_CLASS1 class 1 on MFMA over the item's class-1 samples alone (the rows
+// compacted by k_gather_class1, so the contraction is n_A long, not N), class
+// 2's table = the pair's pooled table minus class 1's (the cutoff schedule);
+// K1_POOLED the pooled table of every pair over all samples, written to
+// a.totals (once per set and run: it does not depend on the labels)
 #define K1_BOTH 0
 #define K1_CLASS1 1
 #define K1_POOLED 2
@@ -197,7 +221,8 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
   const ItemMeta im = a.items[item];
   const SetMeta &sm = a.sets[im.set];  // read in place: the bounds are indexed per tally
   const int G = sm.G, N = a.N;
-  const int NP = (N + 63) & ~63;
+  // contraction length and row stride: class 1's compacted samples, or all
+  const int NP = MODE == K1_CLASS1 ? a.kp : (N + 63) & ~63;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
   // the staged MFMA operands of one sample chunk, then (after the last chunk)
@@ -262,7 +287,7 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
       acc2[i][j] = (v4i){0, 0, 0, 0};
     }
 
-  const uint8_t *xs = a.xs + sm.xoff;
+  const uint8_t *xs = MODE == K1_CLASS1 ? a.xc + (size_t)im.goff * a.kp : a.xs + sm.xoff;
   const int rt0 = (wave >> 1) * 3;  // this wave's row tiles (of 6)
   const int ct0 = (wave & 1) * 3;   // this wave's column tiles (of 6)
 
@@ -305,7 +330,7 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
       one_hot3(x.w, h0.w, h1.w, h2.w);
       if (r < K1_TB) {
         uint8_t *d1 = opA1 + (r * 3) * K1_RS + c * 16;
-        if (MODE == K1_POOLED) {
+        if (MODE != K1_BOTH) {  // pooled: every sample; class 1: compacted rows
           *(uint4 *)(d1) = h0;
           *(uint4 *)(d1 + K1_RS) = h1;
           *(uint4 *)(d1 + 2 * K1_RS) = h2;
@@ -314,13 +339,11 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
           *(uint4 *)(d1) = make_uint4(h0.x & m.x, h0.y & m.y, h0.z & m.z, h0.w & m.w);
           *(uint4 *)(d1 + K1_RS) = make_uint4(h1.x & m.x, h1.y & m.y, h1.z & m.z, h1.w & m.w);
           *(uint4 *)(d1 + 2 * K1_RS) = make_uint4(h2.x & m.x, h2.y & m.y, h2.z & m.z, h2.w & m.w);
-          if (MODE == K1_BOTH) {
-            const uint4 m2 = make_uint4(m.x ^ 0x01010101u, m.y ^ 0x01010101u, m.z ^ 0x01010101u, m.w ^ 0x01010101u);
-            uint8_t *d2 = opA2 + (r * 3) * K1_RS + c * 16;
-            *(uint4 *)(d2) = make_uint4(h0.x & m2.x, h0.y & m2.y, h0.z & m2.z, h0.w & m2.w);
-            *(uint4 *)(d2 + K1_RS) = make_uint4(h1.x & m2.x, h1.y & m2.y, h1.z & m2.z, h1.w & m2.w);
-            *(uint4 *)(d2 + 2 * K1_RS) = make_uint4(h2.x & m2.x, h2.y & m2.y, h2.z & m2.z, h2.w & m2.w);
-          }
+          const uint4 m2 = make_uint4(m.x ^ 0x01010101u, m.y ^ 0x01010101u, m.z ^ 0x01010101u, m.w ^ 0x01010101u);
+          uint8_t *d2 = opA2 + (r * 3) * K1_RS + c * 16;
+          *(uint4 *)(d2) = make_uint4(h0.x & m2.x, h0.y & m2.y, h0.z & m2.z, h0.w & m2.w);
+          *(uint4 *)(d2 + K1_RS) = make_uint4(h1.x & m2.x, h1.y & m2.y, h1.z & m2.z, h1.w & m2.w);
+          *(uint4 *)(d2 + 2 * K1_RS) = make_uint4(h2.x & m2.x, h2.y & m2.y, h2.z & m2.z, h2.w & m2.w);
         }
       } else {
         uint8_t *d = opB + ((r - K1_TB) * 3) * K1_RS + c * 16;
@@ -418,8 +441,7 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
       t2[8] = (int)(q2 & 0xffff) - t1[8];
     }
     int pr = (int)((sh_pb[li] >> lj) & 1u);
-    if (a.feat && im.perm == 0) pair_features(t1, t2, pr, sm.c, a.feat + (sm.pairbase + p) * 12);
-    if (!a.feat_only) {
+    {
     uint32_t info = 0;
     const bool logged = im.perm > 0 && im.perm <= a.label_perms;
     if (!a.win || im.perm == 0 || logged || gj <= sh_win[li]) {
@@ -435,8 +457,7 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
       info |= eddy_loo_mask_fast(t2, f2, pr, &a.ep, &sh_eb, cpairs, &ntests) << INFO_LOO_SHIFT(1);
       info |= INFO_EVAL(0) | INFO_EVAL(1);
     } else {
-      store_tally(tally_at(a, im, p, 0), t1);
-      store_tally(tally_at(a, im, p, 1), t2);
+      store_tally(tally_at(a, im, p), t1);
     }
     a.info[im.infooff + p] = info;
     }
@@ -458,8 +479,84 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_count_test(KArgs a) {
   count_test_body<K1_CLASS1>(a);
 }
+
+// The rows of each item's set restricted to the item's class-1 samples, in
+// ascending sample order, padded to a.kp with state 3 (one-hot zero): row g
+// of item i at a.xc + (goff_i + g) * a.kp.  The cutoff schedule's count
+// kernel contracts over these n_A samples instead of all N.  One workgroup per
+// item: the class-1 sample list in LDS (a ballot per 64 samples), then each
+// thread owns four output positions (their sample ids in registers) and
+// walks the rows, four independent byte loads per row (the set's rows are
+// L2-resident, shared by its permutations) and one 4-byte store.
+__global__ __launch_bounds__(256) void k_gather_class1(KArgs a) {
+  const ItemMeta im = a.items[blockIdx.x];
+  const SetMeta &sm = a.sets[im.set];
+  const int N = a.N, NP = (N + 63) & ~63, KP = a.kp, G = sm.G, NW = NP >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t *lab = a.labels + im.laboff;
+  extern __shared__ __align__(16) unsigned char gdyn[];
+  int *base = (int *)gdyn;                   // [NW + 1]: class-1 samples before word w
+  uint16_t *c1 = (uint16_t *)(base + NW + 1);  // [KP]: class-1 sample ids
+  auto c1word = [&](int w) {
+    const int s0 = w * 64;
+    uint64_t c2 = (uint64_t)lab[s0 >> 5];
+    if (s0 + 32 < N) c2 |= (uint64_t)lab[(s0 >> 5) + 1] << 32;
+    const uint64_t valid = N - s0 >= 64 ? ~0ULL : ((1ULL << (N - s0)) - 1ULL);
+    return ~c2 & valid;
+  };
+  for (int w = tid; w < NW; w += 256) base[w + 1] = __popcll(c1word(w));
+  __syncthreads();
+  if (tid == 0) {
+    base[0] = 0;
+    for (int w = 1; w <= NW; w++) base[w] += base[w - 1];
+  }
+  __syncthreads();
+  const uint64_t lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+  for (int w = wave; w < NW; w += EDDY_NWAVES) {
+    const uint64_t m = c1word(w);
+    if ((m >> lane) & 1) c1[base[w] + __popcll(m & lt)] = (uint16_t)(w * 64 + lane);
+  }
+  const int n1 = base[NW];
+  for (int k = n1 + tid; k < KP; k += 256) c1[k] = 0xffff;  // padding
+  __syncthreads();
+  const uint8_t *xs = a.xs + sm.xoff;
+  uint32_t *out = (uint32_t *)(a.xc + (size_t)im.goff * KP);
+  const int q4 = KP >> 2;
+  // row groups: threads tid and tid + q4 take the same positions, other rows
+  const int per = q4 <= 256 ? 256 / q4 : 1;
+  for (int k4 = tid % q4 + (tid / q4 >= per ? q4 : 0); k4 < q4; k4 += 256) {
+    const int grp = q4 <= 256 ? tid / q4 : 0;
+    int id[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) id[j] = c1[4 * k4 + j];
+#pragma unroll 4
+    for (int g = grp; g < G; g += per) {
+      const uint8_t *row = xs + (size_t)g * NP;
+      uint32_t v = 0;
+#pragma unroll
+      for (int j = 0; j < 4; j++) v |= (uint32_t)(id[j] != 0xffff ? row[id[j]] : 3) << (8 * j);
+      out[(size_t)g * q4 + k4] = v;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_count_test_all(KArgs a) { count_test_body<K1_BOTH>(a); }
 __global__ __launch_bounds__(256) void k_count_pooled(KArgs a) { count_test_body<K1_POOLED>(a); }
+
+// Learned-proposal features of every pair of every set from its pooled table
+// (all samples: both classes of permutation 0, which no permutation changes),
+// into a.feat at the pair's run index.  One block per set, a thread per pair.
+__global__ void k_features(KArgs a, int nsets) {
+  if ((int)blockIdx.x >= nsets) return;
+  const SetMeta &sm = a.sets[blockIdx.x];
+  const int zero[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (long long p = threadIdx.x; p < sm.c; p += blockDim.x) {
+    const uint16_t *q = a.totals + (size_t)(sm.pairbase + p) * EDDY_TAB_CLASS;
+    int t[9];
+    for (int i = 0; i < 9; i++) t[i] = q[i];
+    pair_features(t, zero, prior_bit(a, sm, p), sm.c, a.feat + (sm.pairbase + p) * 12);
+  }
+}
 
 // --------------------------------------------------------------------------
 // K2 (cutoff schedule): leave-one-out tests of each row, class g, up to the
@@ -484,7 +581,7 @@ __global__ __launch_bounds__(256) void k_count_pooled(KArgs a) { count_test_body
 __device__ __forceinline__ uint32_t loo_of_pair(const KArgs &a, const ItemMeta &im, const SetMeta &sm, long long p,
                                                 int g, int full, double cpairs, int *ntests) {
   int tt[9];
-  load_tally(tally_at(a, im, p, g), tt);
+  load_class_tally(a, im, sm, p, g, tt);
   return d_loo_mask(tt, full, prior_bit(a, sm, p), a, sm, cpairs, ntests);
 }
 
@@ -503,7 +600,7 @@ __device__ __forceinline__ int full_of(const KArgs &a, const ItemMeta &im, const
   const uint16_t h = *ih;
   if (h & IH_FEVAL) return h & IH_FULL;
   int tt[9];
-  load_tally(tally_at(a, im, p, g), tt);
+  load_class_tally(a, im, sm, p, g, tt);
   const int f = d_edge_test(tt, prior_bit(a, sm, p), a, sm, cpairs);
   (*ntests)++;
   *ih = (uint16_t)(h | IH_FEVAL | (f ? IH_FULL : 0u));
@@ -655,10 +752,7 @@ __device__ __forceinline__ void loo_body(const KArgs &a) {
     x.p = p;
     x.ih = *info_half(a, im, p, g);
     x.pw = a.prior[sm.poff + (p >> 5)];
-    const uint64_t *q = (const uint64_t *)tally_at(a, im, p, g);
-    x.q0 = q[0];
-    x.q1 = q[1];
-    x.q2 = q[2];
+    class_tally_words(a, im, sm, p, g, x.q0, x.q1, x.q2);
   };
   LooEntry nx;
   locate(wave * 64, nx);
@@ -667,12 +761,7 @@ __device__ __forceinline__ void loo_body(const KArgs &a) {
     locate(e0 + 256, nx);
     if (!cu.ok) continue;
     int tt[9];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      tt[i] = (int)((cu.q0 >> (16 * i)) & 0xffff);
-      tt[4 + i] = (int)((cu.q1 >> (16 * i)) & 0xffff);
-    }
-    tt[8] = (int)(cu.q2 & 0xffff);
+    unpack_tally(cu.q0, cu.q1, cu.q2, tt);
     const int pr = (int)((cu.pw >> (cu.p & 31)) & 1u);
     uint32_t h = cu.ih;
     int full;

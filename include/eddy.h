@@ -231,6 +231,11 @@ typedef struct {
                                         loo 10 plan, 11 packed tests, 12 cutoff
                                         check and fallback; count 13 staging
                                         and MFMA, 14 tests and stores */
+  double split_ms[6];                /* summed device time per kernel of the
+                                        batched sequence: 0 count (with the
+                                        class-1 gather before it), 1 loo,
+                                        2 rows, 3 score, 4 BDeu, 5 tail */
+  long long split_launches[6];
 } eddy_timing;
 int eddy_get_timing(eddy_ctx *ctx, eddy_timing *t);
 
