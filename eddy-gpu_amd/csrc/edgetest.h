@@ -128,36 +128,46 @@ EDDY_HD int eddy_edge_decision(const int *t, int prior, const eddy_edge_params *
 // ---------------------------------------------------------------------------
 // fast forms
 // ---------------------------------------------------------------------------
-// 1/v and 1/(v - 1) from one division (0 for a zero denominator): with
-// inv = 1/(v (v - 1)) rounded once, (v - 1) inv and v inv are within 1.5 ulp
-// of the true reciprocals (v (v - 1) <= 2^22 is exact).
-EDDY_HD void eddy_recip_pair(int v, double *r, double *rm) {
-  if (v >= 2) {
-    const double inv = 1.0 / ((double)v * (double)(v - 1));
-    *r = (double)(v - 1) * inv;
-    *rm = (double)v * inv;
-  } else {
-    *r = v == 1 ? 1.0 : 0.0;
-    *rm = 0.0;
+// The reciprocals of three margins from one division: with P the product of
+// the nonzero ones (exact: three integers <= 20000 multiply to < 2^53) and
+// inv = 1/P rounded once, r_i = (P / v_i) inv, where P / v_i is an exact
+// integer product, is within 1 ulp of 1 / v_i; zero margins get 0.  With
+// `minus1` the reciprocals of v_i - 1 (0 where v_i < 2).
+EDDY_HD void eddy_recip3(const int *v, double *r, int minus1) {
+  double w[3], P = 1.0;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const int x = minus1 ? v[i] - 1 : v[i];
+    w[i] = x > 0 ? (double)x : 0.0;
+    if (x > 0) P = P * w[i];
+  }
+  const double inv = 1.0 / P;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    double o = 1.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      if (j != i && w[j] > 0.0) o = o * w[j];
+    r[i] = w[i] > 0.0 ? o * inv : 0.0;
   }
 }
 
 EDDY_HD double eddy_sq(int v) { return (double)(v * v); }
 
 // x = chi2 / 2 and dof of tally t with row sums R, column sums K, total n,
-// from iR[i] = 1/R[i], iK[j] = 1/K[j] (0 for empty sums, within 1.5 ulp) and
-// in = 1/n.  Yates' rule is decided exactly in integers: (double)K*R/n < 5
+// from iR[i] = 1/R[i], iK[j] = 1/K[j] (0 for empty sums, within 1.5 ulp).  Yates' rule is decided exactly in integers: (double)K*R/n < 5
 // holds iff R*K < 5n, since a quotient of integers <= 2^22 that is not 5
 // differs from 5 by at least 1/n, far more than one rounding.
 //  - Without Yates' correction, the identity sum (O - E)^2 / E =
 //    n (sum O^2 / (R K) - 1) (empty rows and columns left out): each product
 //    O^2 iK iR is within 5 ulp, their sum S <= 3 within 12 ulp of itself, so
-//    |x - x_exact| <= 0.5 n 36 u + 2 u x < 4.1e-12 + 2.3e-16 x (n <= 2046).
+//    |x - x_exact| <= 0.5 n 36 u + 2 u x < 4e-11 + 2.3e-16 x (n <= 20000).
 //  - With it, per cell (|O - E| - y)^2 n iR iK, which differs from the
 //    reference's d*d/E by a few ulp of itself plus 2|d| * 3 ulp(E) <=
-//    2 n 6.7e-16, so |x - x_ref| <= 9 n 6.7e-16 + 1e-15 x < 1.3e-11 + 1e-15 x.
+//    2 n 6.7e-16, so |x - x_ref| <= 9 n 6.7e-16 + 1e-15 x < 1.3e-10 + 1e-15 x
+//    (n <= 20000).
 EDDY_HD double eddy_half_chi2_fast(const int *t, const int *R, const int *K, int n, const double *iR,
-                                   const double *iK, double in, int *dof) {
+                                   const double *iK, int *dof) {
   int zr = 0, zk = 0, yates = 0;
   for (int i = 0; i < 3; i++) {
     zr += R[i] == 0;
@@ -176,7 +186,7 @@ EDDY_HD double eddy_half_chi2_fast(const int *t, const int *R, const int *K, int
     }
     return 0.5 * (double)n * (S - 1.0);
   }
-  const double y = 0.5, dn = (double)n;
+  const double y = 0.5, dn = (double)n, in = 1.0 / dn;
   double chi = 0.0;
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++)
@@ -205,7 +215,6 @@ EDDY_HD int eddy_edge_certain(double x, int dof, int prior, const eddy_edge_para
   return -1;
 }
 
-EDDY_HD double eddy_recip_or0(int v) { return v ? 1.0 / (double)v : 0.0; }
 
 
 // eddy_edge_decision() with the fast x
@@ -219,12 +228,10 @@ EDDY_HD int eddy_edge_decision_fast(const int *t, int prior, const eddy_edge_par
   const int n = R[0] + R[1] + R[2];
   if (n > 0) {
     double iR[3], iK[3];
-    for (int i = 0; i < 3; i++) {
-      iR[i] = eddy_recip_or0(R[i]);
-      iK[i] = eddy_recip_or0(K[i]);
-    }
+    eddy_recip3(R, iR, 0);
+    eddy_recip3(K, iK, 0);
     int dof;
-    const double x = eddy_half_chi2_fast(t, R, K, n, iR, iK, 1.0 / (double)n, &dof);
+    const double x = eddy_half_chi2_fast(t, R, K, n, iR, iK, &dof);
     const int c = eddy_edge_certain(x, dof, prior, ep, eb);
     if (c >= 0) return c;
   }
@@ -241,8 +248,9 @@ EDDY_HD int eddy_edge_decision_fast(const int *t, int prior, const eddy_edge_par
 // (t_ab - 1)^2.  Error: every product is within 5 ulp, the sum S of at most
 // nine nonnegative terms (S <= 3) within 12 ulp of itself, so
 // |x - x_exact| <= 0.5 (n - 1) 36 u + 2 u x < 4.1e-12 + 2.3e-16 x for
-// n <= 2046 (u = 2^-53): with the reference formula's own rounding this stays
-// far inside the 1e-9 (1 + x) guard of eddy_edge_certain.
+// n <= 2046 (4e-11 + 2.3e-16 x for n <= 20000; u = 2^-53): with the reference
+// formula's own rounding this stays far inside the 1e-9 (1 + x) guard of
+// eddy_edge_certain.
 EDDY_HD double eddy_loo_half_chi2(const int *t, int n, int a, int b, const double *T, const double *U,
                                   const double *iR, const double *iRm, const double *iKm) {
   const double W = T[a] + eddy_sq(t[a * 3 + b] - 1) * iKm[b];
@@ -280,7 +288,7 @@ EDDY_HD double eddy_loo_x(const int *t, const int *R, const int *K, int n, int a
     iR1[i] = i == a ? iRm[i] : iR[i];
     iK1[i] = i == b ? iKm[i] : iK[i];
   }
-  return eddy_half_chi2_fast(u, R1, K1, n - 1, iR1, iK1, 1.0 / (double)(n - 1), &d);
+  return eddy_half_chi2_fast(u, R1, K1, n - 1, iR1, iK1, &d);
 }
 
 // margins of t, their reciprocal pairs, n
@@ -289,10 +297,10 @@ EDDY_HD int eddy_loo_margins(const int *t, int *R, int *K, double *iR, double *i
     R[i] = t[i * 3] + t[i * 3 + 1] + t[i * 3 + 2];
     K[i] = t[i] + t[3 + i] + t[6 + i];
   }
-  for (int i = 0; i < 3; i++) {
-    eddy_recip_pair(R[i], &iR[i], &iRm[i]);
-    eddy_recip_pair(K[i], &iK[i], &iKm[i]);
-  }
+  eddy_recip3(R, iR, 0);
+  eddy_recip3(R, iRm, 1);
+  eddy_recip3(K, iK, 0);
+  eddy_recip3(K, iKm, 1);
   return R[0] + R[1] + R[2];
 }
 
@@ -328,15 +336,23 @@ EDDY_HD void eddy_loo_column(const int *t, int b, const double *iK, const double
 // leave-one-out tally.  The fast x is within 1e-9 (1 + x) of the exact value,
 // and each interval end gets the same guard, so an answer here is what the
 // exact decision of each leave-one-out tally gives.
-EDDY_HD int eddy_loo_all_certain(const int *R, const int *K, int n, const double *iR, const double *iK, const int *t,
-                                 int prior, const eddy_edge_params *ep, const eddy_edge_bounds *eb) {
+EDDY_HD int eddy_loo_all_certain(const int *R, const int *K, int n, const double *iR, const double *iK,
+                                 const double *iRm, const double *iKm, const int *t, int prior,
+                                 const eddy_edge_params *ep, const eddy_edge_bounds *eb) {
   if (!ep->shortcut || n < 2) return -1;
   int rmin = 1 << 30, kmin = 1 << 30, zr = 0, zk = 0;
+  double irm = 0.0, ikm = 0.0;  // 1/(rmin - 1), 1/(kmin - 1), within 1 ulp
   for (int i = 0; i < 3; i++) {
     if (R[i] == 0) zr++;
-    else if (R[i] < rmin) rmin = R[i];
+    else if (R[i] < rmin) {
+      rmin = R[i];
+      irm = iRm[i];
+    }
     if (K[i] == 0) zk++;
-    else if (K[i] < kmin) kmin = K[i];
+    else if (K[i] < kmin) {
+      kmin = K[i];
+      ikm = iKm[i];
+    }
   }
   if (rmin < 2 || kmin < 2) return -1;
   if ((long long)(rmin - 1) * (long long)(kmin - 1) < 5LL * (n - 1)) return -1;
@@ -346,7 +362,7 @@ EDDY_HD int eddy_loo_all_certain(const int *R, const int *K, int n, const double
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) S = S + eddy_sq(t[i * 3 + j]) * iR[i] * iK[j];
   const double x = 0.5 * (double)n * (S - 1.0);
-  const double e = 1.0 / (double)(rmin - 1) + 1.0 / (double)(kmin - 1);
+  const double e = irm + ikm;
   const double d = 0.5 * (double)(n - 1) * e * (1.0 + 1e-12);
   const double xg = x + 1e-9 * (1.0 + (x > 0.0 ? x : 0.0));  // >= the exact x
   const double up = xg + d;                                   // >= every leave-one-out x
@@ -366,7 +382,7 @@ EDDY_HD uint32_t eddy_loo_mask_fast(const int *t, int full, int prior, const edd
   double iR[3], iK[3], iRm[3], iKm[3];
   const int n = eddy_loo_margins(t, R, K, iR, iRm, iK, iKm);
   {
-    const int all = eddy_loo_all_certain(R, K, n, iR, iK, t, prior, ep, eb);
+    const int all = eddy_loo_all_certain(R, K, n, iR, iK, iRm, iKm, t, prior, ep, eb);
     if (all >= 0) {  // nonzero cells take `all`, empty cells copy `full`
       uint32_t nz = 0;
       for (int c = 0; c < 9; c++) nz |= (t[c] > 0 ? 1u : 0u) << c;

@@ -2575,7 +2575,7 @@ extern "C" int eddy_loo_all_certain_host(const int *tally9, int c, int prior, co
   int R[3], K[3];
   double iR[3], iK[3], iRm[3], iKm[3];
   const int n = eddy_loo_margins(tally9, R, K, iR, iRm, iK, iKm);
-  return eddy_loo_all_certain(R, K, n, iR, iK, tally9, prior, &ep, &eb);
+  return eddy_loo_all_certain(R, K, n, iR, iK, iRm, iKm, tally9, prior, &ep, &eb);
 }
 
 extern "C" int eddy_edge_loo_mask_host(const int *tally9, int c, int prior, int full, const eddy_params *p) {
@@ -2616,11 +2616,9 @@ extern "C" int eddy_chi2_host(const int *tally9, int fast, int *dof, double *chi
   const int n = R[0] + R[1] + R[2];
   if (n <= 0) return EDDY_ERR_ARG;
   double iR[3], iK[3];
-  for (int i = 0; i < 3; i++) {
-    iR[i] = eddy_recip_or0(R[i]);
-    iK[i] = eddy_recip_or0(K[i]);
-  }
-  *chi2 = 2.0 * eddy_half_chi2_fast(tally9, R, K, n, iR, iK, 1.0 / (double)n, dof);
+  eddy_recip3(R, iR, 0);
+  eddy_recip3(K, iK, 0);
+  *chi2 = 2.0 * eddy_half_chi2_fast(tally9, R, K, n, iR, iK, dof);
   return EDDY_OK;
 }
 
