@@ -162,6 +162,7 @@ struct eddy_ctx {
   std::string err;
   // dataset
   int genes = 0, N = 0, n1 = 0, n2 = 0;
+  int tabw = EDDY_TAB_FULL;  // u16 per stored tally (EDDY_TAB_DENSE when every state is 0..2)
   uint8_t *dX = nullptr;
   std::vector<uint8_t> classOf;
   double *dLG = nullptr;
@@ -442,9 +443,11 @@ extern "C" int eddy_upload_dataset(eddy_ctx *ctx, const uint8_t *x, int genes, i
   }
   const int NP = (samples + 63) & ~63;
   std::vector<uint8_t> padded((size_t)genes * NP, 3);
+  bool dense = true;  // every sample in one of the three states
   for (int g = 0; g < genes; g++)
     for (int s = 0; s < samples; s++) {
       uint8_t v = x[(size_t)g * samples + s];
+      dense = dense && v <= 2;
       padded[(size_t)g * NP + s] = v > 2 ? 3 : v;
     }
   if (ctx->dX) HIPCHK(ctx, hipFree(ctx->dX));
@@ -455,6 +458,7 @@ extern "C" int eddy_upload_dataset(eddy_ctx *ctx, const uint8_t *x, int genes, i
   ctx->N = samples;
   ctx->n1 = n1;
   ctx->n2 = samples - n1;
+  ctx->tabw = dense ? EDDY_TAB_DENSE : EDDY_TAB_FULL;
   ctx->classOf.assign(class_of, class_of + samples);
   // lgamma(n + 4/d) for d = 1..81, n = 0..N: the BDeu arguments of
   // kernel5.cu:205-216 (nijk + N'/(q r), nij + N'/q with N' = 4)
@@ -629,7 +633,7 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   // class 1's samples of every item, compacted (cutoff schedule's count kernel)
   const int kp = ((ctx->n1 + 63) / 64) * 64;
   const bool gather1 = !plain_tests && !ch.tiles.empty();
-  if (!plain_tests) CHK(grow(ctx, sl.tab, sizeof(uint16_t) * EDDY_TAB_STRIDE * ch.pairs));
+  if (!plain_tests) CHK(grow(ctx, sl.tab, sizeof(uint16_t) * ctx->tabw * ch.pairs));
   if (gather1) CHK(grow(ctx, sl.xc, (size_t)kp * ch.gslots));
   CHK(grow(ctx, sl.lfull, sizeof(uint16_t) * (size_t)mp * 2 * ch.gslots));
   CHK(grow(ctx, sl.lcnt, sizeof(uint16_t) * 2 * ch.gslots));
@@ -736,6 +740,7 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   a.info = (uint32_t *)sl.info.p;
   a.tab = (uint16_t *)sl.tab.p;
   a.totals = (uint16_t *)ctx->b_totals.p;
+  a.tabw = ctx->tabw;
   a.lfull = (uint16_t *)sl.lfull.p;
   a.lcnt = (uint16_t *)sl.lcnt.p;
   a.sflags = (uint8_t *)sl.sflags.p;
@@ -1350,7 +1355,7 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
   // the pooled table (all samples, both classes) of every pair of every set:
   // the cutoff schedule's count kernel takes class 2's table as this minus
   // class 1's.  One item per set, launches of at most 64M pairs.
-  GROW(b_totals, sizeof(uint16_t) * EDDY_TAB_CLASS * (size_t)std::max(1LL, pairbase));
+  GROW(b_totals, sizeof(uint16_t) * ctx->tabw * (size_t)std::max(1LL, pairbase));
   {
     const std::vector<uint32_t> lab = label_bits(ctx->classOf);
     Chunk ch;
@@ -1377,6 +1382,7 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
       a.labels = (const uint32_t *)ctx->b_labels.p;
       a.tiles = (const uint32_t *)ctx->b_tiles.p;
       a.totals = (uint16_t *)ctx->b_totals.p;
+      a.tabw = ctx->tabw;
       k_count_pooled<<<(unsigned)ch.tiles.size(), 256, 0, st>>>(a);
       HIPCHK(ctx, hipGetLastError());
       HIPCHK(ctx, hipStreamSynchronize(st));
@@ -1495,9 +1501,11 @@ int compute_features(eddy_ctx *ctx, const std::vector<SetMeta> &meta) {
   const int nsets = (int)meta.size();
   if (nsets == 0) return EDDY_OK;
   KArgs a{};
+  a.N = ctx->N;
   a.sets = (const SetMeta *)ctx->b_sets.p;
   a.prior = (const uint32_t *)ctx->b_prior.p;
   a.totals = (uint16_t *)ctx->b_totals.p;
+  a.tabw = ctx->tabw;
   a.feat = (float *)ctx->b_feat.p;
   k_features<<<nsets, 256, 0, ctx->stream>>>(a, nsets);
   HIPCHK(ctx, hipGetLastError());

@@ -64,11 +64,14 @@ struct Slice {
   long long bytes;
 };
 
-// per-pair tallies in the cutoff schedule: class 1's 3x3 tally as 12 u16 (9
-// counts + padding); class 2's is the pair's pooled table (same layout, once
-// per set) minus class 1's
-#define EDDY_TAB_CLASS 12
-#define EDDY_TAB_STRIDE 12
+// per-pair tallies in the cutoff schedule: class 1's 3x3 tally as packed u16
+// counts; class 2's is the pair's pooled table (same layout, once per set)
+// minus class 1's.  Cells 0..7 take two 8-byte words (8 u16).  Cell 8 is the
+// class size minus the other eight when every sample of the dataset holds one
+// of the three states (each sample then falls in exactly one cell); a dataset
+// with samples outside them stores cell 8 in a third word (12 u16).
+#define EDDY_TAB_DENSE 8
+#define EDDY_TAB_FULL 12
 
 struct KArgs {
   // dataset-level
@@ -96,8 +99,9 @@ struct KArgs {
   const uint32_t *tiles;   // K1 work list: item<<16 | bi<<8 | bj
   // per-pair outputs
   uint32_t *info;
-  uint16_t *tab;           // [pair][EDDY_TAB_CLASS] class 1's counts (cutoff mode)
-  uint16_t *totals;        // [set pairbase + p][EDDY_TAB_CLASS]: pooled table of each pair over all samples
+  uint16_t *tab;           // [pair][tabw] class 1's counts (cutoff mode)
+  uint16_t *totals;        // [set pairbase + p][tabw]: pooled table of each pair over all samples
+  int tabw;                // u16 per tally: EDDY_TAB_DENSE or EDDY_TAB_FULL
   // per-item scratch
   uint16_t *lfull;         // [goff*2 + g*G + t][mpw]
   uint16_t *lcnt;          // [goff*2 + g*G + t]

@@ -45,19 +45,55 @@ __device__ __forceinline__ uint32_t d_loo_mask(const int *t, int full, int prior
 
 __device__ __forceinline__ long long pair_index(int t, int col) { return (long long)t * (t - 1) / 2 + col; }
 
-// A class's 3x3 tally as 12 u16 (9 counts + 3 padding): three 8-byte
-// stores / loads instead of nine 2-byte ones.
-__device__ __forceinline__ void store_tally(uint16_t *dst, const int *t) {
-  uint64_t *d = (uint64_t *)dst;
-  d[0] = (uint64_t)(uint16_t)t[0] | ((uint64_t)(uint16_t)t[1] << 16) | ((uint64_t)(uint16_t)t[2] << 32) |
-         ((uint64_t)(uint16_t)t[3] << 48);
-  d[1] = (uint64_t)(uint16_t)t[4] | ((uint64_t)(uint16_t)t[5] << 16) | ((uint64_t)(uint16_t)t[6] << 32) |
-         ((uint64_t)(uint16_t)t[7] << 48);
-  d[2] = (uint64_t)(uint16_t)t[8];
+// A class's 3x3 tally as packed u16 counts: cells 0..7 in two 8-byte words,
+// cell 8 in a third only when the dataset stores it (a.tabw ==
+// EDDY_TAB_FULL); otherwise it is the class size minus the other eight.
+__device__ __forceinline__ void store_tally(uint16_t *dst, const int *t, int tabw) {
+  const uint64_t w0 = (uint64_t)(uint16_t)t[0] | ((uint64_t)(uint16_t)t[1] << 16) |
+                      ((uint64_t)(uint16_t)t[2] << 32) | ((uint64_t)(uint16_t)t[3] << 48);
+  const uint64_t w1 = (uint64_t)(uint16_t)t[4] | ((uint64_t)(uint16_t)t[5] << 16) |
+                      ((uint64_t)(uint16_t)t[6] << 32) | ((uint64_t)(uint16_t)t[7] << 48);
+  if (tabw == EDDY_TAB_DENSE) {
+    *(ulonglong2 *)dst = make_ulonglong2(w0, w1);  // 16-byte aligned: one store
+  } else {
+    uint64_t *d = (uint64_t *)dst;
+    d[0] = w0;
+    d[1] = w1;
+    d[2] = (uint64_t)(uint16_t)t[8];
+  }
+}
+// cell 8 of a tally of n samples from its cells 0..7 (packed in q0, q1): every
+// partial sum is at most n <= 20000, so the packed additions never carry
+// across a 16-bit field
+__device__ __forceinline__ uint64_t tally_cell8(uint64_t q0, uint64_t q1, int n) {
+  uint64_t s = q0 + q1;
+  s += s >> 32;
+  s += s >> 16;
+  return (uint64_t)(uint32_t)(n - (int)(s & 0xffffu));
+}
+// the tally words (q0, q1, q2) of the table at q: q2 loaded, or derived from
+// the n samples the table counts
+__device__ __forceinline__ void load_tally_words(const uint16_t *q, int tabw, int n, uint64_t &q0, uint64_t &q1,
+                                                 uint64_t &q2) {
+  if (tabw == EDDY_TAB_DENSE) {
+    const ulonglong2 v = *(const ulonglong2 *)q;
+    q0 = v.x;
+    q1 = v.y;
+    q2 = tally_cell8(q0, q1, n);
+  } else {
+    const uint64_t *w = (const uint64_t *)q;
+    q0 = w[0];
+    q1 = w[1];
+    q2 = w[2];
+  }
 }
 // class 1's tally of pair p of the item (cutoff schedule)
 __device__ __forceinline__ uint16_t *tally_at(const KArgs &a, const ItemMeta &im, long long p) {
-  return a.tab + (size_t)(im.infooff + p) * EDDY_TAB_STRIDE;
+  return a.tab + (size_t)(im.infooff + p) * a.tabw;
+}
+// the pooled table of pair p of set sm
+__device__ __forceinline__ const uint16_t *totals_at(const KArgs &a, const SetMeta &sm, long long p) {
+  return a.totals + (size_t)(sm.pairbase + p) * a.tabw;
 }
 
 // the packed tally words of pair p, class g: class 1's as stored; class 2's
@@ -65,15 +101,13 @@ __device__ __forceinline__ uint16_t *tally_at(const KArgs &a, const ItemMeta &im
 // count is at least class 1's, so no borrow crosses a 16-bit field)
 __device__ __forceinline__ void class_tally_words(const KArgs &a, const ItemMeta &im, const SetMeta &sm, long long p,
                                                   int g, uint64_t &q0, uint64_t &q1, uint64_t &q2) {
-  const uint64_t *c = (const uint64_t *)tally_at(a, im, p);
-  q0 = c[0];
-  q1 = c[1];
-  q2 = c[2];
+  load_tally_words(tally_at(a, im, p), a.tabw, a.n1, q0, q1, q2);
   if (g) {
-    const uint64_t *t = (const uint64_t *)(a.totals + (size_t)(sm.pairbase + p) * EDDY_TAB_CLASS);
-    q0 = t[0] - q0;
-    q1 = t[1] - q1;
-    q2 = t[2] - q2;
+    uint64_t t0, t1, t2;
+    load_tally_words(totals_at(a, sm, p), a.tabw, a.N, t0, t1, t2);
+    q0 = t0 - q0;
+    q1 = t1 - q1;
+    q2 = t2 - q2;
   }
 }
 
@@ -379,12 +413,8 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
   // loads' latency hides under the tests
   auto tot_load = [&](int idx, uint64_t &q0, uint64_t &q1, uint64_t &q2) {
     const int li = idx >> 5, lj = idx & 31, gi = bi * K1_TB + li, gj = bj * K1_TB + lj;
-    if (MODE == K1_CLASS1 && idx < K1_TB * K1_TB && gi < G && gj < gi) {
-      const uint64_t *q = (const uint64_t *)(a.totals + (size_t)(sm.pairbase + pair_index(gi, gj)) * EDDY_TAB_CLASS);
-      q0 = q[0];
-      q1 = q[1];
-      q2 = q[2];
-    }
+    if (MODE == K1_CLASS1 && idx < K1_TB * K1_TB && gi < G && gj < gi)
+      load_tally_words(totals_at(a, sm, pair_index(gi, gj)), a.tabw, a.N, q0, q1, q2);
   };
   uint64_t nq0 = 0, nq1 = 0, nq2 = 0;
   tot_load(tid, nq0, nq1, nq2);
@@ -424,7 +454,7 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
       for (int k = 0; k < 3; k++) t1[r * 3 + k] = T[(li * 3 + r) * K1_TS + lj * 3 + k];
     long long p = pair_index(gi, gj);
     if (MODE == K1_POOLED) {
-      store_tally(a.totals + (size_t)(sm.pairbase + p) * EDDY_TAB_CLASS, t1);
+      store_tally(a.totals + (size_t)(sm.pairbase + p) * a.tabw, t1, a.tabw);
       continue;  // the loop's ballots below only serve the test modes (fbits is null here)
     }
     if (MODE == K1_BOTH) {
@@ -457,7 +487,7 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
       info |= eddy_loo_mask_fast(t2, f2, pr, &a.ep, &sh_eb, cpairs, &ntests) << INFO_LOO_SHIFT(1);
       info |= INFO_EVAL(0) | INFO_EVAL(1);
     } else {
-      store_tally(tally_at(a, im, p), t1);
+      store_tally(tally_at(a, im, p), t1, a.tabw);
     }
     a.info[im.infooff + p] = info;
     }
@@ -551,9 +581,10 @@ __global__ void k_features(KArgs a, int nsets) {
   const SetMeta &sm = a.sets[blockIdx.x];
   const int zero[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (long long p = threadIdx.x; p < sm.c; p += blockDim.x) {
-    const uint16_t *q = a.totals + (size_t)(sm.pairbase + p) * EDDY_TAB_CLASS;
+    uint64_t q0, q1, q2;
+    load_tally_words(totals_at(a, sm, p), a.tabw, a.N, q0, q1, q2);
     int t[9];
-    for (int i = 0; i < 9; i++) t[i] = q[i];
+    unpack_tally(q0, q1, q2, t);
     pair_features(t, zero, prior_bit(a, sm, p), sm.c, a.feat + (sm.pairbase + p) * 12);
   }
 }

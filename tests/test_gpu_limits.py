@@ -9,6 +9,10 @@
   main.cu:170-172): 17, 20 and 100 on the 40-gene set of the 200 x 200
   workload (100 caps nothing there: a row has at most 39 candidate parents),
   both schedules, and a whole set through the stream at -mp 20.
+* Samples outside the three states (an expression value other than -1, 0,
+  1 counts in no cell and no observed state, as in kernel1.cu:338-387 and
+  kernel5.cu:197-218): the tallies then store their ninth cell instead of
+  deriving it from the class size.
 Every output is compared bit for bit.
 """
 import ctypes as C
@@ -160,3 +164,37 @@ def test_parent_cap_20_through_stream(engine):
         assert np.array_equal(ids, o["uniq_ids"]) and np.array_equal(edges, o["uni_edges"])
         assert np.array_equal(nodes, o["uni_nodes"]) and np.array_equal(epn, o["uni_epn"])
         assert np.array_equal(lval.view(np.uint64), o["lval"].view(np.uint64))
+
+
+@pytest.mark.parametrize("schedule", [eddy_amd.TESTS_ALL, eddy_amd.TESTS_CUTOFF])
+def test_samples_outside_the_three_states(engine, schedule):
+    wl = eddy_amd.Workload(0)
+    X = wl.X.copy()
+    rng = np.random.default_rng(0x5EED)
+    X[rng.random(X.shape) < 0.03] = 3  # about 3% of the values out of range
+    engine.upload(X, wl.classes)
+    genes, pairs = wl.sets[0]
+    data_ref = X.astype(np.int32) - 1
+    for seed in range(2):
+        lab = wl.classes if seed == 0 else _labels(wl.N, wl.cfg.nA, 50 + seed)
+        prm = wl.params(schedule=schedule, mode=eddy_amd.MODE_BATCHED)
+        print("case: out-of-range states, schedule %d seed %d" % (schedule, seed), flush=True)
+        g = engine.perm_detail(genes, pairs, lab, prm, bits=schedule == eddy_amd.TESTS_ALL, out5=True)
+        t1, t2, n1, n2 = oracle_ref.class_split(data_ref, genes, lab)
+        o = oracle_ref.run_perm(t1, t2, n1, n2, wl.prior_matrix(0), oracle_ref.params(mp=3, threads=_threads()),
+                                want_bits=schedule == eddy_amd.TESTS_ALL)
+        _compare(g, o, schedule == eddy_amd.TESTS_ALL)
+    if schedule == eddy_amd.TESTS_CUTOFF:
+        sets = [(genes, pairs), (genes[:17], pairs[:0])]
+        priors = [wl.prior_matrix(0), np.zeros((17, 17), np.int32)]
+        print("case: out-of-range states, two sets through the stream at r = 4", flush=True)
+        engine.reset_stream()
+        res = engine.run_sets(sets, wl.params(perms=4))
+        oracle_ref.reset_stream()
+        for k in range(2):
+            o = oracle_ref.run_set(data_ref, wl.classes.astype(np.int32), sets[k][0], priors[k],
+                                   oracle_ref.params(mp=3, perms=4, threads=_threads()))
+            n = o["ran"]
+            assert res[k].perms_run == n
+            assert np.array_equal(engine.js(k)[:n].view(np.uint64), o["js"][:n].view(np.uint64)), k
+            assert res[k].U0 == o["U0"]
