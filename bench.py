@@ -140,7 +140,7 @@ def cpu_baseline(wl, perms_run, seconds_1t, threads):
 
 
 def workflow4(args):
-    """Config 4's workflow (uneven classes 300/900, 20000 genes, 2000 sets of 5..500
+    """The 20000-gene x 1200-sample workflow (uneven classes 300/900, 2000 sets of 5..500
     genes): the learned job at r = 100 over every set, then the 200 most
     significant sets again at r = 1000 on a fresh stream.  One warm-up run of
     the first phase, then both phases timed once.  Prints one JSON line."""
@@ -154,7 +154,7 @@ def workflow4(args):
     prm = wl.params(mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=1)
     cfg = eddy_amd.learn_cfg(train_sets=100, epochs=args.epochs, batch=16384, lr=3e-3, margin=args.margin, seed=1,
                              max_rows=args.train_rows, label_perms=args.label_perms)
-    print("bench: config 4, %d sets, N=%d (%d/%d), setup %.1fs" % (len(wl.sets), wl.N, wl.cfg.nA, wl.cfg.nB,
+    print("bench: 20000 x 1200 workflow, %d sets, N=%d (%d/%d), setup %.1fs" % (len(wl.sets), wl.N, wl.cfg.nA, wl.cfg.nB,
                                                                    setup_s), file=sys.stderr, flush=True)
     eng.reset_stream()
     eng.run_learned(wl.sets, prm, cfg)
@@ -167,7 +167,7 @@ def workflow4(args):
     print("bench: r=100 phase %.2fs, r=1000 phase %.2fs" % (t1 - t0, t2 - t1), file=sys.stderr, flush=True)
     G = [len(wl.sets[q][0]) for q in chosen]
     line = {
-        "metric": "config-4 workflow: gene sets/s at r=100 (batched+learned), then r=1000 on the top 200",
+        "metric": "20000 x 1200 workflow: gene sets/s at r=100 (batched+learned), then r=1000 on the top 200",
         "value": len(wl.sets) / (t1 - t0), "unit": "gene sets/s", "n_gpus": 1, "higher_is_better": True,
         "data": "synthetic (seeded linear-Gaussian DAG, tertiles)",
         "config": {"workload": "20000 genes x 1200 samples (300/900), 2000 gene sets (G 5..500 log-uniform), "
@@ -187,6 +187,131 @@ def run_step(eng, wl, prm, cfg):
     eng.reset_stream()
     res, rep = eng.run_learned(wl.sets, prm, cfg)
     return res, rep, eng.timing()
+
+
+# ---------------------------------------------------------------- profile summaries
+# rocprofv3 writes one SQLite database per run (-d DIR -o run: DIR/run_results.db).
+# The summaries below read its `kernels` and `pmc_events` views.  Every job
+# starts with the set-row gather (`k_gather`); the autotuner's calibration
+# runs its own first, so the dispatches from the (skip + 1)-th `k_gather` on
+# are the timed jobs (and the warm-up ones).
+
+def _kname(n):
+    return n.split("(")[0]
+
+
+def _dispatches(db):
+    import sqlite3
+    c = sqlite3.connect(db)
+    return c, c.execute("select dispatch_id, name, start, end from kernels order by start").fetchall()
+
+
+def _job_start(rows, skip):
+    g = [i for i, r in enumerate(rows) if _kname(r[1]) == "k_gather"]
+    if len(g) <= skip:
+        raise SystemExit("profile summary: fewer than %d k_gather dispatches" % (skip + 1))
+    return g[skip], len(g) - skip
+
+
+def summarize_trace(db, skip):
+    _, rows = _dispatches(db)
+    i0, jobs = _job_start(rows, skip)
+    rows = rows[i0:]
+    agg = {}
+    for _, n, s, e in rows:
+        a = agg.setdefault(_kname(n), [0, 0.0, 1e30, 0.0])
+        d = (e - s) / 1e6
+        a[0] += 1
+        a[1] += d
+        a[2] = min(a[2], d)
+        a[3] = max(a[3], d)
+    tot = sum(a[1] for a in agg.values())
+    out = ["| kernel | calls | total ms | ms per job | avg ms | min ms | max ms | % |", "|---|---|---|---|---|---|---|---|"]
+    for k, a in sorted(agg.items(), key=lambda x: -x[1][1]):
+        out.append("| %s | %d | %.2f | %.2f | %.4f | %.4f | %.4f | %.1f |"
+                   % (k, a[0], a[1], a[1] / jobs, a[1] / a[0], a[2], a[3], 100 * a[1] / tot))
+    out.append("")
+    out.append("jobs: %d, span of their dispatches %.1f ms" % (jobs, (rows[-1][3] - rows[0][2]) / 1e6))
+    return "\n".join(out)
+
+
+def _counters(db, skip):
+    """{kernel: {counter: total}}, dispatch counts, jobs, and the dispatches before the jobs."""
+    c, rows = _dispatches(db)
+    i0, jobs = _job_start(rows, skip)
+    after = {r[0] for r in rows[i0:]}
+    vals = {}
+    for d, n, cn, v in c.execute("select dispatch_id, name, counter_name, counter_value from pmc_events"):
+        vals.setdefault(d, [n, {}])[1][cn] = vals.setdefault(d, [n, {}])[1].get(cn, 0.0) + v
+    agg, disp, before = {}, {}, []
+    for d, (n, cv) in vals.items():
+        if d not in after:
+            before.append((d, _kname(n), cv))
+            continue
+        k = _kname(n)
+        disp[k] = disp.get(k, 0) + 1
+        for cn, v in cv.items():
+            agg.setdefault(k, {})[cn] = agg.setdefault(k, {}).get(cn, 0.0) + v
+    return agg, disp, jobs, sorted(before)
+
+
+def summarize_counters(db, skip):
+    agg, disp, jobs, _ = _counters(db, skip)
+    names = sorted({cn for v in agg.values() for cn in v})
+    out = ["per job (%d jobs):" % jobs, "",
+           "| kernel | dispatches | " + " | ".join(names) + " |", "|---|---|" + "---|" * len(names)]
+    for k in sorted(agg, key=lambda k: -agg[k].get(names[0], 0.0)):
+        out.append("| %s | %d | " % (k, disp[k] // jobs) +
+                   " | ".join("%.4g" % (agg[k].get(cn, 0.0) / jobs) for cn in names) + " |")
+    return "\n".join(out)
+
+
+def summarize_traffic(db_fetch, db_write, skip, copy_mib, json_path):
+    """FETCH_SIZE / WRITE_SIZE (KiB per dispatch) scaled by what they read for
+    the calibration copy of copy_mib MiB (bench.py --calibrate-copy: the run's
+    first dispatch), as bytes per dispatch and per job."""
+    res = {}
+    factors = {}
+    for key, db, cn in (("read", db_fetch, "FETCH_SIZE"), ("write", db_write, "WRITE_SIZE")):
+        agg, disp, jobs, before = _counters(db, skip)
+        cal = before[0][2].get(cn, 0.0) if before and "copy" in before[0][1].lower() else 0.0
+        if cal <= 0:
+            raise SystemExit("profile summary: no calibration copy before the jobs in %s" % db)
+        factors[key] = copy_mib * 1024.0 / cal
+        for k, cv in agg.items():
+            r = res.setdefault(k, {"dispatches_per_job": disp[k] / jobs})
+            r[key + "_bytes_per_job"] = cv.get(cn, 0.0) * 1024.0 * factors[key] / jobs
+    kern = {}
+    out = ["scale factors: fetch %.3f, write %.3f (counter KiB per KiB of the %d MiB copy)"
+           % (factors["read"], factors["write"], copy_mib), "",
+           "| kernel | dispatches per job | read MB per dispatch | written MB per dispatch | total MB per job |",
+           "|---|---|---|---|---|"]
+    for k, r in sorted(res.items(), key=lambda x: -(x[1].get("read_bytes_per_job", 0) + x[1].get("write_bytes_per_job", 0))):
+        rd, wr, n = r.get("read_bytes_per_job", 0.0), r.get("write_bytes_per_job", 0.0), r["dispatches_per_job"]
+        kern[k] = {"dispatches_per_step": n, "read_bytes_per_dispatch": rd / n, "write_bytes_per_dispatch": wr / n,
+                   "bytes_per_step": rd + wr}
+        out.append("| %s | %.1f | %.1f | %.1f | %.1f |" % (k, n, rd / n / 1e6, wr / n / 1e6, (rd + wr) / 1e6))
+    if json_path:
+        with open(json_path, "w") as f:
+            json.dump({"fetch_factor": factors["read"], "write_factor": factors["write"], "kernels": kern}, f,
+                      indent=1)
+    return "\n".join(out)
+
+
+def summarize_main(argv):
+    ap = argparse.ArgumentParser(prog="bench.py --summarize")
+    ap.add_argument("kind", choices=["trace", "counters", "traffic"])
+    ap.add_argument("db", nargs="+", help="run_results.db (traffic: the FETCH_SIZE run's, then the WRITE_SIZE run's)")
+    ap.add_argument("--skip", type=int, default=1, help="k_gather dispatches before the first job (the autotuner's)")
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--json", default=None, help="traffic: also write bytes per kernel as JSON here")
+    a = ap.parse_args(argv)
+    if a.kind == "trace":
+        print(summarize_trace(a.db[0], a.skip))
+    elif a.kind == "counters":
+        print(summarize_counters(a.db[0], a.skip))
+    else:
+        print(summarize_traffic(a.db[0], a.db[1], a.skip, a.copy_mib, a.json))
 
 
 def main():
@@ -210,7 +335,7 @@ def main():
     ap.add_argument("--compare-rounds", type=int, default=5,
                     help="interleaved rounds of the learned job, the job without the network and the deferred form")
     ap.add_argument("--workflow4", action="store_true",
-                    help="run config 4's workflow (r=100 over 2000 sets, then r=1000 on the top 200) instead")
+                    help="run the 20000-gene x 1200-sample workflow (r=100 over 2000 sets, then r=1000 on the top 200) instead")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes instead of running the autotuner in setup")
     ap.add_argument("--launch", default="",
@@ -220,7 +345,7 @@ def main():
                          "scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU sharding is not built (see DESIGN.md)",
+        print("bench.py: this engine runs on one GPU; multi-GPU sharding is not built",
               file=sys.stderr)
         sys.exit(2)
     ensure_built()
@@ -262,8 +387,8 @@ def main():
         tw = time.perf_counter()
         run_step(eng, wl, prm, cfg)
         print("bench: warmup %d %.2fs" % (w, time.perf_counter() - tw), file=sys.stderr, flush=True)
-    split_ms = [0.0] * 6
-    split_n = [0] * 6
+    split_ms = [0.0] * 7
+    split_n = [0] * 7
     tests_eval = 0
     items = 0
     host_stream_ms = 0.0
@@ -275,7 +400,7 @@ def main():
     for _ in range(args.steps):
         res, rep, tm = run_step(eng, wl, prm, cfg)
         reps.append(rep)
-        for k in range(6):
+        for k in range(7):
             split_ms[k] += tm.split_ms[k]
             split_n[k] += tm.split_launches[k]
         tests_eval += tm.tests
@@ -383,8 +508,8 @@ def main():
     #                 bytes of bins, per distinct family and scoring class,
     #                 over k_score + k_bdeu (which score them) against LDS
     per_step = lambda x: x / args.steps
-    avg_s = [split_ms[k] / max(1, split_n[k]) / 1000.0 for k in range(6)]
-    lps = [split_n[k] / args.steps for k in range(6)]
+    avg_s = [split_ms[k] / max(1, split_n[k]) / 1000.0 for k in range(7)]
+    lps = [split_n[k] / args.steps for k in range(7)]
     roofs = {}
     if split_n[0]:
         a_cnt = per_step(ops * args.steps) / lps[0] / avg_s[0]
@@ -404,7 +529,7 @@ def main():
                          "peak": PEAK_LDS / 1e9, "unit": "GB/s", "frac": a_bd / PEAK_LDS,
                          "work_per_launch": per_step(stats[0] + stats[4]) / lps[3],
                          "avg_launch_ms": t_bd * 1e3, "launches_per_step": lps[3]}
-    dom = max(range(6), key=lambda k: split_ms[k])
+    dom = max(range(7), key=lambda k: split_ms[k])
     dname = eddy_amd.SPLIT[dom]
     if dname in roofs:
         roof = {"kernel": dname, **{k: roofs[dname][k] for k in ("bound", "achieved", "peak", "unit", "frac")}}
@@ -417,14 +542,19 @@ def main():
     roof["avg_launch_ms"] = avg_s[dom] * 1e3
     roof["launches_per_step"] = lps[dom]
     roof["traffic"] = None
-    # HBM bytes of the same kernel from the committed byte-counter profile of
-    # this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated): its bytes per
-    # step over this run's launches per step
+    # HBM bytes per launch from the committed byte-counter profile of this
+    # workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated): a kernel's bytes per
+    # step over this run's launches per step.  The loo kernel runs under the
+    # name of the instance the launch configuration picked.
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_learned_job_5000x600.json")
     if args.config == 3 and args.nsets is None and os.path.exists(tpath):
         kern = json.load(open(tpath))["kernels"]
-        if dname in kern:
-            roof["traffic"] = kern[dname]["bytes_per_step"] / max(1e-9, lps[dom])
+        prof_name = lambda n: "k_loo_w3" if n == "k_loo" and launch.get("loo_waves") == 3 else n
+        for k, n in enumerate(eddy_amd.SPLIT):
+            if n in roofs and prof_name(n) in kern:
+                roofs[n]["traffic"] = kern[prof_name(n)]["bytes_per_step"] / max(1e-9, lps[k])
+        if prof_name(dname) in kern:
+            roof["traffic"] = kern[prof_name(dname)]["bytes_per_step"] / max(1e-9, lps[dom])
             roof["traffic_source"] = os.path.relpath(tpath, ROOT)
 
     cpu = None
@@ -456,10 +586,9 @@ def main():
         "ci_tests_per_s_ref_equiv": ref_tests * args.steps / elapsed,
         "ci_tests_per_s_evaluated": tests_eval / elapsed,
         "items_per_step": items // args.steps,
-        "kernel_ms_per_step": {eddy_amd.SPLIT[k]: split_ms[k] / args.steps for k in range(6)},
+        "kernel_ms_per_step": {eddy_amd.SPLIT[k]: split_ms[k] / args.steps for k in range(7)},
         "kernel_ms_per_step_note": "HIP events between the kernels of each chunk's sequence on its slot's stream; "
-                                   "two chunks are in flight, so intervals overlap; k_count_test includes the "
-                                   "class-1 gather launched just before it",
+                                   "two chunks are in flight, so intervals overlap",
         "host_stream_ms_per_step": host_stream_ms / args.steps,
         "schedule": {"rows_past_plan": sched[0] // args.steps, "cols_tested_past_plan": sched[1] // args.steps,
                      "planned_entries": sched[2] // args.steps, "rows": sched[3] // args.steps},
@@ -480,4 +609,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize":
+        summarize_main(sys.argv[2:])
+    else:
+        main()

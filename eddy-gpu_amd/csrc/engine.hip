@@ -117,8 +117,8 @@ struct Slot {
   hipStream_t st = nullptr;
   // timing events: 0 start, 1 after count, 2 after loo, 3 after rows,
   // 5 after score, 6 after BDeu, 4 after the tail
-  hipEvent_t tev[7] = {};
-  float kms[6] = {};  // last launch: count, loo, rows, score, BDeu, tail (ms)
+  hipEvent_t tev[8] = {};
+  float kms[7] = {};  // last launch: count, loo, rows, score, BDeu, tail, class-1 gather (ms)
   SlotGraph gr[64];
   int *h_int = nullptr;  // pinned: status, events, families, unique networks (4 x h_cap)
   long long *h_tests = nullptr;
@@ -126,7 +126,7 @@ struct Slot {
   Slice *h_slices = nullptr;  // pinned staging of the capture slice list
   size_t h_slices_cap = 0;
   bool busy = false;
-  bool plain_tests = false, has_tiles = false;
+  bool plain_tests = false, has_tiles = false, gather1 = false;
   Chunk ch;             // the chunk in flight
   std::vector<int> cap;  // its event capacities
   std::vector<int> horder;
@@ -776,7 +776,7 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   a.fbwp = fbwp;
   a.tile_order = ctx->lc.tile_order;
 
-  for (int k = 0; k < 7; k++)
+  for (int k = 0; k < 8; k++)
     if (!sl.tev[k]) HIPCHK(ctx, hipEventCreate(&sl.tev[k]));
   // Where the S-sized tables live.  Up to a few thousand samples they fit in
   // LDS (the rows kernel's deltas and class lists within 64 KB, so it keeps
@@ -845,6 +845,7 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   seq[ns++] = {0, nullptr, 0, 0};
   if (gather1)
     seq[ns++] = {-1, (const void *)k_gather_class1, (unsigned)nitems, (size_t)4 * ((N + 63) / 64 + 1) + 2 * (size_t)kp};
+  seq[ns++] = {7, nullptr, 0, 0};
   if (!ch.tiles.empty())
     seq[ns++] = {-1, plain_tests ? (const void *)k_count_test_all : (const void *)k_count_test,
                  (unsigned)ch.tiles.size(), 0};
@@ -927,6 +928,7 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   sl.busy = true;
   sl.plain_tests = plain_tests;
   sl.has_tiles = !ch.tiles.empty();
+  sl.gather1 = gather1;
   ctx->hp.launch += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hl0).count();
   return EDDY_OK;
 }
@@ -948,14 +950,15 @@ int wait_chunk(eddy_ctx *ctx, Slot &sl, ChunkOut &out) {
     float ms = 0.0f;
     return hipEventElapsedTime(&ms, sl.tev[x], sl.tev[y]) == hipSuccess ? ms : 0.0f;
   };
-  sl.kms[0] = sl.has_tiles ? span(0, 1) : 0.0f;
+  sl.kms[0] = sl.has_tiles ? span(7, 1) : 0.0f;
+  sl.kms[6] = span(0, 7);
   sl.kms[1] = sl.plain_tests ? 0.0f : span(1, 2);
   sl.kms[2] = span(2, 3);
   sl.kms[3] = span(3, 5);
   sl.kms[4] = span(5, 6);
   sl.kms[5] = span(6, 4);
-  for (int k = 0; k < 6; k++) {
-    if ((k == 1 && sl.plain_tests) || (k == 0 && !sl.has_tiles)) continue;
+  for (int k = 0; k < 7; k++) {
+    if ((k == 1 && sl.plain_tests) || (k == 0 && !sl.has_tiles) || (k == 6 && !sl.gather1)) continue;
     ctx->timing.split_ms[k] += sl.kms[k];
     ctx->timing.split_launches[k] += 1;
   }
@@ -963,7 +966,7 @@ int wait_chunk(eddy_ctx *ctx, Slot &sl, ChunkOut &out) {
     if (k == 1 && sl.plain_tests) continue;
     if (k == 0 && !sl.has_tiles) continue;
     // index 3: the score kernels together (score, BDeu, tail)
-    ctx->timing.kernel_ms[k] += k < 3 ? sl.kms[k] : sl.kms[3] + sl.kms[4] + sl.kms[5];
+    ctx->timing.kernel_ms[k] += k == 0 ? sl.kms[0] + sl.kms[6] : k < 3 ? sl.kms[k] : sl.kms[3] + sl.kms[4] + sl.kms[5];
     ctx->timing.launches[k] += 1;
   }
   if (getenv("EDDY_DEBUG")) {

@@ -5,7 +5,7 @@
 //   <dir>/class.txt   one line of labels (classA x nA, classB x nB)
 //   <dir>/sets.gmt    "SET_0001\thttp://synthetic/SET_0001\tG00012\t..."
 //   <dir>/PRIORS/SET_0001.prior   "G00012\tinteracts-with\tG00345"
-// usage: eddy_synth <config 0..4> <dir> [nsets]
+// usage: eddy_synth <workload 0..4> <dir> [nsets]
 #include <sys/stat.h>
 
 #include <cstdio>
@@ -17,7 +17,7 @@
 
 int main(int argc, char **argv) {
   if (argc < 3) {
-    fprintf(stderr, "usage: eddy_synth <config 0..4> <dir> [nsets]\n");
+    fprintf(stderr, "usage: eddy_synth <workload 0..4> <dir> [nsets]\n");
     return 2;
   }
   eddy_synth_cfg c;

@@ -61,15 +61,15 @@ class PermDetail(C.Structure):
 
 KERNELS = ["count_test", "loo", "rows", "score", "pvalue", "gather"]
 # the batched launch sequence's kernels one by one (eddy_timing.split_ms)
-SPLIT = ["k_count_test", "k_loo", "k_rows", "k_score", "k_bdeu", "k_score_tail"]
+SPLIT = ["k_count_test", "k_loo", "k_rows", "k_score", "k_bdeu", "k_score_tail", "k_gather_class1"]
 
 
 class Timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("kernel_ms", C.c_double * 6), ("launches", C.c_longlong * 6),
                 ("items", C.c_longlong), ("pairs", C.c_longlong), ("tests", C.c_longlong),
                 ("host_stream_ms", C.c_double), ("sched", C.c_longlong * 4), ("stats", C.c_longlong * 6),
-                ("phase1_ms", C.c_double), ("phase_ms", C.c_double * 16), ("split_ms", C.c_double * 6),
-                ("split_launches", C.c_longlong * 6)]
+                ("phase1_ms", C.c_double), ("phase_ms", C.c_double * 16), ("split_ms", C.c_double * 7),
+                ("split_launches", C.c_longlong * 7)]
 
 
 class LaunchCfg(C.Structure):

@@ -231,11 +231,11 @@ typedef struct {
                                         loo 10 plan, 11 packed tests, 12 cutoff
                                         check and fallback; count 13 staging
                                         and MFMA, 14 tests and stores */
-  double split_ms[6];                /* summed device time per kernel of the
-                                        batched sequence: 0 count (with the
-                                        class-1 gather before it), 1 loo,
-                                        2 rows, 3 score, 4 BDeu, 5 tail */
-  long long split_launches[6];
+  double split_ms[7];                /* summed device time per kernel of the
+                                        batched sequence: 0 count, 1 loo,
+                                        2 rows, 3 score, 4 BDeu, 5 tail,
+                                        6 the class-1 gather before count */
+  long long split_launches[7];
 } eddy_timing;
 int eddy_get_timing(eddy_ctx *ctx, eddy_timing *t);
 
@@ -268,7 +268,8 @@ typedef struct {
   double ms_best;      /* the same chunk with the chosen configuration */
   long long items, pairs; /* the calibration chunk */
   double kernel_ms[6];  /* the chosen configuration on the calibration chunk (min of
-                           3 serial runs): count, loo, rows, score, BDeu, tail */
+                           3 serial runs): count (without the class-1 gather), loo,
+                           rows, score, BDeu, tail */
 } eddy_tune_report;
 /* Picks the launch configuration for this dataset and these sets: one
  * calibration chunk (sets sampled across the list, permutations drawn from a

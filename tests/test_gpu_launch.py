@@ -80,7 +80,7 @@ def test_batched_results_do_not_depend_on_launch_shapes(engine, shapes):
     prm = wl.params(perms=12, mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF)
     probe = (0, 17, 39)
     engine.set_launch_cfg(**dict(shapes, graph=0))
-    print("case: batched, 40 sets of config 3 at r=12, one-by-one launches", flush=True)
+    print("case: batched, 40 sets of the 5000 x 600 workload at r=12, one-by-one launches", flush=True)
     ref = _run(engine, wl, prm, probe)
     for sh in SHAPES:
         cfg = dict(shapes, graph=0)
@@ -118,7 +118,7 @@ def test_learned_job_and_plain_mode_under_graphs(engine, shapes):
 
 
 def _wide_workload():
-    """Config 0's genes and sets on a 550/550-sample dataset (9 words per class mask)."""
+    """The 200 x 200 workload's genes and set on a 550/550-sample dataset (9 words per class mask)."""
     wl = eddy_amd.Workload(0)
     L = eddy_amd.lib()
     nA = nB = 550
@@ -142,7 +142,7 @@ def test_bdeu_lane_groups_match_oracle_on_wide_classes(engine, shapes):
     lab[rng.permutation(wl.N)[:550]] = 0
     data_ref = wl.X.astype(np.int32) - 1
     t1, t2, n1, n2 = oracle_ref.class_split(data_ref, genes, lab)
-    print("case: oracle, set 0 of config 0 on 550/550 samples", flush=True)
+    print("case: oracle, the 40-gene set on 550/550 samples", flush=True)
     o = oracle_ref.run_perm(t1, t2, n1, n2, wl.prior_matrix(0), oracle_ref.params(mp=prm.mp), want_bits=False)
     for groups in (0, 1, 2, 4):
         engine.set_launch_cfg(**dict(shapes, bdeu_groups=groups))
@@ -161,7 +161,7 @@ def test_autotune_picks_a_valid_shape_and_caches_it(engine, shapes, tmp_path):
     engine.reset_stream()
     ref = _run(engine, wl, prm, (0, 59))
     cache = str(tmp_path / "tune.txt")
-    print("case: autotune, 60 sets of config 3", flush=True)
+    print("case: autotune, 60 sets of the 5000 x 600 workload", flush=True)
     c1, r1 = engine.autotune(wl.sets, prm, cache=cache)
     assert r1["from_cache"] == 0 and r1["trials"] >= 10 and r1["pairs"] > 0
     assert r1["ms_best"] <= r1["ms_default"] * 1.15

@@ -149,7 +149,7 @@ def test_learned_windows_are_exact_and_save_tests(engine):
     wl = eddy_amd.Workload(3, nsets=48)
     engine.upload(wl.X, wl.classes)
     prm = wl.params(perms=40, mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=0)
-    print("case: learned windows, 48 sets of config 3 at r=40", flush=True)
+    print("case: learned windows, 48 sets of the 5000 x 600 workload at r=40", flush=True)
     engine.reset_stream()
     Xtr, Ytr = engine.learn_collect(wl.sets[:16], prm)
     Xte, Yte = engine.learn_collect(wl.sets[16:], prm)
@@ -181,7 +181,7 @@ def test_learned_job_is_exact(engine):
     wl = eddy_amd.Workload(3, nsets=48)
     engine.upload(wl.X, wl.classes)
     prm = wl.params(perms=40, mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=0)
-    print("case: learned job, 48 sets of config 3 at r=40", flush=True)
+    print("case: learned job, 48 sets of the 5000 x 600 workload at r=40", flush=True)
     engine.reset_stream()
     base = engine.run_sets(wl.sets, prm)
     base_js = [engine.js(s).copy() for s in range(len(wl.sets))]

@@ -109,7 +109,7 @@ def test_sets_stream_parity(engine):
 
 
 def test_perm_parity_config1_cutoff(engine):
-    """Config-1 shape (150 genes, 400 samples): networks, BDeu, likelihoods, JS."""
+    """The 1000 x 400 workload's shape (a 150-gene set, 400 samples): networks, BDeu, likelihoods, JS."""
     wl = eddy_amd.Workload(1)
     engine.upload(wl.X, wl.classes)
     genes, pairs = wl.sets[0]
@@ -133,7 +133,7 @@ def test_perm_parity_config1_cutoff(engine):
 def test_perm_parity_large_sample_counts(engine, nA, nB):
     """Near the engine's sample limit (2046): 16 sample chunks in the count
     kernel, 16-27 mask words per class, the tail kernel's largest LDS use;
-    config 0's set of 40 genes on a seeded dataset of nA + nB samples."""
+    the 40-gene set of the 200 x 200 workload on a seeded dataset of nA + nB samples."""
     wl = eddy_amd.Workload(0)
     L = eddy_amd.lib()
     X = np.zeros((wl.cfg.genes, nA + nB), np.uint8)
