@@ -165,6 +165,26 @@ def workflow4(args):
     chosen, res2 = eng.repermute(wl.sets, res, prm, top=200, perms=1000)
     t2 = time.perf_counter()
     print("bench: r=100 phase %.2fs, r=1000 phase %.2fs" % (t1 - t0, t2 - t1), file=sys.stderr, flush=True)
+    # the r = 100 phase without the network (same stream, same results), the
+    # two forms interleaved over a few rounds (medians)
+    uprm = wl.params(mode=eddy_amd.MODE_BATCHED, schedule=eddy_amd.TESTS_CUTOFF, learned=0)
+    tl, tu, same = [t1 - t0], [], True
+    k = min(100, len(wl.sets))
+    tests_l = rep.tests_apply
+    for r in range(args.compare_rounds4):
+        eng.reset_stream()
+        ta = time.perf_counter()
+        ures = eng.run_sets(wl.sets, uprm)
+        tu.append(time.perf_counter() - ta)
+        same = same and all((a.status, a.perms_run, a.U0) == (b.status, b.perms_run, b.U0) and
+                            (a.status != eddy_amd.SET_OK or a.pDDN == b.pDDN) for a, b in zip(ures, res))
+        if r + 1 < args.compare_rounds4:
+            eng.reset_stream()
+            ta = time.perf_counter()
+            eng.run_learned(wl.sets, prm, cfg)
+            tl.append(time.perf_counter() - ta)
+    med = lambda v: sorted(v)[len(v) // 2]
+    tests_u = sum(r.tests_eval for r in ures[k:])
     G = [len(wl.sets[q][0]) for q in chosen]
     line = {
         "metric": "20000 x 1200 workflow: gene sets/s at r=100 (batched+learned), then r=1000 on the top 200",
@@ -179,6 +199,10 @@ def workflow4(args):
         "failed_sets_phase1": int(sum(1 for r in res if r.status != eddy_amd.SET_OK)),
         "learned": {"collect_ms": rep.collect_ms, "train_gpu_ms": rep.train_ms, "first_windowed_set":
                     rep.first_windowed_set, "loss_final": rep.loss_final},
+        "vs_unlearned": {"rounds": len(tu), "learned_ms_median": med(tl) * 1e3, "unlearned_ms_median": med(tu) * 1e3,
+                         "learned_speedup_vs_unlearned": med(tu) / med(tl),
+                         "tests_applied_sets_learned": tests_l, "tests_applied_sets_unlearned": tests_u,
+                         "results_identical": same},
     }
     print(json.dumps(line))
 
@@ -334,6 +358,8 @@ def main():
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--compare-rounds", type=int, default=5,
                     help="interleaved rounds of the learned job, the job without the network and the deferred form")
+    ap.add_argument("--compare-rounds4", type=int, default=3,
+                    help="rounds of the 20000 x 1200 workflow's learned / unlearned comparison")
     ap.add_argument("--workflow4", action="store_true",
                     help="run the 20000-gene x 1200-sample workflow (r=100 over 2000 sets, then r=1000 on the top 200) instead")
     ap.add_argument("--no-autotune", action="store_true",
