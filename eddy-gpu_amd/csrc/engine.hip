@@ -1565,8 +1565,9 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
   CHK(p0_begin(ctx, meta, p));
   GROW(b_js, sizeof(double) * (size_t)std::max(1, nsets) * perms);
   HIPCHK(ctx, hipMemsetAsync(ctx->b_js.p, 0, sizeof(double) * (size_t)std::max(1, nsets) * perms, ctx->stream));
-  GROW(b_counters, sizeof(unsigned long long) * 32);
-  HIPCHK(ctx, hipMemsetAsync(ctx->b_counters.p, 0, sizeof(unsigned long long) * 32, ctx->stream));
+  GROW(b_counters, sizeof(unsigned long long) * EDDY_NCNT * EDDY_CNT_COPIES);
+  HIPCHK(ctx, hipMemsetAsync(ctx->b_counters.p, 0, sizeof(unsigned long long) * EDDY_NCNT * EDDY_CNT_COPIES,
+                             ctx->stream));
   ctx->last_perms = perms;
   ctx->last_G.assign(nsets, 0);
   for (int s = 0; s < nsets; s++) ctx->last_G[s] = meta[s].G;
@@ -2031,8 +2032,12 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     if (nan0[q]) ctx->last_js[q][0] = -999.0;
     if (q >= collect_upto && q >= win_from) r.tests_apply += tests_eval[q];
   }
-  unsigned long long cnt[32];
-  HIPCHK(ctx, hipMemcpy(cnt, ctx->b_counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
+  std::vector<unsigned long long> cntc((size_t)EDDY_NCNT * EDDY_CNT_COPIES);
+  HIPCHK(ctx, hipMemcpy(cntc.data(), ctx->b_counters.p, sizeof(unsigned long long) * cntc.size(),
+                        hipMemcpyDeviceToHost));
+  unsigned long long cnt[EDDY_NCNT] = {};
+  for (int c = 0; c < EDDY_CNT_COPIES; c++)
+    for (int k = 0; k < EDDY_NCNT; k++) cnt[k] += cntc[(size_t)c * EDDY_NCNT + k];
   for (int k = 0; k < 4; k++) ctx->timing.sched[k] = (long long)cnt[k];
   for (int k = 0; k < 6; k++) ctx->timing.stats[k] = (long long)cnt[4 + k];
   int wcr = 0;
@@ -2285,8 +2290,9 @@ extern "C" int eddy_autotune(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets
   rep.items = (long long)master.items.size();
   rep.pairs = master.pairs;
   GROW(b_js, sizeof(double) * (size_t)nsets * (P + 1));
-  GROW(b_counters, sizeof(unsigned long long) * 32);
-  HIPCHK(ctx, hipMemsetAsync(ctx->b_counters.p, 0, sizeof(unsigned long long) * 32, ctx->stream));
+  GROW(b_counters, sizeof(unsigned long long) * EDDY_NCNT * EDDY_CNT_COPIES);
+  HIPCHK(ctx, hipMemsetAsync(ctx->b_counters.p, 0, sizeof(unsigned long long) * EDDY_NCNT * EDDY_CNT_COPIES,
+                             ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   // the calibration leaves the caller's run state as it was
   const eddy_timing keep_timing = ctx->timing;
@@ -2398,8 +2404,9 @@ static int perm_detail_run(eddy_ctx *ctx, const eddy_set_desc *set, const uint8_
   CHK(p0_begin(ctx, meta, p));
   GROW(b_js, sizeof(double) * p->perms);
   HIPCHK(ctx, hipMemsetAsync(ctx->b_js.p, 0, sizeof(double) * p->perms, ctx->stream));
-  GROW(b_counters, sizeof(unsigned long long) * 32);
-  HIPCHK(ctx, hipMemsetAsync(ctx->b_counters.p, 0, sizeof(unsigned long long) * 32, ctx->stream));
+  GROW(b_counters, sizeof(unsigned long long) * EDDY_NCNT * EDDY_CNT_COPIES);
+  HIPCHK(ctx, hipMemsetAsync(ctx->b_counters.p, 0, sizeof(unsigned long long) * EDDY_NCNT * EDDY_CNT_COPIES,
+                             ctx->stream));
   ctx->last_G.assign(1, meta[0].G);
   ctx->last_perms = p->perms;
   const int G = meta[0].G;

@@ -73,6 +73,14 @@ struct Slice {
 #define EDDY_TAB_DENSE 8
 #define EDDY_TAB_FULL 12
 
+// statistics counters (schedule, score statistics, per-phase workgroup
+// time): EDDY_CNT_COPIES copies of EDDY_NCNT counters, one 256-byte line per
+// copy, each workgroup adding into the copy of its index modulo the count
+// (same-address atomics from every workgroup of a launch would serialize);
+// the host sums the copies
+#define EDDY_NCNT 32
+#define EDDY_CNT_COPIES 64
+
 struct KArgs {
   // dataset-level
   int N, n1, n2, S;
@@ -131,7 +139,7 @@ struct KArgs {
   unsigned *labhits;       // [pairbase + p] full-test edges over permutations > 0 (training labels), or null
   int label_perms;         // > 0: permutations 1..label_perms run every full test (and log it) despite win
   const int *win;          // [genebase + t] last column whose full test the count kernel runs, or null
-  unsigned long long *counters;  // schedule statistics (optional)
+  unsigned long long *counters;  // statistics (optional): EDDY_CNT_COPIES x EDDY_NCNT
   uint64_t *bits;          // [((goff*2 + g*G + gene)*3 + v)*bitw + w]: class-g samples (class-list order) in state v
   int bitw;                // words per mask: ceil(max(n1, n2) / 64)
   int rows_maxg;           // largest set of the launch (row kernel LDS layout)

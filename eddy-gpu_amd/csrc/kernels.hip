@@ -131,12 +131,17 @@ __device__ __forceinline__ int prior_bit(const KArgs &a, const SetMeta &sm, long
   return (a.prior[sm.poff + (p >> 5)] >> (p & 31)) & 1;
 }
 
+// adds v to statistics counter k (this workgroup's copy)
+__device__ __forceinline__ void cnt_add(const KArgs &a, int k, unsigned long long v) {
+  atomicAdd(&a.counters[(blockIdx.x & (EDDY_CNT_COPIES - 1)) * EDDY_NCNT + k], v);
+}
+
 // per-phase workgroup time (constant-rate wall clock), summed over items into
-// counters[16 + phase] when the engine collects statistics
+// counter 16 + phase when the engine collects statistics
 __device__ __forceinline__ void phase_mark(const KArgs &a, int phase, unsigned long long &t0) {
   if (!a.counters || threadIdx.x != 0) return;
   unsigned long long t = wall_clock64();
-  if (phase >= 0) atomicAdd(&a.counters[16 + phase], t - t0);
+  if (phase >= 0) cnt_add(a, 16 + phase, t - t0);
   t0 = t;
 }
 
@@ -864,10 +869,10 @@ __device__ __forceinline__ void loo_body(const KArgs &a) {
   if (tid == 0) {
     atomicAdd((unsigned long long *)&a.tests[item], (unsigned long long)tcount);
     if (a.counters) {
-      atomicAdd((unsigned long long *)&a.counters[0], (unsigned long long)nfb);      // rows needing fallback
-      atomicAdd((unsigned long long *)&a.counters[1], (unsigned long long)nfbcols);  // columns tested in fallback
-      atomicAdd((unsigned long long *)&a.counters[2], (unsigned long long)total);    // planned entries
-      atomicAdd((unsigned long long *)&a.counters[3], (unsigned long long)R);        // rows
+      cnt_add(a, 0, (unsigned long long)nfb);      // rows needing fallback
+      cnt_add(a, 1, (unsigned long long)nfbcols);  // columns tested in fallback
+      cnt_add(a, 2, (unsigned long long)total);    // planned entries
+      cnt_add(a, 3, (unsigned long long)R);        // rows
     }
   }
 }
@@ -1687,9 +1692,9 @@ __device__ __forceinline__ void score_body(const KArgs &a) {
   double *fscore = a.famscore + im.foff * 2;
   const int NF = F;
   if (a.counters && tid == 0) {
-    atomicAdd(&a.counters[5], (unsigned long long)NF);      // families
-    atomicAdd(&a.counters[6], (unsigned long long)UU);      // unique networks
-    atomicAdd(&a.counters[7], (unsigned long long)E);       // events
+    cnt_add(a, 5, (unsigned long long)NF);      // families
+    cnt_add(a, 6, (unsigned long long)UU);      // unique networks
+    cnt_add(a, 7, (unsigned long long)E);       // events
   }
   // ---- 4. BDeu per (family, scoring class): kernel5.cu:4-236 -------------
   // each wave scores its own (family, class) pairs: the family's state masks
@@ -1780,8 +1785,8 @@ __device__ __forceinline__ void score_body(const KArgs &a) {
     if (st_bins) atomicAdd(&sh_st[1], st_bins);
     __syncthreads();
     if (tid == 0) {
-      atomicAdd(&a.counters[4], sh_st[0]);  // histogram state bytes read (reference algorithm)
-      atomicAdd(&a.counters[8], sh_st[1]);  // histogram bin bytes
+      cnt_add(a, 4, sh_st[0]);  // histogram state bytes read (reference algorithm)
+      cnt_add(a, 8, sh_st[1]);  // histogram bin bytes
     }
   }
   // what the tail kernel needs besides the family scores
@@ -1963,8 +1968,8 @@ __global__ __launch_bounds__(256) void k_bdeu(KArgs a) {
     if (st_bins) atomicAdd(&sh_st[1], st_bins);
     __syncthreads();
     if (tid == 0) {
-      atomicAdd(&a.counters[4], sh_st[0]);
-      atomicAdd(&a.counters[8], sh_st[1]);
+      cnt_add(a, 4, sh_st[0]);
+      cnt_add(a, 8, sh_st[1]);
     }
   }
 }
