@@ -1008,6 +1008,10 @@ __device__ __forceinline__ void rows_body(const KArgs &a) {
   uint8_t *sflb = (uint8_t *)lp;                                           // [2][GMX]
   uint16_t *cls0 = clsb, *cls1 = clsb + a.n1;
   uint32_t *cand_w = candb + wave * GMX;
+  // the item's events are numbered in LDS (this workgroup is the item's only
+  // writer) and the count stored once at the end
+  __shared__ int sh_evn;
+  if (tid == 0) sh_evn = 0;
   uint16_t *lrow = lrowb + wave * MPW;
   __shared__ int ncls[2];
   __shared__ unsigned long long hfull[2];
@@ -1177,7 +1181,7 @@ __device__ __forceinline__ void rows_body(const KArgs &a) {
         bool same = cnt == nL;
         for (int i = 0; i < cnt && same; i++) same = nl[i] == lrow[i];
         if (same) continue;
-        int e = atomicAdd(&a.evcount[item], 1);
+        int e = atomicAdd(&sh_evn, 1);
         if (e < im.evcap) {
           a.evkey[im.evoff + e] = (uint32_t)s | ((uint32_t)t << 16);
           uint16_t *pp = a.evpar + (size_t)(im.evoff + e) * a.evw;
@@ -1207,7 +1211,7 @@ __device__ __forceinline__ void rows_body(const KArgs &a) {
         }
         if (same && cnt == nL) continue;
         // pass 2: the event record and the row's hash
-        int e = atomicAdd(&a.evcount[item], 1);
+        int e = atomicAdd(&sh_evn, 1);
         uint16_t *pp = nullptr;
         if (e < im.evcap) {
           a.evkey[im.evoff + e] = (uint32_t)s | ((uint32_t)t << 16);
@@ -1232,6 +1236,7 @@ __device__ __forceinline__ void rows_body(const KArgs &a) {
     wave_sync();
   }
   __syncthreads();
+  if (tid == 0) a.evcount[item] = sh_evn;
   for (int s = tid; s < S; s += 256) {
     int g = (s <= a.n1) ? 0 : 1;
     if (GM) a.hnet[im.soff + s] ^= hfull[g];  // delta[s] is that word
