@@ -317,11 +317,28 @@ EDDY_HD void eddy_loo_column(const int *t, int b, const double *iK, const double
   }
 }
 
-// Leave-one-out decisions of one tally: bit `cell` is the decision of t minus
-// one count in `cell`; empty cells copy `full`.  Six divisions per tally serve
-// all nine leave-one-out tallies.  A reduced tally under Yates' rule (rare:
-// an expected count below 5) or within a guard band of a threshold takes the
-// exact decision after the unrolled pass.
+// The margin conditions of the all-at-once argument (below): every nonzero margin >= 2,
+// (Rmin - 1)(Kmin - 1) >= 5 (n - 1) and dof 1, 2 or 4; then *e = an upper
+// bound of 1/(Rmin - 1) + 1/(Kmin - 1) (one division of exact integers,
+// correctly rounded; the 1 + 1e-12 factor below covers its rounding) and *dof
+// are set.  1 if they hold.
+EDDY_HD int eddy_loo_all_margins(const int *R, const int *K, int n, double *e, int *dof) {
+  int rmin = 1 << 30, kmin = 1 << 30, zr = 0, zk = 0;
+  for (int i = 0; i < 3; i++) {
+    if (R[i] == 0) zr++;
+    else if (R[i] < rmin) rmin = R[i];
+    if (K[i] == 0) zk++;
+    else if (K[i] < kmin) kmin = K[i];
+  }
+  if (n < 2 || rmin < 2 || kmin < 2) return 0;
+  const long long pm = (long long)(rmin - 1) * (long long)(kmin - 1);
+  if (pm < 5LL * (n - 1)) return 0;
+  *dof = (2 - zk) * (2 - zr);
+  if (!(*dof == 1 || *dof == 2 || *dof == 4)) return 0;
+  *e = (double)(rmin + kmin - 2) / (double)pm;
+  return 1;
+}
+
 // Every leave-one-out decision of a tally at once, when its statistic is far
 // enough from the thresholds: 1 all nine are "edge", 0 all are "no edge", -1
 // unknown.  With every nonzero margin >= 2, removing one sample from cell
@@ -337,32 +354,14 @@ EDDY_HD void eddy_loo_column(const int *t, int b, const double *iK, const double
 // and each interval end gets the same guard, so an answer here is what the
 // exact decision of each leave-one-out tally gives.
 EDDY_HD int eddy_loo_all_certain(const int *R, const int *K, int n, const double *iR, const double *iK,
-                                 const double *iRm, const double *iKm, const int *t, int prior,
-                                 const eddy_edge_params *ep, const eddy_edge_bounds *eb) {
-  if (!ep->shortcut || n < 2) return -1;
-  int rmin = 1 << 30, kmin = 1 << 30, zr = 0, zk = 0;
-  double irm = 0.0, ikm = 0.0;  // 1/(rmin - 1), 1/(kmin - 1), within 1 ulp
-  for (int i = 0; i < 3; i++) {
-    if (R[i] == 0) zr++;
-    else if (R[i] < rmin) {
-      rmin = R[i];
-      irm = iRm[i];
-    }
-    if (K[i] == 0) zk++;
-    else if (K[i] < kmin) {
-      kmin = K[i];
-      ikm = iKm[i];
-    }
-  }
-  if (rmin < 2 || kmin < 2) return -1;
-  if ((long long)(rmin - 1) * (long long)(kmin - 1) < 5LL * (n - 1)) return -1;
-  const int dof = (2 - zk) * (2 - zr);
-  if (!(dof == 1 || dof == 2 || dof == 4)) return -1;
+                                 const int *t, int prior, const eddy_edge_params *ep, const eddy_edge_bounds *eb) {
+  double e;
+  int dof;
+  if (!ep->shortcut || !eddy_loo_all_margins(R, K, n, &e, &dof)) return -1;
   double S = 0.0;
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) S = S + eddy_sq(t[i * 3 + j]) * iR[i] * iK[j];
   const double x = 0.5 * (double)n * (S - 1.0);
-  const double e = irm + ikm;
   const double d = 0.5 * (double)(n - 1) * e * (1.0 + 1e-12);
   const double xg = x + 1e-9 * (1.0 + (x > 0.0 ? x : 0.0));  // >= the exact x
   const double up = xg + d;                                   // >= every leave-one-out x
@@ -376,19 +375,42 @@ EDDY_HD int eddy_loo_all_certain(const int *R, const int *K, int n, const double
   return -1;
 }
 
-EDDY_HD uint32_t eddy_loo_mask_fast(const int *t, int full, int prior, const eddy_edge_params *ep,
-                                    const eddy_edge_bounds *eb, double cpairs, int *ntests) {
+// The leave-one-out mask that a certain common decision `all` gives:
+// nonzero cells take `all`, empty cells copy `full`.
+EDDY_HD uint32_t eddy_loo_all_mask(const int *t, int all, int full) {
+  uint32_t nz = 0;
+  for (int c = 0; c < 9; c++) nz |= (t[c] > 0 ? 1u : 0u) << c;
+  return (all ? nz : 0u) | (full ? (~nz & 0x1FFu) : 0u);
+}
+
+// The leave-one-out mask of t when the all-at-once argument decides it (two
+// divisions: the margins' reciprocals): *m and 1; else 0.
+EDDY_HD int eddy_loo_mask_try(const int *t, int full, int prior, const eddy_edge_params *ep,
+                              const eddy_edge_bounds *eb, uint32_t *m) {
+  int R[3], K[3];
+  for (int i = 0; i < 3; i++) {
+    R[i] = t[i * 3] + t[i * 3 + 1] + t[i * 3 + 2];
+    K[i] = t[i] + t[3 + i] + t[6 + i];
+  }
+  double iR[3], iK[3];
+  eddy_recip3(R, iR, 0);
+  eddy_recip3(K, iK, 0);
+  const int all = eddy_loo_all_certain(R, K, R[0] + R[1] + R[2], iR, iK, t, prior, ep, eb);
+  if (all < 0) return 0;
+  *m = eddy_loo_all_mask(t, all, full);
+  return 1;
+}
+
+// Leave-one-out decisions of one tally, cell by cell: bit `cell` is the
+// decision of t minus one count in `cell`; empty cells copy `full`.  Four
+// divisions serve all nine leave-one-out tallies.  A reduced tally under
+// Yates' rule (rare: an expected count below 5) or within a guard band of a
+// threshold takes the exact decision after the unrolled pass.
+EDDY_HD uint32_t eddy_loo_mask_cells(const int *t, int full, int prior, const eddy_edge_params *ep,
+                                     const eddy_edge_bounds *eb, double cpairs, int *ntests) {
   int R[3], K[3];
   double iR[3], iK[3], iRm[3], iKm[3];
   const int n = eddy_loo_margins(t, R, K, iR, iRm, iK, iKm);
-  {
-    const int all = eddy_loo_all_certain(R, K, n, iR, iK, iRm, iKm, t, prior, ep, eb);
-    if (all >= 0) {  // nonzero cells take `all`, empty cells copy `full`
-      uint32_t nz = 0;
-      for (int c = 0; c < 9; c++) nz |= (t[c] > 0 ? 1u : 0u) << c;
-      return (all ? nz : 0u) | (full ? (~nz & 0x1FFu) : 0u);
-    }
-  }
   uint32_t m = 0, unsure = 0;
   // unrolled: every array index below is a constant, so all tallies stay in
   // registers
@@ -434,4 +456,13 @@ EDDY_HD uint32_t eddy_loo_mask_fast(const int *t, int full, int prior, const edd
     m |= (uint32_t)eddy_edge_decision(u, prior, ep, eb, cpairs) << cell;
   }
   return m;
+}
+
+// Leave-one-out decisions of one tally: all at once when the argument above
+// decides them, else cell by cell.
+EDDY_HD uint32_t eddy_loo_mask_fast(const int *t, int full, int prior, const eddy_edge_params *ep,
+                                    const eddy_edge_bounds *eb, double cpairs, int *ntests) {
+  uint32_t m;
+  if (eddy_loo_mask_try(t, full, prior, ep, eb, &m)) return m;
+  return eddy_loo_mask_cells(t, full, prior, ep, eb, cpairs, ntests);
 }

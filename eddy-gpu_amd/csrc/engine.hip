@@ -2593,7 +2593,7 @@ extern "C" int eddy_loo_all_certain_host(const int *tally9, int c, int prior, co
   int R[3], K[3];
   double iR[3], iK[3], iRm[3], iKm[3];
   const int n = eddy_loo_margins(tally9, R, K, iR, iRm, iK, iKm);
-  return eddy_loo_all_certain(R, K, n, iR, iK, iRm, iKm, tally9, prior, &ep, &eb);
+  return eddy_loo_all_certain(R, K, n, iR, iK, tally9, prior, &ep, &eb);
 }
 
 extern "C" int eddy_edge_loo_mask_host(const int *tally9, int c, int prior, int full, const eddy_params *p) {

@@ -790,28 +790,70 @@ __device__ __forceinline__ void loo_body(const KArgs &a) {
     x.pw = a.prior[sm.poff + (p >> 5)];
     class_tally_words(a, im, sm, p, g, x.q0, x.q1, x.q2);
   };
+  // Most entries are decided all at once (eddy_loo_mask_try); the others
+  // need the per-cell tests, a long path that one such lane would make its
+  // whole wave take.  They go to the wave's queue instead (row entry lo,
+  // column, full decision), which the wave drains 64 at a time, every lane
+  // on one entry.
+  __shared__ uint32_t hq[EDDY_NWAVES][128];
+  uint32_t *q = hq[wave];
+  const uint64_t lt = lane == 0 ? 0ULL : (~0ULL >> (64 - lane));
+  int hn = 0;  // entries in the queue (the same in every lane)
+  auto drain = [&](int cnt) {
+    wave_sync();
+    if (lane < cnt) {
+      const uint32_t w = q[lane];
+      const int lo = (int)(w & 0xfffu), col = (int)((w >> 12) & 0x7ffffu), full = (int)(w >> 31);
+      const int t = lo >> 1, g = lo & 1;
+      const long long p = pair_index(t, col);
+      int tt[9];
+      load_class_tally(a, im, sm, p, g, tt);
+      const uint32_t m = eddy_loo_mask_cells(tt, full, prior_bit(a, sm, p), &a.ep, &sm.eb, cpairs, &ntests);
+      *info_half(a, im, p, g) = (uint16_t)(IH_FEVAL | (full ? IH_FULL : 0u) | IH_EVAL | (m << IH_LOO_SHIFT));
+      if (full && m == 0x1FFu) atomicAdd(&solid[lo], 1);
+    }
+    wave_sync();
+    // the rest moves to the front
+    if (lane < hn - cnt) q[lane] = q[cnt + lane];
+    hn -= cnt;
+    wave_sync();
+  };
   LooEntry nx;
   locate(wave * 64, nx);
   for (int e0 = wave * 64; e0 < total; e0 += 256) {
     const LooEntry cu = nx;
     locate(e0 + 256, nx);
-    if (!cu.ok) continue;
-    int tt[9];
-    unpack_tally(cu.q0, cu.q1, cu.q2, tt);
-    const int pr = (int)((cu.pw >> (cu.p & 31)) & 1u);
-    uint32_t h = cu.ih;
-    int full;
-    if (h & IH_FEVAL) {
-      full = (int)(h & IH_FULL);
-    } else {  // outside the learned window or deferred: the full test here
-      full = d_edge_test(tt, pr, a, sm, cpairs);
-      ntests++;
-      h |= IH_FEVAL | (full ? IH_FULL : 0u);
+    bool hard = false;
+    uint32_t hw = 0;
+    if (cu.ok) {
+      int tt[9];
+      unpack_tally(cu.q0, cu.q1, cu.q2, tt);
+      const int pr = (int)((cu.pw >> (cu.p & 31)) & 1u);
+      uint32_t h = cu.ih;
+      int full;
+      if (h & IH_FEVAL) {
+        full = (int)(h & IH_FULL);
+      } else {  // outside the learned window or deferred: the full test here
+        full = d_edge_test(tt, pr, a, sm, cpairs);
+        ntests++;
+        h |= IH_FEVAL | (full ? IH_FULL : 0u);
+      }
+      uint32_t m;
+      if (eddy_loo_mask_try(tt, full, pr, &a.ep, &sm.eb, &m)) {
+        *info_half(a, im, cu.p, cu.g) = (uint16_t)(h | IH_EVAL | (m << IH_LOO_SHIFT));
+        if (full && m == 0x1FFu) atomicAdd(&solid[cu.lo], 1);
+      } else {
+        hard = true;
+        const int col = (int)(cu.p - pair_index(cu.lo >> 1, 0));
+        hw = (uint32_t)cu.lo | ((uint32_t)col << 12) | ((uint32_t)full << 31);
+      }
     }
-    const uint32_t m = d_loo_mask(tt, full, pr, a, sm, cpairs, &ntests);
-    *info_half(a, im, cu.p, cu.g) = (uint16_t)(h | IH_EVAL | (m << IH_LOO_SHIFT));
-    if (full && m == 0x1FFu) atomicAdd(&solid[cu.lo], 1);
+    const uint64_t hb = __ballot(hard);
+    if (hard) q[hn + __popcll(hb & lt)] = hw;
+    hn += __popcll(hb);
+    if (hn >= 64) drain(64);
   }
+  if (hn > 0) drain(hn);
   __syncthreads();
   phase_mark(a, 11, tph);
   // ---- C: a row with fewer than mp solid edges up to its plan continues
