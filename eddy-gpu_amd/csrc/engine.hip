@@ -81,8 +81,9 @@ struct Chunk {
   long long evslots = 0;  // sum of the items' first event capacities
 };
 
-// an item's first event capacity (grown on overflow)
-inline int item_evcap(int S, int G) { return std::max(512, 2 * S + 24 * G); }
+// an item's first event capacity (grown on overflow): the size formula, or
+// more when the set's permutation-0 item showed more events (`hint`)
+inline int item_evcap(int S, int G, int hint) { return std::max(std::max(512, 2 * S + 24 * G), hint); }
 
 // device bytes a chunk's per-item buffers take, roughly: per network slot
 // (hashes, representatives, class lists, likelihoods, score tables and the
@@ -978,6 +979,10 @@ int wait_chunk(eddy_ctx *ctx, Slot &sl, ChunkOut &out) {
     }
     fprintf(stderr, "[eddy] chunk items=%d pairs=%lld max_events=%d failing=%d status=%d\n", nitems, sl.ch.pairs, mx,
             nb, bad);
+    for (int i = 0; i < nitems; i++)
+      if (out.status[i])
+        fprintf(stderr, "[eddy]   item %d: set %d perm %d events %d status %d\n", i, sl.ch.items[i].set,
+                sl.ch.items[i].perm, out.evc[i], out.status[i]);
   }
   return EDDY_OK;
 }
@@ -1161,7 +1166,11 @@ int start_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, Chunk
   sl.ch = std::move(ch);
   const int nitems = (int)sl.ch.items.size();
   sl.cap.resize(nitems);
-  for (int i = 0; i < nitems; i++) sl.cap[i] = item_evcap(S, sets[sl.ch.items[i].set].G);
+  for (int i = 0; i < nitems; i++) {
+    const ItemMeta &im = sl.ch.items[i];
+    const SetMeta &sm = sets[im.set];
+    sl.cap[i] = std::max(item_evcap(S, sm.G, sm.evhint), im.capmin);
+  }
   if (nitems == 0) return EDDY_OK;
   return launch_chunk(ctx, sl, sets, p, perms, sl.cap, 0);
 }
@@ -1171,11 +1180,16 @@ int start_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, Chunk
 // slot with its exact count and a new hash seed; finished permutation-0 items
 // are captured when `capture`.  tests_out[i] per item of the chunk, whose
 // items stay in sl.ch.
+// With `defer`, an item of a later permutation whose events or families
+// overflowed is not run again here: its index and exact capacity go to
+// *defer, for the caller to add to a later chunk.
 int finish_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, const eddy_params *p, int perms,
-                 bool capture, std::vector<long long> &tests_out) {
+                 bool capture, std::vector<long long> &tests_out, std::vector<int> *events_out = nullptr,
+                 std::vector<std::pair<int, int>> *defer = nullptr) {
   const int N = ctx->N;
   const int nitems = (int)sl.ch.items.size();
   tests_out.assign(nitems, 0);
+  if (events_out) events_out->assign(nitems, 0);
   if (nitems == 0 || !sl.busy) return EDDY_OK;
   std::vector<int> cap = sl.cap;
   std::vector<int> todo(nitems);
@@ -1206,10 +1220,18 @@ int finish_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
       }
       if (o.status[k] & (ST_EV_OVERFLOW | ST_FAM_OVERFLOW | ST_COLLISION)) {
         if (o.status[k] & ST_EV_OVERFLOW) cap[i] = o.evc[k] + o.evc[k] / 8 + 64;
+        // an overflow also shows as a collision (the networks' hashes count
+        // events the item could not store); a collision alone needs a new
+        // hash seed and so runs again here
+        if (defer && (o.status[k] & (ST_EV_OVERFLOW | ST_FAM_OVERFLOW)) && sl.ch.items[k].perm > 0) {
+          defer->emplace_back(i, cap[i]);
+          continue;
+        }
         next.push_back(i);
         continue;
       }
       tests_out[i] = o.tests[k];
+      if (events_out) (*events_out)[i] = o.evc[k];
       ctx->timing.tests += o.tests[k];
       ctx->timing.items += 1;
       ctx->timing.pairs += sets[sl.ch.items[k].set].c;
@@ -1253,7 +1275,7 @@ void add_item(Chunk &ch, const std::vector<SetMeta> &sets, int set, int perm, co
   ch.pairs += sm.c;
   ch.sslots += N + 2;
   ch.gslots += sm.G;
-  ch.evslots += item_evcap(N + 2, sm.G);
+  ch.evslots += item_evcap(N + 2, sm.G, sm.evhint);
 }
 
 std::vector<uint32_t> label_bits(const std::vector<uint8_t> &cls) {
@@ -1307,6 +1329,7 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
     }
     SetMeta &m = meta[s];
     m.G = d.n_genes;
+    m.evhint = 0;
     m.c = d.n_genes * (d.n_genes - 1) / 2;
     m.xoff = xoff;
     m.poff = (long long)prior.size();
@@ -1663,6 +1686,14 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     // before the older one is finished, so one's kernels overlap the other's
     // tail and the host's bookkeeping.
     std::deque<int> inflight;
+    // items of later permutations whose events overflowed: they run again in
+    // the next chunk launched, with their exact capacity, instead of alone
+    // (a launch sequence of a few items is latency-bound)
+    struct Rerun {
+      int set, perm, cap;
+      std::vector<uint32_t> lab;
+    };
+    std::vector<Rerun> pending;
     int next_slot = 0, p0_pending = 0;
     bool slot_p0[EDDY_SLOTS] = {};  // the slot holds a permutation-0 chunk
     auto start_on_slot = [&](Chunk &&c, int first, int last, bool p0) -> int {
@@ -1687,7 +1718,23 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       Slot &sl = ctx->slot[k];
       inflight.pop_front();
       std::vector<long long> t;
-      CHK(finish_chunk(ctx, sl, meta, p, perms, true, t));
+      std::vector<int> evn;
+      std::vector<std::pair<int, int>> defer;
+      CHK(finish_chunk(ctx, sl, meta, p, perms, true, t, slot_p0[k] ? &evn : nullptr, slot_p0[k] ? nullptr : &defer));
+      for (const auto &d : defer) {
+        const ItemMeta &im = sl.ch.items[d.first];
+        pending.push_back({im.set, im.perm, d.second,
+                           std::vector<uint32_t>(sl.ch.labels.begin() + im.laboff,
+                                                 sl.ch.labels.begin() + im.laboff + words)});
+      }
+      // a set's later items get at least 1.5 times its permutation-0 item's
+      // events as capacity (an item that overflows runs again alone, a
+      // latency-bound launch of its own)
+      if (slot_p0[k])
+        for (size_t i = 0; i < sl.ch.items.size(); i++) {
+          SetMeta &sm = meta[sl.ch.items[i].set];
+          sm.evhint = std::max(sm.evhint, evn[i] + evn[i] / 2 + 64);
+        }
       if (trace) fprintf(stderr, "[eddy] t=%.1f finished sets %d..%d\n", now_ms(), sl.first, sl.last);
       for (size_t i = 0; i < sl.ch.items.size(); i++) {
         tests_eval[sl.ch.items[i].set] += t[i];
@@ -1767,6 +1814,8 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
         sub_chunk(ch, keep, meta, N, kept);
         ch = std::move(kept);
       }
+      pending.erase(std::remove_if(pending.begin(), pending.end(), [&](const Rerun &r0) { return r0.set >= q; }),
+                    pending.end());
       s = q + 1;
       prod.replay_after(q);
       rolled = true;
@@ -1808,6 +1857,13 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
         ctx->k_win = (const int *)ctx->b_win2.p;
         if (r.first_windowed_set < 0) r.first_windowed_set = chunk_first;
       }
+      for (const Rerun &r0 : pending) {
+        add_item(ch, meta, r0.set, r0.perm, r0.lab, N);
+        ch.items.back().nolog = 1;
+        ch.items.back().capmin = r0.cap;
+        ch.evslots += std::max(0, r0.cap - item_evcap(N + 2, meta[r0.set].G, meta[r0.set].evhint));
+      }
+      pending.clear();
       int rc = start_on_slot(std::move(ch), chunk_first, s - 1, false);
       ctx->k_labhits = nullptr;
       ctx->k_label_perms = 0;
@@ -1825,8 +1881,9 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       // labels are not ready yet: the GPU then works while the replay
       // catches up
       const bool starved = s < nsets && prod.ready.load(std::memory_order_acquire) <= s;
-      if (!ch.items.empty() && (ch.pairs >= budget || chunk_bytes(ch, evw) >= kChunkBytes || boundary ||
-                                (starved && ch.pairs >= (budget >> 3)))) {
+      if ((!ch.items.empty() && (ch.pairs >= budget || chunk_bytes(ch, evw) >= kChunkBytes || boundary ||
+                                 (starved && ch.pairs >= (budget >> 3)))) ||
+          (s == nsets && !pending.empty())) {
         if (inflight.size() == EDDY_SLOTS) {
           std::vector<int> stops;
           bool rolled;

@@ -40,6 +40,7 @@ struct SetMeta {
   eddy_edge_bounds eb;  // x intervals where the edge decision is certain (c-dependent)
   long long pairbase;   // sum of c over the run's earlier sets (per-set pair arrays)
   long long genebase;   // sum of G over the run's earlier sets (per-set row arrays)
+  int evhint;           // host only: expected events of a permuted item (0 = none yet)
 };
 
 struct ItemMeta {   // one (gene set, permutation)
@@ -55,6 +56,8 @@ struct ItemMeta {   // one (gene set, permutation)
   int fcap;           // family capacity of this item
   long long fhoff;    // family hash table offset (fhsize entries)
   int fhsize;         // power of two >= 2 * fcap
+  int nolog;          // a re-run: its first run already logged its label hits
+  int capmin;         // host only: event capacity at least this (a re-run's exact count)
 };
 
 // one device-to-device copy of the permutation-0 capture

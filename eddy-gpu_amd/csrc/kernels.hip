@@ -484,7 +484,7 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
       f2 = eddy_edge_decision_fast(t2, pr, &a.ep, &sh_eb, cpairs);
       ntests += 2;
       info = (f1 ? INFO_FULL(0) : 0u) | (f2 ? INFO_FULL(1) : 0u) | INFO_FEVAL(0) | INFO_FEVAL(1);
-      if (a.labhits && im.perm > 0 && (a.label_perms == 0 || logged))
+      if (a.labhits && im.perm > 0 && !im.nolog && (a.label_perms == 0 || logged))
         atomicAdd(&a.labhits[sm.pairbase + p], (unsigned)(f1 + f2));
     }
     if (MODE == K1_BOTH) {
