@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <condition_variable>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -170,6 +171,9 @@ struct eddy_ctx {
   int lgstride = 0;
   double lg15 = 0, lg2 = 0, lg3 = 0;
   PermStream stream_gen;
+  // buffers replaced while a job runs, freed once the device is idle (a free
+  // waits for the whole device, which would stall the chunks in flight)
+  std::vector<void *> dead_dev, dead_host;
   // device buffers, grown on demand
   DevBuf b_sets, b_xs, b_prior, b_items, b_labels, b_tiles, b_info, b_tab, b_lfull, b_lcnt, b_sflags,
       b_evcount, b_hnet, b_rep, b_clsg, b_evorder, b_evfam, b_famkey, b_famscore, b_lval, b_uniq, b_js,
@@ -282,7 +286,7 @@ static int grow(eddy_ctx *ctx, DevBuf &b, size_t bytes) {
     return carve(ctx, b, bytes);
   }
   if (b.n >= bytes) return EDDY_OK;
-  if (b.p && !b.arena) HIPCHK(ctx, hipFree(b.p));
+  if (b.p && !b.arena) ctx->dead_dev.push_back(b.p);
   b.arena = false;
   b.p = nullptr;
   b.n = 0;
@@ -290,6 +294,15 @@ static int grow(eddy_ctx *ctx, DevBuf &b, size_t bytes) {
   HIPCHK(ctx, hipMalloc(&b.p, want));
   b.n = want;
   return EDDY_OK;
+}
+
+// Frees the buffers that grow() and the pinned staging replaced; the device
+// must be idle (or about to be waited for anyway).
+static void free_dead(eddy_ctx *ctx) {
+  for (void *q : ctx->dead_dev) hipFree(q);
+  for (void *q : ctx->dead_host) hipHostFree(q);
+  ctx->dead_dev.clear();
+  ctx->dead_host.clear();
 }
 
 #define GROW(b, bytes)                                 \
@@ -362,6 +375,7 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
   const int nb = list_bufs(ctx, bufs);
   for (int i = 0; i < nb; i++)
     if (bufs[i]->p && !bufs[i]->arena) hipFree(bufs[i]->p);
+  free_dead(ctx);
   for (Slot &sl : ctx->slot) {
     if (sl.h_int) hipHostFree(sl.h_int);
     if (sl.h_tests) hipHostFree(sl.h_tests);
@@ -409,10 +423,16 @@ extern "C" int eddy_stream_permutation(eddy_ctx *ctx, int samples, int n1, int p
   return EDDY_OK;
 }
 
-// worker threads for the stream index: half the CPUs, 2..8
+// threads that recompute class splits from recorded stream states: 1..3
+static int label_threads() {
+  const int hw = (int)std::thread::hardware_concurrency();
+  return hw >= 16 ? 3 : hw >= 8 ? 2 : 1;
+}
+// worker threads for the stream index: the CPUs left after the calling
+// thread, the chain, the label threads and one more, 2..10
 static int index_workers() {
   const int hw = (int)std::thread::hardware_concurrency();
-  return std::max(2, std::min(8, hw / 2));
+  return std::max(2, std::min(10, hw - 3 - label_threads()));
 }
 
 extern "C" int eddy_stream_class2_batch(eddy_ctx *ctx, int samples, int n1, int count, uint32_t *c2) {
@@ -420,7 +440,12 @@ extern "C" int eddy_stream_class2_batch(eddy_ctx *ctx, int samples, int n1, int 
   if (samples > EDDY_MAX_N) return EDDY_ERR_ARG;
   const int words = (samples + 31) / 32;
   PermIndex idx(ctx->stream_gen.save(), samples, n1, index_workers());
-  for (int k = 0; k < count; k++) idx.class2_bits(c2 + (size_t)k * words);
+  PermIndex::Scratch sc;
+  for (int k = 0; k < count; k++) {
+    PermStream::State st;
+    idx.skip_perm(&st);
+    idx.class2_from_state(st, c2 + (size_t)k * words, sc);
+  }
   ctx->stream_gen.restore(idx.save());
   return EDDY_OK;
 }
@@ -680,8 +705,8 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
     sl.h_tests = ctx->arena_h_tests;
     sl.h_cap = eddy_ctx::kArenaItems;
   } else if (sl.h_cap < nitems) {
-    if (sl.h_int) hipHostFree(sl.h_int);
-    if (sl.h_tests) hipHostFree(sl.h_tests);
+    if (sl.h_int) ctx->dead_host.push_back(sl.h_int);
+    if (sl.h_tests) ctx->dead_host.push_back(sl.h_tests);
     sl.h_int = nullptr;
     sl.h_tests = nullptr;
     sl.h_cap = 0;
@@ -1106,7 +1131,7 @@ int capture_items(eddy_ctx *ctx, Slot &sl, const std::vector<int> &idx, const Ch
       sl.h_slices = ctx->arena_h_slices;
       sl.h_slices_cap = eddy_ctx::kArenaSlices;
     } else if (sl.h_slices_cap < sv.size()) {
-      if (sl.h_slices) HIPCHK(ctx, hipHostFree(sl.h_slices));
+      if (sl.h_slices) ctx->dead_host.push_back(sl.h_slices);
       sl.h_slices = nullptr;
       sl.h_slices_cap = 0;
       const size_t want = sv.size() + sv.size() / 2;
@@ -1438,15 +1463,21 @@ static int mlp_init_async(eddy_ctx *ctx, const float *P_pinned, hipStream_t st);
 
 namespace {
 
-// Replays the permutation stream on its own thread in the reference's order
-// (main.cu:538-573): sets in order, permutation 0's draws, then, unless the
-// set stops after permutation 0, the draws of permutations 1..r-1, whose
-// class-2 membership bits it stores.  A set stops after permutation 0 when
-// its JS there is NaN (main.cu:1400-1460), which is only known once that
-// permutation ran; the producer takes it from `stop0` (a set of fewer than
-// two genes always stops) and records the stream state after each set's
-// permutation-0 draws, so a set found to stop restarts the replay there.
+// Replays the permutation stream in the reference's order (main.cu:538-573):
+// sets in order, permutation 0's draws, then, unless the set stops after
+// permutation 0, the draws of permutations 1..r-1, whose class-2 membership
+// bits it stores.  A set stops after permutation 0 when its JS there is NaN
+// (main.cu:1400-1460), which is only known once that permutation ran; the
+// producer takes it from `stop0` (a set of fewer than two genes always
+// stops) and records the stream state after each set's permutation-0 draws,
+// so a set found to stop restarts the replay there.
+// The chain thread only moves past each permutation's draws and records the
+// stream state at its first kept draw; label threads recompute the class
+// splits from those states, a job of up to kJob permutations at a time.  Set
+// s is ready once the chain has passed it and its jobs are done; `ready`
+// counts the sets ready in order.
 struct Producer {
+  static const int kJob = 16;
   eddy_ctx *ctx = nullptr;
   int nsets = 0, perms = 0, N = 0, n1 = 0;
   std::vector<char> stop0;  // guarded by mu while the thread runs
@@ -1460,11 +1491,63 @@ struct Producer {
   std::unique_ptr<PermIndex> idx;  // the stream while the thread runs
   double ms = 0;
 
+  struct Job {
+    int set = 0, k0 = 0;  // permutations k0 .. k0 + st.size() - 1 of the set
+    std::vector<PermStream::State> st;
+  };
+  std::mutex jmu;  // guards everything below
+  std::condition_variable jcv;
+  std::deque<Job> jobs;    // published, not taken
+  std::vector<int> pend;   // per set: jobs published and not finished
+  std::vector<char> chained;  // per set: the chain has passed it
+  int scan = 0;            // the first set not known to be ready
+  bool lquit = false;
+  std::vector<std::thread> lth;
+
+  // under jmu: moves `ready` over the sets that are complete
+  void advance() {
+    while (scan < nsets && chained[scan] && pend[scan] == 0) scan++;
+    ready.store(scan, std::memory_order_release);
+  }
+  void publish(Job &&j) {
+    {
+      std::lock_guard<std::mutex> g(jmu);
+      pend[j.set]++;
+      jobs.push_back(std::move(j));
+    }
+    jcv.notify_one();
+  }
+  void label_loop() {
+    PermIndex::Scratch sc;
+    const int words = (N + 31) / 32;
+    for (;;) {
+      Job j;
+      {
+        std::unique_lock<std::mutex> g(jmu);
+        jcv.wait(g, [&]() { return lquit || !jobs.empty(); });
+        if (lquit) return;
+        j = std::move(jobs.front());
+        jobs.pop_front();
+      }
+      uint32_t *dst = lab[j.set].data() + (size_t)(j.k0 - 1) * words;
+      for (size_t k = 0; k < j.st.size(); k++) idx->class2_from_state(j.st[k], dst + k * words, sc);
+      std::lock_guard<std::mutex> g(jmu);
+      pend[j.set]--;
+      advance();
+    }
+  }
+
   void start(int from) {
     quit.store(false);
     ready.store(from);
     pos = from;
     idx.reset(new PermIndex(ctx->stream_gen.save(), N, n1, index_workers()));
+    pend.assign(nsets, 0);
+    chained.assign(nsets, 0);
+    jobs.clear();
+    scan = from;
+    lquit = false;
+    for (int t = 0; t < label_threads(); t++) lth.emplace_back([this]() { label_loop(); });
     th = std::thread([this, from]() {
       auto h0 = std::chrono::steady_clock::now();
       const int words = (N + 31) / 32;
@@ -1477,22 +1560,39 @@ struct Producer {
           stop = stop0[s] != 0;
           pos = s + 1;
         }
-        ps.class2_bits(nullptr);  // permutation 0's draws
+        ps.skip_perm(nullptr);  // permutation 0's draws
         after0[s] = ps.save();
         if (!stop) {
-          std::vector<uint32_t> &L = lab[s];
-          L.resize((size_t)(perms - 1) * words);
-          for (int pm = 1; pm < perms; pm++) ps.class2_bits(&L[(size_t)(pm - 1) * words]);
+          lab[s].resize((size_t)(perms - 1) * words);
+          for (int pm = 1; pm < perms;) {
+            Job j;
+            j.set = s;
+            j.k0 = pm;
+            for (; pm < perms && (int)j.st.size() < kJob; pm++) {
+              j.st.emplace_back();
+              ps.skip_perm(&j.st.back());
+            }
+            publish(std::move(j));
+          }
         }
-        ready.store(s + 1, std::memory_order_release);
+        std::lock_guard<std::mutex> g(jmu);
+        chained[s] = 1;
+        advance();
       }
       ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
     });
   }
-  // stops the thread; the context's stream continues where it stopped
+  // stops the threads; the context's stream continues where the chain stopped
   void stop() {
     quit.store(true);
     if (th.joinable()) th.join();
+    {
+      std::lock_guard<std::mutex> g(jmu);
+      lquit = true;
+    }
+    jcv.notify_all();
+    for (auto &t : lth) t.join();
+    lth.clear();
     if (idx) {
       ctx->stream_gen.restore(idx->save());
       idx.reset();
@@ -2043,6 +2143,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     }
   }
   HIPCHK(ctx, hipDeviceSynchronize());  // the slots' last captures
+  free_dead(ctx);
   auto t3 = std::chrono::steady_clock::now();
   // p-values on the GPU
   GROW(b_nan0, sizeof(int) * std::max(1, nsets));
