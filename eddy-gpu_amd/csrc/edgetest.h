@@ -217,6 +217,79 @@ EDDY_HD int eddy_edge_certain(double x, int dof, int prior, const eddy_edge_para
 
 
 
+// ---------------------------------------------------------------------------
+// single-precision screens
+// ---------------------------------------------------------------------------
+// Most tallies lie far from every threshold, so their decision is already
+// certain from x computed in single precision, with no FP64 division.  The
+// screens below decide only where an error bound of that x keeps them inside
+// a certain region; everywhere else they return -1 and the caller takes the
+// double-precision path above, so decisions never change.  (The
+// leave-one-out kernel keeps the double-precision forms: a screen there
+// changes its register allocation more than it saves.)
+
+// 1 / v in single precision: the hardware reciprocal (within 1 ulp) on the
+// GPU, the correctly rounded quotient on the host.
+EDDY_HD float eddy_rcp32(float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(v);
+#else
+  return 1.0f / v;
+#endif
+}
+
+// S = sum O^2 / (R K) over the nonempty rows and columns, in single
+// precision.  u = 2^-24: O^2 <= 4e8 rounds once (u), 1/K and 1/R are within
+// 1 ulp (2u), each product and sum rounds once, so every term of a row sum is
+// within 4u, the row sum of three nonnegative terms within 6u, and with 1/R
+// (2u) and the product and sum of each of the three steps (2u each) S is
+// within 14u of itself; S <= 3 (S <= the number of nonempty rows), so
+// |S - S_exact| <= 42u < 2.6e-6.
+EDDY_HD float eddy_S32(const int *t, const int *R, const int *K) {
+  float iK[3], S = 0.0f;
+  for (int j = 0; j < 3; j++) iK[j] = K[j] > 0 ? eddy_rcp32((float)K[j]) : 0.0f;
+  for (int i = 0; i < 3; i++) {
+    const float iR = R[i] > 0 ? eddy_rcp32((float)R[i]) : 0.0f;
+    float r = 0.0f;
+    for (int j = 0; j < 3; j++) r = r + (float)(t[i * 3 + j] * t[i * 3 + j]) * iK[j];
+    S = S + iR * r;
+  }
+  return S;
+}
+// An upper bound of |x32 - x_exact| for x32 = fl(0.5 n fl(S32 - 1)): the
+// subtraction adds u |S - 1| <= 2u, the product with n one rounding of x, so
+// |x32 - x| <= 0.5 n 44u + u |x| < 1.4e-6 n + 6e-8 |x|; the guard doubles it.
+EDDY_HD double eddy_x32_guard(int n, double x) { return 3e-6 * (double)(n + 1) + 1e-6 * eddy_fabs(x); }
+
+// The full test's decision from x in single precision: 1 / 0 when certain,
+// -1 otherwise (Yates' rule, other dofs, or x near an interval end).
+EDDY_HD int eddy_edge_screen32(const int *t, const int *R, const int *K, int n, int prior,
+                               const eddy_edge_params *ep, const eddy_edge_bounds *eb) {
+  if (!ep->shortcut || n <= 0) return -1;
+  int zr = 0, zk = 0, yates = 0;
+  for (int i = 0; i < 3; i++) {
+    zr += R[i] == 0;
+    zk += K[i] == 0;
+  }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      if (R[i] && K[j] && R[i] * K[j] < 5 * n) yates = 1;
+  const int dof = (2 - zk) * (2 - zr);
+  if (yates || !(dof == 1 || dof == 2 || dof == 4)) return -1;
+  const float x32 = 0.5f * ((float)n * (eddy_S32(t, R, K) - 1.0f));
+  const double x = (double)x32, g = eddy_x32_guard(n, x);
+  const bool p1 = dof == 1, p2 = dof == 2, pr = prior == 1;
+  const double xlo = pr ? eddy_pick3(eb->xlo[1], p1, p2) : eddy_pick3(eb->xlo[0], p1, p2);
+  const double xhi = pr ? eddy_pick3(eb->xhi[1], p1, p2) : eddy_pick3(eb->xhi[0], p1, p2);
+  const double xchi = eddy_pick3(ep->xchi, p1, p2);
+  const double xclo = eddy_pick3(ep->xclo, p1, p2);
+  // x_exact lies in [x - g, x + g]
+  if (x + g <= xlo) return 0;
+  if (x - g > xchi) return 0;
+  if (x - g >= xhi && x + g < xclo) return 1;
+  return -1;
+}
+
 // eddy_edge_decision() with the fast x
 EDDY_HD int eddy_edge_decision_fast(const int *t, int prior, const eddy_edge_params *ep, const eddy_edge_bounds *eb,
                                     double cpairs) {
@@ -236,6 +309,21 @@ EDDY_HD int eddy_edge_decision_fast(const int *t, int prior, const eddy_edge_par
     if (c >= 0) return c;
   }
   return eddy_edge_decision(t, prior, ep, eb, cpairs);
+}
+
+// eddy_edge_decision_fast() behind the single-precision screen: the count
+// kernel's full tests, where the two FP64 divisions per test are most of the
+// test's cost and the screen decides nearly every table
+EDDY_HD int eddy_edge_decision_screened(const int *t, int prior, const eddy_edge_params *ep,
+                                        const eddy_edge_bounds *eb, double cpairs) {
+  int R[3], K[3];
+  for (int i = 0; i < 3; i++) {
+    R[i] = t[i * 3] + t[i * 3 + 1] + t[i * 3 + 2];
+    K[i] = t[i] + t[3 + i] + t[6 + i];
+  }
+  const int c32 = eddy_edge_screen32(t, R, K, R[0] + R[1] + R[2], prior, ep, eb);
+  if (c32 >= 0) return c32;
+  return eddy_edge_decision_fast(t, prior, ep, eb, cpairs);
 }
 
 // x = chi2 / 2 of the tally t minus one count in cell (a, b), t[a*3+b] > 0,

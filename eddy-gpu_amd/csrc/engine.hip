@@ -80,6 +80,10 @@ struct Chunk {
   std::vector<uint2> tiles;
   long long pairs = 0, gslots = 0, sslots = 0;
   long long evslots = 0;  // sum of the items' first event capacities
+  // the job's last chunks: an item that overflows would run again after
+  // them, alone on an otherwise idle GPU, so their items get half as much
+  // capacity again
+  bool last = false;
 };
 
 // an item's first event capacity (grown on overflow): the size formula, or
@@ -1194,7 +1198,8 @@ int start_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, Chunk
   for (int i = 0; i < nitems; i++) {
     const ItemMeta &im = sl.ch.items[i];
     const SetMeta &sm = sets[im.set];
-    sl.cap[i] = std::max(item_evcap(S, sm.G, sm.evhint), im.capmin);
+    const int hint = sl.ch.last ? sm.evhint + sm.evhint / 2 : sm.evhint;
+    sl.cap[i] = std::max(item_evcap(S, sm.G, hint), im.capmin);
   }
   if (nitems == 0) return EDDY_OK;
   return launch_chunk(ctx, sl, sets, p, perms, sl.cap, 0);
@@ -1964,6 +1969,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
         ch.evslots += std::max(0, r0.cap - item_evcap(N + 2, meta[r0.set].G, meta[r0.set].evhint));
       }
       pending.clear();
+      ch.last = prod.ready.load(std::memory_order_acquire) >= nsets;
       int rc = start_on_slot(std::move(ch), chunk_first, s - 1, false);
       ctx->k_labhits = nullptr;
       ctx->k_label_perms = 0;
@@ -2768,10 +2774,46 @@ extern "C" int eddy_edge_loo_mask_host(const int *tally9, int c, int prior, int 
   return (int)eddy_loo_mask_fast(tally9, full, prior, &ep, &eb, (double)c, &nt);
 }
 
+extern "C" int eddy_screen32_host(const int *tally9, int c, int prior, const eddy_params *p) {
+  if (!tally9 || !p) return -2;
+  eddy_edge_params ep;
+  eddy_ctx tmp;
+  tmp.lg15 = std::lgamma(1.5);
+  tmp.lg2 = std::lgamma(2.0);
+  tmp.lg3 = std::lgamma(3.0);
+  edge_params(&tmp, p, ep);
+  eddy_edge_bounds eb;
+  edge_bounds(p, c, eb);
+  int R[3], K[3];
+  for (int i = 0; i < 3; i++) {
+    R[i] = tally9[i * 3] + tally9[i * 3 + 1] + tally9[i * 3 + 2];
+    K[i] = tally9[i] + tally9[3 + i] + tally9[6 + i];
+  }
+  const int n = R[0] + R[1] + R[2];
+  return eddy_edge_screen32(tally9, R, K, n, prior, &ep, &eb);
+}
+
 extern "C" int eddy_chi2_host(const int *tally9, int fast, int *dof, double *chi2) {
   if (!tally9 || !dof || !chi2) return EDDY_ERR_ARG;
   if (!fast) {
     *chi2 = eddy_chi2(tally9, dof);
+    return EDDY_OK;
+  }
+  if (fast == 11) {  // the single-precision screens' x (no Yates correction)
+    int R[3], K[3];
+    for (int i = 0; i < 3; i++) {
+      R[i] = tally9[i * 3] + tally9[i * 3 + 1] + tally9[i * 3 + 2];
+      K[i] = tally9[i] + tally9[3 + i] + tally9[6 + i];
+    }
+    const int n = R[0] + R[1] + R[2];
+    if (n <= 0) return EDDY_ERR_ARG;
+    int zr = 0, zk = 0;
+    for (int i = 0; i < 3; i++) {
+      zr += R[i] == 0;
+      zk += K[i] == 0;
+    }
+    *dof = (2 - zk) * (2 - zr);
+    *chi2 = 2.0 * (double)(0.5f * ((float)n * (eddy_S32(tally9, R, K) - 1.0f)));
     return EDDY_OK;
   }
   if (fast >= 2 && fast <= 10) {  // the leave-one-out form for the tally minus one count in cell fast - 2

@@ -480,8 +480,8 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
     uint32_t info = 0;
     const bool logged = im.perm > 0 && im.perm <= a.label_perms;
     if (!a.win || im.perm == 0 || logged || gj <= sh_win[li]) {
-      f1 = eddy_edge_decision_fast(t1, pr, &a.ep, &sh_eb, cpairs);
-      f2 = eddy_edge_decision_fast(t2, pr, &a.ep, &sh_eb, cpairs);
+      f1 = eddy_edge_decision_screened(t1, pr, &a.ep, &sh_eb, cpairs);
+      f2 = eddy_edge_decision_screened(t2, pr, &a.ep, &sh_eb, cpairs);
       ntests += 2;
       info = (f1 ? INFO_FULL(0) : 0u) | (f2 ? INFO_FULL(1) : 0u) | INFO_FEVAL(0) | INFO_FEVAL(1);
       if (a.labhits && im.perm > 0 && !im.nolog && (a.label_perms == 0 || logged))

@@ -27,7 +27,7 @@ EXPORTS = [
     "eddy_edge_decision_host", "eddy_stream_permutation",
     "eddy_mlp_init", "eddy_mlp_get_params", "eddy_mlp_set_params", "eddy_mlp_set_margin", "eddy_mlp_forward",
     "eddy_mlp_gradient", "eddy_mlp_train", "eddy_learn_collect", "eddy_run_learned",
-    "eddy_edge_loo_mask_host", "eddy_chi2_host", "eddy_stream_class2_batch",
+    "eddy_edge_loo_mask_host", "eddy_chi2_host", "eddy_screen32_host", "eddy_stream_class2_batch",
     "eddy_mlp_order_host", "eddy_run_repermute", "eddy_get_launch_cfg", "eddy_set_launch_cfg", "eddy_autotune",
     "eddy_loo_all_certain_host", "eddy_run_perm_detail_noalloc",
 ]
@@ -154,6 +154,7 @@ def lib():
         L.eddy_edge_decision_host.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(Params)]
         L.eddy_edge_loo_mask_host.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(Params)]
         L.eddy_loo_all_certain_host.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(Params)]
+        L.eddy_screen32_host.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(Params)]
         L.eddy_chi2_host.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.eddy_mlp_init.argtypes = [C.c_void_p, C.c_uint64]
         L.eddy_mlp_get_params.argtypes = [C.c_void_p, C.c_void_p, C.c_int]

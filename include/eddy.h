@@ -305,6 +305,13 @@ int eddy_edge_loo_mask_host(const int *tally9, int c, int prior, int full, const
  * (fast = 2 + k, k = 0..8; the cell must be nonempty); host build, for
  * tests. */
 int eddy_chi2_host(const int *tally9, int fast, int *dof, double *chi2);
+/* The count kernel's single-precision screen of the full test, host build,
+ * for tests (no counterpart in the reference: it only decides a test early
+ * where the reference's decision kernel1.cu:420-431 is already certain): 1
+ * edge, 0 none, -1 when it leaves the decision to the double-precision path.
+ * eddy_chi2_host(fast = 11) gives the screen's x (as chi-square, no Yates
+ * correction). */
+int eddy_screen32_host(const int *tally9, int c, int prior, const eddy_params *p);
 /* The learned proposal's training order (no counterpart in the reference):
  * out[e * m + i] = the i-th of the m sampled rows (of n) that epoch e visits,
  * as the training kernels draw it for `seed`; host build, for tests. */

@@ -312,3 +312,79 @@ def test_loo_all_certain_at_decision_flips(c):
                 u[k] -= 1
                 assert O.edge_decision(u, c, pr, op) == a, (t, k, c, pr, a)
     assert answered > 1000
+
+
+def _screen(t, c, pr, ep):
+    t32 = np.ascontiguousarray(np.asarray(t, np.int32).reshape(9))
+    return eddy_amd.lib().eddy_screen32_host(t32.ctypes.data_as(C.c_void_p), c, pr, C.byref(ep))
+
+
+def _exact_half_chi2(t):
+    """x = chi2 / 2 without Yates' correction, in exact rationals:
+    n (sum O^2 / (R K) - 1) / 2 over the nonempty rows and columns."""
+    from fractions import Fraction
+
+    m = np.asarray(t, np.int64).reshape(3, 3)
+    R, K, n = m.sum(1), m.sum(0), int(m.sum())
+    S = Fraction(0)
+    for i in range(3):
+        for j in range(3):
+            if R[i] and K[j]:
+                S += Fraction(int(m[i, j]) ** 2, int(R[i]) * int(K[j]))
+    return Fraction(n, 2) * (S - 1)
+
+
+def test_screen32_error_bound():
+    """The single-precision x of the screens stays within the bound their
+    guard doubles (1.4e-6 n + 6e-8 x) of the exact value, for tables of up to
+    20000 samples; dof is the reference formula's."""
+    rng = np.random.default_rng(23)
+    worst = 0.0
+    for t in _random_tables(rng, 3000, 20000):
+        if t.sum() == 0:
+            continue
+        x32, d32 = engine_chi2(t, 11)
+        _, de = engine_chi2(t, 0)
+        assert d32 == de
+        xe = _exact_half_chi2(t)
+        err = abs(x32 / 2 - float(xe))
+        bound = 1.4e-6 * t.sum() + 6e-8 * abs(float(xe))
+        assert err <= bound, (t, x32 / 2, float(xe))
+        worst = max(worst, err / bound)
+    assert worst > 0.0
+
+
+@pytest.mark.parametrize("c", [1, 780, 44850, 124750])
+def test_screen32_matches_oracle(c):
+    """Wherever the single-precision screen of the full test answers, the
+    oracle's decision agrees; it answers for most tables, both ways."""
+    rng = np.random.default_rng(5 * c + 1)
+    seen = {-1: 0, 0: 0, 1: 0}
+    for p_adjust in (1, 0):
+        ep, op = params(p_adjust)
+        tables = list(_dependent_tables(rng, 1500)) + list(_random_tables(rng, 1500, 20000))
+        for t in tables:
+            pr = int(rng.integers(0, 2))
+            s = _screen(t, c, pr, ep)
+            seen[s] += 1
+            if s >= 0:
+                assert s == O.edge_decision(t, c, pr, op), (t, c, pr, p_adjust)
+    assert seen[0] > 1000 and seen[1] > 200 and seen[-1] < seen[0] + seen[1], seen
+
+
+@pytest.mark.parametrize("c", [45, 11175])
+def test_screen32_at_decision_flips(c):
+    """Next to a decision flip the screen answers only where the oracle
+    agrees, and leaves the tables closest to the flip to the double-precision
+    path."""
+    rng = np.random.default_rng(13 * c + 7)
+    ep, op = params(1)
+    answered = left = 0
+    for t, pr in _boundary_tables(rng, 80, c, op):
+        s = _screen(t, c, pr, ep)
+        if s >= 0:
+            answered += 1
+            assert s == O.edge_decision(t, c, pr, op), (t, c, pr)
+        else:
+            left += 1
+    assert answered > 1000 and left > 0, (answered, left)
