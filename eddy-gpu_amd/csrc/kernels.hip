@@ -1057,6 +1057,8 @@ __device__ __forceinline__ void rows_body(const KArgs &a) {
   uint16_t *lrow = lrowb + wave * MPW;
   __shared__ int ncls[2];
   __shared__ unsigned long long hfull[2];
+  __shared__ unsigned long long rchg[EDDY_NWAVES];  // per wave: samples a row's flips touch
+  __shared__ unsigned long long rtog[EDDY_NWAVES][128];  // per wave: each candidate's toggled samples
 
   unsigned long long tph = 0;
   phase_mark(a, -1, tph);
@@ -1185,8 +1187,104 @@ __device__ __forceinline__ void rows_body(const KArgs &a) {
       wave_sync();
       continue;
     }
-    // leave-one-out networks of class g whose row t differs
     const int ng = ncls[g];
+    if (!WIDE && ncand <= 128) {
+      // Leave-one-out networks of class g whose row t differs, 64 samples
+      // at a time from the class's state masks: for candidate column c with
+      // flip mask fm, T_c = OR over the cells (a, b) of fm of (row t's
+      // state-a mask AND column c's state-b mask) marks the samples for
+      // which the candidate's decision differs from the full one; a sample
+      // none marks keeps the full row.  Lane j holds candidates j and 64 + j.
+      const uint64_t *mt = a.bits + ((size_t)(im.goff * 2 + g * G + t) * 3) * a.bitw;
+      uint32_t ce0 = lane < ncand ? cand_w[lane] : 0u, ce1 = 64 + lane < ncand ? cand_w[64 + lane] : 0u;
+      const uint64_t *mc0 = a.bits + ((size_t)(im.goff * 2 + g * G + (int)(ce0 & 0xffffu)) * 3) * a.bitw;
+      const uint64_t *mc1 = a.bits + ((size_t)(im.goff * 2 + g * G + (int)(ce1 & 0xffffu)) * 3) * a.bitw;
+      const uint32_t fm0 = (ce0 >> 16) & 0x1FFu, fm1 = (ce1 >> 16) & 0x1FFu;
+      auto toggles = [&](uint32_t fm, const uint64_t *mc, int w, uint64_t t0, uint64_t t1, uint64_t t2) -> uint64_t {
+        if (!fm) return 0ULL;
+        const uint64_t c0 = mc[w], c1 = mc[a.bitw + w], c2 = mc[2 * a.bitw + w];
+        uint64_t x = 0ULL;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          const uint64_t tr = r == 0 ? t0 : r == 1 ? t1 : t2;
+          if ((fm >> (r * 3)) & 1u) x |= tr & c0;
+          if ((fm >> (r * 3 + 1)) & 1u) x |= tr & c1;
+          if ((fm >> (r * 3 + 2)) & 1u) x |= tr & c2;
+        }
+        return x;
+      };
+      for (int w = 0; w * 64 < ng; w++) {
+        const uint64_t t0 = mt[w], t1 = mt[a.bitw + w], t2 = mt[2 * a.bitw + w];
+        const uint64_t T0 = toggles(fm0, mc0, w, t0, t1, t2), T1 = toggles(fm1, mc1, w, t0, t1, t2);
+        if (lane == 0) rchg[wave] = 0ULL;
+        unsigned long long *tog = rtog[wave];
+        tog[lane] = T0;
+        tog[64 + lane] = T1;
+        wave_sync();
+        if (T0 | T1) atomicOr(&rchg[wave], T0 | T1);
+        wave_sync();
+        // lane kk takes sample w * 64 + kk when a flip touches it: its row
+        // list is the first mp candidates, in column order, whose decision
+        // for it (full, toggled by bit kk of the candidate's T) is an edge;
+        // compared with the full row and hashed as it is read
+        const bool mine = (rchg[wave] >> lane) & 1ULL;
+        wave_sync();
+        if (!__any(mine)) continue;
+        uint64_t h = row_hash_begin(a.hseed, t);
+        int cnt = 0;
+        bool same = true;
+        for (int c = 0; c < ncand; c++) {
+          const uint32_t ce = cand_w[c];
+          const int bit = (int)((ce >> 25) & 1u) ^ (int)((tog[c] >> lane) & 1ULL);
+          if (mine && bit && cnt < mp) {
+            const uint16_t col = (uint16_t)(ce & 0xffffu);
+            same = same && cnt < nL && lrow[cnt] == col;
+            h = row_hash_step(h, col, cnt);
+            cnt++;
+          }
+          if (!__any(mine && cnt < mp)) break;
+        }
+        const bool ev = mine && !(same && cnt == nL);
+        if (!__any(ev)) continue;
+        int e = -1;
+        uint16_t *pp = nullptr;
+        if (ev) {
+          const int k = w * 64 + lane;
+          const int s = g == 0 ? 1 + k : a.n1 + 1 + k;
+          e = atomicAdd(&sh_evn, 1);
+          if (e < im.evcap) {
+            a.evkey[im.evoff + e] = (uint32_t)s | ((uint32_t)t << 16);
+            pp = a.evpar + (size_t)(im.evoff + e) * a.evw;
+          } else {
+            atomicOr(&a.status[item], ST_EV_OVERFLOW);
+          }
+          atomicXor(&delta[s], (unsigned long long)(hrow ^ row_hash_end(h, cnt)));
+        }
+        // the event's parents: the same walk again
+        if (__any(pp != nullptr)) {
+          uint64_t v = (uint64_t)cnt;
+          int c2 = 0;
+          for (int c = 0; c < ncand; c++) {
+            const uint32_t ce = cand_w[c];
+            const int bit = (int)((ce >> 25) & 1u) ^ (int)((tog[c] >> lane) & 1ULL);
+            if (pp && bit && c2 < cnt) {
+              const uint16_t col = (uint16_t)(ce & 0xffffu);
+              if (a.evw == 4) v |= (uint64_t)col << (16 * (c2 + 1));
+              else pp[1 + c2] = col;
+              c2++;
+            }
+            if (!__any(pp != nullptr && c2 < cnt)) break;
+          }
+          if (pp) {
+            if (a.evw == 4) *(uint64_t *)pp = v;  // mp = 3: n and three parents in one 8-byte store
+            else pp[0] = (uint16_t)cnt;
+          }
+        }
+      }
+      wave_sync();
+      continue;
+    }
+    // leave-one-out networks of class g whose row t differs
     const uint8_t *xt = xs + (size_t)t * NP;
     for (int k = lane; k < ng; k += 64) {
       int sid = (g ? cls1 : cls0)[k];
@@ -1287,7 +1385,7 @@ __device__ __forceinline__ void rows_body(const KArgs &a) {
   phase_mark(a, 8, tph);
 }
 
-__global__ __launch_bounds__(256) void k_rows(KArgs a) { rows_body<false, false>(a); }
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_rows(KArgs a) { rows_body<false, false>(a); }
 __global__ __launch_bounds__(256) void k_rows_g(KArgs a) { rows_body<true, false>(a); }
 __global__ __launch_bounds__(256) void k_rows_w(KArgs a) { rows_body<false, true>(a); }
 __global__ __launch_bounds__(256) void k_rows_gw(KArgs a) { rows_body<true, true>(a); }
