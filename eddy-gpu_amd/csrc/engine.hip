@@ -427,16 +427,17 @@ extern "C" int eddy_stream_permutation(eddy_ctx *ctx, int samples, int n1, int p
   return EDDY_OK;
 }
 
-// threads that recompute class splits from recorded stream states: 1..3
+// threads that recompute class splits from recorded stream states: 1..2
 static int label_threads() {
   const int hw = (int)std::thread::hardware_concurrency();
-  return hw >= 16 ? 3 : hw >= 8 ? 2 : 1;
+  return hw >= 8 ? 2 : 1;
 }
 // worker threads for the stream index: the CPUs left after the calling
-// thread, the chain, the label threads and one more, 2..10
+// thread, the chain, the label threads and two more (the GPU runtime's),
+// 2..8; past 8 they contend for the caches more than they add
 static int index_workers() {
   const int hw = (int)std::thread::hardware_concurrency();
-  return std::max(2, std::min(10, hw - 3 - label_threads()));
+  return std::max(2, std::min(8, hw - 4 - label_threads()));
 }
 
 extern "C" int eddy_stream_class2_batch(eddy_ctx *ctx, int samples, int n1, int count, uint32_t *c2) {
@@ -1482,7 +1483,7 @@ namespace {
 // s is ready once the chain has passed it and its jobs are done; `ready`
 // counts the sets ready in order.
 struct Producer {
-  static const int kJob = 16;
+  static const int kJob = 64;
   eddy_ctx *ctx = nullptr;
   int nsets = 0, perms = 0, N = 0, n1 = 0;
   std::vector<char> stop0;  // guarded by mu while the thread runs
@@ -1494,7 +1495,9 @@ struct Producer {
   int pos = 0;  // sets whose stop0 the thread has read (guarded by mu)
   std::thread th;
   std::unique_ptr<PermIndex> idx;  // the stream while the thread runs
-  double ms = 0;
+  double ms = 0;               // the chain thread's time
+  double chain_wait_ms = 0;    // of which waiting for superblocks
+  std::atomic<long long> label_us{0};  // the label threads' busy time
 
   struct Job {
     int set = 0, k0 = 0;  // permutations k0 .. k0 + st.size() - 1 of the set
@@ -1509,10 +1512,21 @@ struct Producer {
   bool lquit = false;
   std::vector<std::thread> lth;
 
+  std::condition_variable rcv;  // `ready` moved
   // under jmu: moves `ready` over the sets that are complete
   void advance() {
+    const int was = scan;
     while (scan < nsets && chained[scan] && pend[scan] == 0) scan++;
     ready.store(scan, std::memory_order_release);
+    if (scan != was) rcv.notify_all();
+  }
+  // waits until set s is ready (the caller's thread sleeps: the replay's
+  // threads need the cores)
+  void wait_ready(int s) {
+    for (int i = 0; i < 256; i++)
+      if (ready.load(std::memory_order_acquire) > s) return;
+    std::unique_lock<std::mutex> g(jmu);
+    rcv.wait(g, [&]() { return ready.load(std::memory_order_acquire) > s || lquit; });
   }
   void publish(Job &&j) {
     {
@@ -1534,8 +1548,10 @@ struct Producer {
         j = std::move(jobs.front());
         jobs.pop_front();
       }
+      const auto b0 = std::chrono::steady_clock::now();
       uint32_t *dst = lab[j.set].data() + (size_t)(j.k0 - 1) * words;
       for (size_t k = 0; k < j.st.size(); k++) idx->class2_from_state(j.st[k], dst + k * words, sc);
+      label_us += std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - b0).count();
       std::lock_guard<std::mutex> g(jmu);
       pend[j.set]--;
       advance();
@@ -1585,6 +1601,7 @@ struct Producer {
         advance();
       }
       ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+      chain_wait_ms += ps.wait_ms();
     });
   }
   // stops the threads; the context's stream continues where the chain stopped
@@ -1596,6 +1613,7 @@ struct Producer {
       lquit = true;
     }
     jcv.notify_all();
+    rcv.notify_all();
     for (auto &t : lth) t.join();
     lth.clear();
     if (idx) {
@@ -1690,6 +1708,23 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
   const int N = ctx->N, perms = p->perms, S = N + 2;
   std::vector<SetMeta> meta;
   CHK(prepare_sets(ctx, sets, nsets, p, meta));
+  // the stream replay starts at once: it needs only the sets' sizes, and
+  // runs while the rest of the setup does
+  std::unique_ptr<Producer> prodp;
+  if (p->mode == EDDY_MODE_BATCHED) {
+    prodp.reset(new Producer());
+    Producer &pr = *prodp;
+    pr.ctx = ctx;
+    pr.nsets = nsets;
+    pr.perms = perms;
+    pr.N = N;
+    pr.n1 = ctx->n1;
+    pr.stop0.assign(nsets, 0);
+    for (int s = 0; s < nsets; s++) pr.stop0[s] = meta[s].G < 2;
+    pr.lab.assign(nsets, std::vector<uint32_t>());
+    pr.after0.resize(nsets);
+    pr.start(0);
+  }
   CHK(p0_begin(ctx, meta, p));
   GROW(b_js, sizeof(double) * (size_t)std::max(1, nsets) * perms);
   HIPCHK(ctx, hipMemsetAsync(ctx->b_js.p, 0, sizeof(double) * (size_t)std::max(1, nsets) * perms, ctx->stream));
@@ -1760,17 +1795,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
   ctx->timing.phase1_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 
   if (batched) {
-    Producer prod;
-    prod.ctx = ctx;
-    prod.nsets = nsets;
-    prod.perms = perms;
-    prod.N = N;
-    prod.n1 = ctx->n1;
-    prod.stop0.assign(nsets, 0);
-    for (int s = 0; s < nsets; s++) prod.stop0[s] = meta[s].G < 2;
-    prod.lab.assign(nsets, std::vector<uint32_t>());
-    prod.after0.resize(nsets);
-    prod.start(0);
+    Producer &prod = *prodp;
     for (int q = 0; q < nsets; q++) nan0[q] = meta[q].G < 2;
     const char *pe = getenv("EDDY_PROFILE");
     const bool trace = pe && pe[0] == '2';
@@ -1988,7 +2013,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       // catches up
       const bool starved = s < nsets && prod.ready.load(std::memory_order_acquire) <= s;
       if ((!ch.items.empty() && (ch.pairs >= budget || chunk_bytes(ch, evw) >= kChunkBytes || boundary ||
-                                 (starved && ch.pairs >= (budget >> 3)))) ||
+                                 (starved && (inflight.empty() || ch.pairs >= (budget >> 1))))) ||
           (s == nsets && !pending.empty())) {
         if (inflight.size() == EDDY_SLOTS) {
           std::vector<int> stops;
@@ -2089,7 +2114,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       }
       // next set's items
       auto hw = std::chrono::steady_clock::now();
-      while (prod.ready.load(std::memory_order_acquire) <= s) std::this_thread::yield();
+      prod.wait_ready(s);
       auto hb = std::chrono::steady_clock::now();
       ctx->hp.wait += std::chrono::duration<double, std::milli>(hb - hw).count();
       if (!nan0[s]) {
@@ -2105,6 +2130,9 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     }
     prod.stop();
     ctx->timing.host_stream_ms += prod.ms;
+    if (getenv("EDDY_PROFILE"))
+      fprintf(stderr, "[eddy] replay: chain %.1f ms (waiting for superblocks %.1f), label threads busy %.1f ms\n",
+              prod.ms, prod.chain_wait_ms, 1e-3 * (double)prod.label_us.load());
     for (int k = 0; k < EDDY_SLOTS; k++)
       if (slot_ev[k]) hipEventDestroy(slot_ev[k]);
     for (int q = 0; q < collect_upto; q++) r.tests_collect += tests_eval[q];

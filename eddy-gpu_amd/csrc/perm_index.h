@@ -66,7 +66,7 @@ class PermIndex {
     m_ = n > 1 ? ((uint64_t)1 << (32 + sh_)) / (uint32_t)n + 1 : 0;
     slots_.reset(new Slot[R_]);
     for (int i = 0; i < R_; i++) {
-      slots_[i].raw.resize(kH + kL);
+      slots_[i].ckpt.resize((size_t)kNB * kH);
       slots_[i].res.resize(kL);
       slots_[i].pres.resize((size_t)kNB * W_);
     }
@@ -89,11 +89,7 @@ class PermIndex {
   void skip_perm(PermStream::State *st) {
     const int n = n_, W = W_;
     long long p = g_ + n;  // discarded draws
-    if (st) {
-      const Slot &s = get(p / kL);
-      const int j = (int)(p % kL);
-      for (int i = 0; i < kH; i++) st->h[i] = s.raw[j + i];
-    }
+    if (st) state_at(get(p / kL), (int)(p % kL), st->h);
     uint64_t *miss = miss_.data();
     for (int w = 0; w < W; w++) miss[w] = ~0ull;
     if (n & 63) miss[W - 1] = (1ull << (n & 63)) - 1;
@@ -171,12 +167,13 @@ class PermIndex {
     if (n & 31) c2[words - 1] &= (1u << (n & 31)) - 1u;
   }
 
+  // time the chain has waited for superblocks
+  double wait_ms() const { return wait_ms_; }
+
   // The stream state at the current position (the 34 values before it).
   PermStream::State save() {
-    const Slot &s = get(g_ / kL);
-    const int j = (int)(g_ % kL);
     PermStream::State st;
-    for (int i = 0; i < kH; i++) st.h[i] = s.raw[j + i];
+    state_at(get(g_ / kL), (int)(g_ % kL), st.h);
     return st;
   }
 
@@ -189,10 +186,20 @@ class PermIndex {
   };
   struct Slot {
     std::atomic<long long> k{-1};  // superblock held (complete) or -1
-    std::vector<uint32_t> raw;     // kH values before the superblock, then its kL values
-    std::vector<uint16_t> res;     // (raw >> 1) mod n
+    std::vector<uint32_t> ckpt;    // per block: the kH values before it
+    std::vector<uint16_t> res;     // (value >> 1) mod n
     std::vector<uint64_t> pres;    // per block: residues present
   };
+
+  // h = the kH values before position j of the slot's superblock: from the
+  // checkpoint of j's block, up to 127 steps of the recurrence
+  static void state_at(const Slot &s, int j, uint32_t *h) {
+    uint32_t b[kH + kB];
+    const int blk = j / kB, o = j - blk * kB;
+    for (int i = 0; i < kH; i++) b[i] = s.ckpt[(size_t)blk * kH + i];
+    for (int i = kH; i < kH + o; i++) b[i] = b[i - 31] + b[i - 3];
+    for (int i = 0; i < kH; i++) h[i] = b[o + i];
+  }
 
   // From block-aligned position p: moves past every block that lacks one of
   // the residues in miss_ (removing the residues it holds) and returns the
@@ -373,9 +380,13 @@ class PermIndex {
   const Slot &get(long long k) {
     if (k > release_.load(std::memory_order_relaxed)) release_.store(k, std::memory_order_release);
     Slot &s = slots_[k % R_];
-    int spins = 0;
-    while (s.k.load(std::memory_order_acquire) != k) {
-      if (++spins > 64) std::this_thread::yield();
+    if (s.k.load(std::memory_order_acquire) != k) {
+      const auto w0 = std::chrono::steady_clock::now();
+      int spins = 0;
+      while (s.k.load(std::memory_order_acquire) != k) {
+        if (++spins > 64) std::this_thread::yield();
+      }
+      wait_ms_ += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     }
     return s;
   }
@@ -387,6 +398,7 @@ class PermIndex {
   // superblocks are filled first.
   void work() {
     const Mat &P = jump_superblock();
+    std::vector<uint32_t> raw;  // the scalar path's values
     for (;;) {
       long long k = next_.load(std::memory_order_relaxed);
       int spins = 0;
@@ -395,9 +407,7 @@ class PermIndex {
         if (k - R_ < release_.load(std::memory_order_acquire) &&
             next_.compare_exchange_weak(k, k + 1, std::memory_order_acq_rel))
           break;
-        // the ring is full: the chain has R_ superblocks ahead of it, time
-        // enough to sleep instead of taking a core from the other threads
-        if (++spins > 64) std::this_thread::sleep_for(std::chrono::microseconds(20));
+        if (++spins > 64) std::this_thread::yield();
         k = next_.load(std::memory_order_relaxed);
       }
       StateSlot &cur = states_[k % R_];
@@ -411,20 +421,28 @@ class PermIndex {
       nx.v = mat_vec(P, st);
       nx.k.store(k + 1, std::memory_order_release);
       Slot &s = slots_[k % R_];
-      fill(s, st);
+      fill(s, st, raw);
       s.k.store(k, std::memory_order_release);
     }
   }
 
-  void fill(Slot &s, const Vec &st) {
-    uint32_t *b = s.raw.data();
-    for (int i = 0; i < kH; i++) b[i] = st.v[i];
+  // A superblock from its starting state: the residues of its values, the
+  // checkpoint of each block, the presence words of each block.
+  void fill(Slot &s, const Vec &st, std::vector<uint32_t> &raw) {
 #if defined(__x86_64__)
-    if (avx512()) recurrence_avx512(b);
-    else
+    if (avx512() && n_ > 1) {
+      values_avx512(st, s);
+    } else
 #endif
-    recurrence(b);
-    residues(b + kH, s.res.data(), kL);
+    {
+      raw.resize(kH + kL);
+      uint32_t *b = raw.data();
+      for (int i = 0; i < kH; i++) b[i] = st.v[i];
+      recurrence(b);
+      residues(b + kH, s.res.data(), kL);
+      for (int blk = 0; blk < kNB; blk++)
+        for (int i = 0; i < kH; i++) s.ckpt[(size_t)blk * kH + i] = b[blk * kB + i];
+    }
     uint64_t *pr = s.pres.data();
     const uint16_t *r = s.res.data();
 #if defined(__x86_64__)
@@ -458,30 +476,65 @@ class PermIndex {
     for (; i < kH + kL; i++) b[i] = b[i - 31] + b[i - 3];
   }
 #if defined(__x86_64__)
-  // The same values 16 at a time.  Substituting r[i-3] = r[i-6] + r[i-34]
-  // six times gives r[i] = r[i-18] + r[i-31] + r[i-34] + r[i-37] + r[i-40] +
-  // r[i-43] + r[i-46]: every term lies at least 18 back, so a vector of 16
-  // consecutive values is a sum of windows of the three vectors before it.
-  __attribute__((target("avx512f,avx512bw"))) static void recurrence_avx512(uint32_t *b) {
-    static_assert(kH == 34 && (kL - 14) % 16 == 2, "14 scalar values, 16-wide vectors, 2 scalar values");
-    int i = kH;
-    for (; i < 48; i++) b[i] = b[i - 31] + b[i - 3];
-    __m512i p3 = _mm512_loadu_si512((const void *)b), p2 = _mm512_loadu_si512((const void *)(b + 16));
-    __m512i p1 = _mm512_loadu_si512((const void *)(b + 32));
-    for (; i + 16 <= kH + kL; i += 16) {
-      // windows r[i-d .. i-d+15] of (p3 p2 p1) = r[i-48 .. i-1]
+  // The same values 16 at a time, turned into residues as they are made, with
+  // the checkpoints taken from the vectors in registers (no array of values).
+  // Substituting r[i-3] = r[i-6] + r[i-34] five times gives r[i] = r[i-18] +
+  // r[i-31] + r[i-34] + r[i-37] + r[i-40] + r[i-43] + r[i-46]: every term lies
+  // at least 18 back, so a vector of 16 consecutive values is a sum of windows
+  // of the three vectors before it.
+  __attribute__((target("avx512f,avx512bw"))) void values_avx512(const Vec &st, Slot &s) const {
+    static_assert(kH == 34 && kL % kB == 0 && kB % 16 == 0, "layout");
+    // positions -34 .. 13 (the history, then 14 values one at a time), so
+    // that the vectors start at position 14 and a block's start, a multiple
+    // of 128, is element 2 of a vector
+    alignas(64) uint32_t b[48];
+    for (int i = 0; i < kH; i++) b[i] = st.v[i];
+    for (int i = kH; i < 48; i++) b[i] = b[i - 31] + b[i - 3];
+    for (int i = 0; i < kH; i++) s.ckpt[i] = b[i];
+    uint16_t *res = s.res.data();
+    for (int i = 0; i < 14; i++) res[i] = mod_n(b[kH + i] >> 1);
+    __m512i p3 = _mm512_load_si512((const void *)b), p2 = _mm512_load_si512((const void *)(b + 16));
+    __m512i p1 = _mm512_load_si512((const void *)(b + 32));
+    const __m512i vm = _mm512_set1_epi64((long long)m_);
+    const __m512i vn = _mm512_set1_epi32(n_);
+    const __m128i shv = _mm_cvtsi32_si128(32 + sh_);
+    const __m512i lo32 = _mm512_set1_epi64(0xFFFFFFFFLL);
+    int q = 14;  // position of the vector's first value
+    for (; q + 16 <= kL; q += 16) {
+      // windows r[q-d .. q-d+15] of (p3 p2 p1) = r[q-48 .. q-1]
       const __m512i w31 = _mm512_alignr_epi32(p1, p2, 1), w34 = _mm512_alignr_epi32(p2, p3, 14);
       const __m512i w37 = _mm512_alignr_epi32(p2, p3, 11), w40 = _mm512_alignr_epi32(p2, p3, 8);
       const __m512i w43 = _mm512_alignr_epi32(p2, p3, 5), w46 = _mm512_alignr_epi32(p2, p3, 2);
       const __m512i far = _mm512_add_epi32(_mm512_add_epi32(_mm512_add_epi32(w31, w34), _mm512_add_epi32(w37, w40)),
                                            _mm512_add_epi32(w43, w46));
       const __m512i v = _mm512_add_epi32(far, _mm512_alignr_epi32(p1, p2, 14));
-      _mm512_storeu_si512((void *)(b + i), v);
+      // residues (value >> 1) mod n, as residues_avx512
+      const __m512i x = _mm512_srli_epi32(v, 1);
+      const __m512i qe = _mm512_srl_epi64(_mm512_mul_epu32(x, vm), shv);
+      const __m512i qo = _mm512_srl_epi64(_mm512_mul_epu32(_mm512_srli_epi64(x, 32), vm), shv);
+      const __m512i qq = _mm512_or_si512(_mm512_and_si512(qe, lo32), _mm512_slli_epi64(qo, 32));
+      _mm256_storeu_si256((__m256i *)(res + q), _mm512_cvtepi32_epi16(_mm512_sub_epi32(x, _mm512_mullo_epi32(qq, vn))));
+      if (((q + 2) & (kB - 1)) == 0) {
+        // block (q + 2) / kB starts at element 2: the 34 values before it
+        // are p2, p1 and elements 0, 1 of v
+        uint32_t *c = &s.ckpt[(size_t)((q + 2) / kB) * kH];
+        _mm512_storeu_si512((void *)c, p2);
+        _mm512_storeu_si512((void *)(c + 16), p1);
+        _mm512_mask_storeu_epi32((void *)(c + 32), (__mmask16)0x3, v);
+      }
       p3 = p2;
       p2 = p1;
       p1 = v;
     }
-    for (; i < kH + kL; i++) b[i] = b[i - 31] + b[i - 3];
+    // the last values one at a time, from the three vectors in registers
+    alignas(64) uint32_t t[48 + 16];
+    _mm512_store_si512((void *)t, p3);
+    _mm512_store_si512((void *)(t + 16), p2);
+    _mm512_store_si512((void *)(t + 32), p1);
+    for (int i = 48; q < kL; i++, q++) {
+      t[i] = t[i - 31] + t[i - 3];
+      res[q] = mod_n(t[i] >> 1);
+    }
   }
   // Presence words of every block: a byte per residue set by plain stores,
   // then 64 bytes at a time packed to one word by their sign bits and
@@ -551,6 +604,7 @@ class PermIndex {
   int sh_ = 0;
   uint64_t m_ = 0;
   long long g_ = 0;  // draws consumed since the starting state
+  double wait_ms_ = 0;
   struct StateSlot {
     std::atomic<long long> k{-1};  // superblock whose starting state v holds, or -1
     Vec v;
