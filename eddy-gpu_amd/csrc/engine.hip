@@ -1464,6 +1464,7 @@ static int mlp_train_dev(eddy_ctx *ctx, const float *dX, const float *dY, long l
                          int epochs, int batch, float lr, uint64_t seed, hipStream_t st);
 static int mlp_grad_dev(eddy_ctx *ctx, const float *dX, const float *dY, const int *dIdx, int b0, int n,
                         float *dLoss, hipStream_t st);
+static int mlp_loss_dev(eddy_ctx *ctx, const float *dX, const float *dY, int n, float *dLoss, hipStream_t st);
 static void mlp_initial_params(uint64_t seed, float *P);
 static int mlp_init_async(eddy_ctx *ctx, const float *P_pinned, hipStream_t st);
 
@@ -1972,9 +1973,8 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       CHK(apply_stops(stops, rolled));
     }
 
-    // chunks of the later permutations grow from 8M to 64M pairs (again after
-    // the training boundary): the GPU starts early, later launches amortise
-    // their fixed costs
+    // chunks of the later permutations grow from 8M to 64M pairs: the GPU
+    // starts early, later launches amortise their fixed costs
     auto launch = [&]() -> int {
       ctx->k_labhits = chunk_first < collect_upto ? (unsigned *)ctx->b_labhits.p : nullptr;
       ctx->k_label_perms = chunk_first < collect_upto ? label_perms : 0;
@@ -2096,9 +2096,9 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
                                  (const long long *)ctx->b_mlbase.p, collect_upto, n, (float *)ctx->b_my.p, st2));
           const float *fx = (const float *)ctx->b_feat.p, *fy = (const float *)ctx->b_my.p;
           const int nl = (int)std::min<long long>(n, 65536);
-          CHK(mlp_grad_dev(ctx, fx, fy, nullptr, 0, nl, (float *)ctx->b_mlploss.p + 1, st2));
+          CHK(mlp_loss_dev(ctx, fx, fy, nl, (float *)ctx->b_mlploss.p + 1, st2));
           CHK(mlp_train_dev(ctx, fx, fy, n, lc->max_rows, lc->epochs, lc->batch, lc->lr, lc->seed, st2));
-          CHK(mlp_grad_dev(ctx, fx, fy, nullptr, 0, nl, (float *)ctx->b_mlploss.p + 2, st2));
+          CHK(mlp_loss_dev(ctx, fx, fy, nl, (float *)ctx->b_mlploss.p + 2, st2));
           const float kappa = ctx->mlp_kappa;
           ctx->mlp_kappa = lc->margin;
           int rc = compute_windows(ctx, meta, collect_upto, p->mp, ctx->b_win2, st2);
@@ -2108,7 +2108,6 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
           HIPCHK(ctx, hipEventRecord(ctx->train_ev, st2));
           training = true;
           r.trained = 1;
-          budget = budget_min;
           if (trace) fprintf(stderr, "[eddy] t=%.1f training queued (%.1f ms)\n", now_ms(), now_ms() - r.collect_ms);
         }
       }
@@ -2968,6 +2967,13 @@ static int mlp_grad_dev(eddy_ctx *ctx, const float *dX, const float *dY, const i
   return EDDY_OK;
 }
 
+// mean loss over rows 0..n-1 (forward pass only)
+static int mlp_loss_dev(eddy_ctx *ctx, const float *dX, const float *dY, int n, float *dLoss, hipStream_t st) {
+  GROW(b_mscr, sizeof(float) * (size_t)((n + 255) / 256));
+  HIPCHK(ctx, mlp_loss(dX, dY, n, (const float *)ctx->b_mlp.p, (float *)ctx->b_mscr.p, dLoss, st));
+  return EDDY_OK;
+}
+
 extern "C" int eddy_mlp_gradient(eddy_ctx *ctx, const float *X, const float *Y, int n, float *grad, float *loss) {
   if (!ctx || !X || !Y || !grad || n <= 0) return EDDY_ERR_ARG;
   if (!ctx->mlp_ready) return EDDY_ERR_STATE;
@@ -2983,8 +2989,7 @@ extern "C" int eddy_mlp_gradient(eddy_ctx *ctx, const float *X, const float *Y, 
 
 // mean loss over (up to) the first 65536 rows with the current network
 static int mlp_eval_loss(eddy_ctx *ctx, const float *dX, const float *dY, long long n, float *out) {
-  CHK(mlp_grad_dev(ctx, dX, dY, nullptr, 0, (int)std::min<long long>(n, 65536), (float *)ctx->b_mlploss.p,
-                   ctx->stream));
+  CHK(mlp_loss_dev(ctx, dX, dY, (int)std::min<long long>(n, 65536), (float *)ctx->b_mlploss.p, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   HIPCHK(ctx, hipMemcpy(out, ctx->b_mlploss.p, sizeof(float), hipMemcpyDeviceToHost));
   return EDDY_OK;

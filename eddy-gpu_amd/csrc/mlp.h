@@ -26,6 +26,9 @@ hipError_t mlp_forward(const float *dX, int n, const float *dP, float *dOut, hip
 // mean BCE gradient and loss over samples idx[b0 .. b0+n) (rows b0.. when idx is null)
 hipError_t mlp_gradient(const float *dX, const float *dY, const int *dIdx, int b0, int n, const float *dP,
                         float *dPart, float *dGrad, float *dLoss, hipStream_t st);
+// mean BCE loss alone over rows 0..n-1 (dPart: (n + 255) / 256 floats)
+hipError_t mlp_loss(const float *dX, const float *dY, int n, const float *dP, float *dPart, float *dLoss,
+                    hipStream_t st);
 // the same gradient followed by one Adam step (beta 0.9 / 0.999, eps 1e-8)
 hipError_t mlp_train_step(const float *dX, const float *dY, const int *dIdx, int b0, int n, float *dP, float *dPart,
                           float *dGrad, float *dLoss, float *dM, float *dV, float lr, int step, hipStream_t st);

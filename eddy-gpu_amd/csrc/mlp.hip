@@ -152,6 +152,55 @@ __global__ __launch_bounds__(256) void k_mlp_part(const float *X, const float *Y
   }
 }
 
+// Loss only (no gradient): one thread per sample of rows 0..n-1, parameters
+// in LDS, the block's 256 losses summed in sample order into part[blk].
+__global__ __launch_bounds__(256) void k_mlp_loss_part(const float *X, const float *Y, int n, const float *params,
+                                                       float *part) {
+  __shared__ float P[MLP_NPARAMS];
+  __shared__ float L[256];
+  for (int i = threadIdx.x; i < MLP_NPARAMS; i += blockDim.x) P[i] = params[i];
+  __syncthreads();
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.f;
+  if (b < n) {
+    float x[MLP_IN];
+#pragma unroll
+    for (int i = 0; i < MLP_IN; i++) x[i] = X[(size_t)b * MLP_IN + i];
+    float a1[MLP_H];
+#pragma unroll
+    for (int j = 0; j < MLP_H; j++) {
+      float s = P[MLP_OFF_B1 + j];
+#pragma unroll
+      for (int i = 0; i < MLP_IN; i++) s += P[MLP_OFF_W1 + j * MLP_IN + i] * x[i];
+      a1[j] = relu(s);
+    }
+    float z = P[MLP_OFF_B3];
+    for (int k = 0; k < MLP_H; k++) {
+      float s = P[MLP_OFF_B2 + k];
+#pragma unroll
+      for (int j = 0; j < MLP_H; j++) s += P[MLP_OFF_W2 + k * MLP_H + j] * a1[j];
+      z += P[MLP_OFF_W3 + k] * relu(s);
+    }
+    // BCE(sigmoid(z), y) = softplus(z) - y z, as k_mlp_part
+    const float y = Y[b];
+    const float sp = z > 0.f ? z + log1pf(__expf(-z)) : log1pf(__expf(z));
+    l = sp - y * z;
+  }
+  L[threadIdx.x] = l;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float s = 0.f;
+    for (int t = 0; t < 256; t++) s += L[t];
+    part[blockIdx.x] = s;
+  }
+}
+__global__ void k_mlp_loss_reduce(const float *part, int nblk, int n, float *loss) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  float s = 0.f;
+  for (int k = 0; k < nblk; k++) s += part[k];
+  *loss = s / (float)n;
+}
+
 // grad[q] = (sum over blocks of part[blk][q]) / n in block order; loss likewise.
 // With adam != 0 the same thread then applies Adam to parameter q.
 __global__ __launch_bounds__(256) void k_mlp_reduce(const float *part, int nblk, int n, float *grad, float *loss,
@@ -285,6 +334,15 @@ hipError_t mlp_gradient(const float *dX, const float *dY, const int *dIdx, int b
   k_mlp_part<<<nblk, 256, 0, st>>>(dX, dY, dIdx, b0, n, dP, dPart);
   k_mlp_reduce<<<(MLP_NPARAMS + 1 + 255) / 256, 256, 0, st>>>(dPart, nblk, n, dGrad, dLoss, 0, nullptr, nullptr,
                                                                nullptr, 0.f, 0.f, 0.f, 0.f, 0);
+  return hipGetLastError();
+}
+
+hipError_t mlp_loss(const float *dX, const float *dY, int n, const float *dP, float *dPart, float *dLoss,
+                    hipStream_t st) {
+  if (n <= 0) return hipErrorInvalidValue;
+  const int nblk = (n + 255) / 256;
+  k_mlp_loss_part<<<nblk, 256, 0, st>>>(dX, dY, n, dP, dPart);
+  k_mlp_loss_reduce<<<1, 64, 0, st>>>(dPart, nblk, n, dLoss);
   return hipGetLastError();
 }
 
