@@ -124,6 +124,7 @@ struct Slot {
   // timing events: 0 start, 1 after count, 2 after loo, 3 after rows,
   // 5 after score, 6 after BDeu, 4 after the tail
   hipEvent_t tev[8] = {};
+  hipEvent_t done = nullptr;  // end of the launch's readback (blocking sync: the host thread sleeps on it)
   float kms[7] = {};  // last launch: count, loo, rows, score, BDeu, tail, class-1 gather (ms)
   SlotGraph gr[64];
   int *h_int = nullptr;  // pinned: status, events, families, unique networks (4 x h_cap)
@@ -354,6 +355,10 @@ extern "C" int eddy_ctx_create(int device, eddy_ctx **out) {
         eddy_ctx_destroy(ctx);
         return EDDY_ERR_HIP;
       }
+    if (hipEventCreateWithFlags(&sl.done, hipEventBlockingSync | hipEventDisableTiming) != hipSuccess) {
+      eddy_ctx_destroy(ctx);
+      return EDDY_ERR_HIP;
+    }
   }
   // the no-allocation scratch array's address on this device (the name is
   // only a host-side stand-in; kernels and copies take this pointer)
@@ -386,6 +391,7 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
     if (sl.h_slices) hipHostFree(sl.h_slices);
     for (hipEvent_t e : sl.tev)
       if (e) hipEventDestroy(e);
+    if (sl.done) hipEventDestroy(sl.done);
     for (SlotGraph &gr : sl.gr) {
       if (gr.x) hipGraphExecDestroy(gr.x);
       if (gr.g) hipGraphDestroy(gr.g);
@@ -956,6 +962,7 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   HIPCHK(ctx, hipMemcpyAsync(h + 2 * hc, sl.famcount.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
   HIPCHK(ctx, hipMemcpyAsync(h + 3 * hc, sl.uniq.p, sizeof(int) * nitems, hipMemcpyDeviceToHost, st));
   HIPCHK(ctx, hipMemcpyAsync(sl.h_tests, sl.tests.p, sizeof(long long) * nitems, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipEventRecord(sl.done, st));
   sl.busy = true;
   sl.plain_tests = plain_tests;
   sl.has_tiles = !ch.tiles.empty();
@@ -967,6 +974,9 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
 // Waits for the slot's launch and copies its per-item results out.
 int wait_chunk(eddy_ctx *ctx, Slot &sl, ChunkOut &out) {
   auto hl1 = std::chrono::steady_clock::now();
+  // the thread sleeps until the readback is done (a spinning wait would take
+  // a core from the stream replay's threads)
+  HIPCHK(ctx, hipEventSynchronize(sl.done));
   HIPCHK(ctx, hipStreamSynchronize(sl.st));
   ctx->hp.sync += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hl1).count();
   sl.busy = false;
