@@ -159,7 +159,8 @@ struct BoundsKey {
 };
 
 struct HostProf {  // host-side time split of one eddy_run_sets call (ms)
-  double launch = 0, sync = 0, capture = 0, wait = 0, build = 0, js0 = 0, tail = 0;
+  double prep = 0, launch = 0, sync = 0, capture = 0, wait = 0, build = 0, js0 = 0, tail = 0;
+  double prep_split[4] = {};  // prepare_sets: bounds, host tables, uploads + gather, pooled tables
 };
 
 struct eddy_ctx {
@@ -1330,6 +1331,12 @@ std::vector<uint32_t> label_bits(const std::vector<uint8_t> &cls) {
 int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy_params *p,
                  std::vector<SetMeta> &meta) {
   const int N = ctx->N, NP = (N + 63) & ~63;
+  auto tp = std::chrono::steady_clock::now();
+  auto mark = [&](int k) {
+    auto t = std::chrono::steady_clock::now();
+    ctx->hp.prep_split[k] = std::chrono::duration<double, std::milli>(t - tp).count();
+    tp = t;
+  };
   meta.resize(nsets);
   std::vector<int> rows;
   std::vector<long long> rowoff;
@@ -1354,13 +1361,15 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
     std::vector<eddy_edge_bounds> out(todo.size());
     const int nth = (int)std::min<size_t>(16, std::max<size_t>(1, todo.size() / 4));
     std::vector<std::thread> pool;
-    for (int k = 0; k < nth; k++)
-      pool.emplace_back([&, k]() {
-        for (size_t i = k; i < todo.size(); i += nth) edge_bounds(p, todo[i], out[i]);
-      });
+    if (!todo.empty())
+      for (int k = 0; k < nth; k++)
+        pool.emplace_back([&, k]() {
+          for (size_t i = k; i < todo.size(); i += nth) edge_bounds(p, todo[i], out[i]);
+        });
     for (auto &t : pool) t.join();
     for (size_t i = 0; i < todo.size(); i++) ctx->bounds_cache[todo[i]] = out[i];
   }
+  mark(0);
   for (int s = 0; s < nsets; s++) {
     const eddy_set_desc &d = sets[s];
     if (d.n_genes < 0 || d.n_genes > EDDY_MAX_G || (d.n_genes && !d.genes) || d.n_prior < 0 ||
@@ -1401,6 +1410,7 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
       prior[m.poff + (pidx >> 5)] |= 1u << (pidx & 31);
     }
   }
+  mark(1);
   GROW(b_sets, sizeof(SetMeta) * nsets);
   GROW(b_xs, xoff);
   GROW(b_prior, sizeof(uint32_t) * prior.size());
@@ -1423,6 +1433,7 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
   // the cutoff schedule's count kernel takes class 2's table as this minus
   // class 1's.  One item per set, launches of at most 64M pairs.
   GROW(b_totals, sizeof(uint16_t) * ctx->tabw * (size_t)std::max(1LL, pairbase));
+  mark(2);
   {
     const std::vector<uint32_t> lab = label_bits(ctx->classOf);
     Chunk ch;
@@ -1464,6 +1475,7 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
     CHK(run());
   }
   HIPCHK(ctx, hipStreamSynchronize(st));
+  mark(3);
   return EDDY_OK;
 }
 
@@ -1657,7 +1669,7 @@ struct Producer {
 
 // Pooled-table features of every pair of every set (one permutation-0 item
 // per set through the count kernel, no tests) into b_feat at pairbase.
-int compute_features(eddy_ctx *ctx, const std::vector<SetMeta> &meta) {
+int compute_features(eddy_ctx *ctx, const std::vector<SetMeta> &meta, hipStream_t st, bool sync) {
   const int nsets = (int)meta.size();
   if (nsets == 0) return EDDY_OK;
   KArgs a{};
@@ -1667,9 +1679,9 @@ int compute_features(eddy_ctx *ctx, const std::vector<SetMeta> &meta) {
   a.totals = (uint16_t *)ctx->b_totals.p;
   a.tabw = ctx->tabw;
   a.feat = (float *)ctx->b_feat.p;
-  k_features<<<nsets, 256, 0, ctx->stream>>>(a, nsets);
+  k_features<<<nsets, 256, 0, st>>>(a, nsets);
   HIPCHK(ctx, hipGetLastError());
-  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (sync) HIPCHK(ctx, hipStreamSynchronize(st));
   return EDDY_OK;
 }
 
@@ -1719,6 +1731,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
   const int N = ctx->N, perms = p->perms, S = N + 2;
   std::vector<SetMeta> meta;
   CHK(prepare_sets(ctx, sets, nsets, p, meta));
+  ctx->hp.prep = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   // the stream replay starts at once: it needs only the sets' sizes, and
   // runs while the rest of the setup does
   std::unique_ptr<Producer> prodp;
@@ -1778,7 +1791,12 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
   if (p->schedule != EDDY_TESTS_CUTOFF) win_from = nsets;
   if (collect_upto > 0 || (win_from < nsets && !lazy)) {
     GROW(b_feat, sizeof(float) * MLP_IN * std::max(1LL, sumc));
-    CHK(compute_features(ctx, meta));
+    // the learned job reads the features only on stream2 (training, its
+    // losses and windows), so they are computed there and nothing waits
+    if (lc)
+      CHK(compute_features(ctx, meta, ctx->stream2, false));
+    else
+      CHK(compute_features(ctx, meta, ctx->stream, true));
   }
   if (collect_upto > 0) {
     GROW(b_labhits, sizeof(unsigned) * std::max(1LL, sumc));
@@ -1813,7 +1831,10 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     auto now_ms = [&]() {
       return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     };
-    const long long budget_min = 8LL << 20, budget_max = 64LL << 20;
+#ifndef EDDY_CHUNK_MIN_MPAIRS
+#define EDDY_CHUNK_MIN_MPAIRS 8
+#endif
+    const long long budget_min = (long long)EDDY_CHUNK_MIN_MPAIRS << 20, budget_max = 64LL << 20;
     const int evw = mp_eff(p, meta) + 1;
     long long budget = budget_min;
     const int words = (N + 31) / 32;
@@ -2253,9 +2274,10 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
   }
   if (getenv("EDDY_PROFILE"))
     fprintf(stderr,
-            "[eddy] run %d sets: total %.1f ms | setup %.1f launch %.1f sync %.1f capture %.1f wait %.1f build %.1f "
-            "js0 %.1f tail %.1f | kernels %.1f %.1f %.1f %.1f | train %.1f\n",
-            nsets, ctx->timing.total_ms, ctx->timing.phase1_ms, ctx->hp.launch, ctx->hp.sync, ctx->hp.capture, ctx->hp.wait, ctx->hp.build,
+            "[eddy] run %d sets: total %.1f ms | setup %.1f (sets %.2f = %.2f %.2f %.2f %.2f) launch %.1f sync %.1f capture %.1f wait %.1f "
+            "build %.1f js0 %.1f tail %.1f | kernels %.1f %.1f %.1f %.1f | train %.1f\n",
+            nsets, ctx->timing.total_ms, ctx->timing.phase1_ms, ctx->hp.prep, ctx->hp.prep_split[0],
+            ctx->hp.prep_split[1], ctx->hp.prep_split[2], ctx->hp.prep_split[3], ctx->hp.launch, ctx->hp.sync, ctx->hp.capture, ctx->hp.wait, ctx->hp.build,
             ctx->hp.js0, ctx->hp.tail, ctx->timing.kernel_ms[0], ctx->timing.kernel_ms[1], ctx->timing.kernel_ms[2],
             ctx->timing.kernel_ms[3], r.train_ms);
   return EDDY_OK;

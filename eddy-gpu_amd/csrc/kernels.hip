@@ -1717,6 +1717,48 @@ __device__ __forceinline__ void score_body(const KArgs &a) {
     int ps = 0, pr = 0, ia = 0, ib = 0, id = 0;
     const int *esS = es + evst[s], *esR = es + evst[r];
     const uint16_t *etS = et + evst[s], *etR = et + evst[r];
+    if (gs == gr && ns == nr) {
+      // the same class's full network changed at the same rows to the same
+      // parent lists: the same network (anything else takes the row walk).
+      // Four events at a time, every load of a round issued before its
+      // compares; with four u16 per event (mp = 3) a list is one 8-byte word
+      // compared up to its count
+      bool same = true;
+      for (int i0 = 0; i0 < ns && same; i0 += 4) {
+        int eS[4], eR[4];
+        bool rows_eq = true;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int i = min(i0 + u, ns - 1);
+          eS[u] = esS[i];
+          eR[u] = esR[i];
+          rows_eq = rows_eq && etS[i] == etR[i];
+        }
+        if (a.evw == 4) {
+          uint64_t vS[4], vR[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            vS[u] = *(const uint64_t *)(a.evpar + (size_t)(im.evoff + eS[u]) * 4);
+            vR[u] = *(const uint64_t *)(a.evpar + (size_t)(im.evoff + eR[u]) * 4);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int c = (int)(vS[u] & 0xffffu);
+            const uint64_t m = c >= 3 ? ~0ULL : (1ULL << (16 * (c + 1))) - 1ULL;
+            same = same && ((vS[u] ^ vR[u]) & m) == 0ULL;
+          }
+        } else {
+          for (int u = 0; u < 4 && same; u++) {
+            const uint16_t *ls = a.evpar + (size_t)(im.evoff + eS[u]) * a.evw;
+            const uint16_t *lr = a.evpar + (size_t)(im.evoff + eR[u]) * a.evw;
+            same = ls[0] == lr[0];
+            for (int j = 1; j <= ls[0] && same; j++) same = ls[j] == lr[j];
+          }
+        }
+        same = same && rows_eq;
+      }
+      if (same) continue;
+    }
     while (eq) {
       const int ta = ia < ns ? (int)etS[ia] : 0x7fffffff;
       const int tb = ib < nr ? (int)etR[ib] : 0x7fffffff;
@@ -1788,6 +1830,9 @@ __device__ __forceinline__ void score_body(const KArgs &a) {
   }
   if (tid == 0) F = 0;
   __syncthreads();
+  // each key's table slot is kept (famfull for the full rows, evfam for the
+  // events) and turned into its family number once the families are numbered
+  int *evfam = a.evfam + im.evoff;
   for (int i = tid; i < 2 * G + E; i += 256) {
     uint64_t key;
     if (i < 2 * G) {
@@ -1798,11 +1843,19 @@ __device__ __forceinline__ void score_body(const KArgs &a) {
       const int e = i - 2 * G;
       const uint32_t k = ek[e];
       const int s = (int)(k & 0xffffu);
-      if (rep[s] != s) continue;
+      if (rep[s] != s) {
+        evfam[e] = -1;
+        continue;
+      }
       const uint16_t *pp = epar + (size_t)e * a.evw;
       key = fam_key((int)(k >> 16), pp[0], pp + 1);
     }
-    if (ht_insert_min(fk, fv, key, 0, FH) < 0) atomicOr(&flag, 4);
+    const int h = ht_insert_min(fk, fv, key, 0, FH);
+    if (h < 0) atomicOr(&flag, 4);
+    if (i < 2 * G)
+      famfull[i] = h;
+    else
+      evfam[i - 2 * G] = h;
   }
   __syncthreads();
   for (int i = tid; i < FH; i += 256) {
@@ -1819,17 +1872,12 @@ __device__ __forceinline__ void score_body(const KArgs &a) {
     return;
   }
   for (int i = tid; i < 2 * G; i += 256) {
-    int g = i / G, t = i - g * G;
-    size_t lb = (size_t)(im.goff * 2 + g * G + t);
-    int slot = ht_find(fk, fam_key(t, a.lcnt[lb], a.lfull + lb * MPW), FH);
+    const int slot = famfull[i];
     famfull[i] = slot >= 0 ? fv[slot] : -1;
   }
   for (int e = tid; e < E; e += 256) {
-    const uint32_t k = ek[e];
-    const int s = (int)(k & 0xffffu);
-    const uint16_t *pp = epar + (size_t)e * a.evw;
-    int slot = rep[s] == s ? ht_find(fk, fam_key((int)(k >> 16), pp[0], pp + 1), FH) : -1;
-    a.evfam[im.evoff + e] = slot >= 0 ? fv[slot] : -1;
+    const int slot = evfam[e];
+    evfam[e] = slot >= 0 ? fv[slot] : -1;
   }
   __syncthreads();
   phase_mark(a, 3, tph);
@@ -1865,18 +1913,40 @@ __device__ __forceinline__ void score_body(const KArgs &a) {
       const double *lq = a.lgtab + (size_t)q * a.lgstride;
       const double *lqr = a.lgtab + (size_t)(q * r) * a.lgstride;
       const double lq0 = lq[0], lqr0 = lqr[0];
-      for (int j = 0; j < q; j++) {
-        const uint64_t *mp_ = np > 0 ? mk + (size_t)(p0 * 3 + nth_state(f1, j)) * W : nullptr;
-        int nij = 0;
-        double sum1 = 0.0;
-        for (int k = 0; k < r; k++) {
-          const uint64_t *mc = mk + (size_t)(t * 3 + nth_state(f0, k)) * W;
-          int cnt = 0;
-          for (int w = 0; w < W; w++) cnt += __popcll(np > 0 ? (mc[w] & mp_[w]) : mc[w]);
-          nij += cnt;
-          sum1 = sum1 + (lqr[cnt] - lqr0);
+      if (np == 1 && !a.plain) {
+        // one parent: the (child, parent) histogram in class g is the pair's
+        // class-g tally, cell 3 x + y for state x of the pair's higher gene
+        // and y of its lower one (the cutoff schedule stores class 1's)
+        uint64_t w0, w1, w2;
+        class_tally_words(a, im, sm, t > p0 ? pair_index(t, p0) : pair_index(p0, t), g, w0, w1, w2);
+        for (int j = 0; j < q; j++) {
+          const int sj = nth_state(f1, j);
+          int nij = 0;
+          double sum1 = 0.0;
+          for (int k = 0; k < r; k++) {
+            const int sk = nth_state(f0, k);
+            const int cell = t > p0 ? sk * 3 + sj : sj * 3 + sk;
+            const uint64_t wv = cell < 4 ? w0 : cell < 8 ? w1 : w2;
+            const int cnt = (int)((wv >> (16 * (cell & 3))) & 0xffffu);
+            nij += cnt;
+            sum1 = sum1 + (lqr[cnt] - lqr0);
+          }
+          sum2 = sum2 + ((lq0 - lq[nij]) + sum1);
         }
-        sum2 = sum2 + ((lq0 - lq[nij]) + sum1);
+      } else {
+        for (int j = 0; j < q; j++) {
+          const uint64_t *mp_ = np > 0 ? mk + (size_t)(p0 * 3 + nth_state(f1, j)) * W : nullptr;
+          int nij = 0;
+          double sum1 = 0.0;
+          for (int k = 0; k < r; k++) {
+            const uint64_t *mc = mk + (size_t)(t * 3 + nth_state(f0, k)) * W;
+            int cnt = 0;
+            for (int w = 0; w < W; w++) cnt += __popcll(np > 0 ? (mc[w] & mp_[w]) : mc[w]);
+            nij += cnt;
+            sum1 = sum1 + (lqr[cnt] - lqr0);
+          }
+          sum2 = sum2 + ((lq0 - lq[nij]) + sum1);
+        }
       }
       st_bins += (unsigned long long)(4 * q * r);
     }
