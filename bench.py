@@ -20,6 +20,7 @@ same work.  Prints one JSON line:
                  thread, extrapolated decile by decile to the same workload
 """
 import argparse
+import faulthandler
 import json
 import os
 import subprocess
@@ -339,6 +340,7 @@ def summarize_main(argv):
 
 
 def main():
+    faulthandler.enable()  # a crash inside the library prints the Python call it came from
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)

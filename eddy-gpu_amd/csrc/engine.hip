@@ -12,6 +12,7 @@
 #include <thread>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <cstdio>
 #include <cstdlib>
 #include <condition_variable>
@@ -132,6 +133,8 @@ struct Slot {
   int h_cap = 0;
   Slice *h_slices = nullptr;  // pinned staging of the capture slice list
   size_t h_slices_cap = 0;
+  unsigned char *h_up = nullptr;  // pinned staging of the launch's uploads (items, labels, tiles, order)
+  size_t h_up_cap = 0;
   bool busy = false;
   bool plain_tests = false, has_tiles = false, gather1 = false;
   Chunk ch;             // the chunk in flight
@@ -201,6 +204,8 @@ struct eddy_ctx {
   hipEvent_t train_ev = nullptr;
   void *h_train = nullptr;  // pinned staging of the learned job's uploads (read by stream2)
   size_t h_train_cap = 0;
+  unsigned char *h_prep[2] = {};  // pinned staging of the set preparation's uploads
+  size_t h_prep_cap[2] = {};
   // learned proposal: network parameters + Adam state, per-run feature,
   // label and window arrays (device pointers handed to the kernels)
   DevBuf b_mlp, b_mlpm, b_mlpv, b_mlpg, b_mlploss, b_feat, b_labhits, b_score, b_win, b_mx, b_my, b_mxb, b_myb,
@@ -311,6 +316,44 @@ static void free_dead(eddy_ctx *ctx) {
   ctx->dead_host.clear();
 }
 
+// Host-to-device uploads of one launch: the host ranges are packed into one
+// pinned staging buffer (grown as needed; a replaced one is freed when the job
+// ends) and copied from there on the stream, so the host does not wait for
+// the transfers (a copy from pageable memory is a synchronous staged
+// transfer).  The staging is refilled only after the stream has been waited
+// for.  In a no-allocation call the copies go straight from the sources.
+struct Uploads {
+  struct Range {
+    void *dst;
+    const void *src;
+    size_t n, off;
+  };
+  std::vector<Range> v;
+  size_t total = 0;
+  void add(void *dst, const void *src, size_t n) {
+    if (n == 0) return;
+    v.push_back(Range{dst, src, n, total});
+    total += (n + 63) & ~(size_t)63;
+  }
+  int run(eddy_ctx *ctx, unsigned char *&buf, size_t &cap, hipStream_t st) {
+    if (ctx->arena) {
+      for (const Range &r : v) HIPCHK(ctx, hipMemcpyAsync(r.dst, r.src, r.n, hipMemcpyHostToDevice, st));
+      return EDDY_OK;
+    }
+    if (cap < total) {
+      if (buf) ctx->dead_host.push_back(buf);
+      buf = nullptr;
+      cap = 0;
+      const size_t want = total + total / 4;
+      HIPCHK(ctx, hipHostMalloc((void **)&buf, want, hipHostMallocDefault));
+      cap = want;
+    }
+    for (const Range &r : v) std::memcpy(buf + r.off, r.src, r.n);
+    for (const Range &r : v) HIPCHK(ctx, hipMemcpyAsync(r.dst, buf + r.off, r.n, hipMemcpyHostToDevice, st));
+    return EDDY_OK;
+  }
+};
+
 #define GROW(b, bytes)                                 \
   do {                                                 \
     int rc_ = grow(ctx, ctx->b, (size_t)(bytes));      \
@@ -390,6 +433,7 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
     if (sl.h_int) hipHostFree(sl.h_int);
     if (sl.h_tests) hipHostFree(sl.h_tests);
     if (sl.h_slices) hipHostFree(sl.h_slices);
+    if (sl.h_up) hipHostFree(sl.h_up);
     for (hipEvent_t e : sl.tev)
       if (e) hipEventDestroy(e);
     if (sl.done) hipEventDestroy(sl.done);
@@ -405,6 +449,8 @@ extern "C" int eddy_ctx_destroy(eddy_ctx *ctx) {
   if (ctx->ev1) hipEventDestroy(ctx->ev1);
   if (ctx->train_ev) hipEventDestroy(ctx->train_ev);
   if (ctx->h_train) hipHostFree(ctx->h_train);
+  for (unsigned char *q : ctx->h_prep)
+    if (q) hipHostFree(q);
   if (ctx->stream2) hipStreamDestroy(ctx->stream2);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -735,12 +781,14 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
     std::stable_sort(sl.horder.begin(), sl.horder.end(),
                      [&](int x, int y) { return sets[ch.items[x].set].G > sets[ch.items[y].set].G; });
   hipStream_t st = sl.st;
-  HIPCHK(ctx, hipMemcpyAsync(sl.items.p, ch.items.data(), sizeof(ItemMeta) * nitems, hipMemcpyHostToDevice, st));
-  HIPCHK(ctx, hipMemcpyAsync(sl.labels.p, ch.labels.data(), sizeof(uint32_t) * ch.labels.size(), hipMemcpyHostToDevice,
-                             st));
-  if (!ch.tiles.empty())
-    HIPCHK(ctx, hipMemcpyAsync(sl.tiles.p, ch.tiles.data(), sizeof(uint2) * ch.tiles.size(), hipMemcpyHostToDevice, st));
-  HIPCHK(ctx, hipMemcpyAsync(sl.order.p, sl.horder.data(), sizeof(int) * nitems, hipMemcpyHostToDevice, st));
+  {
+    Uploads up;
+    up.add(sl.items.p, ch.items.data(), sizeof(ItemMeta) * nitems);
+    up.add(sl.labels.p, ch.labels.data(), sizeof(uint32_t) * ch.labels.size());
+    up.add(sl.tiles.p, ch.tiles.data(), sizeof(uint2) * ch.tiles.size());
+    up.add(sl.order.p, sl.horder.data(), sizeof(int) * nitems);
+    CHK(up.run(ctx, sl.h_up, sl.h_up_cap, st));
+  }
 
   KArgs a{};
   a.N = N;
@@ -1417,13 +1465,15 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
   GROW(b_genes, sizeof(int) * rows.size());
   GROW(b_rowoff, sizeof(long long) * rowoff.size());
   hipStream_t st = ctx->stream;
-  HIPCHK(ctx, hipMemcpyAsync(ctx->b_sets.p, meta.data(), sizeof(SetMeta) * nsets, hipMemcpyHostToDevice, st));
-  if (!prior.empty())
-    HIPCHK(ctx, hipMemcpyAsync(ctx->b_prior.p, prior.data(), sizeof(uint32_t) * prior.size(), hipMemcpyHostToDevice, st));
+  {
+    Uploads up;
+    up.add(ctx->b_sets.p, meta.data(), sizeof(SetMeta) * nsets);
+    up.add(ctx->b_prior.p, prior.data(), sizeof(uint32_t) * prior.size());
+    up.add(ctx->b_genes.p, rows.data(), sizeof(int) * rows.size());
+    up.add(ctx->b_rowoff.p, rowoff.data(), sizeof(long long) * rowoff.size());
+    CHK(up.run(ctx, ctx->h_prep[0], ctx->h_prep_cap[0], st));
+  }
   if (!rows.empty()) {
-    HIPCHK(ctx, hipMemcpyAsync(ctx->b_genes.p, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->b_rowoff.p, rowoff.data(), sizeof(long long) * rowoff.size(), hipMemcpyHostToDevice,
-                               st));
     int nrows = (int)rows.size();
     k_gather<<<nrows, 64, 0, st>>>(ctx->dX, N, NP, (const int *)ctx->b_genes.p, (const long long *)ctx->b_rowoff.p,
                                    nrows, (uint8_t *)ctx->b_xs.p);
@@ -1442,12 +1492,12 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
       GROW(b_items, sizeof(ItemMeta) * ch.items.size());
       GROW(b_labels, sizeof(uint32_t) * ch.labels.size());
       GROW(b_tiles, sizeof(uint2) * ch.tiles.size());
-      HIPCHK(ctx, hipMemcpyAsync(ctx->b_items.p, ch.items.data(), sizeof(ItemMeta) * ch.items.size(),
-                                 hipMemcpyHostToDevice, st));
-      HIPCHK(ctx, hipMemcpyAsync(ctx->b_labels.p, ch.labels.data(), sizeof(uint32_t) * ch.labels.size(),
-                                 hipMemcpyHostToDevice, st));
-      HIPCHK(ctx, hipMemcpyAsync(ctx->b_tiles.p, ch.tiles.data(), sizeof(uint2) * ch.tiles.size(),
-                                 hipMemcpyHostToDevice, st));
+      // the second staging buffer: the first may still be in transfer
+      Uploads up;
+      up.add(ctx->b_items.p, ch.items.data(), sizeof(ItemMeta) * ch.items.size());
+      up.add(ctx->b_labels.p, ch.labels.data(), sizeof(uint32_t) * ch.labels.size());
+      up.add(ctx->b_tiles.p, ch.tiles.data(), sizeof(uint2) * ch.tiles.size());
+      CHK(up.run(ctx, ctx->h_prep[1], ctx->h_prep_cap[1], st));
       KArgs a{};
       a.N = N;
       a.n1 = ctx->n1;
@@ -1567,7 +1617,9 @@ struct Producer {
       {
         std::unique_lock<std::mutex> g(jmu);
         jcv.wait(g, [&]() { return lquit || !jobs.empty(); });
-        if (lquit) return;
+        // a stop drains the queue first: the sets before a restart point
+        // keep every label their jobs owe them
+        if (jobs.empty()) return;
         j = std::move(jobs.front());
         jobs.pop_front();
       }
