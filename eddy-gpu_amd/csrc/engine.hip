@@ -23,7 +23,7 @@
 #include "eddy.h"
 #include "engine_types.h"
 #include "mlp.h"
-#include "perm_index.h"
+#include "perm_chain.h"
 #include "perm_stream.h"
 
 // kernels (kernels.hip)
@@ -485,26 +485,26 @@ static int label_threads() {
   const int hw = (int)std::thread::hardware_concurrency();
   return hw >= 8 ? 2 : 1;
 }
-// worker threads for the stream index: the CPUs left after the calling
-// thread, the chain, the label threads and two more (the GPU runtime's),
-// 2..8; past 8 they contend for the caches more than they add
-static int index_workers() {
+// the chain's walks run at this nice value: they take the cores the
+// caller's threads and the GPU runtime leave
+static const int kChainNice = 10;
+// threads that walk the stream's permutation chain: the CPUs left after the
+// calling thread, the mapper, the merger, the label threads and one more
+// (the GPU runtime's), 2..12
+static int chain_threads() {
   const int hw = (int)std::thread::hardware_concurrency();
-  return std::max(2, std::min(8, hw - 4 - label_threads()));
+  return std::max(2, std::min(12, hw - 4 - label_threads()));
 }
 
 extern "C" int eddy_stream_class2_batch(eddy_ctx *ctx, int samples, int n1, int count, uint32_t *c2) {
   if (!ctx || samples < 1 || n1 < 0 || n1 > samples || count < 0 || (count && !c2)) return EDDY_ERR_ARG;
   if (samples > EDDY_MAX_N) return EDDY_ERR_ARG;
   const int words = (samples + 31) / 32;
-  PermIndex idx(ctx->stream_gen.save(), samples, n1, index_workers());
-  PermIndex::Scratch sc;
-  for (int k = 0; k < count; k++) {
-    PermStream::State st;
-    idx.skip_perm(&st);
-    idx.class2_from_state(st, c2 + (size_t)k * words, sc);
-  }
-  ctx->stream_gen.restore(idx.save());
+  PermChain chain(ctx->stream_gen.save(), samples, n1, count, chain_threads());
+  if (!chain.wait(count)) return EDDY_ERR_STATE;
+  PermChain::Scratch sc;
+  for (int k = 0; k < count; k++) chain.class2_from_state(chain.kept(k), c2 + (size_t)k * words, sc);
+  ctx->stream_gen.restore(chain.start_state(count));
   return EDDY_OK;
 }
 
@@ -1548,39 +1548,44 @@ namespace {
 // bits it stores.  A set stops after permutation 0 when its JS there is NaN
 // (main.cu:1400-1460), which is only known once that permutation ran; the
 // producer takes it from `stop0` (a set of fewer than two genes always
-// stops) and records the stream state after each set's permutation-0 draws,
-// so a set found to stop restarts the replay there.
-// The chain thread only moves past each permutation's draws and records the
-// stream state at its first kept draw; label threads recompute the class
-// splits from those states, a job of up to kJob permutations at a time.  Set
-// s is ready once the chain has passed it and its jobs are done; `ready`
-// counts the sets ready in order.
+// stops).  Every permutation draws the same way, so the stream is one chain
+// of permutations whatever stops: a stop only changes which chain entries
+// belong to which set.  The chain (PermChain) is walked on its own threads
+// as far as the job can reach; the mapper thread assigns entries to sets in
+// order (a set's permutation 0 at index i, its later ones at i+1..i+r-1) and
+// publishes label jobs of up to kJob permutations, which label threads turn
+// into class splits from the entries' kept states.  Set s is ready once the
+// mapper has passed it and its jobs are done; `ready` counts the sets ready
+// in order.  A stop found among sets the mapper has passed restarts the
+// mapper after it, on the same chain.
 struct Producer {
   static const int kJob = 64;
   eddy_ctx *ctx = nullptr;
   int nsets = 0, perms = 0, N = 0, n1 = 0;
-  std::vector<char> stop0;  // guarded by mu while the thread runs
+  std::vector<char> stop0;  // guarded by mu while the mapper runs
   std::vector<std::vector<uint32_t>> lab;
-  std::vector<PermStream::State> after0;
+  std::vector<long long> idx0;  // per set: chain index of its permutation 0 (set by the mapper)
   std::atomic<int> ready{0};
   std::atomic<bool> quit{false};
   std::mutex mu;
-  int pos = 0;  // sets whose stop0 the thread has read (guarded by mu)
+  int pos = 0;  // sets whose stop0 the mapper has read (guarded by mu)
+  long long consumed = 0;  // chain entries of the sets the mapper has passed
   std::thread th;
-  std::unique_ptr<PermIndex> idx;  // the stream while the thread runs
-  double ms = 0;               // the chain thread's time
-  double chain_wait_ms = 0;    // of which waiting for superblocks
+  std::unique_ptr<PermChain> chain;
+  double ms = 0;               // the mapper's time
+  double chain_wait_ms = 0;    // of which waiting for the chain
+  long long fallback = 0;      // chain entries its merger walked itself
   std::atomic<long long> label_us{0};  // the label threads' busy time
 
   struct Job {
-    int set = 0, k0 = 0;  // permutations k0 .. k0 + st.size() - 1 of the set
-    std::vector<PermStream::State> st;
+    int set = 0, k0 = 0, nk = 0;  // permutations k0 .. k0 + nk - 1 of the set
+    long long c0 = 0;             // chain index of permutation k0
   };
   std::mutex jmu;  // guards everything below
   std::condition_variable jcv;
   std::deque<Job> jobs;    // published, not taken
   std::vector<int> pend;   // per set: jobs published and not finished
-  std::vector<char> chained;  // per set: the chain has passed it
+  std::vector<char> chained;  // per set: the mapper has passed it
   int scan = 0;            // the first set not known to be ready
   bool lquit = false;
   std::vector<std::thread> lth;
@@ -1610,7 +1615,7 @@ struct Producer {
     jcv.notify_one();
   }
   void label_loop() {
-    PermIndex::Scratch sc;
+    PermChain::Scratch sc;
     const int words = (N + 31) / 32;
     for (;;) {
       Job j;
@@ -1620,12 +1625,12 @@ struct Producer {
         // a stop drains the queue first: the sets before a restart point
         // keep every label their jobs owe them
         if (jobs.empty()) return;
-        j = std::move(jobs.front());
+        j = jobs.front();
         jobs.pop_front();
       }
       const auto b0 = std::chrono::steady_clock::now();
       uint32_t *dst = lab[j.set].data() + (size_t)(j.k0 - 1) * words;
-      for (size_t k = 0; k < j.st.size(); k++) idx->class2_from_state(j.st[k], dst + k * words, sc);
+      for (int k = 0; k < j.nk; k++) chain->class2_from_state(chain->kept(j.c0 + k), dst + (size_t)k * words, sc);
       label_us += std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - b0).count();
       std::lock_guard<std::mutex> g(jmu);
       pend[j.set]--;
@@ -1633,21 +1638,31 @@ struct Producer {
     }
   }
 
-  void start(int from) {
+  // The chain for the whole job: as many entries as the sets can use with
+  // the stops known now.
+  void begin() {
+    long long count = 0;
+    for (int s = 0; s < nsets; s++) count += stop0[s] ? 1 : perms;
+    chain.reset(new PermChain(ctx->stream_gen.save(), N, n1, count, chain_threads(), kChainNice));
+    idx0.assign(nsets, 0);
+    start(0, 0);
+  }
+  // the mapper from set `from`, whose permutation 0 is chain entry c
+  void start(int from, long long c) {
     quit.store(false);
     ready.store(from);
     pos = from;
-    idx.reset(new PermIndex(ctx->stream_gen.save(), N, n1, index_workers()));
+    consumed = c;
     pend.assign(nsets, 0);
     chained.assign(nsets, 0);
     jobs.clear();
     scan = from;
     lquit = false;
     for (int t = 0; t < label_threads(); t++) lth.emplace_back([this]() { label_loop(); });
-    th = std::thread([this, from]() {
+    th = std::thread([this, from, c]() {
       auto h0 = std::chrono::steady_clock::now();
       const int words = (N + 31) / 32;
-      PermIndex &ps = *idx;
+      long long i = c;
       for (int s = from; s < nsets; s++) {
         if (quit.load(std::memory_order_relaxed)) break;
         bool stop;
@@ -1656,31 +1671,36 @@ struct Producer {
           stop = stop0[s] != 0;
           pos = s + 1;
         }
-        ps.skip_perm(nullptr);  // permutation 0's draws
-        after0[s] = ps.save();
+        idx0[s] = i;
         if (!stop) {
+          // the set's entries must be final before their jobs go out
+          const auto w0 = std::chrono::steady_clock::now();
+          if (!chain->wait(i + perms - 1)) break;
+          chain_wait_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
           lab[s].resize((size_t)(perms - 1) * words);
-          for (int pm = 1; pm < perms;) {
+          for (int pm = 1; pm < perms; pm += kJob) {
             Job j;
             j.set = s;
             j.k0 = pm;
-            for (; pm < perms && (int)j.st.size() < kJob; pm++) {
-              j.st.emplace_back();
-              ps.skip_perm(&j.st.back());
-            }
+            j.nk = std::min(kJob, perms - pm);
+            j.c0 = i + pm;
             publish(std::move(j));
           }
+        }
+        i += stop ? 1 : perms;
+        {
+          std::lock_guard<std::mutex> g(mu);
+          consumed = i;
         }
         std::lock_guard<std::mutex> g(jmu);
         chained[s] = 1;
         advance();
       }
       ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
-      chain_wait_ms += ps.wait_ms();
     });
   }
-  // stops the threads; the context's stream continues where the chain stopped
-  void stop() {
+  // stops the mapper and the label threads (the chain keeps its entries)
+  void halt() {
     quit.store(true);
     if (th.joinable()) th.join();
     {
@@ -1691,14 +1711,25 @@ struct Producer {
     rcv.notify_all();
     for (auto &t : lth) t.join();
     lth.clear();
-    if (idx) {
-      ctx->stream_gen.restore(idx->save());
-      idx.reset();
+  }
+  // stops everything; the context's stream continues after the permutations
+  // of the sets the mapper has passed
+  void stop() {
+    halt();
+    if (chain) {
+      long long c;
+      {
+        std::lock_guard<std::mutex> g(mu);
+        c = consumed;
+      }
+      if (chain->wait(c)) ctx->stream_gen.restore(chain->start_state(c));
+      fallback = chain->fallback_steps();
+      chain.reset();
     }
   }
   // Marks sets `qs` as stopping after permutation 0; returns the first of
-  // them whose flag the thread has already read (its later draws went to the
-  // wrong sets), or -1.
+  // them whose flag the mapper has already read (its later entries went to
+  // the wrong sets), or -1.
   int mark_stop0(const std::vector<int> &qs) {
     std::lock_guard<std::mutex> g(mu);
     int first = -1;
@@ -1708,13 +1739,12 @@ struct Producer {
     }
     return first;
   }
-  // Restarts the replay after set q's permutation 0 (no set after q may
-  // have been consumed yet).
+  // Restarts the mapper after set q (a stop: its permutation 0 only; no set
+  // after q may have been consumed yet).
   void replay_after(int q) {
-    stop();
-    ctx->stream_gen.restore(after0[q]);
+    halt();
     for (int k = q; k < nsets; k++) std::vector<uint32_t>().swap(lab[k]);
-    start(q + 1);
+    start(q + 1, idx0[q] + 1);
   }
   ~Producer() { stop(); }
 };
@@ -1798,8 +1828,7 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     pr.stop0.assign(nsets, 0);
     for (int s = 0; s < nsets; s++) pr.stop0[s] = meta[s].G < 2;
     pr.lab.assign(nsets, std::vector<uint32_t>());
-    pr.after0.resize(nsets);
-    pr.start(0);
+    pr.begin();
   }
   CHK(p0_begin(ctx, meta, p));
   GROW(b_js, sizeof(double) * (size_t)std::max(1, nsets) * perms);
@@ -2213,8 +2242,10 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     prod.stop();
     ctx->timing.host_stream_ms += prod.ms;
     if (getenv("EDDY_PROFILE"))
-      fprintf(stderr, "[eddy] replay: chain %.1f ms (waiting for superblocks %.1f), label threads busy %.1f ms\n",
-              prod.ms, prod.chain_wait_ms, 1e-3 * (double)prod.label_us.load());
+      fprintf(stderr,
+              "[eddy] replay: mapper %.1f ms (waiting for the chain %.1f), label threads busy %.1f ms, chain "
+              "entries walked on from the validated end %lld\n",
+              prod.ms, prod.chain_wait_ms, 1e-3 * (double)prod.label_us.load(), prod.fallback);
     for (int k = 0; k < EDDY_SLOTS; k++)
       if (slot_ev[k]) hipEventDestroy(slot_ev[k]);
     for (int q = 0; q < collect_upto; q++) r.tests_collect += tests_eval[q];
