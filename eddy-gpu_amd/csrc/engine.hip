@@ -90,6 +90,10 @@ struct Chunk {
 // an item's first event capacity (grown on overflow): the size formula, or
 // more when the set's permutation-0 item showed more events (`hint`)
 inline int item_evcap(int S, int G, int hint) { return std::max(std::max(512, 2 * S + 24 * G), hint); }
+// permutation 0 (the true classes) has no earlier count to go by and usually
+// more events than a permuted split: twice the default, so that an overflow
+// (a second launch of the item alone, waited for) stays rare
+inline int perm0_evcap(int S, int G) { return 2 * item_evcap(S, G, 0); }
 
 // device bytes a chunk's per-item buffers take, roughly: per network slot
 // (hashes, representatives, class lists, likelihoods, score tables and the
@@ -1258,7 +1262,7 @@ int start_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, Chunk
   for (int i = 0; i < nitems; i++) {
     const ItemMeta &im = sl.ch.items[i];
     const SetMeta &sm = sets[im.set];
-    const int hint = sl.ch.last ? sm.evhint + sm.evhint / 2 : sm.evhint;
+    const int hint = im.perm == 0 ? perm0_evcap(S, sm.G) : sl.ch.last ? sm.evhint + sm.evhint / 2 : sm.evhint;
     sl.cap[i] = std::max(item_evcap(S, sm.G, hint), im.capmin);
   }
   if (nitems == 0) return EDDY_OK;
@@ -1365,7 +1369,7 @@ void add_item(Chunk &ch, const std::vector<SetMeta> &sets, int set, int perm, co
   ch.pairs += sm.c;
   ch.sslots += N + 2;
   ch.gslots += sm.G;
-  ch.evslots += item_evcap(N + 2, sm.G, sm.evhint);
+  ch.evslots += perm == 0 ? perm0_evcap(N + 2, sm.G) : item_evcap(N + 2, sm.G, sm.evhint);
 }
 
 std::vector<uint32_t> label_bits(const std::vector<uint8_t> &cls) {
@@ -1598,13 +1602,14 @@ struct Producer {
     ready.store(scan, std::memory_order_release);
     if (scan != was) rcv.notify_all();
   }
-  // waits until set s is ready (the caller's thread sleeps: the replay's
-  // threads need the cores)
-  void wait_ready(int s) {
-    for (int i = 0; i < 256; i++)
-      if (ready.load(std::memory_order_acquire) > s) return;
+  // waits at most `us` microseconds for set s (the caller's thread sleeps:
+  // the replay's threads need the cores); true once it is ready
+  bool wait_ready_for(int s, int us) {
+    if (ready.load(std::memory_order_acquire) > s) return true;
     std::unique_lock<std::mutex> g(jmu);
-    rcv.wait(g, [&]() { return ready.load(std::memory_order_acquire) > s || lquit; });
+    return rcv.wait_for(g, std::chrono::microseconds(us),
+                        [&]() { return ready.load(std::memory_order_acquire) > s || lquit; }) &&
+           ready.load(std::memory_order_acquire) > s;
   }
   void publish(Job &&j) {
     {
@@ -1909,6 +1914,12 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
     for (int q = 0; q < nsets; q++) nan0[q] = meta[q].G < 2;
     const char *pe = getenv("EDDY_PROFILE");
     const bool trace = pe && pe[0] == '2';
+    // the trace's GPU clock: chunk spans against an event recorded now
+    const double trace_t0 = trace ? std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() : 0.0;
+    if (trace) {
+      HIPCHK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+      HIPCHK(ctx, hipEventSynchronize(ctx->ev0));
+    }
     auto now_ms = [&]() {
       return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     };
@@ -1978,7 +1989,13 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
           SetMeta &sm = meta[sl.ch.items[i].set];
           sm.evhint = std::max(sm.evhint, evn[i] + evn[i] / 2 + 64);
         }
-      if (trace) fprintf(stderr, "[eddy] t=%.1f finished sets %d..%d\n", now_ms(), sl.first, sl.last);
+      if (trace) {
+        float g0 = 0.f, g1 = 0.f;
+        hipEventElapsedTime(&g0, ctx->ev0, sl.tev[0]);
+        hipEventElapsedTime(&g1, ctx->ev0, sl.tev[4]);
+        fprintf(stderr, "[eddy] t=%.1f finished sets %d..%d (on the GPU %.1f..%.1f)\n", now_ms(), sl.first, sl.last,
+                trace_t0 + g0, trace_t0 + g1);
+      }
       for (size_t i = 0; i < sl.ch.items.size(); i++) {
         tests_eval[sl.ch.items[i].set] += t[i];
         if (slot_p0[k]) tests0[sl.ch.items[i].set] += t[i];
@@ -2225,9 +2242,22 @@ static int run_job(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const ed
       }
       // next set's items
       auto hw = std::chrono::steady_clock::now();
-      prod.wait_ready(s);
+      if (prod.ready.load(std::memory_order_acquire) <= s) {
+        // its labels are not ready: meanwhile a chunk the GPU has finished is
+        // taken in (its slot can take the next launch, and a permutation-0
+        // chunk's stops are known sooner)
+        if (!inflight.empty() && hipEventQuery(ctx->slot[inflight.front()].done) == hipSuccess) {
+          std::vector<int> stops;
+          bool rolled;
+          CHK(finish_oldest(stops));
+          CHK(apply_stops(stops, rolled));
+          continue;
+        }
+        const bool ok = prod.wait_ready_for(s, 200);
+        ctx->hp.wait += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - hw).count();
+        if (!ok) continue;
+      }
       auto hb = std::chrono::steady_clock::now();
-      ctx->hp.wait += std::chrono::duration<double, std::milli>(hb - hw).count();
       if (!nan0[s]) {
         const std::vector<uint32_t> &L = prod.lab[s];
         for (int pm = 1; pm < perms; pm++) {

@@ -15,7 +15,8 @@
 //
 // The chain is walked in segments on several threads at once.  A segment's
 // walk starts at a guess (a few standard deviations before where its first
-// permutation is expected to start), generates its own stretch of the stream
+// permutation is expected to start, judged from the last validated start when
+// the segment is taken up), generates its own stretch of the stream
 // (the recurrence is linear over Z/2^32, so any position is one product of
 // 34x34 matrix powers away from the start) and records the start positions
 // it passes and the state at each one's first kept draw.  Walks from
@@ -609,38 +610,47 @@ class PermChain {
     std::atomic<int> done{0};
   };
 
-  // Segments of growing length (short ones first, so the first sets are
-  // ready early).  Guesses lie 4.5 standard deviations of the summed
-  // permutation lengths before the expected start; walks go on past the
-  // next segment's latest likely start by kExtend permutations.
-  static const int kExtend = 64;
+  // Segments: a short first one from the chain's start, then 256, 512 and
+  // 1,024 permutations (several of each, so that many threads work near the
+  // validated end from the start), then 2,048.  A walk's guess is taken when
+  // its segment is claimed, from the last validated start: 4.5 standard
+  // deviations of the summed lengths before the expected start; it goes on
+  // past the next segment's latest likely start by kExtend permutations.
+  static const int kExtend = 40;
   void plan() {
     const int n = g_.n;
     double H = 0;
     for (int k = 1; k <= n; k++) H += 1.0 / k;
-    const double mean = n + n * H;                           // draws per permutation
-    const double sd = n * 3.14159265358979 / std::sqrt(6.0);  // of one permutation's length
-    auto est = [&](long long c) { return (double)c * mean; };
-    auto lead = [&](long long c) { return 4.5 * sd * std::sqrt((double)c) + 2.0 * mean; };
+    mean_ = n + n * H;                                     // draws per permutation
+    sd_ = n * 3.14159265358979 / std::sqrt(6.0);           // of one permutation's length
     std::vector<long long> bounds{0};
-    long long len = 64;
-    while (bounds.back() < count_ + 1) {
+    const long long sizes[] = {64, 256, 256, 256, 256, 512, 512, 512, 512, 1024, 1024, 1024, 1024};
+    for (int i = 0; bounds.back() < count_ + 1; i++) {
+      const long long len = i < (int)(sizeof(sizes) / sizeof(sizes[0])) ? sizes[i] : 2048;
       bounds.push_back(std::min(count_ + 1, bounds.back() + len));
-      len = std::min<long long>(2048, len * 2);
     }
     nseg_ = (int)bounds.size() - 1;
     segs_plan_.resize(nseg_);
     for (int j = 0; j < nseg_; j++) {
-      Plan &q = segs_plan_[j];
-      q.c0 = bounds[j];
-      q.c1 = bounds[j + 1];
-      q.from = j == 0 ? 0 : std::max(0LL, (long long)(est(q.c0) - lead(q.c0)));
-      q.until = (long long)(est(q.c1) + lead(q.c1) + kExtend * mean);
+      segs_plan_[j].c0 = bounds[j];
+      segs_plan_[j].c1 = bounds[j + 1];
     }
   }
   struct Plan {
-    long long c0, c1, from, until;
+    long long c0, c1;
   };
+  // where a walk for chain entries [c0, c1) starts and stops, judged from
+  // the validated entries so far
+  void guess(long long c0, long long c1, long long &from, long long &until) const {
+    const long long V = validated();
+    const long long v0 = V > 0 ? V - 1 : 0;          // a validated entry
+    const double p0 = V > 0 ? (double)pos_[v0] : 0.0;  // its start
+    auto est = [&](long long c) { return p0 + (double)(c - v0) * mean_; };
+    auto lead = [&](long long c) { return 4.5 * sd_ * std::sqrt((double)std::max(0LL, c - v0)) + 2.0 * mean_; };
+    from = std::max((long long)p0, (long long)(est(c0) - lead(c0)));
+    until = (long long)(est(c1) + lead(c1) + kExtend * mean_);
+  }
+  double mean_ = 0, sd_ = 0;
 
   void seg_loop() {
 #if defined(__linux__)
@@ -656,9 +666,13 @@ class PermChain {
       const Plan &q = segs_plan_[j];
       s.c0 = q.c0;
       s.c1 = q.c1;
-      s.from = q.from;
-      s.until = q.until;
-      long long p = q.from;
+      if (j == 0) {
+        s.from = 0;
+        s.until = 0;
+      } else {
+        guess(q.c0, q.c1, s.from, s.until);
+      }
+      long long p = s.from;
       // segment 0 starts at the chain's start: it needs only its own entries
       const long long cap = j == 0 ? q.c1 : -1;
       while (!quit_.load(std::memory_order_relaxed)) {
@@ -667,7 +681,7 @@ class PermChain {
         s.pos.push_back(p);
         s.kept.push_back(k);
         p = nx;
-        if (cap >= 0 ? (long long)s.pos.size() >= cap : p > q.until) break;
+        if (cap >= 0 ? (long long)s.pos.size() >= cap : p > s.until) break;
       }
       s.next = p;
       {
