@@ -158,12 +158,12 @@ def workflow4(args):
     print("bench: 20000 x 1200 workflow, %d sets, N=%d (%d/%d), setup %.1fs" % (len(wl.sets), wl.N, wl.cfg.nA, wl.cfg.nB,
                                                                    setup_s), file=sys.stderr, flush=True)
     eng.reset_stream()
-    eng.run_learned(wl.sets, prm, cfg)
+    eng.run_learned(wl.batch, prm, cfg)
     eng.reset_stream()
     t0 = time.perf_counter()
-    res, rep = eng.run_learned(wl.sets, prm, cfg)
+    res, rep = eng.run_learned(wl.batch, prm, cfg)
     t1 = time.perf_counter()
-    chosen, res2 = eng.repermute(wl.sets, res, prm, top=200, perms=1000)
+    chosen, res2 = eng.repermute(wl.batch, res, prm, top=200, perms=1000)
     t2 = time.perf_counter()
     print("bench: r=100 phase %.2fs, r=1000 phase %.2fs" % (t1 - t0, t2 - t1), file=sys.stderr, flush=True)
     # the r = 100 phase without the network (same stream, same results), the
@@ -175,14 +175,14 @@ def workflow4(args):
     for r in range(args.compare_rounds4):
         eng.reset_stream()
         ta = time.perf_counter()
-        ures = eng.run_sets(wl.sets, uprm)
+        ures = eng.run_sets(wl.batch, uprm)
         tu.append(time.perf_counter() - ta)
         same = same and all((a.status, a.perms_run, a.U0) == (b.status, b.perms_run, b.U0) and
                             (a.status != eddy_amd.SET_OK or a.pDDN == b.pDDN) for a, b in zip(ures, res))
         if r + 1 < args.compare_rounds4:
             eng.reset_stream()
             ta = time.perf_counter()
-            eng.run_learned(wl.sets, prm, cfg)
+            eng.run_learned(wl.batch, prm, cfg)
             tl.append(time.perf_counter() - ta)
     med = lambda v: sorted(v)[len(v) // 2]
     tests_u = sum(r.tests_eval for r in ures[k:])
@@ -210,7 +210,7 @@ def workflow4(args):
 
 def run_step(eng, wl, prm, cfg):
     eng.reset_stream()
-    res, rep = eng.run_learned(wl.sets, prm, cfg)
+    res, rep = eng.run_learned(wl.batch, prm, cfg)
     return res, rep, eng.timing()
 
 
@@ -483,12 +483,12 @@ def main():
             same = same and same_as_learned(lres)
             eng.reset_stream()
             t0 = time.perf_counter()
-            ures = eng.run_sets(wl.sets, uprm)
+            ures = eng.run_sets(wl.batch, uprm)
             tu.append(time.perf_counter() - t0)
             same = same and same_as_learned(ures)
             eng.reset_stream()
             t0 = time.perf_counter()
-            zres = eng.run_sets(wl.sets, zprm)
+            zres = eng.run_sets(wl.batch, zprm)
             tz.append(time.perf_counter() - t0)
             same = same and same_as_learned(zres)
         med = lambda v: sorted(v)[len(v) // 2]

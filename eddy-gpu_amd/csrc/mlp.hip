@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "mlp.h"
 
 #define MB 64          // samples per workgroup
@@ -23,16 +25,11 @@
 __device__ __forceinline__ float relu(float x) { return x > 0.f ? x : 0.f; }
 __device__ __forceinline__ float sigmoidf_(float z) { return 1.f / (1.f + __expf(-z)); }
 
-// inference: one thread per sample, parameters in LDS, hidden layer 1 in registers
-__global__ __launch_bounds__(256) void k_mlp_forward(const float *X, int n, const float *params, float *out) {
-  __shared__ float P[MLP_NPARAMS];
-  for (int i = threadIdx.x; i < MLP_NPARAMS; i += blockDim.x) P[i] = params[i];
-  __syncthreads();
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n) return;
-  float x[MLP_IN];
-#pragma unroll
-  for (int i = 0; i < MLP_IN; i++) x[i] = X[(size_t)b * MLP_IN + i];
+// The logit of one sample (features x), the parameters read at wave-uniform
+// addresses (every lane takes the same weight at the same step: scalar loads,
+// no LDS copy), hidden layer 1 in registers; sums in the order of the
+// parameter layout.
+__device__ __forceinline__ float mlp_logit(const float *__restrict__ P, const float *x) {
   float a1[MLP_H];
 #pragma unroll
   for (int j = 0; j < MLP_H; j++) {
@@ -42,13 +39,25 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float *X, int n, cons
     a1[j] = relu(s);
   }
   float z = P[MLP_OFF_B3];
+#pragma unroll 2
   for (int k = 0; k < MLP_H; k++) {
     float s = P[MLP_OFF_B2 + k];
 #pragma unroll
     for (int j = 0; j < MLP_H; j++) s += P[MLP_OFF_W2 + k * MLP_H + j] * a1[j];
     z += P[MLP_OFF_W3 + k] * relu(s);
   }
-  out[b] = sigmoidf_(z);
+  return z;
+}
+
+// inference: one thread per sample, the grid striding over the samples
+__global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ X, int n, const float *__restrict__ P,
+                                                     float *__restrict__ out) {
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < n; b += gridDim.x * blockDim.x) {
+    float x[MLP_IN];
+#pragma unroll
+    for (int i = 0; i < MLP_IN; i++) x[i] = X[(size_t)b * MLP_IN + i];
+    out[b] = sigmoidf_(mlp_logit(P, x));
+  }
 }
 
 // One workgroup = MB samples (rows idx[b0 + b] of X/Y when idx is given, else
@@ -152,35 +161,18 @@ __global__ __launch_bounds__(256) void k_mlp_part(const float *X, const float *Y
   }
 }
 
-// Loss only (no gradient): one thread per sample of rows 0..n-1, parameters
-// in LDS, the block's 256 losses summed in sample order into part[blk].
-__global__ __launch_bounds__(256) void k_mlp_loss_part(const float *X, const float *Y, int n, const float *params,
-                                                       float *part) {
-  __shared__ float P[MLP_NPARAMS];
+// Loss only (no gradient): one thread per sample of rows 0..n-1, the block's
+// 256 losses summed in sample order into part[blk].
+__global__ __launch_bounds__(256) void k_mlp_loss_part(const float *__restrict__ X, const float *__restrict__ Y, int n,
+                                                       const float *__restrict__ P, float *part) {
   __shared__ float L[256];
-  for (int i = threadIdx.x; i < MLP_NPARAMS; i += blockDim.x) P[i] = params[i];
-  __syncthreads();
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   float l = 0.f;
   if (b < n) {
     float x[MLP_IN];
 #pragma unroll
     for (int i = 0; i < MLP_IN; i++) x[i] = X[(size_t)b * MLP_IN + i];
-    float a1[MLP_H];
-#pragma unroll
-    for (int j = 0; j < MLP_H; j++) {
-      float s = P[MLP_OFF_B1 + j];
-#pragma unroll
-      for (int i = 0; i < MLP_IN; i++) s += P[MLP_OFF_W1 + j * MLP_IN + i] * x[i];
-      a1[j] = relu(s);
-    }
-    float z = P[MLP_OFF_B3];
-    for (int k = 0; k < MLP_H; k++) {
-      float s = P[MLP_OFF_B2 + k];
-#pragma unroll
-      for (int j = 0; j < MLP_H; j++) s += P[MLP_OFF_W2 + k * MLP_H + j] * a1[j];
-      z += P[MLP_OFF_W3 + k] * relu(s);
-    }
+    const float z = mlp_logit(P, x);
     // BCE(sigmoid(z), y) = softplus(z) - y z, as k_mlp_part
     const float y = Y[b];
     const float sp = z > 0.f ? z + log1pf(__expf(-z)) : log1pf(__expf(z));
@@ -323,7 +315,8 @@ size_t mlp_part_floats(int n) { return (size_t)((n + MB - 1) / MB) * (MLP_NPARAM
 
 hipError_t mlp_forward(const float *dX, int n, const float *dP, float *dOut, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  k_mlp_forward<<<(n + 255) / 256, 256, 0, st>>>(dX, n, dP, dOut);
+  // enough workgroups to fill the device; each strides over the samples
+  k_mlp_forward<<<std::min((n + 255) / 256, 4096), 256, 0, st>>>(dX, n, dP, dOut);
   return hipGetLastError();
 }
 

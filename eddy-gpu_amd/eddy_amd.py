@@ -218,6 +218,13 @@ class Workload:
             go += G
             po += P
 
+    @property
+    def batch(self):
+        """The sets as a SetBatch (marshalled on first use)."""
+        if getattr(self, "_batch", None) is None:
+            self._batch = SetBatch(self.sets)
+        return self._batch
+
     def params(self, perms=None, mode=MODE_BATCHED, schedule=TESTS_CUTOFF, learned=0, p_adjust=1,
                alpha_edge=0.05, prior_weight=None, lam=2.0, alpha_ddn=0.05, mp=None):
         pw = self.cfg.prior_weight if prior_weight is None else prior_weight
@@ -235,6 +242,25 @@ class Workload:
 
 
 # ---------------------------------------------------------------- engine
+class SetBatch:
+    """Gene sets marshalled once into the C descriptors (eddy_set_desc) and
+    the arrays they point into, for repeated calls on the same sets; the
+    engine's calls take it wherever they take a list of (genes, pairs)."""
+
+    def __init__(self, sets):
+        self.sets = list(sets)
+        self.arr, self.keep = Engine._descs(self.sets)
+
+    def __len__(self):
+        return len(self.sets)
+
+    def __iter__(self):
+        return iter(self.sets)
+
+    def __getitem__(self, i):
+        return self.sets[i]
+
+
 class Engine:
     def __init__(self, device=0):
         L = lib()
@@ -277,6 +303,8 @@ class Engine:
 
     @staticmethod
     def _descs(sets):
+        if isinstance(sets, SetBatch):
+            return sets.arr, sets.keep
         keep = []
         arr = (SetDesc * max(1, len(sets)))()
         for i, (genes, pairs) in enumerate(sets):

@@ -196,3 +196,18 @@ def test_launch_config_is_validated_without_a_gpu():
         assert L.eddy_autotune(h, None, 0, C.byref(prm), None, C.byref(c), C.byref(rep)) == eddy_amd.EDDY_ERR_NODEV
     finally:
         L.eddy_ctx_destroy(h)
+
+
+def test_set_batch_marshals_like_a_list():
+    """A SetBatch holds the same C descriptors a call builds from the list of
+    (genes, pairs) itself, and keeps the arrays they point into alive."""
+    sets = [(np.arange(5) * 3, np.array([[0, 1], [2, 4]])), ([7, 8, 9], []), (np.array([1, 2]), [[1, 0]])]
+    b = eddy_amd.SetBatch(sets)
+    arr, keep = eddy_amd.Engine._descs(sets)
+    assert len(b) == 3 and [len(g) for g, _ in b] == [5, 3, 2]
+    for i in range(3):
+        assert b.arr[i].n_genes == arr[i].n_genes and b.arr[i].n_prior == arr[i].n_prior
+        assert [b.arr[i].genes[k] for k in range(arr[i].n_genes)] == [arr[i].genes[k] for k in range(arr[i].n_genes)]
+        assert [b.arr[i].prior[k] for k in range(2 * arr[i].n_prior)] == \
+               [arr[i].prior[k] for k in range(2 * arr[i].n_prior)]
+    assert eddy_amd.Engine._descs(b)[0] is b.arr
