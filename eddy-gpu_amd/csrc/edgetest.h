@@ -261,33 +261,85 @@ EDDY_HD float eddy_S32(const int *t, const int *R, const int *K) {
 // |x32 - x| <= 0.5 n 44u + u |x| < 1.4e-6 n + 6e-8 |x|; the guard doubles it.
 EDDY_HD double eddy_x32_guard(int n, double x) { return 3e-6 * (double)(n + 1) + 1e-6 * eddy_fabs(x); }
 
+// The screen's thresholds on x32 for tables of n samples: per prior flag and
+// dof (1, 2, 4) {A, B, C, D}; x32 <= A or x32 > B: certainly no edge,
+// C <= x32 < D: certainly an edge.  They fold the guard g = 3e-6 (n + 1) +
+// 1e-6 x (x >= 0) into the interval ends (x + g <= xlo  <=>  x <= (xlo - c) /
+// (1 + 1e-6) with c = 3e-6 (n + 1), and alike) and are rounded toward deciding
+// less: each is computed in double, moved 1e-12 of itself the safe way (more
+// than the double rounding of the quotient) and rounded to float in the same
+// direction, so x32 against them implies the guarded comparisons in real
+// arithmetic.  n < 0 (or the shortcut off) decides nothing.
+struct eddy_screen_thr {
+  float v[2][3][4];
+  int n;
+};
+EDDY_HD float eddy_f32_down(double v) {
+  v = v > 0.0 ? v * (1.0 - 1e-12) : v * (1.0 + 1e-12);
+  float f = (float)v;
+  if ((double)f > v) f = nextafterf(f, -INFINITY);
+  return f;
+}
+EDDY_HD float eddy_f32_up(double v) {
+  v = v > 0.0 ? v * (1.0 + 1e-12) : v * (1.0 - 1e-12);
+  float f = (float)v;
+  if ((double)f < v) f = nextafterf(f, INFINITY);
+  return f;
+}
+EDDY_HD void eddy_screen_thresholds(const eddy_edge_params *ep, const eddy_edge_bounds *eb, int n,
+                                    eddy_screen_thr *th) {
+  const double c = 3e-6 * (double)(n + 1), up = 1.0 + 1e-6, dn = 1.0 - 1e-6;
+  for (int pr = 0; pr < 2; pr++)
+    for (int d = 0; d < 3; d++) {
+      float *T = th->v[pr][d];
+      T[0] = eddy_f32_down((eb->xlo[pr][d] - c) / up);
+      T[1] = eddy_f32_up((ep->xchi[d] + c) / dn);
+      T[2] = eddy_f32_up((eb->xhi[pr][d] + c) / dn);
+      T[3] = eddy_f32_down((ep->xclo[d] - c) / up);
+    }
+  th->n = ep->shortcut && n > 0 ? n : -1;
+}
+
 // The full test's decision from x in single precision: 1 / 0 when certain,
-// -1 otherwise (Yates' rule, other dofs, or x near an interval end).
-EDDY_HD int eddy_edge_screen32(const int *t, const int *R, const int *K, int n, int prior,
-                               const eddy_edge_params *ep, const eddy_edge_bounds *eb) {
-  if (!ep->shortcut || n <= 0) return -1;
-  int zr = 0, zk = 0, yates = 0;
+// -1 otherwise (Yates' rule, other dofs, x near an interval end, or tables of
+// another sample count than the thresholds').  x32 is clamped at 0: the exact
+// x is not negative, so the clamp only moves x32 toward it.
+EDDY_HD int eddy_edge_screen32t(const int *t, const int *R, const int *K, int n, int prior,
+                                const eddy_screen_thr *th) {
+  if (n != th->n) return -1;
+  int zr = 0, zk = 0, rmin = 0x7fffffff, kmin = 0x7fffffff;
   for (int i = 0; i < 3; i++) {
     zr += R[i] == 0;
     zk += K[i] == 0;
+    if (R[i] > 0 && R[i] < rmin) rmin = R[i];
+    if (K[i] > 0 && K[i] < kmin) kmin = K[i];
   }
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++)
-      if (R[i] && K[j] && R[i] * K[j] < 5 * n) yates = 1;
+  // Yates' rule: some cell of a nonempty row and column expects fewer than 5
+  // (the smallest such product is the product of the smallest margins)
   const int dof = (2 - zk) * (2 - zr);
-  if (yates || !(dof == 1 || dof == 2 || dof == 4)) return -1;
-  const float x32 = 0.5f * ((float)n * (eddy_S32(t, R, K) - 1.0f));
-  const double x = (double)x32, g = eddy_x32_guard(n, x);
+  if (rmin * kmin < 5 * n || !(dof == 1 || dof == 2 || dof == 4)) return -1;
+  float x32 = 0.5f * ((float)n * (eddy_S32(t, R, K) - 1.0f));
+  x32 = x32 > 0.0f ? x32 : 0.0f;
+  // every lane reads the same rows (broadcast loads when th is in LDS), then
+  // picks its own prior flag and dof
   const bool p1 = dof == 1, p2 = dof == 2, pr = prior == 1;
-  const double xlo = pr ? eddy_pick3(eb->xlo[1], p1, p2) : eddy_pick3(eb->xlo[0], p1, p2);
-  const double xhi = pr ? eddy_pick3(eb->xhi[1], p1, p2) : eddy_pick3(eb->xhi[0], p1, p2);
-  const double xchi = eddy_pick3(ep->xchi, p1, p2);
-  const double xclo = eddy_pick3(ep->xclo, p1, p2);
-  // x_exact lies in [x - g, x + g]
-  if (x + g <= xlo) return 0;
-  if (x - g > xchi) return 0;
-  if (x - g >= xhi && x + g < xclo) return 1;
+  float T[4];
+  for (int k = 0; k < 4; k++) {
+    const float a = p1 ? th->v[0][0][k] : p2 ? th->v[0][1][k] : th->v[0][2][k];
+    const float b = p1 ? th->v[1][0][k] : p2 ? th->v[1][1][k] : th->v[1][2][k];
+    T[k] = pr ? b : a;
+  }
+  if (x32 <= T[0] || x32 > T[1]) return 0;
+  if (x32 >= T[2] && x32 < T[3]) return 1;
   return -1;
+}
+
+// The same with the thresholds made here (one table's screen)
+EDDY_HD int eddy_edge_screen32(const int *t, const int *R, const int *K, int n, int prior,
+                               const eddy_edge_params *ep, const eddy_edge_bounds *eb) {
+  eddy_screen_thr th;
+  eddy_screen_thresholds(ep, eb, n, &th);
+  return eddy_edge_screen32t(t, R, K, n, prior, &th);
 }
 
 // eddy_edge_decision() with the fast x
@@ -313,15 +365,16 @@ EDDY_HD int eddy_edge_decision_fast(const int *t, int prior, const eddy_edge_par
 
 // eddy_edge_decision_fast() behind the single-precision screen: the count
 // kernel's full tests, where the two FP64 divisions per test are most of the
-// test's cost and the screen decides nearly every table
+// test's cost and the screen decides nearly every table; th holds the
+// thresholds for the class's sample count
 EDDY_HD int eddy_edge_decision_screened(const int *t, int prior, const eddy_edge_params *ep,
-                                        const eddy_edge_bounds *eb, double cpairs) {
+                                        const eddy_edge_bounds *eb, double cpairs, const eddy_screen_thr *th) {
   int R[3], K[3];
   for (int i = 0; i < 3; i++) {
     R[i] = t[i * 3] + t[i * 3 + 1] + t[i * 3 + 2];
     K[i] = t[i] + t[3 + i] + t[6 + i];
   }
-  const int c32 = eddy_edge_screen32(t, R, K, R[0] + R[1] + R[2], prior, ep, eb);
+  const int c32 = eddy_edge_screen32t(t, R, K, R[0] + R[1] + R[2], prior, th);
   if (c32 >= 0) return c32;
   return eddy_edge_decision_fast(t, prior, ep, eb, cpairs);
 }

@@ -212,6 +212,7 @@ struct eddy_ctx {
   size_t h_prep_cap[2] = {};
   // learned proposal: network parameters + Adam state, per-run feature,
   // label and window arrays (device pointers handed to the kernels)
+  DevBuf b_thr;
   DevBuf b_mlp, b_mlpm, b_mlpv, b_mlpg, b_mlploss, b_feat, b_labhits, b_score, b_win, b_mx, b_my, b_mxb, b_myb,
       b_midx, b_mscr, b_mlbase, b_mlinv, b_win2;
   bool mlp_ready = false;
@@ -253,7 +254,7 @@ static int list_bufs(eddy_ctx *ctx, DevBuf **out) {
                    &ctx->b_mx, &ctx->b_my, &ctx->b_mxb, &ctx->b_myb, &ctx->b_midx, &ctx->b_mscr, &ctx->b_mlbase,
                    &ctx->b_mlinv, &ctx->p0.info, &ctx->p0.lcnt, &ctx->p0.lfull, &ctx->p0.rep, &ctx->p0.lval,
                    &ctx->p0.evkey, &ctx->p0.evpar, &ctx->p0.famkey, &ctx->p0.famscore, &ctx->p0.slices, &ctx->b_bits,
-                   &ctx->b_win2, &ctx->b_order, &ctx->b_js0, &ctx->b_totals};
+                   &ctx->b_win2, &ctx->b_order, &ctx->b_js0, &ctx->b_totals, &ctx->b_thr};
   int n = 0;
   for (DevBuf *b : all) out[n++] = b;
   for (Slot &sl : ctx->slot) {
@@ -818,6 +819,7 @@ int launch_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, cons
   a.lgtab = ctx->dLG;
   a.lgstride = ctx->lgstride;
   a.hseed = 0x243F6A8885A308D3ULL + (uint64_t)attempt * 0x9E3779B97F4A7C15ULL;
+  a.thr = (const eddy_screen_thr *)ctx->b_thr.p;
   a.sets = (const SetMeta *)ctx->b_sets.p;
   a.xs = (const uint8_t *)ctx->b_xs.p;
   a.xc = gather1 ? (uint8_t *)sl.xc.p : nullptr;
@@ -1462,8 +1464,19 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
       prior[m.poff + (pidx >> 5)] |= 1u << (pidx & 31);
     }
   }
+  // the count kernel's screen thresholds per set, for the two class sizes
+  std::vector<eddy_screen_thr> thr((size_t)2 * std::max(1, nsets));
+  {
+    eddy_edge_params ep;
+    edge_params(ctx, p, ep);
+    for (int s = 0; s < nsets; s++) {
+      eddy_screen_thresholds(&ep, &meta[s].eb, ctx->n1, &thr[2 * s]);
+      eddy_screen_thresholds(&ep, &meta[s].eb, ctx->n2, &thr[2 * s + 1]);
+    }
+  }
   mark(1);
   GROW(b_sets, sizeof(SetMeta) * nsets);
+  GROW(b_thr, sizeof(eddy_screen_thr) * thr.size());
   GROW(b_xs, xoff);
   GROW(b_prior, sizeof(uint32_t) * prior.size());
   GROW(b_genes, sizeof(int) * rows.size());
@@ -1472,6 +1485,7 @@ int prepare_sets(eddy_ctx *ctx, const eddy_set_desc *sets, int nsets, const eddy
   {
     Uploads up;
     up.add(ctx->b_sets.p, meta.data(), sizeof(SetMeta) * nsets);
+    up.add(ctx->b_thr.p, thr.data(), sizeof(eddy_screen_thr) * thr.size());
     up.add(ctx->b_prior.p, prior.data(), sizeof(uint32_t) * prior.size());
     up.add(ctx->b_genes.p, rows.data(), sizeof(int) * rows.size());
     up.add(ctx->b_rowoff.p, rowoff.data(), sizeof(long long) * rowoff.size());

@@ -98,6 +98,7 @@ struct KArgs {
   uint64_t hseed;
   // sets
   const SetMeta *sets;
+  const eddy_screen_thr *thr;  // per set: the full tests' screen thresholds for class 1's and class 2's sizes
   const uint8_t *xs;       // gathered set rows
   uint8_t *xc;             // [(goff + g) * kp + k]: each item's rows at its class-1 samples (cutoff schedule)
   int kp;                  // class-1 samples rounded up to 64

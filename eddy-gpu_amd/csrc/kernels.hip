@@ -294,6 +294,7 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
   __shared__ uint32_t sh_pb[K1_TB];
   __shared__ int sh_win[K1_TB];
   __shared__ eddy_edge_bounds sh_eb;
+  __shared__ eddy_screen_thr sh_thr[2];  // the screen's thresholds for class 1's and class 2's sizes
   if (tid < K1_TB) {
     const int gi = bi * K1_TB + tid;
     uint32_t pb = 0u;
@@ -315,6 +316,13 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
     sh_win[tid] = wv;
   } else if (tid >= 64 && tid < 64 + 12) {
     ((double *)&sh_eb)[tid - 64] = ((const double *)&sm.eb)[tid - 64];
+  } else if (MODE != K1_POOLED && tid >= 128 && tid < 128 + (int)(2 * sizeof(eddy_screen_thr) / 4)) {
+    static_assert(sizeof(eddy_screen_thr) % 4 == 0, "word copy");
+    const int w = tid - 128;
+    uint32_t v = 0xffffffffu;  // n = -1 in the last word: no thresholds, decide nothing
+    if (a.thr) v = ((const uint32_t *)(a.thr + 2 * im.set))[w];
+    else if (w % (sizeof(eddy_screen_thr) / 4) != sizeof(eddy_screen_thr) / 4 - 1) v = 0u;
+    ((uint32_t *)sh_thr)[w] = v;
   }
 
   v4i acc1[3][3], acc2[3][3];
@@ -480,8 +488,8 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
     uint32_t info = 0;
     const bool logged = im.perm > 0 && im.perm <= a.label_perms;
     if (!a.win || im.perm == 0 || logged || gj <= sh_win[li]) {
-      f1 = eddy_edge_decision_screened(t1, pr, &a.ep, &sh_eb, cpairs);
-      f2 = eddy_edge_decision_screened(t2, pr, &a.ep, &sh_eb, cpairs);
+      f1 = eddy_edge_decision_screened(t1, pr, &a.ep, &sh_eb, cpairs, &sh_thr[0]);
+      f2 = eddy_edge_decision_screened(t2, pr, &a.ep, &sh_eb, cpairs, &sh_thr[1]);
       ntests += 2;
       info = (f1 ? INFO_FULL(0) : 0u) | (f2 ? INFO_FULL(1) : 0u) | INFO_FEVAL(0) | INFO_FEVAL(1);
       if (a.labhits && im.perm > 0 && !im.nolog && (a.label_perms == 0 || logged))
