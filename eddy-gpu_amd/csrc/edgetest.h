@@ -363,6 +363,17 @@ EDDY_HD int eddy_edge_decision_fast(const int *t, int prior, const eddy_edge_par
   return eddy_edge_decision(t, prior, ep, eb, cpairs);
 }
 
+// The screen alone: 1 / 0 when certain, -1 when the double-precision path
+// must decide
+EDDY_HD int eddy_edge_screened_only(const int *t, int prior, const eddy_screen_thr *th) {
+  int R[3], K[3];
+  for (int i = 0; i < 3; i++) {
+    R[i] = t[i * 3] + t[i * 3 + 1] + t[i * 3 + 2];
+    K[i] = t[i] + t[3 + i] + t[6 + i];
+  }
+  return eddy_edge_screen32t(t, R, K, R[0] + R[1] + R[2], prior, th);
+}
+
 // eddy_edge_decision_fast() behind the single-precision screen: the count
 // kernel's full tests, where the two FP64 divisions per test are most of the
 // test's cost and the screen decides nearly every table; th holds the

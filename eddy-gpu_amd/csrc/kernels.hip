@@ -271,6 +271,11 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
   constexpr int OPS_B = NOPS * 96 * K1_RS, TAB_B = NTAB * 96 * K1_TS * 2;
   __shared__ __align__(16) uint8_t rows[OPS_B > TAB_B ? OPS_B : TAB_B];
   __shared__ int tcount;
+  // full tests the screen leaves open (class-1 schedule): (pair << 1 | class),
+  // decided after the tile's pairs by one lane each, so the double-precision
+  // path does not hold a whole wave per undecided lane
+  __shared__ uint16_t undec[2 * K1_TB * K1_TB];
+  __shared__ int nundec;
   uint8_t *opA1 = rows, *opA2 = rows + 96 * K1_RS, *opB = rows + (NOPS - 1) * 96 * K1_RS;
 
   // class-1 membership of samples k..k+15 (k a multiple of 16) as one byte
@@ -285,7 +290,7 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
     return make_uint4((b & 15u) * spread & 0x01010101u, ((b >> 4) & 15u) * spread & 0x01010101u,
                       ((b >> 8) & 15u) * spread & 0x01010101u, ((b >> 12) & 15u) * spread & 0x01010101u);
   };
-  if (tid == 0) tcount = 0;
+  if (tid == 0) tcount = nundec = 0;
   unsigned long long tph = 0;
   phase_mark(a, -1, tph);
   // what the tests read besides the tallies, loaded once while the operands
@@ -488,8 +493,17 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
     uint32_t info = 0;
     const bool logged = im.perm > 0 && im.perm <= a.label_perms;
     if (!a.win || im.perm == 0 || logged || gj <= sh_win[li]) {
-      f1 = eddy_edge_decision_screened(t1, pr, &a.ep, &sh_eb, cpairs, &sh_thr[0]);
-      f2 = eddy_edge_decision_screened(t2, pr, &a.ep, &sh_eb, cpairs, &sh_thr[1]);
+      if (MODE == K1_CLASS1) {
+        f1 = eddy_edge_screened_only(t1, pr, &sh_thr[0]);
+        f2 = eddy_edge_screened_only(t2, pr, &sh_thr[1]);
+        if (f1 < 0) undec[atomicAdd(&nundec, 1)] = (uint16_t)(idx << 1);
+        if (f2 < 0) undec[atomicAdd(&nundec, 1)] = (uint16_t)(idx << 1 | 1);
+        f1 = f1 > 0;
+        f2 = f2 > 0;
+      } else {
+        f1 = eddy_edge_decision_screened(t1, pr, &a.ep, &sh_eb, cpairs, &sh_thr[0]);
+        f2 = eddy_edge_decision_screened(t2, pr, &a.ep, &sh_eb, cpairs, &sh_thr[1]);
+      }
       ntests += 2;
       info = (f1 ? INFO_FULL(0) : 0u) | (f2 ? INFO_FULL(1) : 0u) | INFO_FEVAL(0) | INFO_FEVAL(1);
       if (a.labhits && im.perm > 0 && !im.nolog && (a.label_perms == 0 || logged))
@@ -515,6 +529,39 @@ __device__ __forceinline__ void count_test_body(const KArgs &a) {
   }
   atomicAdd(&tcount, ntests);
   __syncthreads();
+  if (MODE == K1_CLASS1) {
+    // the open full tests, one lane each: an edge sets its bit in the info
+    // word and the full-edge word stored above, and counts for the labels
+    const bool logged = im.perm > 0 && im.perm <= a.label_perms;
+    const int nq = nundec;
+    for (int e = tid; e < nq; e += 256) {
+      const int idx = undec[e] >> 1, g = undec[e] & 1;
+      const int li = idx >> 5, lj = idx & 31, gi = bi * K1_TB + li, gj = bj * K1_TB + lj;
+      int t[9];
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) t[r * 3 + k] = T[(li * 3 + r) * K1_TS + lj * 3 + k];
+      const long long p = pair_index(gi, gj);
+      if (g) {
+        uint64_t q0, q1, q2;
+        load_tally_words(totals_at(a, sm, p), a.tabw, a.N, q0, q1, q2);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          t[i] = (int)((q0 >> (16 * i)) & 0xffff) - t[i];
+          t[4 + i] = (int)((q1 >> (16 * i)) & 0xffff) - t[4 + i];
+        }
+        t[8] = (int)(q2 & 0xffff) - t[8];
+      }
+      const int pr = (int)((sh_pb[li] >> lj) & 1u);
+      if (eddy_edge_decision_fast(t, pr, &a.ep, &sh_eb, cpairs)) {
+        atomicOr(&a.info[im.infooff + p], (uint32_t)INFO_FULL(g));
+        if (a.fbits) atomicOr(&a.fbits[(size_t)(im.goff * 2 + g * G + gi) * a.fbw + bj], 1u << lj);
+        if (a.labhits && im.perm > 0 && !im.nolog && (a.label_perms == 0 || logged))
+          atomicAdd(&a.labhits[sm.pairbase + p], 1u);
+      }
+    }
+  }
   if (tid == 0 && a.tests) atomicAdd((unsigned long long *)&a.tests[item], (unsigned long long)tcount);
   phase_mark(a, 14, tph);
 }
