@@ -1264,7 +1264,7 @@ int start_chunk(eddy_ctx *ctx, Slot &sl, const std::vector<SetMeta> &sets, Chunk
   for (int i = 0; i < nitems; i++) {
     const ItemMeta &im = sl.ch.items[i];
     const SetMeta &sm = sets[im.set];
-    const int hint = im.perm == 0 ? perm0_evcap(S, sm.G) : sl.ch.last ? sm.evhint + sm.evhint / 2 : sm.evhint;
+    const int hint = im.perm == 0 ? perm0_evcap(S, sm.G) : sl.ch.last ? 2 * sm.evhint : sm.evhint;
     sl.cap[i] = std::max(item_evcap(S, sm.G, hint), im.capmin);
   }
   if (nitems == 0) return EDDY_OK;
