@@ -307,7 +307,7 @@ def summarize_traffic(db_fetch, db_write, skip, copy_mib, json_path):
             r = res.setdefault(k, {"dispatches_per_job": disp[k] / jobs})
             r[key + "_bytes_per_job"] = cv.get(cn, 0.0) * 1024.0 * factors[key] / jobs
     kern = {}
-    out = ["scale factors: fetch %.3f, write %.3f (counter KiB per KiB of the %d MiB copy)"
+    out = ["scale factors: fetch %.3f, write %.3f (copied KiB per counted KiB, from the %d MiB copy)"
            % (factors["read"], factors["write"], copy_mib), "",
            "| kernel | dispatches per job | read MB per dispatch | written MB per dispatch | total MB per job |",
            "|---|---|---|---|---|"]
