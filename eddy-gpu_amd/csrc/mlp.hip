@@ -49,14 +49,73 @@ __device__ __forceinline__ float mlp_logit(const float *__restrict__ P, const fl
   return z;
 }
 
-// inference: one thread per sample, the grid striding over the samples
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Inference on the matrix cores.  v_mfma_f32_16x16x4f32 takes A (16 x 4) and
+// B (4 x 16) and accumulates D (16 x 16): lane l holds A[l % 16][l / 16],
+// B[l / 16][l % 16] and, in register r, D[4 (l / 16) + r][l % 16].  Each wave
+// takes 16 samples at a time and computes the layers transposed, units x
+// samples (D = W . H), so the 4 x 4 values a lane holds of layer 1's output
+// (units 16m + 4 (l / 16) + r of sample l % 16) are already layer 2's B
+// operand once contraction step (m, r) lets its index k = l / 16 stand for
+// input unit 16m + 4k + r; A then takes W2's columns in that same order.  The
+// weights stay in registers for every tile of the wave.  The logit is each
+// lane's 16 terms of w3 . h2, then the sums over the four lane groups (xor 16,
+// xor 32): a fixed order, so the output does not depend on the launch shape.
 __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ X, int n, const float *__restrict__ P,
                                                      float *__restrict__ out) {
-  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < n; b += gridDim.x * blockDim.x) {
-    float x[MLP_IN];
+  const int l = threadIdx.x & 63, c = l & 15, g = l >> 4;
+  float w1[4][3], b1[4][4], w2[4][4][4], b2[4][4], w3[4][4];
 #pragma unroll
-    for (int i = 0; i < MLP_IN; i++) x[i] = X[(size_t)b * MLP_IN + i];
-    out[b] = sigmoidf_(mlp_logit(P, x));
+  for (int m = 0; m < 4; m++) {
+#pragma unroll
+    for (int s = 0; s < 3; s++) w1[m][s] = P[MLP_OFF_W1 + (16 * m + c) * MLP_IN + 4 * s + g];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      b1[m][r] = P[MLP_OFF_B1 + 16 * m + 4 * g + r];
+      b2[m][r] = P[MLP_OFF_B2 + 16 * m + 4 * g + r];
+      w3[m][r] = P[MLP_OFF_W3 + 16 * m + 4 * g + r];
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < 4; o++)
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) w2[o][m][r] = P[MLP_OFF_W2 + (16 * o + c) * MLP_H + 16 * m + 4 * g + r];
+  const float b3 = P[MLP_OFF_B3];
+  const int ntile = (n + 15) >> 4;
+  for (int t = blockIdx.x * 4 + (threadIdx.x >> 6); t < ntile; t += gridDim.x * 4) {
+    const int smp = t * 16 + c;
+    const bool live = smp < n;
+    float xb[3];
+#pragma unroll
+    for (int s = 0; s < 3; s++) xb[s] = live ? X[(size_t)smp * MLP_IN + 4 * s + g] : 0.f;
+    f32x4 h1[4], h2[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      f32x4 a = {b1[m][0], b1[m][1], b1[m][2], b1[m][3]};
+#pragma unroll
+      for (int s = 0; s < 3; s++) a = __builtin_amdgcn_mfma_f32_16x16x4f32(w1[m][s], xb[s], a, 0, 0, 0);
+      h1[m] = a;
+      h2[m] = f32x4{b2[m][0], b2[m][1], b2[m][2], b2[m][3]};
+    }
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const float hb = relu(h1[m][r]);
+#pragma unroll
+        for (int o = 0; o < 4; o++) h2[o] = __builtin_amdgcn_mfma_f32_16x16x4f32(w2[o][m][r], hb, h2[o], 0, 0, 0);
+      }
+    float p = 0.f;
+#pragma unroll
+    for (int o = 0; o < 4; o++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) p += w3[o][r] * relu(h2[o][r]);
+    p += __shfl_xor(p, 16);
+    p += __shfl_xor(p, 32);
+    if (g == 0 && live) out[smp] = sigmoidf_(b3 + p);
   }
 }
 
@@ -315,8 +374,10 @@ size_t mlp_part_floats(int n) { return (size_t)((n + MB - 1) / MB) * (MLP_NPARAM
 
 hipError_t mlp_forward(const float *dX, int n, const float *dP, float *dOut, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  // enough workgroups to fill the device; each strides over the samples
-  k_mlp_forward<<<std::min((n + 255) / 256, 4096), 256, 0, st>>>(dX, n, dP, dOut);
+  // a wave per 16 samples up to 2048 waves (two workgroups per CU at the
+  // kernel's two waves per SIMD), each then striding over the tiles
+  const int ntile = (n + 15) / 16;
+  k_mlp_forward<<<std::min((ntile + 3) / 4, 512), 256, 0, st>>>(dX, n, dP, dOut);
   return hipGetLastError();
 }
 
