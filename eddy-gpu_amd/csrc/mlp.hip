@@ -121,12 +121,13 @@ __global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ X
 
 // One workgroup = MB samples (rows idx[b0 + b] of X/Y when idx is given, else
 // rows b0 + b), 256 threads = 4 waves; wave q owns hidden units 16q..16q+15 in
-// the forward and backward passes (lane = sample), then every thread sums a
-// strided subset of the NPARAMS + 1 outputs (gradients, then the loss) over the
-// block's samples in sample order.  part[blk][q] receives the sums.
+// the forward and backward passes (lane = sample, 16 sums in registers while
+// the lane's activations are read once each), then the block's gradient sums
+// over its samples in sample order: W2's 64 x 64 as a 4 x 4 tile per thread,
+// the rest one output per thread at a time.  part[blk][o] receives the sums.
 __global__ __launch_bounds__(256) void k_mlp_part(const float *X, const float *Y, const int *idx, int b0, int n,
                                                   const float *__restrict__ P, float *part) {
-  __shared__ float H1[MB * HS], H2[MB * HS], D1[MB * HS];
+  __shared__ float H1[MB * HS], H2[MB * HS], D1[MB * HS], D2[MB * HS];
   __shared__ float Xs[MB * XS];
   __shared__ float ZQ[4 * MB];
   __shared__ float DZ[MB], LS[MB];
@@ -142,22 +143,36 @@ __global__ __launch_bounds__(256) void k_mlp_part(const float *X, const float *Y
     for (int i = 0; i < MLP_IN; i++) Xs[b * XS + i] = live ? X[(size_t)row * MLP_IN + i] : 0.f;
   }
   __syncthreads();
+  float acc[16];
   // layer 1: this wave's 16 units of h1
-  for (int jj = 0; jj < 16; jj++) {
-    const int j = q * 16 + jj;
-    float s = P[MLP_OFF_B1 + j];
+  {
+    float x[MLP_IN];
 #pragma unroll
-    for (int i = 0; i < MLP_IN; i++) s += P[MLP_OFF_W1 + j * MLP_IN + i] * Xs[b * XS + i];
-    H1[b * HS + j] = live ? relu(s) : 0.f;
+    for (int i = 0; i < MLP_IN; i++) x[i] = Xs[b * XS + i];
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) {
+      const int j = q * 16 + jj;
+      float s = P[MLP_OFF_B1 + j];
+#pragma unroll
+      for (int i = 0; i < MLP_IN; i++) s += P[MLP_OFF_W1 + j * MLP_IN + i] * x[i];
+      H1[b * HS + j] = live ? relu(s) : 0.f;
+    }
   }
   __syncthreads();
-  // layer 2: 16 units of h2 and their share of the logit
+  // layer 2: 16 units of h2 (each b2 + the sum over j in order) and their
+  // share of the logit
+#pragma unroll
+  for (int kk = 0; kk < 16; kk++) acc[kk] = P[MLP_OFF_B2 + q * 16 + kk];
+  for (int j = 0; j < MLP_H; j++) {
+    const float h = H1[b * HS + j];
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) acc[kk] += P[MLP_OFF_W2 + (q * 16 + kk) * MLP_H + j] * h;
+  }
   float zq = 0.f;
+#pragma unroll
   for (int kk = 0; kk < 16; kk++) {
     const int k = q * 16 + kk;
-    float s = P[MLP_OFF_B2 + k];
-    for (int j = 0; j < MLP_H; j++) s += P[MLP_OFF_W2 + k * MLP_H + j] * H1[b * HS + j];
-    const float h = live ? relu(s) : 0.f;
+    const float h = live ? relu(acc[kk]) : 0.f;
     H2[b * HS + k] = h;
     zq += P[MLP_OFF_W3 + k] * h;
   }
@@ -172,24 +187,55 @@ __global__ __launch_bounds__(256) void k_mlp_part(const float *X, const float *Y
     DZ[b] = live ? sigmoidf_(z) - y : 0.f;
   }
   __syncthreads();
-  // backward to layer 1: d1[j] = [h1_j > 0] sum_k W2[k][j] d2[k],
-  // d2[k] = [h2_k > 0] dz w3[k]
+  // backward to layer 1: d2[k] = [h2_k > 0] dz w3[k] (wave 0 keeps it in D2),
+  // d1[j] = [h1_j > 0] sum_k W2[k][j] d2[k], the sum over k in order
   {
     const float dz = DZ[b];
+#pragma unroll
+    for (int jj = 0; jj < 16; jj++) acc[jj] = 0.f;
+    for (int k = 0; k < MLP_H; k++) {
+      const float d2 = H2[b * HS + k] > 0.f ? dz * P[MLP_OFF_W3 + k] : 0.f;
+      if (q == 0) D2[b * HS + k] = d2;
+#pragma unroll
+      for (int jj = 0; jj < 16; jj++) acc[jj] += P[MLP_OFF_W2 + k * MLP_H + q * 16 + jj] * d2;
+    }
+#pragma unroll
     for (int jj = 0; jj < 16; jj++) {
       const int j = q * 16 + jj;
-      float s = 0.f;
-      for (int k = 0; k < MLP_H; k++) {
-        const float d2 = H2[b * HS + k] > 0.f ? dz * P[MLP_OFF_W3 + k] : 0.f;
-        s += P[MLP_OFF_W2 + k * MLP_H + j] * d2;
-      }
-      D1[b * HS + j] = H1[b * HS + j] > 0.f ? s : 0.f;
+      D1[b * HS + j] = H1[b * HS + j] > 0.f ? acc[jj] : 0.f;
     }
   }
   __syncthreads();
-  // block sums of the gradient, one output per thread at a time
   float *out = part + (size_t)blk * (MLP_NPARAMS + 1);
-  for (int o = tid; o <= MLP_NPARAMS; o += 256) {
+  // W2[k][j] = sum_t d2[t][k] h1[t][j]: rows 4 (tid / 16).., columns 4 (tid % 16)..
+  {
+    const int k0 = (tid >> 4) * 4, j0 = (tid & 15) * 4;
+    float g[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) g[u][v] = 0.f;
+    for (int t = 0; t < MB; t++) {
+      float d[4], h[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        d[u] = D2[t * HS + k0 + u];
+        h[u] = H1[t * HS + j0 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) g[u][v] += d[u] * h[v];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) out[MLP_OFF_W2 + (k0 + u) * MLP_H + j0 + v] = g[u][v];
+  }
+  // the other outputs: W1 and b1, then b2.. and the loss (index NPARAMS)
+  constexpr int kRest = MLP_OFF_W2 + (MLP_NPARAMS + 1 - MLP_OFF_B2);
+  for (int r = tid; r < kRest; r += 256) {
+    const int o = r < MLP_OFF_W2 ? r : r - MLP_OFF_W2 + MLP_OFF_B2;
     float s = 0.f;
     if (o < MLP_OFF_B1) {  // W1[j][i]
       const int j = o / MLP_IN, i = o % MLP_IN;
@@ -197,17 +243,9 @@ __global__ __launch_bounds__(256) void k_mlp_part(const float *X, const float *Y
     } else if (o < MLP_OFF_W2) {  // b1[j]
       const int j = o - MLP_OFF_B1;
       for (int t = 0; t < MB; t++) s += D1[t * HS + j];
-    } else if (o < MLP_OFF_B2) {  // W2[k][j]
-      const int r = o - MLP_OFF_W2, k = r / MLP_H, j = r % MLP_H;
-      const float w3 = P[MLP_OFF_W3 + k];
-      for (int t = 0; t < MB; t++) {
-        const float d2 = H2[t * HS + k] > 0.f ? DZ[t] * w3 : 0.f;
-        s += d2 * H1[t * HS + j];
-      }
     } else if (o < MLP_OFF_W3) {  // b2[k]
       const int k = o - MLP_OFF_B2;
-      const float w3 = P[MLP_OFF_W3 + k];
-      for (int t = 0; t < MB; t++) s += H2[t * HS + k] > 0.f ? DZ[t] * w3 : 0.f;
+      for (int t = 0; t < MB; t++) s += D2[t * HS + k];
     } else if (o < MLP_OFF_B3) {  // w3[k]
       const int k = o - MLP_OFF_W3;
       for (int t = 0; t < MB; t++) s += DZ[t] * H2[t * HS + k];
